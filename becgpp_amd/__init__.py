@@ -1,0 +1,17 @@
+"""becgpp_amd -- MI355X-native ground-state solve path of becGPP.
+
+Drop-in for the reference's ground-state API: ``default_cfg``/``run(cfg)``
+and the call signatures of its ``grid``, ``operators``, ``interactions`` and
+``solvers`` modules, computed by hand-written HIP kernels for gfx950 in
+``libbecgpp_hip.so`` (C-ABI: include/becgpp_hip.h).  There is no CPU path.
+"""
+from .constants import CODE_VERSION, DEV
+from .config import CFG_DEFAULTS, default_cfg, canonical_key
+from .grid import make_grid, geometry, resolve_kernel, auto_grid, dV, norm_of, Lz_op
+from .interactions import long_range_phi, kernel_convolution, q2d_kernel, clear_kernel_cache
+from .operators import energy_components, energy_t, apply_H, compute_residual, observables
+from .thomasfermi import extract_tf, mass_quantiles
+from .seeds import make_seed
+from .solvers import ground_state, ground_state_multiseed
+
+__version__ = "0.1.0"

@@ -1,0 +1,187 @@
+// Shared device/host definitions for the becGPP MI355X ground-state library.
+//
+// Layout conventions (identical to the reference's torch tensors):
+//   * fields are C-contiguous, indexing "ij": element (ix, iy[, iz]) sits at
+//     ix*N + iy (2D) or (ix*N + iy)*N + iz (3D); axis 0 = x;
+//   * complex fields are interleaved double2 (torch complex128);
+//   * x_i = (i - N/2) dx, k_j = 2 pi f_j / (N dx) with f_j the FFT frequency
+//     index (Nyquist negative), as becgpp/grid.py:73-75.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BEC_WG 256          // threads per workgroup of the line passes
+#define BEC_MAX_STAGES 16
+#define BEC_MAX_L 1024      // longest line transform held in LDS
+
+struct FftPlan {
+  int L;                      // transform length (2^a 3^b)
+  int nst;                    // number of Stockham stages
+  int rad[BEC_MAX_STAGES];    // radix per stage (8, 4, 3 or 2)
+  const double2* tw;          // tw[m] = exp(-2 pi i m / L), m in [0, L)
+};
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// multiply by +i (S=+1) or -i (S=-1)
+template <int S> __device__ __forceinline__ double2 cmuli(double2 a) {
+  return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+// ----------------------------------------------------------------------------
+// radix-R DFT butterflies, X_p = sum_q v_q exp(SIGN 2 pi i p q / R)
+// ----------------------------------------------------------------------------
+template <int SIGN> __device__ __forceinline__ void bfly2(double2* v) {
+  double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <int SIGN> __device__ __forceinline__ void bfly3(double2* v) {
+  // w = exp(SIGN 2 pi i / 3) = -1/2 + i SIGN sqrt(3)/2
+  const double s = SIGN * 0.86602540378443864676;   // sqrt(3)/2
+  double2 a = v[0], b = v[1], c = v[2];
+  double2 t1 = cadd(b, c);
+  double2 t2 = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
+  double2 d = csub(b, c);
+  double2 t3 = make_double2(-s * d.y, s * d.x);       // i s (b - c)
+  v[0] = cadd(a, t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+template <int SIGN> __device__ __forceinline__ void bfly4(double2* v) {
+  double2 a = v[0], b = v[1], c = v[2], d = v[3];
+  double2 t0 = cadd(a, c), t1 = csub(a, c);
+  double2 t2 = cadd(b, d), t3 = cmuli<SIGN>(csub(b, d));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+template <int SIGN> __device__ __forceinline__ void bfly8(double2* v) {
+  const double h = 0.70710678118654752440;           // sqrt(2)/2
+  double2 e[4] = {v[0], v[2], v[4], v[6]};
+  double2 o[4] = {v[1], v[3], v[5], v[7]};
+  bfly4<SIGN>(e);
+  bfly4<SIGN>(o);
+  // twiddles w8^p, w8 = exp(SIGN i pi/4)
+  double2 o1 = make_double2(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
+  double2 o2 = cmuli<SIGN>(o[2]);
+  double2 o3 = make_double2(h * (-o[3].x - SIGN * o[3].y), h * (-o[3].y + SIGN * o[3].x));
+  v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+}
+template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
+  if constexpr (R == 2) bfly2<SIGN>(v);
+  else if constexpr (R == 3) bfly3<SIGN>(v);
+  else if constexpr (R == 4) bfly4<SIGN>(v);
+  else bfly8<SIGN>(v);
+}
+
+// One Stockham autosort stage of radix R on a line of length L held in LDS.
+// tpl threads cooperate on the line (lane in [0, tpl)); L/tpl <= 8 is
+// guaranteed by the host, so a thread owns at most ceil(8/R) butterflies.
+// Every thread of the workgroup must call it (it contains barriers).
+template <int R, int SIGN>
+__device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, int lane, int tpl,
+                                          const double2* __restrict__ tw, bool active) {
+  constexpr int MAXB = (8 + R - 1) / R;
+  const int nb = L / R;
+  double2 v[MAXB][R];
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = lane + b * tpl;
+      if (j < nb) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = s[j + r * nb];
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const int step = L / (Ns * R);
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = lane + b * tpl;
+      if (j < nb) {
+        const int k = j % Ns;
+        if (Ns > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            double2 w = tw[r * k * step];
+            if (SIGN > 0) w.y = -w.y;
+            v[b][r] = cmul(v[b][r], w);
+          }
+        }
+        bfly<R, SIGN>(v[b]);
+        const int idxD = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[idxD + r * Ns] = v[b][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// In-place unnormalised DFT of the LDS line s (SIGN=-1 forward, +1 inverse).
+template <int SIGN>
+__device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, int lane, int tpl, bool active) {
+  int Ns = 1;
+  for (int st = 0; st < p.nst; ++st) {
+    const int R = p.rad[st];
+    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
+    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
+    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
+    else fft_stage<2, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
+    Ns *= R;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// deterministic workgroup reductions (fixed tree order)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// Sums K per-thread values over the workgroup and writes them to out[0..K).
+// red must hold K*(blockDim.x/64) doubles of LDS.  Called by every thread.
+template <int K>
+__device__ __forceinline__ void block_sum_store(double (&v)[K], double* red, double* out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double s = wave_sum(v[q]);
+    if (lane == 0) red[q * nw + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[threadIdx.x * nw + w];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint64_t order_key(double v) {
+  uint64_t u = __double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_to_double(uint64_t k) {
+  uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double(u);
+}

@@ -1,0 +1,896 @@
+// HIP kernels of the becGPP ground-state path for CDNA4 (gfx950), FP64.
+//
+// All transforms are line transforms: a workgroup loads lpw lines of one axis
+// into LDS (coalesced: whole rows for the contiguous axis, tiles of lpw
+// adjacent columns for strided axes), runs a mixed-radix Stockham FFT in LDS
+// with tpl threads per line, applies a per-frequency multiplier, optionally
+// transforms back, and writes the result through a fused epilogue.  The
+// workloads are HBM/LDS bound FP64 vector work: there is no MFMA on this path.
+#include "bec_internal.h"
+
+#define TWO_PI 6.283185307179586
+
+__device__ __forceinline__ double kfreq(int j, int N, double invNdx) {
+  const int f = (j < (N + 1) / 2) ? j : j - N;
+  return TWO_PI * ((double)f * invNdx);
+}
+__device__ __forceinline__ double xcoord(int i, int N, double dx) { return (double)(i - N / 2) * dx; }
+
+__device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, double s_half) {
+  if (s_mode == 0) return 0.0;
+  const double p = (s_mode == 1) ? r2 : (s_mode == 2 ? r2 * r2 : pow(r2, s_half));
+  return tc * p;
+}
+
+// ---------------------------------------------------------------------------
+// fused grid reduction: every block stores its K sums and KM maxima; the last
+// block to arrive adds the per-block values in block order (deterministic)
+// and writes out[0 .. K+KM).  v/m are per-thread contributions.
+// ---------------------------------------------------------------------------
+template <int K, int KM>
+__device__ __forceinline__ void grid_reduce(const double* v, const double* m, double* partials,
+                                            unsigned* counter, double* out) {
+  constexpr int KT = K + KM;
+  __shared__ double red[KT * 16];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const double s = wave_sum(v[q]);
+    if (lane == 0) red[q * 16 + wid] = s;
+  }
+#pragma unroll
+  for (int q = 0; q < KM; ++q) {
+    const double s = wave_max(m[q]);
+    if (lane == 0) red[(K + q) * 16 + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < KT) {
+    const int q = threadIdx.x;
+    double s = red[q * 16];
+    for (int w = 1; w < nw; ++w) s = (q < K) ? s + red[q * 16 + w] : fmax(s, red[q * 16 + w]);
+    __hip_atomic_store(&partials[(size_t)blockIdx.x * KT + q], s, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  for (int q = 0; q < KT; ++q) {
+    double s = (q < K) ? 0.0 : -INFINITY;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+      const double x = __hip_atomic_load(&partials[(size_t)b * KT + q], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+      s = (q < K) ? s + x : fmax(s, x);
+    }
+    s = (q < K) ? wave_sum(s) : wave_max(s);
+    if (lane == 0) red[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = red[0];
+      for (int w = 1; w < nw; ++w) t = (q < K) ? t + red[w] : fmax(t, red[w]);
+      out[q] = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// axis geometry of the kinetic passes
+// ---------------------------------------------------------------------------
+template <int AX> struct Ax;
+template <> struct Ax<A2Y> {   // rows: line l = ix, element e = iy
+  static constexpr bool contig = true; static constexpr int ndim = 2; static constexpr int lsign = 1;
+  __device__ static long long base(int l, int N) { return (long long)l * N; }
+  __device__ static long long stride(int) { return 1; }
+  __device__ static void coords(int l, int e, int, int& ix, int& iy, int& iz) { ix = l; iy = e; iz = 0; }
+  __device__ static int lidx(int l, int) { return l; }
+};
+template <> struct Ax<A2X> {   // columns: line l = iy, element e = ix
+  static constexpr bool contig = false; static constexpr int ndim = 2; static constexpr int lsign = -1;
+  __device__ static long long base(int l, int) { return (long long)l; }
+  __device__ static long long stride(int N) { return N; }
+  __device__ static void coords(int l, int e, int, int& ix, int& iy, int& iz) { ix = e; iy = l; iz = 0; }
+  __device__ static int lidx(int l, int) { return l; }
+};
+template <> struct Ax<A3Z> {   // line l = ix*N + iy, element e = iz
+  static constexpr bool contig = true; static constexpr int ndim = 3; static constexpr int lsign = 0;
+  __device__ static long long base(int l, int N) { return (long long)l * N; }
+  __device__ static long long stride(int) { return 1; }
+  __device__ static void coords(int l, int e, int N, int& ix, int& iy, int& iz) { ix = l / N; iy = l % N; iz = e; }
+  __device__ static int lidx(int l, int) { return 0; }
+};
+template <> struct Ax<A3Y> {   // line l = ix*N + iz, element e = iy
+  static constexpr bool contig = false; static constexpr int ndim = 3; static constexpr int lsign = 1;
+  __device__ static long long base(int l, int N) { return (long long)(l / N) * N * N + (l % N); }
+  __device__ static long long stride(int N) { return N; }
+  __device__ static void coords(int l, int e, int N, int& ix, int& iy, int& iz) { ix = l / N; iy = e; iz = l % N; }
+  __device__ static int lidx(int l, int N) { return l / N; }
+};
+template <> struct Ax<A3X> {   // line l = iy*N + iz, element e = ix
+  static constexpr bool contig = false; static constexpr int ndim = 3; static constexpr int lsign = -1;
+  __device__ static long long base(int l, int) { return (long long)l; }
+  __device__ static long long stride(int N) { return (long long)N * N; }
+  __device__ static void coords(int l, int e, int N, int& ix, int& iy, int& iz) { ix = e; iy = l / N; iz = l % N; }
+  __device__ static int lidx(int l, int N) { return l / N; }
+};
+
+// Kinetic + rotation line pass.  Along an axis with wavenumber k the operator
+// T - Omega L_z acts as the multiplier kf k^2 + rf c k, where c = +x for the y
+// axis, -y for the x axis and 0 for the z axis (L_z = -i(x d_y - y d_x)).
+//   KE_*: forward transform only; accumulate sum kf-weighted |t_k|^2 / N
+//         (kinetic) and c sum k |t_k|^2 / N (L_z) per line (Parseval), and on
+//         the first pass the local sums rho, V rho, rho^2, rho^3.
+//   KH_*: forward, multiply, inverse; FIRST writes, MID accumulates, LAST adds
+//         the local potential (V + b2 rho + b3 rho^2 + phi) psi - mu psi.
+template <int AX, int KIND>
+__global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
+  extern __shared__ double2 lds[];
+  using X = Ax<AX>;
+  const int N = a.N, tpl = a.tpl, lpw = a.lpw;
+  const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
+  const int l0 = blockIdx.x * lpw;
+  const long long strd = X::stride(N);
+  const bool active = (l0 + c) < a.nlines;
+  double loc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
+    int cc, e;
+    if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+    const int l = l0 + cc;
+    double2 v = make_double2(0.0, 0.0);
+    if (l < a.nlines) {
+      const long long adr = X::base(l, N) + e * strd;
+      v = a.psi[adr];
+      if (a.d) {
+        const double2 dd = a.d[adr];
+        v.x = v.x - a.t * dd.x;
+        v.y = v.y - a.t * dd.y;
+      }
+      if (a.sc != 1.0) { v.x = v.x / a.sc; v.y = v.y / a.sc; }
+      if (KIND == KH_FIRST && a.psi_out) a.psi_out[adr] = v;
+      if (KIND == KE_FIRST) {
+        const double rho = cabs2(v);
+        int ix, iy, iz;
+        X::coords(l, e, N, ix, iy, iz);
+        const double xx = xcoord(ix, N, a.dx), yy = xcoord(iy, N, a.dx);
+        double r2 = xx * xx + yy * yy;
+        if (X::ndim == 3) { const double zz = xcoord(iz, N, a.dx); r2 += zz * zz; }
+        const double V = trap_value(r2, a.tc, a.s_mode, a.s_half);
+        loc[0] += rho;
+        loc[1] += V * rho;
+        loc[2] += rho * rho;
+        loc[3] += rho * rho * rho;
+        if (a.rho_out) a.rho_out[adr] = rho;
+      }
+    }
+    lds[cc * N + e] = v;
+  }
+  __syncthreads();
+  double2* s = lds + c * N;
+  fft_lds<-1>(s, a.fp, lane, tpl, active);
+  const double invN = 1.0 / (double)N;
+  const double lzc = (X::lsign == 0 || !active) ? 0.0
+                   : (double)X::lsign * xcoord(X::lidx(l0 + c, N), N, a.dx);
+  if (KIND == KE_FIRST || KIND == KE_REST) {
+    double kin = 0.0, lz = 0.0;
+    if (active) {
+      for (int k = lane; k < N; k += tpl) {
+        const double kv = kfreq(k, N, a.invNdx);
+        const double a2 = cabs2(s[k]);
+        kin += 0.5 * kv * kv * a2;
+        lz += kv * a2;
+      }
+    }
+    kin *= invN;
+    lz *= lzc * invN;
+    if (KIND == KE_FIRST) {
+      double v6[6] = {kin, lz, loc[0], loc[1], loc[2], loc[3]};
+      grid_reduce<6, 0>(v6, nullptr, a.partials, a.counter, a.red_out);
+    } else {
+      double v2[2] = {kin, lz};
+      grid_reduce<2, 0>(v2, nullptr, a.partials, a.counter, a.red_out);
+    }
+    return;
+  }
+  if (active) {
+    for (int k = lane; k < N; k += tpl) {
+      const double kv = kfreq(k, N, a.invNdx);
+      const double m = a.kf * kv * kv + a.rf * lzc * kv;
+      s[k] = cscale(s[k], m);
+    }
+  }
+  __syncthreads();
+  fft_lds<+1>(s, a.fp, lane, tpl, active);
+  double hsum[2] = {0.0, 0.0};
+  double hmax[1] = {0.0};
+  for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
+    int cc, e;
+    if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+    const int l = l0 + cc;
+    if (l >= a.nlines) continue;
+    const long long adr = X::base(l, N) + e * strd;
+    double2 u = lds[cc * N + e];
+    u.x *= invN;
+    u.y *= invN;
+    if (KIND == KH_FIRST) {
+      a.out[adr] = u;
+    } else if (KIND == KH_MID) {
+      const double2 p = a.acc[adr];
+      a.out[adr] = cadd(p, u);
+    } else {
+      const double2 ps = a.psi[adr];
+      const double rho = cabs2(ps);
+      int ix, iy, iz;
+      X::coords(l, e, N, ix, iy, iz);
+      const double xx = xcoord(ix, N, a.dx), yy = xcoord(iy, N, a.dx);
+      double r2 = xx * xx + yy * yy;
+      if (X::ndim == 3) { const double zz = xcoord(iz, N, a.dx); r2 += zz * zz; }
+      const double V = trap_value(r2, a.tc, a.s_mode, a.s_half);
+      double ph = 0.0;
+      if (a.phi) {
+        ph = a.phi[adr] * a.phisc;
+        if (a.phi_out) a.phi_out[adr] = ph;
+      }
+      const double W = V + a.beta2 * rho + a.beta3 * rho * rho + ph;
+      const double2 p = a.acc ? a.acc[adr] : make_double2(0.0, 0.0);
+      double2 h;
+      h.x = p.x + u.x + W * ps.x - a.mu * ps.x;
+      h.y = p.y + u.y + W * ps.y - a.mu * ps.y;
+      a.out[adr] = h;
+      hsum[0] += cabs2(h);
+      hsum[1] += ps.x * h.x + ps.y * h.y;
+      hmax[0] = fmax(hmax[0], rho);
+    }
+  }
+  if (KIND == KH_LAST && a.partials) grid_reduce<2, 1>(hsum, hmax, a.partials, a.counter, a.red_out);
+}
+
+// ---------------------------------------------------------------------------
+// generic line FFT pass: pad-on-load / crop-on-store, optional multiplier
+// ---------------------------------------------------------------------------
+template <bool CONTIG, int LD, int DIR, int MUL, int ST>
+__global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
+  extern __shared__ double2 lds[];
+  const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw;
+  const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
+  const int l0 = blockIdx.x * lpw;
+  const bool active = (l0 + c) < a.nlines;
+  for (int idx = threadIdx.x; idx < lpw * L; idx += blockDim.x) {
+    int cc, e;
+    if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
+    const int l = l0 + cc;
+    double2 v = make_double2(0.0, 0.0);
+    if (l < a.nlines) {
+      const int o = l / a.ni, i = l - o * a.ni;
+      const long long b = o * a.in_so + i * a.in_si;
+      if (LD == LD_HERM) {
+        const int half = L / 2;
+        const int src = (e <= half) ? e : L - e;
+        const double2 w = ((const double2*)a.in)[b + src * a.in_s];
+        v = (e <= half) ? w : make_double2(w.x, -w.y);
+      } else {
+        const int r = e - a.off_load;
+        if (r >= 0 && r < a.n_load) {
+          if (LD == LD_C) v = ((const double2*)a.in)[b + r * a.in_s];
+          else v = make_double2(((const double*)a.in)[b + r * a.in_s], 0.0);
+        }
+      }
+    }
+    lds[cc * L + e] = v;
+  }
+  __syncthreads();
+  double2* s = lds + c * L;
+  if (DIR == DIR_INV) fft_lds<+1>(s, a.fp, lane, tpl, active);
+  else fft_lds<-1>(s, a.fp, lane, tpl, active);
+  if (DIR == DIR_FMI) {
+    if (active) {
+      const int l = l0 + c;
+      if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
+        int other;
+        if (MUL == MUL_KHAT2) {
+          other = l;                                  // column index ky in [0, H)
+        } else {
+          const int ky = l / a.H, kz = l - (l / a.H) * a.H;
+          other = min(ky, a.M - ky) * a.H + kz;
+        }
+        const long long plane = (MUL == MUL_KHAT2) ? (long long)a.H : (long long)a.H * a.H;
+        for (int k = lane; k < L; k += tpl) {
+          const int kk = min(k, L - k);
+          s[k] = cscale(s[k], a.khat[kk * plane + other]);
+        }
+      } else {
+        // denominator 1/2 (kx^2 + ky^2 [+ kz^2]) + sigma, summed in that order
+        const double sig = *a.sigma;
+        const double ky = kfreq(MUL == MUL_PRE2 ? l : l / a.Nk, a.Nk, a.invNdx);
+        const double kz = (MUL == MUL_PRE3) ? kfreq(l % a.Nk, a.Nk, a.invNdx) : 0.0;
+        for (int k = lane; k < L; k += tpl) {
+          const double kx = kfreq(k, a.Nk, a.invNdx);
+          double k2 = kx * kx + ky * ky;
+          if (MUL == MUL_PRE3) k2 = k2 + kz * kz;
+          const double den = 0.5 * k2 + sig;
+          s[k] = make_double2(s[k].x / den, s[k].y / den);
+        }
+      }
+    }
+    __syncthreads();
+    fft_lds<+1>(s, a.fp, lane, tpl, active);
+  }
+  double dsum[1] = {0.0};
+  for (int idx = threadIdx.x; idx < lpw * a.n_store; idx += blockDim.x) {
+    int cc, e;
+    if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
+    const int l = l0 + cc;
+    if (l >= a.nlines) continue;
+    const int o = l / a.ni, i = l - o * a.ni;
+    const long long adr = o * a.out_so + i * a.out_si + e * a.out_s;
+    const double2 v = lds[cc * L + a.off_store + e];
+    if (ST == ST_C) {
+      ((double2*)a.out)[adr] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
+    } else {
+      const double r = v.x * a.out_scale;
+      ((double*)a.out)[adr] = r;
+      if (a.dot) dsum[0] += r * a.dot[adr];
+    }
+  }
+  if (ST == ST_RE && a.dot) grid_reduce<1, 0>(dsum, nullptr, a.partials, a.counter, a.red_out);
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BEC_WG) void k_norm2(const double2* __restrict__ psi, long long n,
+                                                  double* partials, unsigned* counter, double* out) {
+  double v[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    v[0] += cabs2(psi[i]);
+  grid_reduce<1, 0>(v, nullptr, partials, counter, out);
+}
+
+__global__ __launch_bounds__(BEC_WG) void k_scale(const double2* __restrict__ in, double2* out,
+                                                  double sc, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 v = in[i];
+    out[i] = make_double2(v.x / sc, v.y / sc);
+  }
+}
+
+__global__ __launch_bounds__(BEC_WG) void k_rho(const double2* __restrict__ in, double* out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = cabs2(in[i]);
+}
+
+// d <- az z + ad d + ag g + gam psi   (gam complex)
+__global__ __launch_bounds__(BEC_WG) void k_combine(double2* d, const double2* __restrict__ z,
+                                                    const double2* __restrict__ g,
+                                                    const double2* __restrict__ psi, double az,
+                                                    double ad, double ag, double gre, double gim,
+                                                    long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 p = psi[i];
+    double2 r = make_double2(gre * p.x - gim * p.y, gre * p.y + gim * p.x);
+    if (az != 0.0) { const double2 v = z[i]; r.x += az * v.x; r.y += az * v.y; }
+    if (ad != 0.0) { const double2 v = d[i]; r.x += ad * v.x; r.y += ad * v.y; }
+    if (ag != 0.0) { const double2 v = g[i]; r.x += ag * v.x; r.y += ag * v.y; }
+    d[i] = r;
+  }
+}
+
+// Inner products of the CG step (all sums, scaled by dv on the host):
+//  0,1 <psi,z>   2 <psi,psi>   3,4 <psi,d>   5,6 <psi,g>   7 Re<z,g>
+//  8 Re<z,y>   9,10 <psi,y>   11 Re<g,d>   12 <g,g>   13 Re<g,y>
+//  14 Re<s,y>  15 <s,s>       (y = g - gp, s = psi - pp, <u,v> = sum conj(u) v)
+__global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
+                                                 const double2* __restrict__ psi,
+                                                 const double2* __restrict__ d,
+                                                 const double2* __restrict__ g,
+                                                 const double2* __restrict__ gp,
+                                                 const double2* __restrict__ pp, long long n,
+                                                 double* partials, unsigned* counter, double* out) {
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 zi = z[i], pi = psi[i], di = d[i], gi = g[i], gpi = gp[i], ppi = pp[i];
+    const double2 y = csub(gi, gpi), sv = csub(pi, ppi);
+    v[0] += pi.x * zi.x + pi.y * zi.y;  v[1] += pi.x * zi.y - pi.y * zi.x;
+    v[2] += cabs2(pi);
+    v[3] += pi.x * di.x + pi.y * di.y;  v[4] += pi.x * di.y - pi.y * di.x;
+    v[5] += pi.x * gi.x + pi.y * gi.y;  v[6] += pi.x * gi.y - pi.y * gi.x;
+    v[7] += zi.x * gi.x + zi.y * gi.y;
+    v[8] += zi.x * y.x + zi.y * y.y;
+    v[9] += pi.x * y.x + pi.y * y.y;    v[10] += pi.x * y.y - pi.y * y.x;
+    v[11] += gi.x * di.x + gi.y * di.y;
+    v[12] += cabs2(gi);
+    v[13] += gi.x * y.x + gi.y * y.y;
+    v[14] += sv.x * y.x + sv.y * y.y;
+    v[15] += cabs2(sv);
+  }
+  grid_reduce<16, 0>(v, nullptr, partials, counter, out);
+}
+
+// g = h - mu psi, and sum |g|^2
+__global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
+                                                 const double2* __restrict__ psi, double mu,
+                                                 double2* g, long long n, double* partials,
+                                                 unsigned* counter, double* out) {
+  double v[1] = {0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double2 hi = h[i], p = psi[i];
+    const double2 r = make_double2(hi.x - mu * p.x, hi.y - mu * p.y);
+    if (g) g[i] = r;
+    v[0] += cabs2(r);
+  }
+  grid_reduce<1, 0>(v, nullptr, partials, counter, out);
+}
+
+// moments: mass, sum R2 rho, sum (X^2+Y^2) rho, sum Z^2 rho; max rho
+__global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ psi, int ndim, int N,
+                                                    double dx, double* partials, unsigned* counter,
+                                                    double* out) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double m[1] = {0.0};
+  const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double rho = cabs2(psi[i]);
+    int ix, iy, iz = 0;
+    if (ndim == 2) { ix = (int)(i / N); iy = (int)(i % N); }
+    else { ix = (int)(i / ((long long)N * N)); iy = (int)((i / N) % N); iz = (int)(i % N); }
+    const double xx = xcoord(ix, N, dx), yy = xcoord(iy, N, dx);
+    const double zz = (ndim == 3) ? xcoord(iz, N, dx) : 0.0;
+    double r2 = xx * xx + yy * yy;
+    if (ndim == 3) r2 += zz * zz;
+    v[0] += rho;
+    v[1] += r2 * rho;
+    v[2] += (xx * xx + yy * yy) * rho;
+    v[3] += zz * zz * rho;
+    m[0] = fmax(m[0], rho);
+  }
+  grid_reduce<4, 1>(v, m, partials, counter, out);
+}
+
+// ---------------------------------------------------------------------------
+// Sobolev shift: lower median of the local potential over the bulk
+// (value-range histogram, candidate gather, exact select in one workgroup)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restrict__ psi,
+                                                        const double* __restrict__ phi, double phisc,
+                                                        const double* rhomax, double b2, double b3,
+                                                        int ndim, int N, double dx, double tc,
+                                                        int s_mode, double s_half, long long n,
+                                                        double* loc, SigmaStats* st) {
+  __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64];
+  __shared__ unsigned scnt[BEC_WG / 64];
+  const double thr = 0.05 * (*rhomax);
+  unsigned long long kmn = ~0ull, kmx = 0ull;
+  unsigned cnt = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double rho = cabs2(psi[i]);
+    double val = __longlong_as_double(0x7ff8000000000000ll);    // NaN = outside the bulk
+    if (rho > thr) {
+      int ix, iy, iz = 0;
+      if (ndim == 2) { ix = (int)(i / N); iy = (int)(i % N); }
+      else { ix = (int)(i / ((long long)N * N)); iy = (int)((i / N) % N); iz = (int)(i % N); }
+      const double xx = xcoord(ix, N, dx), yy = xcoord(iy, N, dx);
+      double r2 = xx * xx + yy * yy;
+      if (ndim == 3) { const double zz = xcoord(iz, N, dx); r2 += zz * zz; }
+      double lv = trap_value(r2, tc, s_mode, s_half) + 2.0 * b2 * rho + 3.0 * b3 * (rho * rho);
+      if (phi) lv = lv + phi[i] * phisc;
+      val = lv;
+      const unsigned long long k = order_key(lv);
+      kmn = k < kmn ? k : kmn;
+      kmx = k > kmx ? k : kmx;
+      cnt++;
+    }
+    loc[i] = val;
+  }
+  // wave then block reduce, one atomic per block
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long a = __shfl_down(kmn, off, 64), b = __shfl_down(kmx, off, 64);
+    kmn = a < kmn ? a : kmn;
+    kmx = b > kmx ? b : kmx;
+    cnt += __shfl_down(cnt, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { smin[wid] = kmn; smax[wid] = kmx; scnt[wid] = cnt; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      kmn = smin[w] < kmn ? smin[w] : kmn;
+      kmx = smax[w] > kmx ? smax[w] : kmx;
+      cnt += scnt[w];
+    }
+    if (cnt) {
+      atomicMin(&st->kmin, kmn);
+      atomicMax(&st->kmax, kmx);
+      atomicAdd(&st->cnt, cnt);
+    }
+  }
+}
+
+__device__ __forceinline__ int sigma_bin(double v, double lo, double scale) {
+  int b = (int)((v - lo) * scale);
+  return b < 0 ? 0 : (b >= SIGMA_BINS ? SIGMA_BINS - 1 : b);
+}
+
+__global__ __launch_bounds__(BEC_WG) void k_sigma_hist(const double* __restrict__ loc, long long n,
+                                                       const SigmaStats* st, unsigned* hist) {
+  __shared__ unsigned h[SIGMA_BINS];
+  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) h[b] = 0u;
+  __syncthreads();
+  const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
+  if (st->cnt > 0 && hi > lo) {
+    const double scale = (double)SIGMA_BINS / (hi - lo);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+      const double v = loc[i];
+      if (v == v) atomicAdd(&h[sigma_bin(v, lo, scale)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+// find the bin holding the rank-k element, k = (cnt-1)/2, then gather its members
+__global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restrict__ loc, long long n,
+                                                         SigmaStats* st, const unsigned* hist,
+                                                         double* cand) {
+  __shared__ unsigned part[BEC_WG];
+  __shared__ int sbin;
+  __shared__ unsigned sbefore;
+  const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
+  const unsigned cnt = st->cnt;
+  if (cnt == 0 || !(hi > lo)) return;
+  const unsigned k = (cnt - 1u) / 2u;
+  constexpr int PER = SIGMA_BINS / BEC_WG;
+  unsigned s = 0;
+  for (int j = 0; j < PER; ++j) s += hist[threadIdx.x * PER + j];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned run = 0;
+    int t = 0;
+    for (; t < BEC_WG; ++t) {
+      if (run + part[t] > k) break;
+      run += part[t];
+    }
+    int b = t * PER;
+    for (int j = 0; j < PER; ++j, ++b) {
+      const unsigned hb = hist[b];
+      if (run + hb > k) break;
+      run += hb;
+    }
+    sbin = b;
+    sbefore = run;
+    if (blockIdx.x == 0) { st->bucket = (unsigned)b; st->rank_in = k - run; }
+  }
+  __syncthreads();
+  const int bin = sbin;
+  const double scale = (double)SIGMA_BINS / (hi - lo);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = loc[i];
+    if (v == v && sigma_bin(v, lo, scale) == bin) {
+      const unsigned pos = atomicAdd(&st->ncand, 1u);
+      cand[pos] = v;
+    }
+  }
+}
+
+// one workgroup of 1024 threads: exact selection, sigma = max(median, floor),
+// then reset the histogram and statistics for the next call
+__global__ __launch_bounds__(1024) void k_sigma_select(SigmaStats* st, unsigned* hist,
+                                                       const double* __restrict__ cand, double floor_,
+                                                       double* sigma_out, double* host_slot) {
+  __shared__ double sv[SIGMA_LDS_CAND];
+  __shared__ unsigned dh[256];
+  __shared__ unsigned long long prefix;
+  __shared__ unsigned rank_s;
+  const unsigned cnt = st->cnt;
+  const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
+  double med = floor_;
+  bool have = false;
+  if (cnt > 0 && !(hi > lo)) {
+    med = lo;
+    have = true;
+  } else if (cnt > 0) {
+    const unsigned m = st->ncand, r = st->rank_in;
+    if (m <= SIGMA_LDS_CAND) {
+      unsigned P = 1;
+      while (P < m) P <<= 1;
+      for (unsigned i = threadIdx.x; i < P; i += blockDim.x) sv[i] = (i < m) ? cand[i] : INFINITY;
+      __syncthreads();
+      for (unsigned k = 2; k <= P; k <<= 1) {
+        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+          for (unsigned i = threadIdx.x; i < P; i += blockDim.x) {
+            const unsigned ixj = i ^ j;
+            if (ixj > i) {
+              const double a = sv[i], b = sv[ixj];
+              const bool up = ((i & k) == 0);
+              if ((a > b) == up) { sv[i] = b; sv[ixj] = a; }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      med = sv[r];
+      have = true;
+    } else {
+      // radix select over the order keys of the candidates, 8 bits at a time
+      if (threadIdx.x == 0) { prefix = 0ull; rank_s = r; }
+      __syncthreads();
+      for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += blockDim.x) dh[b] = 0u;
+        __syncthreads();
+        const unsigned long long pre = prefix;
+        const unsigned long long hmask = (shift == 56) ? 0ull : (~0ull << (shift + 8));
+        for (unsigned i = threadIdx.x; i < m; i += blockDim.x) {
+          const unsigned long long kk = order_key(cand[i]);
+          if ((kk & hmask) == pre) atomicAdd(&dh[(kk >> shift) & 0xffu], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          unsigned run = 0, rr = rank_s;
+          int b = 0;
+          for (; b < 256; ++b) {
+            if (run + dh[b] > rr) break;
+            run += dh[b];
+          }
+          prefix = pre | ((unsigned long long)b << shift);
+          rank_s = rr - run;
+        }
+        __syncthreads();
+      }
+      med = key_to_double(prefix);
+      have = true;
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) hist[b] = 0u;
+  if (threadIdx.x == 0) {
+    double sg = have ? med : floor_;
+    sg = fmax(sg, floor_);
+    *sigma_out = sg;
+    if (host_slot) *host_slot = sg;
+    st->kmin = ~0ull;
+    st->kmax = 0ull;
+    st->cnt = 0u;
+    st->bucket = 0u;
+    st->rank_in = 0u;
+    st->ncand = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernel spectrum construction (becgpp/interactions.py:117-148): Gauss-Legendre
+// 4x4 (2D) / 4x4x4 (3D) cell average of the point kernel on the wrap-around
+// displacement grid; the self cell is filled by the host afterwards.
+// kind: 1 = newton (1/r), 2 = log (-ln r).
+// ---------------------------------------------------------------------------
+__constant__ double c_gl_nodes[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563,
+                                     0.8611363115940526};
+__constant__ double c_gl_wts[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461,
+                                   0.3478548451374538};
+
+__global__ __launch_bounds__(BEC_WG) void k_kernel_real(int ndim, int M, double dx, int kind,
+                                                        double2* out) {
+  const long long n = (ndim == 2) ? (long long)M * M : (long long)M * M * M;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    int jx, jy, jz = 0;
+    if (ndim == 2) { jx = (int)(i / M); jy = (int)(i % M); }
+    else { jx = (int)(i / ((long long)M * M)); jy = (int)((i / M) % M); jz = (int)(i % M); }
+    const double DX = (double)(jx <= M / 2 ? jx : jx - M) * dx;
+    const double DY = (double)(jy <= M / 2 ? jy : jy - M) * dx;
+    const double DZ = (double)(jz <= M / 2 ? jz : jz - M) * dx;
+    double ker = 0.0;
+    if (ndim == 2) {
+      for (int a = 0; a < 4; ++a) {
+        for (int b = 0; b < 4; ++b) {
+          const double ux = DX + 0.5 * dx * c_gl_nodes[a];
+          const double uy = DY + 0.5 * dx * c_gl_nodes[b];
+          const double r2 = ux * ux + uy * uy;
+          const double val = (kind == 1) ? 1.0 / sqrt(r2) : -0.5 * log(r2);
+          ker += 0.25 * c_gl_wts[a] * c_gl_wts[b] * val;
+        }
+      }
+    } else {
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+          for (int c = 0; c < 4; ++c) {
+            const double ux = DX + 0.5 * dx * c_gl_nodes[a];
+            const double uy = DY + 0.5 * dx * c_gl_nodes[b];
+            const double uz = DZ + 0.5 * dx * c_gl_nodes[c];
+            ker += (c_gl_wts[a] * c_gl_wts[b] * c_gl_wts[c] / 8.0) / sqrt(ux * ux + uy * uy + uz * uz);
+          }
+    }
+    out[i] = make_double2(ker, 0.0);
+  }
+}
+
+__global__ __launch_bounds__(BEC_WG) void k_extract_khat(int ndim, int M, int H,
+                                                         const double2* __restrict__ kf, double* khat) {
+  const long long n = (ndim == 2) ? (long long)H * H : (long long)H * H * H;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long src;
+    if (ndim == 2) src = (i / H) * M + (i % H);
+    else src = ((i / ((long long)H * H)) * M + (i / H) % H) * M + (i % H);
+    khat[i] = kf[src].x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+static int grid_for(long long n) {
+  long long g = (n + BEC_WG - 1) / BEC_WG;
+  if (g > 2048) g = 2048;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+LineCfg line_cfg(int L) {
+  int need = (L + 7) / 8;
+  int tpl = 1;
+  while (tpl < need) tpl <<= 1;
+  if (tpl > BEC_WG) tpl = BEC_WG;
+  LineCfg c;
+  c.tpl = tpl;
+  c.lpw = BEC_WG / tpl;
+  return c;
+}
+
+template <bool CONTIG, int LD, int DIR, int MUL, int ST>
+static void go_pass(const PassArgs& a, hipStream_t s) {
+  const int blocks = (a.nlines + a.lpw - 1) / a.lpw;
+  const size_t shm = (size_t)a.lpw * a.fp.L * sizeof(double2);
+  const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL;
+  prof_pre(id, s);
+  hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
+  prof_post(id, s);
+}
+
+void launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
+#define BEC_P(C, LDV, D, MU, S)                                                 \
+  if (contig == (C) && ld == (LDV) && dir == (D) && mul == (MU) && st == (S)) { \
+    go_pass<C, LDV, D, MU, S>(a, s);                                            \
+    return;                                                                     \
+  }
+  // plain transforms (kernel spectra, preconditioner)
+  BEC_P(1, LD_C, DIR_FWD, MUL_NONE, ST_C)
+  BEC_P(0, LD_C, DIR_FWD, MUL_NONE, ST_C)
+  BEC_P(1, LD_C, DIR_INV, MUL_NONE, ST_C)
+  BEC_P(0, LD_C, DIR_INV, MUL_NONE, ST_C)
+  // preconditioner middle pass
+  BEC_P(0, LD_C, DIR_FMI, MUL_PRE2, ST_C)
+  BEC_P(0, LD_C, DIR_FMI, MUL_PRE3, ST_C)
+  // convolution
+  BEC_P(1, LD_R, DIR_FWD, MUL_NONE, ST_C)
+  BEC_P(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C)
+  BEC_P(0, LD_C, DIR_FMI, MUL_KHAT3, ST_C)
+  BEC_P(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE)
+#undef BEC_P
+  abort();
+}
+
+template <int AX, int KIND>
+static void go_kin(const KinArgs& a, hipStream_t s) {
+  const int blocks = (a.nlines + a.lpw - 1) / a.lpw;
+  const size_t shm = (size_t)a.lpw * a.N * sizeof(double2);
+  const int id = 100 + 10 * AX + KIND;
+  prof_pre(id, s);
+  hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
+  prof_post(id, s);
+}
+
+void launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
+#define BEC_K(AXV)                                                  \
+  if (ax == AXV) {                                                  \
+    switch (kind) {                                                 \
+      case KE_FIRST: go_kin<AXV, KE_FIRST>(a, s); return;           \
+      case KE_REST: go_kin<AXV, KE_REST>(a, s); return;             \
+      case KH_FIRST: go_kin<AXV, KH_FIRST>(a, s); return;           \
+      case KH_MID: go_kin<AXV, KH_MID>(a, s); return;               \
+      default: go_kin<AXV, KH_LAST>(a, s); return;                  \
+    }                                                               \
+  }
+  BEC_K(A2Y)
+  BEC_K(A2X)
+  BEC_K(A3Z)
+  BEC_K(A3Y)
+  BEC_K(A3X)
+#undef BEC_K
+  abort();
+}
+
+void launch_norm2(const double2* psi, long long n, double* partials, unsigned* counter, double* out,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_norm2, dim3(grid_for(n)), dim3(BEC_WG), 0, s, psi, n, partials, counter, out);
+}
+void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(BEC_WG), 0, s, in, out, sc, n);
+}
+void launch_rho(const double2* psi, double* rho, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_rho, dim3(grid_for(n)), dim3(BEC_WG), 0, s, psi, rho, n);
+}
+void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, double az,
+                    double ad, double ag, double gre, double gim, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(BEC_WG), 0, s, d, z, g, psi, az, ad, ag, gre,
+                     gim, n);
+}
+void launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
+                 const double2* gp, const double2* pp, long long n, double* partials,
+                 unsigned* counter, double* out, hipStream_t s) {
+  prof_pre(SITE_DOTS, s);
+  hipLaunchKernelGGL(k_dots, dim3(grid_for(n)), dim3(BEC_WG), 0, s, z, psi, d, g, gp, pp, n, partials,
+                     counter, out);
+  prof_post(SITE_DOTS, s);
+}
+void launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
+                 double* partials, unsigned* counter, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_res2, dim3(grid_for(n)), dim3(BEC_WG), 0, s, h, psi, mu, g_out, n, partials,
+                     counter, out);
+}
+void launch_moments(const double2* psi, int ndim, int N, double dx, double* partials,
+                    unsigned* counter, double* out, hipStream_t s) {
+  const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
+  hipLaunchKernelGGL(k_moments, dim3(grid_for(n)), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials,
+                     counter, out);
+}
+void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc,
+                  const double* rhomax, double beta2, double beta3, double floor_, hipStream_t s) {
+  const int g = grid_for(p.n);
+  prof_pre(SITE_SIGMA_LOCAL, s);
+  hipLaunchKernelGGL(k_sigma_local, dim3(g), dim3(BEC_WG), 0, s, psi, phi, phisc, rhomax, beta2, beta3,
+                     p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, p.loc, p.stats);
+  prof_post(SITE_SIGMA_LOCAL, s);
+  prof_pre(SITE_SIGMA_HIST, s);
+  hipLaunchKernelGGL(k_sigma_hist, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist);
+  prof_post(SITE_SIGMA_HIST, s);
+  prof_pre(SITE_SIGMA_GATHER, s);
+  hipLaunchKernelGGL(k_sigma_gather, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist, p.cand);
+  prof_post(SITE_SIGMA_GATHER, s);
+  prof_pre(SITE_SIGMA_SELECT, s);
+  hipLaunchKernelGGL(k_sigma_select, dim3(1), dim3(1024), 0, s, p.stats, p.hist, p.cand, floor_,
+                     p.sigma, p.red + R_SIGMA);
+  prof_post(SITE_SIGMA_SELECT, s);
+}
+__global__ __launch_bounds__(BEC_WG) void k_sigma_reset(SigmaStats* st, unsigned* hist, double* sigma) {
+  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) hist[b] = 0u;
+  if (threadIdx.x == 0) {
+    st->kmin = ~0ull;
+    st->kmax = 0ull;
+    st->cnt = 0u;
+    st->bucket = 0u;
+    st->rank_in = 0u;
+    st->ncand = 0u;
+    *sigma = 0.0;
+  }
+}
+
+void launch_init_sigma(const Plan& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_sigma_reset, dim3(1), dim3(BEC_WG), 0, s, p.stats, p.hist, p.sigma);
+}
+void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s) {
+  const long long n = (ndim == 2) ? (long long)M * M : (long long)M * M * M;
+  hipLaunchKernelGGL(k_kernel_real, dim3(grid_for(n)), dim3(BEC_WG), 0, s, ndim, M, dx, kind, out);
+}
+void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat, hipStream_t s) {
+  const long long n = (ndim == 2) ? (long long)H * H : (long long)H * H * H;
+  hipLaunchKernelGGL(k_extract_khat, dim3(grid_for(n)), dim3(BEC_WG), 0, s, ndim, M, H, kf, khat);
+}

@@ -1,0 +1,677 @@
+// Plan management, multi-pass operator drivers and the C-ABI entry points
+// (everything except the solver loop).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/becgpp_hip.h"
+#include "bec_ops.h"
+
+static thread_local std::string g_err;
+
+void bec_set_error(const char* msg) { g_err = msg; }
+
+extern "C" const char* bec_last_error(void) { return g_err.c_str(); }
+
+#define BEC_TRY(...)                                \
+  try {                                             \
+    __VA_ARGS__;                                    \
+    return 0;                                       \
+  } catch (const BecError& e_) {                    \
+    g_err = e_.what();                              \
+    return e_.code;                                 \
+  } catch (const std::exception& e_) {              \
+    g_err = e_.what();                              \
+    return 2;                                       \
+  }
+
+static void bad(const std::string& m) { throw BecError(1, m); }
+
+// ---------------------------------------------------------------------------
+// FFT plans
+// ---------------------------------------------------------------------------
+static bool factor(int L, std::vector<int>& rad) {
+  rad.clear();
+  int n = L;
+  while (n % 8 == 0) { rad.push_back(8); n /= 8; }
+  while (n % 3 == 0) { rad.push_back(3); n /= 3; }
+  while (n % 4 == 0) { rad.push_back(4); n /= 4; }
+  while (n % 2 == 0) { rad.push_back(2); n /= 2; }
+  return n == 1 && (int)rad.size() <= BEC_MAX_STAGES;
+}
+
+static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
+  std::vector<int> rad;
+  if (L < 2 || L > BEC_MAX_L || !factor(L, rad))
+    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b <= " +
+        std::to_string(BEC_MAX_L));
+  fp.L = L;
+  fp.nst = (int)rad.size();
+  for (int i = 0; i < BEC_MAX_STAGES; ++i) fp.rad[i] = i < fp.nst ? rad[i] : 1;
+  // twiddles exp(-2 pi i m / L) in extended precision, reduced to the first octant
+  std::vector<double2> tw(L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < L; ++m) {
+    const long double ang = -2.0L * pi * (long double)m / (long double)L;
+    tw[m].x = (double)cosl(ang);
+    tw[m].y = (double)sinl(ang);
+  }
+  // exact values where they are known
+  for (int m = 0; m < L; ++m) {
+    if ((4 * m) % L == 0) {
+      const int q = (4 * m) / L;
+      const double c[4] = {1.0, 0.0, -1.0, 0.0}, sn[4] = {0.0, -1.0, 0.0, 1.0};
+      tw[m].x = c[q];
+      tw[m].y = sn[q];
+    }
+  }
+  HIPCHK(hipMalloc(tw_dev, sizeof(double2) * L));
+  HIPCHK(hipMemcpyAsync(*tw_dev, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  fp.tw = *tw_dev;
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+bool plan_long_range(const Plan& p, const Phys& ph) { return p.kkind[0] != 0 && fabs(ph.G_C) > 1e-15; }
+
+void plan_sync(Plan& p) { HIPCHK(hipStreamSynchronize(p.stream)); }
+
+void copy_red(Plan& p, int off, int count) {
+  HIPCHK(hipMemcpyAsync(p.hred + off, p.red + off, sizeof(double) * count, hipMemcpyDeviceToHost,
+                        p.stream));
+}
+
+static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
+  PassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.fp = fp;
+  a.tpl = lc.tpl;
+  a.lpw = lc.lpw;
+  a.ni = 1;
+  a.out_scale = 1.0;
+  a.H = p.H;
+  a.M = p.M;
+  a.Nk = p.N;
+  a.invNdx = p.invNdx;
+  a.sigma = p.sigma;
+  a.partials = p.partials;
+  a.counter = p.counters + C_EGRAV;
+  a.red_out = p.red + R_EGRAV;
+  return a;
+}
+
+void run_conv(Plan& p, int which, const double* rho, double scale, double* out, const double* dot) {
+  if (!p.khat[which]) throw BecError(1, "kernel spectrum not built for this plan");
+  const int N = p.N, M = p.M, H = p.H, i0 = (p.M - p.N) / 2;
+  const double fac = scale * p.dv / (double)p.nM;
+  if (p.ndim == 2) {
+    PassArgs c1 = pass_base(p, p.fpM, p.lcM);
+    c1.nlines = N; c1.in_so = N; c1.in_s = 1; c1.n_load = N; c1.off_load = i0;
+    c1.out_so = H; c1.out_s = 1; c1.n_store = H; c1.off_store = 0;
+    c1.in = rho; c1.out = p.s1;
+    launch_pass(1, LD_R, DIR_FWD, MUL_NONE, ST_C, c1, p.stream);
+    PassArgs c2 = pass_base(p, p.fpM, p.lcM);
+    c2.nlines = H; c2.ni = H; c2.in_so = 0; c2.in_si = 1; c2.in_s = H; c2.n_load = N; c2.off_load = i0;
+    c2.out_so = 0; c2.out_si = 1; c2.out_s = H; c2.n_store = N; c2.off_store = i0;
+    c2.in = p.s1; c2.out = p.s1; c2.khat = p.khat[which];
+    launch_pass(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C, c2, p.stream);
+    PassArgs c3 = pass_base(p, p.fpM, p.lcM);
+    c3.nlines = N; c3.in_so = H; c3.in_s = 1;
+    c3.out_so = N; c3.out_s = 1; c3.n_store = N; c3.off_store = i0; c3.out_scale = fac;
+    c3.in = p.s1; c3.out = out; c3.dot = dot;
+    launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
+  } else {
+    const long long NH = (long long)N * H, MH = (long long)M * H;
+    PassArgs c1 = pass_base(p, p.fpM, p.lcM);
+    c1.nlines = N * N; c1.in_so = N; c1.in_s = 1; c1.n_load = N; c1.off_load = i0;
+    c1.out_so = H; c1.out_s = 1; c1.n_store = H; c1.off_store = 0;
+    c1.in = rho; c1.out = p.s1;
+    launch_pass(1, LD_R, DIR_FWD, MUL_NONE, ST_C, c1, p.stream);
+    PassArgs c2 = pass_base(p, p.fpM, p.lcM);
+    c2.nlines = (int)NH; c2.ni = H; c2.in_so = NH; c2.in_si = 1; c2.in_s = H; c2.n_load = N;
+    c2.off_load = i0;
+    c2.out_so = MH; c2.out_si = 1; c2.out_s = H; c2.n_store = M; c2.off_store = 0;
+    c2.in = p.s1; c2.out = p.s2;
+    launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, c2, p.stream);
+    PassArgs c3 = pass_base(p, p.fpM, p.lcM);
+    c3.nlines = (int)MH; c3.ni = (int)MH; c3.in_so = 0; c3.in_si = 1; c3.in_s = MH; c3.n_load = N;
+    c3.off_load = i0;
+    c3.out_so = 0; c3.out_si = 1; c3.out_s = MH; c3.n_store = N; c3.off_store = i0;
+    c3.in = p.s2; c3.out = p.s2; c3.khat = p.khat[which];
+    launch_pass(0, LD_C, DIR_FMI, MUL_KHAT3, ST_C, c3, p.stream);
+    PassArgs c4 = pass_base(p, p.fpM, p.lcM);
+    c4.nlines = (int)NH; c4.ni = H; c4.in_so = MH; c4.in_si = 1; c4.in_s = H; c4.n_load = M;
+    c4.off_load = 0;
+    c4.out_so = NH; c4.out_si = 1; c4.out_s = H; c4.n_store = N; c4.off_store = i0;
+    c4.in = p.s2; c4.out = p.s1;
+    launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, c4, p.stream);
+    PassArgs c5 = pass_base(p, p.fpM, p.lcM);
+    c5.nlines = N * N; c5.in_so = H; c5.in_s = 1;
+    c5.out_so = N; c5.out_s = 1; c5.n_store = N; c5.off_store = i0; c5.out_scale = fac;
+    c5.in = p.s1; c5.out = out; c5.dot = dot;
+    launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
+  }
+}
+
+static KinArgs kin_base(const Plan& p) {
+  KinArgs a;
+  memset(&a, 0, sizeof(a));
+  a.fp = p.fpN;
+  a.tpl = p.lcN.tpl;
+  a.lpw = p.lcN.lpw;
+  a.N = p.N;
+  a.nlines = (int)(p.n / p.N);
+  a.dx = p.dx;
+  a.invNdx = p.invNdx;
+  a.sc = 1.0;
+  a.tc = p.tc;
+  a.s_mode = p.s_mode;
+  a.s_half = p.s_half;
+  a.phisc = 1.0;
+  a.partials = p.partials;
+  return a;
+}
+
+void run_energy_sums(Plan& p, const double2* psi, const double2* d, double t, double* rho_out) {
+  const int axes2[2] = {A2Y, A2X}, axes3[3] = {A3Z, A3Y, A3X};
+  const int* ax = p.ndim == 2 ? axes2 : axes3;
+  for (int k = 0; k < p.ndim; ++k) {
+    KinArgs a = kin_base(p);
+    a.psi = psi;
+    a.d = d;
+    a.t = t;
+    if (k == 0) {
+      a.rho_out = rho_out;
+      a.counter = p.counters + C_EFIRST;
+      a.red_out = p.red + R_EFIRST;
+    } else {
+      a.counter = p.counters + (k == 1 ? C_EREST0 : C_EREST1);
+      a.red_out = p.red + R_EREST + 2 * (k - 1);
+    }
+    launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
+  }
+}
+
+Energy energy_from_sums(const Plan& p, const Phys& ph, bool lr, bool normalise) {
+  const double* r = p.hred;
+  double kin = r[R_EFIRST + 0], lz = r[R_EFIRST + 1];
+  for (int k = 1; k < p.ndim; ++k) {
+    kin += r[R_EREST + 2 * (k - 1)];
+    lz += r[R_EREST + 2 * (k - 1) + 1];
+  }
+  const double dv = p.dv;
+  Energy e;
+  e.nrm2 = r[R_EFIRST + 2] * dv;
+  const double i1 = normalise ? 1.0 / e.nrm2 : 1.0;
+  const double i2 = i1 * i1, i3 = i2 * i1;
+  e.Ekin = kin * dv * i1;
+  e.Etrap = r[R_EFIRST + 3] * dv * i1;
+  e.Econtact = 0.5 * ph.beta2 * (r[R_EFIRST + 4] * dv) * i2;
+  e.Ethree = (ph.beta3 / 3.0) * (r[R_EFIRST + 5] * dv) * i3;
+  e.Lz = lz * dv * i1;
+  e.Erot = -ph.Omega * e.Lz;
+  e.Egrav = lr ? 0.5 * (r[R_EGRAV] * dv) * i2 : 0.0;
+  e.E = e.Ekin + e.Etrap + e.Econtact + e.Ethree + e.Erot + e.Egrav;
+  return e;
+}
+
+void run_H(Plan& p, const double2* psi_in, const double2* d, double t, double div, double2* psi_out,
+           const Phys& ph, bool local_on, const double* phi, double phisc, double* phi_out, double mu,
+           double kf, double rf, double2* out, bool reduce) {
+  const int axes2[2] = {A2Y, A2X}, axes3[3] = {A3Z, A3Y, A3X};
+  const int* ax = p.ndim == 2 ? axes2 : axes3;
+  const double2* psi_rest = psi_out ? psi_out : psi_in;
+  if ((d || div != 1.0) && !psi_out) throw BecError(1, "run_H: a transformed state must be materialised");
+  for (int k = 0; k < p.ndim; ++k) {
+    KinArgs a = kin_base(p);
+    a.kf = kf;
+    a.rf = rf;
+    const bool last = (k == p.ndim - 1);
+    if (k == 0) {
+      a.psi = psi_in;
+      a.d = d;
+      a.t = t;
+      a.sc = div;
+      a.psi_out = psi_out;
+      a.out = last ? out : p.w1;
+    } else {
+      a.psi = psi_rest;
+      a.acc = p.w1;
+      a.out = last ? out : p.w1;
+    }
+    if (last) {
+      if (local_on) {
+        a.beta2 = ph.beta2;
+        a.beta3 = ph.beta3;
+      } else {
+        a.tc = 0.0;
+        a.s_mode = 0;
+      }
+      a.phi = local_on ? phi : nullptr;
+      a.phisc = phisc;
+      a.phi_out = phi_out;
+      a.mu = mu;
+      a.partials = reduce ? p.partials : nullptr;
+      a.counter = p.counters + C_HLAST;
+      a.red_out = p.red + R_HLAST;
+    }
+    const int kind = (k == 0) ? KH_FIRST : (last ? KH_LAST : KH_MID);
+    launch_kin(ax[k], kind, a, p.stream);
+  }
+}
+
+void run_precond(Plan& p, const double2* g, double2* out) {
+  const int N = p.N;
+  const double inv_n = 1.0 / (double)p.n;
+  if (p.ndim == 2) {
+    PassArgs a = pass_base(p, p.fpN, p.lcN);
+    a.nlines = N; a.in_so = N; a.in_s = 1; a.n_load = N; a.out_so = N; a.out_s = 1; a.n_store = N;
+    a.in = g; a.out = p.w1;
+    launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, a, p.stream);
+    PassArgs b = pass_base(p, p.fpN, p.lcN);
+    b.nlines = N; b.ni = N; b.in_si = 1; b.in_s = N; b.n_load = N; b.out_si = 1; b.out_s = N;
+    b.n_store = N; b.in = p.w1; b.out = p.w1;
+    launch_pass(0, LD_C, DIR_FMI, MUL_PRE2, ST_C, b, p.stream);
+    PassArgs c = a;
+    c.in = p.w1; c.out = out; c.out_scale = inv_n;
+    launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, c, p.stream);
+  } else {
+    const long long N2 = (long long)N * N;
+    PassArgs z = pass_base(p, p.fpN, p.lcN);
+    z.nlines = (int)N2; z.in_so = N; z.in_s = 1; z.n_load = N; z.out_so = N; z.out_s = 1; z.n_store = N;
+    z.in = g; z.out = p.w1;
+    launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, z, p.stream);
+    PassArgs y = pass_base(p, p.fpN, p.lcN);
+    y.nlines = (int)N2; y.ni = N; y.in_so = N2; y.in_si = 1; y.in_s = N; y.n_load = N;
+    y.out_so = N2; y.out_si = 1; y.out_s = N; y.n_store = N; y.in = p.w1; y.out = p.w1;
+    launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, y, p.stream);
+    PassArgs x = pass_base(p, p.fpN, p.lcN);
+    x.nlines = (int)N2; x.ni = (int)N2; x.in_si = 1; x.in_s = N2; x.n_load = N;
+    x.out_si = 1; x.out_s = N2; x.n_store = N; x.in = p.w1; x.out = p.w1;
+    launch_pass(0, LD_C, DIR_FMI, MUL_PRE3, ST_C, x, p.stream);
+    PassArgs yi = y;
+    launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, yi, p.stream);
+    PassArgs zi = z;
+    zi.in = p.w1; zi.out = out; zi.out_scale = inv_n;
+    launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, zi, p.stream);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plan lifecycle
+// ---------------------------------------------------------------------------
+static void plan_free(Plan& p) {
+  solver_release(p);
+  void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1], p.w1, p.w2, p.s1, p.s2, p.rho, p.partials,
+                  p.counters, p.red, p.loc, p.cand, p.hist, p.stats, p.sigma};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (p.hred) (void)hipHostFree(p.hred);
+}
+
+extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff,
+                               bec_plan** out) {
+  bec_plan* h = nullptr;
+  BEC_TRY({
+    if (!out) bad("null output pointer");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) bad("ndim must be 2 or 3");
+    if (!(L > 0.0)) bad("L must be positive");
+    if (pad < 2) pad = 2;
+    h = new bec_plan();
+    Plan& p = h->p;
+    memset(&p, 0, sizeof(Plan));
+    p.ndim = ndim;
+    p.N = N;
+    p.pad = pad;
+    p.M = pad * N;
+    p.H = p.M / 2 + 1;
+    if (p.M % 2) bad("padded length must be even");
+    p.n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
+    p.nM = (ndim == 2) ? (long long)p.M * p.M : (long long)p.M * p.M * p.M;
+    p.nS1 = (ndim == 2) ? (long long)N * p.H : (long long)N * N * p.H;
+    p.nS2 = (ndim == 2) ? 0 : (long long)N * p.M * p.H;
+    p.Lbox = L;
+    p.dx = 2.0 * L / N;
+    p.dv = (ndim == 2) ? p.dx * p.dx : p.dx * p.dx * p.dx;
+    p.s = s;
+    p.tc = trap_coeff;
+    p.s_half = s / 2.0;
+    p.s_mode = (s > 0) ? (p.s_half == 1.0 ? 1 : (p.s_half == 2.0 ? 2 : 3)) : 0;
+    p.invNdx = 1.0 / (N * p.dx);
+    p.stream = 0;
+    try {
+      make_fft(N, p.fpN, &p.twN, p.stream);
+      make_fft(p.M, p.fpM, &p.twM, p.stream);
+    } catch (const BecError& e) {
+      throw;
+    }
+    p.lcN = line_cfg(N);
+    p.lcM = line_cfg(p.M);
+    HIPCHK(hipMalloc(&p.w1, sizeof(double2) * p.n));
+    HIPCHK(hipMalloc(&p.w2, sizeof(double2) * p.n));
+    HIPCHK(hipMalloc(&p.s1, sizeof(double2) * p.nS1));
+    if (p.nS2) HIPCHK(hipMalloc(&p.s2, sizeof(double2) * p.nS2));
+    HIPCHK(hipMalloc(&p.rho, sizeof(double) * p.n));
+    // partial sums: enough blocks for the widest launch, 32 values each
+    long long lines_max = std::max<long long>(p.n / N, p.nS2 ? (long long)p.M * p.H : p.H);
+    lines_max = std::max<long long>(lines_max, (long long)p.N * p.N);
+    p.max_blocks = (int)std::max<long long>(4096, lines_max + 8);
+    HIPCHK(hipMalloc(&p.partials, sizeof(double) * 32 * p.max_blocks));
+    HIPCHK(hipMalloc(&p.counters, sizeof(unsigned) * 64));
+    HIPCHK(hipMemsetAsync(p.counters, 0, sizeof(unsigned) * 64, p.stream));
+    HIPCHK(hipMalloc(&p.red, sizeof(double) * 256));
+    HIPCHK(hipMemsetAsync(p.red, 0, sizeof(double) * 256, p.stream));
+    HIPCHK(hipHostMalloc(&p.hred, sizeof(double) * 256, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&p.loc, sizeof(double) * p.n));
+    HIPCHK(hipMalloc(&p.cand, sizeof(double) * p.n));
+    HIPCHK(hipMalloc(&p.hist, sizeof(unsigned) * SIGMA_BINS));
+    HIPCHK(hipMalloc(&p.stats, sizeof(SigmaStats)));
+    HIPCHK(hipMalloc(&p.sigma, sizeof(double)));
+    launch_init_sigma(p, p.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(p.stream));
+    *out = h;
+    h = nullptr;
+  });
+}
+
+extern "C" int bec_plan_destroy(bec_plan* h) {
+  BEC_TRY({
+    if (!h) return 0;
+    (void)hipStreamSynchronize(h->p.stream);
+    plan_free(h->p);
+    delete h;
+  });
+}
+
+extern "C" int bec_plan_set_stream(bec_plan* h, void* stream) {
+  BEC_TRY({
+    if (!h) bad("null plan");
+    h->p.stream = (hipStream_t)stream;
+  });
+}
+
+// forward FFT of a full M^d complex array in place (kernel spectra)
+static void fft_full_M(Plan& p, double2* a) {
+  const int M = p.M;
+  if (p.ndim == 2) {
+    PassArgs r = pass_base(p, p.fpM, p.lcM);
+    r.nlines = M; r.in_so = M; r.in_s = 1; r.n_load = M; r.out_so = M; r.out_s = 1; r.n_store = M;
+    r.in = a; r.out = a;
+    launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, r, p.stream);
+    PassArgs c = pass_base(p, p.fpM, p.lcM);
+    c.nlines = M; c.ni = M; c.in_si = 1; c.in_s = M; c.n_load = M; c.out_si = 1; c.out_s = M;
+    c.n_store = M; c.in = a; c.out = a;
+    launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, c, p.stream);
+  } else {
+    const long long M2 = (long long)M * M;
+    PassArgs z = pass_base(p, p.fpM, p.lcM);
+    z.nlines = (int)M2; z.in_so = M; z.in_s = 1; z.n_load = M; z.out_so = M; z.out_s = 1; z.n_store = M;
+    z.in = a; z.out = a;
+    launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, z, p.stream);
+    PassArgs y = pass_base(p, p.fpM, p.lcM);
+    y.nlines = (int)M2; y.ni = M; y.in_so = M2; y.in_si = 1; y.in_s = M; y.n_load = M;
+    y.out_so = M2; y.out_si = 1; y.out_s = M; y.n_store = M; y.in = a; y.out = a;
+    launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, y, p.stream);
+    PassArgs x = pass_base(p, p.fpM, p.lcM);
+    x.nlines = (int)M2; x.ni = (int)M2; x.in_si = 1; x.in_s = M2; x.n_load = M;
+    x.out_si = 1; x.out_s = M2; x.n_store = M; x.in = a; x.out = a;
+    launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, x, p.stream);
+  }
+}
+
+static void finish_kernel(Plan& p, int which, double2* full) {
+  fft_full_M(p, full);
+  const long long nh = (p.ndim == 2) ? (long long)p.H * p.H : (long long)p.H * p.H * p.H;
+  if (!p.khat[which]) HIPCHK(hipMalloc(&p.khat[which], sizeof(double) * nh));
+  launch_extract_khat(p.ndim, p.M, p.H, full, p.khat[which], p.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p.stream));
+}
+
+// exact self-cell averages (interactions.py:37-43, 130-133, 146)
+static const double SELF_NEWTON_3D = 2.3800773640;
+
+extern "C" int bec_plan_build_kernel(bec_plan* h, int which, int kind) {
+  BEC_TRY({
+    if (!h) bad("null plan");
+    Plan& p = h->p;
+    if (which != BEC_WHICH_K && which != BEC_WHICH_W) bad("which must be K or W");
+    if (kind != BEC_KERNEL_NEWTON && kind != BEC_KERNEL_LOG) bad("device-built kernels: newton or log");
+    if (kind == BEC_KERNEL_LOG && p.ndim != 2) bad("kernel='log' is two-dimensional only");
+    double2* full = nullptr;
+    HIPCHK(hipMalloc(&full, sizeof(double2) * p.nM));
+    try {
+      launch_kernel_real(p.ndim, p.M, p.dx, kind, full, p.stream);
+      double self;
+      if (p.ndim == 3) self = SELF_NEWTON_3D / p.dx;
+      else if (kind == BEC_KERNEL_NEWTON) self = 4.0 * asinh(1.0) / p.dx;
+      else self = -log(p.dx) + (1.5 + 0.5 * log(2.0) - 0.25 * M_PI);
+      const double2 sc = make_double2(self, 0.0);
+      HIPCHK(hipMemcpyAsync(full, &sc, sizeof(double2), hipMemcpyHostToDevice, p.stream));
+      HIPCHK(hipStreamSynchronize(p.stream));
+      finish_kernel(p, which, full);
+    } catch (...) {
+      (void)hipFree(full);
+      throw;
+    }
+    HIPCHK(hipFree(full));
+    p.kkind[which] = kind;
+  });
+}
+
+__global__ void k_real_to_complex(const double* in, double2* out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = make_double2(in[i], 0.0);
+}
+
+extern "C" int bec_plan_set_kernel_table(bec_plan* h, int which, const double* ker_real) {
+  BEC_TRY({
+    if (!h || !ker_real) bad("null argument");
+    Plan& p = h->p;
+    if (which != BEC_WHICH_K && which != BEC_WHICH_W) bad("which must be K or W");
+    double2* full = nullptr;
+    HIPCHK(hipMalloc(&full, sizeof(double2) * p.nM));
+    try {
+      hipLaunchKernelGGL(k_real_to_complex, dim3(2048), dim3(256), 0, p.stream, ker_real, full, p.nM);
+      finish_kernel(p, which, full);
+    } catch (...) {
+      (void)hipFree(full);
+      throw;
+    }
+    HIPCHK(hipFree(full));
+    p.kkind[which] = BEC_KERNEL_TABLE;
+  });
+}
+
+extern "C" int bec_plan_clear_kernels(bec_plan* h) {
+  BEC_TRY({
+    if (!h) bad("null plan");
+    Plan& p = h->p;
+    HIPCHK(hipStreamSynchronize(p.stream));
+    for (int w = 0; w < 2; ++w) {
+      if (p.khat[w]) HIPCHK(hipFree(p.khat[w]));
+      p.khat[w] = nullptr;
+      p.kkind[w] = 0;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// operator entry points
+// ---------------------------------------------------------------------------
+static Phys phys_of(const bec_phys* ph) {
+  if (!ph) bad("null physics parameters");
+  Phys q;
+  q.beta2 = ph->beta2;
+  q.beta3 = ph->beta3;
+  q.Omega = ph->Omega;
+  q.G_C = ph->G_C;
+  return q;
+}
+
+extern "C" int bec_norm2(bec_plan* h, const void* psi, double* sum_out) {
+  BEC_TRY({
+    if (!h || !psi || !sum_out) bad("null argument");
+    Plan& p = h->p;
+    launch_norm2((const double2*)psi, p.n, p.partials, p.counters + C_NORM, p.red + R_NORM, p.stream);
+    HIPCHK(hipGetLastError());
+    copy_red(p, R_NORM, 1);
+    plan_sync(p);
+    *sum_out = p.hred[R_NORM];
+  });
+}
+
+extern "C" int bec_conv(bec_plan* h, int which, const double* rho, double scale, double* out) {
+  BEC_TRY({
+    if (!h || !rho || !out) bad("null argument");
+    if (which != BEC_WHICH_K && which != BEC_WHICH_W) bad("which must be K or W");
+    Plan& p = h->p;
+    run_conv(p, which, rho, scale, out, nullptr);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+extern "C" int bec_energy(bec_plan* h, const void* psi, const bec_phys* php, double* comps,
+                          double* phi_out) {
+  BEC_TRY({
+    if (!h || !psi || !comps) bad("null argument");
+    Plan& p = h->p;
+    const Phys ph = phys_of(php);
+    const bool lr = plan_long_range(p, ph);
+    run_energy_sums(p, (const double2*)psi, nullptr, 0.0, lr ? p.rho : nullptr);
+    if (lr) {
+      double* phi = phi_out ? phi_out : (double*)p.w2;
+      run_conv(p, 0, p.rho, -ph.G_C, phi, p.rho);
+    } else if (phi_out) {
+      HIPCHK(hipMemsetAsync(phi_out, 0, sizeof(double) * p.n, p.stream));
+    }
+    HIPCHK(hipGetLastError());
+    copy_red(p, 0, R_EGRAV + 1);
+    plan_sync(p);
+    const Energy e = energy_from_sums(p, ph, lr, false);
+    const double v[8] = {e.E, e.Ekin, e.Etrap, e.Econtact, e.Ethree, e.Erot, e.Egrav, e.Lz};
+    memcpy(comps, v, sizeof(v));
+  });
+}
+
+extern "C" int bec_apply_H(bec_plan* h, const void* psi, const bec_phys* php, const double* phi,
+                           void* out) {
+  BEC_TRY({
+    if (!h || !psi || !out) bad("null argument");
+    Plan& p = h->p;
+    const Phys ph = phys_of(php);
+    const bool lr = plan_long_range(p, ph);
+    const double* phiu = nullptr;
+    if (phi) phiu = phi;
+    else if (lr) {
+      launch_rho((const double2*)psi, p.rho, p.n, p.stream);
+      double* tmp = (double*)p.w2;
+      run_conv(p, 0, p.rho, -ph.G_C, tmp, nullptr);
+      phiu = tmp;
+    }
+    run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, true, phiu, 1.0, nullptr, 0.0, 0.5,
+          -ph.Omega, (double2*)out, false);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+extern "C" int bec_Lz(bec_plan* h, const void* psi, void* out) {
+  BEC_TRY({
+    if (!h || !psi || !out) bad("null argument");
+    Plan& p = h->p;
+    Phys ph = {0.0, 0.0, 0.0, 0.0};
+    run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, false, nullptr, 1.0, nullptr, 0.0, 0.0,
+          1.0, (double2*)out, false);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+extern "C" int bec_residual(bec_plan* h, const void* psi, const bec_phys* php, double* out3) {
+  BEC_TRY({
+    if (!h || !psi || !out3) bad("null argument");
+    Plan& p = h->p;
+    const Phys ph = phys_of(php);
+    const bool lr = plan_long_range(p, ph);
+    const double* phiu = nullptr;
+    if (lr) {
+      launch_rho((const double2*)psi, p.rho, p.n, p.stream);
+      run_conv(p, 0, p.rho, -ph.G_C, (double*)p.w2, nullptr);
+      phiu = (const double*)p.w2;
+    }
+    // h = H psi (mu = 0); Re <psi, H psi> from the fused sums
+    double2* hbuf = (double2*)p.s1;   // s1 holds >= n complex values
+    if (p.nS1 < p.n) throw BecError(2, "internal: scratch too small");
+    run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, true, phiu, 1.0, nullptr, 0.0, 0.5,
+          -ph.Omega, hbuf, true);
+    copy_red(p, R_HLAST, 3);
+    plan_sync(p);
+    const double mu = p.hred[R_HLAST + 1] * p.dv;
+    launch_res2(hbuf, (const double2*)psi, mu, nullptr, p.n, p.partials, p.counters + C_RES2,
+                p.red + R_RES2, p.stream);
+    HIPCHK(hipGetLastError());
+    copy_red(p, R_RES2, 1);
+    plan_sync(p);
+    const double resid = sqrt(p.hred[R_RES2] * p.dv);
+    out3[0] = resid;
+    out3[1] = mu;
+    out3[2] = resid / std::max(1.0, fabs(mu));
+  });
+}
+
+extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, const bec_phys* php,
+                                const double* phi, double shift_min, void* out, double* sigma_out) {
+  BEC_TRY({
+    if (!h || !g || !psi || !out) bad("null argument");
+    Plan& p = h->p;
+    const Phys ph = phys_of(php);
+    const bool lr = plan_long_range(p, ph);
+    const double* phiu = nullptr;
+    if (lr) {
+      if (phi) phiu = phi;
+      else {
+        launch_rho((const double2*)psi, p.rho, p.n, p.stream);
+        run_conv(p, 0, p.rho, -ph.G_C, (double*)p.w2, nullptr);
+        phiu = (const double*)p.w2;
+      }
+    }
+    // max rho via the moments reduction
+    launch_moments((const double2*)psi, p.ndim, p.N, p.dx, p.partials, p.counters + C_MOM, p.red + R_MOM,
+                   p.stream);
+    launch_sigma(p, (const double2*)psi, phiu, 1.0, p.red + R_MOM + 4, ph.beta2, ph.beta3, shift_min,
+                 p.stream);
+    double2* z = (double2*)out;
+    run_precond(p, (const double2*)g, z);
+    // projection onto the tangent space: z - psi <psi, z> dv
+    launch_dots(z, (const double2*)psi, z, z, z, (const double2*)psi, p.n, p.partials,
+                p.counters + C_DOTS, p.red + R_DOTS, p.stream);
+    copy_red(p, R_DOTS, 2);
+    copy_red(p, R_SIGMA, 1);
+    plan_sync(p);
+    const double cre = p.hred[R_DOTS + 0] * p.dv, cim = p.hred[R_DOTS + 1] * p.dv;
+    launch_combine(z, z, nullptr, (const double2*)psi, 1.0, 0.0, 0.0, -cre, -cim, p.n, p.stream);
+    HIPCHK(hipGetLastError());
+    plan_sync(p);
+    if (sigma_out) *sigma_out = p.hred[R_SIGMA];
+  });
+}
+
+extern "C" int bec_moments(bec_plan* h, const void* psi, double* out5) {
+  BEC_TRY({
+    if (!h || !psi || !out5) bad("null argument");
+    Plan& p = h->p;
+    launch_moments((const double2*)psi, p.ndim, p.N, p.dx, p.partials, p.counters + C_MOM, p.red + R_MOM,
+                   p.stream);
+    HIPCHK(hipGetLastError());
+    copy_red(p, R_MOM, 5);
+    plan_sync(p);
+    memcpy(out5, p.hred + R_MOM, sizeof(double) * 5);
+  });
+}

@@ -1,0 +1,143 @@
+"""Energy functional, mean-field Hamiltonian, residual and observables
+(becgpp/operators.py:10-103), computed by libbecgpp_hip.
+
+The kinetic and rotation terms are applied axis by axis: along an axis with
+wavenumber k, T - Omega L_z is the multiplier k^2/2 + Omega c k (c = -x on the
+y axis, +y on the x axis), so H psi costs one forward and one inverse line
+transform per axis, fused with the local terms; the energy needs forward line
+transforms only (Parseval per line).  Results equal the reference's full
+n-dimensional FFT formulation up to round-off.
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+from .constants import DEV
+from .grid import _ptr, as_field, dV, plan_of
+from .interactions import ensure_kernel, kernel_convolution
+
+_COMP_KEYS = ("E", "Ekin", "Etrap", "Econtact", "Ethree", "Erot", "Egrav", "Lz")
+
+
+def _lr_on(G, gc):
+    return G["kernel"] != "none" and abs(gc) > 1e-15
+
+
+def _plan(G, gc):
+    return ensure_kernel(G, "K") if _lr_on(G, gc) else plan_of(G)
+
+
+def _phys(b2, b3, O, gc):
+    return _lib.phys(b2, b3, O, gc)
+
+
+def energy_components(p, G, beta2, beta3, Omega, G_C):
+    """operators.py:10-29: E and its parts as 0-d tensors, plus phi."""
+    pl = _plan(G, G_C)
+    t = as_field(p, G)
+    phi = torch.empty(t.shape, dtype=torch.float64, device=DEV)
+    comps = (ctypes.c_double * 8)()
+    ph = _phys(beta2, beta3, Omega, G_C)
+    _lib.check(pl.lib.bec_energy(pl.h, _ptr(t), ctypes.byref(ph), comps, _ptr(phi)))
+    out = {k: torch.tensor(comps[i], dtype=torch.float64, device=DEV) for i, k in enumerate(_COMP_KEYS)}
+    out["phi"] = phi
+    return out
+
+
+def energy_t(p, G, beta2, beta3, Omega, G_C):
+    return energy_components(p, G, beta2, beta3, Omega, G_C)["E"]
+
+
+def apply_H(psi, G, cfg, phi=None):
+    """operators.py:36-45: mean-field H psi; phi reuses a long-range potential."""
+    b2, b3, O, gc = cfg["beta2"], cfg["beta3"], cfg["Omega"], cfg["G_C"]
+    pl = _plan(G, gc)
+    t = as_field(psi, G)
+    out = torch.empty_like(t)
+    ph = _phys(b2, b3, O, gc)
+    if phi is not None and not (isinstance(phi, (int, float)) and phi == 0):
+        f = as_field(phi, G, torch.float64)
+        _lib.check(pl.lib.bec_apply_H(pl.h, _ptr(t), ctypes.byref(ph), _ptr(f), _ptr(out)))
+    else:
+        _lib.check(pl.lib.bec_apply_H(pl.h, _ptr(t), ctypes.byref(ph), None, _ptr(out)))
+    return out
+
+
+def compute_residual(psi, G, cfg):
+    """operators.py:48-52 -> (resid_abs, mu, resid_rel)."""
+    gc = cfg["G_C"]
+    pl = _plan(G, gc)
+    t = as_field(psi, G)
+    out = (ctypes.c_double * 3)()
+    ph = _phys(cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
+    _lib.check(pl.lib.bec_residual(pl.h, _ptr(t), ctypes.byref(ph), out))
+    return out[0], out[1], out[2]
+
+
+def moments(psi, G):
+    """sum rho, sum r^2 rho, sum (x^2+y^2) rho, sum z^2 rho, max rho."""
+    pl = plan_of(G)
+    t = as_field(psi, G)
+    out = (ctypes.c_double * 5)()
+    _lib.check(pl.lib.bec_moments(pl.h, _ptr(t), out))
+    return tuple(out)
+
+
+def observables(p, G, cfg):
+    """operators.py:55-103: E, mu, Lz, radii, peak and the d-dimensional
+    virial identity with the long-range term of each kernel."""
+    b2, b3, O, gc = cfg["beta2"], cfg["beta3"], cfg["Omega"], cfg["G_C"]
+    d = float(G["ndim"])
+    s = float(G["s"])
+    dv = dV(G)
+    t = as_field(p, G)
+    comp = {k: v.item() for k, v in energy_components(t, G, b2, b3, O, gc).items() if k != "phi"}
+    E = comp["E"]
+    msum, r2sum, rperp2, z2, peak = moments(t, G)
+    mass = msum * dv
+    mu = E + comp["Econtact"] + 2.0 * comp["Ethree"] + comp["Egrav"]
+    r2mean = r2sum * dv
+    kind = G["kernel"]
+    if kind == "none" or abs(gc) <= 1e-15:
+        T_lr = 0.0
+    elif kind == "newton":
+        T_lr = comp["Egrav"]
+    elif kind == "log":
+        T_lr = -0.5 * gc * mass * mass
+    else:
+        rho = (t.real ** 2 + t.imag ** 2).contiguous()
+        T_lr = 0.5 * gc * (rho * kernel_convolution(rho, G, "W")).sum().item() * dv
+    virial = (2.0 * comp["Ekin"] - s * comp["Etrap"] + d * comp["Econtact"] + 2.0 * d * comp["Ethree"]) + T_lr
+    vscale = sum(abs(comp[k]) for k in ("Ekin", "Etrap", "Econtact", "Ethree", "Egrav")) + abs(T_lr)
+    binding_ref = 1.0 if (abs(abs(O) - 1.0) < 1e-12 and s == 2 and G["ndim"] == 2) else float("nan")
+    out = dict(E=E, mu=mu, Lz=comp["Lz"], rrms=math.sqrt(max(r2mean, 0.0)), peak=peak,
+               E_bind=(E - binding_ref) if math.isfinite(binding_ref) else float("nan"),
+               virial=virial, virial_rel=abs(virial) / max(1.0, vscale), virial_lr_term=T_lr)
+    for k in ("Ekin", "Etrap", "Econtact", "Ethree", "Erot", "Egrav"):
+        out[k] = comp[k]
+    if G["ndim"] == 3:
+        out["R_perp_rms"] = math.sqrt(max(rperp2 * dv, 0.0))
+        out["R_z_rms"] = math.sqrt(max(z2 * dv, 0.0))
+        out["oblateness"] = (out["R_perp_rms"] / math.sqrt(2.0)) / max(out["R_z_rms"], 1e-30)
+    else:
+        out["R_perp_rms"] = out["R_z_rms"] = out["oblateness"] = float("nan")
+    return out
+
+
+def preconditioner(grad, psi, G, cfg, phi=None):
+    """solvers.py:28-45: (proj_tangent(IFFT(FFT(grad)/(k^2/2 + sigma))), sigma)."""
+    gc = cfg["G_C"]
+    pl = _plan(G, gc)
+    g = as_field(grad, G)
+    ps = as_field(psi, G)
+    out = torch.empty_like(g)
+    sig = ctypes.c_double()
+    ph = _phys(cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
+    f = as_field(phi, G, torch.float64) if (phi is not None and _lr_on(G, gc)) else None
+    _lib.check(pl.lib.bec_precondition(pl.h, _ptr(g), _ptr(ps), ctypes.byref(ph),
+                                       _ptr(f) if f is not None else None,
+                                       float(cfg.get("precond_shift_min", 0.5)), _ptr(out),
+                                       ctypes.byref(sig)))
+    return out, sig.value

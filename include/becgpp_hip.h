@@ -1,0 +1,157 @@
+/* C-ABI of libbecgpp_hip.so: the MI355X (gfx950) FP64 ground-state path of
+ * becGPP.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every function returns 0 on success, 1 for an invalid argument
+ *    (the Python layer raises ValueError), 2 for a HIP/runtime failure
+ *    (RuntimeError); bec_last_error() gives the message of the last failure
+ *    on the calling thread.
+ *  - Field pointers are DEVICE pointers to C-contiguous arrays in the
+ *    reference's layout (indexing "ij", axis 0 = x): complex fields are
+ *    interleaved complex128 (double re, double im), real fields float64, with
+ *    N^ndim points.  The caller owns them; the plan owns its workspace.
+ *  - Work is ordered on the plan's stream (bec_plan_set_stream, default 0).
+ *    Functions that return host scalars synchronise that stream.  A plan is
+ *    not re-entrant: one thread per plan, or external locking.
+ *
+ * The reference (becGPP 1.2.1) is pure Python; each entry point names the
+ * Python function whose arithmetic it replaces (file:line in becgpp/).
+ */
+#ifndef BECGPP_HIP_H
+#define BECGPP_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bec_plan bec_plan;
+
+/* kernel kinds (grid.py:23-55 resolve_kernel) */
+#define BEC_KERNEL_NONE 0
+#define BEC_KERNEL_NEWTON 1
+#define BEC_KERNEL_LOG 2
+#define BEC_KERNEL_TABLE 3   /* real-space table supplied by the caller (q2d, W) */
+
+/* which spectrum: the interaction kernel K or the q2d virial kernel W
+ * (interactions.py:112 _build_kernel_fft(G, which)) */
+#define BEC_WHICH_K 0
+#define BEC_WHICH_W 1
+
+/* cg_beta variants (solvers.py:75-82) */
+#define BEC_CG_PR_PRECOND 0
+#define BEC_CG_PR 1
+#define BEC_CG_NONE 2
+
+/* stop reasons (solvers.py:117,180,207,215) */
+#define BEC_STOP_MAXIT 0
+#define BEC_STOP_CONVERGED 1
+#define BEC_STOP_ENERGY_CONVERGED 2
+#define BEC_STOP_LINE_SEARCH_STALLED 3
+
+typedef struct bec_phys {
+  double beta2, beta3, Omega, G_C;
+} bec_phys;
+
+/* solver knobs, one field per CFG key read by solvers.py:59-88 */
+typedef struct bec_solver_cfg {
+  int maxit;
+  double res_tol;
+  double energy_tol;
+  int conv_window;
+  double step;
+  double step_max;
+  int linesearch_max;
+  int linesearch_stall;
+  double precond_shift_min;
+  int cg_beta;
+  int cg_restart;
+  int verbose;
+  int check;
+} bec_solver_cfg;
+
+typedef struct bec_solve_out {
+  int iters;            /* iterations performed (solvers.py:128) */
+  int converged;        /* loop flag before the final residual (solvers.py:206,214) */
+  int stop_reason;      /* BEC_STOP_* */
+  int n_trace;          /* trace rows written */
+  double E;             /* energy of the last accepted iterate */
+  double resid_rel;     /* relative residual of the last accepted iterate */
+  double best_resid;    /* lowest relative residual seen (solvers.py:199-201) */
+  double sigma;         /* last Sobolev shift (solvers.py:232) */
+  double walltime;      /* seconds, from after the initial energy (solvers.py:107) */
+  long long n_energy;   /* energy evaluations (line-search trials + initial) */
+} bec_solve_out;
+
+const char* bec_last_error(void);
+
+/* Plan: grid geometry, FFT twiddles, workspaces, kernel spectra.
+ * Replaces make_grid (grid.py:69-101) for the device side: x_i = (i-N/2) dx,
+ * dx = 2L/N, k = 2 pi fftfreq(N, dx), V = trap_coeff r^s (0 if s = 0),
+ * padded length M = pad N.  N and pad N must factor into 2^a 3^b with
+ * pad N <= 1024; ndim is 2 or 3. */
+int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff, bec_plan** out);
+int bec_plan_destroy(bec_plan* p);
+int bec_plan_set_stream(bec_plan* p, void* hip_stream);
+
+/* Kernel spectrum of the free-space convolution (interactions.py:112-151):
+ * cell-averaged kernel on the wrap-around displacement grid, exact self cell,
+ * FFT, stored as the real-even half spectrum [(M/2+1)^ndim].
+ * bec_plan_build_kernel builds newton (2D/3D) or log (2D) on the device;
+ * bec_plan_set_kernel_table takes a caller-built real-space table [M^ndim]
+ * (device pointer; q2d K_eff and its virial kernel W, interactions.py:64-94). */
+int bec_plan_build_kernel(bec_plan* p, int which, int kind);
+int bec_plan_set_kernel_table(bec_plan* p, int which, const double* ker_real);
+/* clear_kernel_cache (interactions.py:174-176) */
+int bec_plan_clear_kernels(bec_plan* p);
+
+/* sum |psi|^2 over the grid (norm_of, grid.py:108-109, before dV and sqrt) */
+int bec_norm2(bec_plan* p, const void* psi, double* sum_out);
+
+/* out = scale * dV * (K * rho)_crop  (kernel_convolution, interactions.py:154-164,
+ * with scale = 1; long_range_phi, interactions.py:167-171, with scale = -G_C). */
+int bec_conv(bec_plan* p, int which, const double* rho, double scale, double* out);
+
+/* energy_components (operators.py:10-29): comps[8] = E, Ekin, Etrap, Econtact,
+ * Ethree, Erot, Egrav, Lz (host); phi_out (device, nullable) receives the
+ * long-range potential (zeros when the kernel is off). */
+int bec_energy(bec_plan* p, const void* psi, const bec_phys* ph, double* comps, double* phi_out);
+
+/* apply_H (operators.py:36-45); phi (nullable) reuses a long-range potential,
+ * otherwise it is computed when the kernel is on. */
+int bec_apply_H(bec_plan* p, const void* psi, const bec_phys* ph, const double* phi, void* out);
+
+/* Lz_op (grid.py:112-116): out = -i (x d_y - y d_x) psi */
+int bec_Lz(bec_plan* p, const void* psi, void* out);
+
+/* compute_residual (operators.py:48-52): out3 = resid_abs, mu, resid_rel */
+int bec_residual(bec_plan* p, const void* psi, const bec_phys* ph, double* out3);
+
+/* _preconditioner (solvers.py:28-45): out = proj_tangent(IFFT(FFT(g)/(k^2/2+sigma)));
+ * sigma_out (host) = max(lower median of the bulk local potential, shift_min). */
+int bec_precondition(bec_plan* p, const void* g, const void* psi, const bec_phys* ph,
+                     const double* phi, double shift_min, void* out, double* sigma_out);
+
+/* moments for observables (operators.py:55-103): out5 = sum rho, sum r^2 rho,
+ * sum (x^2+y^2) rho, sum z^2 rho, max rho (host, before dV). */
+int bec_moments(bec_plan* p, const void* psi, double* out5);
+
+/* ground_state loop (solvers.py:48-226): Riemannian Sobolev-preconditioned
+ * CG with Polak-Ribiere beta, Barzilai-Borwein trial step and Armijo line
+ * search.  psi_io holds psi0 on entry (normalised internally) and the
+ * returned iterate (best-residual rule, solvers.py:225-226) on exit.
+ * trace (host, nullable) receives up to trace_cap rows of
+ * (walltime, iteration, E, resid_rel) (solvers.py:110-111,197-198). */
+int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
+                     bec_solve_out* out, double* trace, int trace_cap);
+
+/* Measurement hooks (no counterpart in the reference, which has no profiler
+ * hooks): time every launch of one kernel with HIP events on its stream.
+ * id 0 switches timing off.  Ids: kinetic passes 100 + 10*axis + kind, line
+ * FFT passes 2000 + 1000*contig + 100*load + 10*dir + mul, inner products 10,
+ * shift selection 20-23 (becgpp_amd/csrc/bec_internal.h). */
+int bec_prof_watch(int id);
+int bec_prof_read(double* total_ms, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

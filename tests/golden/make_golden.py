@@ -1,0 +1,46 @@
+"""Extract the golden values this build's tests pin against from the
+reference's committed result files (run in a container where the reference is
+mounted; the output JSON is committed, the reference is not needed after).
+
+Sources (under the reference root):
+  benchmarks/results/verification/validation.csv   analytic-gate relative errors
+  benchmarks/results/itp_timing/timing.csv          converged energies at res 1e-8
+  benchmarks/results/verification/pcg_variants.csv  converged energies at res 1e-6
+  benchmarks/results/verification/kernels_log_q2d.csv  log/q2d/thin-disc energies
+"""
+import csv
+import json
+import os
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_results.json")
+
+
+def rows(rel):
+    with open(os.path.join(REF, rel), newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def main():
+    out = {}
+    out["validation"] = [dict(test=r["test"], case=r["case"], quantity=r["quantity"],
+                              rel_error=float(r["rel_error"]))
+                         for r in rows("benchmarks/results/verification/validation.csv")]
+    out["itp_timing"] = [dict(case=r["case"], method=r["method"], N=int(r["N"]), E_final=float(r["E_final"]),
+                              E_ref=float(r["E_ref"]), iters=int(r["iters_or_steps"]), stop=r["stop"],
+                              resid=float(r["resid"]) if r["resid"] not in ("", "nan") else None)
+                         for r in rows("benchmarks/results/itp_timing/timing.csv")]
+    out["pcg_variants"] = [dict(case=r["case"], cg_beta=r["cg_beta"], E=float(r["E"]), iters=int(r["iters"]),
+                                resid=float(r["resid"]), stop=r["stop"])
+                           for r in rows("benchmarks/results/verification/pcg_variants.csv")]
+    out["kernels_log_q2d"] = [{k: v for k, v in r.items()}
+                              for r in rows("benchmarks/results/verification/kernels_log_q2d.csv")]
+    out["_source"] = "becGPP 1.2.1 benchmarks/results (reference repository)"
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(f"wrote {OUT}")
+
+
+if __name__ == "__main__":
+    main()

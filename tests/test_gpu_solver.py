@@ -1,0 +1,97 @@
+"""Solver parity on the GPU: the device loop against the oracle's loop from the
+same psi0 (converged E, mu, L_z to 1e-8 relative, density to 1e-6 in L2), the
+reference's exact limits and its stop-rule regressions."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import cfg
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _both(c, psi0):
+    import becgpp_amd as B
+    Go = O.make_grid(c)
+    Gd = B.make_grid(c)
+    po, _, oo = O.ground_state(c, Go, psi0)
+    pd, _, od = B.ground_state(c, Gd, torch.as_tensor(psi0).cuda(), verbose=False)
+    return Go, po, oo, pd.cpu().numpy(), od
+
+
+def _dens_l2(a, b, G):
+    ra, rb = np.abs(a) ** 2, np.abs(b) ** 2
+    return float(np.sqrt(((ra - rb) ** 2).sum()) / np.sqrt((rb ** 2).sum()))
+
+
+@pytest.mark.parametrize("kw", [
+    dict(beta2=30.0, res_tol=1e-8),
+    dict(beta2=30.0, beta3=5.0, Omega=0.4, res_tol=1e-8),
+    dict(beta2=30.0, G_C=3.0, kernel="newton", res_tol=1e-8),
+    dict(beta2=30.0, G_C=3.0, kernel="log", res_tol=1e-8, cg_beta="pr"),
+    dict(dimension="quasi2D", beta2=30.0, G_C=3.0, kernel="q2d", l_z=0.5, res_tol=1e-8),
+    dict(dimension="3D", beta2=20.0, G_C=3.0, kernel="newton", Ngrid=24, L=5, res_tol=1e-8),
+])
+def test_converged_state_matches_oracle(kw):
+    c = cfg(**kw)
+    Go = O.make_grid(c)
+    psi0 = O.noisy_gaussian(Go, 1e-3, seed=7)
+    Go, po, oo, pd, od = _both(c, psi0)
+    for k in ("E", "mu", "Lz"):
+        assert abs(od[k] - oo[k]) <= 1e-8 * max(1.0, abs(oo[k])), (k, od[k], oo[k])
+    assert _dens_l2(pd, po, Go) < 1e-6
+    # res_tol=1e-8 is at the FP64 residual floor: either side may stop by stall just above it
+    assert od["resid_rel"] < 1e-7 and oo["resid_rel"] < 1e-7
+
+
+def test_first_iterations_track_oracle():
+    """Before round-off separates them, the device and oracle trajectories agree."""
+    c = cfg(beta2=50.0, maxit=15, res_tol=1e-14, record_trace=True)
+    Go = O.make_grid(c)
+    psi0 = O.noisy_gaussian(Go, 1e-3, seed=3)
+    Eo = []
+    O.ground_state(c, Go, psi0, callback=lambda it, E, r, t: Eo.append((it, E, r)))
+    import becgpp_amd as B
+    _, _, od = B.ground_state(c, B.make_grid(c), torch.as_tensor(psi0).cuda(), verbose=False)
+    tr = od["trace"][1:]
+    assert len(tr) == len(Eo)
+    for (t, it, E, r), (ito, Eor, ro) in zip(tr, Eo):
+        assert it == ito + 1
+        assert abs(E - Eor) <= 1e-11 * abs(Eor)
+        assert abs(r - ro) <= 1e-6 * ro
+
+
+@pytest.mark.parametrize("kw,exact", [
+    (dict(dimension="2D"), 1.0),
+    (dict(dimension="3D", Ngrid=32), 1.5),
+    (dict(dimension="2D", trap_coeff=0.125), 0.5),
+    (dict(dimension="2D", Omega=0.6), 1.0),
+])
+def test_noninteracting_ground_state_energy(kw, exact):
+    """Exact harmonic-oscillator energies (reference tests/test_audit.py:109-119)."""
+    import becgpp_amd as B
+    L = 9.0 if kw.get("trap_coeff") else 6.0
+    kw = dict(dict(Ngrid=48), **kw)
+    c = B.default_cfg(mode="single", L=L, beta2=0.0, res_tol=1e-7, maxit=600, **kw)
+    _, _, obs = B.ground_state(c, verbose=False)
+    assert abs(obs["E"] - exact) < 1e-7, obs["E"]
+
+
+def test_solver_stops_when_line_search_stalls():
+    """reference tests/test_audit.py:165-172."""
+    import becgpp_amd as B
+    c = B.default_cfg(mode="single", L=6, Ngrid=48, beta2=0.0, res_tol=1e-12, energy_tol=1e-9,
+                      conv_window=200, maxit=1000)
+    _, _, obs = B.ground_state(c, verbose=False)
+    assert abs(obs["E"] - 1.0) < 1e-10
+    assert obs["stop_reason"] in ("line_search_stalled", "energy_converged"), obs["stop_reason"]
+    assert obs["iters"] < 400, obs["iters"]
+
+
+def test_iteration_count():
+    """reference tests/test_audit.py:252-255."""
+    import becgpp_amd as B
+    c = B.default_cfg(mode="single", L=6, Ngrid=48, beta2=50.0, res_tol=1e-14, maxit=5)
+    _, _, obs = B.ground_state(c, verbose=False)
+    assert obs["iters"] == 5

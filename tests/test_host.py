@@ -1,0 +1,90 @@
+"""Host-side logic (CPU, no GPU calls): configuration API, geometry and kernel
+resolution, and the C-ABI library's exported symbols."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_default_cfg_rejects_unknown_and_maps_aliases():
+    """reference tests/test_audit.py:146-151."""
+    from becgpp_amd import default_cfg
+    assert default_cfg(res=1e-7)["res_tol"] == 1e-7
+    assert default_cfg(N=64)["Ngrid"] == 64
+    assert default_cfg(etol=1e-6)["energy_tol"] == 1e-6
+    assert default_cfg(dim="3D")["dimension"] == "3D"
+    assert default_cfg(precond_shift=0.1)["precond_shift_min"] == 0.1
+    with pytest.raises(ValueError):
+        default_cfg(omega=0.9)
+    with pytest.raises(ValueError):
+        default_cfg(res=1e-7, res_tol=1e-6)
+
+
+def test_defaults_are_safe():
+    """reference tests/test_kernels.py:80-83."""
+    from becgpp_amd.config import CFG_DEFAULTS
+    assert CFG_DEFAULTS["Omega"] == 0.0 and CFG_DEFAULTS["G_C"] == 0.0 and CFG_DEFAULTS["Ngrid"] <= 128
+    assert CFG_DEFAULTS["cg_beta"] == "pr_precond"
+
+
+def test_default_cfg_returns_fresh_copies():
+    from becgpp_amd import default_cfg
+    a = default_cfg()
+    a["beta2"] = -1.0
+    assert default_cfg()["beta2"] == 100.0
+
+
+def test_geometry_and_kernel_resolution():
+    """reference tests/test_audit.py:237-239 and tests/test_kernels.py:66-73."""
+    from becgpp_amd import default_cfg, geometry, resolve_kernel
+    with pytest.raises(ValueError):
+        geometry(dict(dimension="1D"))
+    assert geometry(dict(dimension="quasi2d")) == (2, True)
+    assert resolve_kernel(default_cfg(dimension="quasi2D", G_C=1.0, l_z=0.3))[0] == "q2d"
+    assert resolve_kernel(default_cfg(dimension="quasi2D", G_C=1.0, l_z=0.0))[0] == "newton"
+    assert resolve_kernel(default_cfg(dimension="2D", G_C=1.0))[0] == "log"
+    assert resolve_kernel(default_cfg(dimension="3D", G_C=1.0))[0] == "newton"
+    assert resolve_kernel(default_cfg(dimension="3D", G_C=0.0, kernel="newton"))[0] == "none"
+    with pytest.raises(ValueError):
+        resolve_kernel(default_cfg(dimension="3D", G_C=1.0, kernel="log"))
+    with pytest.raises(ValueError):
+        resolve_kernel(default_cfg(dimension="2D", G_C=1.0, kernel="q2d", l_z=0.0))
+    with pytest.raises(ValueError):
+        resolve_kernel(default_cfg(dimension="2D", G_C=1.0, kernel="yukawa"))
+
+
+def test_auto_grid_repulsive_long_range():
+    """reference tests/test_audit.py:242-244."""
+    from becgpp_amd import auto_grid, default_cfg
+    _, _, R = auto_grid(default_cfg(G_C=-1.0, beta2=10.0))
+    assert R > 0
+
+
+def _declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "becgpp_hip.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(bec_[A-Za-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol():
+    from becgpp_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libbecgpp_hip.so is not built (python -m becgpp_amd._build)")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    syms = _declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "becgpp_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in txt.replace("the oracle", ""), f
