@@ -10,7 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define BEC_WG 256          // threads per workgroup of the line passes
+#define BEC_WG 256          // threads per workgroup of the line passes (maximum)
+#define BEC_XCDS 8          // accelerator complex dies of an MI355X (one L2 each)
 #define BEC_MAX_STAGES 16
 #define BEC_MAX_L 1024      // longest line transform held in LDS
 

@@ -38,7 +38,7 @@ struct PassArgs {
   const double* khat; int H, M;        // real-even kernel spectrum, half table [H]^d
   const double* sigma; int Nk; double invNdx;   // Sobolev shift and grid frequencies
   const double* dot;                   // optional: accumulate sum(out_re * dot)
-  double* partials; unsigned* counter; double* red_out;
+  double* partials;                    // per-block partial rows (reducing launches)
 };
 
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
@@ -54,7 +54,7 @@ struct KinArgs {
   double beta2, beta3, tc, s_half; int s_mode;           // local terms (KE_FIRST, KH_LAST)
   const double* phi; double phisc; double* phi_out;
   double mu;
-  double* partials; unsigned* counter; double* red_out;
+  double* partials;                    // per-block partial rows (reducing launches)
 };
 
 struct SigmaStats {
@@ -62,8 +62,22 @@ struct SigmaStats {
   unsigned int cnt, bucket, rank_in, ncand;
 };
 
+// a pending two-stage reduction: per-block rows -> out / hout
+struct RedJob {
+  const double* part;
+  int nblk, K, KM;
+  double* out;
+  double* hout;
+};
+#define BEC_MAX_JOBS 4
+struct RedJobs {
+  RedJob j[BEC_MAX_JOBS];
+  int n;
+};
+
 #define SIGMA_BINS 2048
 #define SIGMA_LDS_CAND 4096
+#define SIGMA_MAX_BLOCKS 256
 
 struct Plan {
   int ndim, N, pad, M, H;
@@ -79,11 +93,13 @@ struct Plan {
   double2* w1; double2* w2;          // n-point complex scratch
   double2* s1; double2* s2;          // conv spectra
   double* rho;                       // n-point real scratch
-  double* partials; unsigned* counters; int max_blocks;
+  double* partials; long long part_stride; int max_blocks;   // one region per PartSite
+  RedJobs pend;                      // reductions awaiting k_finalize
   double* red;                       // device reduction outputs (256 doubles)
-  double* hred;                      // pinned host mirror
+  double* hred;                      // pinned, device-mapped host mirror (host view)
+  double* hred_dev;                  // the same memory as seen by kernels
   // Sobolev shift selection
-  double* loc; double* cand; unsigned* hist; SigmaStats* stats; double* sigma;
+  double* loc; double* cand; unsigned* hist; SigmaStats* stats; double* sigma; void* sig_part;
   // solver state (allocated on first solve)
   double2* st[8]; double* stphi[2];
   bool st_alloc;
@@ -103,29 +119,27 @@ enum RedSlot {
   R_TMP = 64
 };
 
-// counters (one per reducing launch site so concurrent launches never share)
-enum CounterSlot { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DOTS, C_MOM,
-                   C_RES2, C_COUNT };
+// partial-row regions, one per reducing launch site
+enum PartSite { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DOTS, C_MOM,
+                C_RES2, C_COUNT };
 
 // ---------------------------------------------------------------------------
 // launch wrappers (bec_kernels.hip)
 // ---------------------------------------------------------------------------
 LineCfg line_cfg(int L);
-void launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
-void launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
-void launch_norm2(const double2* psi, long long n, double* partials, unsigned* counter, double* out,
-                  hipStream_t s);
+int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
+int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
+int launch_norm2(const double2* psi, long long n, double* partials, hipStream_t s);
 void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s);
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, double az,
                     double ad, double ag, double gre, double gim, long long n, hipStream_t s);
-void launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
-                 const double2* gp, const double2* pp, long long n, double* partials,
-                 unsigned* counter, double* out, hipStream_t s);
-void launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
-                 double* partials, unsigned* counter, double* out, hipStream_t s);
+int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
+                const double2* gp, const double2* pp, long long n, double* partials, hipStream_t s);
+int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
+                double* partials, hipStream_t s);
 void launch_rho(const double2* psi, double* rho, long long n, hipStream_t s);
-void launch_moments(const double2* psi, int ndim, int N, double dx, double* partials,
-                    unsigned* counter, double* out, hipStream_t s);
+int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, hipStream_t s);
+void launch_finalize(const RedJobs& jobs, hipStream_t s);
 void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc,
                   const double* rhomax, double beta2, double beta3, double floor_, hipStream_t s);
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s);

@@ -16,6 +16,20 @@ __device__ __forceinline__ double kfreq(int j, int N, double invNdx) {
 }
 __device__ __forceinline__ double xcoord(int i, int N, double dx) { return (double)(i - N / 2) * dx; }
 
+// XCD-aware block order: the dispatcher deals workgroups round-robin over the
+// BEC_XCDS chiplets, each with its own L2.  Logical block (b % XCDS) * (nb /
+// XCDS) + b / XCDS gives every XCD one contiguous range of lines, so adjacent
+// strided columns (which share cache lines) are read through one L2.  nb is
+// a multiple of BEC_XCDS.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return (b % BEC_XCDS) * (nb / BEC_XCDS) + b / BEC_XCDS;
+}
+
+// agent-scope load of data written by other workgroups of the same launch
+template <typename T> __device__ __forceinline__ T ald(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, double s_half) {
   if (s_mode == 0) return 0.0;
   const double p = (s_mode == 1) ? r2 : (s_mode == 2 ? r2 * r2 : pow(r2, s_half));
@@ -23,16 +37,20 @@ __device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, d
 }
 
 // ---------------------------------------------------------------------------
-// fused grid reduction: every block stores its K sums and KM maxima; the last
-// block to arrive adds the per-block values in block order (deterministic)
-// and writes out[0 .. K+KM).  v/m are per-thread contributions.
+// Two-stage deterministic reductions.  Each workgroup folds its threads' K
+// sums and KM maxima in a fixed tree order and stores one row of KP values
+// (KP = K+KM rounded up to a power of two) with plain stores; k_finalize, the
+// next launch on the stream, folds the rows in block order.  The kernel
+// boundary makes the rows visible across the XCDs' separate L2 caches, so no
+// workgroup issues a device-wide fence (on a multi-XCD device an agent-scope
+// fence writes back and invalidates the whole L2).
 // ---------------------------------------------------------------------------
 template <int K, int KM>
-__device__ __forceinline__ void grid_reduce(const double* v, const double* m, double* partials,
-                                            unsigned* counter, double* out) {
+__device__ __forceinline__ void block_partials(const double* v, const double* m, double* partials) {
   constexpr int KT = K + KM;
-  __shared__ double red[KT * 16];
-  __shared__ int is_last;
+  constexpr int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
+  static_assert(KT <= 16, "at most 16 reduced values");
+  __shared__ double red[16 * 16];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
@@ -49,36 +67,41 @@ __device__ __forceinline__ void grid_reduce(const double* v, const double* m, do
     const int q = threadIdx.x;
     double s = red[q * 16];
     for (int w = 1; w < nw; ++w) s = (q < K) ? s + red[q * 16 + w] : fmax(s, red[q * 16 + w]);
-    __hip_atomic_store(&partials[(size_t)blockIdx.x * KT + q], s, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    partials[(size_t)blockIdx.x * KP + q] = s;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  for (int q = 0; q < KT; ++q) {
-    double s = (q < K) ? 0.0 : -INFINITY;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-      const double x = __hip_atomic_load(&partials[(size_t)b * KT + q], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-      s = (q < K) ? s + x : fmax(s, x);
+}
+
+// Folds up to BEC_MAX_JOBS pending reductions (one workgroup of BEC_WG
+// threads): thread t owns value q = t % KP of blocks t/KP, t/KP + WG/KP, ...;
+// the per-thread results are then combined in thread order.
+__global__ __launch_bounds__(BEC_WG) void k_finalize(RedJobs jobs) {
+  __shared__ double red[BEC_WG];
+  for (int jb = 0; jb < jobs.n; ++jb) {
+    const RedJob J = jobs.j[jb];
+    const int KT = J.K + J.KM;
+    const int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
+    const int q = threadIdx.x % KP, stride = blockDim.x / KP;
+    const bool isum = q < J.K;
+    double s = isum ? 0.0 : -INFINITY;
+    if (q < KT) {
+      for (int b = threadIdx.x / KP; b < J.nblk; b += stride) {
+        const double x = J.part[(size_t)b * KP + q];
+        s = isum ? s + x : fmax(s, x);
+      }
     }
-    s = (q < K) ? wave_sum(s) : wave_max(s);
-    if (lane == 0) red[wid] = s;
+    red[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = red[0];
-      for (int w = 1; w < nw; ++w) t = (q < K) ? t + red[w] : fmax(t, red[w]);
-      out[q] = t;
+    if (threadIdx.x < KT) {
+      double t = red[threadIdx.x];
+      for (int j = 1; j < stride; ++j) {
+        const double x = red[threadIdx.x + j * KP];
+        t = isum ? t + x : fmax(t, x);
+      }
+      J.out[threadIdx.x] = t;
+      if (J.hout) J.hout[threadIdx.x] = t;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -135,7 +158,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   using X = Ax<AX>;
   const int N = a.N, tpl = a.tpl, lpw = a.lpw;
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
-  const int l0 = blockIdx.x * lpw;
+  const int l0 = xcd_block(blockIdx.x, gridDim.x) * lpw;
   const long long strd = X::stride(N);
   const bool active = (l0 + c) < a.nlines;
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -191,10 +214,10 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     lz *= lzc * invN;
     if (KIND == KE_FIRST) {
       double v6[6] = {kin, lz, loc[0], loc[1], loc[2], loc[3]};
-      grid_reduce<6, 0>(v6, nullptr, a.partials, a.counter, a.red_out);
+      block_partials<6, 0>(v6, nullptr, a.partials);
     } else {
       double v2[2] = {kin, lz};
-      grid_reduce<2, 0>(v2, nullptr, a.partials, a.counter, a.red_out);
+      block_partials<2, 0>(v2, nullptr, a.partials);
     }
     return;
   }
@@ -248,7 +271,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       hmax[0] = fmax(hmax[0], rho);
     }
   }
-  if (KIND == KH_LAST && a.partials) grid_reduce<2, 1>(hsum, hmax, a.partials, a.counter, a.red_out);
+  if (KIND == KH_LAST && a.partials) block_partials<2, 1>(hsum, hmax, a.partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -259,7 +282,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   extern __shared__ double2 lds[];
   const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw;
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
-  const int l0 = blockIdx.x * lpw;
+  const int l0 = xcd_block(blockIdx.x, gridDim.x) * lpw;
   const bool active = (l0 + c) < a.nlines;
   for (int idx = threadIdx.x; idx < lpw * L; idx += blockDim.x) {
     int cc, e;
@@ -338,19 +361,19 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       if (a.dot) dsum[0] += r * a.dot[adr];
     }
   }
-  if (ST == ST_RE && a.dot) grid_reduce<1, 0>(dsum, nullptr, a.partials, a.counter, a.red_out);
+  if (ST == ST_RE && a.dot) block_partials<1, 0>(dsum, nullptr, a.partials);
 }
 
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BEC_WG) void k_norm2(const double2* __restrict__ psi, long long n,
-                                                  double* partials, unsigned* counter, double* out) {
+                                                  double* partials) {
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     v[0] += cabs2(psi[i]);
-  grid_reduce<1, 0>(v, nullptr, partials, counter, out);
+  block_partials<1, 0>(v, nullptr, partials);
 }
 
 __global__ __launch_bounds__(BEC_WG) void k_scale(const double2* __restrict__ in, double2* out,
@@ -395,7 +418,7 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
                                                  const double2* __restrict__ g,
                                                  const double2* __restrict__ gp,
                                                  const double2* __restrict__ pp, long long n,
-                                                 double* partials, unsigned* counter, double* out) {
+                                                 double* partials) {
   double v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = 0.0;
@@ -416,14 +439,13 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
     v[14] += sv.x * y.x + sv.y * y.y;
     v[15] += cabs2(sv);
   }
-  grid_reduce<16, 0>(v, nullptr, partials, counter, out);
+  block_partials<16, 0>(v, nullptr, partials);
 }
 
 // g = h - mu psi, and sum |g|^2
 __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
                                                  const double2* __restrict__ psi, double mu,
-                                                 double2* g, long long n, double* partials,
-                                                 unsigned* counter, double* out) {
+                                                 double2* g, long long n, double* partials) {
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -432,13 +454,12 @@ __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
     if (g) g[i] = r;
     v[0] += cabs2(r);
   }
-  grid_reduce<1, 0>(v, nullptr, partials, counter, out);
+  block_partials<1, 0>(v, nullptr, partials);
 }
 
 // moments: mass, sum R2 rho, sum (X^2+Y^2) rho, sum Z^2 rho; max rho
 __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ psi, int ndim, int N,
-                                                    double dx, double* partials, unsigned* counter,
-                                                    double* out) {
+                                                    double dx, double* partials) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   double m[1] = {0.0};
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
@@ -458,7 +479,7 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
     v[3] += zz * zz * rho;
     m[0] = fmax(m[0], rho);
   }
-  grid_reduce<4, 1>(v, m, partials, counter, out);
+  block_partials<4, 1>(v, m, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -470,12 +491,10 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restric
                                                         const double* rhomax, double b2, double b3,
                                                         int ndim, int N, double dx, double tc,
                                                         int s_mode, double s_half, long long n,
-                                                        double* loc, SigmaStats* st) {
-  __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64];
-  __shared__ unsigned scnt[BEC_WG / 64];
+                                                        double* loc, unsigned long long* part) {
+  __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64], scnt[BEC_WG / 64];
   const double thr = 0.05 * (*rhomax);
-  unsigned long long kmn = ~0ull, kmx = 0ull;
-  unsigned cnt = 0;
+  unsigned long long kmn = ~0ull, kmx = 0ull, cnt = 0ull;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double rho = cabs2(psi[i]);
@@ -497,11 +516,10 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restric
     }
     loc[i] = val;
   }
-  // wave then block reduce, one atomic per block
   for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long a = __shfl_down(kmn, off, 64), b = __shfl_down(kmx, off, 64);
+    const unsigned long long a = __shfl_down(kmn, off, 64), c = __shfl_down(kmx, off, 64);
     kmn = a < kmn ? a : kmn;
-    kmx = b > kmx ? b : kmx;
+    kmx = c > kmx ? c : kmx;
     cnt += __shfl_down(cnt, off, 64);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -513,11 +531,9 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restric
       kmx = smax[w] > kmx ? smax[w] : kmx;
       cnt += scnt[w];
     }
-    if (cnt) {
-      atomicMin(&st->kmin, kmn);
-      atomicMax(&st->kmax, kmx);
-      atomicAdd(&st->cnt, cnt);
-    }
+    part[4 * blockIdx.x + 0] = kmn;
+    part[4 * blockIdx.x + 1] = kmx;
+    part[4 * blockIdx.x + 2] = cnt;
   }
 }
 
@@ -526,76 +542,113 @@ __device__ __forceinline__ int sigma_bin(double v, double lo, double scale) {
   return b < 0 ? 0 : (b >= SIGMA_BINS ? SIGMA_BINS - 1 : b);
 }
 
+// fold the local pass's per-block (min, max, count) rows; every block does it
+// (at most SIGMA_MAX_BLOCKS rows) and block 0 records the result
+__device__ __forceinline__ void sigma_fold(const unsigned long long* part, int nb1, SigmaStats* st,
+                                           unsigned long long& kmn, unsigned long long& kmx,
+                                           unsigned long long& cnt) {
+  __shared__ unsigned long long a[BEC_WG], c[BEC_WG], d[BEC_WG];
+  unsigned long long mn = ~0ull, mx = 0ull, ct = 0ull;
+  for (int r = threadIdx.x; r < nb1; r += blockDim.x) {
+    const unsigned long long x = part[4 * r], y = part[4 * r + 1];
+    mn = x < mn ? x : mn;
+    mx = y > mx ? y : mx;
+    ct += part[4 * r + 2];
+  }
+  a[threadIdx.x] = mn; c[threadIdx.x] = mx; d[threadIdx.x] = ct;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      a[threadIdx.x] = a[threadIdx.x + h] < a[threadIdx.x] ? a[threadIdx.x + h] : a[threadIdx.x];
+      c[threadIdx.x] = c[threadIdx.x + h] > c[threadIdx.x] ? c[threadIdx.x + h] : c[threadIdx.x];
+      d[threadIdx.x] += d[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  kmn = a[0]; kmx = c[0]; cnt = d[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->kmin = kmn;
+    st->kmax = kmx;
+    st->cnt = (unsigned)cnt;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(BEC_WG) void k_sigma_hist(const double* __restrict__ loc, long long n,
-                                                       const SigmaStats* st, unsigned* hist) {
+                                                       const unsigned long long* part, int nb1,
+                                                       SigmaStats* st, unsigned* hist) {
   __shared__ unsigned h[SIGMA_BINS];
+  unsigned long long kmn, kmx, cnt;
+  sigma_fold(part, nb1, st, kmn, kmx, cnt);
+  const double lo = key_to_double(kmn), hi = key_to_double(kmx);
+  if (!(cnt > 0 && hi > lo)) return;
   for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) h[b] = 0u;
   __syncthreads();
-  const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
-  if (st->cnt > 0 && hi > lo) {
-    const double scale = (double)SIGMA_BINS / (hi - lo);
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-      const double v = loc[i];
-      if (v == v) atomicAdd(&h[sigma_bin(v, lo, scale)], 1u);
-    }
+  const double scale = (double)SIGMA_BINS / (hi - lo);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = loc[i];
+    if (v == v) atomicAdd(&h[sigma_bin(v, lo, scale)], 1u);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x)
     if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-// find the bin holding the rank-k element, k = (cnt-1)/2, then gather its members
+// every block locates the bin holding the rank-k element, k = (cnt-1)/2
+// (parallel prefix over the histogram), then gathers that bin's members
 __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restrict__ loc, long long n,
-                                                         SigmaStats* st, const unsigned* hist,
-                                                         double* cand) {
-  __shared__ unsigned part[BEC_WG];
+                                                         SigmaStats* st, const unsigned* hist, double* cand) {
+  __shared__ unsigned pre[BEC_WG];
   __shared__ int sbin;
-  __shared__ unsigned sbefore;
+  __shared__ unsigned srank;
   const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
   const unsigned cnt = st->cnt;
-  if (cnt == 0 || !(hi > lo)) return;
+  if (!(cnt > 0 && hi > lo)) return;
   const unsigned k = (cnt - 1u) / 2u;
   constexpr int PER = SIGMA_BINS / BEC_WG;
-  unsigned s = 0;
-  for (int j = 0; j < PER; ++j) s += hist[threadIdx.x * PER + j];
-  part[threadIdx.x] = s;
+  unsigned own = 0;
+  for (int j = 0; j < PER; ++j) own += hist[threadIdx.x * PER + j];
+  pre[threadIdx.x] = own;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned run = 0;
-    int t = 0;
-    for (; t < BEC_WG; ++t) {
-      if (run + part[t] > k) break;
-      run += part[t];
-    }
-    int b = t * PER;
-    for (int j = 0; j < PER; ++j, ++b) {
+  for (int off = 1; off < BEC_WG; off <<= 1) {          // inclusive Hillis-Steele scan
+    const unsigned add = (int)threadIdx.x >= off ? pre[threadIdx.x - off] : 0u;
+    __syncthreads();
+    pre[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const unsigned before = pre[threadIdx.x] - own;
+  if (before <= k && k < pre[threadIdx.x]) {             // exactly one thread
+    unsigned run = before;
+    int b = threadIdx.x * PER;
+    for (int j = 0; j < PER - 1; ++j, ++b) {
       const unsigned hb = hist[b];
       if (run + hb > k) break;
       run += hb;
     }
     sbin = b;
-    sbefore = run;
-    if (blockIdx.x == 0) { st->bucket = (unsigned)b; st->rank_in = k - run; }
+    srank = k - run;
   }
   __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->bucket = (unsigned)sbin;
+    st->rank_in = srank;
+  }
   const int bin = sbin;
   const double scale = (double)SIGMA_BINS / (hi - lo);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double v = loc[i];
-    if (v == v && sigma_bin(v, lo, scale) == bin) {
-      const unsigned pos = atomicAdd(&st->ncand, 1u);
-      cand[pos] = v;
-    }
+    if (v == v && sigma_bin(v, lo, scale) == bin) cand[atomicAdd(&st->ncand, 1u)] = v;
   }
 }
 
-// one workgroup of 1024 threads: exact selection, sigma = max(median, floor),
+// one workgroup: exact selection (bitonic sort in LDS, or an 8-bit radix
+// select over the order keys for a large bin), sigma = max(median, floor),
 // then reset the histogram and statistics for the next call
-__global__ __launch_bounds__(1024) void k_sigma_select(SigmaStats* st, unsigned* hist,
-                                                       const double* __restrict__ cand, double floor_,
-                                                       double* sigma_out, double* host_slot) {
+__global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* st, unsigned* hist,
+                                                         const double* __restrict__ cand, double floor_,
+                                                         double* sigma_out, double* host_slot) {
   __shared__ double sv[SIGMA_LDS_CAND];
   __shared__ unsigned dh[256];
   __shared__ unsigned long long prefix;
@@ -614,13 +667,13 @@ __global__ __launch_bounds__(1024) void k_sigma_select(SigmaStats* st, unsigned*
       while (P < m) P <<= 1;
       for (unsigned i = threadIdx.x; i < P; i += blockDim.x) sv[i] = (i < m) ? cand[i] : INFINITY;
       __syncthreads();
-      for (unsigned k = 2; k <= P; k <<= 1) {
-        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+      for (unsigned kk = 2; kk <= P; kk <<= 1) {
+        for (unsigned j = kk >> 1; j > 0; j >>= 1) {
           for (unsigned i = threadIdx.x; i < P; i += blockDim.x) {
             const unsigned ixj = i ^ j;
             if (ixj > i) {
               const double a = sv[i], b = sv[ixj];
-              const bool up = ((i & k) == 0);
+              const bool up = ((i & kk) == 0);
               if ((a > b) == up) { sv[i] = b; sv[ixj] = a; }
             }
           }
@@ -630,27 +683,26 @@ __global__ __launch_bounds__(1024) void k_sigma_select(SigmaStats* st, unsigned*
       med = sv[r];
       have = true;
     } else {
-      // radix select over the order keys of the candidates, 8 bits at a time
       if (threadIdx.x == 0) { prefix = 0ull; rank_s = r; }
       __syncthreads();
       for (int shift = 56; shift >= 0; shift -= 8) {
         for (int b = threadIdx.x; b < 256; b += blockDim.x) dh[b] = 0u;
         __syncthreads();
-        const unsigned long long pre = prefix;
+        const unsigned long long pr = prefix;
         const unsigned long long hmask = (shift == 56) ? 0ull : (~0ull << (shift + 8));
         for (unsigned i = threadIdx.x; i < m; i += blockDim.x) {
           const unsigned long long kk = order_key(cand[i]);
-          if ((kk & hmask) == pre) atomicAdd(&dh[(kk >> shift) & 0xffu], 1u);
+          if ((kk & hmask) == pr) atomicAdd(&dh[(kk >> shift) & 0xffu], 1u);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
           unsigned run = 0, rr = rank_s;
           int b = 0;
-          for (; b < 256; ++b) {
+          for (; b < 255; ++b) {
             if (run + dh[b] > rr) break;
             run += dh[b];
           }
-          prefix = pre | ((unsigned long long)b << shift);
+          prefix = pr | ((unsigned long long)b << shift);
           rank_s = rr - run;
         }
         __syncthreads();
@@ -739,7 +791,7 @@ __global__ __launch_bounds__(BEC_WG) void k_extract_khat(int ndim, int M, int H,
 // ---------------------------------------------------------------------------
 static int grid_for(long long n) {
   long long g = (n + BEC_WG - 1) / BEC_WG;
-  if (g > 2048) g = 2048;
+  if (g > 1024) g = 1024;
   if (g < 1) g = 1;
   return (int)g;
 }
@@ -755,21 +807,40 @@ LineCfg line_cfg(int L) {
   return c;
 }
 
+// Lines per workgroup for a launch: at least one full wavefront, at most
+// BEC_WG threads, aiming at ~1024 workgroups so that small (single-instance
+// 2D) problems still spread over all 256 CUs.
+static int pick_lpw(int tpl, int nlines) {
+  const int lmin = (tpl >= 64) ? 1 : 64 / tpl;
+  const int lmax = BEC_WG / tpl > lmin ? BEC_WG / tpl : lmin;
+  const int want = nlines / 1024;
+  int l = 1;
+  while (l * 2 <= want && l * 2 <= lmax) l *= 2;
+  return l < lmin ? lmin : l;
+}
+// workgroup count rounded up to a whole number of XCD rounds (see xcd_block)
+static int xcd_grid(int nlines, int lpw) {
+  const int b = (nlines + lpw - 1) / lpw;
+  return ((b + BEC_XCDS - 1) / BEC_XCDS) * BEC_XCDS;
+}
+
 template <bool CONTIG, int LD, int DIR, int MUL, int ST>
-static void go_pass(const PassArgs& a, hipStream_t s) {
-  const int blocks = (a.nlines + a.lpw - 1) / a.lpw;
+static int go_pass(const PassArgs& a0, hipStream_t s) {
+  PassArgs a = a0;
+  a.lpw = pick_lpw(a.tpl, a.nlines);
+  const int blocks = xcd_grid(a.nlines, a.lpw);
   const size_t shm = (size_t)a.lpw * a.fp.L * sizeof(double2);
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL;
   prof_pre(id, s);
   hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
   prof_post(id, s);
+  return blocks;
 }
 
-void launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
+int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
 #define BEC_P(C, LDV, D, MU, S)                                                 \
   if (contig == (C) && ld == (LDV) && dir == (D) && mul == (MU) && st == (S)) { \
-    go_pass<C, LDV, D, MU, S>(a, s);                                            \
-    return;                                                                     \
+    return go_pass<C, LDV, D, MU, S>(a, s);                                     \
   }
   // plain transforms (kernel spectra, preconditioner)
   BEC_P(1, LD_C, DIR_FWD, MUL_NONE, ST_C)
@@ -786,27 +857,31 @@ void launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a
   BEC_P(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE)
 #undef BEC_P
   abort();
+  return 0;
 }
 
 template <int AX, int KIND>
-static void go_kin(const KinArgs& a, hipStream_t s) {
-  const int blocks = (a.nlines + a.lpw - 1) / a.lpw;
+static int go_kin(const KinArgs& a0, hipStream_t s) {
+  KinArgs a = a0;
+  a.lpw = pick_lpw(a.tpl, a.nlines);
+  const int blocks = xcd_grid(a.nlines, a.lpw);
   const size_t shm = (size_t)a.lpw * a.N * sizeof(double2);
   const int id = 100 + 10 * AX + KIND;
   prof_pre(id, s);
   hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
   prof_post(id, s);
+  return blocks;
 }
 
-void launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
+int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
 #define BEC_K(AXV)                                                  \
   if (ax == AXV) {                                                  \
     switch (kind) {                                                 \
-      case KE_FIRST: go_kin<AXV, KE_FIRST>(a, s); return;           \
-      case KE_REST: go_kin<AXV, KE_REST>(a, s); return;             \
-      case KH_FIRST: go_kin<AXV, KH_FIRST>(a, s); return;           \
-      case KH_MID: go_kin<AXV, KH_MID>(a, s); return;               \
-      default: go_kin<AXV, KH_LAST>(a, s); return;                  \
+      case KE_FIRST: return go_kin<AXV, KE_FIRST>(a, s);            \
+      case KE_REST: return go_kin<AXV, KE_REST>(a, s);              \
+      case KH_FIRST: return go_kin<AXV, KH_FIRST>(a, s);            \
+      case KH_MID: return go_kin<AXV, KH_MID>(a, s);                \
+      default: return go_kin<AXV, KH_LAST>(a, s);                   \
     }                                                               \
   }
   BEC_K(A2Y)
@@ -816,11 +891,13 @@ void launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
   BEC_K(A3X)
 #undef BEC_K
   abort();
+  return 0;
 }
 
-void launch_norm2(const double2* psi, long long n, double* partials, unsigned* counter, double* out,
-                  hipStream_t s) {
-  hipLaunchKernelGGL(k_norm2, dim3(grid_for(n)), dim3(BEC_WG), 0, s, psi, n, partials, counter, out);
+int launch_norm2(const double2* psi, long long n, double* partials, hipStream_t s) {
+  const int g = grid_for(n);
+  hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BEC_WG), 0, s, psi, n, partials);
+  return g;
 }
 void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(BEC_WG), 0, s, in, out, sc, n);
@@ -830,44 +907,54 @@ void launch_rho(const double2* psi, double* rho, long long n, hipStream_t s) {
 }
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, double az,
                     double ad, double ag, double gre, double gim, long long n, hipStream_t s) {
+  prof_pre(SITE_COMBINE, s);
   hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(BEC_WG), 0, s, d, z, g, psi, az, ad, ag, gre,
                      gim, n);
+  prof_post(SITE_COMBINE, s);
 }
-void launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
-                 const double2* gp, const double2* pp, long long n, double* partials,
-                 unsigned* counter, double* out, hipStream_t s) {
+int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
+                const double2* gp, const double2* pp, long long n, double* partials, hipStream_t s) {
+  const int gr = grid_for(n);
   prof_pre(SITE_DOTS, s);
-  hipLaunchKernelGGL(k_dots, dim3(grid_for(n)), dim3(BEC_WG), 0, s, z, psi, d, g, gp, pp, n, partials,
-                     counter, out);
+  hipLaunchKernelGGL(k_dots, dim3(gr), dim3(BEC_WG), 0, s, z, psi, d, g, gp, pp, n, partials);
   prof_post(SITE_DOTS, s);
+  return gr;
 }
-void launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
-                 double* partials, unsigned* counter, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_res2, dim3(grid_for(n)), dim3(BEC_WG), 0, s, h, psi, mu, g_out, n, partials,
-                     counter, out);
+int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
+                double* partials, hipStream_t s) {
+  const int gr = grid_for(n);
+  hipLaunchKernelGGL(k_res2, dim3(gr), dim3(BEC_WG), 0, s, h, psi, mu, g_out, n, partials);
+  return gr;
 }
-void launch_moments(const double2* psi, int ndim, int N, double dx, double* partials,
-                    unsigned* counter, double* out, hipStream_t s) {
+int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, hipStream_t s) {
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
-  hipLaunchKernelGGL(k_moments, dim3(grid_for(n)), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials,
-                     counter, out);
+  const int gr = grid_for(n);
+  hipLaunchKernelGGL(k_moments, dim3(gr), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials);
+  return gr;
+}
+void launch_finalize(const RedJobs& jobs, hipStream_t s) {
+  if (jobs.n > 0) hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BEC_WG), 0, s, jobs);
 }
 void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc,
                   const double* rhomax, double beta2, double beta3, double floor_, hipStream_t s) {
+  long long g1 = (p.n + 4 * BEC_WG - 1) / (4 * BEC_WG);
+  if (g1 > SIGMA_MAX_BLOCKS) g1 = SIGMA_MAX_BLOCKS;
   const int g = grid_for(p.n);
+  unsigned long long* part = (unsigned long long*)p.sig_part;
   prof_pre(SITE_SIGMA_LOCAL, s);
-  hipLaunchKernelGGL(k_sigma_local, dim3(g), dim3(BEC_WG), 0, s, psi, phi, phisc, rhomax, beta2, beta3,
-                     p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, p.loc, p.stats);
+  hipLaunchKernelGGL(k_sigma_local, dim3((int)g1), dim3(BEC_WG), 0, s, psi, phi, phisc, rhomax, beta2,
+                     beta3, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, p.loc, part);
   prof_post(SITE_SIGMA_LOCAL, s);
   prof_pre(SITE_SIGMA_HIST, s);
-  hipLaunchKernelGGL(k_sigma_hist, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist);
+  hipLaunchKernelGGL(k_sigma_hist, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,
+                     (int)g1, p.stats, p.hist);
   prof_post(SITE_SIGMA_HIST, s);
   prof_pre(SITE_SIGMA_GATHER, s);
   hipLaunchKernelGGL(k_sigma_gather, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist, p.cand);
   prof_post(SITE_SIGMA_GATHER, s);
   prof_pre(SITE_SIGMA_SELECT, s);
-  hipLaunchKernelGGL(k_sigma_select, dim3(1), dim3(1024), 0, s, p.stats, p.hist, p.cand, floor_,
-                     p.sigma, p.red + R_SIGMA);
+  hipLaunchKernelGGL(k_sigma_select, dim3(1), dim3(BEC_WG), 0, s, p.stats, p.hist, p.cand, floor_,
+                     p.sigma, p.hred_dev + R_SIGMA);
   prof_post(SITE_SIGMA_SELECT, s);
 }
 __global__ __launch_bounds__(BEC_WG) void k_sigma_reset(SigmaStats* st, unsigned* hist, double* sigma) {
@@ -882,7 +969,6 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_reset(SigmaStats* st, unsigned
     *sigma = 0.0;
   }
 }
-
 void launch_init_sigma(const Plan& p, hipStream_t s) {
   hipLaunchKernelGGL(k_sigma_reset, dim3(1), dim3(BEC_WG), 0, s, p.stats, p.hist, p.sigma);
 }

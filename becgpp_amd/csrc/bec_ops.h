@@ -26,8 +26,10 @@ struct Energy {
 };
 
 bool plan_long_range(const Plan& p, const Phys& ph);
-void plan_sync(Plan& p);
-void copy_red(Plan& p, int off, int count);   // device red -> host hred (async)
+void plan_sync(Plan& p);                       // flush pending reductions, then synchronise
+double* part_of(Plan& p, int site);            // partial-row region of a reducing launch site
+void add_job(Plan& p, int site, int nblk, int K, int KM, int slot);   // queue a k_finalize job
+void flush_jobs(Plan& p);                      // launch k_finalize for the queued jobs
 
 // out = scale * dV * (K * rho)_crop, optionally sum(out * dot) -> red[R_EGRAV]
 void run_conv(Plan& p, int which, const double* rho, double scale, double* out, const double* dot);

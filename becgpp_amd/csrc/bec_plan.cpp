@@ -79,11 +79,28 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
 // ---------------------------------------------------------------------------
 bool plan_long_range(const Plan& p, const Phys& ph) { return p.kkind[0] != 0 && fabs(ph.G_C) > 1e-15; }
 
-void plan_sync(Plan& p) { HIPCHK(hipStreamSynchronize(p.stream)); }
+double* part_of(Plan& p, int site) { return p.partials + (size_t)site * p.part_stride; }
 
-void copy_red(Plan& p, int off, int count) {
-  HIPCHK(hipMemcpyAsync(p.hred + off, p.red + off, sizeof(double) * count, hipMemcpyDeviceToHost,
-                        p.stream));
+void add_job(Plan& p, int site, int nblk, int K, int KM, int slot) {
+  if (p.pend.n == BEC_MAX_JOBS) flush_jobs(p);
+  RedJob& j = p.pend.j[p.pend.n++];
+  j.part = part_of(p, site);
+  j.nblk = nblk;
+  j.K = K;
+  j.KM = KM;
+  j.out = p.red + slot;
+  j.hout = p.hred_dev + slot;
+}
+
+void flush_jobs(Plan& p) {
+  if (p.pend.n == 0) return;
+  launch_finalize(p.pend, p.stream);
+  p.pend.n = 0;
+}
+
+void plan_sync(Plan& p) {
+  flush_jobs(p);
+  HIPCHK(hipStreamSynchronize(p.stream));
 }
 
 static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
@@ -99,9 +116,7 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
   a.Nk = p.N;
   a.invNdx = p.invNdx;
   a.sigma = p.sigma;
-  a.partials = p.partials;
-  a.counter = p.counters + C_EGRAV;
-  a.red_out = p.red + R_EGRAV;
+  a.partials = p.partials + (size_t)C_EGRAV * p.part_stride;
   return a;
 }
 
@@ -124,7 +139,8 @@ void run_conv(Plan& p, int which, const double* rho, double scale, double* out, 
     c3.nlines = N; c3.in_so = H; c3.in_s = 1;
     c3.out_so = N; c3.out_s = 1; c3.n_store = N; c3.off_store = i0; c3.out_scale = fac;
     c3.in = p.s1; c3.out = out; c3.dot = dot;
-    launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
+    const int nb = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
+    if (dot) add_job(p, C_EGRAV, nb, 1, 0, R_EGRAV);
   } else {
     const long long NH = (long long)N * H, MH = (long long)M * H;
     PassArgs c1 = pass_base(p, p.fpM, p.lcM);
@@ -154,7 +170,8 @@ void run_conv(Plan& p, int which, const double* rho, double scale, double* out, 
     c5.nlines = N * N; c5.in_so = H; c5.in_s = 1;
     c5.out_so = N; c5.out_s = 1; c5.n_store = N; c5.off_store = i0; c5.out_scale = fac;
     c5.in = p.s1; c5.out = out; c5.dot = dot;
-    launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
+    const int nb = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
+    if (dot) add_job(p, C_EGRAV, nb, 1, 0, R_EGRAV);
   }
 }
 
@@ -173,7 +190,7 @@ static KinArgs kin_base(const Plan& p) {
   a.s_mode = p.s_mode;
   a.s_half = p.s_half;
   a.phisc = 1.0;
-  a.partials = p.partials;
+  a.partials = nullptr;
   return a;
 }
 
@@ -187,13 +204,13 @@ void run_energy_sums(Plan& p, const double2* psi, const double2* d, double t, do
     a.t = t;
     if (k == 0) {
       a.rho_out = rho_out;
-      a.counter = p.counters + C_EFIRST;
-      a.red_out = p.red + R_EFIRST;
+      a.partials = part_of(p, C_EFIRST);
     } else {
-      a.counter = p.counters + (k == 1 ? C_EREST0 : C_EREST1);
-      a.red_out = p.red + R_EREST + 2 * (k - 1);
+      a.partials = part_of(p, k == 1 ? C_EREST0 : C_EREST1);
     }
-    launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
+    const int nb = launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
+    if (k == 0) add_job(p, C_EFIRST, nb, 6, 0, R_EFIRST);
+    else add_job(p, k == 1 ? C_EREST0 : C_EREST1, nb, 2, 0, R_EREST + 2 * (k - 1));
   }
 }
 
@@ -256,12 +273,11 @@ void run_H(Plan& p, const double2* psi_in, const double2* d, double t, double di
       a.phisc = phisc;
       a.phi_out = phi_out;
       a.mu = mu;
-      a.partials = reduce ? p.partials : nullptr;
-      a.counter = p.counters + C_HLAST;
-      a.red_out = p.red + R_HLAST;
+      a.partials = reduce ? part_of(p, C_HLAST) : nullptr;
     }
     const int kind = (k == 0) ? KH_FIRST : (last ? KH_LAST : KH_MID);
-    launch_kin(ax[k], kind, a, p.stream);
+    const int nb = launch_kin(ax[k], kind, a, p.stream);
+    if (last && reduce) add_job(p, C_HLAST, nb, 2, 1, R_HLAST);
   }
 }
 
@@ -308,7 +324,7 @@ void run_precond(Plan& p, const double2* g, double2* out) {
 static void plan_free(Plan& p) {
   solver_release(p);
   void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1], p.w1, p.w2, p.s1, p.s2, p.rho, p.partials,
-                  p.counters, p.red, p.loc, p.cand, p.hist, p.stats, p.sigma};
+                  p.red, p.loc, p.cand, p.hist, p.stats, p.sigma, p.sig_part};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p.hred) (void)hipHostFree(p.hred);
@@ -362,12 +378,14 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     long long lines_max = std::max<long long>(p.n / N, p.nS2 ? (long long)p.M * p.H : p.H);
     lines_max = std::max<long long>(lines_max, (long long)p.N * p.N);
     p.max_blocks = (int)std::max<long long>(4096, lines_max + 8);
-    HIPCHK(hipMalloc(&p.partials, sizeof(double) * 32 * p.max_blocks));
-    HIPCHK(hipMalloc(&p.counters, sizeof(unsigned) * 64));
-    HIPCHK(hipMemsetAsync(p.counters, 0, sizeof(unsigned) * 64, p.stream));
+    p.part_stride = 16LL * p.max_blocks;
+    HIPCHK(hipMalloc(&p.partials, sizeof(double) * p.part_stride * C_COUNT));
+    HIPCHK(hipMalloc(&p.sig_part, sizeof(unsigned long long) * 4 * SIGMA_MAX_BLOCKS));
     HIPCHK(hipMalloc(&p.red, sizeof(double) * 256));
     HIPCHK(hipMemsetAsync(p.red, 0, sizeof(double) * 256, p.stream));
-    HIPCHK(hipHostMalloc(&p.hred, sizeof(double) * 256, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&p.hred, sizeof(double) * 256, hipHostMallocMapped | hipHostMallocCoherent));
+    memset(p.hred, 0, sizeof(double) * 256);
+    HIPCHK(hipHostGetDevicePointer((void**)&p.hred_dev, p.hred, 0));
     HIPCHK(hipMalloc(&p.loc, sizeof(double) * p.n));
     HIPCHK(hipMalloc(&p.cand, sizeof(double) * p.n));
     HIPCHK(hipMalloc(&p.hist, sizeof(unsigned) * SIGMA_BINS));
@@ -521,9 +539,8 @@ extern "C" int bec_norm2(bec_plan* h, const void* psi, double* sum_out) {
   BEC_TRY({
     if (!h || !psi || !sum_out) bad("null argument");
     Plan& p = h->p;
-    launch_norm2((const double2*)psi, p.n, p.partials, p.counters + C_NORM, p.red + R_NORM, p.stream);
+    add_job(p, C_NORM, launch_norm2((const double2*)psi, p.n, part_of(p, C_NORM), p.stream), 1, 0, R_NORM);
     HIPCHK(hipGetLastError());
-    copy_red(p, R_NORM, 1);
     plan_sync(p);
     *sum_out = p.hred[R_NORM];
   });
@@ -554,7 +571,6 @@ extern "C" int bec_energy(bec_plan* h, const void* psi, const bec_phys* php, dou
       HIPCHK(hipMemsetAsync(phi_out, 0, sizeof(double) * p.n, p.stream));
     }
     HIPCHK(hipGetLastError());
-    copy_red(p, 0, R_EGRAV + 1);
     plan_sync(p);
     const Energy e = energy_from_sums(p, ph, lr, false);
     const double v[8] = {e.E, e.Ekin, e.Etrap, e.Econtact, e.Ethree, e.Erot, e.Egrav, e.Lz};
@@ -611,13 +627,11 @@ extern "C" int bec_residual(bec_plan* h, const void* psi, const bec_phys* php, d
     if (p.nS1 < p.n) throw BecError(2, "internal: scratch too small");
     run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, true, phiu, 1.0, nullptr, 0.0, 0.5,
           -ph.Omega, hbuf, true);
-    copy_red(p, R_HLAST, 3);
     plan_sync(p);
     const double mu = p.hred[R_HLAST + 1] * p.dv;
-    launch_res2(hbuf, (const double2*)psi, mu, nullptr, p.n, p.partials, p.counters + C_RES2,
-                p.red + R_RES2, p.stream);
+    add_job(p, C_RES2, launch_res2(hbuf, (const double2*)psi, mu, nullptr, p.n, part_of(p, C_RES2), p.stream),
+            1, 0, R_RES2);
     HIPCHK(hipGetLastError());
-    copy_red(p, R_RES2, 1);
     plan_sync(p);
     const double resid = sqrt(p.hred[R_RES2] * p.dv);
     out3[0] = resid;
@@ -643,17 +657,16 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
       }
     }
     // max rho via the moments reduction
-    launch_moments((const double2*)psi, p.ndim, p.N, p.dx, p.partials, p.counters + C_MOM, p.red + R_MOM,
-                   p.stream);
+    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM), p.stream), 4, 1,
+            R_MOM);
+    flush_jobs(p);
     launch_sigma(p, (const double2*)psi, phiu, 1.0, p.red + R_MOM + 4, ph.beta2, ph.beta3, shift_min,
                  p.stream);
     double2* z = (double2*)out;
     run_precond(p, (const double2*)g, z);
     // projection onto the tangent space: z - psi <psi, z> dv
-    launch_dots(z, (const double2*)psi, z, z, z, (const double2*)psi, p.n, p.partials,
-                p.counters + C_DOTS, p.red + R_DOTS, p.stream);
-    copy_red(p, R_DOTS, 2);
-    copy_red(p, R_SIGMA, 1);
+    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, (const double2*)psi, p.n, part_of(p, C_DOTS),
+                                   p.stream), 16, 0, R_DOTS);
     plan_sync(p);
     const double cre = p.hred[R_DOTS + 0] * p.dv, cim = p.hred[R_DOTS + 1] * p.dv;
     launch_combine(z, z, nullptr, (const double2*)psi, 1.0, 0.0, 0.0, -cre, -cim, p.n, p.stream);
@@ -667,10 +680,10 @@ extern "C" int bec_moments(bec_plan* h, const void* psi, double* out5) {
   BEC_TRY({
     if (!h || !psi || !out5) bad("null argument");
     Plan& p = h->p;
-    launch_moments((const double2*)psi, p.ndim, p.N, p.dx, p.partials, p.counters + C_MOM, p.red + R_MOM,
-                   p.stream);
+    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM), p.stream), 4, 1,
+            R_MOM);
+    flush_jobs(p);
     HIPCHK(hipGetLastError());
-    copy_red(p, R_MOM, 5);
     plan_sync(p);
     memcpy(out5, p.hred + R_MOM, sizeof(double) * 5);
   });

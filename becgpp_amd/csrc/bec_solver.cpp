@@ -82,8 +82,7 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
     int n_tr = 0;
 
     // psi = psi0 / norm_of(psi0)                                   (solvers.py:56-58)
-    launch_norm2((const double2*)psi_io, n, p.partials, p.counters + C_NORM, p.red + R_NORM, s);
-    copy_red(p, R_NORM, 1);
+    add_job(p, C_NORM, launch_norm2((const double2*)psi_io, n, part_of(p, C_NORM), s), 1, 0, R_NORM);
     plan_sync(p);
     const double nrm0 = sqrt(p.hred[R_NORM] * dv);
     int c = 0;
@@ -95,7 +94,6 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
     run_energy_sums(p, P[c], nullptr, 0.0, lr ? p.rho : nullptr);
     if (lr) run_conv(p, 0, p.rho, -ph.G_C, phi, p.rho);
     HIPCHK(hipGetLastError());
-    copy_red(p, 0, R_EGRAV + 1);
     plan_sync(p);
     Energy e0 = energy_from_sums(p, ph, lr, false);
     double E = e0.E;
@@ -109,7 +107,6 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
     run_H(p, P[c], nullptr, 0.0, 1.0, nullptr, ph, true, lr ? phi : nullptr, 1.0, nullptr, mu, 0.5,
           -ph.Omega, Gr[c], true);
     HIPCHK(hipGetLastError());
-    copy_red(p, R_HLAST, 3);
     plan_sync(p);
     double resid_rel = sqrt(p.hred[R_HLAST] * dv) / std::max(1.0, fabs(mu));
     double best_resid = resid_rel;
@@ -132,16 +129,14 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
       const bool run_it = it < maxit;
       if (run_it) {
         // sigma, z = _preconditioner(grad, psi, phi)                 (solvers.py:129)
+        flush_jobs(p);                               // rho max of the residual pass
         launch_sigma(p, P[c], lr ? phi : nullptr, 1.0, p.red + R_HLAST + 2, ph.beta2, ph.beta3, floor_, s);
         run_precond(p, Gr[c], Z);
         const double2* pp = have_prev ? P[1 - c] : P[c];
         const double2* gp = have_prev ? Gr[1 - c] : Gr[c];
-        launch_dots(Z, P[c], D, Gr[c], gp, pp, n, p.partials, p.counters + C_DOTS, p.red + R_DOTS, s);
+        add_job(p, C_DOTS, launch_dots(Z, P[c], D, Gr[c], gp, pp, n, part_of(p, C_DOTS), s), 16, 0, R_DOTS);
         HIPCHK(hipGetLastError());
-        copy_red(p, R_DOTS, 16);
-        copy_red(p, R_SIGMA, 1);
       }
-      if (pending) copy_red(p, R_HLAST, 3);
       plan_sync(p);
       if (pending) {
         // residual bookkeeping of the step accepted at iteration it_acc (solvers.py:196-221)
@@ -240,7 +235,6 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
         run_energy_sums(p, P[c], D, trial, lr ? p.rho : nullptr);
         if (lr) run_conv(p, 0, p.rho, -ph.G_C, phit, p.rho);
         HIPCHK(hipGetLastError());
-        copy_red(p, 0, R_EGRAV + 1);
         plan_sync(p);
         ++n_energy;
         et = energy_from_sums(p, ph, lr, true);
