@@ -38,7 +38,8 @@ struct PassArgs {
   const double* khat; int H, M;        // real-even kernel spectrum, half table [H]^d
   const double* sigma; int Nk; double invNdx;   // Sobolev shift and grid frequencies
   const double* dot;                   // optional: accumulate sum(out_re * dot)
-  double* partials;                    // per-block partial rows (reducing launches)
+  double* partials;                    // per-block partial rows, device (nullable)
+  double* hpartials;                   // the same rows in pinned host memory (nullable)
 };
 
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
@@ -54,7 +55,8 @@ struct KinArgs {
   double beta2, beta3, tc, s_half; int s_mode;           // local terms (KE_FIRST, KH_LAST)
   const double* phi; double phisc; double* phi_out;
   double mu;
-  double* partials;                    // per-block partial rows (reducing launches)
+  double* partials;                    // per-block partial rows, device (nullable)
+  double* hpartials;                   // the same rows in pinned host memory (nullable)
 };
 
 struct SigmaStats {
@@ -62,14 +64,11 @@ struct SigmaStats {
   unsigned int cnt, bucket, rank_in, ncand;
 };
 
-// a pending two-stage reduction: per-block rows -> out / hout
+// a pending reduction: per-block rows in pinned host memory -> hred[slot ..]
 struct RedJob {
-  const double* part;
-  int nblk, K, KM;
-  double* out;
-  double* hout;
+  int site, nblk, K, KM, slot;
 };
-#define BEC_MAX_JOBS 4
+#define BEC_MAX_JOBS 16
 struct RedJobs {
   RedJob j[BEC_MAX_JOBS];
   int n;
@@ -94,7 +93,9 @@ struct Plan {
   double2* s1; double2* s2;          // conv spectra
   double* rho;                       // n-point real scratch
   double* partials; long long part_stride; int max_blocks;   // one region per PartSite
-  RedJobs pend;                      // reductions awaiting k_finalize
+  double* hpart; double* hpart_dev;  // pinned host copy of the regions (host view / kernel view)
+  RedJobs pend;                      // reductions whose rows the host folds at the next sync
+  int hlast_blocks;                  // rows of the last H pass (rho max for the shift mask)
   double* red;                       // device reduction outputs (256 doubles)
   double* hred;                      // pinned, device-mapped host mirror (host view)
   double* hred_dev;                  // the same memory as seen by kernels
@@ -129,19 +130,19 @@ enum PartSite { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DO
 LineCfg line_cfg(int L);
 int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
 int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
-int launch_norm2(const double2* psi, long long n, double* partials, hipStream_t s);
+int launch_norm2(const double2* psi, long long n, double* hpart, hipStream_t s);
 void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s);
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, double az,
                     double ad, double ag, double gre, double gim, long long n, hipStream_t s);
 int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
-                const double2* gp, const double2* pp, long long n, double* partials, hipStream_t s);
+                const double2* gp, const double2* pp, long long n, double* hpart, hipStream_t s);
 int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
-                double* partials, hipStream_t s);
+                double* hpart, hipStream_t s);
 void launch_rho(const double2* psi, double* rho, long long n, hipStream_t s);
-int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, hipStream_t s);
-void launch_finalize(const RedJobs& jobs, hipStream_t s);
-void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc,
-                  const double* rhomax, double beta2, double beta3, double floor_, hipStream_t s);
+int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
+                   hipStream_t s);
+void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc, const double* mrows,
+                  int nrows, int mkp, int mq, double beta2, double beta3, double floor_, hipStream_t s);
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s);
 void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat, hipStream_t s);
 void launch_init_sigma(const Plan& p, hipStream_t s);
@@ -150,6 +151,7 @@ void launch_init_sigma(const Plan& p, hipStream_t s);
 //   kinetic passes 100 + 10*axis + kind;  line-FFT passes
 //   2000 + 1000*contig + 100*load + 10*dir + mul;  elementwise kernels < 100
 void prof_pre(int id, hipStream_t s);
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop);   // events for hipExtLaunchKernelGGL
 void prof_post(int id, hipStream_t s);
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,
                 SITE_SIGMA_HIST = 21, SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };

@@ -7,6 +7,7 @@
 // transforms back, and writes the result through a fused epilogue.  The
 // workloads are HBM/LDS bound FP64 vector work: there is no MFMA on this path.
 #include "bec_internal.h"
+#include <hip/hip_ext.h>
 
 #define TWO_PI 6.283185307179586
 
@@ -39,14 +40,16 @@ __device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, d
 // ---------------------------------------------------------------------------
 // Two-stage deterministic reductions.  Each workgroup folds its threads' K
 // sums and KM maxima in a fixed tree order and stores one row of KP values
-// (KP = K+KM rounded up to a power of two) with plain stores; k_finalize, the
-// next launch on the stream, folds the rows in block order.  The kernel
-// boundary makes the rows visible across the XCDs' separate L2 caches, so no
-// workgroup issues a device-wide fence (on a multi-XCD device an agent-scope
-// fence writes back and invalidates the whole L2).
+// (KP = K+KM rounded up to a power of two) with plain stores; the
+// rows are also written to pinned host memory, and the host folds them in
+// block order after the stream synchronisation it performs anyway (no fold
+// launch); device consumers fold the device rows themselves in a later launch.
+// No workgroup issues a device-wide fence (on a multi-XCD device an
+// agent-scope fence writes back and invalidates the whole L2).
 // ---------------------------------------------------------------------------
 template <int K, int KM>
-__device__ __forceinline__ void block_partials(const double* v, const double* m, double* partials) {
+__device__ __forceinline__ void block_partials(const double* v, const double* m, double* partials,
+                                               double* hpart) {
   constexpr int KT = K + KM;
   constexpr int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
   static_assert(KT <= 16, "at most 16 reduced values");
@@ -67,40 +70,8 @@ __device__ __forceinline__ void block_partials(const double* v, const double* m,
     const int q = threadIdx.x;
     double s = red[q * 16];
     for (int w = 1; w < nw; ++w) s = (q < K) ? s + red[q * 16 + w] : fmax(s, red[q * 16 + w]);
-    partials[(size_t)blockIdx.x * KP + q] = s;
-  }
-}
-
-// Folds up to BEC_MAX_JOBS pending reductions (one workgroup of BEC_WG
-// threads): thread t owns value q = t % KP of blocks t/KP, t/KP + WG/KP, ...;
-// the per-thread results are then combined in thread order.
-__global__ __launch_bounds__(BEC_WG) void k_finalize(RedJobs jobs) {
-  __shared__ double red[BEC_WG];
-  for (int jb = 0; jb < jobs.n; ++jb) {
-    const RedJob J = jobs.j[jb];
-    const int KT = J.K + J.KM;
-    const int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
-    const int q = threadIdx.x % KP, stride = blockDim.x / KP;
-    const bool isum = q < J.K;
-    double s = isum ? 0.0 : -INFINITY;
-    if (q < KT) {
-      for (int b = threadIdx.x / KP; b < J.nblk; b += stride) {
-        const double x = J.part[(size_t)b * KP + q];
-        s = isum ? s + x : fmax(s, x);
-      }
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < KT) {
-      double t = red[threadIdx.x];
-      for (int j = 1; j < stride; ++j) {
-        const double x = red[threadIdx.x + j * KP];
-        t = isum ? t + x : fmax(t, x);
-      }
-      J.out[threadIdx.x] = t;
-      if (J.hout) J.hout[threadIdx.x] = t;
-    }
-    __syncthreads();
+    if (partials) partials[(size_t)blockIdx.x * KP + q] = s;
+    if (hpart) hpart[(size_t)blockIdx.x * KP + q] = s;
   }
 }
 
@@ -214,10 +185,10 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     lz *= lzc * invN;
     if (KIND == KE_FIRST) {
       double v6[6] = {kin, lz, loc[0], loc[1], loc[2], loc[3]};
-      block_partials<6, 0>(v6, nullptr, a.partials);
+      block_partials<6, 0>(v6, nullptr, a.partials, a.hpartials);
     } else {
       double v2[2] = {kin, lz};
-      block_partials<2, 0>(v2, nullptr, a.partials);
+      block_partials<2, 0>(v2, nullptr, a.partials, a.hpartials);
     }
     return;
   }
@@ -271,7 +242,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       hmax[0] = fmax(hmax[0], rho);
     }
   }
-  if (KIND == KH_LAST && a.partials) block_partials<2, 1>(hsum, hmax, a.partials);
+  if (KIND == KH_LAST && (a.partials || a.hpartials)) block_partials<2, 1>(hsum, hmax, a.partials, a.hpartials);
 }
 
 // ---------------------------------------------------------------------------
@@ -361,19 +332,19 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       if (a.dot) dsum[0] += r * a.dot[adr];
     }
   }
-  if (ST == ST_RE && a.dot) block_partials<1, 0>(dsum, nullptr, a.partials);
+  if (ST == ST_RE && a.dot) block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials);
 }
 
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BEC_WG) void k_norm2(const double2* __restrict__ psi, long long n,
-                                                  double* partials) {
+                                                  double* hpart) {
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     v[0] += cabs2(psi[i]);
-  block_partials<1, 0>(v, nullptr, partials);
+  block_partials<1, 0>(v, nullptr, nullptr, hpart);
 }
 
 __global__ __launch_bounds__(BEC_WG) void k_scale(const double2* __restrict__ in, double2* out,
@@ -418,7 +389,7 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
                                                  const double2* __restrict__ g,
                                                  const double2* __restrict__ gp,
                                                  const double2* __restrict__ pp, long long n,
-                                                 double* partials) {
+                                                 double* hpart) {
   double v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = 0.0;
@@ -439,13 +410,13 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
     v[14] += sv.x * y.x + sv.y * y.y;
     v[15] += cabs2(sv);
   }
-  block_partials<16, 0>(v, nullptr, partials);
+  block_partials<16, 0>(v, nullptr, nullptr, hpart);
 }
 
 // g = h - mu psi, and sum |g|^2
 __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
                                                  const double2* __restrict__ psi, double mu,
-                                                 double2* g, long long n, double* partials) {
+                                                 double2* g, long long n, double* hpart) {
   double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -454,12 +425,12 @@ __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
     if (g) g[i] = r;
     v[0] += cabs2(r);
   }
-  block_partials<1, 0>(v, nullptr, partials);
+  block_partials<1, 0>(v, nullptr, nullptr, hpart);
 }
 
 // moments: mass, sum R2 rho, sum (X^2+Y^2) rho, sum Z^2 rho; max rho
 __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ psi, int ndim, int N,
-                                                    double dx, double* partials) {
+                                                    double dx, double* partials, double* hpart) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   double m[1] = {0.0};
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
@@ -479,7 +450,7 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
     v[3] += zz * zz * rho;
     m[0] = fmax(m[0], rho);
   }
-  block_partials<4, 1>(v, m, partials);
+  block_partials<4, 1>(v, m, partials, hpart);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,12 +459,21 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restrict__ psi,
                                                         const double* __restrict__ phi, double phisc,
-                                                        const double* rhomax, double b2, double b3,
-                                                        int ndim, int N, double dx, double tc,
-                                                        int s_mode, double s_half, long long n,
+                                                        const double* mrows, int nrows, int mkp, int mq,
+                                                        double b2, double b3, int ndim, int N, double dx,
+                                                        double tc, int s_mode, double s_half, long long n,
                                                         double* loc, unsigned long long* part) {
   __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64], scnt[BEC_WG / 64];
-  const double thr = 0.05 * (*rhomax);
+  __shared__ double smx[BEC_WG / 64];
+  // max rho of the state: fold the per-block maxima of the pass that computed it
+  double rm = 0.0;
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) rm = fmax(rm, mrows[(size_t)r * mkp + mq]);
+  rm = wave_max(rm);
+  if ((threadIdx.x & 63) == 0) smx[threadIdx.x >> 6] = rm;
+  __syncthreads();
+  rm = smx[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) rm = fmax(rm, smx[w]);
+  const double thr = 0.05 * rm;
   unsigned long long kmn = ~0ull, kmx = 0ull, cnt = 0ull;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -831,9 +811,12 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   const int blocks = xcd_grid(a.nlines, a.lpw);
   const size_t shm = (size_t)a.lpw * a.fp.L * sizeof(double2);
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL;
-  prof_pre(id, s);
-  hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
-  prof_post(id, s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (prof_events(id, &e0, &e1))
+    hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1,
+                          0, a);
+  else
+    hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
   return blocks;
 }
 
@@ -867,9 +850,11 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   const int blocks = xcd_grid(a.nlines, a.lpw);
   const size_t shm = (size_t)a.lpw * a.N * sizeof(double2);
   const int id = 100 + 10 * AX + KIND;
-  prof_pre(id, s);
-  hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
-  prof_post(id, s);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (prof_events(id, &e0, &e1))
+    hipExtLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1, 0, a);
+  else
+    hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
   return blocks;
 }
 
@@ -894,9 +879,9 @@ int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
   return 0;
 }
 
-int launch_norm2(const double2* psi, long long n, double* partials, hipStream_t s) {
+int launch_norm2(const double2* psi, long long n, double* hpart, hipStream_t s) {
   const int g = grid_for(n);
-  hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BEC_WG), 0, s, psi, n, partials);
+  hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BEC_WG), 0, s, psi, n, hpart);
   return g;
 }
 void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s) {
@@ -913,37 +898,35 @@ void launch_combine(double2* d, const double2* z, const double2* g, const double
   prof_post(SITE_COMBINE, s);
 }
 int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
-                const double2* gp, const double2* pp, long long n, double* partials, hipStream_t s) {
+                const double2* gp, const double2* pp, long long n, double* hpart, hipStream_t s) {
   const int gr = grid_for(n);
   prof_pre(SITE_DOTS, s);
-  hipLaunchKernelGGL(k_dots, dim3(gr), dim3(BEC_WG), 0, s, z, psi, d, g, gp, pp, n, partials);
+  hipLaunchKernelGGL(k_dots, dim3(gr), dim3(BEC_WG), 0, s, z, psi, d, g, gp, pp, n, hpart);
   prof_post(SITE_DOTS, s);
   return gr;
 }
 int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
-                double* partials, hipStream_t s) {
+                double* hpart, hipStream_t s) {
   const int gr = grid_for(n);
-  hipLaunchKernelGGL(k_res2, dim3(gr), dim3(BEC_WG), 0, s, h, psi, mu, g_out, n, partials);
+  hipLaunchKernelGGL(k_res2, dim3(gr), dim3(BEC_WG), 0, s, h, psi, mu, g_out, n, hpart);
   return gr;
 }
-int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, hipStream_t s) {
+int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
+                   hipStream_t s) {
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
   const int gr = grid_for(n);
-  hipLaunchKernelGGL(k_moments, dim3(gr), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials);
+  hipLaunchKernelGGL(k_moments, dim3(gr), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials, hpart);
   return gr;
 }
-void launch_finalize(const RedJobs& jobs, hipStream_t s) {
-  if (jobs.n > 0) hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BEC_WG), 0, s, jobs);
-}
-void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc,
-                  const double* rhomax, double beta2, double beta3, double floor_, hipStream_t s) {
+void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc, const double* mrows,
+                  int nrows, int mkp, int mq, double beta2, double beta3, double floor_, hipStream_t s) {
   long long g1 = (p.n + 4 * BEC_WG - 1) / (4 * BEC_WG);
   if (g1 > SIGMA_MAX_BLOCKS) g1 = SIGMA_MAX_BLOCKS;
   const int g = grid_for(p.n);
   unsigned long long* part = (unsigned long long*)p.sig_part;
   prof_pre(SITE_SIGMA_LOCAL, s);
-  hipLaunchKernelGGL(k_sigma_local, dim3((int)g1), dim3(BEC_WG), 0, s, psi, phi, phisc, rhomax, beta2,
-                     beta3, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, p.loc, part);
+  hipLaunchKernelGGL(k_sigma_local, dim3((int)g1), dim3(BEC_WG), 0, s, psi, phi, phisc, mrows, nrows, mkp,
+                     mq, beta2, beta3, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, p.loc, part);
   prof_post(SITE_SIGMA_LOCAL, s);
   prof_pre(SITE_SIGMA_HIST, s);
   hipLaunchKernelGGL(k_sigma_hist, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,

@@ -26,10 +26,10 @@ struct Energy {
 };
 
 bool plan_long_range(const Plan& p, const Phys& ph);
-void plan_sync(Plan& p);                       // flush pending reductions, then synchronise
-double* part_of(Plan& p, int site);            // partial-row region of a reducing launch site
-void add_job(Plan& p, int site, int nblk, int K, int KM, int slot);   // queue a k_finalize job
-void flush_jobs(Plan& p);                      // launch k_finalize for the queued jobs
+void plan_sync(Plan& p);                       // synchronise, then fold the queued reductions
+double* part_of(Plan& p, int site);            // device partial-row region of a reducing launch site
+double* hpart_of(Plan& p, int site);           // pinned host region, as seen by kernels
+void add_job(Plan& p, int site, int nblk, int K, int KM, int slot);   // queue a host fold
 
 // out = scale * dV * (K * rho)_crop, optionally sum(out * dot) -> red[R_EGRAV]
 void run_conv(Plan& p, int which, const double* rho, double scale, double* out, const double* dot);

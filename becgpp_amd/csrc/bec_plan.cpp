@@ -80,27 +80,44 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
 bool plan_long_range(const Plan& p, const Phys& ph) { return p.kkind[0] != 0 && fabs(ph.G_C) > 1e-15; }
 
 double* part_of(Plan& p, int site) { return p.partials + (size_t)site * p.part_stride; }
+double* hpart_of(Plan& p, int site) { return p.hpart_dev + (size_t)site * p.part_stride; }
 
-void add_job(Plan& p, int site, int nblk, int K, int KM, int slot) {
-  if (p.pend.n == BEC_MAX_JOBS) flush_jobs(p);
-  RedJob& j = p.pend.j[p.pend.n++];
-  j.part = part_of(p, site);
-  j.nblk = nblk;
-  j.K = K;
-  j.KM = KM;
-  j.out = p.red + slot;
-  j.hout = p.hred_dev + slot;
-}
-
-void flush_jobs(Plan& p) {
-  if (p.pend.n == 0) return;
-  launch_finalize(p.pend, p.stream);
+// Fold the pinned per-block rows of every queued reduction into hred, rows
+// in block order (deterministic), sums for the first K values, maxima for
+// the next KM.
+static void fold_jobs(Plan& p) {
+  for (int i = 0; i < p.pend.n; ++i) {
+    const RedJob& j = p.pend.j[i];
+    const int KT = j.K + j.KM;
+    const int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
+    const double* rows = p.hpart + (size_t)j.site * p.part_stride;
+    for (int q = 0; q < KT; ++q) {
+      double s = (q < j.K) ? 0.0 : -INFINITY;
+      for (int b = 0; b < j.nblk; ++b) {
+        const double x = rows[(size_t)b * KP + q];
+        s = (q < j.K) ? s + x : std::max(s, x);
+      }
+      p.hred[j.slot + q] = s;
+    }
+  }
   p.pend.n = 0;
 }
 
+void add_job(Plan& p, int site, int nblk, int K, int KM, int slot) {
+  for (int i = 0; i < p.pend.n; ++i)
+    if (p.pend.j[i].site == site) throw BecError(2, "internal: reduction site reused before a sync");
+  if (p.pend.n == BEC_MAX_JOBS) throw BecError(2, "internal: too many pending reductions");
+  RedJob& j = p.pend.j[p.pend.n++];
+  j.site = site;
+  j.nblk = nblk;
+  j.K = K;
+  j.KM = KM;
+  j.slot = slot;
+}
+
 void plan_sync(Plan& p) {
-  flush_jobs(p);
   HIPCHK(hipStreamSynchronize(p.stream));
+  fold_jobs(p);
 }
 
 static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
@@ -116,7 +133,8 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
   a.Nk = p.N;
   a.invNdx = p.invNdx;
   a.sigma = p.sigma;
-  a.partials = p.partials + (size_t)C_EGRAV * p.part_stride;
+  a.partials = nullptr;
+  a.hpartials = p.hpart_dev + (size_t)C_EGRAV * p.part_stride;
   return a;
 }
 
@@ -204,9 +222,9 @@ void run_energy_sums(Plan& p, const double2* psi, const double2* d, double t, do
     a.t = t;
     if (k == 0) {
       a.rho_out = rho_out;
-      a.partials = part_of(p, C_EFIRST);
+      a.hpartials = hpart_of(p, C_EFIRST);
     } else {
-      a.partials = part_of(p, k == 1 ? C_EREST0 : C_EREST1);
+      a.hpartials = hpart_of(p, k == 1 ? C_EREST0 : C_EREST1);
     }
     const int nb = launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
     if (k == 0) add_job(p, C_EFIRST, nb, 6, 0, R_EFIRST);
@@ -274,10 +292,14 @@ void run_H(Plan& p, const double2* psi_in, const double2* d, double t, double di
       a.phi_out = phi_out;
       a.mu = mu;
       a.partials = reduce ? part_of(p, C_HLAST) : nullptr;
+      a.hpartials = reduce ? hpart_of(p, C_HLAST) : nullptr;
     }
     const int kind = (k == 0) ? KH_FIRST : (last ? KH_LAST : KH_MID);
     const int nb = launch_kin(ax[k], kind, a, p.stream);
-    if (last && reduce) add_job(p, C_HLAST, nb, 2, 1, R_HLAST);
+    if (last && reduce) {
+      add_job(p, C_HLAST, nb, 2, 1, R_HLAST);
+      p.hlast_blocks = nb;
+    }
   }
 }
 
@@ -328,6 +350,7 @@ static void plan_free(Plan& p) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p.hred) (void)hipHostFree(p.hred);
+  if (p.hpart) (void)hipHostFree(p.hpart);
 }
 
 extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff,
@@ -381,6 +404,9 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     p.part_stride = 16LL * p.max_blocks;
     HIPCHK(hipMalloc(&p.partials, sizeof(double) * p.part_stride * C_COUNT));
     HIPCHK(hipMalloc(&p.sig_part, sizeof(unsigned long long) * 4 * SIGMA_MAX_BLOCKS));
+    HIPCHK(hipHostMalloc(&p.hpart, sizeof(double) * p.part_stride * C_COUNT,
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&p.hpart_dev, p.hpart, 0));
     HIPCHK(hipMalloc(&p.red, sizeof(double) * 256));
     HIPCHK(hipMemsetAsync(p.red, 0, sizeof(double) * 256, p.stream));
     HIPCHK(hipHostMalloc(&p.hred, sizeof(double) * 256, hipHostMallocMapped | hipHostMallocCoherent));
@@ -539,7 +565,7 @@ extern "C" int bec_norm2(bec_plan* h, const void* psi, double* sum_out) {
   BEC_TRY({
     if (!h || !psi || !sum_out) bad("null argument");
     Plan& p = h->p;
-    add_job(p, C_NORM, launch_norm2((const double2*)psi, p.n, part_of(p, C_NORM), p.stream), 1, 0, R_NORM);
+    add_job(p, C_NORM, launch_norm2((const double2*)psi, p.n, hpart_of(p, C_NORM), p.stream), 1, 0, R_NORM);
     HIPCHK(hipGetLastError());
     plan_sync(p);
     *sum_out = p.hred[R_NORM];
@@ -629,7 +655,7 @@ extern "C" int bec_residual(bec_plan* h, const void* psi, const bec_phys* php, d
           -ph.Omega, hbuf, true);
     plan_sync(p);
     const double mu = p.hred[R_HLAST + 1] * p.dv;
-    add_job(p, C_RES2, launch_res2(hbuf, (const double2*)psi, mu, nullptr, p.n, part_of(p, C_RES2), p.stream),
+    add_job(p, C_RES2, launch_res2(hbuf, (const double2*)psi, mu, nullptr, p.n, hpart_of(p, C_RES2), p.stream),
             1, 0, R_RES2);
     HIPCHK(hipGetLastError());
     plan_sync(p);
@@ -656,16 +682,16 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
         phiu = (const double*)p.w2;
       }
     }
-    // max rho via the moments reduction
-    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM), p.stream), 4, 1,
-            R_MOM);
-    flush_jobs(p);
-    launch_sigma(p, (const double2*)psi, phiu, 1.0, p.red + R_MOM + 4, ph.beta2, ph.beta3, shift_min,
+    // max rho: the sigma pass folds the per-block maxima of the moments pass
+    const int nbm = launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM),
+                                   hpart_of(p, C_MOM), p.stream);
+    add_job(p, C_MOM, nbm, 4, 1, R_MOM);
+    launch_sigma(p, (const double2*)psi, phiu, 1.0, part_of(p, C_MOM), nbm, 8, 4, ph.beta2, ph.beta3, shift_min,
                  p.stream);
     double2* z = (double2*)out;
     run_precond(p, (const double2*)g, z);
     // projection onto the tangent space: z - psi <psi, z> dv
-    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, (const double2*)psi, p.n, part_of(p, C_DOTS),
+    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, (const double2*)psi, p.n, hpart_of(p, C_DOTS),
                                    p.stream), 16, 0, R_DOTS);
     plan_sync(p);
     const double cre = p.hred[R_DOTS + 0] * p.dv, cim = p.hred[R_DOTS + 1] * p.dv;
@@ -680,9 +706,8 @@ extern "C" int bec_moments(bec_plan* h, const void* psi, double* out5) {
   BEC_TRY({
     if (!h || !psi || !out5) bad("null argument");
     Plan& p = h->p;
-    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM), p.stream), 4, 1,
-            R_MOM);
-    flush_jobs(p);
+    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream),
+            4, 1, R_MOM);
     HIPCHK(hipGetLastError());
     plan_sync(p);
     memcpy(out5, p.hred + R_MOM, sizeof(double) * 5);

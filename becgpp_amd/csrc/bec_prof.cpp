@@ -44,6 +44,23 @@ void prof_pre(int id, hipStream_t s) {
   (void)hipEventRecord(g_pool[g_used].a, s);
 }
 
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop) {
+  if (g_watch == 0 || id != g_watch) return false;
+  if (g_used == g_pool.size()) {
+    if (g_pool.size() < 4096) {
+      EvPair e;
+      if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return false;
+      g_pool.push_back(e);
+    } else {
+      collect();
+    }
+  }
+  *start = g_pool[g_used].a;
+  *stop = g_pool[g_used].b;
+  ++g_used;
+  return true;
+}
+
 void prof_post(int id, hipStream_t s) {
   if (g_watch == 0 || id != g_watch) return;
   (void)hipEventRecord(g_pool[g_used].b, s);

@@ -82,7 +82,7 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
     int n_tr = 0;
 
     // psi = psi0 / norm_of(psi0)                                   (solvers.py:56-58)
-    add_job(p, C_NORM, launch_norm2((const double2*)psi_io, n, part_of(p, C_NORM), s), 1, 0, R_NORM);
+    add_job(p, C_NORM, launch_norm2((const double2*)psi_io, n, hpart_of(p, C_NORM), s), 1, 0, R_NORM);
     plan_sync(p);
     const double nrm0 = sqrt(p.hred[R_NORM] * dv);
     int c = 0;
@@ -129,12 +129,13 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
       const bool run_it = it < maxit;
       if (run_it) {
         // sigma, z = _preconditioner(grad, psi, phi)                 (solvers.py:129)
-        flush_jobs(p);                               // rho max of the residual pass
-        launch_sigma(p, P[c], lr ? phi : nullptr, 1.0, p.red + R_HLAST + 2, ph.beta2, ph.beta3, floor_, s);
+        // the mask's max rho comes from the per-block maxima of the last H pass
+        launch_sigma(p, P[c], lr ? phi : nullptr, 1.0, part_of(p, C_HLAST), p.hlast_blocks, 4, 2, ph.beta2,
+                     ph.beta3, floor_, s);
         run_precond(p, Gr[c], Z);
         const double2* pp = have_prev ? P[1 - c] : P[c];
         const double2* gp = have_prev ? Gr[1 - c] : Gr[c];
-        add_job(p, C_DOTS, launch_dots(Z, P[c], D, Gr[c], gp, pp, n, part_of(p, C_DOTS), s), 16, 0, R_DOTS);
+        add_job(p, C_DOTS, launch_dots(Z, P[c], D, Gr[c], gp, pp, n, hpart_of(p, C_DOTS), s), 16, 0, R_DOTS);
         HIPCHK(hipGetLastError());
       }
       plan_sync(p);

@@ -54,9 +54,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rotating_lattice_2d_n384", choices=sorted(WORKLOADS))
-    ap.add_argument("--watch", type=int, default=2023, help="launch-site id of the roofline kernel")
+    ap.add_argument("--watch", type=int, default=114, help="launch-site id of the roofline kernel")
     ap.add_argument("--cpu-sample-iters", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--maxit", type=int, default=None,
+                    help="cap the CG iterations per solve (bounded profiling runs only)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
 
 
@@ -84,6 +88,14 @@ def main():
 
     wl = WORKLOADS[a.workload]
     cfg = B.default_cfg(**wl["cfg"])
+    if a.maxit is not None:
+        cfg["maxit"] = int(a.maxit)
+    if a.calibrate_copy:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     G = B.make_grid(cfg)
     n = int(G["N"]) ** int(G["ndim"])
     nseeds = int(cfg["nseeds"])
