@@ -12,6 +12,7 @@ from .interactions import long_range_phi, kernel_convolution, q2d_kernel, clear_
 from .operators import energy_components, energy_t, apply_H, compute_residual, observables
 from .thomasfermi import extract_tf, mass_quantiles
 from .seeds import make_seed
-from .solvers import ground_state, ground_state_multiseed
+from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, solve_native,
+                      solve_native_batch)
 
 __version__ = "0.1.0"

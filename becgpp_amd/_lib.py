@@ -66,6 +66,9 @@ SIGNATURES = {
     "bec_ground_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Phys), ctypes.POINTER(SolverCfg),
                                         ctypes.c_void_p, ctypes.POINTER(SolveOut), c_double_p,
                                         ctypes.c_int]),
+    "bec_ground_state_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Phys),
+                                              ctypes.POINTER(SolverCfg), ctypes.c_void_p,
+                                              ctypes.POINTER(SolveOut)]),
     "bec_prof_watch": (ctypes.c_int, [ctypes.c_int]),
     "bec_prof_read": (ctypes.c_int, [c_double_p, ctypes.POINTER(ctypes.c_longlong)]),
 }

@@ -1,9 +1,16 @@
 // Host-side plan structure and kernel-launch wrappers of the becGPP library.
+//
+// Every kernel works on a batch of nb instances of one grid geometry: fields
+// are stored instance after instance ([b][n], n = N^ndim points), lines are
+// numbered across instances (line l belongs to instance l / nl), and the
+// per-instance scalars of a launch travel by value in its arguments.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
 #include "bec_common.h"
+
+#define BEC_BMAX 32          // instances per launch (per-instance scalars by value)
 
 // line-pass axis modes (which axis a pass transforms, and how lines map to
 // memory).  2D: axis 1 = y (contiguous), axis 0 = x (stride N).
@@ -23,21 +30,28 @@ struct LineCfg {       // threads per line / lines per workgroup for a length
   int tpl, lpw;
 };
 
+// per-instance physics table entry (device memory, constant during a solve)
+struct InstPhys {
+  double beta2, beta3, Omega, G_C;
+};
+
 // Arguments of the generic line-FFT pass (pad-on-load / crop-on-store).
 struct PassArgs {
   FftPlan fp;
   int tpl, lpw;
-  int nlines, ni;                      // line l = o*ni + i
-  long long in_so, in_si, in_s;        // input: base = o*in_so + i*in_si, element stride in_s
+  int nb, nl;                          // instances, lines per instance
+  unsigned long long act;              // active instances (bit b)
+  int ni;                              // line l' = o*ni + i within an instance
+  long long in_so, in_si, in_s, in_ib; // input: b*in_ib + o*in_so + i*in_si, element stride in_s
   int n_load, off_load;                // loaded window [off_load, off_load+n_load) of the line
-  long long out_so, out_si, out_s;
+  long long out_so, out_si, out_s, out_ib;
   int n_store, off_store;              // stored window
   double out_scale;
   const void* in;
   void* out;
   const double* khat; int H, M;        // real-even kernel spectrum, half table [H]^d
-  const double* sigma; int Nk; double invNdx;   // Sobolev shift and grid frequencies
-  const double* dot;                   // optional: accumulate sum(out_re * dot)
+  const double* sigma; int Nk; double invNdx;   // per-instance Sobolev shifts and grid frequencies
+  const double* dot;                   // optional: rows of sum(out_re * dot) (dot laid out like out)
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
 };
@@ -45,18 +59,28 @@ struct PassArgs {
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
 struct KinArgs {
   FftPlan fp;
-  int tpl, lpw, N, nlines;
+  int tpl, lpw, N, nl, nb;
+  unsigned long long act;
+  long long istride;                                     // points per instance
   double dx, invNdx;
-  const double2* psi; const double2* d; double t, sc;    // loaded value (psi - t d) sc
-  double2* psi_out;                                      // KH_FIRST: materialised loaded value
+  const double2* psi; const double2* d;                  // loaded value (psi - t[b] d) / sc[b]
+  double2* psi_w;                                        // KH_FIRST: loaded value written back
+  double2* sdiff;                                        // KH_FIRST: loaded value - psi
   double* rho_out;                                       // KE_FIRST: |loaded|^2
   const double2* acc; double2* out;
-  double kf, rf;                                         // multiplier kf k^2 + rf lzc k
-  double beta2, beta3, tc, s_half; int s_mode;           // local terms (KE_FIRST, KH_LAST)
-  const double* phi; double phisc; double* phi_out;
-  double mu;
-  double* partials;                    // per-block partial rows, device (nullable)
-  double* hpartials;                   // the same rows in pinned host memory (nullable)
+  double2* ydiff;                                        // KH_LAST: new out - previous out
+  double kf, rf; int rf_omega;                           // multiplier kf k^2 + rf c k (rf = -Omega[b])
+  const InstPhys* iphys; int local_on;                   // local terms in KH_LAST / KE_FIRST
+  double tc, s_half; int s_mode;
+  const double* phi;                                     // long-range potential rows x phisc[b]
+  double* phi_out;
+  double t[BEC_BMAX], sc[BEC_BMAX], mu[BEC_BMAX], phisc[BEC_BMAX];
+  double* partials; double* hpartials;
+};
+
+// per-instance combination d <- az z + ad d + ag g + gam psi
+struct CombArgs {
+  double az[BEC_BMAX], ad[BEC_BMAX], ag[BEC_BMAX], gre[BEC_BMAX], gim[BEC_BMAX];
 };
 
 struct SigmaStats {
@@ -64,9 +88,11 @@ struct SigmaStats {
   unsigned int cnt, bucket, rank_in, ncand;
 };
 
-// a pending reduction: per-block rows in pinned host memory -> hred[slot ..]
+// a pending reduction: per-block rows in pinned host memory -> hred
 struct RedJob {
-  int site, nblk, K, KM, slot;
+  int site, nblk, K, KM, slot;        // nblk rows per instance
+  int nb;                             // instances of the launch
+  unsigned long long act;             // instances whose rows were written
 };
 #define BEC_MAX_JOBS 16
 struct RedJobs {
@@ -80,7 +106,7 @@ struct RedJobs {
 
 struct Plan {
   int ndim, N, pad, M, H;
-  long long n, nM, nS1, nS2;         // field points, padded points, conv spectra sizes
+  long long n, nM, nS1, nS2;         // points per instance field, padded points, conv spectra sizes
   double Lbox, dx, dv, s, tc, s_half; int s_mode;
   double invNdx;
   hipStream_t stream;
@@ -88,36 +114,36 @@ struct Plan {
   double2* twN; double2* twM;
   LineCfg lcN, lcM;
   double* khat[2];                   // [0] = K, [1] = W (virial) real-even spectra
-  int kkind[2];                      // 0 none, 1 newton, 2 log, 3 q2d / external
-  double2* w1; double2* w2;          // n-point complex scratch
-  double2* s1; double2* s2;          // conv spectra
-  double* rho;                       // n-point real scratch
-  double* partials; long long part_stride; int max_blocks;   // one region per PartSite
+  int kkind[2];                      // 0 none, 1 newton, 2 log, 3 table
+  // batch workspace, sized for cap instances
+  int cap;
+  double2* w1;                       // n complex per instance (H accumulator, preconditioner)
+  double2* s1; double2* s2;          // conv spectra per instance
+  double* rho;                       // n real per instance
+  double* loc; double* cand; unsigned* hist; SigmaStats* stats; double* sigma; void* sig_part;
+  InstPhys* iphys;                   // per-instance physics of the current call
+  double* partials; long long part_stride; int max_rows;   // one region per PartSite
   double* hpart; double* hpart_dev;  // pinned host copy of the regions (host view / kernel view)
   RedJobs pend;                      // reductions whose rows the host folds at the next sync
-  int hlast_blocks;                  // rows of the last H pass (rho max for the shift mask)
-  double* red;                       // device reduction outputs (256 doubles)
-  double* hred;                      // pinned, device-mapped host mirror (host view)
-  double* hred_dev;                  // the same memory as seen by kernels
-  // Sobolev shift selection
-  double* loc; double* cand; unsigned* hist; SigmaStats* stats; double* sigma; void* sig_part;
-  // solver state (allocated on first solve)
+  int hlast_rows;                    // rows per instance of the last H pass (rho max)
+  double* hred; double* hred_dev;    // pinned results [instance][RSTRIDE]
+  // solver state, sized for scap instances
+  int scap;
   double2* st[8]; double* stphi[2];
-  bool st_alloc;
 };
 
-// reduction slot layout in Plan::red (device) / Plan::hred (host)
+// reduction slots, RSTRIDE doubles per instance in Plan::hred
+#define RSTRIDE 64
 enum RedSlot {
   R_EFIRST = 0,    // 6: kin, lz, srho, sV, s2, s3
-  R_EREST = 8,     // 2 per remaining axis (up to 2 axes): 8..11
+  R_EREST = 8,     // 2 per remaining axis: 8..11
   R_EGRAV = 12,    // 1: sum(phi_raw * rho)
   R_HLAST = 16,    // 3: |h|^2, Re(psi* h), max rho
   R_NORM = 20,     // 1
-  R_DOTS = 24,     // 17 inner products
-  R_SIGMA = 48,    // 1 sigma value
-  R_MOM = 52,      // 5: mass, R2 rho, rperp2 rho, z2 rho, max rho
-  R_RES2 = 60,     // 1: |h - mu psi|^2
-  R_TMP = 64
+  R_DOTS = 24,     // 16 inner products
+  R_SIGMA = 44,    // 1 sigma value
+  R_MOM = 48,      // 5: mass, R2 rho, rperp2 rho, z2 rho, max rho
+  R_RES2 = 56,     // 1: |h - mu psi|^2
 };
 
 // partial-row regions, one per reducing launch site
@@ -125,33 +151,35 @@ enum PartSite { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DO
                 C_RES2, C_COUNT };
 
 // ---------------------------------------------------------------------------
-// launch wrappers (bec_kernels.hip)
+// launch wrappers (bec_kernels.hip); reducing launches return rows/instance
 // ---------------------------------------------------------------------------
 LineCfg line_cfg(int L);
 int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
 int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
-int launch_norm2(const double2* psi, long long n, double* hpart, hipStream_t s);
-void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s);
-void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, double az,
-                    double ad, double ag, double gre, double gim, long long n, hipStream_t s);
-int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
-                const double2* gp, const double2* pp, long long n, double* hpart, hipStream_t s);
-int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
-                double* hpart, hipStream_t s);
-void launch_rho(const double2* psi, double* rho, long long n, hipStream_t s);
+int elem_blocks(long long n);
+int launch_norm2(const double2* psi, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s);
+void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s);
+void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, const CombArgs& c,
+                    long long n, int nb, unsigned long long act, hipStream_t s);
+int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g, const double2* y,
+                const double2* sv, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s);
+int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n, double* hpart,
+                hipStream_t s);
+void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s);
 int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
                    hipStream_t s);
-void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc, const double* mrows,
-                  int nrows, int mkp, int mq, double beta2, double beta3, double floor_, hipStream_t s);
+void launch_copy(const double2* src, double2* dst, long long n, int nb, unsigned long long mask, hipStream_t s);
+void launch_sigma(const Plan& p, int nb, unsigned long long act, const double2* psi, const double* phi,
+                  const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s);
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s);
 void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat, hipStream_t s);
-void launch_init_sigma(const Plan& p, hipStream_t s);
+void launch_init_sigma(const Plan& p, int nb, hipStream_t s);
 
 // live timing of one watched launch site (bec_prof.cpp); site ids:
 //   kinetic passes 100 + 10*axis + kind;  line-FFT passes
 //   2000 + 1000*contig + 100*load + 10*dir + mul;  elementwise kernels < 100
 void prof_pre(int id, hipStream_t s);
-bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop);   // events for hipExtLaunchKernelGGL
 void prof_post(int id, hipStream_t s);
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop);   // events for hipExtLaunchKernelGGL
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,
                 SITE_SIGMA_HIST = 21, SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };

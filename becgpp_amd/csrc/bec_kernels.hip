@@ -4,8 +4,9 @@
 // into LDS (coalesced: whole rows for the contiguous axis, tiles of lpw
 // adjacent columns for strided axes), runs a mixed-radix Stockham FFT in LDS
 // with tpl threads per line, applies a per-frequency multiplier, optionally
-// transforms back, and writes the result through a fused epilogue.  The
-// workloads are HBM/LDS bound FP64 vector work: there is no MFMA on this path.
+// transforms back, and writes the result through a fused epilogue.  Every
+// launch covers a batch of instances of one grid (line l of instance l / nl).
+// The work is HBM/LDS-bound FP64 vector arithmetic: there is no MFMA here.
 #include "bec_internal.h"
 #include <hip/hip_ext.h>
 
@@ -26,10 +27,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return (b % BEC_XCDS) * (nb / BEC_XCDS) + b / BEC_XCDS;
 }
 
-// agent-scope load of data written by other workgroups of the same launch
-template <typename T> __device__ __forceinline__ T ald(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+__device__ __forceinline__ bool inst_on(unsigned long long act, int b) { return (act >> b) & 1ull; }
 
 __device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, double s_half) {
   if (s_mode == 0) return 0.0;
@@ -38,18 +36,17 @@ __device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, d
 }
 
 // ---------------------------------------------------------------------------
-// Two-stage deterministic reductions.  Each workgroup folds its threads' K
-// sums and KM maxima in a fixed tree order and stores one row of KP values
-// (KP = K+KM rounded up to a power of two) with plain stores; the
-// rows are also written to pinned host memory, and the host folds them in
-// block order after the stream synchronisation it performs anyway (no fold
-// launch); device consumers fold the device rows themselves in a later launch.
-// No workgroup issues a device-wide fence (on a multi-XCD device an
-// agent-scope fence writes back and invalidates the whole L2).
+// Deterministic reductions.  Each workgroup folds its threads' K sums and KM
+// maxima in a fixed tree order and stores one row of KP values (KP = K+KM
+// rounded up to a power of two) at row index `row`, in device memory and/or
+// pinned host memory.  The host folds the host rows in row order after the
+// stream synchronisation it performs anyway; device consumers fold the device
+// rows in a later launch.  No workgroup issues a device-wide fence (on a
+// multi-XCD device an agent-scope fence writes back and invalidates the L2).
 // ---------------------------------------------------------------------------
 template <int K, int KM>
 __device__ __forceinline__ void block_partials(const double* v, const double* m, double* partials,
-                                               double* hpart) {
+                                               double* hpart, long long row) {
   constexpr int KT = K + KM;
   constexpr int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
   static_assert(KT <= 16, "at most 16 reduced values");
@@ -70,13 +67,13 @@ __device__ __forceinline__ void block_partials(const double* v, const double* m,
     const int q = threadIdx.x;
     double s = red[q * 16];
     for (int w = 1; w < nw; ++w) s = (q < K) ? s + red[q * 16 + w] : fmax(s, red[q * 16 + w]);
-    if (partials) partials[(size_t)blockIdx.x * KP + q] = s;
-    if (hpart) hpart[(size_t)blockIdx.x * KP + q] = s;
+    if (partials) partials[row * KP + q] = s;
+    if (hpart) hpart[row * KP + q] = s;
   }
 }
 
 // ---------------------------------------------------------------------------
-// axis geometry of the kinetic passes
+// axis geometry of the kinetic passes (within one instance)
 // ---------------------------------------------------------------------------
 template <int AX> struct Ax;
 template <> struct Ax<A2Y> {   // rows: line l = ix, element e = iy
@@ -115,47 +112,63 @@ template <> struct Ax<A3X> {   // line l = iy*N + iz, element e = ix
   __device__ static int lidx(int l, int N) { return l / N; }
 };
 
+template <int AX>
+__device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
+  using X = Ax<AX>;
+  int ix, iy, iz;
+  X::coords(li, e, a.N, ix, iy, iz);
+  const double xx = xcoord(ix, a.N, a.dx), yy = xcoord(iy, a.N, a.dx);
+  double r2 = xx * xx + yy * yy;
+  if (X::ndim == 3) { const double zz = xcoord(iz, a.N, a.dx); r2 += zz * zz; }
+  return trap_value(r2, a.tc, a.s_mode, a.s_half);
+}
+
 // Kinetic + rotation line pass.  Along an axis with wavenumber k the operator
 // T - Omega L_z acts as the multiplier kf k^2 + rf c k, where c = +x for the y
 // axis, -y for the x axis and 0 for the z axis (L_z = -i(x d_y - y d_x)).
-//   KE_*: forward transform only; accumulate sum kf-weighted |t_k|^2 / N
-//         (kinetic) and c sum k |t_k|^2 / N (L_z) per line (Parseval), and on
-//         the first pass the local sums rho, V rho, rho^2, rho^3.
+//   KE_*: forward transform only; accumulate sum k^2/2 |t_k|^2 / N (kinetic)
+//         and c sum k |t_k|^2 / N (L_z) per line (Parseval), and on the first
+//         pass the local sums rho, V rho, rho^2, rho^3.
 //   KH_*: forward, multiply, inverse; FIRST writes, MID accumulates, LAST adds
 //         the local potential (V + b2 rho + b3 rho^2 + phi) psi - mu psi.
 template <int AX, int KIND>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 lds[];
   using X = Ax<AX>;
-  const int N = a.N, tpl = a.tpl, lpw = a.lpw;
+  const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
-  const int l0 = xcd_block(blockIdx.x, gridDim.x) * lpw;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int l0 = lb * lpw;
+  const int total = a.nb * nl;
   const long long strd = X::stride(N);
-  const bool active = (l0 + c) < a.nlines;
+  const int mb = (l0 + c) / nl;
+  const bool active = (l0 + c) < total && inst_on(a.act, mb);
+  const bool brow = l0 < total && inst_on(a.act, l0 / nl);   // this block stores a partial row
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
     int cc, e;
     if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
     double2 v = make_double2(0.0, 0.0);
-    if (l < a.nlines) {
-      const long long adr = X::base(l, N) + e * strd;
-      v = a.psi[adr];
+    const int b = l / nl;
+    if (l < total && inst_on(a.act, b)) {
+      const int li = l - b * nl;
+      const long long adr = (long long)b * a.istride + X::base(li, N) + e * strd;
+      const double2 p0 = a.psi[adr];
+      v = p0;
       if (a.d) {
         const double2 dd = a.d[adr];
-        v.x = v.x - a.t * dd.x;
-        v.y = v.y - a.t * dd.y;
+        v.x = v.x - a.t[b] * dd.x;
+        v.y = v.y - a.t[b] * dd.y;
       }
-      if (a.sc != 1.0) { v.x = v.x / a.sc; v.y = v.y / a.sc; }
-      if (KIND == KH_FIRST && a.psi_out) a.psi_out[adr] = v;
+      if (a.sc[b] != 1.0) { v.x = v.x / a.sc[b]; v.y = v.y / a.sc[b]; }
+      if (KIND == KH_FIRST) {
+        if (a.psi_w) a.psi_w[adr] = v;
+        if (a.sdiff) a.sdiff[adr] = csub(v, p0);
+      }
       if (KIND == KE_FIRST) {
         const double rho = cabs2(v);
-        int ix, iy, iz;
-        X::coords(l, e, N, ix, iy, iz);
-        const double xx = xcoord(ix, N, a.dx), yy = xcoord(iy, N, a.dx);
-        double r2 = xx * xx + yy * yy;
-        if (X::ndim == 3) { const double zz = xcoord(iz, N, a.dx); r2 += zz * zz; }
-        const double V = trap_value(r2, a.tc, a.s_mode, a.s_half);
+        const double V = local_V<AX>(li, e, a);
         loc[0] += rho;
         loc[1] += V * rho;
         loc[2] += rho * rho;
@@ -169,8 +182,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   double2* s = lds + c * N;
   fft_lds<-1>(s, a.fp, lane, tpl, active);
   const double invN = 1.0 / (double)N;
-  const double lzc = (X::lsign == 0 || !active) ? 0.0
-                   : (double)X::lsign * xcoord(X::lidx(l0 + c, N), N, a.dx);
+  const int li0 = (l0 + c) - mb * nl;
+  const double lzc = (X::lsign == 0 || !active) ? 0.0 : (double)X::lsign * xcoord(X::lidx(li0, N), N, a.dx);
   if (KIND == KE_FIRST || KIND == KE_REST) {
     double kin = 0.0, lz = 0.0;
     if (active) {
@@ -183,19 +196,21 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     }
     kin *= invN;
     lz *= lzc * invN;
+    if (!brow) return;
     if (KIND == KE_FIRST) {
       double v6[6] = {kin, lz, loc[0], loc[1], loc[2], loc[3]};
-      block_partials<6, 0>(v6, nullptr, a.partials, a.hpartials);
+      block_partials<6, 0>(v6, nullptr, a.partials, a.hpartials, lb);
     } else {
       double v2[2] = {kin, lz};
-      block_partials<2, 0>(v2, nullptr, a.partials, a.hpartials);
+      block_partials<2, 0>(v2, nullptr, a.partials, a.hpartials, lb);
     }
     return;
   }
   if (active) {
+    const double rf = a.rf_omega ? -a.iphys[mb].Omega : a.rf;
     for (int k = lane; k < N; k += tpl) {
       const double kv = kfreq(k, N, a.invNdx);
-      const double m = a.kf * kv * kv + a.rf * lzc * kv;
+      const double m = a.kf * kv * kv + rf * lzc * kv;
       s[k] = cscale(s[k], m);
     }
   }
@@ -207,8 +222,10 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     int cc, e;
     if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
-    if (l >= a.nlines) continue;
-    const long long adr = X::base(l, N) + e * strd;
+    const int b = l / nl;
+    if (l >= total || !inst_on(a.act, b)) continue;
+    const int li = l - b * nl;
+    const long long adr = (long long)b * a.istride + X::base(li, N) + e * strd;
     double2 u = lds[cc * N + e];
     u.x *= invN;
     u.y *= invN;
@@ -220,29 +237,30 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     } else {
       const double2 ps = a.psi[adr];
       const double rho = cabs2(ps);
-      int ix, iy, iz;
-      X::coords(l, e, N, ix, iy, iz);
-      const double xx = xcoord(ix, N, a.dx), yy = xcoord(iy, N, a.dx);
-      double r2 = xx * xx + yy * yy;
-      if (X::ndim == 3) { const double zz = xcoord(iz, N, a.dx); r2 += zz * zz; }
-      const double V = trap_value(r2, a.tc, a.s_mode, a.s_half);
-      double ph = 0.0;
-      if (a.phi) {
-        ph = a.phi[adr] * a.phisc;
-        if (a.phi_out) a.phi_out[adr] = ph;
+      double W = 0.0;
+      if (a.local_on) {
+        double ph = 0.0;
+        if (a.phi) {
+          ph = a.phi[adr] * a.phisc[b];
+          if (a.phi_out) a.phi_out[adr] = ph;
+        }
+        const InstPhys ip = a.iphys[b];
+        W = local_V<AX>(li, e, a) + ip.beta2 * rho + ip.beta3 * rho * rho + ph;
       }
-      const double W = V + a.beta2 * rho + a.beta3 * rho * rho + ph;
       const double2 p = a.acc ? a.acc[adr] : make_double2(0.0, 0.0);
+      const double mu = a.mu[b];
       double2 h;
-      h.x = p.x + u.x + W * ps.x - a.mu * ps.x;
-      h.y = p.y + u.y + W * ps.y - a.mu * ps.y;
+      h.x = p.x + u.x + W * ps.x - mu * ps.x;
+      h.y = p.y + u.y + W * ps.y - mu * ps.y;
+      if (a.ydiff) a.ydiff[adr] = csub(h, a.out[adr]);
       a.out[adr] = h;
       hsum[0] += cabs2(h);
       hsum[1] += ps.x * h.x + ps.y * h.y;
       hmax[0] = fmax(hmax[0], rho);
     }
   }
-  if (KIND == KH_LAST && (a.partials || a.hpartials)) block_partials<2, 1>(hsum, hmax, a.partials, a.hpartials);
+  if (KIND == KH_LAST && brow && (a.partials || a.hpartials))
+    block_partials<2, 1>(hsum, hmax, a.partials, a.hpartials, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,28 +269,34 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
 template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   extern __shared__ double2 lds[];
-  const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw;
+  const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
-  const int l0 = xcd_block(blockIdx.x, gridDim.x) * lpw;
-  const bool active = (l0 + c) < a.nlines;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int l0 = lb * lpw;
+  const int total = a.nb * nl;
+  const int mb = (l0 + c) / nl;
+  const bool active = (l0 + c) < total && inst_on(a.act, mb);
+  const bool brow = l0 < total && inst_on(a.act, l0 / nl);   // this block stores a partial row
   for (int idx = threadIdx.x; idx < lpw * L; idx += blockDim.x) {
     int cc, e;
     if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
+    const int b = l / nl;
     double2 v = make_double2(0.0, 0.0);
-    if (l < a.nlines) {
-      const int o = l / a.ni, i = l - o * a.ni;
-      const long long b = o * a.in_so + i * a.in_si;
+    if (l < total && inst_on(a.act, b)) {
+      const int li = l - b * nl;
+      const int o = li / a.ni, i = li - o * a.ni;
+      const long long base = b * a.in_ib + o * a.in_so + i * a.in_si;
       if (LD == LD_HERM) {
         const int half = L / 2;
         const int src = (e <= half) ? e : L - e;
-        const double2 w = ((const double2*)a.in)[b + src * a.in_s];
+        const double2 w = ((const double2*)a.in)[base + src * a.in_s];
         v = (e <= half) ? w : make_double2(w.x, -w.y);
       } else {
         const int r = e - a.off_load;
         if (r >= 0 && r < a.n_load) {
-          if (LD == LD_C) v = ((const double2*)a.in)[b + r * a.in_s];
-          else v = make_double2(((const double*)a.in)[b + r * a.in_s], 0.0);
+          if (LD == LD_C) v = ((const double2*)a.in)[base + r * a.in_s];
+          else v = make_double2(((const double*)a.in)[base + r * a.in_s], 0.0);
         }
       }
     }
@@ -284,13 +308,13 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   else fft_lds<-1>(s, a.fp, lane, tpl, active);
   if (DIR == DIR_FMI) {
     if (active) {
-      const int l = l0 + c;
+      const int li = (l0 + c) - mb * nl;
       if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
         int other;
         if (MUL == MUL_KHAT2) {
-          other = l;                                  // column index ky in [0, H)
+          other = li;                                  // column index ky in [0, H)
         } else {
-          const int ky = l / a.H, kz = l - (l / a.H) * a.H;
+          const int ky = li / a.H, kz = li - (li / a.H) * a.H;
           other = min(ky, a.M - ky) * a.H + kz;
         }
         const long long plane = (MUL == MUL_KHAT2) ? (long long)a.H : (long long)a.H * a.H;
@@ -300,9 +324,9 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         }
       } else {
         // denominator 1/2 (kx^2 + ky^2 [+ kz^2]) + sigma, summed in that order
-        const double sig = *a.sigma;
-        const double ky = kfreq(MUL == MUL_PRE2 ? l : l / a.Nk, a.Nk, a.invNdx);
-        const double kz = (MUL == MUL_PRE3) ? kfreq(l % a.Nk, a.Nk, a.invNdx) : 0.0;
+        const double sig = a.sigma[mb];
+        const double ky = kfreq(MUL == MUL_PRE2 ? li : li / a.Nk, a.Nk, a.invNdx);
+        const double kz = (MUL == MUL_PRE3) ? kfreq(li % a.Nk, a.Nk, a.invNdx) : 0.0;
         for (int k = lane; k < L; k += tpl) {
           const double kx = kfreq(k, a.Nk, a.invNdx);
           double k2 = kx * kx + ky * ky;
@@ -320,9 +344,11 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     int cc, e;
     if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
-    if (l >= a.nlines) continue;
-    const int o = l / a.ni, i = l - o * a.ni;
-    const long long adr = o * a.out_so + i * a.out_si + e * a.out_s;
+    const int b = l / nl;
+    if (l >= total || !inst_on(a.act, b)) continue;
+    const int li = l - b * nl;
+    const int o = li / a.ni, i = li - o * a.ni;
+    const long long adr = b * a.out_ib + o * a.out_so + i * a.out_si + e * a.out_s;
     const double2 v = lds[cc * L + a.off_store + e];
     if (ST == ST_C) {
       ((double2*)a.out)[adr] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
@@ -332,71 +358,93 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       if (a.dot) dsum[0] += r * a.dot[adr];
     }
   }
-  if (ST == ST_RE && a.dot) block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials);
+  if (ST == ST_RE && a.dot && brow) block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials, lb);
 }
 
 // ---------------------------------------------------------------------------
-// elementwise kernels
+// elementwise kernels: grid (chunks, instances); reducing ones store row
+// blockIdx.y * gridDim.x + blockIdx.x
 // ---------------------------------------------------------------------------
+#define ELEM_LOOP(n) \
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+struct DblArr {
+  double v[BEC_BMAX];
+};
+
 __global__ __launch_bounds__(BEC_WG) void k_norm2(const double2* __restrict__ psi, long long n,
-                                                  double* hpart) {
+                                                  unsigned long long act, double* hpart) {
+  const int b = blockIdx.y;
+  if (!inst_on(act, b)) return;
+  const double2* p = psi + (long long)b * n;
   double v[1] = {0.0};
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    v[0] += cabs2(psi[i]);
-  block_partials<1, 0>(v, nullptr, nullptr, hpart);
+  ELEM_LOOP(n) v[0] += cabs2(p[i]);
+  block_partials<1, 0>(v, nullptr, nullptr, hpart, (long long)b * gridDim.x + blockIdx.x);
 }
 
-__global__ __launch_bounds__(BEC_WG) void k_scale(const double2* __restrict__ in, double2* out,
-                                                  double sc, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double2 v = in[i];
-    out[i] = make_double2(v.x / sc, v.y / sc);
+__global__ __launch_bounds__(BEC_WG) void k_scale(const double2* __restrict__ in, double2* out, DblArr div,
+                                                  long long n) {
+  const int b = blockIdx.y;
+  const double sc = div.v[b];
+  ELEM_LOOP(n) {
+    const double2 v = in[(long long)b * n + i];
+    out[(long long)b * n + i] = make_double2(v.x / sc, v.y / sc);
   }
 }
 
 __global__ __launch_bounds__(BEC_WG) void k_rho(const double2* __restrict__ in, double* out, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    out[i] = cabs2(in[i]);
+  const long long off = (long long)blockIdx.y * n;
+  ELEM_LOOP(n) out[off + i] = cabs2(in[off + i]);
 }
 
-// d <- az z + ad d + ag g + gam psi   (gam complex)
+__global__ __launch_bounds__(BEC_WG) void k_copy(const double2* __restrict__ src, double2* dst, long long n,
+                                                 unsigned long long mask) {
+  const int b = blockIdx.y;
+  if (!inst_on(mask, b)) return;
+  const long long off = (long long)b * n;
+  ELEM_LOOP(n) dst[off + i] = src[off + i];
+}
+
+// d <- az z + ad d + ag g + gam psi   (gam complex), per instance
 __global__ __launch_bounds__(BEC_WG) void k_combine(double2* d, const double2* __restrict__ z,
                                                     const double2* __restrict__ g,
-                                                    const double2* __restrict__ psi, double az,
-                                                    double ad, double ag, double gre, double gim,
-                                                    long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double2 p = psi[i];
+                                                    const double2* __restrict__ psi, CombArgs c, long long n,
+                                                    unsigned long long act) {
+  const int b = blockIdx.y;
+  if (!inst_on(act, b)) return;
+  const long long off = (long long)b * n;
+  const double az = c.az[b], ad = c.ad[b], ag = c.ag[b], gre = c.gre[b], gim = c.gim[b];
+  ELEM_LOOP(n) {
+    const long long j = off + i;
+    const double2 p = psi[j];
     double2 r = make_double2(gre * p.x - gim * p.y, gre * p.y + gim * p.x);
-    if (az != 0.0) { const double2 v = z[i]; r.x += az * v.x; r.y += az * v.y; }
-    if (ad != 0.0) { const double2 v = d[i]; r.x += ad * v.x; r.y += ad * v.y; }
-    if (ag != 0.0) { const double2 v = g[i]; r.x += ag * v.x; r.y += ag * v.y; }
-    d[i] = r;
+    if (az != 0.0) { const double2 v = z[j]; r.x += az * v.x; r.y += az * v.y; }
+    if (ad != 0.0) { const double2 v = d[j]; r.x += ad * v.x; r.y += ad * v.y; }
+    if (ag != 0.0) { const double2 v = g[j]; r.x += ag * v.x; r.y += ag * v.y; }
+    d[j] = r;
   }
 }
 
-// Inner products of the CG step (all sums, scaled by dv on the host):
+// Inner products of the CG step (sums, scaled by dv on the host):
 //  0,1 <psi,z>   2 <psi,psi>   3,4 <psi,d>   5,6 <psi,g>   7 Re<z,g>
 //  8 Re<z,y>   9,10 <psi,y>   11 Re<g,d>   12 <g,g>   13 Re<g,y>
-//  14 Re<s,y>  15 <s,s>       (y = g - gp, s = psi - pp, <u,v> = sum conj(u) v)
+//  14 Re<s,y>  15 <s,s>    (y = g - g_prev, s = psi - psi_prev, <u,v> = sum conj(u) v)
 __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
                                                  const double2* __restrict__ psi,
                                                  const double2* __restrict__ d,
                                                  const double2* __restrict__ g,
-                                                 const double2* __restrict__ gp,
-                                                 const double2* __restrict__ pp, long long n,
-                                                 double* hpart) {
+                                                 const double2* __restrict__ yv,
+                                                 const double2* __restrict__ sv, long long n,
+                                                 unsigned long long act, double* hpart) {
+  const int b = blockIdx.y;
+  if (!inst_on(act, b)) return;
+  const long long off = (long long)b * n;
   double v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double2 zi = z[i], pi = psi[i], di = d[i], gi = g[i], gpi = gp[i], ppi = pp[i];
-    const double2 y = csub(gi, gpi), sv = csub(pi, ppi);
+  ELEM_LOOP(n) {
+    const long long j = off + i;
+    const double2 zi = z[j], pi = psi[j], di = d[j], gi = g[j], y = yv[j], s = sv[j];
     v[0] += pi.x * zi.x + pi.y * zi.y;  v[1] += pi.x * zi.y - pi.y * zi.x;
     v[2] += cabs2(pi);
     v[3] += pi.x * di.x + pi.y * di.y;  v[4] += pi.x * di.y - pi.y * di.x;
@@ -407,35 +455,33 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
     v[11] += gi.x * di.x + gi.y * di.y;
     v[12] += cabs2(gi);
     v[13] += gi.x * y.x + gi.y * y.y;
-    v[14] += sv.x * y.x + sv.y * y.y;
-    v[15] += cabs2(sv);
+    v[14] += s.x * y.x + s.y * y.y;
+    v[15] += cabs2(s);
   }
-  block_partials<16, 0>(v, nullptr, nullptr, hpart);
+  block_partials<16, 0>(v, nullptr, nullptr, hpart, (long long)b * gridDim.x + blockIdx.x);
 }
 
-// g = h - mu psi, and sum |g|^2
+// g = h - mu psi, and sum |g|^2 (one instance)
 __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
                                                  const double2* __restrict__ psi, double mu,
                                                  double2* g, long long n, double* hpart) {
   double v[1] = {0.0};
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+  ELEM_LOOP(n) {
     const double2 hi = h[i], p = psi[i];
     const double2 r = make_double2(hi.x - mu * p.x, hi.y - mu * p.y);
     if (g) g[i] = r;
     v[0] += cabs2(r);
   }
-  block_partials<1, 0>(v, nullptr, nullptr, hpart);
+  block_partials<1, 0>(v, nullptr, nullptr, hpart, blockIdx.x);
 }
 
-// moments: mass, sum R2 rho, sum (X^2+Y^2) rho, sum Z^2 rho; max rho
+// moments (one instance): mass, sum R2 rho, sum (X^2+Y^2) rho, sum Z^2 rho; max rho
 __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ psi, int ndim, int N,
                                                     double dx, double* partials, double* hpart) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   double m[1] = {0.0};
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+  ELEM_LOOP(n) {
     const double rho = cabs2(psi[i]);
     int ix, iy, iz = 0;
     if (ndim == 2) { ix = (int)(i / N); iy = (int)(i % N); }
@@ -450,34 +496,41 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
     v[3] += zz * zz * rho;
     m[0] = fmax(m[0], rho);
   }
-  block_partials<4, 1>(v, m, partials, hpart);
+  block_partials<4, 1>(v, m, partials, hpart, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
 // Sobolev shift: lower median of the local potential over the bulk
-// (value-range histogram, candidate gather, exact select in one workgroup)
+// {rho > 0.05 max rho} (value-range histogram, candidate gather, exact select
+// in one workgroup), per instance (grid y).  No atomics across the grid on a
+// single address except the candidate counter.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restrict__ psi,
-                                                        const double* __restrict__ phi, double phisc,
+                                                        const double* __restrict__ phi,
                                                         const double* mrows, int nrows, int mkp, int mq,
-                                                        double b2, double b3, int ndim, int N, double dx,
+                                                        const InstPhys* iphys, int ndim, int N, double dx,
                                                         double tc, int s_mode, double s_half, long long n,
-                                                        double* loc, unsigned long long* part) {
+                                                        unsigned long long act, double* loc,
+                                                        unsigned long long* part) {
   __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64], scnt[BEC_WG / 64];
   __shared__ double smx[BEC_WG / 64];
+  const int b = blockIdx.y;
+  if (!inst_on(act, b)) return;
+  const long long off = (long long)b * n;
   // max rho of the state: fold the per-block maxima of the pass that computed it
   double rm = 0.0;
-  for (int r = threadIdx.x; r < nrows; r += blockDim.x) rm = fmax(rm, mrows[(size_t)r * mkp + mq]);
+  const double* mr = mrows + (long long)b * nrows * mkp;
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) rm = fmax(rm, mr[(long long)r * mkp + mq]);
   rm = wave_max(rm);
   if ((threadIdx.x & 63) == 0) smx[threadIdx.x >> 6] = rm;
   __syncthreads();
   rm = smx[0];
   for (int w = 1; w < (int)(blockDim.x >> 6); ++w) rm = fmax(rm, smx[w]);
   const double thr = 0.05 * rm;
+  const double b2 = iphys[b].beta2, b3 = iphys[b].beta3;
   unsigned long long kmn = ~0ull, kmx = 0ull, cnt = 0ull;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double rho = cabs2(psi[i]);
+  ELEM_LOOP(n) {
+    const double rho = cabs2(psi[off + i]);
     double val = __longlong_as_double(0x7ff8000000000000ll);    // NaN = outside the bulk
     if (rho > thr) {
       int ix, iy, iz = 0;
@@ -487,20 +540,20 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restric
       double r2 = xx * xx + yy * yy;
       if (ndim == 3) { const double zz = xcoord(iz, N, dx); r2 += zz * zz; }
       double lv = trap_value(r2, tc, s_mode, s_half) + 2.0 * b2 * rho + 3.0 * b3 * (rho * rho);
-      if (phi) lv = lv + phi[i] * phisc;
+      if (phi) lv = lv + phi[off + i];
       val = lv;
       const unsigned long long k = order_key(lv);
       kmn = k < kmn ? k : kmn;
       kmx = k > kmx ? k : kmx;
       cnt++;
     }
-    loc[i] = val;
+    loc[off + i] = val;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long a = __shfl_down(kmn, off, 64), c = __shfl_down(kmx, off, 64);
-    kmn = a < kmn ? a : kmn;
-    kmx = c > kmx ? c : kmx;
-    cnt += __shfl_down(cnt, off, 64);
+  for (int o2 = 32; o2 > 0; o2 >>= 1) {
+    const unsigned long long x = __shfl_down(kmn, o2, 64), y = __shfl_down(kmx, o2, 64);
+    kmn = x < kmn ? x : kmn;
+    kmx = y > kmx ? y : kmx;
+    cnt += __shfl_down(cnt, o2, 64);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) { smin[wid] = kmn; smax[wid] = kmx; scnt[wid] = cnt; }
@@ -511,9 +564,10 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restric
       kmx = smax[w] > kmx ? smax[w] : kmx;
       cnt += scnt[w];
     }
-    part[4 * blockIdx.x + 0] = kmn;
-    part[4 * blockIdx.x + 1] = kmx;
-    part[4 * blockIdx.x + 2] = cnt;
+    unsigned long long* pr = part + ((long long)b * gridDim.x + blockIdx.x) * 4;
+    pr[0] = kmn;
+    pr[1] = kmx;
+    pr[2] = cnt;
   }
 }
 
@@ -522,8 +576,8 @@ __device__ __forceinline__ int sigma_bin(double v, double lo, double scale) {
   return b < 0 ? 0 : (b >= SIGMA_BINS ? SIGMA_BINS - 1 : b);
 }
 
-// fold the local pass's per-block (min, max, count) rows; every block does it
-// (at most SIGMA_MAX_BLOCKS rows) and block 0 records the result
+// fold the local pass's per-block (min, max, count) rows of one instance;
+// every block does it (at most SIGMA_MAX_BLOCKS rows), block 0 records it
 __device__ __forceinline__ void sigma_fold(const unsigned long long* part, int nb1, SigmaStats* st,
                                            unsigned long long& kmn, unsigned long long& kmx,
                                            unsigned long long& cnt) {
@@ -556,32 +610,43 @@ __device__ __forceinline__ void sigma_fold(const unsigned long long* part, int n
 
 __global__ __launch_bounds__(BEC_WG) void k_sigma_hist(const double* __restrict__ loc, long long n,
                                                        const unsigned long long* part, int nb1,
-                                                       SigmaStats* st, unsigned* hist) {
+                                                       SigmaStats* stats, unsigned* hists,
+                                                       unsigned long long act) {
   __shared__ unsigned h[SIGMA_BINS];
+  const int b = blockIdx.y;
+  if (!inst_on(act, b)) return;
+  const long long off = (long long)b * n;
+  unsigned* hist = hists + (long long)b * SIGMA_BINS;
   unsigned long long kmn, kmx, cnt;
-  sigma_fold(part, nb1, st, kmn, kmx, cnt);
+  sigma_fold(part + (long long)b * nb1 * 4, nb1, stats + b, kmn, kmx, cnt);
   const double lo = key_to_double(kmn), hi = key_to_double(kmx);
   if (!(cnt > 0 && hi > lo)) return;
-  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) h[b] = 0u;
+  for (int k = threadIdx.x; k < SIGMA_BINS; k += blockDim.x) h[k] = 0u;
   __syncthreads();
   const double scale = (double)SIGMA_BINS / (hi - lo);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double v = loc[i];
+  ELEM_LOOP(n) {
+    const double v = loc[off + i];
     if (v == v) atomicAdd(&h[sigma_bin(v, lo, scale)], 1u);
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x)
-    if (h[b]) atomicAdd(&hist[b], h[b]);
+  for (int k = threadIdx.x; k < SIGMA_BINS; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
 }
 
 // every block locates the bin holding the rank-k element, k = (cnt-1)/2
 // (parallel prefix over the histogram), then gathers that bin's members
 __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restrict__ loc, long long n,
-                                                         SigmaStats* st, const unsigned* hist, double* cand) {
+                                                         SigmaStats* stats, const unsigned* hists, double* cands,
+                                                         unsigned long long act) {
   __shared__ unsigned pre[BEC_WG];
   __shared__ int sbin;
   __shared__ unsigned srank;
+  const int b = blockIdx.y;
+  if (!inst_on(act, b)) return;
+  SigmaStats* st = stats + b;
+  const unsigned* hist = hists + (long long)b * SIGMA_BINS;
+  double* cand = cands + (long long)b * n;
+  const long long off = (long long)b * n;
   const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
   const unsigned cnt = st->cnt;
   if (!(cnt > 0 && hi > lo)) return;
@@ -591,22 +656,22 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
   for (int j = 0; j < PER; ++j) own += hist[threadIdx.x * PER + j];
   pre[threadIdx.x] = own;
   __syncthreads();
-  for (int off = 1; off < BEC_WG; off <<= 1) {          // inclusive Hillis-Steele scan
-    const unsigned add = (int)threadIdx.x >= off ? pre[threadIdx.x - off] : 0u;
+  for (int o2 = 1; o2 < BEC_WG; o2 <<= 1) {          // inclusive Hillis-Steele scan
+    const unsigned add = (int)threadIdx.x >= o2 ? pre[threadIdx.x - o2] : 0u;
     __syncthreads();
     pre[threadIdx.x] += add;
     __syncthreads();
   }
   const unsigned before = pre[threadIdx.x] - own;
-  if (before <= k && k < pre[threadIdx.x]) {             // exactly one thread
+  if (before <= k && k < pre[threadIdx.x]) {           // exactly one thread
     unsigned run = before;
-    int b = threadIdx.x * PER;
-    for (int j = 0; j < PER - 1; ++j, ++b) {
-      const unsigned hb = hist[b];
+    int bb = threadIdx.x * PER;
+    for (int j = 0; j < PER - 1; ++j, ++bb) {
+      const unsigned hb = hist[bb];
       if (run + hb > k) break;
       run += hb;
     }
-    sbin = b;
+    sbin = bb;
     srank = k - run;
   }
   __syncthreads();
@@ -616,23 +681,28 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
   }
   const int bin = sbin;
   const double scale = (double)SIGMA_BINS / (hi - lo);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const double v = loc[i];
+  ELEM_LOOP(n) {
+    const double v = loc[off + i];
     if (v == v && sigma_bin(v, lo, scale) == bin) cand[atomicAdd(&st->ncand, 1u)] = v;
   }
 }
 
-// one workgroup: exact selection (bitonic sort in LDS, or an 8-bit radix
-// select over the order keys for a large bin), sigma = max(median, floor),
-// then reset the histogram and statistics for the next call
-__global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* st, unsigned* hist,
-                                                         const double* __restrict__ cand, double floor_,
-                                                         double* sigma_out, double* host_slot) {
+// one workgroup per instance: exact selection (bitonic sort in LDS, or an
+// 8-bit radix select over the order keys for a large bin), sigma =
+// max(median, floor), then reset the histogram and statistics
+__global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* stats, unsigned* hists,
+                                                         const double* __restrict__ cands, long long n,
+                                                         double floor_, double* sigmas, double* hred,
+                                                         unsigned long long act) {
   __shared__ double sv[SIGMA_LDS_CAND];
   __shared__ unsigned dh[256];
   __shared__ unsigned long long prefix;
   __shared__ unsigned rank_s;
+  const int b = blockIdx.x;
+  if (!inst_on(act, b)) return;
+  SigmaStats* st = stats + b;
+  unsigned* hist = hists + (long long)b * SIGMA_BINS;
+  const double* cand = cands + (long long)b * n;
   const unsigned cnt = st->cnt;
   const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
   double med = floor_;
@@ -652,9 +722,9 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* st, unsigne
           for (unsigned i = threadIdx.x; i < P; i += blockDim.x) {
             const unsigned ixj = i ^ j;
             if (ixj > i) {
-              const double a = sv[i], b = sv[ixj];
+              const double x = sv[i], y = sv[ixj];
               const bool up = ((i & kk) == 0);
-              if ((a > b) == up) { sv[i] = b; sv[ixj] = a; }
+              if ((x > y) == up) { sv[i] = y; sv[ixj] = x; }
             }
           }
           __syncthreads();
@@ -666,7 +736,7 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* st, unsigne
       if (threadIdx.x == 0) { prefix = 0ull; rank_s = r; }
       __syncthreads();
       for (int shift = 56; shift >= 0; shift -= 8) {
-        for (int b = threadIdx.x; b < 256; b += blockDim.x) dh[b] = 0u;
+        for (int q = threadIdx.x; q < 256; q += blockDim.x) dh[q] = 0u;
         __syncthreads();
         const unsigned long long pr = prefix;
         const unsigned long long hmask = (shift == 56) ? 0ull : (~0ull << (shift + 8));
@@ -677,12 +747,12 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* st, unsigne
         __syncthreads();
         if (threadIdx.x == 0) {
           unsigned run = 0, rr = rank_s;
-          int b = 0;
-          for (; b < 255; ++b) {
-            if (run + dh[b] > rr) break;
-            run += dh[b];
+          int q = 0;
+          for (; q < 255; ++q) {
+            if (run + dh[q] > rr) break;
+            run += dh[q];
           }
-          prefix = pr | ((unsigned long long)b << shift);
+          prefix = pr | ((unsigned long long)q << shift);
           rank_s = rr - run;
         }
         __syncthreads();
@@ -692,18 +762,33 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* st, unsigne
     }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) hist[b] = 0u;
+  for (int q = threadIdx.x; q < SIGMA_BINS; q += blockDim.x) hist[q] = 0u;
   if (threadIdx.x == 0) {
     double sg = have ? med : floor_;
     sg = fmax(sg, floor_);
-    *sigma_out = sg;
-    if (host_slot) *host_slot = sg;
+    sigmas[b] = sg;
+    if (hred) hred[(long long)b * RSTRIDE + R_SIGMA] = sg;
     st->kmin = ~0ull;
     st->kmax = 0ull;
     st->cnt = 0u;
     st->bucket = 0u;
     st->rank_in = 0u;
     st->ncand = 0u;
+  }
+}
+
+__global__ __launch_bounds__(BEC_WG) void k_sigma_reset(SigmaStats* stats, unsigned* hists, double* sigmas) {
+  const int b = blockIdx.x;
+  for (int q = threadIdx.x; q < SIGMA_BINS; q += blockDim.x) hists[(long long)b * SIGMA_BINS + q] = 0u;
+  if (threadIdx.x == 0) {
+    SigmaStats* st = stats + b;
+    st->kmin = ~0ull;
+    st->kmax = 0ull;
+    st->cnt = 0u;
+    st->bucket = 0u;
+    st->rank_in = 0u;
+    st->ncand = 0u;
+    sigmas[b] = 0.0;
   }
 }
 
@@ -721,8 +806,7 @@ __constant__ double c_gl_wts[4] = {0.3478548451374538, 0.6521451548625461, 0.652
 __global__ __launch_bounds__(BEC_WG) void k_kernel_real(int ndim, int M, double dx, int kind,
                                                         double2* out) {
   const long long n = (ndim == 2) ? (long long)M * M : (long long)M * M * M;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+  ELEM_LOOP(n) {
     int jx, jy, jz = 0;
     if (ndim == 2) { jx = (int)(i / M); jy = (int)(i % M); }
     else { jx = (int)(i / ((long long)M * M)); jy = (int)((i / M) % M); jz = (int)(i % M); }
@@ -757,8 +841,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kernel_real(int ndim, int M, double 
 __global__ __launch_bounds__(BEC_WG) void k_extract_khat(int ndim, int M, int H,
                                                          const double2* __restrict__ kf, double* khat) {
   const long long n = (ndim == 2) ? (long long)H * H : (long long)H * H * H;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+  ELEM_LOOP(n) {
     long long src;
     if (ndim == 2) src = (i / H) * M + (i % H);
     else src = ((i / ((long long)H * H)) * M + (i / H) % H) * M + (i % H);
@@ -769,7 +852,7 @@ __global__ __launch_bounds__(BEC_WG) void k_extract_khat(int ndim, int M, int H,
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-static int grid_for(long long n) {
+int elem_blocks(long long n) {
   long long g = (n + BEC_WG - 1) / BEC_WG;
   if (g > 1024) g = 1024;
   if (g < 1) g = 1;
@@ -788,15 +871,16 @@ LineCfg line_cfg(int L) {
 }
 
 // Lines per workgroup for a launch: at least one full wavefront, at most
-// BEC_WG threads, aiming at ~1024 workgroups so that small (single-instance
-// 2D) problems still spread over all 256 CUs.
-static int pick_lpw(int tpl, int nlines) {
+// BEC_WG threads, a power of two dividing the lines of one instance when
+// possible (so that a reducing workgroup never spans two instances), aiming
+// at ~1024 workgroups so that small problems still spread over all 256 CUs.
+static int pick_lpw(int tpl, int nl, int total) {
   const int lmin = (tpl >= 64) ? 1 : 64 / tpl;
   const int lmax = BEC_WG / tpl > lmin ? BEC_WG / tpl : lmin;
-  const int want = nlines / 1024;
-  int l = 1;
-  while (l * 2 <= want && l * 2 <= lmax) l *= 2;
-  return l < lmin ? lmin : l;
+  const int want = total / 1024;
+  int l = lmin;
+  while (l * 2 <= lmax && l * 2 <= want && nl % (l * 2) == 0) l *= 2;
+  return l;
 }
 // workgroup count rounded up to a whole number of XCD rounds (see xcd_block)
 static int xcd_grid(int nlines, int lpw) {
@@ -807,8 +891,9 @@ static int xcd_grid(int nlines, int lpw) {
 template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 static int go_pass(const PassArgs& a0, hipStream_t s) {
   PassArgs a = a0;
-  a.lpw = pick_lpw(a.tpl, a.nlines);
-  const int blocks = xcd_grid(a.nlines, a.lpw);
+  const int total = a.nb * a.nl;
+  a.lpw = pick_lpw(a.tpl, a.nl, total);
+  const int blocks = xcd_grid(total, a.lpw);
   const size_t shm = (size_t)a.lpw * a.fp.L * sizeof(double2);
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -817,7 +902,7 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
                           0, a);
   else
     hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
-  return blocks;
+  return (a.nl + a.lpw - 1) / a.lpw;
 }
 
 int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
@@ -846,8 +931,9 @@ int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a,
 template <int AX, int KIND>
 static int go_kin(const KinArgs& a0, hipStream_t s) {
   KinArgs a = a0;
-  a.lpw = pick_lpw(a.tpl, a.nlines);
-  const int blocks = xcd_grid(a.nlines, a.lpw);
+  const int total = a.nb * a.nl;
+  a.lpw = pick_lpw(a.tpl, a.nl, total);
+  const int blocks = xcd_grid(total, a.lpw);
   const size_t shm = (size_t)a.lpw * a.N * sizeof(double2);
   const int id = 100 + 10 * AX + KIND;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -855,7 +941,7 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
     hipExtLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1, 0, a);
   else
     hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
-  return blocks;
+  return (a.nl + a.lpw - 1) / a.lpw;
 }
 
 int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
@@ -879,87 +965,79 @@ int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
   return 0;
 }
 
-int launch_norm2(const double2* psi, long long n, double* hpart, hipStream_t s) {
-  const int g = grid_for(n);
-  hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BEC_WG), 0, s, psi, n, hpart);
+int launch_norm2(const double2* psi, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s) {
+  const int g = elem_blocks(n);
+  hipLaunchKernelGGL(k_norm2, dim3(g, nb), dim3(BEC_WG), 0, s, psi, n, act, hpart);
   return g;
 }
-void launch_scale(const double2* in, double2* out, double sc, long long n, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(BEC_WG), 0, s, in, out, sc, n);
+void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s) {
+  DblArr d;
+  for (int b = 0; b < BEC_BMAX; ++b) d.v[b] = b < nb ? divs[b] : 1.0;
+  hipLaunchKernelGGL(k_scale, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, in, out, d, n);
 }
-void launch_rho(const double2* psi, double* rho, long long n, hipStream_t s) {
-  hipLaunchKernelGGL(k_rho, dim3(grid_for(n)), dim3(BEC_WG), 0, s, psi, rho, n);
+void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_rho, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, rho, n);
 }
-void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, double az,
-                    double ad, double ag, double gre, double gim, long long n, hipStream_t s) {
+void launch_copy(const double2* src, double2* dst, long long n, int nb, unsigned long long mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);
+}
+void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, const CombArgs& c,
+                    long long n, int nb, unsigned long long act, hipStream_t s) {
   prof_pre(SITE_COMBINE, s);
-  hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(BEC_WG), 0, s, d, z, g, psi, az, ad, ag, gre,
-                     gim, n);
+  hipLaunchKernelGGL(k_combine, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, d, z, g, psi, c, n, act);
   prof_post(SITE_COMBINE, s);
 }
-int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g,
-                const double2* gp, const double2* pp, long long n, double* hpart, hipStream_t s) {
-  const int gr = grid_for(n);
+int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g, const double2* y,
+                const double2* sv, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s) {
+  const int gr = elem_blocks(n);
   prof_pre(SITE_DOTS, s);
-  hipLaunchKernelGGL(k_dots, dim3(gr), dim3(BEC_WG), 0, s, z, psi, d, g, gp, pp, n, hpart);
+  hipLaunchKernelGGL(k_dots, dim3(gr, nb), dim3(BEC_WG), 0, s, z, psi, d, g, y, sv, n, act, hpart);
   prof_post(SITE_DOTS, s);
   return gr;
 }
-int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n,
-                double* hpart, hipStream_t s) {
-  const int gr = grid_for(n);
+int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n, double* hpart,
+                hipStream_t s) {
+  const int gr = elem_blocks(n);
   hipLaunchKernelGGL(k_res2, dim3(gr), dim3(BEC_WG), 0, s, h, psi, mu, g_out, n, hpart);
   return gr;
 }
 int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
                    hipStream_t s) {
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
-  const int gr = grid_for(n);
+  const int gr = elem_blocks(n);
   hipLaunchKernelGGL(k_moments, dim3(gr), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials, hpart);
   return gr;
 }
-void launch_sigma(const Plan& p, const double2* psi, const double* phi, double phisc, const double* mrows,
-                  int nrows, int mkp, int mq, double beta2, double beta3, double floor_, hipStream_t s) {
+void launch_sigma(const Plan& p, int nb, unsigned long long act, const double2* psi, const double* phi,
+                  const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s) {
   long long g1 = (p.n + 4 * BEC_WG - 1) / (4 * BEC_WG);
   if (g1 > SIGMA_MAX_BLOCKS) g1 = SIGMA_MAX_BLOCKS;
-  const int g = grid_for(p.n);
+  const int g = elem_blocks(p.n);
   unsigned long long* part = (unsigned long long*)p.sig_part;
   prof_pre(SITE_SIGMA_LOCAL, s);
-  hipLaunchKernelGGL(k_sigma_local, dim3((int)g1), dim3(BEC_WG), 0, s, psi, phi, phisc, mrows, nrows, mkp,
-                     mq, beta2, beta3, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, p.loc, part);
+  hipLaunchKernelGGL(k_sigma_local, dim3((int)g1, nb), dim3(BEC_WG), 0, s, psi, phi, mrows, nrows, mkp, mq,
+                     p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part);
   prof_post(SITE_SIGMA_LOCAL, s);
   prof_pre(SITE_SIGMA_HIST, s);
-  hipLaunchKernelGGL(k_sigma_hist, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,
-                     (int)g1, p.stats, p.hist);
+  hipLaunchKernelGGL(k_sigma_hist, dim3(g, nb), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,
+                     (int)g1, p.stats, p.hist, act);
   prof_post(SITE_SIGMA_HIST, s);
   prof_pre(SITE_SIGMA_GATHER, s);
-  hipLaunchKernelGGL(k_sigma_gather, dim3(g), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist, p.cand);
+  hipLaunchKernelGGL(k_sigma_gather, dim3(g, nb), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist, p.cand, act);
   prof_post(SITE_SIGMA_GATHER, s);
   prof_pre(SITE_SIGMA_SELECT, s);
-  hipLaunchKernelGGL(k_sigma_select, dim3(1), dim3(BEC_WG), 0, s, p.stats, p.hist, p.cand, floor_,
-                     p.sigma, p.hred_dev + R_SIGMA);
+  hipLaunchKernelGGL(k_sigma_select, dim3(nb), dim3(BEC_WG), 0, s, p.stats, p.hist, p.cand, p.n, floor_, p.sigma,
+                     p.hred_dev, act);
   prof_post(SITE_SIGMA_SELECT, s);
 }
-__global__ __launch_bounds__(BEC_WG) void k_sigma_reset(SigmaStats* st, unsigned* hist, double* sigma) {
-  for (int b = threadIdx.x; b < SIGMA_BINS; b += blockDim.x) hist[b] = 0u;
-  if (threadIdx.x == 0) {
-    st->kmin = ~0ull;
-    st->kmax = 0ull;
-    st->cnt = 0u;
-    st->bucket = 0u;
-    st->rank_in = 0u;
-    st->ncand = 0u;
-    *sigma = 0.0;
-  }
-}
-void launch_init_sigma(const Plan& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_sigma_reset, dim3(1), dim3(BEC_WG), 0, s, p.stats, p.hist, p.sigma);
+void launch_init_sigma(const Plan& p, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_sigma_reset, dim3(nb), dim3(BEC_WG), 0, s, p.stats, p.hist, p.sigma);
 }
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s) {
   const long long n = (ndim == 2) ? (long long)M * M : (long long)M * M * M;
-  hipLaunchKernelGGL(k_kernel_real, dim3(grid_for(n)), dim3(BEC_WG), 0, s, ndim, M, dx, kind, out);
+  hipLaunchKernelGGL(k_kernel_real, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, ndim, M, dx, kind, out);
 }
 void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat, hipStream_t s) {
   const long long n = (ndim == 2) ? (long long)H * H : (long long)H * H * H;
-  hipLaunchKernelGGL(k_extract_khat, dim3(grid_for(n)), dim3(BEC_WG), 0, s, ndim, M, H, kf, khat);
+  hipLaunchKernelGGL(k_extract_khat, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, ndim, M, H, kf, khat);
 }

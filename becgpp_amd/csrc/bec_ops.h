@@ -1,4 +1,6 @@
 // Multi-pass operator drivers shared by the C-ABI entry points and the solver.
+// Every driver covers a batch of instances (fields laid out [b][n]); the
+// single-instance entry points use a batch of one.
 #pragma once
 #include "bec_internal.h"
 #include <stdexcept>
@@ -17,35 +19,57 @@ struct BecError : std::runtime_error {
                             __FILE__ + ":" + std::to_string(__LINE__));                  \
   } while (0)
 
-struct Phys { double beta2, beta3, Omega, G_C; };
+typedef InstPhys Phys;
 
-// energy components of (psi - t d) [/ its norm when normalise]
+// the instances a driver call covers
+struct Batch {
+  int nb;
+  unsigned long long act;     // bit b set: instance b takes part
+};
+inline Batch one_instance() { return Batch{1, 1ull}; }
+
+// energy components of (psi - t d) [/ its norm when normalise], one instance
 struct Energy {
   double E, Ekin, Etrap, Econtact, Ethree, Erot, Egrav, Lz;
   double nrm2;    // sum |psi - t d|^2 dv of the unnormalised state
 };
 
 bool plan_long_range(const Plan& p, const Phys& ph);
+void plan_reserve(Plan& p, int nb);            // grow the batch workspace to nb instances
+void plan_set_phys(Plan& p, const Phys* ph, int nb);   // per-instance physics table (synchronous)
 void plan_sync(Plan& p);                       // synchronise, then fold the queued reductions
 double* part_of(Plan& p, int site);            // device partial-row region of a reducing launch site
 double* hpart_of(Plan& p, int site);           // pinned host region, as seen by kernels
-void add_job(Plan& p, int site, int nblk, int K, int KM, int slot);   // queue a host fold
+void add_job(Plan& p, int site, int nblk, int K, int KM, int slot, const Batch& B);   // queue a host fold
+inline const double* hres(const Plan& p, int b) { return p.hred + (long long)b * RSTRIDE; }
 
-// out = scale * dV * (K * rho)_crop, optionally sum(out * dot) -> red[R_EGRAV]
-void run_conv(Plan& p, int which, const double* rho, double scale, double* out, const double* dot);
-// kinetic/L_z/local sums of psi - t d into red[R_EFIRST..], red[R_EREST..];
+// out = scale * dV * (K * rho)_crop per instance; with dot, the sums
+// sum(out_unscaled... ) -> hred[b][R_EGRAV] use out as computed (scale included)
+void run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot);
+// kinetic/L_z/local sums of psi - t[b] d into hred[b][R_EFIRST..], [R_EREST..];
 // rho_out (optional) receives |psi - t d|^2
-void run_energy_sums(Plan& p, const double2* psi, const double2* d, double t, double* rho_out);
-// assemble Energy from red (host copy must be current); sgrav = sum(phi_raw rho)
-Energy energy_from_sums(const Plan& p, const Phys& ph, bool lr, bool normalise);
-// H passes: out = (kf k^2 + rf c k) psi + [local] psi - mu psi with psi = (psi_in - t d)/div
-// materialised into psi_out when given; phi*phisc is the long-range potential
-// (copied to phi_out when given).  Reductions -> red[R_HLAST..] when reduce.
-void run_H(Plan& p, const double2* psi_in, const double2* d, double t, double div, double2* psi_out,
-           const Phys& ph, bool local_on, const double* phi, double phisc, double* phi_out, double mu,
-           double kf, double rf, double2* out, bool reduce);
-// z = IFFT(FFT(g) / (k^2/2 + sigma)) with sigma read from p.sigma (device)
-void run_precond(Plan& p, const double2* g, double2* out);
+void run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
+                     double* rho_out);
+// assemble instance b's Energy from hred (after plan_sync); sgrav = sum(phi rho)
+// with phi the potential as stored (its scale is passed as gscale)
+Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool normalise, double gscale);
+// H passes: out = (kf k^2 + rf c k) psi + [local] psi - mu psi with psi =
+// (psi_in - t d) / div; with psi_w the transformed state is written back
+// (psi_w may equal psi_in) and sdiff receives its change; ydiff receives the
+// change of out; phi * phisc[b] is the long-range potential (stored to
+// phi_out).  Reductions -> hred[b][R_HLAST..] and device rows when reduce.
+struct HArgs {
+  const double2* psi; const double2* d; const double* t; const double* div;
+  double2* psi_w; double2* sdiff;
+  bool local_on; const double* phi; const double* phisc; double* phi_out;
+  const double* mu;
+  double kf, rf; int rf_omega;
+  double2* out; double2* ydiff;
+  bool reduce;
+};
+void run_H(Plan& p, const Batch& B, const HArgs& h);
+// z = IFFT(FFT(g) / (k^2/2 + sigma[b])) with sigma read from p.sigma (device)
+void run_precond(Plan& p, const Batch& B, const double2* g, double2* out);
 
 void solver_release(Plan& p);
 void bec_set_error(const char* msg);

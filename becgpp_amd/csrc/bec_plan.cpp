@@ -1,5 +1,5 @@
-// Plan management, multi-pass operator drivers and the C-ABI entry points
-// (everything except the solver loop).
+// Plan management, multi-pass operator drivers and the single-instance C-ABI
+// entry points (the solver loop lives in bec_solver.cpp).
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -51,7 +51,7 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
   fp.L = L;
   fp.nst = (int)rad.size();
   for (int i = 0; i < BEC_MAX_STAGES; ++i) fp.rad[i] = i < fp.nst ? rad[i] : 1;
-  // twiddles exp(-2 pi i m / L) in extended precision, reduced to the first octant
+  // twiddles exp(-2 pi i m / L) evaluated in extended precision
   std::vector<double2> tw(L);
   const long double pi = 3.141592653589793238462643383279502884L;
   for (int m = 0; m < L; ++m) {
@@ -59,8 +59,7 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
     tw[m].x = (double)cosl(ang);
     tw[m].y = (double)sinl(ang);
   }
-  // exact values where they are known
-  for (int m = 0; m < L; ++m) {
+  for (int m = 0; m < L; ++m) {      // exact values on the axes
     if ((4 * m) % L == 0) {
       const int q = (4 * m) / L;
       const double c[4] = {1.0, 0.0, -1.0, 0.0}, sn[4] = {0.0, -1.0, 0.0, 1.0};
@@ -69,41 +68,96 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
     }
   }
   HIPCHK(hipMalloc(tw_dev, sizeof(double2) * L));
-  HIPCHK(hipMemcpyAsync(*tw_dev, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpy(*tw_dev, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
   fp.tw = *tw_dev;
 }
 
 // ---------------------------------------------------------------------------
-// helpers
+// workspace, reductions, physics table
 // ---------------------------------------------------------------------------
 bool plan_long_range(const Plan& p, const Phys& ph) { return p.kkind[0] != 0 && fabs(ph.G_C) > 1e-15; }
 
 double* part_of(Plan& p, int site) { return p.partials + (size_t)site * p.part_stride; }
 double* hpart_of(Plan& p, int site) { return p.hpart_dev + (size_t)site * p.part_stride; }
 
-// Fold the pinned per-block rows of every queued reduction into hred, rows
-// in block order (deterministic), sums for the first K values, maxima for
-// the next KM.
+static void free_batch(Plan& p) {
+  void* ptrs[] = {p.w1, p.s1, p.s2, p.rho, p.loc, p.cand, p.hist, p.stats, p.sigma, p.sig_part, p.iphys,
+                  p.partials};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (p.hpart) (void)hipHostFree(p.hpart);
+  if (p.hred) (void)hipHostFree(p.hred);
+  p.w1 = nullptr; p.s1 = nullptr; p.s2 = nullptr; p.rho = nullptr; p.loc = nullptr; p.cand = nullptr;
+  p.hist = nullptr; p.stats = nullptr; p.sigma = nullptr; p.sig_part = nullptr; p.iphys = nullptr;
+  p.partials = nullptr; p.hpart = nullptr; p.hred = nullptr; p.hpart_dev = nullptr; p.hred_dev = nullptr;
+  p.cap = 0;
+}
+
+void plan_reserve(Plan& p, int nb) {
+  if (nb < 1 || nb > BEC_BMAX) bad("batch size must be in [1, " + std::to_string(BEC_BMAX) + "]");
+  if (nb <= p.cap) return;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  free_batch(p);
+  const long long B = nb;
+  HIPCHK(hipMalloc(&p.w1, sizeof(double2) * p.n * B));
+  HIPCHK(hipMalloc(&p.s1, sizeof(double2) * p.nS1 * B));
+  if (p.nS2) HIPCHK(hipMalloc(&p.s2, sizeof(double2) * p.nS2 * B));
+  HIPCHK(hipMalloc(&p.rho, sizeof(double) * p.n * B));
+  HIPCHK(hipMalloc(&p.loc, sizeof(double) * p.n * B));
+  HIPCHK(hipMalloc(&p.cand, sizeof(double) * p.n * B));
+  HIPCHK(hipMalloc(&p.hist, sizeof(unsigned) * SIGMA_BINS * B));
+  HIPCHK(hipMalloc(&p.stats, sizeof(SigmaStats) * B));
+  HIPCHK(hipMalloc(&p.sigma, sizeof(double) * B));
+  HIPCHK(hipMalloc(&p.sig_part, sizeof(unsigned long long) * 4 * SIGMA_MAX_BLOCKS * B));
+  HIPCHK(hipMalloc(&p.iphys, sizeof(InstPhys) * B));
+  // partial rows: at most one per line of an instance, or one per elementwise block
+  const long long rows = std::max<long long>(std::max<long long>(p.n / p.N, 1024), (long long)p.N + 8);
+  p.max_rows = (int)rows;
+  p.part_stride = 16LL * rows * B + 16 * BEC_XCDS;
+  HIPCHK(hipMalloc(&p.partials, sizeof(double) * p.part_stride * C_COUNT));
+  HIPCHK(hipHostMalloc(&p.hpart, sizeof(double) * p.part_stride * C_COUNT,
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  HIPCHK(hipHostGetDevicePointer((void**)&p.hpart_dev, p.hpart, 0));
+  HIPCHK(hipHostMalloc(&p.hred, sizeof(double) * RSTRIDE * B, hipHostMallocMapped | hipHostMallocCoherent));
+  memset(p.hred, 0, sizeof(double) * RSTRIDE * B);
+  HIPCHK(hipHostGetDevicePointer((void**)&p.hred_dev, p.hred, 0));
+  launch_init_sigma(p, nb, p.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(p.stream));
+  p.cap = nb;
+}
+
+void plan_set_phys(Plan& p, const Phys* ph, int nb) {
+  HIPCHK(hipStreamSynchronize(p.stream));
+  HIPCHK(hipMemcpy(p.iphys, ph, sizeof(InstPhys) * nb, hipMemcpyHostToDevice));
+}
+
+// Fold the pinned per-block rows of every queued reduction into hred, per
+// active instance, rows in order (deterministic): sums for the first K
+// values, maxima for the next KM.
 static void fold_jobs(Plan& p) {
   for (int i = 0; i < p.pend.n; ++i) {
     const RedJob& j = p.pend.j[i];
     const int KT = j.K + j.KM;
     const int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
     const double* rows = p.hpart + (size_t)j.site * p.part_stride;
-    for (int q = 0; q < KT; ++q) {
-      double s = (q < j.K) ? 0.0 : -INFINITY;
-      for (int b = 0; b < j.nblk; ++b) {
-        const double x = rows[(size_t)b * KP + q];
-        s = (q < j.K) ? s + x : std::max(s, x);
+    for (int b = 0; b < j.nb; ++b) {
+      if (!((j.act >> b) & 1ull)) continue;
+      const double* rb = rows + (size_t)b * j.nblk * KP;
+      for (int q = 0; q < KT; ++q) {
+        double s = (q < j.K) ? 0.0 : -INFINITY;
+        for (int r = 0; r < j.nblk; ++r) {
+          const double x = rb[(size_t)r * KP + q];
+          s = (q < j.K) ? s + x : std::max(s, x);
+        }
+        p.hred[(size_t)b * RSTRIDE + j.slot + q] = s;
       }
-      p.hred[j.slot + q] = s;
     }
   }
   p.pend.n = 0;
 }
 
-void add_job(Plan& p, int site, int nblk, int K, int KM, int slot) {
+void add_job(Plan& p, int site, int nblk, int K, int KM, int slot, const Batch& B) {
   for (int i = 0; i < p.pend.n; ++i)
     if (p.pend.j[i].site == site) throw BecError(2, "internal: reduction site reused before a sync");
   if (p.pend.n == BEC_MAX_JOBS) throw BecError(2, "internal: too many pending reductions");
@@ -113,6 +167,8 @@ void add_job(Plan& p, int site, int nblk, int K, int KM, int slot) {
   j.K = K;
   j.KM = KM;
   j.slot = slot;
+  j.nb = B.nb;
+  j.act = B.act;
 }
 
 void plan_sync(Plan& p) {
@@ -120,12 +176,17 @@ void plan_sync(Plan& p) {
   fold_jobs(p);
 }
 
-static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
+// ---------------------------------------------------------------------------
+// drivers
+// ---------------------------------------------------------------------------
+static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, const Batch& B) {
   PassArgs a;
   memset(&a, 0, sizeof(a));
   a.fp = fp;
   a.tpl = lc.tpl;
   a.lpw = lc.lpw;
+  a.nb = B.nb;
+  a.act = B.act;
   a.ni = 1;
   a.out_scale = 1.0;
   a.H = p.H;
@@ -133,107 +194,108 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc) {
   a.Nk = p.N;
   a.invNdx = p.invNdx;
   a.sigma = p.sigma;
-  a.partials = nullptr;
-  a.hpartials = p.hpart_dev + (size_t)C_EGRAV * p.part_stride;
   return a;
 }
 
-void run_conv(Plan& p, int which, const double* rho, double scale, double* out, const double* dot) {
+void run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out,
+              const double* dot) {
   if (!p.khat[which]) throw BecError(1, "kernel spectrum not built for this plan");
   const int N = p.N, M = p.M, H = p.H, i0 = (p.M - p.N) / 2;
   const double fac = scale * p.dv / (double)p.nM;
+  const long long n = p.n, S1 = p.nS1, S2 = p.nS2;
+  int nb_rows = 0;
   if (p.ndim == 2) {
-    PassArgs c1 = pass_base(p, p.fpM, p.lcM);
-    c1.nlines = N; c1.in_so = N; c1.in_s = 1; c1.n_load = N; c1.off_load = i0;
-    c1.out_so = H; c1.out_s = 1; c1.n_store = H; c1.off_store = 0;
+    PassArgs c1 = pass_base(p, p.fpM, p.lcM, B);
+    c1.nl = N; c1.in_so = N; c1.in_s = 1; c1.in_ib = n; c1.n_load = N; c1.off_load = i0;
+    c1.out_so = H; c1.out_s = 1; c1.out_ib = S1; c1.n_store = H; c1.off_store = 0;
     c1.in = rho; c1.out = p.s1;
     launch_pass(1, LD_R, DIR_FWD, MUL_NONE, ST_C, c1, p.stream);
-    PassArgs c2 = pass_base(p, p.fpM, p.lcM);
-    c2.nlines = H; c2.ni = H; c2.in_so = 0; c2.in_si = 1; c2.in_s = H; c2.n_load = N; c2.off_load = i0;
-    c2.out_so = 0; c2.out_si = 1; c2.out_s = H; c2.n_store = N; c2.off_store = i0;
+    PassArgs c2 = pass_base(p, p.fpM, p.lcM, B);
+    c2.nl = H; c2.ni = H; c2.in_si = 1; c2.in_s = H; c2.in_ib = S1; c2.n_load = N; c2.off_load = i0;
+    c2.out_si = 1; c2.out_s = H; c2.out_ib = S1; c2.n_store = N; c2.off_store = i0;
     c2.in = p.s1; c2.out = p.s1; c2.khat = p.khat[which];
     launch_pass(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C, c2, p.stream);
-    PassArgs c3 = pass_base(p, p.fpM, p.lcM);
-    c3.nlines = N; c3.in_so = H; c3.in_s = 1;
-    c3.out_so = N; c3.out_s = 1; c3.n_store = N; c3.off_store = i0; c3.out_scale = fac;
-    c3.in = p.s1; c3.out = out; c3.dot = dot;
-    const int nb = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
-    if (dot) add_job(p, C_EGRAV, nb, 1, 0, R_EGRAV);
+    PassArgs c3 = pass_base(p, p.fpM, p.lcM, B);
+    c3.nl = N; c3.in_so = H; c3.in_s = 1; c3.in_ib = S1;
+    c3.out_so = N; c3.out_s = 1; c3.out_ib = n; c3.n_store = N; c3.off_store = i0; c3.out_scale = fac;
+    c3.in = p.s1; c3.out = out; c3.dot = dot; c3.hpartials = dot ? hpart_of(p, C_EGRAV) : nullptr;
+    nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
   } else {
     const long long NH = (long long)N * H, MH = (long long)M * H;
-    PassArgs c1 = pass_base(p, p.fpM, p.lcM);
-    c1.nlines = N * N; c1.in_so = N; c1.in_s = 1; c1.n_load = N; c1.off_load = i0;
-    c1.out_so = H; c1.out_s = 1; c1.n_store = H; c1.off_store = 0;
+    PassArgs c1 = pass_base(p, p.fpM, p.lcM, B);
+    c1.nl = N * N; c1.in_so = N; c1.in_s = 1; c1.in_ib = n; c1.n_load = N; c1.off_load = i0;
+    c1.out_so = H; c1.out_s = 1; c1.out_ib = S1; c1.n_store = H; c1.off_store = 0;
     c1.in = rho; c1.out = p.s1;
     launch_pass(1, LD_R, DIR_FWD, MUL_NONE, ST_C, c1, p.stream);
-    PassArgs c2 = pass_base(p, p.fpM, p.lcM);
-    c2.nlines = (int)NH; c2.ni = H; c2.in_so = NH; c2.in_si = 1; c2.in_s = H; c2.n_load = N;
+    PassArgs c2 = pass_base(p, p.fpM, p.lcM, B);
+    c2.nl = (int)NH; c2.ni = H; c2.in_so = NH; c2.in_si = 1; c2.in_s = H; c2.in_ib = S1; c2.n_load = N;
     c2.off_load = i0;
-    c2.out_so = MH; c2.out_si = 1; c2.out_s = H; c2.n_store = M; c2.off_store = 0;
+    c2.out_so = MH; c2.out_si = 1; c2.out_s = H; c2.out_ib = S2; c2.n_store = M; c2.off_store = 0;
     c2.in = p.s1; c2.out = p.s2;
     launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, c2, p.stream);
-    PassArgs c3 = pass_base(p, p.fpM, p.lcM);
-    c3.nlines = (int)MH; c3.ni = (int)MH; c3.in_so = 0; c3.in_si = 1; c3.in_s = MH; c3.n_load = N;
-    c3.off_load = i0;
-    c3.out_so = 0; c3.out_si = 1; c3.out_s = MH; c3.n_store = N; c3.off_store = i0;
+    PassArgs c3 = pass_base(p, p.fpM, p.lcM, B);
+    c3.nl = (int)MH; c3.ni = (int)MH; c3.in_si = 1; c3.in_s = MH; c3.in_ib = S2; c3.n_load = N; c3.off_load = i0;
+    c3.out_si = 1; c3.out_s = MH; c3.out_ib = S2; c3.n_store = N; c3.off_store = i0;
     c3.in = p.s2; c3.out = p.s2; c3.khat = p.khat[which];
     launch_pass(0, LD_C, DIR_FMI, MUL_KHAT3, ST_C, c3, p.stream);
-    PassArgs c4 = pass_base(p, p.fpM, p.lcM);
-    c4.nlines = (int)NH; c4.ni = H; c4.in_so = MH; c4.in_si = 1; c4.in_s = H; c4.n_load = M;
+    PassArgs c4 = pass_base(p, p.fpM, p.lcM, B);
+    c4.nl = (int)NH; c4.ni = H; c4.in_so = MH; c4.in_si = 1; c4.in_s = H; c4.in_ib = S2; c4.n_load = M;
     c4.off_load = 0;
-    c4.out_so = NH; c4.out_si = 1; c4.out_s = H; c4.n_store = N; c4.off_store = i0;
+    c4.out_so = NH; c4.out_si = 1; c4.out_s = H; c4.out_ib = S1; c4.n_store = N; c4.off_store = i0;
     c4.in = p.s2; c4.out = p.s1;
     launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, c4, p.stream);
-    PassArgs c5 = pass_base(p, p.fpM, p.lcM);
-    c5.nlines = N * N; c5.in_so = H; c5.in_s = 1;
-    c5.out_so = N; c5.out_s = 1; c5.n_store = N; c5.off_store = i0; c5.out_scale = fac;
-    c5.in = p.s1; c5.out = out; c5.dot = dot;
-    const int nb = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
-    if (dot) add_job(p, C_EGRAV, nb, 1, 0, R_EGRAV);
+    PassArgs c5 = pass_base(p, p.fpM, p.lcM, B);
+    c5.nl = N * N; c5.in_so = H; c5.in_s = 1; c5.in_ib = S1;
+    c5.out_so = N; c5.out_s = 1; c5.out_ib = n; c5.n_store = N; c5.off_store = i0; c5.out_scale = fac;
+    c5.in = p.s1; c5.out = out; c5.dot = dot; c5.hpartials = dot ? hpart_of(p, C_EGRAV) : nullptr;
+    nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
   }
+  if (dot) add_job(p, C_EGRAV, nb_rows, 1, 0, R_EGRAV, B);
 }
 
-static KinArgs kin_base(const Plan& p) {
+static KinArgs kin_base(const Plan& p, const Batch& B) {
   KinArgs a;
   memset(&a, 0, sizeof(a));
   a.fp = p.fpN;
   a.tpl = p.lcN.tpl;
   a.lpw = p.lcN.lpw;
   a.N = p.N;
-  a.nlines = (int)(p.n / p.N);
+  a.nl = (int)(p.n / p.N);
+  a.nb = B.nb;
+  a.act = B.act;
+  a.istride = p.n;
   a.dx = p.dx;
   a.invNdx = p.invNdx;
-  a.sc = 1.0;
   a.tc = p.tc;
   a.s_mode = p.s_mode;
   a.s_half = p.s_half;
-  a.phisc = 1.0;
-  a.partials = nullptr;
+  a.iphys = p.iphys;
+  for (int b = 0; b < BEC_BMAX; ++b) { a.t[b] = 0.0; a.sc[b] = 1.0; a.mu[b] = 0.0; a.phisc[b] = 1.0; }
   return a;
 }
 
-void run_energy_sums(Plan& p, const double2* psi, const double2* d, double t, double* rho_out) {
-  const int axes2[2] = {A2Y, A2X}, axes3[3] = {A3Z, A3Y, A3X};
-  const int* ax = p.ndim == 2 ? axes2 : axes3;
+static const int kAxes2[2] = {A2Y, A2X}, kAxes3[3] = {A3Z, A3Y, A3X};
+
+void run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
+                     double* rho_out) {
+  const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
   for (int k = 0; k < p.ndim; ++k) {
-    KinArgs a = kin_base(p);
+    KinArgs a = kin_base(p, B);
     a.psi = psi;
     a.d = d;
-    a.t = t;
-    if (k == 0) {
-      a.rho_out = rho_out;
-      a.hpartials = hpart_of(p, C_EFIRST);
-    } else {
-      a.hpartials = hpart_of(p, k == 1 ? C_EREST0 : C_EREST1);
-    }
-    const int nb = launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
-    if (k == 0) add_job(p, C_EFIRST, nb, 6, 0, R_EFIRST);
-    else add_job(p, k == 1 ? C_EREST0 : C_EREST1, nb, 2, 0, R_EREST + 2 * (k - 1));
+    if (t)
+      for (int b = 0; b < B.nb; ++b) a.t[b] = t[b];
+    const int site = (k == 0) ? C_EFIRST : (k == 1 ? C_EREST0 : C_EREST1);
+    a.hpartials = hpart_of(p, site);
+    if (k == 0) a.rho_out = rho_out;
+    const int rows = launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
+    if (k == 0) add_job(p, C_EFIRST, rows, 6, 0, R_EFIRST, B);
+    else add_job(p, site, rows, 2, 0, R_EREST + 2 * (k - 1), B);
   }
 }
 
-Energy energy_from_sums(const Plan& p, const Phys& ph, bool lr, bool normalise) {
-  const double* r = p.hred;
+Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool normalise, double gscale) {
+  const double* r = hres(p, b);
   double kin = r[R_EFIRST + 0], lz = r[R_EFIRST + 1];
   for (int k = 1; k < p.ndim; ++k) {
     kin += r[R_EREST + 2 * (k - 1)];
@@ -250,90 +312,90 @@ Energy energy_from_sums(const Plan& p, const Phys& ph, bool lr, bool normalise) 
   e.Ethree = (ph.beta3 / 3.0) * (r[R_EFIRST + 5] * dv) * i3;
   e.Lz = lz * dv * i1;
   e.Erot = -ph.Omega * e.Lz;
-  e.Egrav = lr ? 0.5 * (r[R_EGRAV] * dv) * i2 : 0.0;
+  e.Egrav = lr ? 0.5 * gscale * (r[R_EGRAV] * dv) * i2 : 0.0;
   e.E = e.Ekin + e.Etrap + e.Econtact + e.Ethree + e.Erot + e.Egrav;
   return e;
 }
 
-void run_H(Plan& p, const double2* psi_in, const double2* d, double t, double div, double2* psi_out,
-           const Phys& ph, bool local_on, const double* phi, double phisc, double* phi_out, double mu,
-           double kf, double rf, double2* out, bool reduce) {
-  const int axes2[2] = {A2Y, A2X}, axes3[3] = {A3Z, A3Y, A3X};
-  const int* ax = p.ndim == 2 ? axes2 : axes3;
-  const double2* psi_rest = psi_out ? psi_out : psi_in;
-  if ((d || div != 1.0) && !psi_out) throw BecError(1, "run_H: a transformed state must be materialised");
+void run_H(Plan& p, const Batch& B, const HArgs& h) {
+  const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
+  const bool transformed = h.d || h.div;
+  if (transformed && !h.psi_w) throw BecError(2, "internal: a transformed state must be written back");
+  const double2* psi_rest = h.psi_w ? h.psi_w : h.psi;
   for (int k = 0; k < p.ndim; ++k) {
-    KinArgs a = kin_base(p);
-    a.kf = kf;
-    a.rf = rf;
+    KinArgs a = kin_base(p, B);
+    a.kf = h.kf;
+    a.rf = h.rf;
+    a.rf_omega = h.rf_omega;
     const bool last = (k == p.ndim - 1);
     if (k == 0) {
-      a.psi = psi_in;
-      a.d = d;
-      a.t = t;
-      a.sc = div;
-      a.psi_out = psi_out;
-      a.out = last ? out : p.w1;
+      a.psi = h.psi;
+      a.d = h.d;
+      for (int b = 0; b < B.nb; ++b) {
+        if (h.t) a.t[b] = h.t[b];
+        if (h.div) a.sc[b] = h.div[b];
+      }
+      a.psi_w = h.psi_w;
+      a.sdiff = h.sdiff;
+      a.out = last ? h.out : p.w1;
     } else {
       a.psi = psi_rest;
       a.acc = p.w1;
-      a.out = last ? out : p.w1;
+      a.out = last ? h.out : p.w1;
     }
     if (last) {
-      if (local_on) {
-        a.beta2 = ph.beta2;
-        a.beta3 = ph.beta3;
-      } else {
-        a.tc = 0.0;
-        a.s_mode = 0;
+      a.local_on = h.local_on ? 1 : 0;
+      if (!h.local_on) a.s_mode = 0;
+      a.phi = h.local_on ? h.phi : nullptr;
+      a.phi_out = h.phi_out;
+      for (int b = 0; b < B.nb; ++b) {
+        if (h.phisc) a.phisc[b] = h.phisc[b];
+        if (h.mu) a.mu[b] = h.mu[b];
       }
-      a.phi = local_on ? phi : nullptr;
-      a.phisc = phisc;
-      a.phi_out = phi_out;
-      a.mu = mu;
-      a.partials = reduce ? part_of(p, C_HLAST) : nullptr;
-      a.hpartials = reduce ? hpart_of(p, C_HLAST) : nullptr;
+      a.ydiff = h.ydiff;
+      a.partials = h.reduce ? part_of(p, C_HLAST) : nullptr;
+      a.hpartials = h.reduce ? hpart_of(p, C_HLAST) : nullptr;
     }
     const int kind = (k == 0) ? KH_FIRST : (last ? KH_LAST : KH_MID);
-    const int nb = launch_kin(ax[k], kind, a, p.stream);
-    if (last && reduce) {
-      add_job(p, C_HLAST, nb, 2, 1, R_HLAST);
-      p.hlast_blocks = nb;
+    const int rows = launch_kin(ax[k], kind, a, p.stream);
+    if (last && h.reduce) {
+      add_job(p, C_HLAST, rows, 2, 1, R_HLAST, B);
+      p.hlast_rows = rows;
     }
   }
 }
 
-void run_precond(Plan& p, const double2* g, double2* out) {
+void run_precond(Plan& p, const Batch& B, const double2* g, double2* out) {
   const int N = p.N;
+  const long long n = p.n;
   const double inv_n = 1.0 / (double)p.n;
   if (p.ndim == 2) {
-    PassArgs a = pass_base(p, p.fpN, p.lcN);
-    a.nlines = N; a.in_so = N; a.in_s = 1; a.n_load = N; a.out_so = N; a.out_s = 1; a.n_store = N;
-    a.in = g; a.out = p.w1;
+    PassArgs a = pass_base(p, p.fpN, p.lcN, B);
+    a.nl = N; a.in_so = N; a.in_s = 1; a.in_ib = n; a.n_load = N; a.out_so = N; a.out_s = 1; a.out_ib = n;
+    a.n_store = N; a.in = g; a.out = p.w1;
     launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, a, p.stream);
-    PassArgs b = pass_base(p, p.fpN, p.lcN);
-    b.nlines = N; b.ni = N; b.in_si = 1; b.in_s = N; b.n_load = N; b.out_si = 1; b.out_s = N;
-    b.n_store = N; b.in = p.w1; b.out = p.w1;
+    PassArgs b = pass_base(p, p.fpN, p.lcN, B);
+    b.nl = N; b.ni = N; b.in_si = 1; b.in_s = N; b.in_ib = n; b.n_load = N; b.out_si = 1; b.out_s = N;
+    b.out_ib = n; b.n_store = N; b.in = p.w1; b.out = p.w1;
     launch_pass(0, LD_C, DIR_FMI, MUL_PRE2, ST_C, b, p.stream);
     PassArgs c = a;
     c.in = p.w1; c.out = out; c.out_scale = inv_n;
     launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, c, p.stream);
   } else {
     const long long N2 = (long long)N * N;
-    PassArgs z = pass_base(p, p.fpN, p.lcN);
-    z.nlines = (int)N2; z.in_so = N; z.in_s = 1; z.n_load = N; z.out_so = N; z.out_s = 1; z.n_store = N;
-    z.in = g; z.out = p.w1;
+    PassArgs z = pass_base(p, p.fpN, p.lcN, B);
+    z.nl = (int)N2; z.in_so = N; z.in_s = 1; z.in_ib = n; z.n_load = N; z.out_so = N; z.out_s = 1; z.out_ib = n;
+    z.n_store = N; z.in = g; z.out = p.w1;
     launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, z, p.stream);
-    PassArgs y = pass_base(p, p.fpN, p.lcN);
-    y.nlines = (int)N2; y.ni = N; y.in_so = N2; y.in_si = 1; y.in_s = N; y.n_load = N;
-    y.out_so = N2; y.out_si = 1; y.out_s = N; y.n_store = N; y.in = p.w1; y.out = p.w1;
+    PassArgs y = pass_base(p, p.fpN, p.lcN, B);
+    y.nl = (int)N2; y.ni = N; y.in_so = N2; y.in_si = 1; y.in_s = N; y.in_ib = n; y.n_load = N;
+    y.out_so = N2; y.out_si = 1; y.out_s = N; y.out_ib = n; y.n_store = N; y.in = p.w1; y.out = p.w1;
     launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, y, p.stream);
-    PassArgs x = pass_base(p, p.fpN, p.lcN);
-    x.nlines = (int)N2; x.ni = (int)N2; x.in_si = 1; x.in_s = N2; x.n_load = N;
-    x.out_si = 1; x.out_s = N2; x.n_store = N; x.in = p.w1; x.out = p.w1;
+    PassArgs x = pass_base(p, p.fpN, p.lcN, B);
+    x.nl = (int)N2; x.ni = (int)N2; x.in_si = 1; x.in_s = N2; x.in_ib = n; x.n_load = N;
+    x.out_si = 1; x.out_s = N2; x.out_ib = n; x.n_store = N; x.in = p.w1; x.out = p.w1;
     launch_pass(0, LD_C, DIR_FMI, MUL_PRE3, ST_C, x, p.stream);
-    PassArgs yi = y;
-    launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, yi, p.stream);
+    launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, y, p.stream);
     PassArgs zi = z;
     zi.in = p.w1; zi.out = out; zi.out_scale = inv_n;
     launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, zi, p.stream);
@@ -345,18 +407,16 @@ void run_precond(Plan& p, const double2* g, double2* out) {
 // ---------------------------------------------------------------------------
 static void plan_free(Plan& p) {
   solver_release(p);
-  void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1], p.w1, p.w2, p.s1, p.s2, p.rho, p.partials,
-                  p.red, p.loc, p.cand, p.hist, p.stats, p.sigma, p.sig_part};
+  free_batch(p);
+  void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1]};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (p.hred) (void)hipHostFree(p.hred);
-  if (p.hpart) (void)hipHostFree(p.hpart);
 }
 
 extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff,
                                bec_plan** out) {
   bec_plan* h = nullptr;
-  BEC_TRY({
+  try {
     if (!out) bad("null output pointer");
     *out = nullptr;
     if (ndim != 2 && ndim != 3) bad("ndim must be 2 or 3");
@@ -384,45 +444,22 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     p.s_mode = (s > 0) ? (p.s_half == 1.0 ? 1 : (p.s_half == 2.0 ? 2 : 3)) : 0;
     p.invNdx = 1.0 / (N * p.dx);
     p.stream = 0;
-    try {
-      make_fft(N, p.fpN, &p.twN, p.stream);
-      make_fft(p.M, p.fpM, &p.twM, p.stream);
-    } catch (const BecError& e) {
-      throw;
-    }
+    make_fft(N, p.fpN, &p.twN, p.stream);
+    make_fft(p.M, p.fpM, &p.twM, p.stream);
     p.lcN = line_cfg(N);
     p.lcM = line_cfg(p.M);
-    HIPCHK(hipMalloc(&p.w1, sizeof(double2) * p.n));
-    HIPCHK(hipMalloc(&p.w2, sizeof(double2) * p.n));
-    HIPCHK(hipMalloc(&p.s1, sizeof(double2) * p.nS1));
-    if (p.nS2) HIPCHK(hipMalloc(&p.s2, sizeof(double2) * p.nS2));
-    HIPCHK(hipMalloc(&p.rho, sizeof(double) * p.n));
-    // partial sums: enough blocks for the widest launch, 32 values each
-    long long lines_max = std::max<long long>(p.n / N, p.nS2 ? (long long)p.M * p.H : p.H);
-    lines_max = std::max<long long>(lines_max, (long long)p.N * p.N);
-    p.max_blocks = (int)std::max<long long>(4096, lines_max + 8);
-    p.part_stride = 16LL * p.max_blocks;
-    HIPCHK(hipMalloc(&p.partials, sizeof(double) * p.part_stride * C_COUNT));
-    HIPCHK(hipMalloc(&p.sig_part, sizeof(unsigned long long) * 4 * SIGMA_MAX_BLOCKS));
-    HIPCHK(hipHostMalloc(&p.hpart, sizeof(double) * p.part_stride * C_COUNT,
-                         hipHostMallocMapped | hipHostMallocCoherent));
-    HIPCHK(hipHostGetDevicePointer((void**)&p.hpart_dev, p.hpart, 0));
-    HIPCHK(hipMalloc(&p.red, sizeof(double) * 256));
-    HIPCHK(hipMemsetAsync(p.red, 0, sizeof(double) * 256, p.stream));
-    HIPCHK(hipHostMalloc(&p.hred, sizeof(double) * 256, hipHostMallocMapped | hipHostMallocCoherent));
-    memset(p.hred, 0, sizeof(double) * 256);
-    HIPCHK(hipHostGetDevicePointer((void**)&p.hred_dev, p.hred, 0));
-    HIPCHK(hipMalloc(&p.loc, sizeof(double) * p.n));
-    HIPCHK(hipMalloc(&p.cand, sizeof(double) * p.n));
-    HIPCHK(hipMalloc(&p.hist, sizeof(unsigned) * SIGMA_BINS));
-    HIPCHK(hipMalloc(&p.stats, sizeof(SigmaStats)));
-    HIPCHK(hipMalloc(&p.sigma, sizeof(double)));
-    launch_init_sigma(p, p.stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(p.stream));
+    plan_reserve(p, 1);
     *out = h;
-    h = nullptr;
-  });
+    return 0;
+  } catch (const BecError& e) {
+    g_err = e.what();
+    if (h) { plan_free(h->p); delete h; }
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    if (h) { plan_free(h->p); delete h; }
+    return 2;
+  }
 }
 
 extern "C" int bec_plan_destroy(bec_plan* h) {
@@ -444,27 +481,28 @@ extern "C" int bec_plan_set_stream(bec_plan* h, void* stream) {
 // forward FFT of a full M^d complex array in place (kernel spectra)
 static void fft_full_M(Plan& p, double2* a) {
   const int M = p.M;
+  const Batch B = one_instance();
   if (p.ndim == 2) {
-    PassArgs r = pass_base(p, p.fpM, p.lcM);
-    r.nlines = M; r.in_so = M; r.in_s = 1; r.n_load = M; r.out_so = M; r.out_s = 1; r.n_store = M;
+    PassArgs r = pass_base(p, p.fpM, p.lcM, B);
+    r.nl = M; r.in_so = M; r.in_s = 1; r.n_load = M; r.out_so = M; r.out_s = 1; r.n_store = M;
     r.in = a; r.out = a;
     launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, r, p.stream);
-    PassArgs c = pass_base(p, p.fpM, p.lcM);
-    c.nlines = M; c.ni = M; c.in_si = 1; c.in_s = M; c.n_load = M; c.out_si = 1; c.out_s = M;
+    PassArgs c = pass_base(p, p.fpM, p.lcM, B);
+    c.nl = M; c.ni = M; c.in_si = 1; c.in_s = M; c.n_load = M; c.out_si = 1; c.out_s = M;
     c.n_store = M; c.in = a; c.out = a;
     launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, c, p.stream);
   } else {
     const long long M2 = (long long)M * M;
-    PassArgs z = pass_base(p, p.fpM, p.lcM);
-    z.nlines = (int)M2; z.in_so = M; z.in_s = 1; z.n_load = M; z.out_so = M; z.out_s = 1; z.n_store = M;
+    PassArgs z = pass_base(p, p.fpM, p.lcM, B);
+    z.nl = (int)M2; z.in_so = M; z.in_s = 1; z.n_load = M; z.out_so = M; z.out_s = 1; z.n_store = M;
     z.in = a; z.out = a;
     launch_pass(1, LD_C, DIR_FWD, MUL_NONE, ST_C, z, p.stream);
-    PassArgs y = pass_base(p, p.fpM, p.lcM);
-    y.nlines = (int)M2; y.ni = M; y.in_so = M2; y.in_si = 1; y.in_s = M; y.n_load = M;
+    PassArgs y = pass_base(p, p.fpM, p.lcM, B);
+    y.nl = (int)M2; y.ni = M; y.in_so = M2; y.in_si = 1; y.in_s = M; y.n_load = M;
     y.out_so = M2; y.out_si = 1; y.out_s = M; y.n_store = M; y.in = a; y.out = a;
     launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, y, p.stream);
-    PassArgs x = pass_base(p, p.fpM, p.lcM);
-    x.nlines = (int)M2; x.ni = (int)M2; x.in_si = 1; x.in_s = M2; x.n_load = M;
+    PassArgs x = pass_base(p, p.fpM, p.lcM, B);
+    x.nl = (int)M2; x.ni = (int)M2; x.in_si = 1; x.in_s = M2; x.n_load = M;
     x.out_si = 1; x.out_s = M2; x.n_store = M; x.in = a; x.out = a;
     launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, x, p.stream);
   }
@@ -497,9 +535,9 @@ extern "C" int bec_plan_build_kernel(bec_plan* h, int which, int kind) {
       if (p.ndim == 3) self = SELF_NEWTON_3D / p.dx;
       else if (kind == BEC_KERNEL_NEWTON) self = 4.0 * asinh(1.0) / p.dx;
       else self = -log(p.dx) + (1.5 + 0.5 * log(2.0) - 0.25 * M_PI);
-      const double2 sc = make_double2(self, 0.0);
-      HIPCHK(hipMemcpyAsync(full, &sc, sizeof(double2), hipMemcpyHostToDevice, p.stream));
       HIPCHK(hipStreamSynchronize(p.stream));
+      const double2 sc = make_double2(self, 0.0);
+      HIPCHK(hipMemcpy(full, &sc, sizeof(double2), hipMemcpyHostToDevice));
       finish_kernel(p, which, full);
     } catch (...) {
       (void)hipFree(full);
@@ -549,7 +587,7 @@ extern "C" int bec_plan_clear_kernels(bec_plan* h) {
 }
 
 // ---------------------------------------------------------------------------
-// operator entry points
+// single-instance operator entry points
 // ---------------------------------------------------------------------------
 static Phys phys_of(const bec_phys* ph) {
   if (!ph) bad("null physics parameters");
@@ -565,10 +603,12 @@ extern "C" int bec_norm2(bec_plan* h, const void* psi, double* sum_out) {
   BEC_TRY({
     if (!h || !psi || !sum_out) bad("null argument");
     Plan& p = h->p;
-    add_job(p, C_NORM, launch_norm2((const double2*)psi, p.n, hpart_of(p, C_NORM), p.stream), 1, 0, R_NORM);
+    const Batch B = one_instance();
+    add_job(p, C_NORM, launch_norm2((const double2*)psi, p.n, 1, 1ull, hpart_of(p, C_NORM), p.stream), 1, 0,
+            R_NORM, B);
     HIPCHK(hipGetLastError());
     plan_sync(p);
-    *sum_out = p.hred[R_NORM];
+    *sum_out = hres(p, 0)[R_NORM];
   });
 }
 
@@ -577,7 +617,7 @@ extern "C" int bec_conv(bec_plan* h, int which, const double* rho, double scale,
     if (!h || !rho || !out) bad("null argument");
     if (which != BEC_WHICH_K && which != BEC_WHICH_W) bad("which must be K or W");
     Plan& p = h->p;
-    run_conv(p, which, rho, scale, out, nullptr);
+    run_conv(p, which, one_instance(), rho, scale, out, nullptr);
     HIPCHK(hipGetLastError());
   });
 }
@@ -589,19 +629,45 @@ extern "C" int bec_energy(bec_plan* h, const void* psi, const bec_phys* php, dou
     Plan& p = h->p;
     const Phys ph = phys_of(php);
     const bool lr = plan_long_range(p, ph);
-    run_energy_sums(p, (const double2*)psi, nullptr, 0.0, lr ? p.rho : nullptr);
+    const Batch B = one_instance();
+    run_energy_sums(p, B, (const double2*)psi, nullptr, nullptr, lr ? p.rho : nullptr);
     if (lr) {
-      double* phi = phi_out ? phi_out : (double*)p.w2;
-      run_conv(p, 0, p.rho, -ph.G_C, phi, p.rho);
+      double* phi = phi_out ? phi_out : p.loc;
+      run_conv(p, 0, B, p.rho, -ph.G_C, phi, p.rho);
     } else if (phi_out) {
       HIPCHK(hipMemsetAsync(phi_out, 0, sizeof(double) * p.n, p.stream));
     }
     HIPCHK(hipGetLastError());
     plan_sync(p);
-    const Energy e = energy_from_sums(p, ph, lr, false);
+    const Energy e = energy_from_sums(p, 0, ph, lr, false, 1.0);
     const double v[8] = {e.E, e.Ekin, e.Etrap, e.Econtact, e.Ethree, e.Erot, e.Egrav, e.Lz};
     memcpy(comps, v, sizeof(v));
   });
+}
+
+// H psi for one instance: phi (nullable) is the potential as given; when
+// absent and the kernel is on it is computed into scratch
+static void apply_H_one(Plan& p, const Phys& ph, const double2* psi, const double* phi_in, double2* out,
+                        bool reduce) {
+  const Batch B = one_instance();
+  plan_set_phys(p, &ph, 1);
+  const bool lr = plan_long_range(p, ph);
+  const double* phi = phi_in;
+  if (!phi && lr) {
+    launch_rho(psi, p.rho, p.n, 1, p.stream);
+    run_conv(p, 0, B, p.rho, -ph.G_C, p.loc, nullptr);
+    phi = p.loc;
+  }
+  HArgs a;
+  memset(&a, 0, sizeof(a));
+  a.psi = psi;
+  a.local_on = true;
+  a.phi = phi;
+  a.kf = 0.5;
+  a.rf_omega = 1;
+  a.out = out;
+  a.reduce = reduce;
+  run_H(p, B, a);
 }
 
 extern "C" int bec_apply_H(bec_plan* h, const void* psi, const bec_phys* php, const double* phi,
@@ -609,18 +675,7 @@ extern "C" int bec_apply_H(bec_plan* h, const void* psi, const bec_phys* php, co
   BEC_TRY({
     if (!h || !psi || !out) bad("null argument");
     Plan& p = h->p;
-    const Phys ph = phys_of(php);
-    const bool lr = plan_long_range(p, ph);
-    const double* phiu = nullptr;
-    if (phi) phiu = phi;
-    else if (lr) {
-      launch_rho((const double2*)psi, p.rho, p.n, p.stream);
-      double* tmp = (double*)p.w2;
-      run_conv(p, 0, p.rho, -ph.G_C, tmp, nullptr);
-      phiu = tmp;
-    }
-    run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, true, phiu, 1.0, nullptr, 0.0, 0.5,
-          -ph.Omega, (double2*)out, false);
+    apply_H_one(p, phys_of(php), (const double2*)psi, phi, (double2*)out, false);
     HIPCHK(hipGetLastError());
   });
 }
@@ -629,9 +684,14 @@ extern "C" int bec_Lz(bec_plan* h, const void* psi, void* out) {
   BEC_TRY({
     if (!h || !psi || !out) bad("null argument");
     Plan& p = h->p;
-    Phys ph = {0.0, 0.0, 0.0, 0.0};
-    run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, false, nullptr, 1.0, nullptr, 0.0, 0.0,
-          1.0, (double2*)out, false);
+    HArgs a;
+    memset(&a, 0, sizeof(a));
+    a.psi = (const double2*)psi;
+    a.local_on = false;
+    a.kf = 0.0;
+    a.rf = 1.0;
+    a.out = (double2*)out;
+    run_H(p, one_instance(), a);
     HIPCHK(hipGetLastError());
   });
 }
@@ -641,25 +701,17 @@ extern "C" int bec_residual(bec_plan* h, const void* psi, const bec_phys* php, d
     if (!h || !psi || !out3) bad("null argument");
     Plan& p = h->p;
     const Phys ph = phys_of(php);
-    const bool lr = plan_long_range(p, ph);
-    const double* phiu = nullptr;
-    if (lr) {
-      launch_rho((const double2*)psi, p.rho, p.n, p.stream);
-      run_conv(p, 0, p.rho, -ph.G_C, (double*)p.w2, nullptr);
-      phiu = (const double*)p.w2;
-    }
-    // h = H psi (mu = 0); Re <psi, H psi> from the fused sums
-    double2* hbuf = (double2*)p.s1;   // s1 holds >= n complex values
+    // h = H psi (mu = 0) into the conv scratch, Re <psi, H psi> from the fused sums
+    double2* hbuf = p.s1;
     if (p.nS1 < p.n) throw BecError(2, "internal: scratch too small");
-    run_H(p, (const double2*)psi, nullptr, 0.0, 1.0, nullptr, ph, true, phiu, 1.0, nullptr, 0.0, 0.5,
-          -ph.Omega, hbuf, true);
+    apply_H_one(p, ph, (const double2*)psi, nullptr, hbuf, true);
     plan_sync(p);
-    const double mu = p.hred[R_HLAST + 1] * p.dv;
+    const double mu = hres(p, 0)[R_HLAST + 1] * p.dv;
     add_job(p, C_RES2, launch_res2(hbuf, (const double2*)psi, mu, nullptr, p.n, hpart_of(p, C_RES2), p.stream),
-            1, 0, R_RES2);
+            1, 0, R_RES2, one_instance());
     HIPCHK(hipGetLastError());
     plan_sync(p);
-    const double resid = sqrt(p.hred[R_RES2] * p.dv);
+    const double resid = sqrt(hres(p, 0)[R_RES2] * p.dv);
     out3[0] = resid;
     out3[1] = mu;
     out3[2] = resid / std::max(1.0, fabs(mu));
@@ -672,33 +724,39 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
     if (!h || !g || !psi || !out) bad("null argument");
     Plan& p = h->p;
     const Phys ph = phys_of(php);
+    const Batch B = one_instance();
+    plan_set_phys(p, &ph, 1);
     const bool lr = plan_long_range(p, ph);
     const double* phiu = nullptr;
     if (lr) {
       if (phi) phiu = phi;
-      else {
-        launch_rho((const double2*)psi, p.rho, p.n, p.stream);
-        run_conv(p, 0, p.rho, -ph.G_C, (double*)p.w2, nullptr);
-        phiu = (const double*)p.w2;
+      else {   // the potential goes to rho: the shift selection uses loc and cand
+        launch_rho((const double2*)psi, p.rho, p.n, 1, p.stream);
+        run_conv(p, 0, B, p.rho, -ph.G_C, p.rho, nullptr);
+        phiu = p.rho;
       }
     }
     // max rho: the sigma pass folds the per-block maxima of the moments pass
-    const int nbm = launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM),
-                                   hpart_of(p, C_MOM), p.stream);
-    add_job(p, C_MOM, nbm, 4, 1, R_MOM);
-    launch_sigma(p, (const double2*)psi, phiu, 1.0, part_of(p, C_MOM), nbm, 8, 4, ph.beta2, ph.beta3, shift_min,
-                 p.stream);
+    const int nbm = launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM), hpart_of(p, C_MOM),
+                                   p.stream);
+    add_job(p, C_MOM, nbm, 4, 1, R_MOM, B);
+    launch_sigma(p, 1, 1ull, (const double2*)psi, phiu, part_of(p, C_MOM), nbm, 8, 4, shift_min, p.stream);
     double2* z = (double2*)out;
-    run_precond(p, (const double2*)g, z);
+    run_precond(p, B, (const double2*)g, z);
     // projection onto the tangent space: z - psi <psi, z> dv
-    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, (const double2*)psi, p.n, hpart_of(p, C_DOTS),
-                                   p.stream), 16, 0, R_DOTS);
+    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, z, p.n, 1, 1ull, hpart_of(p, C_DOTS), p.stream),
+            16, 0, R_DOTS, B);
     plan_sync(p);
-    const double cre = p.hred[R_DOTS + 0] * p.dv, cim = p.hred[R_DOTS + 1] * p.dv;
-    launch_combine(z, z, nullptr, (const double2*)psi, 1.0, 0.0, 0.0, -cre, -cim, p.n, p.stream);
+    const double* r = hres(p, 0);
+    CombArgs cb;
+    memset(&cb, 0, sizeof(cb));
+    cb.az[0] = 1.0;
+    cb.gre[0] = -r[R_DOTS + 0] * p.dv;
+    cb.gim[0] = -r[R_DOTS + 1] * p.dv;
+    launch_combine(z, z, nullptr, (const double2*)psi, cb, p.n, 1, 1ull, p.stream);
     HIPCHK(hipGetLastError());
     plan_sync(p);
-    if (sigma_out) *sigma_out = p.hred[R_SIGMA];
+    if (sigma_out) *sigma_out = hres(p, 0)[R_SIGMA];
   });
 }
 
@@ -707,9 +765,9 @@ extern "C" int bec_moments(bec_plan* h, const void* psi, double* out5) {
     if (!h || !psi || !out5) bad("null argument");
     Plan& p = h->p;
     add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream),
-            4, 1, R_MOM);
+            4, 1, R_MOM, one_instance());
     HIPCHK(hipGetLastError());
     plan_sync(p);
-    memcpy(out5, p.hred + R_MOM, sizeof(double) * 5);
+    memcpy(out5, hres(p, 0) + R_MOM, sizeof(double) * 5);
   });
 }

@@ -1,35 +1,43 @@
-// Ground-state solver loop (becgpp/solvers.py:48-226) driving the device
-// kernels.  Fields stay in HBM; each iteration synchronises once for the
+// Ground-state solver loop (becgpp/solvers.py:48-226) over a batch of
+// instances of one grid.  Fields stay in HBM, laid out [instance][point];
+// every kernel launch covers all instances that take part in the current
+// phase.  The scalar logic of each instance is the reference's, in the same
+// order, run on the host between synchronisations: one per iteration for the
 // inner products of the CG step (batched with the previous step's residual)
-// and once per Armijo trial.  Scalar logic is the reference's, in the same
-// order, on the host.
+// and one per Armijo trial round.
+//
+// psi_prev and grad_prev are kept as the differences s = psi - psi_prev and
+// y = grad - grad_prev, written by the H passes when a step is accepted, so
+// every instance updates psi and grad in place.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
 #include <chrono>
+#include <vector>
 
 #include "../../include/becgpp_hip.h"
 #include "bec_ops.h"
 
-enum { S_P0 = 0, S_P1, S_G0, S_G1, S_D, S_Z, S_BEST, S_COUNT };
+enum { S_PSI = 0, S_SD, S_G, S_Y, S_D, S_Z, S_BEST, S_COUNT };
 
 void solver_release(Plan& p) {
-  if (!p.st_alloc) return;
   for (int i = 0; i < 8; ++i)
     if (p.st[i]) (void)hipFree(p.st[i]);
   for (int i = 0; i < 2; ++i)
     if (p.stphi[i]) (void)hipFree(p.stphi[i]);
   memset(p.st, 0, sizeof(p.st));
   memset(p.stphi, 0, sizeof(p.stphi));
-  p.st_alloc = false;
+  p.scap = 0;
 }
 
-static void solver_alloc(Plan& p) {
-  if (p.st_alloc) return;
-  for (int i = 0; i < S_COUNT; ++i) HIPCHK(hipMalloc(&p.st[i], sizeof(double2) * p.n));
-  for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&p.stphi[i], sizeof(double) * p.n));
-  p.st_alloc = true;
+static void solver_alloc(Plan& p, int nb) {
+  if (nb <= p.scap) return;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  solver_release(p);
+  for (int i = 0; i < S_COUNT; ++i) HIPCHK(hipMalloc(&p.st[i], sizeof(double2) * p.n * nb));
+  for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&p.stphi[i], sizeof(double) * p.n * nb));
+  p.scap = nb;
 }
 
 static double now_s() {
@@ -47,138 +55,201 @@ static inline C2 operator*(double s, C2 a) { return cm(s * a.re, s * a.im); }
 static inline C2 operator*(C2 a, C2 b) { return cm(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
 static inline C2 conj(C2 a) { return cm(a.re, -a.im); }
 
-extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solver_cfg* cf,
-                                void* psi_io, bec_solve_out* out, double* trace, int trace_cap) {
-  try {
-    if (!h || !php || !cf || !psi_io || !out) throw BecError(1, "null argument");
-    Plan& p = h->p;
-    const Phys ph = {php->beta2, php->beta3, php->Omega, php->G_C};
-    if (cf->cg_beta < 0 || cf->cg_beta > 2) throw BecError(1, "cg_beta must be 'pr', 'pr_precond' or 'none'");
-    solver_alloc(p);
-    const bool lr = plan_long_range(p, ph);
-    const double dv = p.dv;
-    const long long n = p.n;
-    double2* P[2] = {p.st[S_P0], p.st[S_P1]};
-    double2* Gr[2] = {p.st[S_G0], p.st[S_G1]};
-    double2* D = p.st[S_D];
-    double2* Z = p.st[S_Z];
-    double2* BEST = p.st[S_BEST];
-    double* phi = p.stphi[0];
-    double* phit = p.stphi[1];
-    hipStream_t s = p.stream;
+// per-instance scalar state of the loop (the reference's local variables)
+struct Inst {
+  double E, mu, resid_rel, best_resid, alpha, sigma, zg_prev, gg_prev, zg_cur, gg_cur, E_ref, trial, gd,
+      trial_acc;
+  double cz_re, cz_im;
+  int it_ref, n_fail, n_iter, stop, it_acc, ls;
+  bool converged, done, pending, lr;
+  long long n_energy;
+  Energy et;
+};
 
-    const int maxit = cf->maxit;
-    const double res_tol = cf->res_tol, etol = cf->energy_tol;
-    const int check = std::max(1, cf->check);
-    double alpha = cf->step;
-    const double alpha_min = 1e-8, alpha_max = cf->step_max;
-    const int ls_max = cf->linesearch_max;
-    const int stall_max = std::max(1, cf->linesearch_stall);
-    const double c1 = 1e-4, shrink = 0.5, growth = 1.1;
-    const double bb_min = 0.05, bb_max = alpha_max, bb_mix = 0.5;
-    const int restart_period = cf->cg_restart;
-    const int conv_window = cf->conv_window;
-    const double floor_ = cf->precond_shift_min;
-    int n_tr = 0;
+static Batch mask_batch(int nb, unsigned long long m) { return Batch{nb, m}; }
 
-    // psi = psi0 / norm_of(psi0)                                   (solvers.py:56-58)
-    add_job(p, C_NORM, launch_norm2((const double2*)psi_io, n, hpart_of(p, C_NORM), s), 1, 0, R_NORM);
-    plan_sync(p);
-    const double nrm0 = sqrt(p.hred[R_NORM] * dv);
-    int c = 0;
-    launch_scale((const double2*)psi_io, P[c], nrm0, n, s);
-    HIPCHK(hipMemsetAsync(D, 0, sizeof(double2) * n, s));
+static int lmin_of(int tpl) { return tpl >= 64 ? 1 : 64 / tpl; }
 
-    // E, phi = energy_phi(psi)                                       (solvers.py:106)
-    long long n_energy = 1;
-    run_energy_sums(p, P[c], nullptr, 0.0, lr ? p.rho : nullptr);
-    if (lr) run_conv(p, 0, p.rho, -ph.G_C, phi, p.rho);
-    HIPCHK(hipGetLastError());
-    plan_sync(p);
-    Energy e0 = energy_from_sums(p, ph, lr, false);
-    double E = e0.E;
-    double mu = e0.E + e0.Econtact + 2.0 * e0.Ethree + e0.Egrav;
-    const double t0 = now_s();
-    if (trace && trace_cap > 0) {
-      trace[0] = 0.0; trace[1] = 0.0; trace[2] = E; trace[3] = NAN;
-      n_tr = 1;
+static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
+                       bec_solve_out* out, double* trace, int trace_cap) {
+  if (nb < 1 || nb > BEC_BMAX) throw BecError(1, "batch size must be in [1, 32]");
+  if (cf->cg_beta < 0 || cf->cg_beta > 2) throw BecError(1, "cg_beta must be 'pr', 'pr_precond' or 'none'");
+  if (nb > 1) {
+    // reducing line passes need whole workgroups per instance
+    const int nlk = (int)(p.n / p.N);
+    if (nlk % lmin_of(p.lcN.tpl) != 0 || nlk % lmin_of(p.lcM.tpl) != 0)
+      throw BecError(1, "batched solves need a grid with N >= 32 (2D) or N >= 16 (3D)");
+  }
+  std::vector<Phys> ph(nb);
+  for (int b = 0; b < nb; ++b) ph[b] = Phys{php[b].beta2, php[b].beta3, php[b].Omega, php[b].G_C};
+  plan_reserve(p, nb);
+  solver_alloc(p, nb);
+  plan_set_phys(p, ph.data(), nb);
+  const double dv = p.dv;
+  const long long n = p.n;
+  double2* PSI = p.st[S_PSI];
+  double2* SD = p.st[S_SD];
+  double2* G = p.st[S_G];
+  double2* Y = p.st[S_Y];
+  double2* D = p.st[S_D];
+  double2* Z = p.st[S_Z];
+  double2* BEST = p.st[S_BEST];
+  double* phi = p.stphi[0];     // long-range potential of psi (scaled)
+  double* phit = p.stphi[1];    // unscaled convolution of the latest trial
+  hipStream_t s = p.stream;
+
+  const int maxit = cf->maxit;
+  const double res_tol = cf->res_tol, etol = cf->energy_tol;
+  const int check = std::max(1, cf->check);
+  const double alpha_min = 1e-8, alpha_max = cf->step_max;
+  const int ls_max = cf->linesearch_max;
+  const int stall_max = std::max(1, cf->linesearch_stall);
+  const double c1 = 1e-4, shrink = 0.5, growth = 1.1;
+  const double bb_min = 0.05, bb_max = alpha_max, bb_mix = 0.5;
+  const int restart_period = cf->cg_restart;
+  const int conv_window = cf->conv_window;
+  const double floor_ = cf->precond_shift_min;
+  const unsigned long long all = (nb == 64) ? ~0ull : ((1ull << nb) - 1ull);
+
+  std::vector<Inst> I(nb);
+  unsigned long long lrmask = 0;
+  for (int b = 0; b < nb; ++b) {
+    memset(&I[b], 0, sizeof(Inst));
+    I[b].lr = plan_long_range(p, ph[b]);
+    if (I[b].lr) lrmask |= 1ull << b;
+    I[b].alpha = cf->step;
+    I[b].sigma = floor_;
+    I[b].stop = BEC_STOP_MAXIT;
+  }
+  if (lrmask && !p.khat[0]) throw BecError(1, "kernel spectrum not built for this plan");
+  const bool lr_any = lrmask != 0;
+  int n_tr = 0;
+
+  // psi = psi0 / norm_of(psi0)                                        (solvers.py:56-58)
+  add_job(p, C_NORM, launch_norm2(psi_io, n, nb, all, hpart_of(p, C_NORM), s), 1, 0, R_NORM, Batch{nb, all});
+  plan_sync(p);
+  std::vector<double> nrm0(nb);
+  for (int b = 0; b < nb; ++b) nrm0[b] = sqrt(hres(p, b)[R_NORM] * dv);
+  launch_scale(psi_io, PSI, nrm0.data(), n, nb, s);
+  HIPCHK(hipMemsetAsync(SD, 0, sizeof(double2) * n * nb, s));
+  HIPCHK(hipMemsetAsync(Y, 0, sizeof(double2) * n * nb, s));
+  HIPCHK(hipMemsetAsync(D, 0, sizeof(double2) * n * nb, s));
+  HIPCHK(hipMemsetAsync(phi, 0, sizeof(double) * n * nb, s));
+  HIPCHK(hipMemsetAsync(phit, 0, sizeof(double) * n * nb, s));
+
+  // E, phi = energy_phi(psi)                                            (solvers.py:106)
+  const Batch ALL{nb, all};
+  run_energy_sums(p, ALL, PSI, nullptr, nullptr, lr_any ? p.rho : nullptr);
+  if (lr_any) run_conv(p, 0, mask_batch(nb, lrmask), p.rho, 1.0, phi, p.rho);
+  HIPCHK(hipGetLastError());
+  plan_sync(p);
+  std::vector<double> mu_arr(nb), phisc(nb), tarr(nb), divarr(nb);
+  for (int b = 0; b < nb; ++b) {
+    const Energy e0 = energy_from_sums(p, b, ph[b], I[b].lr, false, -ph[b].G_C);
+    I[b].E = e0.E;
+    I[b].mu = e0.E + e0.Econtact + 2.0 * e0.Ethree + e0.Egrav;
+    I[b].E_ref = e0.E;
+    I[b].n_energy = 1;
+    mu_arr[b] = I[b].mu;
+    phisc[b] = I[b].lr ? -ph[b].G_C : 0.0;
+  }
+  const double t0 = now_s();
+  if (trace && trace_cap > 0 && nb == 1) {
+    trace[0] = 0.0; trace[1] = 0.0; trace[2] = I[0].E; trace[3] = NAN;
+    n_tr = 1;
+  }
+  // grad, mu, resid = residual(psi, phi); phi becomes the scaled potential  (solvers.py:112)
+  {
+    HArgs a;
+    memset(&a, 0, sizeof(a));
+    a.psi = PSI;
+    a.local_on = true;
+    a.phi = lr_any ? phi : nullptr;
+    a.phisc = phisc.data();
+    a.phi_out = lr_any ? phi : nullptr;
+    a.mu = mu_arr.data();
+    a.kf = 0.5;
+    a.rf_omega = 1;
+    a.out = G;
+    a.reduce = true;
+    run_H(p, ALL, a);
+  }
+  HIPCHK(hipGetLastError());
+  plan_sync(p);
+  for (int b = 0; b < nb; ++b) {
+    I[b].resid_rel = sqrt(hres(p, b)[R_HLAST] * dv) / std::max(1.0, fabs(I[b].mu));
+    I[b].best_resid = I[b].resid_rel;
+  }
+  launch_copy(PSI, BEST, n, nb, all, s);
+
+  int it = 0;
+  while (true) {
+    unsigned long long run = 0;          // instances that execute iteration `it`
+    for (int b = 0; b < nb; ++b)
+      if (!I[b].done && it < maxit) run |= 1ull << b;
+    if (run) {
+      // sigma, z = _preconditioner(grad, psi, phi)                      (solvers.py:129)
+      launch_sigma(p, nb, run, PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2, floor_, s);
+      run_precond(p, mask_batch(nb, run), G, Z);
+      add_job(p, C_DOTS, launch_dots(Z, PSI, D, G, Y, SD, n, nb, run, hpart_of(p, C_DOTS), s), 16, 0, R_DOTS,
+              mask_batch(nb, run));
+      HIPCHK(hipGetLastError());
     }
-    // grad, mu, resid = residual(psi, phi)                           (solvers.py:112)
-    run_H(p, P[c], nullptr, 0.0, 1.0, nullptr, ph, true, lr ? phi : nullptr, 1.0, nullptr, mu, 0.5,
-          -ph.Omega, Gr[c], true);
-    HIPCHK(hipGetLastError());
     plan_sync(p);
-    double resid_rel = sqrt(p.hred[R_HLAST] * dv) / std::max(1.0, fabs(mu));
-    double best_resid = resid_rel;
-    HIPCHK(hipMemcpyAsync(BEST, P[c], sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
-
-    bool have_prev = false;          // psi_prev / grad_prev differ from psi / grad
-    bool converged = false;
-    int stop = BEC_STOP_MAXIT;
-    double E_ref = E;
-    int it_ref = 0;
-    int n_fail = 0, n_iter = 0;
-    double sigma = floor_;
-    double zg_prev = 0.0, gg_prev = 0.0;
-    bool pending = false;            // an accepted step whose residual is still on the device
-    int it_acc = 0;
-    double trial_acc = 0.0;
-
-    int it = 0;
-    while (true) {
-      const bool run_it = it < maxit;
-      if (run_it) {
-        // sigma, z = _preconditioner(grad, psi, phi)                 (solvers.py:129)
-        // the mask's max rho comes from the per-block maxima of the last H pass
-        launch_sigma(p, P[c], lr ? phi : nullptr, 1.0, part_of(p, C_HLAST), p.hlast_blocks, 4, 2, ph.beta2,
-                     ph.beta3, floor_, s);
-        run_precond(p, Gr[c], Z);
-        const double2* pp = have_prev ? P[1 - c] : P[c];
-        const double2* gp = have_prev ? Gr[1 - c] : Gr[c];
-        add_job(p, C_DOTS, launch_dots(Z, P[c], D, Gr[c], gp, pp, n, hpart_of(p, C_DOTS), s), 16, 0, R_DOTS);
-        HIPCHK(hipGetLastError());
+    unsigned long long bestmask = 0;
+    for (int b = 0; b < nb; ++b) {
+      Inst& q = I[b];
+      if (!q.pending) continue;
+      // residual bookkeeping of the step accepted at iteration it_acc     (solvers.py:196-221)
+      q.pending = false;
+      q.resid_rel = sqrt(hres(p, b)[R_HLAST] * dv) / std::max(1.0, fabs(q.mu));
+      if (trace && nb == 1 && n_tr < trace_cap) {
+        double* r = trace + 4 * n_tr;
+        r[0] = now_s() - t0; r[1] = q.it_acc + 1; r[2] = q.E; r[3] = q.resid_rel;
+        ++n_tr;
       }
-      plan_sync(p);
-      if (pending) {
-        // residual bookkeeping of the step accepted at iteration it_acc (solvers.py:196-221)
-        pending = false;
-        resid_rel = sqrt(p.hred[R_HLAST] * dv) / std::max(1.0, fabs(mu));
-        if (trace && n_tr < trace_cap) {
-          double* r = trace + 4 * n_tr;
-          r[0] = now_s() - t0; r[1] = it_acc + 1; r[2] = E; r[3] = resid_rel;
-          ++n_tr;
-        }
-        if (resid_rel < best_resid) {
-          best_resid = resid_rel;
-          HIPCHK(hipMemcpyAsync(BEST, P[c], sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
-        }
-        if (cf->verbose && it_acc % check == 0)
-          printf("  [gs] it=%d E=%.8f res=%.2e best=%.2e step=%.2e\n", it_acc, E, resid_rel, best_resid,
-                 trial_acc);
-        if (resid_rel < res_tol) {
-          converged = true;
-          stop = BEC_STOP_CONVERGED;
-          if (cf->verbose) printf("[gs] converged it=%d: E=%.10g res=%.3e\n", it_acc, E, resid_rel);
-          break;
-        }
-        if (it_acc - it_ref >= conv_window) {
-          const double dE_win = fabs(E - E_ref) / std::max(1.0, fabs(E));
-          if (dE_win < etol) {
-            converged = true;
-            stop = BEC_STOP_ENERGY_CONVERGED;
-            if (cf->verbose)
-              printf("[gs] energy converged it=%d: dE_window=%.2e res=%.3e best_res=%.3e\n", it_acc, dE_win,
-                     resid_rel, best_resid);
-            break;
-          }
-          E_ref = E;
-          it_ref = it_acc;
-        }
+      if (q.resid_rel < q.best_resid) {
+        q.best_resid = q.resid_rel;
+        bestmask |= 1ull << b;
       }
-      if (!run_it) break;
-      n_iter = it + 1;
-      const double* r = p.hred + R_DOTS;
-      sigma = p.hred[R_SIGMA];
+      if (cf->verbose && q.it_acc % check == 0)
+        printf("  [gs%s%d] it=%d E=%.8f res=%.2e best=%.2e step=%.2e\n", nb > 1 ? " " : "", nb > 1 ? b : 0,
+               q.it_acc, q.E, q.resid_rel, q.best_resid, q.trial_acc);
+      if (q.resid_rel < res_tol) {
+        q.converged = true;
+        q.stop = BEC_STOP_CONVERGED;
+        q.done = true;
+        continue;
+      }
+      if (q.it_acc - q.it_ref >= conv_window) {
+        const double dE_win = fabs(q.E - q.E_ref) / std::max(1.0, fabs(q.E));
+        if (dE_win < etol) {
+          q.converged = true;
+          q.stop = BEC_STOP_ENERGY_CONVERGED;
+          q.done = true;
+          continue;
+        }
+        q.E_ref = q.E;
+        q.it_ref = q.it_acc;
+      }
+    }
+    if (bestmask) launch_copy(PSI, BEST, n, nb, bestmask, s);
+    unsigned long long act = 0;
+    for (int b = 0; b < nb; ++b) {
+      if (I[b].done) continue;
+      if (it >= maxit) { I[b].done = true; continue; }
+      act |= 1ull << b;
+    }
+    if (!act) break;
+    // per-instance CG direction and trial step                          (solvers.py:128-159)
+    CombArgs cb;
+    memset(&cb, 0, sizeof(cb));
+    for (int b = 0; b < nb; ++b) {
+      if (!((act >> b) & 1ull)) continue;
+      Inst& q = I[b];
+      q.n_iter = it + 1;
+      const double* r = hres(p, b) + R_DOTS;
+      q.sigma = hres(p, b)[R_SIGMA];
       const C2 cz = cm(r[0] * dv, r[1] * dv);        // <psi, z>
       const double pp_ = r[2] * dv;                  // <psi, psi>
       const C2 pd = cm(r[3] * dv, r[4] * dv);        // <psi, d>
@@ -187,24 +258,18 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
       const C2 py = cm(r[9] * dv, r[10] * dv);       // <psi, y>
       const double gdr = r[11] * dv, gg = r[12] * dv, gy = r[13] * dv;
       const double sy = r[14] * dv, ss = r[15] * dv;
-      // projected z_p = z - psi c
-      const double zg = zg_raw - (conj(cz) * pg).re;   // Re <z_p, g>
+      const double zg = zg_raw - (conj(cz) * pg).re;   // Re <z_p, g>, z_p = z - psi <psi, z>
       const double zy = zy_raw - (conj(cz) * py).re;   // Re <z_p, y>
-      if (it == 0) { zg_prev = zg; gg_prev = gg; }     // z_prev = z, grad_prev = grad (solvers.py:115,121)
+      if (it == 0) { q.zg_prev = zg; q.gg_prev = gg; }  // z_prev = z, grad_prev = grad (solvers.py:115,121)
       double beta;
       if (it == 0 || cf->cg_beta == BEC_CG_NONE || (restart_period > 0 && it % restart_period == 0)) {
         beta = 0.0;
       } else {
-        if (cf->cg_beta == BEC_CG_PR_PRECOND) {
-          const double den = std::max(zg_prev, 1e-300);
-          beta = std::max(0.0, zy / den);
-        } else {
-          const double den = std::max(gg_prev, 1e-300);
-          beta = std::max(0.0, gy / den);
-        }
+        if (cf->cg_beta == BEC_CG_PR_PRECOND) beta = std::max(0.0, zy / std::max(q.zg_prev, 1e-300));
+        else beta = std::max(0.0, gy / std::max(q.gg_prev, 1e-300));
         if (!std::isfinite(beta)) beta = 0.0;
       }
-      // dirc = proj_tangent(z_p + beta d)                             (solvers.py:142-149)
+      // dirc = proj_tangent(z_p + beta d), with the two descent fallbacks   (solvers.py:142-149)
       const C2 cw = cz - pp_ * cz + beta * pd;
       double az = 1.0, ad = beta, ag = 0.0;
       C2 gam = cm(0.0, 0.0) - (cz + cw);
@@ -217,87 +282,158 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
         az = 0.0; ad = 0.0; ag = 1.0; gam = cm(-pg.re, -pg.im);
         gd = gg - (conj(pg) * pg).re;
       }
-      if (it > 0) {                                    // Barzilai-Borwein (solvers.py:150-158)
-        if (sy > 1e-30 && std::isfinite(sy)) {
-          double bb = ss / sy;
-          if (std::isfinite(bb)) {
-            bb = std::min(std::max(bb, bb_min), bb_max);
-            alpha = (1.0 - bb_mix) * alpha + bb_mix * bb;
-          }
+      if (it > 0 && sy > 1e-30 && std::isfinite(sy)) {   // Barzilai-Borwein (solvers.py:150-158)
+        double bb = ss / sy;
+        if (std::isfinite(bb)) {
+          bb = std::min(std::max(bb, bb_min), bb_max);
+          q.alpha = (1.0 - bb_mix) * q.alpha + bb_mix * bb;
         }
       }
-      alpha = std::min(std::max(alpha, alpha_min), alpha_max);
-      launch_combine(D, Z, Gr[c], P[c], az, ad, ag, gam.re, gam.im, n, s);
-      // Armijo backtracking                                          (solvers.py:160-174)
-      bool accepted = false;
-      double trial = alpha;
-      Energy et{};
-      for (int ls = 0; ls < ls_max; ++ls) {
-        run_energy_sums(p, P[c], D, trial, lr ? p.rho : nullptr);
-        if (lr) run_conv(p, 0, p.rho, -ph.G_C, phit, p.rho);
-        HIPCHK(hipGetLastError());
-        plan_sync(p);
-        ++n_energy;
-        et = energy_from_sums(p, ph, lr, true);
-        if (std::isfinite(et.E) && et.E <= E - c1 * trial * gd) {
-          accepted = true;
-          break;
-        }
-        trial *= shrink;
-        if (trial < alpha_min) break;
-      }
-      if (!accepted) {                                 // (solvers.py:175-185)
-        n_fail += 1;
-        alpha = std::max(alpha * 0.5, alpha_min);
-        launch_combine(D, Z, nullptr, P[c], 1.0, 0.0, 0.0, -cz.re, -cz.im, n, s);   // direction = z
-        if (alpha <= alpha_min || n_fail >= stall_max) {
-          stop = BEC_STOP_LINE_SEARCH_STALLED;
-          if (cf->verbose)
-            printf("[gs] line search stalled it=%d: E=%.10g res=%.3e best_res=%.3e\n", it, E, resid_rel,
-                   best_resid);
-          break;
-        }
-        ++it;
-        continue;
-      }
-      // accept                                                        (solvers.py:186-196)
-      n_fail = 0;
-      zg_prev = zg;
-      gg_prev = gg;
-      E = et.E;
-      mu = et.E + et.Econtact + 2.0 * et.Ethree + et.Egrav;
-      alpha = std::min(alpha_max, trial * growth);
-      const double nrm = sqrt(et.nrm2);
-      const double inv2 = 1.0 / et.nrm2;
-      // psi_new = (psi - trial d) / nrm materialised into the other buffer;
-      // grad_new = H psi_new - mu psi_new into the other grad buffer
-      run_H(p, P[c], D, trial, nrm, P[1 - c], ph, true, lr ? phit : nullptr, inv2, lr ? phi : nullptr, mu,
-            0.5, -ph.Omega, Gr[1 - c], true);
+      q.alpha = std::min(std::max(q.alpha, alpha_min), alpha_max);
+      q.gd = gd;
+      q.trial = q.alpha;
+      q.ls = 0;
+      q.cz_re = cz.re;
+      q.cz_im = cz.im;
+      q.zg_cur = zg;           // committed to zg_prev / gg_prev on acceptance
+      q.gg_cur = gg;
+      cb.az[b] = az; cb.ad[b] = ad; cb.ag[b] = ag; cb.gre[b] = gam.re; cb.gim[b] = gam.im;
+    }
+    launch_combine(D, Z, G, PSI, cb, n, nb, act, s);
+    // Armijo backtracking, one round per trial for all searching instances (solvers.py:160-174)
+    unsigned long long T = act, acc = 0, fail = 0;
+    while (T) {
+      for (int b = 0; b < nb; ++b) tarr[b] = I[b].trial;
+      const Batch BT = mask_batch(nb, T);
+      run_energy_sums(p, BT, PSI, D, tarr.data(), lr_any ? p.rho : nullptr);
+      if (T & lrmask) run_conv(p, 0, mask_batch(nb, T & lrmask), p.rho, 1.0, phit, p.rho);
       HIPCHK(hipGetLastError());
-      c = 1 - c;
-      have_prev = true;
-      pending = true;
-      it_acc = it;
-      trial_acc = trial;
-      ++it;
+      plan_sync(p);
+      for (int b = 0; b < nb; ++b) {
+        if (!((T >> b) & 1ull)) continue;
+        Inst& q = I[b];
+        ++q.n_energy;
+        q.et = energy_from_sums(p, b, ph[b], q.lr, true, -ph[b].G_C);
+        if (std::isfinite(q.et.E) && q.et.E <= q.E - c1 * q.trial * q.gd) {
+          acc |= 1ull << b;
+          T &= ~(1ull << b);
+          continue;
+        }
+        q.trial *= shrink;
+        q.ls += 1;
+        if (q.trial < alpha_min || q.ls >= ls_max) {
+          fail |= 1ull << b;
+          T &= ~(1ull << b);
+        }
+      }
     }
-    if (best_resid < resid_rel) {
-      HIPCHK(hipMemcpyAsync(psi_io, BEST, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
-    } else {
-      HIPCHK(hipMemcpyAsync(psi_io, P[c], sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    if (fail) {                                                    // (solvers.py:175-185)
+      CombArgs cf2;
+      memset(&cf2, 0, sizeof(cf2));
+      for (int b = 0; b < nb; ++b) {
+        if (!((fail >> b) & 1ull)) continue;
+        Inst& q = I[b];
+        q.n_fail += 1;
+        q.alpha = std::max(q.alpha * 0.5, alpha_min);
+        cf2.az[b] = 1.0;                                           // direction = z_p
+        cf2.gre[b] = -q.cz_re;
+        cf2.gim[b] = -q.cz_im;
+        if (q.alpha <= alpha_min || q.n_fail >= stall_max) {
+          q.stop = BEC_STOP_LINE_SEARCH_STALLED;
+          q.done = true;
+          if (cf->verbose)
+            printf("[gs] line search stalled it=%d: E=%.10g res=%.3e best_res=%.3e\n", it, q.E, q.resid_rel,
+                   q.best_resid);
+        }
+      }
+      launch_combine(D, Z, nullptr, PSI, cf2, n, nb, fail, s);
     }
-    plan_sync(p);
-    out->iters = n_iter;
-    out->converged = converged ? 1 : 0;
-    out->stop_reason = stop;
-    out->n_trace = n_tr;
-    out->E = E;
-    out->resid_rel = resid_rel;
-    out->best_resid = best_resid;
-    out->sigma = sigma;
-    out->walltime = now_s() - t0;
-    out->n_energy = n_energy;
-    return 0;
+    if (acc) {                                                     // (solvers.py:186-196)
+      for (int b = 0; b < nb; ++b) {
+        if (!((acc >> b) & 1ull)) continue;
+        Inst& q = I[b];
+        q.n_fail = 0;
+        q.zg_prev = q.zg_cur;
+        q.gg_prev = q.gg_cur;
+        q.E = q.et.E;
+        q.mu = q.et.E + q.et.Econtact + 2.0 * q.et.Ethree + q.et.Egrav;
+        q.alpha = std::min(alpha_max, q.trial * growth);
+        const double inv2 = 1.0 / q.et.nrm2;
+        tarr[b] = q.trial;
+        divarr[b] = sqrt(q.et.nrm2);
+        mu_arr[b] = q.mu;
+        phisc[b] = q.lr ? -ph[b].G_C * inv2 : 0.0;
+        q.pending = true;
+        q.it_acc = it;
+        q.trial_acc = q.trial;
+      }
+      // psi <- (psi - trial d)/nrm in place (s = change), grad <- H psi - mu psi (y = change)
+      HArgs a;
+      memset(&a, 0, sizeof(a));
+      a.psi = PSI;
+      a.d = D;
+      a.t = tarr.data();
+      a.div = divarr.data();
+      a.psi_w = PSI;
+      a.sdiff = SD;
+      a.local_on = true;
+      a.phi = lr_any ? phit : nullptr;
+      a.phisc = phisc.data();
+      a.phi_out = lr_any ? phi : nullptr;
+      a.mu = mu_arr.data();
+      a.kf = 0.5;
+      a.rf_omega = 1;
+      a.out = G;
+      a.ydiff = Y;
+      a.reduce = true;
+      run_H(p, mask_batch(nb, acc), a);
+      HIPCHK(hipGetLastError());
+    }
+    ++it;
+  }
+  // return the lowest-residual iterate seen                             (solvers.py:225-226)
+  unsigned long long from_best = 0;
+  for (int b = 0; b < nb; ++b)
+    if (I[b].best_resid < I[b].resid_rel) from_best |= 1ull << b;
+  launch_copy(BEST, psi_io, n, nb, from_best, s);
+  launch_copy(PSI, psi_io, n, nb, all & ~from_best, s);
+  plan_sync(p);
+  const double wall = now_s() - t0;
+  for (int b = 0; b < nb; ++b) {
+    bec_solve_out& o = out[b];
+    o.iters = I[b].n_iter;
+    o.converged = I[b].converged ? 1 : 0;
+    o.stop_reason = I[b].stop;
+    o.n_trace = (nb == 1) ? n_tr : 0;
+    o.E = I[b].E;
+    o.resid_rel = I[b].resid_rel;
+    o.best_resid = I[b].best_resid;
+    o.sigma = I[b].sigma;
+    o.walltime = wall;
+    o.n_energy = I[b].n_energy;
+  }
+  return 0;
+}
+
+extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solver_cfg* cf, void* psi_io,
+                                bec_solve_out* out, double* trace, int trace_cap) {
+  try {
+    if (!h || !php || !cf || !psi_io || !out) throw BecError(1, "null argument");
+    return solve_batch(h->p, 1, php, cf, (double2*)psi_io, out, trace, trace_cap);
+  } catch (const BecError& e) {
+    bec_set_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    bec_set_error(e.what());
+    return 2;
+  }
+}
+
+extern "C" int bec_ground_state_batch(bec_plan* h, int nb, const bec_phys* php, const bec_solver_cfg* cf,
+                                      void* psi_io, bec_solve_out* out) {
+  try {
+    if (!h || !php || !cf || !psi_io || !out) throw BecError(1, "null argument");
+    return solve_batch(h->p, nb, php, cf, (double2*)psi_io, out, nullptr, 0);
   } catch (const BecError& e) {
     bec_set_error(e.what());
     return e.code;

@@ -8,11 +8,21 @@ observables are recomputed from the returned state, as in solvers.py:227-229.
 """
 import ctypes
 
+import torch
+
 from . import _lib
 from .grid import _ptr, as_field, make_grid, plan_of
 from .interactions import ensure_kernel
 from .operators import compute_residual, observables
 from .seeds import make_seed
+
+BATCH_MAX = 32
+
+
+def batchable(G):
+    """Batched solves need whole workgroups of lines per instance: N >= 32 in
+    2D, N >= 16 in 3D."""
+    return G["N"] >= (32 if G["ndim"] == 2 else 16)
 
 
 def _solver_cfg(cfg, verbose):
@@ -53,32 +63,87 @@ def ground_state(cfg, G=None, psi0=None, verbose=True):
     psi_init = psi0 if psi0 is not None else make_seed(G, cfg)
     record = bool(cfg.get("record_trace", False))
     psi, res, trace = solve_native(cfg, G, psi_init, verbose=verbose, record=record)
+    obs = _finish(psi, res, G, cfg)
+    if record:
+        obs["trace"] = trace
+    return psi.detach(), G, obs
+
+
+def solve_native_batch(cfgs, G, psi0s, verbose=False):
+    """Run up to 32 solves on one grid together (bec_ground_state_batch).
+
+    cfgs share the grid, kernel and solver knobs; beta2, beta3, Omega and G_C
+    may differ per instance.  Returns (list of psi, list of raw results)."""
+    nb = len(cfgs)
+    if nb != len(psi0s):
+        raise ValueError("one initial state per configuration")
+    if not 1 <= nb <= BATCH_MAX:
+        raise ValueError(f"batch size must be in [1, {BATCH_MAX}]")
+    lr_any = G["kernel"] != "none" and any(abs(float(c["G_C"])) > 1e-15 for c in cfgs)
+    pl = ensure_kernel(G, "K") if lr_any else plan_of(G)
+    psi = torch.stack([as_field(p0, G) for p0 in psi0s]).contiguous()
+    sc = _solver_cfg(cfgs[0], verbose)
+    ph = (_lib.Phys * nb)(*[_lib.phys(c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cfgs])
+    res = (_lib.SolveOut * nb)()
+    _lib.check(pl.lib.bec_ground_state_batch(pl.h, nb, ph, ctypes.byref(sc), _ptr(psi), res))
+    return [psi[b] for b in range(nb)], [res[b] for b in range(nb)]
+
+
+def _finish(psi, res, G, cfg):
+    """Final residual and observables of a returned state (solvers.py:227-235)."""
     resid_abs, _mu, resid_rel = compute_residual(psi, G, cfg)
     converged = bool(res.converged or (resid_rel < float(cfg["res_tol"])))
     obs = observables(psi, G, cfg)
     obs.update(iters=res.iters, walltime=res.walltime, resid_abs=resid_abs, resid_rel=resid_rel,
                min_resid=res.best_resid, converged=converged,
                stop_reason=_lib.STOP_REASONS[res.stop_reason], preconditioner_shift=res.sigma)
-    if record:
-        obs["trace"] = trace
-    return psi.detach(), G, obs
+    return obs
 
 
-def ground_state_multiseed(cfg, G=None, verbose=False):
-    """nseeds solves with the reference's seed cycle; lowest energy wins
-    (solvers.py:238-263)."""
-    n = int(cfg.get("nseeds", 1))
+def ground_state_batch(cfgs, G=None, psi0s=None, verbose=False):
+    """ground_state for many configurations of one grid at once: each entry of
+    the result is what ground_state(cfgs[i], G, psi0s[i]) returns, computed
+    by the batched device loop (32 instances per launch at most)."""
     if G is None:
-        G = make_grid(cfg)
+        G = make_grid(cfgs[0])
+    if psi0s is None:
+        psi0s = [make_seed(G, c) for c in cfgs]
+    out = []
+    for lo in range(0, len(cfgs), BATCH_MAX):
+        cs, ps = cfgs[lo:lo + BATCH_MAX], psi0s[lo:lo + BATCH_MAX]
+        psis, rs = solve_native_batch(cs, G, ps, verbose=verbose)
+        for c, p, r in zip(cs, psis, rs):
+            out.append((p.detach(), G, _finish(p, r, G, c)))
+    return out
+
+
+def _seed_cfgs(cfg, n):
+    """Per-seed configurations of ground_state_multiseed (solvers.py:244-251)."""
     rotating = abs(float(cfg.get("Omega", 0.0))) >= 0.3
     seed_cycle = ("triangular", "tf", "gaussian") if rotating else ("tf", "gaussian")
-    best = None
-    Es = []
+    cs = []
     for j in range(max(1, n)):
         c = dict(cfg)
         c["rng"] = int(cfg.get("rng", 0)) + j
         c["seed"] = seed_cycle[j % len(seed_cycle)]
-        psi, G, obs = ground_state(c, G=G, verbose=verbose)
+        cs.append(c)
+    return cs
+
+
+def ground_state_multiseed(cfg, G=None, verbose=False):
+    """nseeds solves with the reference's seed cycle; lowest energy wins
+    (solvers.py:238-263).  The seeds are solved as one batch."""
+    n = int(cfg.get("nseeds", 1))
+    if G is None:
+        G = make_grid(cfg)
+    cs = _seed_cfgs(cfg, n)
+    if len(cs) == 1 or not batchable(G):
+        runs = [ground_state(c, G=G, verbose=verbose) for c in cs]
+    else:
+        runs = ground_state_batch(cs, G=G, verbose=verbose)
+    best = None
+    Es = []
+    for c, (psi, _G, obs) in zip(cs, runs):
         Es.append(obs["E"])
         if best is None or obs["E"] < best[0]:
             best = (obs["E"], psi, obs, c)
@@ -87,4 +152,3 @@ def ground_state_multiseed(cfg, G=None, verbose=False):
     obs["seed_used"] = cbest["seed"]
     obs["seed_energies"] = ";".join(f"{e:.10g}" for e in Es)
     return psi, G, obs
-

@@ -143,6 +143,16 @@ int bec_moments(bec_plan* p, const void* psi, double* out5);
 int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
                      bec_solve_out* out, double* trace, int trace_cap);
 
+/* Batched ground_state: nb (<= 32) independent solves on one grid, e.g. the
+ * seeds of ground_state_multiseed (solvers.py:238-263) or the points of a
+ * sweep without continuation (modes.py:295-387), advanced together so that
+ * every kernel launch covers all instances still iterating.  ph[nb] gives
+ * each instance's physics (the grid, kernel and solver knobs are shared);
+ * psi_io holds the nb states one after the other; out[nb] as above (no
+ * trace).  Each instance follows the ground_state loop exactly. */
+int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
+                           bec_solve_out* out);
+
 /* Measurement hooks (no counterpart in the reference, which has no profiler
  * hooks): time every launch of one kernel with HIP events on its stream.
  * id 0 switches timing off.  Ids: kinetic passes 100 + 10*axis + kind, line

@@ -1,0 +1,139 @@
+"""Batched solves (bec_ground_state_batch) on the GPU: every instance of a
+batch must reach what the oracle's single solve reaches from the same psi0,
+instances must not leak into each other, and ground_state_multiseed must pick
+the same seed as the oracle's sequential loop (solvers.py:238-263)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import cfg
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _dens_l2(a, b):
+    ra, rb = np.abs(a) ** 2, np.abs(b) ** 2
+    return float(np.sqrt(((ra - rb) ** 2).sum()) / np.sqrt((rb ** 2).sum()))
+
+
+def _close(a, b, tol):
+    return abs(a - b) <= tol * max(1.0, abs(b))
+
+
+# one grid, per-instance physics: contact, three-body, rotation, long-range on/off
+MIXED = [
+    dict(beta2=30.0),
+    dict(beta2=30.0, beta3=5.0, Omega=0.4),
+    dict(beta2=30.0, G_C=3.0),
+    dict(beta2=10.0, G_C=-1.0, Omega=0.2),
+    dict(beta2=0.0),
+]
+
+
+@pytest.mark.parametrize("kernel", ["newton", "log"])
+def test_mixed_batch_matches_oracle(kernel):
+    base = cfg(kernel=kernel, res_tol=1e-8)
+    cs = [dict(base, **kw) for kw in MIXED]
+    Go = O.make_grid(cs[0])
+    psi0s = [O.noisy_gaussian(Go, 1e-3, seed=11 + i) for i in range(len(cs))]
+    import becgpp_amd as B
+    G = B.make_grid(cs[2])
+    runs = B.ground_state_batch(cs, G=G, psi0s=[torch.as_tensor(p).cuda() for p in psi0s])
+    for c, p0, (pd, _, od) in zip(cs, psi0s, runs):
+        po, _, oo = O.ground_state(c, O.make_grid(c), p0)
+        for k in ("E", "mu", "Lz"):
+            assert _close(od[k], oo[k], 1e-8), (c, k, od[k], oo[k])
+        assert _dens_l2(pd.cpu().numpy(), po) < 1e-6
+        assert od["resid_rel"] < 1e-7
+
+
+def test_batch_first_iterations_track_oracle():
+    """A few iterations only: every instance of the batch is still on the
+    oracle's trajectory (energies to 1e-11)."""
+    base = cfg(maxit=12, res_tol=1e-14)
+    cs = [dict(base, beta2=b2, Omega=om) for b2, om in ((50.0, 0.0), (20.0, 0.5), (80.0, 0.3))]
+    Go = O.make_grid(base)
+    psi0s = [O.noisy_gaussian(Go, 1e-3, seed=3 + i) for i in range(len(cs))]
+    import becgpp_amd as B
+    psis, rs = B.solve_native_batch(cs, B.make_grid(base), [torch.as_tensor(p).cuda() for p in psi0s])
+    for c, p0, pd, r in zip(cs, psi0s, psis, rs):
+        Eo = []
+        O.ground_state(c, Go, p0, callback=lambda it, E, rr, t: Eo.append(E))
+        assert r.iters == 12
+        assert abs(r.E - Eo[-1]) <= 1e-11 * abs(Eo[-1]), (c, r.E, Eo[-1])
+
+
+def test_identical_instances_give_identical_results():
+    """32 copies of one problem (the largest batch): the per-instance
+    reductions partition identically, so the results agree bit for bit and
+    equal what the single-instance entry point returns to round-off."""
+    c = cfg(beta2=40.0, Omega=0.3, G_C=2.0, kernel="newton", Ngrid=32, maxit=40, res_tol=1e-14)
+    import becgpp_amd as B
+    G = B.make_grid(c)
+    Go = O.make_grid(c)
+    p0 = torch.as_tensor(O.noisy_gaussian(Go, 1e-3, seed=5)).cuda()
+    psis, rs = B.solve_native_batch([c] * 32, G, [p0] * 32)
+    ref = psis[0].cpu().numpy()
+    for b in range(1, 32):
+        assert rs[b].iters == rs[0].iters and rs[b].E == rs[0].E
+        assert np.array_equal(psis[b].cpu().numpy(), ref)
+    ps, r1, _ = B.solve_native(c, G, p0)
+    assert r1.iters == rs[0].iters
+    assert abs(r1.E - rs[0].E) <= 1e-12 * abs(r1.E)
+
+
+def test_instances_stop_independently():
+    """A non-interacting instance converges early (exact E = 1) and its state
+    stays frozen while the rest of the batch keeps iterating."""
+    base = cfg(res_tol=1e-7, maxit=600)
+    cs = [dict(base, beta2=0.0), dict(base, beta2=200.0, Omega=0.5), dict(base, beta2=0.0, Omega=0.6)]
+    import becgpp_amd as B
+    runs = B.ground_state_batch(cs, G=B.make_grid(base))
+    (_, _, o0), (_, _, o1), (_, _, o2) = runs
+    assert abs(o0["E"] - 1.0) < 1e-7 and abs(o2["E"] - 1.0) < 1e-7
+    assert o0["iters"] < o1["iters"]
+    single = B.ground_state(cs[1], verbose=False)[2]
+    assert _close(o1["E"], single["E"], 1e-8)
+
+
+def test_batch_3d():
+    base = cfg(dimension="3D", Ngrid=24, L=5.0, kernel="newton", res_tol=1e-8)
+    cs = [dict(base, beta2=20.0, G_C=3.0), dict(base, beta2=20.0, beta3=2.0, G_C=3.0, Omega=0.5)]
+    Go = O.make_grid(base)
+    psi0s = [O.noisy_gaussian(Go, 1e-3, seed=21 + i) for i in range(2)]
+    import becgpp_amd as B
+    runs = B.ground_state_batch(cs, G=B.make_grid(base), psi0s=[torch.as_tensor(p).cuda() for p in psi0s])
+    for c, p0, (pd, _, od) in zip(cs, psi0s, runs):
+        po, _, oo = O.ground_state(c, Go, p0)
+        for k in ("E", "mu", "Lz"):
+            assert _close(od[k], oo[k], 1e-8), (k, od[k], oo[k])
+        assert _dens_l2(pd.cpu().numpy(), po) < 1e-6
+
+
+def test_multiseed_matches_oracle():
+    """Same seeds (rng + j, seed cycle), same winner, same energies."""
+    c = cfg(beta2=40.0, Omega=0.3, nseeds=4, rng=2, res_tol=1e-8)
+    import becgpp_amd as B
+    _, _, od = B.ground_state_multiseed(c)
+    _, _, oo = O.ground_state_multiseed(c)
+    Ed = [float(e) for e in od["seed_energies"].split(";")]
+    Eo = [float(e) for e in oo["seed_energies"].split(";")]
+    assert len(Ed) == 4
+    for a, b in zip(Ed, Eo):
+        assert _close(a, b, 1e-8), (Ed, Eo)
+    assert od["seed_used"] == oo["seed_used"]
+
+
+def test_batch_argument_errors():
+    import becgpp_amd as B
+    c = cfg(Ngrid=48)
+    G = B.make_grid(c)
+    p0 = torch.zeros((48, 48), dtype=torch.complex128, device="cuda")
+    with pytest.raises(ValueError):
+        B.solve_native_batch([c] * 33, G, [p0] * 33)
+    small = cfg(Ngrid=16, L=4.0)
+    Gs = B.make_grid(small)
+    assert not B.solvers.batchable(Gs)
+    with pytest.raises(ValueError):
+        B.solve_native_batch([small] * 2, Gs, [torch.ones((16, 16), dtype=torch.complex128, device="cuda")] * 2)
