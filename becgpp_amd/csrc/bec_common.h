@@ -88,6 +88,7 @@ template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
 // One Stockham autosort stage of radix R on a line of length L held in LDS.
 // tpl threads cooperate on the line (lane in [0, tpl)); L/tpl <= 8 is
 // guaranteed by the host, so a thread owns at most ceil(8/R) butterflies.
+// tw is the twiddle table staged in LDS by the calling kernel.
 // Every thread of the workgroup must call it (it contains barriers).
 template <int R, int SIGN>
 __device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, int lane, int tpl,
@@ -131,18 +132,27 @@ __device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, int lane, i
   __syncthreads();
 }
 
-// In-place unnormalised DFT of the LDS line s (SIGN=-1 forward, +1 inverse).
+// In-place unnormalised DFT of the LDS line s (SIGN=-1 forward, +1 inverse);
+// twl = the plan's twiddle table copied to LDS (stage_twiddles).
 template <int SIGN>
-__device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, int lane, int tpl, bool active) {
+__device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const double2* twl, int lane, int tpl,
+                                        bool active) {
   int Ns = 1;
   for (int st = 0; st < p.nst; ++st) {
     const int R = p.rad[st];
-    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
-    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
-    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
-    else fft_stage<2, SIGN>(s, p.L, Ns, lane, tpl, p.tw, active);
+    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
+    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
+    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
+    else fft_stage<2, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
     Ns *= R;
   }
+}
+
+// Copies the twiddle table (L entries) into LDS; the caller's next barrier
+// publishes it.  Issued together with the line loads, so its latency overlaps
+// theirs instead of sitting inside every FFT stage.
+__device__ __forceinline__ void stage_twiddles(double2* twl, const FftPlan& p) {
+  for (int i = threadIdx.x; i < p.L; i += blockDim.x) twl[i] = p.tw[i];
 }
 
 // ----------------------------------------------------------------------------

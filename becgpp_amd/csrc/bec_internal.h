@@ -35,12 +35,38 @@ struct InstPhys {
   double beta2, beta3, Omega, G_C;
 };
 
+// Which instances of a batch a launch covers: a host bit mask, or (in the
+// device-driven solver loop) the per-instance flag words fl[b] tested
+// against bit.
+struct Act {
+  unsigned long long m;
+  const unsigned* fl;
+  unsigned bit;
+};
+
+inline Act mask_act(unsigned long long m) { return Act{m, nullptr, 0u}; }
+
+// per-instance flags of the device-driven loop
+enum InstFlag : unsigned {
+  FL_DIR = 1,       // preconditioner, shift and inner products this step
+  FL_COMB = 2,      // a new search direction is formed this step
+  FL_SEARCH = 4,    // an Armijo trial is evaluated this step
+  FL_SLR = 8,       // ... with its long-range convolution
+  FL_FAIL = 16,     // the line search failed: direction <- preconditioned gradient
+  FL_ACC = 32,      // the trial was accepted: psi and the gradient are updated
+  FL_BEST = 64,     // the state is the lowest-residual one so far: keep a copy
+};
+
+// per-instance scalars of the device-driven loop, DS_N doubles per instance
+enum DevScal { DS_T = 0, DS_SC = 1, DS_MU = 2, DS_PHISC = 3, DS_AZ = 4, DS_AD = 5, DS_AG = 6, DS_GRE = 7,
+               DS_GIM = 8, DS_N = 16 };
+
 // Arguments of the generic line-FFT pass (pad-on-load / crop-on-store).
 struct PassArgs {
   FftPlan fp;
   int tpl, lpw;
   int nb, nl;                          // instances, lines per instance
-  unsigned long long act;              // active instances (bit b)
+  Act act;                             // active instances
   int ni;                              // line l' = o*ni + i within an instance
   long long in_so, in_si, in_s, in_ib; // input: b*in_ib + o*in_so + i*in_si, element stride in_s
   int n_load, off_load;                // loaded window [off_load, off_load+n_load) of the line
@@ -54,13 +80,14 @@ struct PassArgs {
   const double* dot;                   // optional: rows of sum(out_re * dot) (dot laid out like out)
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
+  unsigned long long* ucount;          // measurement: instances covered (nullable)
 };
 
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
 struct KinArgs {
   FftPlan fp;
   int tpl, lpw, N, nl, nb;
-  unsigned long long act;
+  Act act;
   long long istride;                                     // points per instance
   double dx, invNdx;
   const double2* psi; const double2* d;                  // loaded value (psi - t[b] d) / sc[b]
@@ -75,7 +102,9 @@ struct KinArgs {
   const double* phi;                                     // long-range potential rows x phisc[b]
   double* phi_out;
   double t[BEC_BMAX], sc[BEC_BMAX], mu[BEC_BMAX], phisc[BEC_BMAX];
+  const double* tab; unsigned tabm;                      // device scalars: bit k of tabm reads DS slot k
   double* partials; double* hpartials;
+  unsigned long long* ucount;                            // measurement: instances covered (nullable)
 };
 
 // per-instance combination d <- az z + ad d + ag g + gam psi
@@ -130,6 +159,9 @@ struct Plan {
   // solver state, sized for scap instances
   int scap;
   double2* st[8]; double* stphi[2];
+  // device-driven loop state, sized for scap instances
+  void* dinst; unsigned* flags; double* dtab; double* dtrace; int dtrace_cap;
+  int* hdone; int* hdone_dev;        // pinned: instance finished (written by the controller)
 };
 
 // reduction slots, RSTRIDE doubles per instance in Plan::hred
@@ -158,18 +190,19 @@ int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a,
 int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
 int elem_blocks(long long n);
 int launch_norm2(const double2* psi, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s);
+int reduce_blocks(long long n);      // workgroups per instance of the reducing elementwise kernels
 void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s);
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, const CombArgs& c,
-                    long long n, int nb, unsigned long long act, hipStream_t s);
+                    const double* tab, long long n, int nb, Act act, hipStream_t s);
 int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g, const double2* y,
-                const double2* sv, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s);
+                const double2* sv, long long n, int nb, Act act, double* partials, double* hpart, hipStream_t s);
 int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n, double* hpart,
                 hipStream_t s);
 void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s);
 int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
                    hipStream_t s);
-void launch_copy(const double2* src, double2* dst, long long n, int nb, unsigned long long mask, hipStream_t s);
-void launch_sigma(const Plan& p, int nb, unsigned long long act, const double2* psi, const double* phi,
+void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s);
+void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
                   const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s);
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s);
 void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat, hipStream_t s);
@@ -178,8 +211,12 @@ void launch_init_sigma(const Plan& p, int nb, hipStream_t s);
 // live timing of one watched launch site (bec_prof.cpp); site ids:
 //   kinetic passes 100 + 10*axis + kind;  line-FFT passes
 //   2000 + 1000*contig + 100*load + 10*dir + mul;  elementwise kernels < 100
-void prof_pre(int id, hipStream_t s);
+void prof_pre(int id, hipStream_t s, int units = 1);
 void prof_post(int id, hipStream_t s);
-bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop);   // events for hipExtLaunchKernelGGL
+// units: instances the launch covers (summed per watched launch)
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, int units);   // events for hipExtLaunchKernelGGL
+// device counter that watched launches of the device-driven loop increment
+// once per instance they cover (their instance count is not known on the host)
+unsigned long long* prof_counter(int id);
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,
                 SITE_SIGMA_HIST = 21, SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };

@@ -28,6 +28,17 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 __device__ __forceinline__ bool inst_on(unsigned long long act, int b) { return (act >> b) & 1ull; }
+__device__ __forceinline__ bool act_on(const Act& a, int b) {
+  return a.fl ? (a.fl[b] & a.bit) != 0u : ((a.m >> b) & 1ull) != 0ull;
+}
+// does any line of [l0, l0 + lpw) belong to an active instance?  (block-uniform)
+__device__ __forceinline__ bool block_on(const Act& a, int l0, int lpw, int nl, int total) {
+  if (l0 >= total) return false;
+  const int b0 = l0 / nl, b1 = min(l0 + lpw - 1, total - 1) / nl;
+  for (int b = b0; b <= b1; ++b)
+    if (act_on(a, b)) return true;
+  return false;
+}
 
 __device__ __forceinline__ double trap_value(double r2, double tc, int s_mode, double s_half) {
   if (s_mode == 0) return 0.0;
@@ -112,6 +123,12 @@ template <> struct Ax<A3X> {   // line l = iy*N + iz, element e = ix
   __device__ static int lidx(int l, int N) { return l / N; }
 };
 
+// per-instance scalar of a kinetic pass: from the device table when its bit
+// is set in tabm, otherwise the launch argument
+__device__ __forceinline__ double kin_scal(const KinArgs& a, int slot, int b, const double* byval) {
+  return (a.tab && ((a.tabm >> slot) & 1u)) ? a.tab[(long long)b * DS_N + slot] : byval[b];
+}
+
 template <int AX>
 __device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
   using X = Ax<AX>;
@@ -133,7 +150,9 @@ __device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
 //         the local potential (V + b2 rho + b3 rho^2 + phi) psi - mu psi.
 template <int AX, int KIND>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 smem[];
+  double2* twl = smem;                 // twiddles [N]
+  double2* lds = smem + a.N;           // lines [lpw][N]
   using X = Ax<AX>;
   const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
@@ -141,27 +160,32 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
   const long long strd = X::stride(N);
+  if (!block_on(a.act, l0, lpw, nl, total)) return;
   const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && inst_on(a.act, mb);
-  const bool brow = l0 < total && inst_on(a.act, l0 / nl);   // this block stores a partial row
+  const bool active = (l0 + c) < total && act_on(a.act, mb);
+  const bool brow = act_on(a.act, l0 / nl);   // this block stores a partial row
+  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
+  stage_twiddles(twl, a.fp);
   for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
     int cc, e;
     if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
     double2 v = make_double2(0.0, 0.0);
     const int b = l / nl;
-    if (l < total && inst_on(a.act, b)) {
+    if (l < total && act_on(a.act, b)) {
       const int li = l - b * nl;
       const long long adr = (long long)b * a.istride + X::base(li, N) + e * strd;
       const double2 p0 = a.psi[adr];
       v = p0;
       if (a.d) {
         const double2 dd = a.d[adr];
-        v.x = v.x - a.t[b] * dd.x;
-        v.y = v.y - a.t[b] * dd.y;
+        const double tb = kin_scal(a, DS_T, b, a.t);
+        v.x = v.x - tb * dd.x;
+        v.y = v.y - tb * dd.y;
       }
-      if (a.sc[b] != 1.0) { v.x = v.x / a.sc[b]; v.y = v.y / a.sc[b]; }
+      const double scb = kin_scal(a, DS_SC, b, a.sc);
+      if (scb != 1.0) { v.x = v.x / scb; v.y = v.y / scb; }
       if (KIND == KH_FIRST) {
         if (a.psi_w) a.psi_w[adr] = v;
         if (a.sdiff) a.sdiff[adr] = csub(v, p0);
@@ -180,7 +204,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   }
   __syncthreads();
   double2* s = lds + c * N;
-  fft_lds<-1>(s, a.fp, lane, tpl, active);
+  fft_lds<-1>(s, a.fp, twl, lane, tpl, active);
   const double invN = 1.0 / (double)N;
   const int li0 = (l0 + c) - mb * nl;
   const double lzc = (X::lsign == 0 || !active) ? 0.0 : (double)X::lsign * xcoord(X::lidx(li0, N), N, a.dx);
@@ -215,7 +239,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     }
   }
   __syncthreads();
-  fft_lds<+1>(s, a.fp, lane, tpl, active);
+  fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
@@ -223,7 +247,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
     const int b = l / nl;
-    if (l >= total || !inst_on(a.act, b)) continue;
+    if (l >= total || !act_on(a.act, b)) continue;
     const int li = l - b * nl;
     const long long adr = (long long)b * a.istride + X::base(li, N) + e * strd;
     double2 u = lds[cc * N + e];
@@ -241,14 +265,14 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       if (a.local_on) {
         double ph = 0.0;
         if (a.phi) {
-          ph = a.phi[adr] * a.phisc[b];
+          ph = a.phi[adr] * kin_scal(a, DS_PHISC, b, a.phisc);
           if (a.phi_out) a.phi_out[adr] = ph;
         }
         const InstPhys ip = a.iphys[b];
         W = local_V<AX>(li, e, a) + ip.beta2 * rho + ip.beta3 * rho * rho + ph;
       }
       const double2 p = a.acc ? a.acc[adr] : make_double2(0.0, 0.0);
-      const double mu = a.mu[b];
+      const double mu = kin_scal(a, DS_MU, b, a.mu);
       double2 h;
       h.x = p.x + u.x + W * ps.x - mu * ps.x;
       h.y = p.y + u.y + W * ps.y - mu * ps.y;
@@ -268,22 +292,27 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
 // ---------------------------------------------------------------------------
 template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 smem[];
   const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
+  double2* twl = smem;                 // twiddles [L]
+  double2* lds = smem + L;             // lines [lpw][L]
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
+  if (!block_on(a.act, l0, lpw, nl, total)) return;
   const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && inst_on(a.act, mb);
-  const bool brow = l0 < total && inst_on(a.act, l0 / nl);   // this block stores a partial row
+  const bool active = (l0 + c) < total && act_on(a.act, mb);
+  const bool brow = act_on(a.act, l0 / nl);   // this block stores a partial row
+  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
+  stage_twiddles(twl, a.fp);
   for (int idx = threadIdx.x; idx < lpw * L; idx += blockDim.x) {
     int cc, e;
     if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
     const int b = l / nl;
     double2 v = make_double2(0.0, 0.0);
-    if (l < total && inst_on(a.act, b)) {
+    if (l < total && act_on(a.act, b)) {
       const int li = l - b * nl;
       const int o = li / a.ni, i = li - o * a.ni;
       const long long base = b * a.in_ib + o * a.in_so + i * a.in_si;
@@ -304,8 +333,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   }
   __syncthreads();
   double2* s = lds + c * L;
-  if (DIR == DIR_INV) fft_lds<+1>(s, a.fp, lane, tpl, active);
-  else fft_lds<-1>(s, a.fp, lane, tpl, active);
+  if (DIR == DIR_INV) fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
+  else fft_lds<-1>(s, a.fp, twl, lane, tpl, active);
   if (DIR == DIR_FMI) {
     if (active) {
       const int li = (l0 + c) - mb * nl;
@@ -337,7 +366,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       }
     }
     __syncthreads();
-    fft_lds<+1>(s, a.fp, lane, tpl, active);
+    fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
   }
   double dsum[1] = {0.0};
   for (int idx = threadIdx.x; idx < lpw * a.n_store; idx += blockDim.x) {
@@ -345,7 +374,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
     const int l = l0 + cc;
     const int b = l / nl;
-    if (l >= total || !inst_on(a.act, b)) continue;
+    if (l >= total || !act_on(a.act, b)) continue;
     const int li = l - b * nl;
     const int o = li / a.ni, i = li - o * a.ni;
     const long long adr = b * a.out_ib + o * a.out_so + i * a.out_si + e * a.out_s;
@@ -398,22 +427,28 @@ __global__ __launch_bounds__(BEC_WG) void k_rho(const double2* __restrict__ in, 
 }
 
 __global__ __launch_bounds__(BEC_WG) void k_copy(const double2* __restrict__ src, double2* dst, long long n,
-                                                 unsigned long long mask) {
+                                                 Act mask) {
   const int b = blockIdx.y;
-  if (!inst_on(mask, b)) return;
+  if (!act_on(mask, b)) return;
   const long long off = (long long)b * n;
   ELEM_LOOP(n) dst[off + i] = src[off + i];
 }
 
-// d <- az z + ad d + ag g + gam psi   (gam complex), per instance
+// d <- az z + ad d + ag g + gam psi   (gam complex), per instance; the
+// coefficients come from the launch arguments or the device table tab
 __global__ __launch_bounds__(BEC_WG) void k_combine(double2* d, const double2* __restrict__ z,
                                                     const double2* __restrict__ g,
-                                                    const double2* __restrict__ psi, CombArgs c, long long n,
-                                                    unsigned long long act) {
+                                                    const double2* __restrict__ psi, CombArgs c,
+                                                    const double* __restrict__ tab, long long n, Act act) {
   const int b = blockIdx.y;
-  if (!inst_on(act, b)) return;
+  if (!act_on(act, b)) return;
   const long long off = (long long)b * n;
-  const double az = c.az[b], ad = c.ad[b], ag = c.ag[b], gre = c.gre[b], gim = c.gim[b];
+  double az = c.az[b], ad = c.ad[b], ag = c.ag[b], gre = c.gre[b], gim = c.gim[b];
+  if (tab) {
+    const double* t = tab + (long long)b * DS_N;
+    az = t[DS_AZ]; ad = t[DS_AD]; ag = t[DS_AG]; gre = t[DS_GRE]; gim = t[DS_GIM];
+  }
+  if (!g) ag = 0.0;
   ELEM_LOOP(n) {
     const long long j = off + i;
     const double2 p = psi[j];
@@ -435,9 +470,9 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
                                                  const double2* __restrict__ g,
                                                  const double2* __restrict__ yv,
                                                  const double2* __restrict__ sv, long long n,
-                                                 unsigned long long act, double* hpart) {
+                                                 Act act, double* partials, double* hpart) {
   const int b = blockIdx.y;
-  if (!inst_on(act, b)) return;
+  if (!act_on(act, b)) return;
   const long long off = (long long)b * n;
   double v[16];
 #pragma unroll
@@ -458,7 +493,7 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
     v[14] += s.x * y.x + s.y * y.y;
     v[15] += cabs2(s);
   }
-  block_partials<16, 0>(v, nullptr, nullptr, hpart, (long long)b * gridDim.x + blockIdx.x);
+  block_partials<16, 0>(v, nullptr, partials, hpart, (long long)b * gridDim.x + blockIdx.x);
 }
 
 // g = h - mu psi, and sum |g|^2 (one instance)
@@ -510,12 +545,11 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restric
                                                         const double* mrows, int nrows, int mkp, int mq,
                                                         const InstPhys* iphys, int ndim, int N, double dx,
                                                         double tc, int s_mode, double s_half, long long n,
-                                                        unsigned long long act, double* loc,
-                                                        unsigned long long* part) {
+                                                        Act act, double* loc, unsigned long long* part) {
   __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64], scnt[BEC_WG / 64];
   __shared__ double smx[BEC_WG / 64];
   const int b = blockIdx.y;
-  if (!inst_on(act, b)) return;
+  if (!act_on(act, b)) return;
   const long long off = (long long)b * n;
   // max rho of the state: fold the per-block maxima of the pass that computed it
   double rm = 0.0;
@@ -610,11 +644,10 @@ __device__ __forceinline__ void sigma_fold(const unsigned long long* part, int n
 
 __global__ __launch_bounds__(BEC_WG) void k_sigma_hist(const double* __restrict__ loc, long long n,
                                                        const unsigned long long* part, int nb1,
-                                                       SigmaStats* stats, unsigned* hists,
-                                                       unsigned long long act) {
+                                                       SigmaStats* stats, unsigned* hists, Act act) {
   __shared__ unsigned h[SIGMA_BINS];
   const int b = blockIdx.y;
-  if (!inst_on(act, b)) return;
+  if (!act_on(act, b)) return;
   const long long off = (long long)b * n;
   unsigned* hist = hists + (long long)b * SIGMA_BINS;
   unsigned long long kmn, kmx, cnt;
@@ -637,12 +670,12 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_hist(const double* __restrict_
 // (parallel prefix over the histogram), then gathers that bin's members
 __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restrict__ loc, long long n,
                                                          SigmaStats* stats, const unsigned* hists, double* cands,
-                                                         unsigned long long act) {
+                                                         Act act) {
   __shared__ unsigned pre[BEC_WG];
   __shared__ int sbin;
   __shared__ unsigned srank;
   const int b = blockIdx.y;
-  if (!inst_on(act, b)) return;
+  if (!act_on(act, b)) return;
   SigmaStats* st = stats + b;
   const unsigned* hist = hists + (long long)b * SIGMA_BINS;
   double* cand = cands + (long long)b * n;
@@ -692,14 +725,13 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
 // max(median, floor), then reset the histogram and statistics
 __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* stats, unsigned* hists,
                                                          const double* __restrict__ cands, long long n,
-                                                         double floor_, double* sigmas, double* hred,
-                                                         unsigned long long act) {
+                                                         double floor_, double* sigmas, double* hred, Act act) {
   __shared__ double sv[SIGMA_LDS_CAND];
   __shared__ unsigned dh[256];
   __shared__ unsigned long long prefix;
   __shared__ unsigned rank_s;
   const int b = blockIdx.x;
-  if (!inst_on(act, b)) return;
+  if (!act_on(act, b)) return;
   SigmaStats* st = stats + b;
   unsigned* hist = hists + (long long)b * SIGMA_BINS;
   const double* cand = cands + (long long)b * n;
@@ -874,14 +906,19 @@ LineCfg line_cfg(int L) {
 // BEC_WG threads, a power of two dividing the lines of one instance when
 // possible (so that a reducing workgroup never spans two instances), aiming
 // at ~1024 workgroups so that small problems still spread over all 256 CUs.
-static int pick_lpw(int tpl, int nl, int total) {
+// A strided pass takes at least 4 adjacent columns per workgroup, so that
+// every row segment it touches is 64 contiguous bytes.
+static int pick_lpw(int tpl, int nl, int total, bool strided) {
   const int lmin = (tpl >= 64) ? 1 : 64 / tpl;
   const int lmax = BEC_WG / tpl > lmin ? BEC_WG / tpl : lmin;
   const int want = total / 1024;
   int l = lmin;
+  if (strided)
+    while (l < 4 && l * 2 <= lmax && nl % (l * 2) == 0) l *= 2;
   while (l * 2 <= lmax && l * 2 <= want && nl % (l * 2) == 0) l *= 2;
   return l;
 }
+static unsigned long long lowmask(int nb) { return nb >= 64 ? ~0ull : ((1ull << nb) - 1ull); }
 // workgroup count rounded up to a whole number of XCD rounds (see xcd_block)
 static int xcd_grid(int nlines, int lpw) {
   const int b = (nlines + lpw - 1) / lpw;
@@ -892,12 +929,13 @@ template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 static int go_pass(const PassArgs& a0, hipStream_t s) {
   PassArgs a = a0;
   const int total = a.nb * a.nl;
-  a.lpw = pick_lpw(a.tpl, a.nl, total);
+  a.lpw = pick_lpw(a.tpl, a.nl, total, !CONTIG);
   const int blocks = xcd_grid(total, a.lpw);
-  const size_t shm = (size_t)a.lpw * a.fp.L * sizeof(double2);
+  const size_t shm = (size_t)(a.lpw + 1) * a.fp.L * sizeof(double2);
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (prof_events(id, &e0, &e1))
+  if (a.act.fl) a.ucount = prof_counter(id);
+  if (prof_events(id, &e0, &e1, a.act.fl ? 0 : __builtin_popcountll(a.act.m & lowmask(a.nb))))
     hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1,
                           0, a);
   else
@@ -932,12 +970,13 @@ template <int AX, int KIND>
 static int go_kin(const KinArgs& a0, hipStream_t s) {
   KinArgs a = a0;
   const int total = a.nb * a.nl;
-  a.lpw = pick_lpw(a.tpl, a.nl, total);
+  a.lpw = pick_lpw(a.tpl, a.nl, total, !Ax<AX>::contig);
   const int blocks = xcd_grid(total, a.lpw);
-  const size_t shm = (size_t)a.lpw * a.N * sizeof(double2);
+  const size_t shm = (size_t)(a.lpw + 1) * a.N * sizeof(double2);
   const int id = 100 + 10 * AX + KIND;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (prof_events(id, &e0, &e1))
+  if (a.act.fl) a.ucount = prof_counter(id);
+  if (prof_events(id, &e0, &e1, a.act.fl ? 0 : __builtin_popcountll(a.act.m & lowmask(a.nb))))
     hipExtLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1, 0, a);
   else
     hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
@@ -978,20 +1017,27 @@ void launch_scale(const double2* in, double2* out, const double* divs, long long
 void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s) {
   hipLaunchKernelGGL(k_rho, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, rho, n);
 }
-void launch_copy(const double2* src, double2* dst, long long n, int nb, unsigned long long mask, hipStream_t s) {
+void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);
 }
+static int act_units(const Act& a, int nb) { return a.fl ? nb : __builtin_popcountll(a.m & lowmask(nb)); }
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, const CombArgs& c,
-                    long long n, int nb, unsigned long long act, hipStream_t s) {
-  prof_pre(SITE_COMBINE, s);
-  hipLaunchKernelGGL(k_combine, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, d, z, g, psi, c, n, act);
+                    const double* tab, long long n, int nb, Act act, hipStream_t s) {
+  prof_pre(SITE_COMBINE, s, act_units(act, nb));
+  hipLaunchKernelGGL(k_combine, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, d, z, g, psi, c, tab, n, act);
   prof_post(SITE_COMBINE, s);
 }
+int reduce_blocks(long long n) {
+  long long g = (n + BEC_WG - 1) / BEC_WG;
+  if (g > 256) g = 256;
+  if (g < 1) g = 1;
+  return (int)g;
+}
 int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g, const double2* y,
-                const double2* sv, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s) {
-  const int gr = elem_blocks(n);
-  prof_pre(SITE_DOTS, s);
-  hipLaunchKernelGGL(k_dots, dim3(gr, nb), dim3(BEC_WG), 0, s, z, psi, d, g, y, sv, n, act, hpart);
+                const double2* sv, long long n, int nb, Act act, double* partials, double* hpart, hipStream_t s) {
+  const int gr = reduce_blocks(n);
+  prof_pre(SITE_DOTS, s, act_units(act, nb));
+  hipLaunchKernelGGL(k_dots, dim3(gr, nb), dim3(BEC_WG), 0, s, z, psi, d, g, y, sv, n, act, partials, hpart);
   prof_post(SITE_DOTS, s);
   return gr;
 }
@@ -1008,7 +1054,7 @@ int launch_moments(const double2* psi, int ndim, int N, double dx, double* parti
   hipLaunchKernelGGL(k_moments, dim3(gr), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials, hpart);
   return gr;
 }
-void launch_sigma(const Plan& p, int nb, unsigned long long act, const double2* psi, const double* phi,
+void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
                   const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s) {
   long long g1 = (p.n + 4 * BEC_WG - 1) / (4 * BEC_WG);
   if (g1 > SIGMA_MAX_BLOCKS) g1 = SIGMA_MAX_BLOCKS;

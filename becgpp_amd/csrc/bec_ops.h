@@ -21,12 +21,33 @@ struct BecError : std::runtime_error {
 
 typedef InstPhys Phys;
 
-// the instances a driver call covers
+// the instances a driver call covers.  Host-driven calls give a bit mask;
+// the device-driven solver loop gives the flag words and the flag bit
+// instead, reads its per-instance scalars from the device table tab, and
+// keeps every reduction as device rows (dev) for its controller kernels.
 struct Batch {
   int nb;
   unsigned long long act;     // bit b set: instance b takes part
+  const unsigned* fl = nullptr;
+  unsigned bit = 0;
+  const double* tab = nullptr;
+  bool dev = false;
 };
 inline Batch one_instance() { return Batch{1, 1ull}; }
+inline Batch dev_batch(const Plan& p, int nb, unsigned bit) {
+  Batch B{nb, 0ull};
+  B.fl = p.flags;
+  B.bit = bit;
+  B.tab = p.dtab;
+  B.dev = true;
+  return B;
+}
+inline Act act_of(const Batch& B) { return Act{B.act, B.fl, B.bit}; }
+
+// rows per instance of the reducing launches of one driver call
+struct EnergyRows {
+  int r0, r1, r2;             // first pass (6 sums), second and third axis (2 sums each)
+};
 
 // energy components of (psi - t d) [/ its norm when normalise], one instance
 struct Energy {
@@ -45,11 +66,11 @@ inline const double* hres(const Plan& p, int b) { return p.hred + (long long)b *
 
 // out = scale * dV * (K * rho)_crop per instance; with dot, the sums
 // sum(out_unscaled... ) -> hred[b][R_EGRAV] use out as computed (scale included)
-void run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot);
+int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot);
 // kinetic/L_z/local sums of psi - t[b] d into hred[b][R_EFIRST..], [R_EREST..];
 // rho_out (optional) receives |psi - t d|^2
-void run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
-                     double* rho_out);
+EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
+                           double* rho_out);
 // assemble instance b's Energy from hred (after plan_sync); sgrav = sum(phi rho)
 // with phi the potential as stored (its scale is passed as gscale)
 Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool normalise, double gscale);
@@ -67,7 +88,7 @@ struct HArgs {
   double2* out; double2* ydiff;
   bool reduce;
 };
-void run_H(Plan& p, const Batch& B, const HArgs& h);
+int run_H(Plan& p, const Batch& B, const HArgs& h);   // returns rows per instance of the last pass
 // z = IFFT(FFT(g) / (k^2/2 + sigma[b])) with sigma read from p.sigma (device)
 void run_precond(Plan& p, const Batch& B, const double2* g, double2* out);
 

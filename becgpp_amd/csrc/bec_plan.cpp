@@ -186,7 +186,7 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, c
   a.tpl = lc.tpl;
   a.lpw = lc.lpw;
   a.nb = B.nb;
-  a.act = B.act;
+  a.act = act_of(B);
   a.ni = 1;
   a.out_scale = 1.0;
   a.H = p.H;
@@ -197,8 +197,8 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, c
   return a;
 }
 
-void run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out,
-              const double* dot) {
+int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out,
+             const double* dot) {
   if (!p.khat[which]) throw BecError(1, "kernel spectrum not built for this plan");
   const int N = p.N, M = p.M, H = p.H, i0 = (p.M - p.N) / 2;
   const double fac = scale * p.dv / (double)p.nM;
@@ -218,7 +218,8 @@ void run_conv(Plan& p, int which, const Batch& B, const double* rho, double scal
     PassArgs c3 = pass_base(p, p.fpM, p.lcM, B);
     c3.nl = N; c3.in_so = H; c3.in_s = 1; c3.in_ib = S1;
     c3.out_so = N; c3.out_s = 1; c3.out_ib = n; c3.n_store = N; c3.off_store = i0; c3.out_scale = fac;
-    c3.in = p.s1; c3.out = out; c3.dot = dot; c3.hpartials = dot ? hpart_of(p, C_EGRAV) : nullptr;
+    c3.in = p.s1; c3.out = out; c3.dot = dot;
+    if (dot) (B.dev ? c3.partials : c3.hpartials) = B.dev ? part_of(p, C_EGRAV) : hpart_of(p, C_EGRAV);
     nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
   } else {
     const long long NH = (long long)N * H, MH = (long long)M * H;
@@ -247,10 +248,12 @@ void run_conv(Plan& p, int which, const Batch& B, const double* rho, double scal
     PassArgs c5 = pass_base(p, p.fpM, p.lcM, B);
     c5.nl = N * N; c5.in_so = H; c5.in_s = 1; c5.in_ib = S1;
     c5.out_so = N; c5.out_s = 1; c5.out_ib = n; c5.n_store = N; c5.off_store = i0; c5.out_scale = fac;
-    c5.in = p.s1; c5.out = out; c5.dot = dot; c5.hpartials = dot ? hpart_of(p, C_EGRAV) : nullptr;
+    c5.in = p.s1; c5.out = out; c5.dot = dot;
+    if (dot) (B.dev ? c5.partials : c5.hpartials) = B.dev ? part_of(p, C_EGRAV) : hpart_of(p, C_EGRAV);
     nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
   }
-  if (dot) add_job(p, C_EGRAV, nb_rows, 1, 0, R_EGRAV, B);
+  if (dot && !B.dev) add_job(p, C_EGRAV, nb_rows, 1, 0, R_EGRAV, B);
+  return nb_rows;
 }
 
 static KinArgs kin_base(const Plan& p, const Batch& B) {
@@ -262,7 +265,7 @@ static KinArgs kin_base(const Plan& p, const Batch& B) {
   a.N = p.N;
   a.nl = (int)(p.n / p.N);
   a.nb = B.nb;
-  a.act = B.act;
+  a.act = act_of(B);
   a.istride = p.n;
   a.dx = p.dx;
   a.invNdx = p.invNdx;
@@ -276,22 +279,31 @@ static KinArgs kin_base(const Plan& p, const Batch& B) {
 
 static const int kAxes2[2] = {A2Y, A2X}, kAxes3[3] = {A3Z, A3Y, A3X};
 
-void run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
-                     double* rho_out) {
+EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
+                           double* rho_out) {
   const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
+  EnergyRows er{0, 0, 0};
   for (int k = 0; k < p.ndim; ++k) {
     KinArgs a = kin_base(p, B);
     a.psi = psi;
     a.d = d;
     if (t)
       for (int b = 0; b < B.nb; ++b) a.t[b] = t[b];
+    if (B.dev && d) {
+      a.tab = B.tab;
+      a.tabm = 1u << DS_T;
+    }
     const int site = (k == 0) ? C_EFIRST : (k == 1 ? C_EREST0 : C_EREST1);
-    a.hpartials = hpart_of(p, site);
+    if (B.dev) a.partials = part_of(p, site);
+    else a.hpartials = hpart_of(p, site);
     if (k == 0) a.rho_out = rho_out;
     const int rows = launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
+    (k == 0 ? er.r0 : (k == 1 ? er.r1 : er.r2)) = rows;
+    if (B.dev) continue;
     if (k == 0) add_job(p, C_EFIRST, rows, 6, 0, R_EFIRST, B);
     else add_job(p, site, rows, 2, 0, R_EREST + 2 * (k - 1), B);
   }
+  return er;
 }
 
 Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool normalise, double gscale) {
@@ -317,7 +329,8 @@ Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool norm
   return e;
 }
 
-void run_H(Plan& p, const Batch& B, const HArgs& h) {
+int run_H(Plan& p, const Batch& B, const HArgs& h) {
+  int last_rows = 0;
   const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
   const bool transformed = h.d || h.div;
   if (transformed && !h.psi_w) throw BecError(2, "internal: a transformed state must be written back");
@@ -338,6 +351,10 @@ void run_H(Plan& p, const Batch& B, const HArgs& h) {
       a.psi_w = h.psi_w;
       a.sdiff = h.sdiff;
       a.out = last ? h.out : p.w1;
+      if (B.dev) {
+        a.tab = B.tab;
+        a.tabm = (h.d ? 1u << DS_T : 0u) | (h.div ? 1u << DS_SC : 0u);
+      }
     } else {
       a.psi = psi_rest;
       a.acc = p.w1;
@@ -354,15 +371,21 @@ void run_H(Plan& p, const Batch& B, const HArgs& h) {
       }
       a.ydiff = h.ydiff;
       a.partials = h.reduce ? part_of(p, C_HLAST) : nullptr;
-      a.hpartials = h.reduce ? hpart_of(p, C_HLAST) : nullptr;
+      a.hpartials = (h.reduce && !B.dev) ? hpart_of(p, C_HLAST) : nullptr;
+      if (B.dev) {
+        a.tab = B.tab;
+        a.tabm |= (h.mu ? 0u : 1u << DS_MU) | (h.phisc ? 0u : 1u << DS_PHISC);
+      }
     }
     const int kind = (k == 0) ? KH_FIRST : (last ? KH_LAST : KH_MID);
     const int rows = launch_kin(ax[k], kind, a, p.stream);
+    if (last) last_rows = rows;
     if (last && h.reduce) {
-      add_job(p, C_HLAST, rows, 2, 1, R_HLAST, B);
+      if (!B.dev) add_job(p, C_HLAST, rows, 2, 1, R_HLAST, B);
       p.hlast_rows = rows;
     }
   }
+  return last_rows;
 }
 
 void run_precond(Plan& p, const Batch& B, const double2* g, double2* out) {
@@ -740,11 +763,13 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
     const int nbm = launch_moments((const double2*)psi, p.ndim, p.N, p.dx, part_of(p, C_MOM), hpart_of(p, C_MOM),
                                    p.stream);
     add_job(p, C_MOM, nbm, 4, 1, R_MOM, B);
-    launch_sigma(p, 1, 1ull, (const double2*)psi, phiu, part_of(p, C_MOM), nbm, 8, 4, shift_min, p.stream);
+    launch_sigma(p, 1, mask_act(1ull), (const double2*)psi, phiu, part_of(p, C_MOM), nbm, 8, 4, shift_min,
+                 p.stream);
     double2* z = (double2*)out;
     run_precond(p, B, (const double2*)g, z);
     // projection onto the tangent space: z - psi <psi, z> dv
-    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, z, p.n, 1, 1ull, hpart_of(p, C_DOTS), p.stream),
+    add_job(p, C_DOTS, launch_dots(z, (const double2*)psi, z, z, z, z, p.n, 1, mask_act(1ull), nullptr,
+                                   hpart_of(p, C_DOTS), p.stream),
             16, 0, R_DOTS, B);
     plan_sync(p);
     const double* r = hres(p, 0);
@@ -753,7 +778,7 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
     cb.az[0] = 1.0;
     cb.gre[0] = -r[R_DOTS + 0] * p.dv;
     cb.gim[0] = -r[R_DOTS + 1] * p.dv;
-    launch_combine(z, z, nullptr, (const double2*)psi, cb, p.n, 1, 1ull, p.stream);
+    launch_combine(z, z, nullptr, (const double2*)psi, cb, nullptr, p.n, 1, mask_act(1ull), p.stream);
     HIPCHK(hipGetLastError());
     plan_sync(p);
     if (sigma_out) *sigma_out = hres(p, 0)[R_SIGMA];

@@ -180,7 +180,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
     I[b].resid_rel = sqrt(hres(p, b)[R_HLAST] * dv) / std::max(1.0, fabs(I[b].mu));
     I[b].best_resid = I[b].resid_rel;
   }
-  launch_copy(PSI, BEST, n, nb, all, s);
+  launch_copy(PSI, BEST, n, nb, mask_act(all), s);
 
   int it = 0;
   while (true) {
@@ -189,10 +189,11 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
       if (!I[b].done && it < maxit) run |= 1ull << b;
     if (run) {
       // sigma, z = _preconditioner(grad, psi, phi)                      (solvers.py:129)
-      launch_sigma(p, nb, run, PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2, floor_, s);
+      launch_sigma(p, nb, mask_act(run), PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2,
+                   floor_, s);
       run_precond(p, mask_batch(nb, run), G, Z);
-      add_job(p, C_DOTS, launch_dots(Z, PSI, D, G, Y, SD, n, nb, run, hpart_of(p, C_DOTS), s), 16, 0, R_DOTS,
-              mask_batch(nb, run));
+      add_job(p, C_DOTS, launch_dots(Z, PSI, D, G, Y, SD, n, nb, mask_act(run), nullptr, hpart_of(p, C_DOTS), s),
+              16, 0, R_DOTS, mask_batch(nb, run));
       HIPCHK(hipGetLastError());
     }
     plan_sync(p);
@@ -233,7 +234,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
         q.it_ref = q.it_acc;
       }
     }
-    if (bestmask) launch_copy(PSI, BEST, n, nb, bestmask, s);
+    if (bestmask) launch_copy(PSI, BEST, n, nb, mask_act(bestmask), s);
     unsigned long long act = 0;
     for (int b = 0; b < nb; ++b) {
       if (I[b].done) continue;
@@ -290,6 +291,9 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
         }
       }
       q.alpha = std::min(std::max(q.alpha, alpha_min), alpha_max);
+      if (cf->verbose >= 2)
+        printf("  [gs%d] it=%d E=%.15e gd=%.3e beta=%.3e alpha=%.3e zg=%.3e zy=%.3e gg=%.3e sy=%.3e ss=%.3e pg=%.2e\n",
+               b, it, q.E, gd, beta, q.alpha, zg, zy, gg, sy, ss, pg.re);
       q.gd = gd;
       q.trial = q.alpha;
       q.ls = 0;
@@ -299,7 +303,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
       q.gg_cur = gg;
       cb.az[b] = az; cb.ad[b] = ad; cb.ag[b] = ag; cb.gre[b] = gam.re; cb.gim[b] = gam.im;
     }
-    launch_combine(D, Z, G, PSI, cb, n, nb, act, s);
+    launch_combine(D, Z, G, PSI, cb, nullptr, n, nb, mask_act(act), s);
     // Armijo backtracking, one round per trial for all searching instances (solvers.py:160-174)
     unsigned long long T = act, acc = 0, fail = 0;
     while (T) {
@@ -319,6 +323,9 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
           T &= ~(1ull << b);
           continue;
         }
+        if (cf->verbose >= 2)
+          printf("  [gs%d] it=%d reject trial=%.3e Et-E=%.3e need=%.3e\n", b, it, q.trial, q.et.E - q.E,
+                 -c1 * q.trial * q.gd);
         q.trial *= shrink;
         q.ls += 1;
         if (q.trial < alpha_min || q.ls >= ls_max) {
@@ -346,7 +353,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
                    q.best_resid);
         }
       }
-      launch_combine(D, Z, nullptr, PSI, cf2, n, nb, fail, s);
+      launch_combine(D, Z, nullptr, PSI, cf2, nullptr, n, nb, mask_act(fail), s);
     }
     if (acc) {                                                     // (solvers.py:186-196)
       for (int b = 0; b < nb; ++b) {
@@ -395,8 +402,8 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   unsigned long long from_best = 0;
   for (int b = 0; b < nb; ++b)
     if (I[b].best_resid < I[b].resid_rel) from_best |= 1ull << b;
-  launch_copy(BEST, psi_io, n, nb, from_best, s);
-  launch_copy(PSI, psi_io, n, nb, all & ~from_best, s);
+  launch_copy(BEST, psi_io, n, nb, mask_act(from_best), s);
+  launch_copy(PSI, psi_io, n, nb, mask_act(all & ~from_best), s);
   plan_sync(p);
   const double wall = now_s() - t0;
   for (int b = 0; b < nb; ++b) {

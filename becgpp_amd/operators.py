@@ -29,8 +29,12 @@ def _plan(G, gc):
     return ensure_kernel(G, "K") if _lr_on(G, gc) else plan_of(G)
 
 
-def _phys(b2, b3, O, gc):
-    return _lib.phys(b2, b3, O, gc)
+def phys_for(G, b2, b3, O, gc):
+    """Physics record for the C-ABI.  The long-range term is on only when the
+    grid's kernel is not "none" and |G_C| > 1e-15 (operators.py:21-22,
+    solvers.py:36): otherwise G_C goes down as 0, so a spectrum that a plan of
+    the same geometry still holds from another grid is never applied."""
+    return _lib.phys(b2, b3, O, gc if _lr_on(G, gc) else 0.0)
 
 
 def energy_components(p, G, beta2, beta3, Omega, G_C):
@@ -39,7 +43,7 @@ def energy_components(p, G, beta2, beta3, Omega, G_C):
     t = as_field(p, G)
     phi = torch.empty(t.shape, dtype=torch.float64, device=DEV)
     comps = (ctypes.c_double * 8)()
-    ph = _phys(beta2, beta3, Omega, G_C)
+    ph = phys_for(G, beta2, beta3, Omega, G_C)
     _lib.check(pl.lib.bec_energy(pl.h, _ptr(t), ctypes.byref(ph), comps, _ptr(phi)))
     out = {k: torch.tensor(comps[i], dtype=torch.float64, device=DEV) for i, k in enumerate(_COMP_KEYS)}
     out["phi"] = phi
@@ -56,7 +60,7 @@ def apply_H(psi, G, cfg, phi=None):
     pl = _plan(G, gc)
     t = as_field(psi, G)
     out = torch.empty_like(t)
-    ph = _phys(b2, b3, O, gc)
+    ph = phys_for(G, b2, b3, O, gc)
     if phi is not None and not (isinstance(phi, (int, float)) and phi == 0):
         f = as_field(phi, G, torch.float64)
         _lib.check(pl.lib.bec_apply_H(pl.h, _ptr(t), ctypes.byref(ph), _ptr(f), _ptr(out)))
@@ -71,7 +75,7 @@ def compute_residual(psi, G, cfg):
     pl = _plan(G, gc)
     t = as_field(psi, G)
     out = (ctypes.c_double * 3)()
-    ph = _phys(cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
+    ph = phys_for(G, cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
     _lib.check(pl.lib.bec_residual(pl.h, _ptr(t), ctypes.byref(ph), out))
     return out[0], out[1], out[2]
 
@@ -134,7 +138,7 @@ def preconditioner(grad, psi, G, cfg, phi=None):
     ps = as_field(psi, G)
     out = torch.empty_like(g)
     sig = ctypes.c_double()
-    ph = _phys(cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
+    ph = phys_for(G, cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
     f = as_field(phi, G, torch.float64) if (phi is not None and _lr_on(G, gc)) else None
     _lib.check(pl.lib.bec_precondition(pl.h, _ptr(g), _ptr(ps), ctypes.byref(ph),
                                        _ptr(f) if f is not None else None,

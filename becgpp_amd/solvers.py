@@ -13,7 +13,7 @@ import torch
 from . import _lib
 from .grid import _ptr, as_field, make_grid, plan_of
 from .interactions import ensure_kernel
-from .operators import compute_residual, observables
+from .operators import compute_residual, observables, phys_for
 from .seeds import make_seed
 
 BATCH_MAX = 32
@@ -44,7 +44,7 @@ def solve_native(cfg, G, psi0, verbose=False, record=False):
     pl = ensure_kernel(G, "K") if lr else plan_of(G)
     psi = as_field(psi0, G).clone()
     sc = _solver_cfg(cfg, verbose)
-    ph = _lib.phys(cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
+    ph = phys_for(G, cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
     res = _lib.SolveOut()
     cap = int(cfg["maxit"]) + 2 if record else 0
     tr = (ctypes.c_double * (4 * cap))() if cap else None
@@ -83,7 +83,7 @@ def solve_native_batch(cfgs, G, psi0s, verbose=False):
     pl = ensure_kernel(G, "K") if lr_any else plan_of(G)
     psi = torch.stack([as_field(p0, G) for p0 in psi0s]).contiguous()
     sc = _solver_cfg(cfgs[0], verbose)
-    ph = (_lib.Phys * nb)(*[_lib.phys(c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cfgs])
+    ph = (_lib.Phys * nb)(*[phys_for(G, c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cfgs])
     res = (_lib.SolveOut * nb)()
     _lib.check(pl.lib.bec_ground_state_batch(pl.h, nb, ph, ctypes.byref(sc), _ptr(psi), res))
     return [psi[b] for b in range(nb)], [res[b] for b in range(nb)]

@@ -1,15 +1,26 @@
 """Benchmark of the ground-state solve path on one MI355X.
 
 Workload (default): 2D rotating vortex lattice -- s=2, Omega=0.9, beta2=200,
-G_C=0, L=12, Ngrid=384 (3*2^7), complex128, res_tol=1e-4, nseeds=3 with the
+G_C=0, L=12, Ngrid=384 (3*2^7), complex128, res_tol=1e-6, nseeds=3 with the
 reference's seed cycle (triangular, tf, gaussian; rng 0,1,2).  One step is one
 multi-seed ground-state solve: the three solves from the pre-generated seeds
-(inputs resident in HBM before timing) and the lowest-energy selection.
+(inputs resident in HBM before timing) run as one batch through the batched
+device loop, then the final residual/observables of each and the
+lowest-energy selection.  res_tol=1e-6 is the tolerance at which the
+reference's own lattice runs reach |dE/E| ~ 1e-9 (its pcg_variants results);
+the line reports the selected energy's distance to the same state polished
+to res_tol=1e-9.  linesearch_max=20 (the reference's default is 8): with 8
+Armijo halvings the solver stops as "line_search_stalled" near res 2e-4 on
+this workload, on the GPU and in the CPU oracle alike, when a stiff state
+needs a step below alpha/256.
 
-Prints one JSON line: value = selected ground states per second; also CG
-iterations/s, the roofline of the dominant kernel (achieved algorithmic GB/s
-from HIP events on the launch stream, over the timed region) and the CPU
-oracle timed on a bounded sample of the same workload.
+Prints one JSON line: value = selected ground states per second; also solves
+and CG iterations per second, the same workload through the single-solve path
+(one solve after the other, measured once after the timed region), the
+roofline of the dominant kernel (achieved algorithmic GB/s from HIP events on
+the launch stream over the timed region, bytes counted per instance the
+launch covered) and the CPU oracle timed on a bounded sample of the same
+workload.
 """
 import argparse
 import ctypes
@@ -27,8 +38,9 @@ PEAK_HBM_GBS = 8000.0
 WORKLOADS = {
     "rotating_lattice_2d_n384": dict(
         cfg=dict(mode="single", dimension="2D", s=2, Omega=0.9, beta2=200.0, beta3=0.0, G_C=0.0,
-                 kernel="none", L=12.0, Ngrid=384, nseeds=3, res_tol=1e-4, maxit=60000),
-        desc="2D rotating vortex lattice, Omega=0.9, beta2=200, L=12, N=384, nseeds=3, res_tol=1e-4"),
+                 kernel="none", L=12.0, Ngrid=384, nseeds=3, res_tol=1e-6, maxit=60000, linesearch_max=20),
+        desc="2D rotating vortex lattice, Omega=0.9, beta2=200, L=12, N=384, nseeds=3, res_tol=1e-6, "
+             "linesearch_max=20"),
     "harmonic_2d_n128": dict(
         cfg=dict(mode="single", dimension="2D", s=2, Omega=0.0, beta2=100.0, beta3=0.0, G_C=0.0,
                  kernel="none", L=16.0, Ngrid=128, nseeds=1, res_tol=1e-4, maxit=60000),
@@ -59,6 +71,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--maxit", type=int, default=None,
                     help="cap the CG iterations per solve (bounded profiling runs only)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-solve comparison run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
@@ -107,7 +121,18 @@ def main():
         runs.append((c, make_seed(G, c).contiguous()))
     torch.cuda.synchronize()
 
+    cs = [c for c, _ in runs]
+    psi0s = [p for _, p in runs]
+
     def step():
+        res = B.ground_state_batch(cs, G=G, psi0s=psi0s)
+        Es = [obs["E"] for _, _, obs in res]
+        psi, _, best = min(res, key=lambda r: r[2]["E"])
+        best = dict(best, psi=psi)
+        stops = [f'{obs["stop_reason"]}@{obs["iters"]}' for _, _, obs in res]
+        return best, sum(obs["iters"] for _, _, obs in res), Es, stops
+
+    def step_single():
         best, iters, Es = None, 0, []
         for c, psi0 in runs:
             _, _, obs = B.ground_state(c, G=G, psi0=psi0, verbose=False)
@@ -126,35 +151,53 @@ def main():
     tot_iters = 0
     results = []
     for _ in range(a.steps):
-        best, iters, Es = step()
+        best, iters, Es, stops = step()
         tot_iters += iters
-        results.append((best, Es))
+        results.append((best, Es, stops))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = ctypes.c_double()
     cnt = ctypes.c_longlong()
-    lib.bec_prof_read(ctypes.byref(ms), ctypes.byref(cnt))
+    units = ctypes.c_longlong()
+    lib.bec_prof_read(ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(units))
     lib.bec_prof_watch(0)
     wall = t1 - t0
     kname, kbytes = KERNELS.get(a.watch, (f"site {a.watch}", lambda n: 32 * n))
     avg_s = (ms.value / 1e3) / max(1, cnt.value)
-    achieved = kbytes(n) / avg_s / 1e9 if cnt.value else None
-    best, Es = results[-1]
+    upl = units.value / max(1, cnt.value)              # instances per timed launch
+    achieved = kbytes(n) * upl / avg_s / 1e9 if cnt.value else None
+    best, Es, _ = results[-1]
     out = dict(
         metric="ground states/sec at |dE/E|<=1e-8; CG iters/sec; HBM GB/s vs peak",
         value=a.steps / wall, unit="ground states/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
         ms_per_step=1e3 * wall / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64", data="synthetic",
         config=dict(workload=a.workload, desc=wl["desc"], Ngrid=int(G["N"]), ndim=int(G["ndim"]),
-                    nseeds=nseeds, res_tol=float(cfg["res_tol"])),
+                    nseeds=nseeds, res_tol=float(cfg["res_tol"]), path="batched"),
         solves_per_s=a.steps * nseeds / wall, cg_iters_per_s=tot_iters / wall,
         iters_per_solve=tot_iters / (a.steps * nseeds), E_selected=best["E"], seed_energies=Es,
+        stop_reasons=results[-1][2],
         converged=bool(best["converged"]), resid_rel=best["resid_rel"],
         roofline=dict(bound="hbm", kernel=kname, site=a.watch, launches=cnt.value,
-                      avg_launch_us=avg_s * 1e6, achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
-                      frac=(achieved / PEAK_HBM_GBS) if achieved else None, traffic=None,
-                      algorithmic_bytes_per_launch=kbytes(n)),
+                      instances_per_launch=upl, avg_launch_us=avg_s * 1e6, achieved=achieved,
+                      peak=PEAK_HBM_GBS, unit="GB/s", frac=(achieved / PEAK_HBM_GBS) if achieved else None,
+                      traffic=None, algorithmic_bytes_per_launch=kbytes(n) * upl,
+                      algorithmic_bytes_per_instance=kbytes(n)),
     )
+    # |dE/E| of the selected state: polish it to the FP64 residual floor
+    pc = dict(cs[Es.index(best["E"])], res_tol=1e-9, maxit=20000)
+    _, _, pol = B.ground_state(pc, G=G, psi0=best["psi"], verbose=False)
+    out["E_polished"] = pol["E"]
+    out["dE_rel_vs_polished"] = abs(best["E"] - pol["E"]) / abs(pol["E"])
+    if not a.no_single:
+        t0 = time.perf_counter()
+        sb, si, sE = step_single()
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        out["single_solve_path"] = dict(ground_states_per_s=1.0 / ts, solves_per_s=nseeds / ts,
+                                        cg_iters_per_s=si / ts, iters_per_solve=si / nseeds,
+                                        E_selected=sb["E"], seed_energies=sE)
+        out["batched_speedup"] = out["value"] * ts
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, runs[0][1].cpu().numpy(), a.cpu_sample_iters,
                                            out["iters_per_solve"], nseeds)

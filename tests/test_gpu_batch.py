@@ -98,7 +98,7 @@ def test_instances_stop_independently():
 
 
 def test_batch_3d():
-    base = cfg(dimension="3D", Ngrid=24, L=5.0, kernel="newton", res_tol=1e-8)
+    base = cfg(dimension="3D", Ngrid=24, L=5.0, kernel="newton", G_C=3.0, res_tol=1e-8)
     cs = [dict(base, beta2=20.0, G_C=3.0), dict(base, beta2=20.0, beta3=2.0, G_C=3.0, Omega=0.5)]
     Go = O.make_grid(base)
     psi0s = [O.noisy_gaussian(Go, 1e-3, seed=21 + i) for i in range(2)]

@@ -164,3 +164,24 @@ def test_norm_of():
     psi, _ = random_state(Go, 4)
     assert abs(B.norm_of(_dev(psi), Gd).item() - O.norm_of(psi, Go)) < 1e-14 * O.norm_of(psi, Go)
     _ = math
+
+
+def test_kernel_none_grid_ignores_G_C():
+    """A grid whose kernel resolves to "none" never applies a long-range term,
+    even when a grid of the same geometry loaded a spectrum into the shared
+    plan before (the reference decides from G["kernel"], operators.py:21-22)."""
+    import becgpp_amd as B
+    c_lr = cfg(G_C=3.0, kernel="newton", beta2=30.0)
+    c_none = cfg(G_C=0.0, kernel="newton", beta2=30.0)
+    G_lr, G_none = B.make_grid(c_lr), B.make_grid(c_none)
+    assert G_lr["kernel"] == "newton" and G_none["kernel"] == "none"
+    Go = O.make_grid(c_none)
+    psi, _ = random_state(Go, 6)
+    B.energy_components(_dev(psi), G_lr, 30.0, 0.0, 0.0, 3.0)
+    e_gc = B.energy_components(_dev(psi), G_none, 30.0, 0.0, 0.0, 3.0)
+    e_0 = B.energy_components(_dev(psi), G_none, 30.0, 0.0, 0.0, 0.0)
+    eo = O.energy_components(psi, Go, 30.0, 0.0, 0.0, 3.0)
+    assert e_gc["E"].item() == e_0["E"].item()
+    assert abs(e_gc["E"].item() - eo["E"]) <= TOL * abs(eo["E"])
+    h = B.apply_H(_dev(psi), G_none, dict(c_none, G_C=3.0)).cpu().numpy()
+    assert relerr(h, O.apply_H(psi, Go, dict(c_none, G_C=3.0))) < TOL
