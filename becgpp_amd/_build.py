@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbecgpp_hip.so")
-SOURCES = ["bec_kernels.hip", "bec_plan.cpp", "bec_solver.cpp", "bec_prof.cpp"]
+SOURCES = ["bec_kernels.hip", "bec_ctrl.hip", "bec_plan.cpp", "bec_solver.cpp", "bec_prof.cpp"]
 ARCH = os.environ.get("BECGPP_ARCH", "gfx950")
 
 FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17",

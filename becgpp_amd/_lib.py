@@ -28,7 +28,8 @@ class SolverCfg(ctypes.Structure):
                 ("step", ctypes.c_double), ("step_max", ctypes.c_double),
                 ("linesearch_max", ctypes.c_int), ("linesearch_stall", ctypes.c_int),
                 ("precond_shift_min", ctypes.c_double), ("cg_beta", ctypes.c_int),
-                ("cg_restart", ctypes.c_int), ("verbose", ctypes.c_int), ("check", ctypes.c_int)]
+                ("cg_restart", ctypes.c_int), ("verbose", ctypes.c_int), ("check", ctypes.c_int),
+                ("host_loop", ctypes.c_int)]
 
 
 class SolveOut(ctypes.Structure):

@@ -1,0 +1,268 @@
+// Controller kernels of the device-driven solver loop (becgpp/solvers.py:48-226).
+//
+// One step of the loop is a fixed launch sequence (shift selection,
+// preconditioner, inner products, k_ctrl_dir, best copy, direction, trial
+// energy, k_ctrl_armijo, failure direction, H of the accepted state).  Every
+// launch covers the instances whose flag word carries its bit, and the two
+// controllers rewrite the flags and the per-instance scalar table between
+// them, so the host never waits inside a step.  An instance advances by one
+// CG iteration per step when its first Armijo trial is accepted and retries
+// its trial in the following steps otherwise, exactly as the reference's
+// inner loop does.
+//
+// Each controller runs one workgroup per instance: it folds that instance's
+// partial rows in a fixed order (deterministic), then thread 0 applies the
+// reference's scalar logic to the instance's state.
+#include <math.h>
+
+#include "../../include/becgpp_hip.h"
+#include "bec_internal.h"
+
+#define CTRL_WG 256
+
+// Sums (q < K) or maxima (q >= K) of the R rows of KP values at rows[],
+// folded as: lane j of value q takes rows j, j + J, ... in order, then a fixed
+// pairwise tree over the J lanes.  out[q] for q < KP; every thread calls it.
+template <int KP>
+__device__ void fold_rows(const double* __restrict__ rows, int R, int K, double* out, double* sh) {
+  constexpr int J = CTRL_WG / KP;
+  const int q = threadIdx.x % KP, j = threadIdx.x / KP;
+  const bool is_sum = q < K;
+  double acc = is_sum ? 0.0 : -INFINITY;
+  for (int r = j; r < R; r += J) {
+    const double x = rows[(long long)r * KP + q];
+    acc = is_sum ? acc + x : fmax(acc, x);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = J / 2; h > 0; h >>= 1) {
+    if (j < h) {
+      const double y = sh[threadIdx.x + h * KP];
+      sh[threadIdx.x] = is_sum ? sh[threadIdx.x] + y : fmax(sh[threadIdx.x], y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < KP) out[threadIdx.x] = sh[threadIdx.x];
+  __syncthreads();
+}
+
+__device__ __forceinline__ double clk_now() { return (double)wall_clock64(); }
+
+// ---------------------------------------------------------------------------
+// k_ctrl_dir: residual bookkeeping of the step accepted last (solvers.py:
+// 187-221), stop rules, then the CG direction, Barzilai-Borwein trial step
+// and Armijo start (solvers.py:128-159) from this step's inner products.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
+  __shared__ double sh[CTRL_WG];
+  __shared__ double r[16], hl[4];
+  const int b = blockIdx.x;
+  DevInst* q = c.st + b;
+  const unsigned f = c.fl[b];
+  const bool dir = (f & FL_DIR) != 0u;
+  if (dir) {
+    fold_rows<16>(c.r_dots + (long long)b * c.n_dots * 16, c.n_dots, 16, r, sh);
+    if (q->pending) fold_rows<4>(c.r_hl + (long long)b * c.n_hl * 4, c.n_hl, 2, hl, sh);
+  }
+  if (threadIdx.x != 0) return;
+  if (!dir) {                         // retrying a trial, or finished
+    c.fl[b] = (f & FL_SEARCH) ? (FL_SEARCH | (q->lr ? FL_SLR : 0u)) : 0u;
+    return;
+  }
+  const double dv = c.dv;
+  unsigned nf = 0u;
+  bool done = false;
+  if (q->clk0 == 0.0) q->clk0 = clk_now();
+  if (q->pending) {
+    q->pending = 0;
+    q->resid_rel = sqrt(hl[0] * dv) / fmax(1.0, fabs(q->mu));
+    if (c.trace && b == 0 && q->n_tr < c.trace_cap) {
+      double* t = c.trace + 4 * q->n_tr;
+      t[0] = (clk_now() - q->clk0) / c.clk_hz;
+      t[1] = q->it_acc + 1;
+      t[2] = q->E;
+      t[3] = q->resid_rel;
+      q->n_tr += 1;
+    }
+    if (q->resid_rel < q->best_resid) {
+      q->best_resid = q->resid_rel;
+      nf |= FL_BEST;
+    }
+    if (c.verbose && q->it_acc % c.check == 0)
+      printf("  [gs%s%d] it=%d E=%.8f res=%.2e best=%.2e step=%.2e\n", c.nb > 1 ? " " : "", c.nb > 1 ? b : 0,
+             q->it_acc, q->E, q->resid_rel, q->best_resid, q->trial_acc);
+    if (q->resid_rel < c.res_tol) {
+      q->converged = 1;
+      q->stop = BEC_STOP_CONVERGED;
+      done = true;
+    } else if (q->it_acc - q->it_ref >= c.conv_window) {
+      const double dE_win = fabs(q->E - q->E_ref) / fmax(1.0, fabs(q->E));
+      if (dE_win < c.etol) {
+        q->converged = 1;
+        q->stop = BEC_STOP_ENERGY_CONVERGED;
+        done = true;
+      } else {
+        q->E_ref = q->E;
+        q->it_ref = q->it_acc;
+      }
+    }
+  }
+  if (!done && q->it >= c.maxit) done = true;
+  if (done) {
+    q->done = 1;
+    c.hdone[b] = 1;
+    c.fl[b] = nf;                     // the best copy still happens this step
+    return;
+  }
+  const int it = q->it;
+  q->n_iter = it + 1;
+  q->sigma = c.sigma[b];
+  // <psi,z>, <psi,psi>, <psi,d>, <psi,g>, Re<z,g>, Re<z,y>, <psi,y>, Re<g,d>, <g,g>, Re<g,y>, Re<s,y>, <s,s>
+  const double czr = r[0] * dv, czi = r[1] * dv;
+  const double pp = r[2] * dv;
+  const double pdr = r[3] * dv, pdi = r[4] * dv;
+  const double pgr = r[5] * dv, pgi = r[6] * dv;
+  const double zg_raw = r[7] * dv, zy_raw = r[8] * dv;
+  const double pyr = r[9] * dv, pyi = r[10] * dv;
+  const double gdr = r[11] * dv, gg = r[12] * dv, gy = r[13] * dv;
+  const double sy = r[14] * dv, ss = r[15] * dv;
+  // Re(conj(a) b)
+  const double cz_pg = czr * pgr + czi * pgi;
+  const double cz_py = czr * pyr + czi * pyi;
+  const double zg = zg_raw - cz_pg;                  // Re <z_p, g>, z_p = z - psi <psi, z>
+  const double zy = zy_raw - cz_py;                  // Re <z_p, y>
+  if (it == 0) { q->zg_prev = zg; q->gg_prev = gg; }
+  double beta;
+  if (it == 0 || c.cg_beta == BEC_CG_NONE || (c.restart_period > 0 && it % c.restart_period == 0)) {
+    beta = 0.0;
+  } else {
+    if (c.cg_beta == BEC_CG_PR_PRECOND) beta = fmax(0.0, zy / fmax(q->zg_prev, 1e-300));
+    else beta = fmax(0.0, gy / fmax(q->gg_prev, 1e-300));
+    if (!isfinite(beta)) beta = 0.0;
+  }
+  // dirc = proj_tangent(z_p + beta d) = z + beta d + gam psi   (solvers.py:142-149)
+  const double cwr = czr - pp * czr + beta * pdr, cwi = czi - pp * czi + beta * pdi;
+  double az = 1.0, ad = beta, ag = 0.0;
+  double gre = -(czr + cwr), gim = -(czi + cwi);
+  double gd = zg_raw + beta * gdr - (pgr * (czr + cwr) + pgi * (czi + cwi));
+  if (gd <= 0) {
+    az = 1.0; ad = 0.0; ag = 0.0; gre = -czr; gim = -czi;
+    gd = zg_raw - cz_pg;
+  }
+  if (gd <= 0) {
+    az = 0.0; ad = 0.0; ag = 1.0; gre = -pgr; gim = -pgi;
+    gd = gg - (pgr * pgr + pgi * pgi);
+  }
+  if (it > 0 && sy > 1e-30 && isfinite(sy)) {      // Barzilai-Borwein (solvers.py:150-158)
+    double bb = ss / sy;
+    if (isfinite(bb)) {
+      bb = fmin(fmax(bb, c.bb_min), c.bb_max);
+      q->alpha = (1.0 - c.bb_mix) * q->alpha + c.bb_mix * bb;
+    }
+  }
+  q->alpha = fmin(fmax(q->alpha, c.alpha_min), c.alpha_max);
+  if (c.verbose >= 2)
+    printf("  [gs%d] it=%d E=%.15e gd=%.3e beta=%.3e alpha=%.3e zg=%.3e zy=%.3e gg=%.3e sy=%.3e ss=%.3e\n", b, it,
+           q->E, gd, beta, q->alpha, zg, zy, gg, sy, ss);
+  q->gd = gd;
+  q->trial = q->alpha;
+  q->ls = 0;
+  q->cz_re = czr;
+  q->cz_im = czi;
+  q->zg_cur = zg;                    // committed to zg_prev / gg_prev on acceptance
+  q->gg_cur = gg;
+  double* tb = c.tab + (long long)b * DS_N;
+  tb[DS_AZ] = az; tb[DS_AD] = ad; tb[DS_AG] = ag; tb[DS_GRE] = gre; tb[DS_GIM] = gim;
+  tb[DS_T] = q->trial;
+  c.fl[b] = nf | FL_COMB | FL_SEARCH | (q->lr ? FL_SLR : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// k_ctrl_armijo: energy of the trial psi - t d (normalised in the sums,
+// operators.py:32-33 via energy_components), the Armijo test, and the
+// accept / shrink / fail transitions (solvers.py:160-196).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
+  __shared__ double sh[CTRL_WG];
+  __shared__ double e0[8], e1[2], e2[2], gr[1];
+  const int b = blockIdx.x;
+  DevInst* q = c.st + b;
+  const unsigned f = c.fl[b];
+  const bool search = (f & FL_SEARCH) != 0u;
+  if (search) {
+    fold_rows<8>(c.r_e0 + (long long)b * c.n_e0 * 8, c.n_e0, 6, e0, sh);
+    fold_rows<2>(c.r_e1 + (long long)b * c.n_e1 * 2, c.n_e1, 2, e1, sh);
+    if (c.ndim == 3) fold_rows<2>(c.r_e2 + (long long)b * c.n_e2 * 2, c.n_e2, 2, e2, sh);
+    if (f & FL_SLR) fold_rows<1>(c.r_gr + (long long)b * c.n_gr, c.n_gr, 1, gr, sh);
+  }
+  if (threadIdx.x != 0) return;
+  if (!search) {
+    c.fl[b] = 0u;
+    return;
+  }
+  const double dv = c.dv;
+  const InstPhys ph = c.iphys[b];
+  double kin = e0[0] + e1[0], lz = e0[1] + e1[1];
+  if (c.ndim == 3) { kin += e2[0]; lz += e2[1]; }
+  const double nrm2 = e0[2] * dv;
+  const double i1 = 1.0 / nrm2, i2 = i1 * i1, i3 = i2 * i1;
+  const double Ekin = kin * dv * i1, Etrap = e0[3] * dv * i1;
+  const double Econtact = 0.5 * ph.beta2 * (e0[4] * dv) * i2;
+  const double Ethree = (ph.beta3 / 3.0) * (e0[5] * dv) * i3;
+  const double Erot = -ph.Omega * (lz * dv * i1);
+  const double Egrav = (f & FL_SLR) ? 0.5 * (-ph.G_C) * (gr[0] * dv) * i2 : 0.0;
+  const double Et = Ekin + Etrap + Econtact + Ethree + Erot + Egrav;
+  q->n_energy += 1;
+  double* tb = c.tab + (long long)b * DS_N;
+  if (isfinite(Et) && Et <= q->E - c.c1 * q->trial * q->gd) {   // accepted (solvers.py:186-196)
+    q->n_fail = 0;
+    q->zg_prev = q->zg_cur;
+    q->gg_prev = q->gg_cur;
+    q->E = Et;
+    q->mu = Et + Econtact + 2.0 * Ethree + Egrav;
+    q->alpha = fmin(c.alpha_max, q->trial * c.growth);
+    tb[DS_T] = q->trial;
+    tb[DS_SC] = sqrt(nrm2);
+    tb[DS_MU] = q->mu;
+    tb[DS_PHISC] = q->lr ? -ph.G_C * i1 : 0.0;
+    q->pending = 1;
+    q->it_acc = q->it;
+    q->trial_acc = q->trial;
+    q->it += 1;
+    c.fl[b] = FL_ACC | FL_DIR;
+    return;
+  }
+  if (c.verbose >= 2)
+    printf("  [gs%d] it=%d reject trial=%.3e Et-E=%.3e need=%.3e\n", b, q->it, q->trial, Et - q->E,
+           -c.c1 * q->trial * q->gd);
+  q->trial *= c.shrink;
+  q->ls += 1;
+  if (!(q->trial < c.alpha_min || q->ls >= c.ls_max)) {          // next trial in the following step
+    tb[DS_T] = q->trial;
+    c.fl[b] = FL_SEARCH | (q->lr ? FL_SLR : 0u);
+    return;
+  }
+  // line search failed: direction = z_p, psi unchanged (solvers.py:175-185)
+  q->n_fail += 1;
+  q->alpha = fmax(q->alpha * 0.5, c.alpha_min);
+  tb[DS_AZ] = 1.0; tb[DS_AD] = 0.0; tb[DS_AG] = 0.0; tb[DS_GRE] = -q->cz_re; tb[DS_GIM] = -q->cz_im;
+  q->it += 1;
+  if (q->alpha <= c.alpha_min || q->n_fail >= c.stall_max) {
+    q->stop = BEC_STOP_LINE_SEARCH_STALLED;
+    q->done = 1;
+    c.hdone[b] = 1;
+    if (c.verbose)
+      printf("[gs] line search stalled it=%d: E=%.10g res=%.3e best_res=%.3e\n", q->it - 1, q->E, q->resid_rel,
+             q->best_resid);
+    c.fl[b] = 0u;
+    return;
+  }
+  c.fl[b] = FL_FAIL | FL_DIR;
+}
+
+void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl_dir, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
+}
+void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl_armijo, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
+}

@@ -183,6 +183,41 @@ enum PartSite { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DO
                 C_RES2, C_COUNT };
 
 // ---------------------------------------------------------------------------
+// device-driven solver loop (bec_ctrl.hip): per-instance state of the loop of
+// solvers.py:48-226 kept in device memory, advanced by two controller kernels
+// per step.  Doubles first, then counters.
+// ---------------------------------------------------------------------------
+struct DevInst {
+  double E, mu, resid_rel, best_resid, alpha, sigma, zg_prev, gg_prev, zg_cur, gg_cur, E_ref;
+  double trial, gd, trial_acc, cz_re, cz_im;
+  double clk0;
+  long long n_energy;
+  int it, it_ref, n_fail, n_iter, stop, it_acc, ls, converged, done, pending, lr, n_tr;
+};
+
+struct CtrlArgs {
+  int nb, ndim;
+  double dv;
+  // solver knobs (solvers.py:59-88)
+  int maxit, check, ls_max, stall_max, restart_period, conv_window, cg_beta, verbose;
+  double res_tol, etol, alpha_min, alpha_max, c1, shrink, growth, bb_min, bb_max, bb_mix;
+  // device partial rows of this step's reducing launches, rows per instance
+  const double* r_dots; int n_dots;      // 16 sums per row
+  const double* r_hl; int n_hl;          // |h|^2, Re<psi,h>, max rho (4 per row)
+  const double* r_e0; int n_e0;          // first energy pass: 6 sums (8 per row)
+  const double* r_e1; int n_e1;          // second axis: 2 sums
+  const double* r_e2; int n_e2;          // third axis (3D): 2 sums
+  const double* r_gr; int n_gr;          // sum phi_raw rho (1 per row)
+  const double* sigma;                   // Sobolev shift per instance
+  const InstPhys* iphys;
+  DevInst* st; unsigned* fl; double* tab;
+  double* trace; int trace_cap; double clk_hz;   // convergence history of instance 0
+  int* hdone;                            // host-mapped: instance finished
+};
+void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s);
+void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // launch wrappers (bec_kernels.hip); reducing launches return rows/instance
 // ---------------------------------------------------------------------------
 LineCfg line_cfg(int L);

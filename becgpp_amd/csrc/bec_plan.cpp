@@ -332,7 +332,7 @@ Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool norm
 int run_H(Plan& p, const Batch& B, const HArgs& h) {
   int last_rows = 0;
   const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
-  const bool transformed = h.d || h.div;
+  const bool transformed = h.d || h.div;   // (device-driven: the step and norm come from the table)
   if (transformed && !h.psi_w) throw BecError(2, "internal: a transformed state must be written back");
   const double2* psi_rest = h.psi_w ? h.psi_w : h.psi;
   for (int k = 0; k < p.ndim; ++k) {
@@ -351,9 +351,9 @@ int run_H(Plan& p, const Batch& B, const HArgs& h) {
       a.psi_w = h.psi_w;
       a.sdiff = h.sdiff;
       a.out = last ? h.out : p.w1;
-      if (B.dev) {
+      if (B.dev && h.d) {              // accepting pass: (psi - t d) / ||psi - t d|| from the table
         a.tab = B.tab;
-        a.tabm = (h.d ? 1u << DS_T : 0u) | (h.div ? 1u << DS_SC : 0u);
+        a.tabm = (1u << DS_T) | (1u << DS_SC);
       }
     } else {
       a.psi = psi_rest;

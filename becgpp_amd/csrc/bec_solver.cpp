@@ -21,7 +21,10 @@
 
 enum { S_PSI = 0, S_SD, S_G, S_Y, S_D, S_Z, S_BEST, S_COUNT };
 
+static void dev_release(Plan& p);
+
 void solver_release(Plan& p) {
+  dev_release(p);
   for (int i = 0; i < 8; ++i)
     if (p.st[i]) (void)hipFree(p.st[i]);
   for (int i = 0; i < 2; ++i)
@@ -38,6 +41,35 @@ static void solver_alloc(Plan& p, int nb) {
   for (int i = 0; i < S_COUNT; ++i) HIPCHK(hipMalloc(&p.st[i], sizeof(double2) * p.n * nb));
   for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&p.stphi[i], sizeof(double) * p.n * nb));
   p.scap = nb;
+}
+
+static void dev_release(Plan& p) {
+  void* d[] = {p.dinst, p.flags, p.dtab, p.dtrace};
+  for (void* q : d)
+    if (q) (void)hipFree(q);
+  if (p.hdone) (void)hipHostFree(p.hdone);
+  p.dinst = nullptr; p.flags = nullptr; p.dtab = nullptr; p.dtrace = nullptr; p.dtrace_cap = 0;
+  p.hdone = nullptr; p.hdone_dev = nullptr;
+}
+
+// device-loop state for nb instances (sized with the solver state) and a
+// trace buffer of trace_cap rows
+static void dev_alloc(Plan& p, int nb, int trace_cap) {
+  if (!p.dinst) {
+    HIPCHK(hipMalloc(&p.dinst, sizeof(DevInst) * BEC_BMAX));
+    HIPCHK(hipMalloc(&p.flags, sizeof(unsigned) * BEC_BMAX));
+    HIPCHK(hipMalloc(&p.dtab, sizeof(double) * DS_N * BEC_BMAX));
+    HIPCHK(hipHostMalloc(&p.hdone, sizeof(int) * BEC_BMAX, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&p.hdone_dev, p.hdone, 0));
+  }
+  if (trace_cap > p.dtrace_cap) {
+    if (p.dtrace) (void)hipFree(p.dtrace);
+    p.dtrace = nullptr;
+    p.dtrace_cap = 0;
+    HIPCHK(hipMalloc(&p.dtrace, sizeof(double) * 4 * (size_t)trace_cap));
+    p.dtrace_cap = trace_cap;
+  }
+  (void)nb;
 }
 
 static double now_s() {
@@ -69,6 +101,176 @@ struct Inst {
 static Batch mask_batch(int nb, unsigned long long m) { return Batch{nb, m}; }
 
 static int lmin_of(int tpl) { return tpl >= 64 ? 1 : 64 / tpl; }
+
+// The loop with its scalar logic on the device (bec_ctrl.hip): the host
+// launches whole steps and polls the instances' done flags one chunk of
+// steps behind, so the device always has work queued.  On return I[] holds
+// each instance's final state and n_tr the trace rows written.
+static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<Inst>& I, bool lr_any,
+                        double* trace, int trace_cap, int& n_tr) {
+  const long long n = p.n;
+  hipStream_t s = p.stream;
+  double2* PSI = p.st[S_PSI];
+  double2* SD = p.st[S_SD];
+  double2* G = p.st[S_G];
+  double2* Y = p.st[S_Y];
+  double2* D = p.st[S_D];
+  double2* Z = p.st[S_Z];
+  double2* BEST = p.st[S_BEST];
+  double* phi = p.stphi[0];
+  double* phit = p.stphi[1];
+  dev_alloc(p, nb, trace ? trace_cap : 0);
+  std::vector<DevInst> di(nb);
+  std::vector<unsigned> fl0(nb, FL_DIR);
+  for (int b = 0; b < nb; ++b) {
+    memset(&di[b], 0, sizeof(DevInst));
+    di[b].E = I[b].E;
+    di[b].mu = I[b].mu;
+    di[b].E_ref = I[b].E_ref;
+    di[b].resid_rel = I[b].resid_rel;
+    di[b].best_resid = I[b].best_resid;
+    di[b].alpha = I[b].alpha;
+    di[b].sigma = I[b].sigma;
+    di[b].stop = BEC_STOP_MAXIT;
+    di[b].lr = I[b].lr ? 1 : 0;
+    di[b].n_energy = I[b].n_energy;
+    di[b].n_tr = n_tr;
+    p.hdone[b] = 0;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpy(p.dinst, di.data(), sizeof(DevInst) * nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.flags, fl0.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice));
+
+  CtrlArgs c;
+  memset(&c, 0, sizeof(c));
+  c.nb = nb;
+  c.ndim = p.ndim;
+  c.dv = p.dv;
+  c.maxit = cf->maxit;
+  c.check = std::max(1, cf->check);
+  c.ls_max = cf->linesearch_max;
+  c.stall_max = std::max(1, cf->linesearch_stall);
+  c.restart_period = cf->cg_restart;
+  c.conv_window = cf->conv_window;
+  c.cg_beta = cf->cg_beta;
+  c.verbose = cf->verbose;
+  c.res_tol = cf->res_tol;
+  c.etol = cf->energy_tol;
+  c.alpha_min = 1e-8;
+  c.alpha_max = cf->step_max;
+  c.c1 = 1e-4;
+  c.shrink = 0.5;
+  c.growth = 1.1;
+  c.bb_min = 0.05;
+  c.bb_max = cf->step_max;
+  c.bb_mix = 0.5;
+  c.r_dots = part_of(p, C_DOTS);
+  c.r_hl = part_of(p, C_HLAST);
+  c.n_hl = p.hlast_rows;
+  c.r_e0 = part_of(p, C_EFIRST);
+  c.r_e1 = part_of(p, C_EREST0);
+  c.r_e2 = part_of(p, C_EREST1);
+  c.r_gr = part_of(p, C_EGRAV);
+  c.sigma = p.sigma;
+  c.iphys = p.iphys;
+  c.st = (DevInst*)p.dinst;
+  c.fl = p.flags;
+  c.tab = p.dtab;
+  c.trace = trace ? p.dtrace : nullptr;
+  c.trace_cap = trace ? trace_cap : 0;
+  int dev = 0, khz = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  c.clk_hz = 1e3 * (double)std::max(khz, 1);
+  c.hdone = p.hdone_dev;
+  const double floor_ = cf->precond_shift_min;
+
+  auto step = [&]() {
+    const Batch BD = dev_batch(p, nb, FL_DIR);
+    launch_sigma(p, nb, act_of(BD), PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2, floor_,
+                 s);
+    run_precond(p, BD, G, Z);
+    c.n_dots = launch_dots(Z, PSI, D, G, Y, SD, n, nb, act_of(BD), part_of(p, C_DOTS), nullptr, s);
+    launch_ctrl_dir(c, s);
+    launch_copy(PSI, BEST, n, nb, act_of(dev_batch(p, nb, FL_BEST)), s);
+    CombArgs none;
+    memset(&none, 0, sizeof(none));
+    launch_combine(D, Z, G, PSI, none, p.dtab, n, nb, act_of(dev_batch(p, nb, FL_COMB)), s);
+    const EnergyRows er = run_energy_sums(p, dev_batch(p, nb, FL_SEARCH), PSI, D, nullptr,
+                                          lr_any ? p.rho : nullptr);
+    c.n_e0 = er.r0;
+    c.n_e1 = er.r1;
+    c.n_e2 = er.r2;
+    if (lr_any) c.n_gr = run_conv(p, 0, dev_batch(p, nb, FL_SLR), p.rho, 1.0, phit, p.rho);
+    launch_ctrl_armijo(c, s);
+    launch_combine(D, Z, nullptr, PSI, none, p.dtab, n, nb, act_of(dev_batch(p, nb, FL_FAIL)), s);
+    HArgs a;
+    memset(&a, 0, sizeof(a));
+    a.psi = PSI;
+    a.d = D;
+    a.psi_w = PSI;
+    a.sdiff = SD;
+    a.local_on = true;
+    a.phi = lr_any ? phit : nullptr;
+    a.phi_out = lr_any ? phi : nullptr;
+    a.kf = 0.5;
+    a.rf_omega = 1;
+    a.out = G;
+    a.ydiff = Y;
+    a.reduce = true;
+    c.n_hl = run_H(p, dev_batch(p, nb, FL_ACC), a);
+  };
+
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  // every step resolves a trial of each running instance: maxit iterations of
+  // at most linesearch_max trials bound the number of steps
+  const long long max_steps = (long long)(cf->maxit + 2) * (long long)(std::max(1, cf->linesearch_max) + 1) + 64;
+  const int K = 4;                    // steps per polling chunk
+  try {
+    for (long long chunk = 0;; ++chunk) {
+      for (int k = 0; k < K; ++k) step();
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(ev[chunk & 1], s));
+      if (chunk > 0) {
+        HIPCHK(hipEventSynchronize(ev[(chunk - 1) & 1]));
+        int ndone = 0;
+        for (int b = 0; b < nb; ++b) ndone += ((volatile int*)p.hdone)[b] != 0;
+        if (ndone == nb) break;
+      }
+      if ((chunk + 1) * K > max_steps) throw BecError(2, "internal: the device loop did not finish");
+    }
+    HIPCHK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    throw;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  HIPCHK(hipMemcpy(di.data(), p.dinst, sizeof(DevInst) * nb, hipMemcpyDeviceToHost));
+  for (int b = 0; b < nb; ++b) {
+    Inst& q = I[b];
+    q.E = di[b].E;
+    q.resid_rel = di[b].resid_rel;
+    q.best_resid = di[b].best_resid;
+    q.n_iter = di[b].n_iter;
+    q.converged = di[b].converged != 0;
+    q.stop = di[b].stop;
+    q.sigma = di[b].sigma;
+    q.n_energy = di[b].n_energy;
+    q.done = true;
+  }
+  if (trace && di[0].n_tr > n_tr) {
+    std::vector<double> rows(4 * (size_t)di[0].n_tr);
+    HIPCHK(hipMemcpy(rows.data(), p.dtrace, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
+    for (int r = n_tr; r < di[0].n_tr; ++r)
+      for (int k = 0; k < 4; ++k) trace[4 * r + k] = rows[4 * r + k];
+    n_tr = di[0].n_tr;
+  }
+}
 
 static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
                        bec_solve_out* out, double* trace, int trace_cap) {
@@ -182,6 +384,9 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   }
   launch_copy(PSI, BEST, n, nb, mask_act(all), s);
 
+  if (!cf->host_loop) {
+    device_loop(p, nb, cf, I, lr_any, trace && nb == 1 ? trace : nullptr, trace_cap, n_tr);
+  } else {
   int it = 0;
   while (true) {
     unsigned long long run = 0;          // instances that execute iteration `it`
@@ -274,7 +479,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
       const C2 cw = cz - pp_ * cz + beta * pd;
       double az = 1.0, ad = beta, ag = 0.0;
       C2 gam = cm(0.0, 0.0) - (cz + cw);
-      double gd = zg + beta * gdr - (conj(pg) * (cz + cw)).re;
+      double gd = zg_raw + beta * gdr - (conj(pg) * (cz + cw)).re;
       if (gd <= 0) {
         az = 1.0; ad = 0.0; ag = 0.0; gam = cm(-cz.re, -cz.im);
         gd = zg_raw - (conj(pg) * cz).re;
@@ -397,6 +602,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
       HIPCHK(hipGetLastError());
     }
     ++it;
+  }
   }
   // return the lowest-residual iterate seen                             (solvers.py:225-226)
   unsigned long long from_best = 0;

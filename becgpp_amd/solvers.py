@@ -34,7 +34,7 @@ def _solver_cfg(cfg, verbose):
         int(cfg.get("conv_window", 2000)), float(cfg.get("step", 1.0)), float(cfg.get("step_max", 3.0)),
         int(cfg.get("linesearch_max", 8)), int(cfg.get("linesearch_stall", 10)),
         float(cfg.get("precond_shift_min", 0.5)), _lib.CG_BETA[cg_beta], int(cfg.get("cg_restart", 30)),
-        1 if verbose else 0, int(cfg.get("check", 200)))
+        1 if verbose else 0, int(cfg.get("check", 200)), 1 if cfg.get("host_loop", False) else 0)
 
 
 def solve_native(cfg, G, psi0, verbose=False, record=False):

@@ -66,6 +66,9 @@ typedef struct bec_solver_cfg {
   int cg_restart;
   int verbose;
   int check;
+  int host_loop;   /* 0: the loop's scalar logic runs in controller kernels on the
+                      device (default); 1: on the host, one synchronisation per
+                      iteration and Armijo trial (no reference counterpart) */
 } bec_solver_cfg;
 
 typedef struct bec_solve_out {

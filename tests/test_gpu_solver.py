@@ -95,3 +95,21 @@ def test_iteration_count():
     c = B.default_cfg(mode="single", L=6, Ngrid=48, beta2=50.0, res_tol=1e-14, maxit=5)
     _, _, obs = B.ground_state(c, verbose=False)
     assert obs["iters"] == 5
+
+
+@pytest.mark.parametrize("kw", [dict(beta2=30.0, Omega=0.4), dict(beta2=30.0, G_C=3.0, kernel="newton"),
+                                dict(dimension="3D", beta2=20.0, G_C=3.0, kernel="newton", Ngrid=24, L=5)])
+def test_device_and_host_loops_agree(kw):
+    """The loop's scalar logic on the device (default) and on the host: the
+    same trajectory to round-off over the first iterations, and the same
+    converged state."""
+    import becgpp_amd as B
+    c = cfg(res_tol=1e-8, record_trace=True, **kw)
+    Gd = B.make_grid(c)
+    psi0 = torch.as_tensor(O.noisy_gaussian(O.make_grid(c), 1e-3, seed=9)).cuda()
+    _, _, od = B.ground_state(c, Gd, psi0, verbose=False)
+    _, _, oh = B.ground_state(dict(c, host_loop=True), Gd, psi0, verbose=False)
+    for (_, i1, E1, _), (_, i2, E2, _) in list(zip(od["trace"], oh["trace"]))[:10]:
+        assert i1 == i2 and abs(E1 - E2) <= 1e-12 * abs(E2)
+    for k in ("E", "mu", "Lz"):
+        assert abs(od[k] - oh[k]) <= 1e-9 * max(1.0, abs(oh[k])), (k, od[k], oh[k])
