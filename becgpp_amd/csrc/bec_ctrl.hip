@@ -48,23 +48,40 @@ __device__ void fold_rows(const double* __restrict__ rows, int R, int K, double*
 
 __device__ __forceinline__ double clk_now() { return (double)wall_clock64(); }
 
+// one progress record into the instance's ring in host-mapped memory; the
+// host prints the records after each poll (count after the records)
+__device__ __forceinline__ void vlog_put(const CtrlArgs& c, int b, DevInst* q, int kind, int it, double E,
+                                         double res, double best, double step) {
+  if (!c.vlog) return;
+  double* r = c.vlog + ((long long)b * VLOG_CAP + (q->n_log % VLOG_CAP)) * VLOG_REC;
+  r[0] = kind; r[1] = it; r[2] = E; r[3] = res; r[4] = best; r[5] = step;
+  q->n_log += 1;
+  c.vlog[(long long)BEC_BMAX * VLOG_CAP * VLOG_REC + b] = (double)q->n_log;
+}
+
+// the instance record is worked on in LDS: one coalesced load and store
+// instead of a chain of dependent global accesses by thread 0
+constexpr int DEVINST_WORDS = (int)(sizeof(DevInst) / sizeof(unsigned long long));
+static_assert(sizeof(DevInst) % sizeof(unsigned long long) == 0, "DevInst is a whole number of 8-byte words");
+__device__ __forceinline__ void inst_load(DevInst* dst, const DevInst* src) {
+  const unsigned long long* s = (const unsigned long long*)src;
+  unsigned long long* d = (unsigned long long*)dst;
+  for (int i = threadIdx.x; i < DEVINST_WORDS; i += blockDim.x) d[i] = s[i];
+}
+__device__ __forceinline__ void inst_store(DevInst* dst, const DevInst* src) {
+  const unsigned long long* s = (const unsigned long long*)src;
+  unsigned long long* d = (unsigned long long*)dst;
+  for (int i = threadIdx.x; i < DEVINST_WORDS; i += blockDim.x) d[i] = s[i];
+}
+
 // ---------------------------------------------------------------------------
 // k_ctrl_dir: residual bookkeeping of the step accepted last (solvers.py:
 // 187-221), stop rules, then the CG direction, Barzilai-Borwein trial step
 // and Armijo start (solvers.py:128-159) from this step's inner products.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
-  __shared__ double sh[CTRL_WG];
-  __shared__ double r[16], hl[4];
-  const int b = blockIdx.x;
-  DevInst* q = c.st + b;
-  const unsigned f = c.fl[b];
-  const bool dir = (f & FL_DIR) != 0u;
-  if (dir) {
-    fold_rows<16>(c.r_dots + (long long)b * c.n_dots * 16, c.n_dots, 16, r, sh);
-    if (q->pending) fold_rows<4>(c.r_hl + (long long)b * c.n_hl * 4, c.n_hl, 2, hl, sh);
-  }
-  if (threadIdx.x != 0) return;
+// thread 0 of k_ctrl_dir: the scalar logic on the LDS copy of the record
+__device__ __forceinline__ void ctrl_dir_scalar(const CtrlArgs& c, int b, unsigned f, bool dir, DevInst* q,
+                                             const double* r, const double* hl) {
   if (!dir) {                         // retrying a trial, or finished
     c.fl[b] = (f & FL_SEARCH) ? (FL_SEARCH | (q->lr ? FL_SLR : 0u)) : 0u;
     return;
@@ -89,8 +106,7 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
       nf |= FL_BEST;
     }
     if (c.verbose && q->it_acc % c.check == 0)
-      printf("  [gs%s%d] it=%d E=%.8f res=%.2e best=%.2e step=%.2e\n", c.nb > 1 ? " " : "", c.nb > 1 ? b : 0,
-             q->it_acc, q->E, q->resid_rel, q->best_resid, q->trial_acc);
+      vlog_put(c, b, q, VLOG_PROGRESS, q->it_acc, q->E, q->resid_rel, q->best_resid, q->trial_acc);
     if (q->resid_rel < c.res_tol) {
       q->converged = 1;
       q->stop = BEC_STOP_CONVERGED;
@@ -161,9 +177,6 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
     }
   }
   q->alpha = fmin(fmax(q->alpha, c.alpha_min), c.alpha_max);
-  if (c.verbose >= 2)
-    printf("  [gs%d] it=%d E=%.15e gd=%.3e beta=%.3e alpha=%.3e zg=%.3e zy=%.3e gg=%.3e sy=%.3e ss=%.3e\n", b, it,
-           q->E, gd, beta, q->alpha, zg, zy, gg, sy, ss);
   q->gd = gd;
   q->trial = q->alpha;
   q->ls = 0;
@@ -182,24 +195,9 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
 // operators.py:32-33 via energy_components), the Armijo test, and the
 // accept / shrink / fail transitions (solvers.py:160-196).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
-  __shared__ double sh[CTRL_WG];
-  __shared__ double e0[8], e1[2], e2[2], gr[1];
-  const int b = blockIdx.x;
-  DevInst* q = c.st + b;
-  const unsigned f = c.fl[b];
-  const bool search = (f & FL_SEARCH) != 0u;
-  if (search) {
-    fold_rows<8>(c.r_e0 + (long long)b * c.n_e0 * 8, c.n_e0, 6, e0, sh);
-    fold_rows<2>(c.r_e1 + (long long)b * c.n_e1 * 2, c.n_e1, 2, e1, sh);
-    if (c.ndim == 3) fold_rows<2>(c.r_e2 + (long long)b * c.n_e2 * 2, c.n_e2, 2, e2, sh);
-    if (f & FL_SLR) fold_rows<1>(c.r_gr + (long long)b * c.n_gr, c.n_gr, 1, gr, sh);
-  }
-  if (threadIdx.x != 0) return;
-  if (!search) {
-    c.fl[b] = 0u;
-    return;
-  }
+// thread 0 of k_ctrl_armijo: the Armijo test and transitions on the LDS copy
+__device__ __forceinline__ void ctrl_armijo_scalar(const CtrlArgs& c, int b, unsigned f, DevInst* q, const double* e0,
+                                                const double* e1, const double* e2, const double* gr) {
   const double dv = c.dv;
   const InstPhys ph = c.iphys[b];
   double kin = e0[0] + e1[0], lz = e0[1] + e1[1];
@@ -229,17 +227,14 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
     q->it_acc = q->it;
     q->trial_acc = q->trial;
     q->it += 1;
-    c.fl[b] = FL_ACC | FL_DIR;
+    c.fl[b] = (f & FL_BEST) | FL_ACC | FL_DIR;
     return;
   }
-  if (c.verbose >= 2)
-    printf("  [gs%d] it=%d reject trial=%.3e Et-E=%.3e need=%.3e\n", b, q->it, q->trial, Et - q->E,
-           -c.c1 * q->trial * q->gd);
   q->trial *= c.shrink;
   q->ls += 1;
   if (!(q->trial < c.alpha_min || q->ls >= c.ls_max)) {          // next trial in the following step
     tb[DS_T] = q->trial;
-    c.fl[b] = FL_SEARCH | (q->lr ? FL_SLR : 0u);
+    c.fl[b] = (f & FL_BEST) | FL_SEARCH | (q->lr ? FL_SLR : 0u);
     return;
   }
   // line search failed: direction = z_p, psi unchanged (solvers.py:175-185)
@@ -251,13 +246,51 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
     q->stop = BEC_STOP_LINE_SEARCH_STALLED;
     q->done = 1;
     c.hdone[b] = 1;
-    if (c.verbose)
-      printf("[gs] line search stalled it=%d: E=%.10g res=%.3e best_res=%.3e\n", q->it - 1, q->E, q->resid_rel,
-             q->best_resid);
-    c.fl[b] = 0u;
+    if (c.verbose) vlog_put(c, b, q, VLOG_STALL, q->it - 1, q->E, q->resid_rel, q->best_resid, q->trial);
+    c.fl[b] = f & FL_BEST;
     return;
   }
-  c.fl[b] = FL_FAIL | FL_DIR;
+  c.fl[b] = (f & FL_BEST) | FL_FAIL | FL_DIR;
+}
+
+__global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
+  __shared__ double sh[CTRL_WG];
+  __shared__ double r[16], hl[4];
+  __shared__ DevInst qs;
+  const int b = blockIdx.x;
+  DevInst* q = &qs;
+  const unsigned f = c.fl[b];
+  const bool dir = (f & FL_DIR) != 0u;
+  inst_load(q, c.st + b);
+  __syncthreads();
+  if (dir) {
+    fold_rows<16>(c.r_dots + (long long)b * c.n_dots * 16, c.n_dots, 16, r, sh);
+    if (q->pending) fold_rows<4>(c.r_hl + (long long)b * c.n_hl * 4, c.n_hl, 2, hl, sh);
+  }
+  if (threadIdx.x == 0) ctrl_dir_scalar(c, b, f, dir, q, r, hl);
+  __syncthreads();
+  inst_store(c.st + b, q);
+}
+
+__global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
+  __shared__ double sh[CTRL_WG];
+  __shared__ double e0[8], e1[2], e2[2], gr[1];
+  __shared__ DevInst qs;
+  const int b = blockIdx.x;
+  const unsigned f = c.fl[b];
+  const bool search = (f & FL_SEARCH) != 0u;
+  if (!search) {                      // finished this step: only a pending best copy remains
+    if (threadIdx.x == 0) c.fl[b] = f & FL_BEST;
+    return;
+  }
+  inst_load(&qs, c.st + b);
+  fold_rows<8>(c.r_e0 + (long long)b * c.n_e0 * 8, c.n_e0, 6, e0, sh);
+  fold_rows<2>(c.r_e1 + (long long)b * c.n_e1 * 2, c.n_e1, 2, e1, sh);
+  if (c.ndim == 3) fold_rows<2>(c.r_e2 + (long long)b * c.n_e2 * 2, c.n_e2, 2, e2, sh);
+  if (f & FL_SLR) fold_rows<1>(c.r_gr + (long long)b * c.n_gr, c.n_gr, 1, gr, sh);
+  if (threadIdx.x == 0) ctrl_armijo_scalar(c, b, f, &qs, e0, e1, e2, gr);
+  __syncthreads();
+  inst_store(c.st + b, &qs);
 }
 
 void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s) {

@@ -24,7 +24,7 @@ enum KinKind { KE_FIRST = 0, KE_REST = 1, KH_FIRST = 2, KH_MID = 3, KH_LAST = 4 
 enum LoadKind { LD_C = 0, LD_R = 1, LD_HERM = 2 };
 enum DirKind { DIR_FWD = 0, DIR_INV = 1, DIR_FMI = 2 };
 enum MulKind { MUL_NONE = 0, MUL_KHAT2 = 1, MUL_KHAT3 = 2, MUL_PRE2 = 3, MUL_PRE3 = 4 };
-enum StoreKind { ST_C = 0, ST_RE = 1 };
+enum StoreKind { ST_C = 0, ST_RE = 1, ST_CDOT = 2 };   // ST_CDOT: complex store + the 16 CG inner products
 
 struct LineCfg {       // threads per line / lines per workgroup for a length
   int tpl, lpw;
@@ -81,6 +81,7 @@ struct PassArgs {
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
   unsigned long long* ucount;          // measurement: instances covered (nullable)
+  const double2* dpsi; const double2* dd; const double2* dg; const double2* dy; const double2* ds;  // ST_CDOT
 };
 
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
@@ -105,6 +106,11 @@ struct KinArgs {
   const double* tab; unsigned tabm;                      // device scalars: bit k of tabm reads DS slot k
   double* partials; double* hpartials;
   unsigned long long* ucount;                            // measurement: instances covered (nullable)
+  // device-driven loop fusions
+  double2* hy_out;                                       // KE_FIRST: also IFFT(m FFT(loaded)) of this axis
+  const double2* z; const double2* g; double2* d_out;    // KE_FIRST: instances with comb_bit form
+  unsigned comb_bit;                                     //   d = az z + ad d + ag g + gam psi first
+  int acc_sc;                                            // KH_MID/KH_LAST: acc is divided by sc[b]
 };
 
 // per-instance combination d <- az z + ad d + ag g + gam psi
@@ -162,6 +168,8 @@ struct Plan {
   // device-driven loop state, sized for scap instances
   void* dinst; unsigned* flags; double* dtab; double* dtrace; int dtrace_cap;
   int* hdone; int* hdone_dev;        // pinned: instance finished (written by the controller)
+  double* hvlog; double* hvlog_dev;  // pinned: progress records + per-instance record counts
+  hipStream_t cstream;               // plan-owned stream the loop's steps are captured on
 };
 
 // reduction slots, RSTRIDE doubles per instance in Plan::hred
@@ -192,7 +200,7 @@ struct DevInst {
   double trial, gd, trial_acc, cz_re, cz_im;
   double clk0;
   long long n_energy;
-  int it, it_ref, n_fail, n_iter, stop, it_acc, ls, converged, done, pending, lr, n_tr;
+  int it, it_ref, n_fail, n_iter, stop, it_acc, ls, converged, done, pending, lr, n_tr, n_log, pad_;
 };
 
 struct CtrlArgs {
@@ -213,7 +221,12 @@ struct CtrlArgs {
   DevInst* st; unsigned* fl; double* tab;
   double* trace; int trace_cap; double clk_hz;   // convergence history of instance 0
   int* hdone;                            // host-mapped: instance finished
+  double* vlog;                          // host-mapped progress records, VLOG_CAP per instance
 };
+// progress record of the device-driven loop (printed by the host when verbose)
+#define VLOG_CAP 64
+#define VLOG_REC 6                       // kind, it, E, resid, best, step
+enum { VLOG_PROGRESS = 1, VLOG_STALL = 2 };
 void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s);
 void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s);
 
@@ -237,6 +250,10 @@ void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_
 int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
                    hipStream_t s);
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s);
+// device-driven loop, after the Armijo controller: BEST <- psi for FL_BEST,
+// D <- az Z + gam psi (table coefficients) for FL_FAIL
+void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
+                 const unsigned* fl, long long n, int nb, hipStream_t s);
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
                   const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s);
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s);
@@ -253,5 +270,6 @@ bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, int units);   // e
 // device counter that watched launches of the device-driven loop increment
 // once per instance they cover (their instance count is not known on the host)
 unsigned long long* prof_counter(int id);
+bool prof_active();                  // some launch site is being timed
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,
                 SITE_SIGMA_HIST = 21, SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };

@@ -129,6 +129,24 @@ __device__ __forceinline__ double kin_scal(const KinArgs& a, int slot, int b, co
   return (a.tab && ((a.tabm >> slot) & 1u)) ? a.tab[(long long)b * DS_N + slot] : byval[b];
 }
 
+// the per-instance scalars a kinetic pass uses, read once per workgroup
+struct KinScal {
+  double t, sc, mu, phisc, beta2, beta3, rf;
+};
+__device__ __forceinline__ KinScal kin_scal_all(const KinArgs& a, int b) {
+  KinScal k;
+  k.t = kin_scal(a, DS_T, b, a.t);
+  k.sc = kin_scal(a, DS_SC, b, a.sc);
+  k.mu = kin_scal(a, DS_MU, b, a.mu);
+  k.phisc = kin_scal(a, DS_PHISC, b, a.phisc);
+  const bool need_phys = a.local_on || a.rf_omega;
+  const InstPhys ip = need_phys ? a.iphys[b] : InstPhys{0.0, 0.0, 0.0, 0.0};
+  k.beta2 = ip.beta2;
+  k.beta3 = ip.beta3;
+  k.rf = a.rf_omega ? -ip.Omega : a.rf;
+  return k;
+}
+
 template <int AX>
 __device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
   using X = Ax<AX>;
@@ -143,64 +161,128 @@ __device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
 // Kinetic + rotation line pass.  Along an axis with wavenumber k the operator
 // T - Omega L_z acts as the multiplier kf k^2 + rf c k, where c = +x for the y
 // axis, -y for the x axis and 0 for the z axis (L_z = -i(x d_y - y d_x)).
+// The loaded value is (psi - t[b] d) / sc[b].
 //   KE_*: forward transform only; accumulate sum k^2/2 |t_k|^2 / N (kinetic)
 //         and c sum k |t_k|^2 / N (L_z) per line (Parseval), and on the first
-//         pass the local sums rho, V rho, rho^2, rho^3.
+//         pass the local sums rho, V rho, rho^2, rho^3.  With hy_out the first
+//         pass also stores IFFT(m FFT(loaded)) of its axis (the first term of
+//         H of the trial state, reused when the trial is accepted).
 //   KH_*: forward, multiply, inverse; FIRST writes, MID accumulates, LAST adds
-//         the local potential (V + b2 rho + b3 rho^2 + phi) psi - mu psi.
+//         the local potential (V + b2 rho + b3 rho^2 + phi) psi - mu psi.  The
+//         pass that loads a transformed state writes it back (psi_w, and its
+//         change to sdiff).
+// A thread owns at most BEC_EPT elements of the block's lines (lpw N <=
+// BEC_EPT blockDim, since N <= BEC_EPT tpl): it issues all of their loads
+// before using any, so the memory latency is paid once per phase.
+#define BEC_EPT 8
 template <int AX, int KIND>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 smem[];
   double2* twl = smem;                 // twiddles [N]
   double2* lds = smem + a.N;           // lines [lpw][N]
   using X = Ax<AX>;
+  constexpr bool KE = (KIND == KE_FIRST || KIND == KE_REST);
   const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
   const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
   const long long strd = X::stride(N);
-  if (!block_on(a.act, l0, lpw, nl, total)) return;
+  if (l0 >= total) return;
+  // instances of the block's lines: b0 .. b1 (one when lpw divides nl)
+  const int b0 = l0 / nl, b1 = min(l0 + lpw - 1, total - 1) / nl;
+  const bool on0 = act_on(a.act, b0), on1 = (b1 == b0) ? on0 : act_on(a.act, b1);
+  bool any = on0 || on1;
+  for (int b = b0 + 1; b < b1 && !any; ++b) any = act_on(a.act, b);
+  if (!any) return;
+  auto inst_active = [&](int b) { return b == b0 ? on0 : (b == b1 ? on1 : act_on(a.act, b)); };
+  // a workgroup's lines span instances b0 and b1 only (lpw <= nl)
+  const KinScal S0 = kin_scal_all(a, b0);
+  const KinScal S1 = (b1 == b0) ? S0 : kin_scal_all(a, b1);
   const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && act_on(a.act, mb);
-  const bool brow = act_on(a.act, l0 / nl);   // this block stores a partial row
+  const bool active = (l0 + c) < total && inst_active(mb);
+  const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
-  double loc[4] = {0.0, 0.0, 0.0, 0.0};
   stage_twiddles(twl, a.fp);
-  for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
-    int cc, e;
-    if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
-    const int l = l0 + cc;
-    double2 v = make_double2(0.0, 0.0);
-    const int b = l / nl;
-    if (l < total && act_on(a.act, b)) {
-      const int li = l - b * nl;
-      const long long adr = (long long)b * a.istride + X::base(li, N) + e * strd;
-      const double2 p0 = a.psi[adr];
-      v = p0;
-      if (a.d) {
-        const double2 dd = a.d[adr];
-        const double tb = kin_scal(a, DS_T, b, a.t);
-        v.x = v.x - tb * dd.x;
-        v.y = v.y - tb * dd.y;
-      }
-      const double scb = kin_scal(a, DS_SC, b, a.sc);
-      if (scb != 1.0) { v.x = v.x / scb; v.y = v.y / scb; }
-      if (KIND == KH_FIRST) {
-        if (a.psi_w) a.psi_w[adr] = v;
-        if (a.sdiff) a.sdiff[adr] = csub(v, p0);
-      }
-      if (KIND == KE_FIRST) {
-        const double rho = cabs2(v);
-        const double V = local_V<AX>(li, e, a);
-        loc[0] += rho;
-        loc[1] += V * rho;
-        loc[2] += rho * rho;
-        loc[3] += rho * rho * rho;
-        if (a.rho_out) a.rho_out[adr] = rho;
+  const int nel = lpw * N;
+  double loc[4] = {0.0, 0.0, 0.0, 0.0};
+  {
+    // phase 1: issue every load of this thread's elements
+    long long adr[BEC_EPT];
+    int bel[BEC_EPT];
+    double2 p0[BEC_EPT], dd[BEC_EPT];
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int idx = threadIdx.x + j * blockDim.x;
+      bel[j] = -1;
+      if (idx < nel) {
+        int cc, e;
+        if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+        const int l = l0 + cc;
+        const int b = l / nl;
+        if (l < total && inst_active(b)) {
+          bel[j] = b;
+          adr[j] = (long long)b * a.istride + X::base(l - b * nl, N) + e * strd;
+          p0[j] = a.psi[adr[j]];
+          if (a.d) dd[j] = a.d[adr[j]];
+        }
       }
     }
-    lds[cc * N + e] = v;
+    // new search directions (device-driven loop): d = az z + ad d + ag g + gam psi
+    if (KIND == KE_FIRST && a.d_out) {
+      // coefficients of b0 and b1, and whether they form a new direction
+      const double* t0 = a.tab + (long long)b0 * DS_N;
+      const double* t1 = a.tab + (long long)b1 * DS_N;
+      const bool c0 = (a.act.fl[b0] & a.comb_bit) != 0u, c1 = (a.act.fl[b1] & a.comb_bit) != 0u;
+      const double cf0[5] = {t0[DS_AZ], t0[DS_AD], t0[DS_AG], t0[DS_GRE], t0[DS_GIM]};
+      const double cf1[5] = {t1[DS_AZ], t1[DS_AD], t1[DS_AG], t1[DS_GRE], t1[DS_GIM]};
+#pragma unroll
+      for (int j = 0; j < BEC_EPT; ++j) {
+        const int b = bel[j];
+        if (b < 0 || !((b == b0) ? c0 : c1)) continue;
+        const double* cf = (b == b0) ? cf0 : cf1;
+        const double az = cf[0], ad = cf[1], ag = cf[2], gre = cf[3], gim = cf[4];
+        double2 r = make_double2(gre * p0[j].x - gim * p0[j].y, gre * p0[j].y + gim * p0[j].x);
+        if (az != 0.0) { const double2 zz = a.z[adr[j]]; r.x += az * zz.x; r.y += az * zz.y; }
+        if (ad != 0.0) { r.x += ad * dd[j].x; r.y += ad * dd[j].y; }
+        if (ag != 0.0) { const double2 gg = a.g[adr[j]]; r.x += ag * gg.x; r.y += ag * gg.y; }
+        dd[j] = r;
+        a.d_out[adr[j]] = r;
+      }
+    }
+    // phase 2: the loaded values
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int idx = threadIdx.x + j * blockDim.x;
+      if (idx >= nel) continue;
+      int cc, e;
+      if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+      double2 v = make_double2(0.0, 0.0);
+      const int b = bel[j];
+      if (b >= 0) {
+        const KinScal& S = (b == b0) ? S0 : S1;
+        v = p0[j];
+        if (a.d) {
+          v.x = v.x - S.t * dd[j].x;
+          v.y = v.y - S.t * dd[j].y;
+        }
+        if (S.sc != 1.0) { v.x = v.x / S.sc; v.y = v.y / S.sc; }
+        if (KIND == KH_FIRST || KIND == KH_LAST) {
+          if (a.psi_w) a.psi_w[adr[j]] = v;
+          if (a.sdiff) a.sdiff[adr[j]] = csub(v, p0[j]);
+        }
+        if (KIND == KE_FIRST) {
+          const double rho = cabs2(v);
+          const double V = local_V<AX>(l0 + cc - b * nl, e, a);
+          loc[0] += rho;
+          loc[1] += V * rho;
+          loc[2] += rho * rho;
+          loc[3] += rho * rho * rho;
+          if (a.rho_out) a.rho_out[adr[j]] = rho;
+        }
+      }
+      lds[cc * N + e] = v;
+    }
   }
   __syncthreads();
   double2* s = lds + c * N;
@@ -208,78 +290,113 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const double invN = 1.0 / (double)N;
   const int li0 = (l0 + c) - mb * nl;
   const double lzc = (X::lsign == 0 || !active) ? 0.0 : (double)X::lsign * xcoord(X::lidx(li0, N), N, a.dx);
-  if (KIND == KE_FIRST || KIND == KE_REST) {
-    double kin = 0.0, lz = 0.0;
-    if (active) {
-      for (int k = lane; k < N; k += tpl) {
-        const double kv = kfreq(k, N, a.invNdx);
+  const bool mult = !KE || (KIND == KE_FIRST && a.hy_out);
+  const double rfl = active ? ((mb == b0) ? S0.rf : S1.rf) : 0.0;
+  double kin = 0.0, lz = 0.0;
+  if (active) {
+    for (int k = lane; k < N; k += tpl) {
+      const double kv = kfreq(k, N, a.invNdx);
+      if (KE) {
         const double a2 = cabs2(s[k]);
         kin += 0.5 * kv * kv * a2;
         lz += kv * a2;
       }
+      if (mult) s[k] = cscale(s[k], a.kf * kv * kv + rfl * lzc * kv);
     }
+  }
+  if (KE) {
     kin *= invN;
     lz *= lzc * invN;
-    if (!brow) return;
-    if (KIND == KE_FIRST) {
-      double v6[6] = {kin, lz, loc[0], loc[1], loc[2], loc[3]};
-      block_partials<6, 0>(v6, nullptr, a.partials, a.hpartials, lb);
-    } else {
-      double v2[2] = {kin, lz};
-      block_partials<2, 0>(v2, nullptr, a.partials, a.hpartials, lb);
+    if (brow) {
+      if (KIND == KE_FIRST) {
+        double v6[6] = {kin, lz, loc[0], loc[1], loc[2], loc[3]};
+        block_partials<6, 0>(v6, nullptr, a.partials, a.hpartials, lb);
+      } else {
+        double v2[2] = {kin, lz};
+        block_partials<2, 0>(v2, nullptr, a.partials, a.hpartials, lb);
+      }
     }
-    return;
-  }
-  if (active) {
-    const double rf = a.rf_omega ? -a.iphys[mb].Omega : a.rf;
-    for (int k = lane; k < N; k += tpl) {
-      const double kv = kfreq(k, N, a.invNdx);
-      const double m = a.kf * kv * kv + rf * lzc * kv;
-      s[k] = cscale(s[k], m);
-    }
+    if (!mult) return;
   }
   __syncthreads();
   fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
-  for (int idx = threadIdx.x; idx < lpw * N; idx += blockDim.x) {
-    int cc, e;
-    if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
-    const int l = l0 + cc;
-    const int b = l / nl;
-    if (l >= total || !act_on(a.act, b)) continue;
-    const int li = l - b * nl;
-    const long long adr = (long long)b * a.istride + X::base(li, N) + e * strd;
-    double2 u = lds[cc * N + e];
-    u.x *= invN;
-    u.y *= invN;
-    if (KIND == KH_FIRST) {
-      a.out[adr] = u;
-    } else if (KIND == KH_MID) {
-      const double2 p = a.acc[adr];
-      a.out[adr] = cadd(p, u);
-    } else {
-      const double2 ps = a.psi[adr];
-      const double rho = cabs2(ps);
+  {
+    long long adr[BEC_EPT];
+    int bel[BEC_EPT];
+    double2 ps[BEC_EPT], acc[BEC_EPT], old[BEC_EPT];
+    double phv[BEC_EPT];
+    // phase 1: loads of the epilogue
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int idx = threadIdx.x + j * blockDim.x;
+      bel[j] = -1;
+      if (idx < nel) {
+        int cc, e;
+        if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+        const int l = l0 + cc;
+        const int b = l / nl;
+        if (l < total && inst_active(b)) {
+          bel[j] = b;
+          adr[j] = (long long)b * a.istride + X::base(l - b * nl, N) + e * strd;
+          if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? a.acc[adr[j]] : make_double2(0.0, 0.0);
+          if (KIND == KH_LAST) {
+            ps[j] = a.psi[adr[j]];
+            phv[j] = (a.local_on && a.phi) ? a.phi[adr[j]] : 0.0;
+            if (a.ydiff) old[j] = a.out[adr[j]];
+          }
+        }
+      }
+    }
+    // phase 2: results
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int b = bel[j];
+      if (b < 0) continue;
+      const int idx = threadIdx.x + j * blockDim.x;
+      int cc, e;
+      if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+      double2 u = lds[cc * N + e];
+      u.x *= invN;
+      u.y *= invN;
+      if (KE) {
+        a.hy_out[adr[j]] = u;
+        continue;
+      }
+      if (KIND == KH_FIRST) {
+        a.out[adr[j]] = u;
+        continue;
+      }
+      const KinScal& S = (b == b0) ? S0 : S1;
+      double2 p = acc[j];
+      if (a.acc_sc) {
+        p.x /= S.sc;
+        p.y /= S.sc;
+      }
+      if (KIND == KH_MID) {
+        a.out[adr[j]] = cadd(p, u);
+        continue;
+      }
+      const double2 psv = ps[j];
+      const double rho = cabs2(psv);
       double W = 0.0;
       if (a.local_on) {
         double ph = 0.0;
         if (a.phi) {
-          ph = a.phi[adr] * kin_scal(a, DS_PHISC, b, a.phisc);
-          if (a.phi_out) a.phi_out[adr] = ph;
+          ph = phv[j] * S.phisc;
+          if (a.phi_out) a.phi_out[adr[j]] = ph;
         }
-        const InstPhys ip = a.iphys[b];
-        W = local_V<AX>(li, e, a) + ip.beta2 * rho + ip.beta3 * rho * rho + ph;
+        W = local_V<AX>(l0 + cc - b * nl, e, a) + S.beta2 * rho + S.beta3 * rho * rho + ph;
       }
-      const double2 p = a.acc ? a.acc[adr] : make_double2(0.0, 0.0);
-      const double mu = kin_scal(a, DS_MU, b, a.mu);
+      const double mu = S.mu;
       double2 h;
-      h.x = p.x + u.x + W * ps.x - mu * ps.x;
-      h.y = p.y + u.y + W * ps.y - mu * ps.y;
-      if (a.ydiff) a.ydiff[adr] = csub(h, a.out[adr]);
-      a.out[adr] = h;
+      h.x = p.x + u.x + W * psv.x - mu * psv.x;
+      h.y = p.y + u.y + W * psv.y - mu * psv.y;
+      if (a.ydiff) a.ydiff[adr[j]] = csub(h, old[j]);
+      a.out[adr[j]] = h;
       hsum[0] += cabs2(h);
-      hsum[1] += ps.x * h.x + ps.y * h.y;
+      hsum[1] += psv.x * h.x + psv.y * h.y;
       hmax[0] = fmax(hmax[0], rho);
     }
   }
@@ -290,6 +407,26 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
 // ---------------------------------------------------------------------------
 // generic line FFT pass: pad-on-load / crop-on-store, optional multiplier
 // ---------------------------------------------------------------------------
+// The 16 inner products of one CG step (sums, scaled by dv on the host):
+//  0,1 <psi,z>   2 <psi,psi>   3,4 <psi,d>   5,6 <psi,g>   7 Re<z,g>
+//  8 Re<z,y>   9,10 <psi,y>   11 Re<g,d>   12 <g,g>   13 Re<g,y>
+//  14 Re<s,y>  15 <s,s>    (y = g - g_prev, s = psi - psi_prev, <u,v> = sum conj(u) v)
+__device__ __forceinline__ void dot_accum(double* v, double2 zi, double2 pi, double2 di, double2 gi, double2 y,
+                                          double2 s) {
+  v[0] += pi.x * zi.x + pi.y * zi.y;  v[1] += pi.x * zi.y - pi.y * zi.x;
+  v[2] += cabs2(pi);
+  v[3] += pi.x * di.x + pi.y * di.y;  v[4] += pi.x * di.y - pi.y * di.x;
+  v[5] += pi.x * gi.x + pi.y * gi.y;  v[6] += pi.x * gi.y - pi.y * gi.x;
+  v[7] += zi.x * gi.x + zi.y * gi.y;
+  v[8] += zi.x * y.x + zi.y * y.y;
+  v[9] += pi.x * y.x + pi.y * y.y;    v[10] += pi.x * y.y - pi.y * y.x;
+  v[11] += gi.x * di.x + gi.y * di.y;
+  v[12] += cabs2(gi);
+  v[13] += gi.x * y.x + gi.y * y.y;
+  v[14] += s.x * y.x + s.y * y.y;
+  v[15] += cabs2(s);
+}
+
 template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   extern __shared__ double2 smem[];
@@ -300,19 +437,31 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
-  if (!block_on(a.act, l0, lpw, nl, total)) return;
+  if (l0 >= total) return;
+  const int b0 = l0 / nl, b1 = min(l0 + lpw - 1, total - 1) / nl;
+  const bool on0 = act_on(a.act, b0), on1 = (b1 == b0) ? on0 : act_on(a.act, b1);
+  bool any = on0 || on1;
+  for (int b = b0 + 1; b < b1 && !any; ++b) any = act_on(a.act, b);
+  if (!any) return;
+  auto inst_active = [&](int b) { return b == b0 ? on0 : (b == b1 ? on1 : act_on(a.act, b)); };
   const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && act_on(a.act, mb);
-  const bool brow = act_on(a.act, l0 / nl);   // this block stores a partial row
+  const bool active = (l0 + c) < total && inst_active(mb);
+  const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
   stage_twiddles(twl, a.fp);
-  for (int idx = threadIdx.x; idx < lpw * L; idx += blockDim.x) {
-    int cc, e;
-    if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
-    const int l = l0 + cc;
-    const int b = l / nl;
-    double2 v = make_double2(0.0, 0.0);
-    if (l < total && act_on(a.act, b)) {
+  {
+    // every load of this thread's elements is issued before any is used
+    double2 v[BEC_EPT];
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int idx = threadIdx.x + j * blockDim.x;
+      v[j] = make_double2(0.0, 0.0);
+      if (idx >= lpw * L) continue;
+      int cc, e;
+      if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
+      const int l = l0 + cc;
+      const int b = l / nl;
+      if (l >= total || !inst_active(b)) continue;
       const int li = l - b * nl;
       const int o = li / a.ni, i = li - o * a.ni;
       const long long base = b * a.in_ib + o * a.in_so + i * a.in_si;
@@ -320,16 +469,23 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         const int half = L / 2;
         const int src = (e <= half) ? e : L - e;
         const double2 w = ((const double2*)a.in)[base + src * a.in_s];
-        v = (e <= half) ? w : make_double2(w.x, -w.y);
+        v[j] = (e <= half) ? w : make_double2(w.x, -w.y);
       } else {
         const int r = e - a.off_load;
         if (r >= 0 && r < a.n_load) {
-          if (LD == LD_C) v = ((const double2*)a.in)[base + r * a.in_s];
-          else v = make_double2(((const double*)a.in)[base + r * a.in_s], 0.0);
+          if (LD == LD_C) v[j] = ((const double2*)a.in)[base + r * a.in_s];
+          else v[j] = make_double2(((const double*)a.in)[base + r * a.in_s], 0.0);
         }
       }
     }
-    lds[cc * L + e] = v;
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int idx = threadIdx.x + j * blockDim.x;
+      if (idx >= lpw * L) continue;
+      int cc, e;
+      if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
+      lds[cc * L + e] = v[j];
+    }
   }
   __syncthreads();
   double2* s = lds + c * L;
@@ -369,25 +525,56 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
   }
   double dsum[1] = {0.0};
-  for (int idx = threadIdx.x; idx < lpw * a.n_store; idx += blockDim.x) {
-    int cc, e;
-    if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
-    const int l = l0 + cc;
-    const int b = l / nl;
-    if (l >= total || !act_on(a.act, b)) continue;
-    const int li = l - b * nl;
-    const int o = li / a.ni, i = li - o * a.ni;
-    const long long adr = b * a.out_ib + o * a.out_so + i * a.out_si + e * a.out_s;
-    const double2 v = lds[cc * L + a.off_store + e];
-    if (ST == ST_C) {
-      ((double2*)a.out)[adr] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
-    } else {
-      const double r = v.x * a.out_scale;
-      ((double*)a.out)[adr] = r;
-      if (a.dot) dsum[0] += r * a.dot[adr];
+  double zv[ST == ST_CDOT ? 16 : 1];
+#pragma unroll
+  for (int q = 0; q < (ST == ST_CDOT ? 16 : 1); ++q) zv[q] = 0.0;
+  {
+    const int nst = lpw * a.n_store;
+    long long adr[BEC_EPT];
+    double dv[BEC_EPT];
+    double2 fp[ST == ST_CDOT ? BEC_EPT : 1], fd[ST == ST_CDOT ? BEC_EPT : 1], fg[ST == ST_CDOT ? BEC_EPT : 1];
+    double2 fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int idx = threadIdx.x + j * blockDim.x;
+      adr[j] = -1;
+      if (idx >= nst) continue;
+      int cc, e;
+      if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
+      const int l = l0 + cc;
+      const int b = l / nl;
+      if (l >= total || !inst_active(b)) continue;
+      const int li = l - b * nl;
+      const int o = li / a.ni, i = li - o * a.ni;
+      adr[j] = b * a.out_ib + o * a.out_so + i * a.out_si + e * a.out_s;
+      if (ST == ST_RE && a.dot) dv[j] = a.dot[adr[j]];
+      if (ST == ST_CDOT) {
+        fp[j] = a.dpsi[adr[j]]; fd[j] = a.dd[adr[j]]; fg[j] = a.dg[adr[j]];
+        fy[j] = a.dy[adr[j]]; fs[j] = a.ds[adr[j]];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      if (adr[j] < 0) continue;
+      const int idx = threadIdx.x + j * blockDim.x;
+      int cc, e;
+      if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
+      const double2 v = lds[cc * L + a.off_store + e];
+      if (ST == ST_CDOT) {
+        const double2 zi = make_double2(v.x * a.out_scale, v.y * a.out_scale);
+        ((double2*)a.out)[adr[j]] = zi;
+        dot_accum(zv, zi, fp[j], fd[j], fg[j], fy[j], fs[j]);
+      } else if (ST == ST_C) {
+        ((double2*)a.out)[adr[j]] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
+      } else {
+        const double r = v.x * a.out_scale;
+        ((double*)a.out)[adr[j]] = r;
+        if (a.dot) dsum[0] += r * dv[j];
+      }
     }
   }
   if (ST == ST_RE && a.dot && brow) block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials, lb);
+  if (ST == ST_CDOT && brow) block_partials<16, 0>(zv, nullptr, a.partials, a.hpartials, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,10 +647,7 @@ __global__ __launch_bounds__(BEC_WG) void k_combine(double2* d, const double2* _
   }
 }
 
-// Inner products of the CG step (sums, scaled by dv on the host):
-//  0,1 <psi,z>   2 <psi,psi>   3,4 <psi,d>   5,6 <psi,g>   7 Re<z,g>
-//  8 Re<z,y>   9,10 <psi,y>   11 Re<g,d>   12 <g,g>   13 Re<g,y>
-//  14 Re<s,y>  15 <s,s>    (y = g - g_prev, s = psi - psi_prev, <u,v> = sum conj(u) v)
+// the CG inner products (dot_accum) of whole fields
 __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
                                                  const double2* __restrict__ psi,
                                                  const double2* __restrict__ d,
@@ -479,24 +663,35 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
   for (int q = 0; q < 16; ++q) v[q] = 0.0;
   ELEM_LOOP(n) {
     const long long j = off + i;
-    const double2 zi = z[j], pi = psi[j], di = d[j], gi = g[j], y = yv[j], s = sv[j];
-    v[0] += pi.x * zi.x + pi.y * zi.y;  v[1] += pi.x * zi.y - pi.y * zi.x;
-    v[2] += cabs2(pi);
-    v[3] += pi.x * di.x + pi.y * di.y;  v[4] += pi.x * di.y - pi.y * di.x;
-    v[5] += pi.x * gi.x + pi.y * gi.y;  v[6] += pi.x * gi.y - pi.y * gi.x;
-    v[7] += zi.x * gi.x + zi.y * gi.y;
-    v[8] += zi.x * y.x + zi.y * y.y;
-    v[9] += pi.x * y.x + pi.y * y.y;    v[10] += pi.x * y.y - pi.y * y.x;
-    v[11] += gi.x * di.x + gi.y * di.y;
-    v[12] += cabs2(gi);
-    v[13] += gi.x * y.x + gi.y * y.y;
-    v[14] += s.x * y.x + s.y * y.y;
-    v[15] += cabs2(s);
+    dot_accum(v, z[j], psi[j], d[j], g[j], yv[j], sv[j]);
   }
   block_partials<16, 0>(v, nullptr, partials, hpart, (long long)b * gridDim.x + blockIdx.x);
 }
 
 // g = h - mu psi, and sum |g|^2 (one instance)
+// after the Armijo controller: keep the lowest-residual state, and after a
+// failed line search restart the direction from the preconditioned gradient
+__global__ __launch_bounds__(BEC_WG) void k_post(const double2* __restrict__ psi, double2* best, double2* d,
+                                                 const double2* __restrict__ z, const double* __restrict__ tab,
+                                                 const unsigned* __restrict__ fl, long long n) {
+  const int b = blockIdx.y;
+  const unsigned f = fl[b];
+  const bool cp = (f & FL_BEST) != 0u, fail = (f & FL_FAIL) != 0u;
+  if (!cp && !fail) return;
+  const long long off = (long long)b * n;
+  const double* t = tab + (long long)b * DS_N;
+  const double az = t[DS_AZ], gre = t[DS_GRE], gim = t[DS_GIM];
+  ELEM_LOOP(n) {
+    const long long j = off + i;
+    const double2 p = psi[j];
+    if (cp) best[j] = p;
+    if (fail) {
+      const double2 zz = z[j];
+      d[j] = make_double2(az * zz.x + gre * p.x - gim * p.y, az * zz.y + gre * p.y + gim * p.x);
+    }
+  }
+}
+
 __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
                                                  const double2* __restrict__ psi, double mu,
                                                  double2* g, long long n, double* hpart) {
@@ -932,7 +1127,7 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   a.lpw = pick_lpw(a.tpl, a.nl, total, !CONTIG);
   const int blocks = xcd_grid(total, a.lpw);
   const size_t shm = (size_t)(a.lpw + 1) * a.fp.L * sizeof(double2);
-  const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL;
+  const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL + (ST == ST_CDOT ? 5 : 0);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (a.act.fl) a.ucount = prof_counter(id);
   if (prof_events(id, &e0, &e1, a.act.fl ? 0 : __builtin_popcountll(a.act.m & lowmask(a.nb))))
@@ -952,6 +1147,7 @@ int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a,
   BEC_P(1, LD_C, DIR_FWD, MUL_NONE, ST_C)
   BEC_P(0, LD_C, DIR_FWD, MUL_NONE, ST_C)
   BEC_P(1, LD_C, DIR_INV, MUL_NONE, ST_C)
+  BEC_P(1, LD_C, DIR_INV, MUL_NONE, ST_CDOT)
   BEC_P(0, LD_C, DIR_INV, MUL_NONE, ST_C)
   // preconditioner middle pass
   BEC_P(0, LD_C, DIR_FMI, MUL_PRE2, ST_C)
@@ -1016,6 +1212,10 @@ void launch_scale(const double2* in, double2* out, const double* divs, long long
 }
 void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s) {
   hipLaunchKernelGGL(k_rho, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, rho, n);
+}
+void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
+                 const unsigned* fl, long long n, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, best, d, z, tab, fl, n);
 }
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);

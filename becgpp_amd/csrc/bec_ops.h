@@ -69,8 +69,15 @@ inline const double* hres(const Plan& p, int b) { return p.hred + (long long)b *
 int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot);
 // kinetic/L_z/local sums of psi - t[b] d into hred[b][R_EFIRST..], [R_EREST..];
 // rho_out (optional) receives |psi - t d|^2
+// Fusions of the device-driven loop into the first energy pass: instances
+// flagged comb_bit first form d = az z + ad d + ag g + gam psi (coefficients
+// from the table); hy_out receives that pass's share of H (psi - t d).
+struct EnergyFuse {
+  double2* hy_out;
+  const double2* z; const double2* g; double2* d_out; unsigned comb_bit;
+};
 EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
-                           double* rho_out);
+                           double* rho_out, const EnergyFuse* fuse = nullptr);
 // assemble instance b's Energy from hred (after plan_sync); sgrav = sum(phi rho)
 // with phi the potential as stored (its scale is passed as gscale)
 Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool normalise, double gscale);
@@ -89,8 +96,19 @@ struct HArgs {
   bool reduce;
 };
 int run_H(Plan& p, const Batch& B, const HArgs& h);   // returns rows per instance of the last pass
-// z = IFFT(FFT(g) / (k^2/2 + sigma[b])) with sigma read from p.sigma (device)
-void run_precond(Plan& p, const Batch& B, const double2* g, double2* out);
+// H of the accepted trial (psi - t d) / sc in the device-driven loop: the first
+// axis' share is in p.w1 from the trial's energy pass (EnergyFuse::hy_out),
+// the remaining axes are transformed here; psi is updated in place (change to
+// sdiff), out receives H psi - mu psi (change to ydiff).
+int run_H_accept_dev(Plan& p, const Batch& B, const HArgs& h);
+// z = IFFT(FFT(g) / (k^2/2 + sigma[b])) with sigma read from p.sigma (device).
+// With dots, the last pass also accumulates the 16 CG inner products of z
+// with (psi, d, g, y, s) into device rows (part_of(C_DOTS)); returns their
+// rows per instance (0 without dots).
+struct DotFields {
+  const double2* psi; const double2* d; const double2* g; const double2* y; const double2* s;
+};
+int run_precond(Plan& p, const Batch& B, const double2* g, double2* out, const DotFields* dots = nullptr);
 
 void solver_release(Plan& p);
 void bec_set_error(const char* msg);

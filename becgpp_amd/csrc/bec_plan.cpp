@@ -280,7 +280,7 @@ static KinArgs kin_base(const Plan& p, const Batch& B) {
 static const int kAxes2[2] = {A2Y, A2X}, kAxes3[3] = {A3Z, A3Y, A3X};
 
 EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
-                           double* rho_out) {
+                           double* rho_out, const EnergyFuse* fuse) {
   const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
   EnergyRows er{0, 0, 0};
   for (int k = 0; k < p.ndim; ++k) {
@@ -297,6 +297,15 @@ EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const do
     if (B.dev) a.partials = part_of(p, site);
     else a.hpartials = hpart_of(p, site);
     if (k == 0) a.rho_out = rho_out;
+    if (k == 0 && fuse) {
+      a.hy_out = fuse->hy_out;
+      a.kf = 0.5;
+      a.rf_omega = 1;
+      a.z = fuse->z;
+      a.g = fuse->g;
+      a.d_out = fuse->d_out;
+      a.comb_bit = fuse->comb_bit;
+    }
     const int rows = launch_kin(ax[k], k == 0 ? KE_FIRST : KE_REST, a, p.stream);
     (k == 0 ? er.r0 : (k == 1 ? er.r1 : er.r2)) = rows;
     if (B.dev) continue;
@@ -388,7 +397,47 @@ int run_H(Plan& p, const Batch& B, const HArgs& h) {
   return last_rows;
 }
 
-void run_precond(Plan& p, const Batch& B, const double2* g, double2* out) {
+int run_H_accept_dev(Plan& p, const Batch& B, const HArgs& h) {
+  const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
+  int rows = 0;
+  for (int k = 1; k < p.ndim; ++k) {
+    const bool last = (k == p.ndim - 1);
+    KinArgs a = kin_base(p, B);
+    a.kf = 0.5;
+    a.rf_omega = 1;
+    a.psi = h.psi;
+    a.d = h.d;
+    a.tab = B.tab;
+    a.tabm = (1u << DS_T) | (1u << DS_SC);
+    a.acc = p.w1;
+    a.acc_sc = (k == 1) ? 1 : 0;     // the trial's share is of the unnormalised state
+    if (last) {
+      a.out = h.out;
+      a.psi_w = h.psi_w;
+      a.sdiff = h.sdiff;
+      a.local_on = 1;
+      a.phi = h.phi;
+      a.phi_out = h.phi_out;
+      a.tabm |= (1u << DS_MU) | (1u << DS_PHISC);
+      a.ydiff = h.ydiff;
+      a.partials = part_of(p, C_HLAST);
+    } else {
+      a.out = p.w1;
+    }
+    rows = launch_kin(ax[k], last ? KH_LAST : KH_MID, a, p.stream);
+  }
+  p.hlast_rows = rows;
+  return rows;
+}
+
+static int last_precond_pass(Plan& p, PassArgs c, const DotFields* dots) {
+  if (!dots) return (launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, c, p.stream), 0);
+  c.dpsi = dots->psi; c.dd = dots->d; c.dg = dots->g; c.dy = dots->y; c.ds = dots->s;
+  c.partials = part_of(p, C_DOTS);
+  return launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_CDOT, c, p.stream);
+}
+
+int run_precond(Plan& p, const Batch& B, const double2* g, double2* out, const DotFields* dots) {
   const int N = p.N;
   const long long n = p.n;
   const double inv_n = 1.0 / (double)p.n;
@@ -403,7 +452,7 @@ void run_precond(Plan& p, const Batch& B, const double2* g, double2* out) {
     launch_pass(0, LD_C, DIR_FMI, MUL_PRE2, ST_C, b, p.stream);
     PassArgs c = a;
     c.in = p.w1; c.out = out; c.out_scale = inv_n;
-    launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, c, p.stream);
+    return last_precond_pass(p, c, dots);
   } else {
     const long long N2 = (long long)N * N;
     PassArgs z = pass_base(p, p.fpN, p.lcN, B);
@@ -421,7 +470,7 @@ void run_precond(Plan& p, const Batch& B, const double2* g, double2* out) {
     launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, y, p.stream);
     PassArgs zi = z;
     zi.in = p.w1; zi.out = out; zi.out_scale = inv_n;
-    launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_C, zi, p.stream);
+    return last_precond_pass(p, zi, dots);
   }
 }
 

@@ -72,6 +72,8 @@ void prof_post(int id, hipStream_t s) {
   ++g_used;
 }
 
+bool prof_active() { return g_watch != 0; }
+
 unsigned long long* prof_counter(int id) {
   if (g_watch == 0 || id != g_watch) return nullptr;
   return g_dcount;

@@ -48,6 +48,11 @@ static void dev_release(Plan& p) {
   for (void* q : d)
     if (q) (void)hipFree(q);
   if (p.hdone) (void)hipHostFree(p.hdone);
+  if (p.hvlog) (void)hipHostFree(p.hvlog);
+  p.hvlog = nullptr;
+  p.hvlog_dev = nullptr;
+  if (p.cstream) (void)hipStreamDestroy(p.cstream);
+  p.cstream = nullptr;
   p.dinst = nullptr; p.flags = nullptr; p.dtab = nullptr; p.dtrace = nullptr; p.dtrace_cap = 0;
   p.hdone = nullptr; p.hdone_dev = nullptr;
 }
@@ -61,7 +66,11 @@ static void dev_alloc(Plan& p, int nb, int trace_cap) {
     HIPCHK(hipMalloc(&p.dtab, sizeof(double) * DS_N * BEC_BMAX));
     HIPCHK(hipHostMalloc(&p.hdone, sizeof(int) * BEC_BMAX, hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer((void**)&p.hdone_dev, p.hdone, 0));
+    const size_t nv = (size_t)BEC_BMAX * VLOG_CAP * VLOG_REC + BEC_BMAX;
+    HIPCHK(hipHostMalloc(&p.hvlog, sizeof(double) * nv, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&p.hvlog_dev, p.hvlog, 0));
   }
+  if (!p.cstream) HIPCHK(hipStreamCreateWithFlags(&p.cstream, hipStreamNonBlocking));
   if (trace_cap > p.dtrace_cap) {
     if (p.dtrace) (void)hipFree(p.dtrace);
     p.dtrace = nullptr;
@@ -101,6 +110,24 @@ struct Inst {
 static Batch mask_batch(int nb, unsigned long long m) { return Batch{nb, m}; }
 
 static int lmin_of(int tpl) { return tpl >= 64 ? 1 : 64 / tpl; }
+
+// Prints the progress records the controllers wrote since the last call.
+static void vlog_flush(const Plan& p, int nb, std::vector<int>& printed) {
+  const double* cnt = p.hvlog + (size_t)BEC_BMAX * VLOG_CAP * VLOG_REC;
+  for (int b = 0; b < nb; ++b) {
+    const int n = (int)((volatile const double*)cnt)[b];
+    for (int k = std::max(printed[b], n - VLOG_CAP); k < n; ++k) {
+      const double* r = p.hvlog + ((size_t)b * VLOG_CAP + (size_t)(k % VLOG_CAP)) * VLOG_REC;
+      if ((int)r[0] == VLOG_PROGRESS)
+        printf("  [gs%s%d] it=%d E=%.8f res=%.2e best=%.2e step=%.2e\n", nb > 1 ? " " : "", nb > 1 ? b : 0,
+               (int)r[1], r[2], r[3], r[4], r[5]);
+      else
+        printf("[gs] line search stalled it=%d: E=%.10g res=%.3e best_res=%.3e\n", (int)r[1], r[2], r[3], r[4]);
+    }
+    printed[b] = n;
+  }
+  fflush(stdout);
+}
 
 // The loop with its scalar logic on the device (bec_ctrl.hip): the host
 // launches whole steps and polls the instances' done flags one chunk of
@@ -183,27 +210,37 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
   c.clk_hz = 1e3 * (double)std::max(khz, 1);
   c.hdone = p.hdone_dev;
+  c.vlog = cf->verbose ? p.hvlog_dev : nullptr;
+  std::vector<int> printed(nb, 0);
+  if (cf->verbose)
+    for (int b = 0; b < nb; ++b) p.hvlog[(size_t)BEC_BMAX * VLOG_CAP * VLOG_REC + b] = 0.0;
   const double floor_ = cf->precond_shift_min;
 
   auto step = [&]() {
+    hipStream_t st = p.stream;           // the capture stream while the loop runs
     const Batch BD = dev_batch(p, nb, FL_DIR);
     launch_sigma(p, nb, act_of(BD), PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2, floor_,
-                 s);
-    run_precond(p, BD, G, Z);
-    c.n_dots = launch_dots(Z, PSI, D, G, Y, SD, n, nb, act_of(BD), part_of(p, C_DOTS), nullptr, s);
-    launch_ctrl_dir(c, s);
-    launch_copy(PSI, BEST, n, nb, act_of(dev_batch(p, nb, FL_BEST)), s);
-    CombArgs none;
-    memset(&none, 0, sizeof(none));
-    launch_combine(D, Z, G, PSI, none, p.dtab, n, nb, act_of(dev_batch(p, nb, FL_COMB)), s);
+                 st);
+    // z = P^-1 g, with the CG inner products accumulated by its last pass
+    const DotFields df{PSI, D, G, Y, SD};
+    c.n_dots = run_precond(p, BD, G, Z, &df);
+    launch_ctrl_dir(c, st);
+    // trial energy of psi - t d; instances with a new direction form d in the
+    // same pass, and the pass keeps its share of H for the accepting step
+    EnergyFuse fu;
+    fu.hy_out = p.w1;
+    fu.z = Z;
+    fu.g = G;
+    fu.d_out = D;
+    fu.comb_bit = FL_COMB;
     const EnergyRows er = run_energy_sums(p, dev_batch(p, nb, FL_SEARCH), PSI, D, nullptr,
-                                          lr_any ? p.rho : nullptr);
+                                          lr_any ? p.rho : nullptr, &fu);
     c.n_e0 = er.r0;
     c.n_e1 = er.r1;
     c.n_e2 = er.r2;
     if (lr_any) c.n_gr = run_conv(p, 0, dev_batch(p, nb, FL_SLR), p.rho, 1.0, phit, p.rho);
-    launch_ctrl_armijo(c, s);
-    launch_combine(D, Z, nullptr, PSI, none, p.dtab, n, nb, act_of(dev_batch(p, nb, FL_FAIL)), s);
+    launch_ctrl_armijo(c, st);
+    launch_post(PSI, BEST, D, Z, p.dtab, p.flags, n, nb, st);
     HArgs a;
     memset(&a, 0, sizeof(a));
     a.psi = PSI;
@@ -218,38 +255,74 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     a.out = G;
     a.ydiff = Y;
     a.reduce = true;
-    c.n_hl = run_H(p, dev_batch(p, nb, FL_ACC), a);
+    c.n_hl = run_H_accept_dev(p, dev_batch(p, nb, FL_ACC), a);
   };
 
+  // Steps run on the plan's capture stream, after the work queued so far on
+  // the caller's stream.  A chunk of K steps is captured once into a hipGraph
+  // and replayed (one launch per chunk); while a kernel is being timed the
+  // steps are launched directly, so that its events bracket each launch.
+  hipStream_t cs = p.cstream;
   hipEvent_t ev[2] = {nullptr, nullptr};
   HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  HIPCHK(hipEventRecord(ev[1], s));
+  HIPCHK(hipStreamWaitEvent(cs, ev[1], 0));
+  const int K = 4;                    // steps per polling chunk
+  const bool use_graph = !prof_active();
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  p.stream = cs;
   // every step resolves a trial of each running instance: maxit iterations of
   // at most linesearch_max trials bound the number of steps
   const long long max_steps = (long long)(cf->maxit + 2) * (long long)(std::max(1, cf->linesearch_max) + 1) + 64;
-  const int K = 4;                    // steps per polling chunk
   try {
+    if (use_graph) {
+      HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int k = 0; k < K; ++k) step();
+      } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(cs, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(cs, &graph));
+      HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
     for (long long chunk = 0;; ++chunk) {
-      for (int k = 0; k < K; ++k) step();
+      if (use_graph) {
+        HIPCHK(hipGraphLaunch(gexec, cs));
+      } else {
+        for (int k = 0; k < K; ++k) step();
+      }
       HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(ev[chunk & 1], s));
+      HIPCHK(hipEventRecord(ev[chunk & 1], cs));
       if (chunk > 0) {
         HIPCHK(hipEventSynchronize(ev[(chunk - 1) & 1]));
+        if (cf->verbose) vlog_flush(p, nb, printed);
         int ndone = 0;
         for (int b = 0; b < nb; ++b) ndone += ((volatile int*)p.hdone)[b] != 0;
         if (ndone == nb) break;
       }
       if ((chunk + 1) * K > max_steps) throw BecError(2, "internal: the device loop did not finish");
     }
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipStreamSynchronize(cs));
   } catch (...) {
-    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(cs);
+    p.stream = s;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     throw;
   }
+  p.stream = s;
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
+  if (cf->verbose) vlog_flush(p, nb, printed);
   HIPCHK(hipMemcpy(di.data(), p.dinst, sizeof(DevInst) * nb, hipMemcpyDeviceToHost));
   for (int b = 0; b < nb; ++b) {
     Inst& q = I[b];

@@ -122,7 +122,11 @@ def test_multiseed_matches_oracle():
     assert len(Ed) == 4
     for a, b in zip(Ed, Eo):
         assert _close(a, b, 1e-8), (Ed, Eo)
-    assert od["seed_used"] == oo["seed_used"]
+    assert _close(od["E"], oo["E"], 1e-8)
+    # seeds that reach the same minimum tie to round-off: either may be chosen
+    cycle = ("triangular", "tf", "gaussian")
+    tied = {cycle[j % 3] for j, e in enumerate(Eo) if _close(e, min(Eo), 1e-8)}
+    assert od["seed_used"] in tied, (od["seed_used"], tied, Eo)
 
 
 def test_batch_argument_errors():

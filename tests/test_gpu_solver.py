@@ -111,5 +111,8 @@ def test_device_and_host_loops_agree(kw):
     _, _, oh = B.ground_state(dict(c, host_loop=True), Gd, psi0, verbose=False)
     for (_, i1, E1, _), (_, i2, E2, _) in list(zip(od["trace"], oh["trace"]))[:10]:
         assert i1 == i2 and abs(E1 - E2) <= 1e-12 * abs(E2)
-    for k in ("E", "mu", "Lz"):
-        assert abs(od[k] - oh[k]) <= 1e-9 * max(1.0, abs(oh[k])), (k, od[k], oh[k])
+    # two converged states at res_tol 1e-8: E agrees to second order, mu and
+    # L_z to first order in the residual
+    assert abs(od["E"] - oh["E"]) <= 1e-9 * max(1.0, abs(oh["E"])), (od["E"], oh["E"])
+    for k in ("mu", "Lz"):
+        assert abs(od[k] - oh[k]) <= 1e-8 * max(1.0, abs(oh[k])), (k, od[k], oh[k])
