@@ -147,16 +147,6 @@ __device__ __forceinline__ KinScal kin_scal_all(const KinArgs& a, int b) {
   return k;
 }
 
-template <int AX>
-__device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
-  using X = Ax<AX>;
-  int ix, iy, iz;
-  X::coords(li, e, a.N, ix, iy, iz);
-  const double xx = xcoord(ix, a.N, a.dx), yy = xcoord(iy, a.N, a.dx);
-  double r2 = xx * xx + yy * yy;
-  if (X::ndim == 3) { const double zz = xcoord(iz, a.N, a.dx); r2 += zz * zz; }
-  return trap_value(r2, a.tc, a.s_mode, a.s_half);
-}
 
 // Kinetic + rotation line pass.  Along an axis with wavenumber k the operator
 // T - Omega L_z acts as the multiplier kf k^2 + rf c k, where c = +x for the y
@@ -171,10 +161,58 @@ __device__ __forceinline__ double local_V(int li, int e, const KinArgs& a) {
 //         the local potential (V + b2 rho + b3 rho^2 + phi) psi - mu psi.  The
 //         pass that loads a transformed state writes it back (psi_w, and its
 //         change to sdiff).
-// A thread owns at most BEC_EPT elements of the block's lines (lpw N <=
-// BEC_EPT blockDim, since N <= BEC_EPT tpl): it issues all of their loads
-// before using any, so the memory latency is paid once per phase.
+// Loads and stores: thread t owns one line of the block -- its FFT line
+// t / tpl on a contiguous axis, column t mod lpw on a strided one (adjacent
+// threads read adjacent columns) -- and its elements e0 + j tpl, j < BEC_EPT
+// (N <= BEC_EPT tpl).  The line's addressing is computed once per thread, and
+// all of a phase's loads are issued before any is used.
 #define BEC_EPT 8
+struct LineOwn {
+  int cc, e0;                 // block line and first element of the thread
+};
+template <bool CONTIG>
+__device__ __forceinline__ LineOwn line_owner(int tpl, int lpw) {
+  LineOwn o;
+  if (CONTIG) {
+    o.cc = threadIdx.x >> (__ffs(tpl) - 1);
+    o.e0 = threadIdx.x & (tpl - 1);
+  } else {
+    o.cc = threadIdx.x & (lpw - 1);
+    o.e0 = threadIdx.x >> (__ffs(lpw) - 1);
+  }
+  return o;
+}
+
+// squared coordinates of a line's fixed axes; the element axis is filled per element
+template <int AX>
+__device__ __forceinline__ void line_sq(int li, int N, double dx, double* c2) {
+  using X = Ax<AX>;
+  int ix, iy, iz;
+  X::coords(li, N / 2, N, ix, iy, iz);
+  const double xx = xcoord(ix, N, dx), yy = xcoord(iy, N, dx), zz = xcoord(iz, N, dx);
+  c2[0] = xx * xx;
+  c2[1] = yy * yy;
+  c2[2] = (X::ndim == 3) ? zz * zz : 0.0;
+}
+// element axis of each mode: 0 = x, 1 = y, 2 = z
+template <int AX> struct AxE;
+template <> struct AxE<A2Y> { static constexpr int k = 1; };
+template <> struct AxE<A2X> { static constexpr int k = 0; };
+template <> struct AxE<A3Z> { static constexpr int k = 2; };
+template <> struct AxE<A3Y> { static constexpr int k = 1; };
+template <> struct AxE<A3X> { static constexpr int k = 0; };
+// trap potential at element e of a line, r^2 summed as x^2 + y^2 [+ z^2]
+template <int AX>
+__device__ __forceinline__ double line_V(const double* c2, int e, const KinArgs& a) {
+  constexpr int K = AxE<AX>::k;
+  const double ee = xcoord(e, a.N, a.dx);
+  const double x2 = (K == 0) ? ee * ee : c2[0];
+  const double y2 = (K == 1) ? ee * ee : c2[1];
+  double r2 = x2 + y2;
+  if (Ax<AX>::ndim == 3) r2 += (K == 2) ? ee * ee : c2[2];
+  return trap_value(r2, a.tc, a.s_mode, a.s_half);
+}
+
 template <int AX, int KIND>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 smem[];
@@ -192,96 +230,94 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   // instances of the block's lines: b0 .. b1 (one when lpw divides nl)
   const int b0 = l0 / nl, b1 = min(l0 + lpw - 1, total - 1) / nl;
   const bool on0 = act_on(a.act, b0), on1 = (b1 == b0) ? on0 : act_on(a.act, b1);
-  bool any = on0 || on1;
-  for (int b = b0 + 1; b < b1 && !any; ++b) any = act_on(a.act, b);
-  if (!any) return;
-  auto inst_active = [&](int b) { return b == b0 ? on0 : (b == b1 ? on1 : act_on(a.act, b)); };
-  // a workgroup's lines span instances b0 and b1 only (lpw <= nl)
+  if (!on0 && !on1) return;            // lpw <= nl: no instance between b0 and b1
+  // a workgroup's lines span instances b0 and b1 only
   const KinScal S0 = kin_scal_all(a, b0);
   const KinScal S1 = (b1 == b0) ? S0 : kin_scal_all(a, b1);
   const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && inst_active(mb);
+  const bool active = (l0 + c) < total && (mb == b0 ? on0 : on1);
   const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
   stage_twiddles(twl, a.fp);
-  const int nel = lpw * N;
+  // the line this thread loads and stores
+  const LineOwn ow = line_owner<X::contig>(tpl, lpw);
+  const int ol = l0 + ow.cc;
+  const int ob = ol / nl;
+  const bool oon = ol < total && (ob == b0 ? on0 : on1);
+  const int oli = ol - ob * nl;
+  const long long obase = (long long)ob * a.istride + X::base(oli, N);
+  const KinScal& OS = (ob == b0) ? S0 : S1;
+  double c2[3] = {0.0, 0.0, 0.0};
+  if (KIND == KE_FIRST || KIND == KH_LAST) line_sq<AX>(oli, N, a.dx, c2);
+  double2* olds = lds + ow.cc * N;
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
   {
     // phase 1: issue every load of this thread's elements
-    long long adr[BEC_EPT];
-    int bel[BEC_EPT];
     double2 p0[BEC_EPT], dd[BEC_EPT];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int idx = threadIdx.x + j * blockDim.x;
-      bel[j] = -1;
-      if (idx < nel) {
-        int cc, e;
-        if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
-        const int l = l0 + cc;
-        const int b = l / nl;
-        if (l < total && inst_active(b)) {
-          bel[j] = b;
-          adr[j] = (long long)b * a.istride + X::base(l - b * nl, N) + e * strd;
-          p0[j] = a.psi[adr[j]];
-          if (a.d) dd[j] = a.d[adr[j]];
-        }
+      const int e = ow.e0 + j * tpl;
+      if (oon && e < N) {
+        const long long adr = obase + e * strd;
+        p0[j] = a.psi[adr];
+        if (a.d) dd[j] = a.d[adr];
       }
     }
     // new search directions (device-driven loop): d = az z + ad d + ag g + gam psi
-    if (KIND == KE_FIRST && a.d_out) {
-      // coefficients of b0 and b1, and whether they form a new direction
-      const double* t0 = a.tab + (long long)b0 * DS_N;
-      const double* t1 = a.tab + (long long)b1 * DS_N;
-      const bool c0 = (a.act.fl[b0] & a.comb_bit) != 0u, c1 = (a.act.fl[b1] & a.comb_bit) != 0u;
-      const double cf0[5] = {t0[DS_AZ], t0[DS_AD], t0[DS_AG], t0[DS_GRE], t0[DS_GIM]};
-      const double cf1[5] = {t1[DS_AZ], t1[DS_AD], t1[DS_AG], t1[DS_GRE], t1[DS_GIM]};
+    if (KIND == KE_FIRST && a.d_out && oon && (a.act.fl[ob] & a.comb_bit)) {
+      const double* t = a.tab + (long long)ob * DS_N;
+      const double az = t[DS_AZ], ad = t[DS_AD], ag = t[DS_AG], gre = t[DS_GRE], gim = t[DS_GIM];
+      double2 zz[BEC_EPT], gg[BEC_EPT];
 #pragma unroll
       for (int j = 0; j < BEC_EPT; ++j) {
-        const int b = bel[j];
-        if (b < 0 || !((b == b0) ? c0 : c1)) continue;
-        const double* cf = (b == b0) ? cf0 : cf1;
-        const double az = cf[0], ad = cf[1], ag = cf[2], gre = cf[3], gim = cf[4];
+        const int e = ow.e0 + j * tpl;
+        if (e < N) {
+          const long long adr = obase + e * strd;
+          if (az != 0.0) zz[j] = a.z[adr];
+          if (ag != 0.0) gg[j] = a.g[adr];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < BEC_EPT; ++j) {
+        const int e = ow.e0 + j * tpl;
+        if (e >= N) continue;
         double2 r = make_double2(gre * p0[j].x - gim * p0[j].y, gre * p0[j].y + gim * p0[j].x);
-        if (az != 0.0) { const double2 zz = a.z[adr[j]]; r.x += az * zz.x; r.y += az * zz.y; }
+        if (az != 0.0) { r.x += az * zz[j].x; r.y += az * zz[j].y; }
         if (ad != 0.0) { r.x += ad * dd[j].x; r.y += ad * dd[j].y; }
-        if (ag != 0.0) { const double2 gg = a.g[adr[j]]; r.x += ag * gg.x; r.y += ag * gg.y; }
+        if (ag != 0.0) { r.x += ag * gg[j].x; r.y += ag * gg[j].y; }
         dd[j] = r;
-        a.d_out[adr[j]] = r;
+        a.d_out[obase + e * strd] = r;
       }
     }
     // phase 2: the loaded values
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int idx = threadIdx.x + j * blockDim.x;
-      if (idx >= nel) continue;
-      int cc, e;
-      if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
+      const int e = ow.e0 + j * tpl;
+      if (e >= N) continue;
       double2 v = make_double2(0.0, 0.0);
-      const int b = bel[j];
-      if (b >= 0) {
-        const KinScal& S = (b == b0) ? S0 : S1;
+      if (oon) {
+        const long long adr = obase + e * strd;
         v = p0[j];
         if (a.d) {
-          v.x = v.x - S.t * dd[j].x;
-          v.y = v.y - S.t * dd[j].y;
+          v.x = v.x - OS.t * dd[j].x;
+          v.y = v.y - OS.t * dd[j].y;
         }
-        if (S.sc != 1.0) { v.x = v.x / S.sc; v.y = v.y / S.sc; }
+        if (OS.sc != 1.0) { v.x = v.x / OS.sc; v.y = v.y / OS.sc; }
         if (KIND == KH_FIRST || KIND == KH_LAST) {
-          if (a.psi_w) a.psi_w[adr[j]] = v;
-          if (a.sdiff) a.sdiff[adr[j]] = csub(v, p0[j]);
+          if (a.psi_w) a.psi_w[adr] = v;
+          if (a.sdiff) a.sdiff[adr] = csub(v, p0[j]);
         }
         if (KIND == KE_FIRST) {
           const double rho = cabs2(v);
-          const double V = local_V<AX>(l0 + cc - b * nl, e, a);
+          const double V = line_V<AX>(c2, e, a);
           loc[0] += rho;
           loc[1] += V * rho;
           loc[2] += rho * rho;
           loc[3] += rho * rho * rho;
-          if (a.rho_out) a.rho_out[adr[j]] = rho;
+          if (a.rho_out) a.rho_out[adr] = rho;
         }
       }
-      lds[cc * N + e] = v;
+      olds[e] = v;
     }
   }
   __syncthreads();
@@ -322,60 +358,46 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
-  {
-    long long adr[BEC_EPT];
-    int bel[BEC_EPT];
+  if (oon) {
     double2 ps[BEC_EPT], acc[BEC_EPT], old[BEC_EPT];
     double phv[BEC_EPT];
     // phase 1: loads of the epilogue
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int idx = threadIdx.x + j * blockDim.x;
-      bel[j] = -1;
-      if (idx < nel) {
-        int cc, e;
-        if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
-        const int l = l0 + cc;
-        const int b = l / nl;
-        if (l < total && inst_active(b)) {
-          bel[j] = b;
-          adr[j] = (long long)b * a.istride + X::base(l - b * nl, N) + e * strd;
-          if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? a.acc[adr[j]] : make_double2(0.0, 0.0);
-          if (KIND == KH_LAST) {
-            ps[j] = a.psi[adr[j]];
-            phv[j] = (a.local_on && a.phi) ? a.phi[adr[j]] : 0.0;
-            if (a.ydiff) old[j] = a.out[adr[j]];
-          }
-        }
+      const int e = ow.e0 + j * tpl;
+      if (e >= N) continue;
+      const long long adr = obase + e * strd;
+      if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? a.acc[adr] : make_double2(0.0, 0.0);
+      if (KIND == KH_LAST) {
+        ps[j] = a.psi[adr];
+        phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
+        if (a.ydiff) old[j] = a.out[adr];
       }
     }
     // phase 2: results
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int b = bel[j];
-      if (b < 0) continue;
-      const int idx = threadIdx.x + j * blockDim.x;
-      int cc, e;
-      if (X::contig) { cc = idx / N; e = idx - cc * N; } else { cc = idx % lpw; e = idx / lpw; }
-      double2 u = lds[cc * N + e];
+      const int e = ow.e0 + j * tpl;
+      if (e >= N) continue;
+      const long long adr = obase + e * strd;
+      double2 u = olds[e];
       u.x *= invN;
       u.y *= invN;
       if (KE) {
-        a.hy_out[adr[j]] = u;
+        a.hy_out[adr] = u;
         continue;
       }
       if (KIND == KH_FIRST) {
-        a.out[adr[j]] = u;
+        a.out[adr] = u;
         continue;
       }
-      const KinScal& S = (b == b0) ? S0 : S1;
       double2 p = acc[j];
       if (a.acc_sc) {
-        p.x /= S.sc;
-        p.y /= S.sc;
+        p.x /= OS.sc;
+        p.y /= OS.sc;
       }
       if (KIND == KH_MID) {
-        a.out[adr[j]] = cadd(p, u);
+        a.out[adr] = cadd(p, u);
         continue;
       }
       const double2 psv = ps[j];
@@ -384,17 +406,17 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       if (a.local_on) {
         double ph = 0.0;
         if (a.phi) {
-          ph = phv[j] * S.phisc;
-          if (a.phi_out) a.phi_out[adr[j]] = ph;
+          ph = phv[j] * OS.phisc;
+          if (a.phi_out) a.phi_out[adr] = ph;
         }
-        W = local_V<AX>(l0 + cc - b * nl, e, a) + S.beta2 * rho + S.beta3 * rho * rho + ph;
+        W = line_V<AX>(c2, e, a) + OS.beta2 * rho + OS.beta3 * rho * rho + ph;
       }
-      const double mu = S.mu;
+      const double mu = OS.mu;
       double2 h;
       h.x = p.x + u.x + W * psv.x - mu * psv.x;
       h.y = p.y + u.y + W * psv.y - mu * psv.y;
-      if (a.ydiff) a.ydiff[adr[j]] = csub(h, old[j]);
-      a.out[adr[j]] = h;
+      if (a.ydiff) a.ydiff[adr] = csub(h, old[j]);
+      a.out[adr] = h;
       hsum[0] += cabs2(h);
       hsum[1] += psv.x * h.x + psv.y * h.y;
       hmax[0] = fmax(hmax[0], rho);
@@ -404,9 +426,6 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     block_partials<2, 1>(hsum, hmax, a.partials, a.hpartials, lb);
 }
 
-// ---------------------------------------------------------------------------
-// generic line FFT pass: pad-on-load / crop-on-store, optional multiplier
-// ---------------------------------------------------------------------------
 // The 16 inner products of one CG step (sums, scaled by dv on the host):
 //  0,1 <psi,z>   2 <psi,psi>   3,4 <psi,d>   5,6 <psi,g>   7 Re<z,g>
 //  8 Re<z,y>   9,10 <psi,y>   11 Re<g,d>   12 <g,g>   13 Re<g,y>
@@ -449,42 +468,39 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
   stage_twiddles(twl, a.fp);
+  // the line this thread loads and stores, and its elements e0 + j tpl
+  const LineOwn ow = line_owner<CONTIG>(tpl, lpw);
+  const int ol = l0 + ow.cc;
+  const int ob = ol / nl;
+  const bool oon = ol < total && inst_active(ob);
+  const int oli = ol - ob * nl;
+  const int oo = oli / a.ni, oi = oli - oo * a.ni;
+  double2* olds = lds + ow.cc * L;
   {
-    // every load of this thread's elements is issued before any is used
+    const long long ibase = ob * a.in_ib + oo * a.in_so + oi * a.in_si;
     double2 v[BEC_EPT];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int idx = threadIdx.x + j * blockDim.x;
+      const int e = ow.e0 + j * tpl;
       v[j] = make_double2(0.0, 0.0);
-      if (idx >= lpw * L) continue;
-      int cc, e;
-      if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
-      const int l = l0 + cc;
-      const int b = l / nl;
-      if (l >= total || !inst_active(b)) continue;
-      const int li = l - b * nl;
-      const int o = li / a.ni, i = li - o * a.ni;
-      const long long base = b * a.in_ib + o * a.in_so + i * a.in_si;
+      if (!oon || e >= L) continue;
       if (LD == LD_HERM) {
         const int half = L / 2;
         const int src = (e <= half) ? e : L - e;
-        const double2 w = ((const double2*)a.in)[base + src * a.in_s];
+        const double2 w = ((const double2*)a.in)[ibase + src * a.in_s];
         v[j] = (e <= half) ? w : make_double2(w.x, -w.y);
       } else {
         const int r = e - a.off_load;
         if (r >= 0 && r < a.n_load) {
-          if (LD == LD_C) v[j] = ((const double2*)a.in)[base + r * a.in_s];
-          else v[j] = make_double2(((const double*)a.in)[base + r * a.in_s], 0.0);
+          if (LD == LD_C) v[j] = ((const double2*)a.in)[ibase + r * a.in_s];
+          else v[j] = make_double2(((const double*)a.in)[ibase + r * a.in_s], 0.0);
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int idx = threadIdx.x + j * blockDim.x;
-      if (idx >= lpw * L) continue;
-      int cc, e;
-      if (CONTIG) { cc = idx / L; e = idx - cc * L; } else { cc = idx % lpw; e = idx / lpw; }
-      lds[cc * L + e] = v[j];
+      const int e = ow.e0 + j * tpl;
+      if (e < L) olds[e] = v[j];
     }
   }
   __syncthreads();
@@ -528,47 +544,37 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   double zv[ST == ST_CDOT ? 16 : 1];
 #pragma unroll
   for (int q = 0; q < (ST == ST_CDOT ? 16 : 1); ++q) zv[q] = 0.0;
-  {
-    const int nst = lpw * a.n_store;
-    long long adr[BEC_EPT];
+  if (oon) {
+    const long long obase = ob * a.out_ib + oo * a.out_so + oi * a.out_si;
     double dv[BEC_EPT];
     double2 fp[ST == ST_CDOT ? BEC_EPT : 1], fd[ST == ST_CDOT ? BEC_EPT : 1], fg[ST == ST_CDOT ? BEC_EPT : 1];
     double2 fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int idx = threadIdx.x + j * blockDim.x;
-      adr[j] = -1;
-      if (idx >= nst) continue;
-      int cc, e;
-      if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
-      const int l = l0 + cc;
-      const int b = l / nl;
-      if (l >= total || !inst_active(b)) continue;
-      const int li = l - b * nl;
-      const int o = li / a.ni, i = li - o * a.ni;
-      adr[j] = b * a.out_ib + o * a.out_so + i * a.out_si + e * a.out_s;
-      if (ST == ST_RE && a.dot) dv[j] = a.dot[adr[j]];
+      const int e = ow.e0 + j * tpl;
+      if (e >= a.n_store) continue;
+      const long long adr = obase + e * a.out_s;
+      if (ST == ST_RE && a.dot) dv[j] = a.dot[adr];
       if (ST == ST_CDOT) {
-        fp[j] = a.dpsi[adr[j]]; fd[j] = a.dd[adr[j]]; fg[j] = a.dg[adr[j]];
-        fy[j] = a.dy[adr[j]]; fs[j] = a.ds[adr[j]];
+        fp[j] = a.dpsi[adr]; fd[j] = a.dd[adr]; fg[j] = a.dg[adr];
+        fy[j] = a.dy[adr]; fs[j] = a.ds[adr];
       }
     }
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      if (adr[j] < 0) continue;
-      const int idx = threadIdx.x + j * blockDim.x;
-      int cc, e;
-      if (CONTIG) { cc = idx / a.n_store; e = idx - cc * a.n_store; } else { cc = idx % lpw; e = idx / lpw; }
-      const double2 v = lds[cc * L + a.off_store + e];
+      const int e = ow.e0 + j * tpl;
+      if (e >= a.n_store) continue;
+      const long long adr = obase + e * a.out_s;
+      const double2 v = olds[a.off_store + e];
       if (ST == ST_CDOT) {
         const double2 zi = make_double2(v.x * a.out_scale, v.y * a.out_scale);
-        ((double2*)a.out)[adr[j]] = zi;
+        ((double2*)a.out)[adr] = zi;
         dot_accum(zv, zi, fp[j], fd[j], fg[j], fy[j], fs[j]);
       } else if (ST == ST_C) {
-        ((double2*)a.out)[adr[j]] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
+        ((double2*)a.out)[adr] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
       } else {
         const double r = v.x * a.out_scale;
-        ((double*)a.out)[adr[j]] = r;
+        ((double*)a.out)[adr] = r;
         if (a.dot) dsum[0] += r * dv[j];
       }
     }
