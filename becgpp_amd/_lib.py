@@ -71,8 +71,7 @@ SIGNATURES = {
                                               ctypes.POINTER(SolverCfg), ctypes.c_void_p,
                                               ctypes.POINTER(SolveOut)]),
     "bec_prof_watch": (ctypes.c_int, [ctypes.c_int]),
-    "bec_prof_read": (ctypes.c_int, [c_double_p, ctypes.POINTER(ctypes.c_longlong),
-                                      ctypes.POINTER(ctypes.c_longlong)]),
+    "bec_prof_read": (ctypes.c_int, [c_double_p, ctypes.POINTER(ctypes.c_longlong), c_double_p]),
 }
 
 _LIB = None

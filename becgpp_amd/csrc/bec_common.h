@@ -19,6 +19,9 @@ struct FftPlan {
   int L;                      // transform length (2^a 3^b)
   int nst;                    // number of Stockham stages
   int rad[BEC_MAX_STAGES];    // radix per stage (8, 4, 3 or 2)
+  int ns[BEC_MAX_STAGES];     // Ns of the stage: product of the earlier radices
+  int step[BEC_MAX_STAGES];   // twiddle stride L / (Ns R)
+  unsigned mag[BEC_MAX_STAGES];   // floor(2^32 / Ns) + 1: j / Ns = umulhi(j, mag) for j < 2^12
   const double2* tw;          // tw[m] = exp(-2 pi i m / L), m in [0, L)
 };
 
@@ -91,7 +94,7 @@ template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
 // tw is the twiddle table staged in LDS by the calling kernel.
 // Every thread of the workgroup must call it (it contains barriers).
 template <int R, int SIGN>
-__device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, int lane, int tpl,
+__device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, unsigned mag, int step, int lane, int tpl,
                                           const double2* __restrict__ tw, bool active) {
   constexpr int MAXB = (8 + R - 1) / R;
   const int nb = L / R;
@@ -108,12 +111,11 @@ __device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, int lane, i
   }
   __syncthreads();
   if (active) {
-    const int step = L / (Ns * R);
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
       const int j = lane + b * tpl;
       if (j < nb) {
-        const int k = j % Ns;
+        const int k = (Ns == 1) ? 0 : j - (int)__umulhi((unsigned)j, mag) * Ns;   // j mod Ns
         if (Ns > 1) {
 #pragma unroll
           for (int r = 1; r < R; ++r) {
@@ -137,14 +139,13 @@ __device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, int lane, i
 template <int SIGN>
 __device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const double2* twl, int lane, int tpl,
                                         bool active) {
-  int Ns = 1;
   for (int st = 0; st < p.nst; ++st) {
-    const int R = p.rad[st];
-    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
-    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
-    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
-    else fft_stage<2, SIGN>(s, p.L, Ns, lane, tpl, twl, active);
-    Ns *= R;
+    const int R = p.rad[st], Ns = p.ns[st], step = p.step[st];
+    const unsigned mag = p.mag[st];
+    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
   }
 }
 

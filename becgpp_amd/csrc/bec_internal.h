@@ -80,7 +80,8 @@ struct PassArgs {
   const double* dot;                   // optional: rows of sum(out_re * dot) (dot laid out like out)
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
-  unsigned long long* ucount;          // measurement: instances covered (nullable)
+  unsigned long long* ucount;          // measurement: algorithmic bytes of the instances covered
+  unsigned long long ubytes;           //   (ubytes per instance, added once per instance; nullable)
   const double2* dpsi; const double2* dd; const double2* dg; const double2* dy; const double2* ds;  // ST_CDOT
 };
 
@@ -105,7 +106,8 @@ struct KinArgs {
   double t[BEC_BMAX], sc[BEC_BMAX], mu[BEC_BMAX], phisc[BEC_BMAX];
   const double* tab; unsigned tabm;                      // device scalars: bit k of tabm reads DS slot k
   double* partials; double* hpartials;
-  unsigned long long* ucount;                            // measurement: instances covered (nullable)
+  unsigned long long* ucount;                            // measurement: algorithmic bytes of the instances
+  unsigned long long ubytes;                             //   covered (ubytes each; nullable)
   // device-driven loop fusions
   double2* hy_out;                                       // KE_FIRST: also IFFT(m FFT(loaded)) of this axis
   const double2* z; const double2* g; double2* d_out;    // KE_FIRST: instances with comb_bit form
@@ -263,12 +265,12 @@ void launch_init_sigma(const Plan& p, int nb, hipStream_t s);
 // live timing of one watched launch site (bec_prof.cpp); site ids:
 //   kinetic passes 100 + 10*axis + kind;  line-FFT passes
 //   2000 + 1000*contig + 100*load + 10*dir + mul;  elementwise kernels < 100
-void prof_pre(int id, hipStream_t s, int units = 1);
+void prof_pre(int id, hipStream_t s, double bytes = 0.0);
 void prof_post(int id, hipStream_t s);
-// units: instances the launch covers (summed per watched launch)
-bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, int units);   // events for hipExtLaunchKernelGGL
-// device counter that watched launches of the device-driven loop increment
-// once per instance they cover (their instance count is not known on the host)
+// bytes: algorithmic bytes of the launch (summed per watched launch)
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, double bytes);   // events for hipExtLaunchKernelGGL
+// device counter to which watched launches of the device-driven loop add their
+// algorithmic bytes per instance covered (their instances are not known on the host)
 unsigned long long* prof_counter(int id);
 bool prof_active();                  // some launch site is being timed
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,

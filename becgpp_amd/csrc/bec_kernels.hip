@@ -237,7 +237,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const int mb = (l0 + c) / nl;
   const bool active = (l0 + c) < total && (mb == b0 ? on0 : on1);
   const bool brow = on0;               // this block stores a partial row
-  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
+  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
   stage_twiddles(twl, a.fp);
   // the line this thread loads and stores
   const LineOwn ow = line_owner<X::contig>(tpl, lpw);
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   const int mb = (l0 + c) / nl;
   const bool active = (l0 + c) < total && inst_active(mb);
   const bool brow = on0;               // this block stores a partial row
-  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, 1ull);
+  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
   stage_twiddles(twl, a.fp);
   // the line this thread loads and stores, and its elements e0 + j tpl
   const LineOwn ow = line_owner<CONTIG>(tpl, lpw);
@@ -1120,6 +1120,43 @@ static int pick_lpw(int tpl, int nl, int total, bool strided) {
   return l;
 }
 static unsigned long long lowmask(int nb) { return nb >= 64 ? ~0ull : ((1ull << nb) - 1ull); }
+// Algorithmic bytes per instance of a line pass: every array element the pass
+// must read or write, once (re-reads through the caches are not counted).
+template <int LD, int ST>
+static double pass_bytes(const PassArgs& a) {
+  const double lines = a.nl;
+  const double in_el = (LD == LD_R) ? 8.0 : 16.0;
+  const double loaded = (LD == LD_HERM) ? (double)(a.fp.L / 2 + 1) : (double)a.n_load;
+  double b = lines * loaded * in_el;
+  const double st_el = (ST == ST_RE) ? 8.0 : 16.0;
+  b += lines * a.n_store * st_el;
+  if (ST == ST_RE && a.dot) b += lines * a.n_store * 8.0;
+  if (ST == ST_CDOT) b += lines * a.n_store * 5.0 * 16.0;
+  return b;
+}
+template <int KIND>
+static double kin_bytes(const KinArgs& a) {
+  const double n = (double)a.nl * a.N, c = 16.0 * n, r = 8.0 * n;
+  double b = c;                                          // psi
+  if (a.d) b += c;
+  if (KIND == KE_FIRST) {
+    if (a.rho_out) b += r;
+    if (a.hy_out) b += c;
+    if (a.d_out) b += 2.0 * c;                           // z read, new d written (direction steps)
+  }
+  if (KIND == KH_FIRST || KIND == KH_LAST) {
+    if (a.psi_w) b += c;
+    if (a.sdiff) b += c;
+  }
+  if (KIND == KH_FIRST || KIND == KH_MID || KIND == KH_LAST) b += c;   // out
+  if ((KIND == KH_MID || KIND == KH_LAST) && a.acc) b += c;
+  if (KIND == KH_LAST) {
+    if (a.local_on && a.phi) b += r;
+    if (a.phi_out) b += r;
+    if (a.ydiff) b += 2.0 * c;                           // previous out read, difference written
+  }
+  return b;
+}
 // workgroup count rounded up to a whole number of XCD rounds (see xcd_block)
 static int xcd_grid(int nlines, int lpw) {
   const int b = (nlines + lpw - 1) / lpw;
@@ -1135,8 +1172,12 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   const size_t shm = (size_t)(a.lpw + 1) * a.fp.L * sizeof(double2);
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL + (ST == ST_CDOT ? 5 : 0);
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (a.act.fl) a.ucount = prof_counter(id);
-  if (prof_events(id, &e0, &e1, a.act.fl ? 0 : __builtin_popcountll(a.act.m & lowmask(a.nb))))
+  const double bpi = pass_bytes<LD, ST>(a);
+  if (a.act.fl) {
+    a.ucount = prof_counter(id);
+    a.ubytes = (unsigned long long)bpi;
+  }
+  if (prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb))))
     hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1,
                           0, a);
   else
@@ -1177,8 +1218,12 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   const size_t shm = (size_t)(a.lpw + 1) * a.N * sizeof(double2);
   const int id = 100 + 10 * AX + KIND;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (a.act.fl) a.ucount = prof_counter(id);
-  if (prof_events(id, &e0, &e1, a.act.fl ? 0 : __builtin_popcountll(a.act.m & lowmask(a.nb))))
+  const double bpi = kin_bytes<KIND>(a);
+  if (a.act.fl) {
+    a.ucount = prof_counter(id);
+    a.ubytes = (unsigned long long)bpi;
+  }
+  if (prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb))))
     hipExtLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1, 0, a);
   else
     hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
@@ -1226,10 +1271,10 @@ void launch_post(const double2* psi, double2* best, double2* d, const double2* z
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);
 }
-static int act_units(const Act& a, int nb) { return a.fl ? nb : __builtin_popcountll(a.m & lowmask(nb)); }
+static double act_units(const Act& a, int nb) { return a.fl ? nb : __builtin_popcountll(a.m & lowmask(nb)); }
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, const CombArgs& c,
                     const double* tab, long long n, int nb, Act act, hipStream_t s) {
-  prof_pre(SITE_COMBINE, s, act_units(act, nb));
+  prof_pre(SITE_COMBINE, s, 48.0 * (double)n * act_units(act, nb));
   hipLaunchKernelGGL(k_combine, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, d, z, g, psi, c, tab, n, act);
   prof_post(SITE_COMBINE, s);
 }
@@ -1242,7 +1287,7 @@ int reduce_blocks(long long n) {
 int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g, const double2* y,
                 const double2* sv, long long n, int nb, Act act, double* partials, double* hpart, hipStream_t s) {
   const int gr = reduce_blocks(n);
-  prof_pre(SITE_DOTS, s, act_units(act, nb));
+  prof_pre(SITE_DOTS, s, 96.0 * (double)n * act_units(act, nb));
   hipLaunchKernelGGL(k_dots, dim3(gr, nb), dim3(BEC_WG), 0, s, z, psi, d, g, y, sv, n, act, partials, hpart);
   prof_post(SITE_DOTS, s);
   return gr;

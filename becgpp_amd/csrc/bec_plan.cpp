@@ -50,7 +50,16 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
         std::to_string(BEC_MAX_L));
   fp.L = L;
   fp.nst = (int)rad.size();
-  for (int i = 0; i < BEC_MAX_STAGES; ++i) fp.rad[i] = i < fp.nst ? rad[i] : 1;
+  int ns = 1;
+  for (int i = 0; i < BEC_MAX_STAGES; ++i) {
+    fp.rad[i] = i < fp.nst ? rad[i] : 1;
+    fp.ns[i] = ns;
+    fp.step[i] = i < fp.nst ? L / (ns * rad[i]) : 1;
+    // j / ns == umulhi(j, mag) for 0 <= j < 2^12: mag = floor(2^32/ns) + 1 exceeds
+    // 2^32/ns by at most 1/ns, so the error j (mag ns - 2^32) / 2^32 < j ns / 2^32 < 1/ns
+    fp.mag[i] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns) + 1u : 0u;
+    if (i < fp.nst) ns *= rad[i];
+  }
   // twiddles exp(-2 pi i m / L) evaluated in extended precision
   std::vector<double2> tw(L);
   const long double pi = 3.141592653589793238462643383279502884L;

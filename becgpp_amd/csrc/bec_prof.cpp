@@ -1,8 +1,8 @@
 // Live kernel timing: HIP events recorded on the launch stream around every
 // launch of one watched kernel (identified by its launch-site id), summed on
-// demand together with the work units (instances) each launch covered.  Used
-// by bench.py for the achieved-bandwidth figure of the dominant kernel; off
-// (no events) unless a kernel is watched.
+// demand together with the algorithmic bytes each launch moved.  Used by
+// bench.py for the achieved-bandwidth figure of the dominant kernel; off (no
+// events) unless a kernel is watched.
 #include <vector>
 
 #include "../../include/becgpp_hip.h"
@@ -11,14 +11,15 @@
 namespace {
 struct EvPair {
   hipEvent_t a, b;
-  int units;
+  double bytes;
 };
 int g_watch = 0;
 unsigned long long* g_dcount = nullptr;    // device: instances covered by watched device-driven launches
 std::vector<EvPair> g_pool;
 size_t g_used = 0;
 double g_ms = 0.0;
-long long g_cnt = 0, g_units = 0;
+long long g_cnt = 0;
+double g_bytes = 0.0;
 
 void collect() {
   for (size_t i = 0; i < g_used; ++i) {
@@ -27,7 +28,7 @@ void collect() {
         hipEventElapsedTime(&ms, g_pool[i].a, g_pool[i].b) == hipSuccess) {
       g_ms += ms;
       ++g_cnt;
-      g_units += g_pool[i].units;
+      g_bytes += g_pool[i].bytes;
     }
   }
   g_used = 0;
@@ -47,19 +48,19 @@ EvPair* next_pair() {
 }
 }  // namespace
 
-void prof_pre(int id, hipStream_t s, int units) {
+void prof_pre(int id, hipStream_t s, double bytes) {
   if (g_watch == 0 || id != g_watch) return;
   EvPair* e = next_pair();
   if (!e) return;
-  e->units = units;
+  e->bytes = bytes;
   (void)hipEventRecord(e->a, s);
 }
 
-bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, int units) {
+bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, double bytes) {
   if (g_watch == 0 || id != g_watch) return false;
   EvPair* e = next_pair();
   if (!e) return false;
-  e->units = units;
+  e->bytes = bytes;
   *start = e->a;
   *stop = e->b;
   ++g_used;
@@ -84,7 +85,7 @@ extern "C" int bec_prof_watch(int id) {
   g_watch = id;
   g_ms = 0.0;
   g_cnt = 0;
-  g_units = 0;
+  g_bytes = 0.0;
   if (id != 0) {
     if (!g_dcount && hipMalloc(&g_dcount, sizeof(unsigned long long)) != hipSuccess) {
       g_dcount = nullptr;
@@ -95,12 +96,12 @@ extern "C" int bec_prof_watch(int id) {
   return 0;
 }
 
-extern "C" int bec_prof_read(double* total_ms, long long* count, long long* units) {
+extern "C" int bec_prof_read(double* total_ms, long long* count, double* bytes) {
   collect();
   unsigned long long dev = 0;
   if (g_dcount && hipMemcpy(&dev, g_dcount, sizeof(dev), hipMemcpyDeviceToHost) != hipSuccess) return 2;
   if (total_ms) *total_ms = g_ms;
   if (count) *count = g_cnt;
-  if (units) *units = g_units + (long long)dev;
+  if (bytes) *bytes = g_bytes + (double)dev;
   return 0;
 }

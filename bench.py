@@ -29,6 +29,7 @@ import ctypes
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -47,25 +48,40 @@ WORKLOADS = {
         cfg=dict(mode="single", dimension="2D", s=2, Omega=0.0, beta2=100.0, beta3=0.0, G_C=0.0,
                  kernel="none", L=16.0, Ngrid=128, nseeds=1, res_tol=1e-4, maxit=60000),
         desc="2D harmonic plain solve, beta2=100, L=16, N=128"),
+    # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
+    "droplet_sweep_2d_n256": dict(
+        cfg=dict(mode="single", dimension="2D", s=2, trap_coeff=0.0, Omega=0.0, beta2=-120.0, beta3=1000.0,
+                 G_C=0.0, kernel="none", L=20.0, Ngrid=256, nseeds=4, res_tol=1e-6, maxit=60000,
+                 linesearch_max=20),
+        points=[dict(beta2=float(b)) for b in
+                [-200.0 + 160.0 * i / 15.0 for i in range(16)]],
+        desc="2D cubic-quintic droplet sweep, no trap, beta2 = 16 values in [-200, -40], beta3=1e3, "
+             "L=20, N=256, nseeds=4 (64 solves), res_tol=1e-6, linesearch_max=20"),
 }
 
-# watched kernel: launch-site id -> (name, algorithmic bytes per launch as a function of n)
+# watched kernel: launch-site id -> (name, algorithmic bytes per instance of its
+# launches in the solver loop).  The launches themselves report their exact
+# algorithmic bytes (each element read or written once), which is what
+# "achieved" divides by the time; these per-instance figures are the
+# documented formulas for the loop's launches.
 KERNELS = {
-    2023: ("k_pass<strided,fwd*div*inv,precond> (Sobolev preconditioner, x pass)", lambda n: 32 * n),
+    114: ("k_kin<A2X,KH_LAST> (accepted state: psi update, x pass of H psi, local terms, mu psi, residual rows)",
+          lambda n: 128 * n),   # reads psi, d, H share, old g; writes psi, s = change, g, y = change
+    100: ("k_kin<A2Y,KE_FIRST> (trial energy: new direction, y pass, Parseval and local sums, H share)",
+          lambda n: 80 * n),    # reads psi, d, z; writes d, H share (long range: +8n for |psi|^2)
+    111: ("k_kin<A2X,KE_REST> (trial energy, x pass)", lambda n: 32 * n),
+    2023: ("k_pass<strided,fwd*(k^2/2+sigma)^-1*inv> (Sobolev preconditioner, x pass)", lambda n: 32 * n),
     3000: ("k_pass<contig,fwd> (preconditioner, y pass forward)", lambda n: 32 * n),
-    3010: ("k_pass<contig,inv> (preconditioner, y pass inverse)", lambda n: 32 * n),
-    100: ("k_kin<A2Y,KE_FIRST> (energy, y pass + local sums)", lambda n: 32 * n),
-    111: ("k_kin<A2X,KE_REST> (energy, x pass)", lambda n: 32 * n),
-    102: ("k_kin<A2Y,KH_FIRST> (H psi, y pass, materialises psi)", lambda n: 64 * n),
-    114: ("k_kin<A2X,KH_LAST> (H psi - mu psi, x pass + local terms)", lambda n: 48 * n),
-    10: ("k_dots (CG inner products)", lambda n: 96 * n),
+    3015: ("k_pass<contig,inv,dots> (preconditioner y pass inverse + 16 CG inner products)",
+           lambda n: 112 * n),
 }
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=2,
+                    help="timed steps (0: no timed region, only the event-timed replay -- for profiling)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rotating_lattice_2d_n384", choices=sorted(WORKLOADS))
     ap.add_argument("--watch", type=int, default=114, help="launch-site id of the roofline kernel")
@@ -121,32 +137,36 @@ def main():
     nseeds = int(cfg["nseeds"])
     rotating = abs(float(cfg["Omega"])) >= 0.3
     cycle = ("triangular", "tf", "gaussian") if rotating else ("tf", "gaussian")
-    runs = []
-    for j in range(nseeds):
-        c = dict(cfg, rng=int(cfg["rng"]) + j, seed=cycle[j % len(cycle)])
-        runs.append((c, make_seed(G, c).contiguous()))
+    points = wl.get("points", [{}])
+    runs = []                                   # (point index, cfg, psi0)
+    for ip, over in enumerate(points):
+        for j in range(nseeds):
+            c = dict(cfg, **over)
+            c.update(rng=int(cfg["rng"]) + j, seed=cycle[j % len(cycle)])
+            runs.append((ip, c, make_seed(G, c).contiguous()))
     torch.cuda.synchronize()
+    cs = [c for _, c, _ in runs]
+    psi0s = [p for _, _, p in runs]
+    nsol = len(runs)
 
-    cs = [c for c, _ in runs]
-    psi0s = [p for _, p in runs]
+    def select(res):
+        """Per point: the lowest-energy seed (solvers.py:238-263)."""
+        best = {}
+        for (ip, _, _), (psi, _, obs) in zip(runs, res):
+            if ip not in best or obs["E"] < best[ip][1]["E"]:
+                best[ip] = (psi, obs)
+        return best
 
     def step():
         res = B.ground_state_batch(cs, G=G, psi0s=psi0s)
+        best = select(res)
         Es = [obs["E"] for _, _, obs in res]
-        psi, _, best = min(res, key=lambda r: r[2]["E"])
-        best = dict(best, psi=psi)
         stops = [f'{obs["stop_reason"]}@{obs["iters"]}' for _, _, obs in res]
         return best, sum(obs["iters"] for _, _, obs in res), Es, stops
 
     def step_single():
-        best, iters, Es = None, 0, []
-        for c, psi0 in runs:
-            _, _, obs = B.ground_state(c, G=G, psi0=psi0, verbose=False)
-            iters += obs["iters"]
-            Es.append(obs["E"])
-            if best is None or obs["E"] < best["E"]:
-                best = obs
-        return best, iters, Es
+        res = [B.ground_state(c, G=G, psi0=p0, verbose=False) for c, p0 in zip(cs, psi0s)]
+        return select(res), sum(obs["iters"] for _, _, obs in res), [obs["E"] for _, _, obs in res]
 
     lib = _lib.load()
     for _ in range(a.warmup):
@@ -162,62 +182,89 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+
     # The timed steps replay hipGraphs, whose launches cannot carry timing
     # events; the roofline kernel is timed with HIP events on its stream in a
     # replay of the same batched workload with direct launches.
     lib.bec_prof_watch(int(a.watch))
-    B.ground_state_batch([dict(c, maxit=int(a.measure_iters), res_tol=0.0) for c in cs], G=G, psi0s=psi0s)
+    B.ground_state_batch([dict(c, maxit=int(a.measure_iters), res_tol=0.0) for c in cs[:32]], G=G,
+                         psi0s=psi0s[:32])
     torch.cuda.synchronize()
     ms = ctypes.c_double()
     cnt = ctypes.c_longlong()
-    units = ctypes.c_longlong()
-    lib.bec_prof_read(ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(units))
+    nbytes = ctypes.c_double()
+    lib.bec_prof_read(ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(nbytes))
     lib.bec_prof_watch(0)
-    kname, kbytes = KERNELS.get(a.watch, (f"site {a.watch}", lambda n: 32 * n))
+    kname, kbytes = KERNELS.get(a.watch, (f"site {a.watch}", lambda n: None))
     avg_s = (ms.value / 1e3) / max(1, cnt.value)
-    upl = units.value / max(1, cnt.value)              # instances per timed launch
-    achieved = kbytes(n) * upl / avg_s / 1e9 if cnt.value else None
-    best, Es, _ = results[-1]
+    bpl = nbytes.value / max(1, cnt.value)             # algorithmic bytes per timed launch
+    achieved = bpl / avg_s / 1e9 if cnt.value else None
+    per_inst = kbytes(n)
+    roof = dict(bound="hbm", kernel=kname, site=a.watch, launches=cnt.value,
+                timed_in=f"direct-launch replay of the batch, {a.measure_iters} CG iterations",
+                avg_launch_us=avg_s * 1e6, achieved=achieved,
+                peak=PEAK_HBM_GBS, unit="GB/s", frac=(achieved / PEAK_HBM_GBS) if achieved else None,
+                traffic=None, algorithmic_bytes_per_launch=bpl,
+                algorithmic_bytes_per_instance=per_inst,
+                instances_per_launch=(bpl / per_inst) if per_inst else None)
+    # HBM bytes per launch of the roofline kernel, from the committed counter
+    # summary of this workload's replay (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+    # calibrated on a device copy)
+    tfile = os.path.join(HERE, "profiles", f"{a.workload}_hbm_traffic.md")
+    if a.watch == 114 and os.path.exists(tfile):
+        m = re.search(r"traffic_bytes_per_dispatch: ([0-9.]+)", open(tfile).read())
+        if m:
+            roof["traffic"] = float(m.group(1))
+            roof["traffic_source"] = os.path.relpath(tfile, HERE)
+            roof["traffic_over_algorithmic"] = roof["traffic"] / bpl if bpl else None
+    if a.steps == 0:
+        # profiling runs: no timed region, so that a kernel trace of this
+        # command holds exactly the launches the events timed
+        print(json.dumps(dict(metric="roofline kernel only (--steps 0)", value=None, roofline=roof,
+                              config=dict(workload=a.workload))), flush=True)
+        return
+    best, Es, stops = results[-1]
+    npt = len(points)
+    b0psi, b0 = best[0]
     out = dict(
         metric="ground states/sec at |dE/E|<=1e-8; CG iters/sec; HBM GB/s vs peak",
-        value=a.steps / wall, unit="ground states/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
+        value=a.steps * npt / wall, unit="ground states/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
         ms_per_step=1e3 * wall / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64", data="synthetic",
         config=dict(workload=a.workload, desc=wl["desc"], Ngrid=int(G["N"]), ndim=int(G["ndim"]),
-                    nseeds=nseeds, res_tol=float(cfg["res_tol"]), path="batched"),
-        solves_per_s=a.steps * nseeds / wall, cg_iters_per_s=tot_iters / wall,
-        iters_per_solve=tot_iters / (a.steps * nseeds), E_selected=best["E"], seed_energies=Es,
-        stop_reasons=results[-1][2],
-        converged=bool(best["converged"]), resid_rel=best["resid_rel"],
-        roofline=dict(bound="hbm", kernel=kname, site=a.watch, launches=cnt.value,
-                      timed_in=f"direct-launch replay of the batch, {a.measure_iters} CG iterations",
-                      instances_per_launch=upl, avg_launch_us=avg_s * 1e6, achieved=achieved,
-                      peak=PEAK_HBM_GBS, unit="GB/s", frac=(achieved / PEAK_HBM_GBS) if achieved else None,
-                      traffic=None, algorithmic_bytes_per_launch=kbytes(n) * upl,
-                      algorithmic_bytes_per_instance=kbytes(n)),
+                    points=npt, nseeds=nseeds, solves_per_step=nsol, res_tol=float(cfg["res_tol"]),
+                    path="batched"),
+        solves_per_s=a.steps * nsol / wall, cg_iters_per_s=tot_iters / wall,
+        iters_per_solve=tot_iters / (a.steps * nsol),
+        E_selected=[best[i][1]["E"] for i in range(npt)] if npt > 1 else b0["E"],
+        seed_energies=Es if npt == 1 else None, stop_reasons=stops if npt == 1 else None,
+        converged=all(bool(best[i][1]["converged"]) for i in range(npt)),
+        resid_rel=max(best[i][1]["resid_rel"] for i in range(npt)),
+        roofline=roof,
     )
-    # |dE/E| of the selected state: polish it to the FP64 residual floor
+    # |dE/E| of the first point's selected state: polish it to the FP64 residual floor
     if not a.no_polish:
-        pc = dict(cs[Es.index(best["E"])], res_tol=1e-9, maxit=20000)
-        _, _, pol = B.ground_state(pc, G=G, psi0=best["psi"], verbose=False)
+        ib = min((k for k in range(nsol) if runs[k][0] == 0), key=lambda k: Es[k])
+        pc = dict(cs[ib], res_tol=1e-9, maxit=20000)
+        _, _, pol = B.ground_state(pc, G=G, psi0=b0psi, verbose=False)
         out["E_polished"] = pol["E"]
-        out["dE_rel_vs_polished"] = abs(best["E"] - pol["E"]) / abs(pol["E"])
+        out["dE_rel_vs_polished"] = abs(b0["E"] - pol["E"]) / abs(pol["E"])
     if not a.no_single:
         t0 = time.perf_counter()
-        sb, si, sE = step_single()
+        sbest, si, sE = step_single()
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
-        out["single_solve_path"] = dict(ground_states_per_s=1.0 / ts, solves_per_s=nseeds / ts,
-                                        cg_iters_per_s=si / ts, iters_per_solve=si / nseeds,
-                                        E_selected=sb["E"], seed_energies=sE)
-        out["batched_speedup"] = out["value"] * ts
+        out["single_solve_path"] = dict(ground_states_per_s=npt / ts, solves_per_s=nsol / ts,
+                                        cg_iters_per_s=si / ts, iters_per_solve=si / nsol,
+                                        E_selected=sbest[0][1]["E"], seed_energies=sE if npt == 1 else None)
+        out["batched_speedup"] = out["value"] / out["single_solve_path"]["ground_states_per_s"]
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, runs[0][1].cpu().numpy(), a.cpu_sample_iters,
-                                           out["iters_per_solve"], nseeds)
+        out["cpu_baseline"] = cpu_baseline(cs[0], psi0s[0].cpu().numpy(), a.cpu_sample_iters,
+                                           out["iters_per_solve"], nsol / npt)
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(cfg, psi0, sample_iters, iters_per_solve, nseeds):
+def cpu_baseline(cfg, psi0, sample_iters, iters_per_solve, solves_per_state):
     """The oracle (numpy/scipy, all host threads) on the first seed of the same
     workload for a bounded number of CG iterations; scaled to ground states/s
     with the GPU's measured iterations per solve."""
@@ -231,10 +278,11 @@ def cpu_baseline(cfg, psi0, sample_iters, iters_per_solve, nseeds):
     _, _, obs = O.ground_state(c, Go, psi0)
     dt = time.perf_counter() - t0
     its = obs["iters"] / dt
-    return dict(value=its / (iters_per_solve * nseeds), unit="ground states/s", cores=threads,
+    return dict(value=its / (iters_per_solve * solves_per_state), unit="ground states/s", cores=threads,
                 kind="port", cg_iters_per_s=its, cpu=cpu_model(),
-                sample=f"{obs['iters']} CG iterations of seed 0 of the same workload ({dt:.1f} s), "
-                       f"scaled by the GPU's {iters_per_solve:.0f} iterations/solve x {nseeds} seeds")
+                sample=f"{obs['iters']} CG iterations of the first solve of the same workload ({dt:.1f} s), "
+                       f"scaled by the GPU's {iters_per_solve:.0f} iterations/solve x "
+                       f"{solves_per_state:g} solves per ground state")
 
 
 if __name__ == "__main__":

@@ -158,12 +158,13 @@ int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_so
 
 /* Measurement hooks (no counterpart in the reference, which has no profiler
  * hooks): time every launch of one kernel with HIP events on its stream;
- * units sums the instances each timed launch covered (1 per launch for
- * single-instance kernels).  id 0 switches timing off.  Ids: kinetic passes 100 + 10*axis + kind, line
+ * bytes sums the algorithmic bytes of the timed launches (each array element
+ * the launch must read or write counted once, per instance it covered).
+ * id 0 switches timing off.  Ids: kinetic passes 100 + 10*axis + kind, line
  * FFT passes 2000 + 1000*contig + 100*load + 10*dir + mul, inner products 10,
  * shift selection 20-23 (becgpp_amd/csrc/bec_internal.h). */
 int bec_prof_watch(int id);
-int bec_prof_read(double* total_ms, long long* count, long long* units);
+int bec_prof_read(double* total_ms, long long* count, double* bytes);
 
 #ifdef __cplusplus
 }
