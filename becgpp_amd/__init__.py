@@ -9,7 +9,7 @@ from .constants import CODE_VERSION, DEV
 from .config import CFG_DEFAULTS, default_cfg, canonical_key
 from .grid import make_grid, geometry, resolve_kernel, auto_grid, dV, norm_of, Lz_op
 from .interactions import long_range_phi, kernel_convolution, q2d_kernel, clear_kernel_cache
-from .operators import energy_components, energy_t, apply_H, compute_residual, observables
+from .operators import energy_components, energy_t, apply_H, compute_residual, observables, sensitivities
 from .thomasfermi import extract_tf, mass_quantiles
 from .seeds import make_seed
 from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, solve_native,

@@ -16,9 +16,9 @@
 #define BEC_MAX_L 1024      // longest line transform held in LDS
 
 struct FftPlan {
-  int L;                      // transform length (2^a 3^b)
+  int L;                      // transform length (2^a 3^b 5^c)
   int nst;                    // number of Stockham stages
-  int rad[BEC_MAX_STAGES];    // radix per stage (8, 4, 3 or 2)
+  int rad[BEC_MAX_STAGES];    // radix per stage (8, 4, 3, 2 or 5)
   int ns[BEC_MAX_STAGES];     // Ns of the stage: product of the earlier radices
   int step[BEC_MAX_STAGES];   // twiddle stride L / (Ns R)
   unsigned mag[BEC_MAX_STAGES];   // floor(2^32 / Ns) + 1: j / Ns = umulhi(j, mag) for j < 2^12
@@ -81,10 +81,29 @@ template <int SIGN> __device__ __forceinline__ void bfly8(double2* v) {
   v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
   v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
 }
+template <int SIGN> __device__ __forceinline__ void bfly5(double2* v) {
+  // w = exp(SIGN 2 pi i / 5): cos and sin of 2 pi / 5 and 4 pi / 5
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = SIGN * 0.95105651629515357212, s2 = SIGN * 0.58778525229247312917;
+  const double2 a = v[0];
+  const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const double2 r1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+  const double2 r2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+  // i (s1 t3 + s2 t4) and i (s2 t3 - s1 t4)
+  const double2 u1 = make_double2(-(s1 * t3.y + s2 * t4.y), s1 * t3.x + s2 * t4.x);
+  const double2 u2 = make_double2(-(s2 * t3.y - s1 * t4.y), s2 * t3.x - s1 * t4.x);
+  v[0] = make_double2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+  v[1] = cadd(r1, u1);
+  v[4] = csub(r1, u1);
+  v[2] = cadd(r2, u2);
+  v[3] = csub(r2, u2);
+}
 template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
   if constexpr (R == 2) bfly2<SIGN>(v);
   else if constexpr (R == 3) bfly3<SIGN>(v);
   else if constexpr (R == 4) bfly4<SIGN>(v);
+  else if constexpr (R == 5) bfly5<SIGN>(v);
   else bfly8<SIGN>(v);
 }
 
@@ -145,6 +164,7 @@ __device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const doub
     if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
   }
 }

@@ -40,13 +40,14 @@ static bool factor(int L, std::vector<int>& rad) {
   while (n % 3 == 0) { rad.push_back(3); n /= 3; }
   while (n % 4 == 0) { rad.push_back(4); n /= 4; }
   while (n % 2 == 0) { rad.push_back(2); n /= 2; }
+  while (n % 5 == 0) { rad.push_back(5); n /= 5; }
   return n == 1 && (int)rad.size() <= BEC_MAX_STAGES;
 }
 
 static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
   std::vector<int> rad;
   if (L < 2 || L > BEC_MAX_L || !factor(L, rad))
-    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b <= " +
+    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b 5^c <= " +
         std::to_string(BEC_MAX_L));
   fp.L = L;
   fp.nst = (int)rad.size();

@@ -50,8 +50,55 @@ def energy_components(p, G, beta2, beta3, Omega, G_C):
     return out
 
 
+class _Energy(torch.autograd.Function):
+    """E[psi] with the adjoint as its backward.  For real E of complex psi,
+    PyTorch's gradient is 2 dE/d(conj psi), and dE/d(conj psi) = H psi dV with
+    the mean-field H of apply_H (the long-range term's derivative is phi psi,
+    since the kernel is symmetric), so the backward is 2 H psi dV."""
+
+    @staticmethod
+    def forward(ctx, psi, G, beta2, beta3, Omega, G_C):
+        t = as_field(psi.detach(), G)
+        comps = energy_components(t, G, beta2, beta3, Omega, G_C)
+        ctx.save_for_backward(t, comps["phi"])
+        ctx.G = G
+        ctx.cfg = dict(beta2=beta2, beta3=beta3, Omega=Omega, G_C=G_C)
+        return comps["E"]
+
+    @staticmethod
+    def backward(ctx, gE):
+        t, phi = ctx.saved_tensors
+        h = apply_H(t, ctx.G, ctx.cfg, phi=phi if _lr_on(ctx.G, ctx.cfg["G_C"]) else None)
+        return 2.0 * dV(ctx.G) * gE * h, None, None, None, None, None
+
+
 def energy_t(p, G, beta2, beta3, Omega, G_C):
-    return energy_components(p, G, beta2, beta3, Omega, G_C)["E"]
+    """operators.py:32-33: the energy as a 0-d tensor that autograd
+    differentiates with respect to psi (backward 2 H psi dV)."""
+    if not (torch.is_tensor(p) and p.requires_grad):
+        return energy_components(p, G, beta2, beta3, Omega, G_C)["E"]
+    return _Energy.apply(p, G, beta2, beta3, Omega, G_C)
+
+
+def sensitivities(psi, G, cfg):
+    """Hellmann-Feynman derivatives of the ground-state energy at a converged
+    state: dE/dbeta2 = Econtact / beta2 (= 1/2 sum rho^2 dV), dE/dbeta3 =
+    Ethree / beta3, dE/dG_C = Egrav / G_C, dE/dOmega = -L_z."""
+    t = as_field(psi, G)
+    rho = (t.real ** 2 + t.imag ** 2)
+    dv = dV(G)
+    c = {k: v.item() for k, v in energy_components(t, G, cfg["beta2"], cfg["beta3"], cfg["Omega"],
+                                                   cfg["G_C"]).items() if k != "phi"}
+    out = dict(dE_dbeta2=0.5 * float((rho * rho).sum()) * dv,
+               dE_dbeta3=float((rho * rho * rho).sum()) * dv / 3.0,
+               dE_dOmega=-c["Lz"])
+    gc = float(cfg["G_C"])
+    if _lr_on(G, gc):
+        out["dE_dG_C"] = c["Egrav"] / gc
+    else:
+        out["dE_dG_C"] = 0.0 if G["kernel"] == "none" else -0.5 * float(
+            (rho * kernel_convolution(rho, G)).sum()) * dv
+    return out
 
 
 def apply_H(psi, G, cfg, phi=None):

@@ -89,7 +89,7 @@ const char* bec_last_error(void);
 /* Plan: grid geometry, FFT twiddles, workspaces, kernel spectra.
  * Replaces make_grid (grid.py:69-101) for the device side: x_i = (i-N/2) dx,
  * dx = 2L/N, k = 2 pi fftfreq(N, dx), V = trap_coeff r^s (0 if s = 0),
- * padded length M = pad N.  N and pad N must factor into 2^a 3^b with
+ * padded length M = pad N.  N and pad N must factor into 2^a 3^b 5^c with
  * pad N <= 1024; ndim is 2 or 3. */
 int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff, bec_plan** out);
 int bec_plan_destroy(bec_plan* p);

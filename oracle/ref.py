@@ -807,16 +807,31 @@ def noisy_gaussian(G, noise=1e-3, seed=0, width=1.0):
 EULER_GAMMA = 0.5772156649015329
 
 
+class _OracleOps:
+    """The numeric operations gate_rows evaluates; the defaults are the oracle's
+    own.  A test may pass another implementation (numpy in, numpy out)."""
+
+    def __init__(self):
+        self.make_grid = make_grid
+        self.long_range_phi = long_range_phi
+        self.energy_components = energy_components
+        self.clear_kernel_cache = clear_kernel_cache
+
+
 def _gauss_rho(G, a):
     psi = np.exp(-G["R2"] / (2 * a * a)).astype(np.complex128)
     psi = psi / norm_of(psi, G)
     return np.abs(psi) ** 2
 
 
-def gate_rows(Nv=256, order_sizes=None):
+def gate_rows(Nv=256, order_sizes=None, ops=None):
     """Recompute the rows of modes.mode_validate (modes.py:30-201) as
     (test, case, quantity) -> rel_error.  ``order_sizes`` optionally restricts
-    the kernel-order ladders."""
+    the kernel-order ladders; ``ops`` supplies make_grid, long_range_phi,
+    energy_components and clear_kernel_cache (default: this oracle)."""
+    ops = ops or _OracleOps()
+    make_grid, long_range_phi, energy_components = ops.make_grid, ops.long_range_phi, ops.energy_components
+    clear_kernel_cache = ops.clear_kernel_cache
     rows = {}
     base = dict(s=2, trap_coeff=0.5, Omega=0.0, beta2=0.0, beta3=0.0, pad=2)
     G = make_grid(dict(base, dimension="2D", kernel="newton", G_C=1.0, L=12.0, Ngrid=Nv))

@@ -29,7 +29,9 @@ def _dev(a):
 
 @pytest.mark.parametrize("kw", [dict(Ngrid=48), dict(Ngrid=64, L=8), dict(Ngrid=96), dict(Ngrid=128, L=9),
                                 dict(Ngrid=384, L=12), dict(dimension="3D", Ngrid=24, L=5),
-                                dict(dimension="3D", Ngrid=32, L=6)])
+                                dict(dimension="3D", Ngrid=32, L=6),
+                                dict(Ngrid=80, L=8), dict(Ngrid=160, L=10), dict(Ngrid=120, L=9),
+                                dict(dimension="3D", Ngrid=40, L=6)])
 def test_lz_op(kw):
     import becgpp_amd as B
     c, Go, Gd = _pair(kw)
@@ -63,7 +65,9 @@ def test_energy_components(kw):
 
 
 @pytest.mark.parametrize("kw", PHYSICS[1:5] + [dict(dimension="2D", G_C=2.0, kernel="newton", Ngrid=96, L=8),
-                                               dict(dimension="3D", G_C=2.0, kernel="newton", Ngrid=32, L=6)])
+                                               dict(dimension="3D", G_C=2.0, kernel="newton", Ngrid=32, L=6),
+                                               dict(dimension="2D", G_C=2.0, kernel="log", Ngrid=80, L=8),
+                                               dict(dimension="3D", G_C=2.0, kernel="newton", Ngrid=20, L=5)])
 def test_kernel_convolution(kw):
     import becgpp_amd as B
     c, Go, Gd = _pair(kw)
