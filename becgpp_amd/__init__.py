@@ -15,4 +15,6 @@ from .seeds import make_seed
 from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, solve_native,
                       solve_native_batch)
 
+from .modes import run, run_id, DISPATCH
+
 __version__ = "0.1.0"

@@ -1,0 +1,338 @@
+"""run(cfg) and the modes of the ground-state path (becgpp/modes.py:30-500).
+
+The dispatch, mode names, run ids and error behaviour are the reference's.
+The modes that sit on the ground-state path are computed here by the GPU
+solver: ``single`` (ground_state / ground_state_multiseed), ``sweep`` (every
+fresh seed of every point as one batched solve when continuation is off;
+point by point with the continuation candidate otherwise), ``validate`` (the
+analytic gate, every convolution and energy by libbecgpp_hip) and ``smoke``.
+Results are returned as the reference returns them; checkpoints, figures,
+CSV/JSON files, vortex and LLL diagnostics are not produced by this package,
+and the modes ``tf_only`` and ``convergence`` are not part of it (they raise).
+"""
+import hashlib
+import json
+import math
+
+import numpy as np
+import torch
+
+from .config import CFG_DEFAULTS, canonical_key
+from .constants import CODE_VERSION, DEV
+from .grid import dV, geometry, make_grid, norm_of, resolve_kernel
+from .interactions import long_range_phi
+from .operators import energy_components
+from .solvers import batchable, ground_state, ground_state_batch, ground_state_multiseed
+from .thomasfermi import extract_tf
+
+# keys that cannot change the computed state (becgpp/io.py:9-19)
+_RID_EXCLUDE = frozenset({
+    "mode", "tag", "maxit", "check", "record_trace", "mem_warn_gb",
+    "want_vortices", "want_lll", "tf_cache", "validate_N",
+    "sweep_param", "sweep_values", "sweep_autobox", "sweep_continuation",
+    "conv_Ngrids", "conv_box_factors", "conv_pads",
+    "show_inline", "save_figs", "save_ckpt", "zip_output",
+})
+
+
+def run_id(cfg):
+    """tag_md5[:10] of the code version and every state-defining key (io.py:22-26)."""
+    keys = sorted(k for k in CFG_DEFAULTS if k not in _RID_EXCLUDE)
+    raw = json.dumps(dict(code=CODE_VERSION, **{k: cfg.get(k, CFG_DEFAULTS[k]) for k in keys}),
+                     sort_keys=True, default=str)
+    return f"{cfg.get('tag', 'becgpp')}_{hashlib.md5(raw.encode()).hexdigest()[:10]}"
+
+
+def _record(obs, G, cfg, rid):
+    d = {k: v for k, v in obs.items() if k != "trace"}
+    d.update(rid=rid, seed_used=obs.get("seed_used", cfg.get("seed")),
+             seed_energy_spread=obs.get("seed_energy_spread", 0.0),
+             dimension=cfg["dimension"], kernel=G["kernel"], L=cfg["L"], N=cfg["Ngrid"], s=cfg["s"],
+             Omega=cfg["Omega"], beta2=cfg["beta2"], beta3=cfg["beta3"], G_C=cfg["G_C"],
+             l_z=cfg.get("l_z", 0.0), device=str(DEV))
+    return d
+
+
+# -----------------------------------------------------------------------------
+# single
+# -----------------------------------------------------------------------------
+def mode_single(cfg):
+    """modes.py:207-262: one configuration; nseeds > 1 solves the seeds as one
+    batch and keeps the lowest energy."""
+    rid = run_id(cfg)
+    G = make_grid(cfg)
+    if int(cfg.get("nseeds", 1)) > 1:
+        psi, G, obs = ground_state_multiseed(cfg, G=G, verbose=True)
+    else:
+        psi, G, obs = ground_state(cfg, G=G, verbose=True)
+    d = _record(obs, G, cfg, rid)
+    print(f"[single] {cfg['dimension']} s={cfg['s']} Omega={cfg['Omega']} beta2={cfg['beta2']} "
+          f"beta3={cfg['beta3']} G_C={cfg['G_C']} kernel={G['kernel']}")
+    print(f"         E={d['E']:.8f} mu={d['mu']:.6f} Lz={d['Lz']:.5f} virial_rel={d['virial_rel']:.2e}")
+    print(f"         resid={d['resid_rel']:.2e} converged={d['converged']} it={d['iters']}")
+    d["psi"] = psi
+    return d
+
+
+# -----------------------------------------------------------------------------
+# sweep
+# -----------------------------------------------------------------------------
+def _sweep_args(cfg):
+    param = canonical_key(cfg.get("sweep_param", "G_C"))
+    tmpl = CFG_DEFAULTS.get(param)
+    if param not in CFG_DEFAULTS or isinstance(tmpl, bool) or not isinstance(tmpl, (int, float)):
+        numeric = sorted(k for k, v in CFG_DEFAULTS.items()
+                         if isinstance(v, (int, float)) and not isinstance(v, bool))
+        raise ValueError(f"sweep_param={cfg.get('sweep_param')!r} is not a numeric CFG key; "
+                         f"choose one of {numeric}")
+    values = list(cfg.get("sweep_values", []))
+    bad = [v for v in values if isinstance(v, bool) or not isinstance(v, (int, float))]
+    if bad:
+        raise ValueError(f"sweep_values must be numbers, got {bad}")
+    return param, values
+
+
+def _seed_cfgs(c, nseeds):
+    rotating = abs(float(c.get("Omega", 0.0))) >= 0.3
+    cycle = ("triangular", "tf", "gaussian") if rotating else ("tf", "gaussian")
+    out = []
+    for j in range(max(1, nseeds)):
+        cj = dict(c)
+        cj["rng"] = int(c.get("rng", 0)) + j
+        cj["seed"] = cycle[j % len(cycle)] if nseeds > 1 else c.get("seed", "tf")
+        out.append(cj)
+    return out
+
+
+def _grid_key(c):
+    return (geometry(c), float(c["L"]), int(c["Ngrid"]), int(c.get("pad", 2)), float(c["s"]),
+            float(c.get("trap_coeff", 0.5)), resolve_kernel(c)[0], float(c.get("l_z", 0.0)))
+
+
+def mode_sweep(cfg):
+    """modes.py:295-387: vary one numeric key; each point keeps the lowest
+    energy over its fresh seeds (and the previous point's state when
+    sweep_continuation is on).  Without continuation all points' seeds are
+    independent and are solved as batches of up to 32 per grid."""
+    param, values = _sweep_args(cfg)
+    use_cont = bool(cfg.get("sweep_continuation", True))
+    nseeds = int(cfg.get("nseeds", 1))
+    print(f"SWEEP over {param!r} = {values}  (continuation={use_cont}, nseeds={nseeds})")
+    if not values:
+        print("[sweep] sweep_values is empty: nothing to do")
+        return []
+    pts = []
+    for val in values:
+        c = dict(cfg)
+        c[param] = val
+        c["tag"] = f"{cfg.get('tag', 'becgpp')}_{param}{val:g}"
+        pts.append(c)
+    rows = [None] * len(pts)
+    if not use_cont:
+        # every fresh seed of every point is independent: batch them per grid
+        groups = {}
+        for ip, c in enumerate(pts):
+            groups.setdefault(_grid_key(c), []).append(ip)
+        for ips in groups.values():
+            G = make_grid(pts[ips[0]])
+            jobs = [(ip, cj) for ip in ips for cj in _seed_cfgs(pts[ip], nseeds)]
+            if batchable(G):
+                res = ground_state_batch([cj for _, cj in jobs], G=G)
+            else:
+                res = [ground_state(cj, G=G, verbose=False) for _, cj in jobs]
+            for ip in ips:
+                cands = [(f"seed:{cj['seed']}", r) for (jp, cj), r in zip(jobs, res) if jp == ip]
+                rows[ip] = _pick(cands, pts[ip], param, values[ip])
+        return rows
+    prev = None
+    for ip, c in enumerate(pts):
+        G = make_grid(c)
+        cands = []
+        if prev is not None and tuple(prev.shape) == tuple(G["R2"].shape):
+            cands.append(("continuation", ground_state(c, G=G, psi0=prev, verbose=False)))
+        if nseeds > 1 or not cands:
+            seeds = _seed_cfgs(c, nseeds)
+            if len(seeds) > 1 and batchable(G):
+                res = ground_state_batch(seeds, G=G)
+            else:
+                res = [ground_state(cj, G=G, verbose=False) for cj in seeds]
+            best = min(range(len(res)), key=lambda j: res[j][2]["E"])
+            cands.append((f"seed:{seeds[best]['seed']}", res[best]))
+        rows[ip] = _pick(cands, c, param, values[ip])
+        prev = rows[ip]["psi"]
+    return rows
+
+
+def _pick(cands, c, param, val):
+    cands = sorted(cands, key=lambda t: t[1][2]["E"])      # variational: lowest energy wins
+    branch, (psi, G, obs) = cands[0]
+    d = _record(obs, G, c, run_id(c))
+    d.update({param: val}, branch=branch, walltime=sum(t[1][2]["walltime"] for t in cands),
+             candidate_E=";".join(f"{t[0]}={t[1][2]['E']:.10g}" for t in cands), psi=psi)
+    print(f"    {param}={val} [{branch}] E={d['E']:.8f} mu={d['mu']:.5f} Lz={d['Lz']:.4f} "
+          f"res={d['resid_rel']:.2e} conv={d['converged']}")
+    return d
+
+
+# -----------------------------------------------------------------------------
+# validate: the analytic gate (modes.py:30-201)
+# -----------------------------------------------------------------------------
+_EULER_GAMMA = 0.5772156649015329
+
+
+def _gauss_rho(G, a):
+    psi = torch.exp(-G["R2"] / (2 * a * a)).to(torch.complex128)
+    psi = psi / norm_of(psi, G)
+    return (psi.real ** 2 + psi.imag ** 2).contiguous()
+
+
+def _center(phi):
+    c = phi.shape[0] // 2
+    return phi[(c,) * phi.dim()].item()
+
+
+def _F(t):
+    if abs(t - 1.0) < 1e-12:
+        return 1.0
+    return math.acos(t) / math.sqrt(1 - t * t) if t < 1 else math.acosh(t) / math.sqrt(t * t - 1)
+
+
+def mode_validate(cfg):
+    """Gaussian Coulomb integrals (2D 1/r, -ln r, quasi-2D K_eff; 3D 1/r),
+    the LLL identities, the flat-top TF closed form and the kernel
+    convergence orders, with the reference's thresholds."""
+    rows = []
+    Nv = int(cfg.get("validate_N", 256))
+
+    def row(test, case, q, e):
+        rows.append(dict(test=test, case=case, quantity=q, rel_error=e))
+        return e
+
+    G = make_grid(dict(cfg, dimension="2D", s=2, kernel="newton", G_C=1.0, L=12.0, Ngrid=Nv))
+    dv = dV(G)
+    mg = mp = 0.0
+    for a in (0.5, 1.0, 2.0):
+        rho = _gauss_rho(G, a)
+        phi = long_range_phi(rho, G, 1.0)
+        Eg = 0.5 * float((phi * rho).sum()) * dv
+        ex = math.sqrt(math.pi / 8.0) / a
+        mg = max(mg, row("2D_gauss", f"a={a}", "Egrav", abs(Eg + ex) / ex))
+        ex0 = math.sqrt(math.pi) / a
+        mp = max(mp, row("2D_gauss", f"a={a}", "phi0", abs(_center(phi) + ex0) / ex0))
+    Gl = make_grid(dict(cfg, dimension="2D", s=2, kernel="log", G_C=1.0, L=12.0, Ngrid=Nv))
+    mlg = mlp = 0.0
+    for a in (0.5, 2.0, 3.0):
+        rho = _gauss_rho(Gl, a)
+        phi = long_range_phi(rho, Gl, 1.0)
+        Eg = 0.5 * float((phi * rho).sum()) * dv
+        Eg_ex = 0.5 * (math.log(math.sqrt(2.0) * a) - 0.5 * _EULER_GAMMA)
+        p0_ex = math.log(a) - 0.5 * _EULER_GAMMA
+        mlg = max(mlg, row("2D_log_gauss", f"a={a}", "Egrav", abs(Eg - Eg_ex) / abs(Eg_ex)))
+        mlp = max(mlp, row("2D_log_gauss", f"a={a}", "phi0", abs(_center(phi) - p0_ex) / abs(p0_ex)))
+    mq = 0.0
+    for lz in (0.25, 1.0):
+        Gq = make_grid(dict(cfg, dimension="quasi2D", s=2, kernel="q2d", l_z=lz, G_C=1.0, L=12.0, Ngrid=Nv))
+        for a in (1.0, 2.0):
+            rho = _gauss_rho(Gq, a)
+            phi = long_range_phi(rho, Gq, 1.0)
+            Eg = 0.5 * float((phi * rho).sum()) * dv
+            sp = a / math.sqrt(2.0)
+            p0_ex = -(2 / math.sqrt(math.pi)) / (math.sqrt(2) * sp) * _F(lz / sp)
+            Eg_ex = -0.5 * (2 / math.sqrt(math.pi)) / (math.sqrt(2) * a) * _F(lz / a)
+            mq = max(mq, row("q2D_gauss", f"lz={lz},a={a}", "Egrav", abs(Eg - Eg_ex) / abs(Eg_ex)),
+                     row("q2D_gauss", f"lz={lz},a={a}", "phi0", abs(_center(phi) - p0_ex) / abs(p0_ex)))
+    ml = 0.0
+    for m in (0, 1, 10, 30):
+        z = (G["X"] + 1j * G["Y"]).to(torch.complex128)
+        psi = (z ** m) * torch.exp(-0.5 * G["R2"])
+        psi = psi / norm_of(psi, G)
+        comp = energy_components(psi, G, 0.0, 0.0, 1.0, 0.0)
+        Esp = (comp["Ekin"] + comp["Etrap"] + comp["Erot"]).item()
+        r2 = float((G["R2"] * (psi.real ** 2 + psi.imag ** 2)).sum()) * dv
+        ml = max(ml, row("2D_LLL", f"m={m}", "max|E-1|,|Lz-m|,|r2-m-1|",
+                         max(abs(Esp - 1.0), abs(comp["Lz"].item() - m), abs(r2 - (m + 1)))))
+    mtf = 0.0
+    for ndim in (2, 3):
+        Gf = make_grid(dict(cfg, dimension=("2D" if ndim == 2 else "3D"), s=2, Omega=1.0, kernel="none",
+                            beta2=-1.0, beta3=1.0, G_C=0.0, L=(20.0 if ndim == 2 else 8.0),
+                            Ngrid=(192 if ndim == 2 else 96)))
+        for b2, b3 in ((-0.5, 1.0), (-1.5, 1.0)):
+            tf = extract_tf(dict(cfg, dimension=("2D" if ndim == 2 else "3D"), s=2, Omega=1.0, kernel="none",
+                                 beta2=b2, beta3=b3, G_C=0.0), Gf)
+            rho0 = -3 * b2 / (4 * b3)
+            mu = b2 * rho0 + b3 * rho0 ** 2
+            e = max(abs(tf["rho0"] - rho0) / rho0, abs(tf["mu"] - mu) / max(abs(mu), 1e-12))
+            mtf = max(mtf, row(f"{ndim}D_flat_tf", f"b2={b2}", "rho0_mu", e))
+    G3 = make_grid(dict(cfg, dimension="3D", s=2, kernel="newton", G_C=1.0, L=8.0, Ngrid=96))
+    m3 = 0.0
+    for sig in (1.0, 1.5):
+        phi = long_range_phi(_gauss_rho(G3, sig), G3, 1.0)
+        ex = -2.0 / (math.sqrt(math.pi) * sig)
+        m3 = max(m3, row("3D_gauss", f"sig={sig}", "phi0", abs(_center(phi) - ex) / abs(ex)))
+    kernel_orders = {}
+    for name, ndim, kern, Ns, sig, L in (("2D", 2, "newton", (192, 256, 384, 512), 1.0, 10.0),
+                                        ("2D_log", 2, "log", (96, 128, 192, 256), 1.0, 10.0),
+                                        ("3D", 3, "newton", (64, 96, 128, 160), 1.0, 8.0)):
+        ex = (math.log(sig) - 0.5 * _EULER_GAMMA) if kern == "log" else (
+            (-math.sqrt(math.pi) / sig) if ndim == 2 else (-2.0 / (math.sqrt(math.pi) * sig)))
+        dxs, errs = [], []
+        for Ni in Ns:
+            Gk = make_grid(dict(cfg, dimension=("2D" if ndim == 2 else "3D"), s=2, kernel=kern, G_C=1.0,
+                                L=L, Ngrid=int(Ni)))
+            phi = long_range_phi(_gauss_rho(Gk, sig), Gk, 1.0)
+            dxs.append(Gk["dx"])
+            errs.append(abs(_center(phi) - ex) / abs(ex))
+        order = float(np.polyfit(np.log(dxs), np.log(np.maximum(errs, 1e-16)), 1)[0])
+        kernel_orders[name] = order
+        for dxi, ei in zip(dxs, errs):
+            rows.append(dict(test=f"{name}_kernel_order", case=f"dx={dxi:.4f}", quantity="phi0_relerr",
+                             rel_error=ei, order=order))
+    passed = (mg < 5e-3 and mp < 5e-3 and mlg < 5e-3 and mlp < 5e-3 and mq < 5e-3
+              and ml < 1e-4 and mtf < 1e-12 and m3 < 5e-3)
+    order_ok = all(o > 1.7 for o in kernel_orders.values())
+    print(f"[validate] 2D-Coulomb Eg={mg:.2e} phi0={mp:.2e} | 2D-log Eg={mlg:.2e} phi0={mlp:.2e} | "
+          f"q2D={mq:.2e} | LLL={ml:.2e} | flatTF={mtf:.2e} | 3D-phi0={m3:.2e} | "
+          f"kernel_order={kernel_orders} | PASS={passed and order_ok}")
+    return dict(passed=passed and order_ok, mg=mg, mp=mp, mlg=mlg, mlp=mlp, mq=mq, ml=ml, mtf=mtf, m3=m3,
+                kernel_orders=kernel_orders, rows=rows)
+
+
+# -----------------------------------------------------------------------------
+# smoke
+# -----------------------------------------------------------------------------
+def mode_smoke(cfg):
+    """modes.py:440-459: the gate at validate_N=256 and one small run."""
+    gate = mode_validate(dict(cfg, validate_N=256))
+    small = dict(cfg, L=10.0, Ngrid=(128 if geometry(cfg)[0] == 2 else 72), maxit=1500, res_tol=3e-3,
+                 nseeds=1, tag="smoke")
+    d = mode_single(small)
+    run_ok = bool(d) and bool(d.get("converged", False))
+    ok = bool(gate["passed"]) and run_ok
+    print(f"[smoke] validation_passed={gate['passed']} run_converged={run_ok} OVERALL={ok}")
+    return dict(gate=gate, run=d, ok=ok)
+
+
+def _not_here(name):
+    def fn(cfg):
+        raise NotImplementedError(f"mode {name!r} is not part of this package (ground-state path only); "
+                                  "use the reference for it")
+    return fn
+
+
+DISPATCH = dict(validate=mode_validate, smoke=mode_smoke, single=mode_single,
+                tf_only=_not_here("tf_only"), sweep=mode_sweep, convergence=_not_here("convergence"))
+
+
+def run(cfg):
+    """Dispatch a configuration to its mode and return the result (modes.py:473-500)."""
+    mode = cfg.get("mode", "smoke")
+    fn = DISPATCH.get(mode)
+    if fn is None:
+        raise ValueError(f"unknown mode {mode!r}; choose {sorted(DISPATCH)}")
+    kkind = resolve_kernel(cfg)[0]
+    requested = str(cfg.get("kernel", "auto")).lower()
+    if requested in ("newton", "log") and kkind == "none" and abs(float(cfg.get("G_C", 0.0))) <= 1e-15:
+        print(f"[warn] kernel={requested!r} is set but G_C=0, so the long-range term is OFF "
+              f"(reported kernel='none'). Set G_C>0 to activate it, e.g. G_C=20.")
+    print(f"becGPP-amd {CODE_VERSION} | device={DEV} | mode={mode} | dim={cfg.get('dimension')} kernel={kkind}")
+    return fn(cfg)

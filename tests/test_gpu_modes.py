@@ -1,0 +1,50 @@
+"""run(cfg) on the GPU (becgpp/modes.py): the validation gate passes, single
+and sweep return what the solver returns for the same seeds, and a sweep
+without continuation (batched) equals the point-by-point solves."""
+import pytest
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(B, **kw):
+    return B.default_cfg(**dict(dict(L=6.0, Ngrid=48, res_tol=1e-8, maxit=20000), **kw))
+
+
+def test_validate_mode_passes():
+    import becgpp_amd as B
+    g = B.run(B.default_cfg(mode="validate", validate_N=256))
+    assert g["passed"], g
+    assert g["ml"] < 1e-12 and g["mtf"] < 1e-12
+
+
+def test_single_mode_equals_multiseed():
+    import becgpp_amd as B
+    c = _cfg(B, mode="single", beta2=40.0, nseeds=2)
+    d = B.run(c)
+    _, _, obs = B.ground_state_multiseed(c)
+    assert abs(d["E"] - obs["E"]) <= 1e-10 * abs(obs["E"])
+    assert d["rid"] == B.run_id(c) and d["converged"]
+
+
+def test_sweep_without_continuation_is_batched_seeds():
+    import becgpp_amd as B
+    c = _cfg(B, mode="sweep", sweep_param="beta2", sweep_values=[20.0, 60.0], nseeds=2,
+             sweep_continuation=False)
+    rows = B.run(c)
+    assert [r["beta2"] for r in rows] == [20.0, 60.0]
+    for r in rows:
+        cp = dict(c, beta2=r["beta2"])
+        _, _, oo = O.ground_state_multiseed(dict(cp, mode="single"))
+        assert abs(r["E"] - oo["E"]) <= 1e-8 * abs(oo["E"]), (r["E"], oo["E"])
+
+
+def test_sweep_with_continuation_keeps_lowest_candidate():
+    import becgpp_amd as B
+    c = _cfg(B, mode="sweep", sweep_param="Omega", sweep_values=[0.0, 0.2], nseeds=2, beta2=30.0)
+    rows = B.run(c)
+    assert rows[1]["branch"] in ("continuation", "seed:tf", "seed:gaussian")
+    cand = [float(x.split("=")[1]) for x in rows[1]["candidate_E"].split(";")]   # printed to 10 digits
+    assert abs(rows[1]["E"] - min(cand)) <= 1e-9 * abs(min(cand))

@@ -88,3 +88,30 @@ def test_product_never_imports_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in txt.replace("the oracle", ""), f
+
+
+def test_run_dispatch_errors():
+    """modes.py:473-478: an unknown mode is a ValueError; the modes outside the
+    ground-state path say so; sweep arguments are validated (modes.py:312-323)."""
+    import becgpp_amd as B
+    with pytest.raises(ValueError):
+        B.run(B.default_cfg(mode="single") | {"mode": "nope"})
+    for m in ("tf_only", "convergence"):
+        with pytest.raises(NotImplementedError):
+            B.run(B.default_cfg(mode=m))
+    with pytest.raises(ValueError):
+        B.modes.mode_sweep(B.default_cfg(mode="sweep", sweep_param="seed"))
+    with pytest.raises(ValueError):
+        B.modes.mode_sweep(B.default_cfg(mode="sweep", sweep_values=[1.0, True]))
+
+
+def test_run_id_is_the_reference_formula():
+    """io.py:22-26: tag + md5 of the code version and the state-defining keys;
+    excluded keys (maxit, mode, tag...) leave the hash unchanged."""
+    import becgpp_amd as B
+    c = B.default_cfg()
+    a = B.run_id(c)
+    assert a.startswith("becgpp_") and len(a) == len("becgpp_") + 10
+    assert B.run_id(dict(c, maxit=5, mode="sweep")) == a
+    assert B.run_id(dict(c, beta2=1.0)) != a
+    assert B.run_id(dict(c, tag="x")) == "x_" + a.split("_")[1]
