@@ -48,6 +48,24 @@ WORKLOADS = {
         cfg=dict(mode="single", dimension="2D", s=2, Omega=0.0, beta2=100.0, beta3=0.0, G_C=0.0,
                  kernel="none", L=16.0, Ngrid=128, nseeds=1, res_tol=1e-4, maxit=60000),
         desc="2D harmonic plain solve, beta2=100, L=16, N=128"),
+    # 3D self-gravitating boson star: 1/r kernel, N=128 (convolution on 256^3), one solve
+    "newton_3d_n128": dict(
+        cfg=dict(mode="single", dimension="3D", s=2, trap_coeff=0.5, Omega=0.0, beta2=0.0, beta3=0.0, G_C=20.0,
+                 kernel="newton", L=12.0, Ngrid=128, nseeds=1, seed="gaussian", seed_noise=1e-3,
+                 res_tol=1e-6, maxit=60000, linesearch_max=20),
+        desc="3D Newton boson star, G_C=20, beta2=0, L=12, N=128 (conv 256^3), Gaussian seed with 1e-3 "
+             "noise, res_tol=1e-6, linesearch_max=20"),
+    # quasi-2D pancakes: K_eff(l_z=0.5); Omega x G_C = 8 points x 4 seeds = 32 solves on three grids,
+    # instance i on N = (128, 192, 256)[i mod 3]
+    "q2d_batch_uneven": dict(
+        cfg=dict(mode="single", dimension="quasi2D", s=2, trap_coeff=0.5, Omega=0.0, beta2=50.0, beta3=0.0,
+                 G_C=1.0, kernel="q2d", l_z=0.5, L=16.0, Ngrid=128, nseeds=4, res_tol=1e-6, maxit=60000,
+                 linesearch_max=20),
+        points=[dict(Omega=om, G_C=gc) for om in (0.0, 0.5) for gc in (1.0, 2.0, 5.0, 10.0)],
+        instance_grid=lambda i: dict(Ngrid=(128, 192, 256)[i % 3]),
+        desc="quasi-2D q2d kernel l_z=0.5, beta2=50, L=16, Omega in {0,0.5} x G_C in {1,2,5,10}, nseeds=4 "
+             "(32 solves), N = 128/192/256 round-robin by instance id, one batch per grid, res_tol=1e-6, "
+             "linesearch_max=20"),
     # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
     "droplet_sweep_2d_n256": dict(
         cfg=dict(mode="single", dimension="2D", s=2, trap_coeff=0.0, Omega=0.0, beta2=-120.0, beta3=1000.0,
@@ -67,6 +85,8 @@ WORKLOADS = {
 KERNELS = {
     114: ("k_kin<A2X,KH_LAST> (accepted state: psi update, x pass of H psi, local terms, mu psi, residual rows)",
           lambda n: 128 * n),   # reads psi, d, H share, old g; writes psi, s = change, g, y = change
+    144: ("k_kin<A3X,KH_LAST> (accepted state: psi update, x pass of H psi, local terms, mu psi, residual rows)",
+          lambda n: 144 * n),   # as 114, plus phi read and stored (long range)
     100: ("k_kin<A2Y,KE_FIRST> (trial energy: new direction, y pass, Parseval and local sums, H share)",
           lambda n: 80 * n),    # reads psi, d, z; writes d, H share (long range: +8n for |psi|^2)
     111: ("k_kin<A2X,KE_REST> (trial energy, x pass)", lambda n: 32 * n),
@@ -84,7 +104,9 @@ def parse():
                     help="timed steps (0: no timed region, only the event-timed replay -- for profiling)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rotating_lattice_2d_n384", choices=sorted(WORKLOADS))
-    ap.add_argument("--watch", type=int, default=114, help="launch-site id of the roofline kernel")
+    ap.add_argument("--watch", type=int, default=None,
+                    help="launch-site id of the roofline kernel (default: the accepted-state H pass, "
+                         "114 in 2D, 144 in 3D)")
     ap.add_argument("--measure-iters", type=int, default=300,
                     help="CG iterations of the event-timed replay of the roofline kernel")
     ap.add_argument("--cpu-sample-iters", type=int, default=60)
@@ -132,40 +154,61 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    G = B.make_grid(cfg)
-    n = int(G["N"]) ** int(G["ndim"])
     nseeds = int(cfg["nseeds"])
-    rotating = abs(float(cfg["Omega"])) >= 0.3
-    cycle = ("triangular", "tf", "gaussian") if rotating else ("tf", "gaussian")
+    if a.watch is None:
+        a.watch = 144 if str(cfg["dimension"]).lower() == "3d" else 114
     points = wl.get("points", [{}])
-    runs = []                                   # (point index, cfg, psi0)
+    igrid = wl.get("instance_grid", lambda i: {})
+    grids = {}                                  # grid key -> G
+    runs = []                                   # (point index, cfg, psi0, grid key)
     for ip, over in enumerate(points):
+        cp = dict(cfg, **over)
+        rotating = abs(float(cp["Omega"])) >= 0.3
+        cycle = ("triangular", "tf", "gaussian") if rotating else ("tf", "gaussian")
         for j in range(nseeds):
-            c = dict(cfg, **over)
-            c.update(rng=int(cfg["rng"]) + j, seed=cycle[j % len(cycle)])
-            runs.append((ip, c, make_seed(G, c).contiguous()))
+            c = dict(cp, **igrid(len(runs)))
+            c.update(rng=int(cfg["rng"]) + j, seed=cycle[j % len(cycle)] if nseeds > 1 else cfg["seed"])
+            gk = (int(c["Ngrid"]), float(c["L"]))
+            if gk not in grids:
+                grids[gk] = B.make_grid(c)
+            runs.append((ip, c, make_seed(grids[gk], c).contiguous(), gk))
     torch.cuda.synchronize()
-    cs = [c for _, c, _ in runs]
-    psi0s = [p for _, _, p in runs]
+    cs = [c for _, c, _, _ in runs]
+    psi0s = [p for _, _, p, _ in runs]
     nsol = len(runs)
+    gk0 = runs[0][3]
+    G = grids[gk0]
+    n = int(G["N"]) ** int(G["ndim"])
+    groups = {}
+    for k, r in enumerate(runs):
+        groups.setdefault(r[3], []).append(k)
 
     def select(res):
         """Per point: the lowest-energy seed (solvers.py:238-263)."""
         best = {}
-        for (ip, _, _), (psi, _, obs) in zip(runs, res):
+        for (ip, _, _, _), (psi, _, obs) in zip(runs, res):
             if ip not in best or obs["E"] < best[ip][1]["E"]:
                 best[ip] = (psi, obs)
         return best
 
+    def solve_all():
+        """Every instance through the batched solver, one batch set per grid."""
+        res = [None] * nsol
+        for gk, ks in groups.items():
+            out = B.ground_state_batch([cs[k] for k in ks], G=grids[gk], psi0s=[psi0s[k] for k in ks])
+            for k, r in zip(ks, out):
+                res[k] = r
+        return res
+
     def step():
-        res = B.ground_state_batch(cs, G=G, psi0s=psi0s)
+        res = solve_all()
         best = select(res)
         Es = [obs["E"] for _, _, obs in res]
         stops = [f'{obs["stop_reason"]}@{obs["iters"]}' for _, _, obs in res]
         return best, sum(obs["iters"] for _, _, obs in res), Es, stops
 
     def step_single():
-        res = [B.ground_state(c, G=G, psi0=p0, verbose=False) for c, p0 in zip(cs, psi0s)]
+        res = [B.ground_state(c, G=grids[r[3]], psi0=p0, verbose=False) for c, p0, r in zip(cs, psi0s, runs)]
         return select(res), sum(obs["iters"] for _, _, obs in res), [obs["E"] for _, _, obs in res]
 
     lib = _lib.load()
@@ -187,8 +230,9 @@ def main():
     # events; the roofline kernel is timed with HIP events on its stream in a
     # replay of the same batched workload with direct launches.
     lib.bec_prof_watch(int(a.watch))
-    B.ground_state_batch([dict(c, maxit=int(a.measure_iters), res_tol=0.0) for c in cs[:32]], G=G,
-                         psi0s=psi0s[:32])
+    k0 = groups[gk0][:32]
+    B.ground_state_batch([dict(cs[k], maxit=int(a.measure_iters), res_tol=0.0) for k in k0], G=G,
+                         psi0s=[psi0s[k] for k in k0])
     torch.cuda.synchronize()
     ms = ctypes.c_double()
     cnt = ctypes.c_longlong()
@@ -211,7 +255,7 @@ def main():
     # summary of this workload's replay (rocprofv3 FETCH_SIZE / WRITE_SIZE,
     # calibrated on a device copy)
     tfile = os.path.join(HERE, "profiles", f"{a.workload}_hbm_traffic.md")
-    if a.watch == 114 and os.path.exists(tfile):
+    if a.watch in (114, 144) and os.path.exists(tfile):
         m = re.search(r"traffic_bytes_per_dispatch: ([0-9.]+)", open(tfile).read())
         if m:
             roof["traffic"] = float(m.group(1))
@@ -231,7 +275,8 @@ def main():
         value=a.steps * npt / wall, unit="ground states/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
         ms_per_step=1e3 * wall / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64", data="synthetic",
-        config=dict(workload=a.workload, desc=wl["desc"], Ngrid=int(G["N"]), ndim=int(G["ndim"]),
+        config=dict(workload=a.workload, desc=wl["desc"], Ngrid=sorted({gk[0] for gk in grids}) if len(grids) > 1
+                    else int(G["N"]), ndim=int(G["ndim"]),
                     points=npt, nseeds=nseeds, solves_per_step=nsol, res_tol=float(cfg["res_tol"]),
                     path="batched"),
         solves_per_s=a.steps * nsol / wall, cg_iters_per_s=tot_iters / wall,
@@ -246,7 +291,7 @@ def main():
     if not a.no_polish:
         ib = min((k for k in range(nsol) if runs[k][0] == 0), key=lambda k: Es[k])
         pc = dict(cs[ib], res_tol=1e-9, maxit=20000)
-        _, _, pol = B.ground_state(pc, G=G, psi0=b0psi, verbose=False)
+        _, _, pol = B.ground_state(pc, G=grids[runs[ib][3]], psi0=b0psi, verbose=False)
         out["E_polished"] = pol["E"]
         out["dE_rel_vs_polished"] = abs(b0["E"] - pol["E"]) / abs(pol["E"])
     if not a.no_single:
