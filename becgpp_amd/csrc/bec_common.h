@@ -22,6 +22,7 @@ struct FftPlan {
   int ns[BEC_MAX_STAGES];     // Ns of the stage: product of the earlier radices
   int step[BEC_MAX_STAGES];   // twiddle stride L / (Ns R)
   unsigned mag[BEC_MAX_STAGES];   // floor(2^32 / Ns) + 1: j / Ns = umulhi(j, mag) for j < 2^12
+  int reg;                    // 1: L = 8 tpl and the first and last radix divide 8 (fft_reg)
   const double2* tw;          // tw[m] = exp(-2 pi i m / L), m in [0, L)
 };
 
@@ -166,6 +167,110 @@ __device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const doub
     else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Register-resident line transform.  A thread holds the 8 elements lane + m tpl
+// (m < 8) of its line, L = 8 tpl.  The first stage (Ns = 1) takes its radix-R
+// butterflies j = lane + b tpl straight from those registers (inputs
+// j + r L/R = lane + (b + r 8/R) tpl); the last stage (Ns = L/R) leaves its
+// outputs j + r Ns = lane + (b + r 8/R) tpl in them.  Only the stages between
+// go through the LDS line, so a forward transform, a pointwise multiply and an
+// inverse transform keep the spectrum in registers across the multiply.
+// ----------------------------------------------------------------------------
+template <int R, int SIGN>
+__device__ __forceinline__ void reg_first(double2 (&v)[8], double2* s, int lane, int tpl, bool active) {
+  constexpr int Q = 8 / R;
+  __syncthreads();                     // earlier readers of s are done
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < Q; ++b) {
+      double2 u[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[r] = v[b + r * Q];
+      bfly<R, SIGN>(u);
+      const int j = lane + b * tpl;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[j * R + r] = u[r];
+    }
+  }
+  __syncthreads();
+}
+template <int R, int SIGN>
+__device__ __forceinline__ void reg_last(double2 (&v)[8], const double2* s, int L, int lane, int tpl,
+                                         const double2* __restrict__ tw, bool active) {
+  constexpr int Q = 8 / R;
+  const int ns = L / R;                // twiddle stride L / (Ns R) = 1
+  if (!active) return;
+#pragma unroll
+  for (int b = 0; b < Q; ++b) {
+    const int j = lane + b * tpl;
+    double2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = s[j + r * ns];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      double2 w = tw[r * j];
+      if (SIGN > 0) w.y = -w.y;
+      u[r] = cmul(u[r], w);
+    }
+    bfly<R, SIGN>(u);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[b + r * Q] = u[r];
+  }
+}
+template <int SIGN>
+__device__ __forceinline__ void fft_reg(double2 (&v)[8], double2* s, const FftPlan& p, const double2* twl,
+                                        int lane, int tpl, bool active) {
+  const int n = p.nst;
+  if (n == 1) {                        // L = 8, one thread per line
+    if (active) bfly<8, SIGN>(v);
+    return;
+  }
+  const int r0 = p.rad[0], rl = p.rad[n - 1];
+  if (r0 == 8) reg_first<8, SIGN>(v, s, lane, tpl, active);
+  else if (r0 == 4) reg_first<4, SIGN>(v, s, lane, tpl, active);
+  else reg_first<2, SIGN>(v, s, lane, tpl, active);
+  for (int st = 1; st < n - 1; ++st) {
+    const int R = p.rad[st], Ns = p.ns[st], step = p.step[st];
+    const unsigned mag = p.mag[st];
+    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+  }
+  if (rl == 8) reg_last<8, SIGN>(v, s, p.L, lane, tpl, twl, active);
+  else if (rl == 4) reg_last<4, SIGN>(v, s, p.L, lane, tpl, twl, active);
+  else reg_last<2, SIGN>(v, s, p.L, lane, tpl, twl, active);
+}
+// The DFT of a line whose elements lane + m tpl (m < 8, those below L) are in
+// v: register-resident when the plan allows it, else through the LDS line s
+// (fft_lds).  Every thread of the workgroup calls it; results return in v.
+template <int SIGN>
+__device__ __forceinline__ void fft_line(double2 (&v)[8], double2* s, const FftPlan& p, const double2* twl,
+                                         int lane, int tpl, bool active) {
+  if (p.reg) {
+    fft_reg<SIGN>(v, s, p, twl, lane, tpl, active);
+    return;
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int e = lane + m * tpl;
+      if (e < p.L) s[e] = v[m];
+    }
+  }
+  __syncthreads();
+  fft_lds<SIGN>(s, p, twl, lane, tpl, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int e = lane + m * tpl;
+      v[m] = (e < p.L) ? s[e] : make_double2(0.0, 0.0);
+    }
   }
 }
 

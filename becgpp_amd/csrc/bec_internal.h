@@ -235,7 +235,8 @@ void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s);
 // ---------------------------------------------------------------------------
 // launch wrappers (bec_kernels.hip); reducing launches return rows/instance
 // ---------------------------------------------------------------------------
-LineCfg line_cfg(int L);
+LineCfg line_cfg(const FftPlan& fp);
+int lpw_min(int tpl);                  // fewest lines per workgroup: whole wavefronts
 int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
 int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
 int elem_blocks(long long n);

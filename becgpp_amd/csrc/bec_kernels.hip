@@ -170,12 +170,17 @@ __device__ __forceinline__ KinScal kin_scal_all(const KinArgs& a, int b) {
 struct LineOwn {
   int cc, e0;                 // block line and first element of the thread
 };
+// LDS pitch of a block's lines.  On a strided axis adjacent threads hold the
+// same element of adjacent lines; one extra element per line puts those 16-byte
+// accesses in different banks.
+template <bool CONTIG>
+__host__ __device__ __forceinline__ int lpitch(int L) { return CONTIG ? L : L + 1; }
 template <bool CONTIG>
 __device__ __forceinline__ LineOwn line_owner(int tpl, int lpw) {
   LineOwn o;
   if (CONTIG) {
-    o.cc = threadIdx.x >> (__ffs(tpl) - 1);
-    o.e0 = threadIdx.x & (tpl - 1);
+    o.cc = threadIdx.x / tpl;
+    o.e0 = threadIdx.x - o.cc * tpl;
   } else {
     o.cc = threadIdx.x & (lpw - 1);
     o.e0 = threadIdx.x >> (__ffs(lpw) - 1);
@@ -217,11 +222,10 @@ template <int AX, int KIND>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 smem[];
   double2* twl = smem;                 // twiddles [N]
-  double2* lds = smem + a.N;           // lines [lpw][N]
   using X = Ax<AX>;
+  double2* lds = smem + a.N;           // lines [lpw][lpitch<X::contig>(N)]
   constexpr bool KE = (KIND == KE_FIRST || KIND == KE_REST);
   const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
-  const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -234,13 +238,12 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   // a workgroup's lines span instances b0 and b1 only
   const KinScal S0 = kin_scal_all(a, b0);
   const KinScal S1 = (b1 == b0) ? S0 : kin_scal_all(a, b1);
-  const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && (mb == b0 ? on0 : on1);
   const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
   stage_twiddles(twl, a.fp);
-  // the line this thread loads and stores
+  // the line this thread loads, transforms and stores, and its elements lane + j tpl
   const LineOwn ow = line_owner<X::contig>(tpl, lpw);
+  const int lane = ow.e0;
   const int ol = l0 + ow.cc;
   const int ob = ol / nl;
   const bool oon = ol < total && (ob == b0 ? on0 : on1);
@@ -249,8 +252,9 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const KinScal& OS = (ob == b0) ? S0 : S1;
   double c2[3] = {0.0, 0.0, 0.0};
   if (KIND == KE_FIRST || KIND == KH_LAST) line_sq<AX>(oli, N, a.dx, c2);
-  double2* olds = lds + ow.cc * N;
+  double2* s = lds + ow.cc * lpitch<X::contig>(N);
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
+  double2 x[BEC_EPT];
   {
     // phase 1: issue every load of this thread's elements
     double2 p0[BEC_EPT], dd[BEC_EPT];
@@ -293,6 +297,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
       const int e = ow.e0 + j * tpl;
+      x[j] = make_double2(0.0, 0.0);
       if (e >= N) continue;
       double2 v = make_double2(0.0, 0.0);
       if (oon) {
@@ -317,27 +322,27 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
           if (a.rho_out) a.rho_out[adr] = rho;
         }
       }
-      olds[e] = v;
+      x[j] = v;
     }
   }
-  __syncthreads();
-  double2* s = lds + c * N;
-  fft_lds<-1>(s, a.fp, twl, lane, tpl, active);
+  fft_line<-1>(x, s, a.fp, twl, lane, tpl, oon);
   const double invN = 1.0 / (double)N;
-  const int li0 = (l0 + c) - mb * nl;
-  const double lzc = (X::lsign == 0 || !active) ? 0.0 : (double)X::lsign * xcoord(X::lidx(li0, N), N, a.dx);
+  const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
   const bool mult = !KE || (KIND == KE_FIRST && a.hy_out);
-  const double rfl = active ? ((mb == b0) ? S0.rf : S1.rf) : 0.0;
+  const double rfl = oon ? OS.rf : 0.0;
   double kin = 0.0, lz = 0.0;
-  if (active) {
-    for (int k = lane; k < N; k += tpl) {
+  if (oon) {
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int k = lane + j * tpl;
+      if (k >= N) continue;
       const double kv = kfreq(k, N, a.invNdx);
       if (KE) {
-        const double a2 = cabs2(s[k]);
+        const double a2 = cabs2(x[j]);
         kin += 0.5 * kv * kv * a2;
         lz += kv * a2;
       }
-      if (mult) s[k] = cscale(s[k], a.kf * kv * kv + rfl * lzc * kv);
+      if (mult) x[j] = cscale(x[j], a.kf * kv * kv + rfl * lzc * kv);
     }
   }
   if (KE) {
@@ -354,8 +359,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     }
     if (!mult) return;
   }
-  __syncthreads();
-  fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
+  fft_line<+1>(x, s, a.fp, twl, lane, tpl, oon);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   if (oon) {
@@ -380,7 +384,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       const int e = ow.e0 + j * tpl;
       if (e >= N) continue;
       const long long adr = obase + e * strd;
-      double2 u = olds[e];
+      double2 u = x[j];
       u.x *= invN;
       u.y *= invN;
       if (KE) {
@@ -451,8 +455,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   extern __shared__ double2 smem[];
   const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
   double2* twl = smem;                 // twiddles [L]
-  double2* lds = smem + L;             // lines [lpw][L]
-  const int c = threadIdx.x / tpl, lane = threadIdx.x - c * tpl;
+  double2* lds = smem + L;             // lines [lpw][lpitch<CONTIG>(L)]
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -463,25 +466,26 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   for (int b = b0 + 1; b < b1 && !any; ++b) any = act_on(a.act, b);
   if (!any) return;
   auto inst_active = [&](int b) { return b == b0 ? on0 : (b == b1 ? on1 : act_on(a.act, b)); };
-  const int mb = (l0 + c) / nl;
-  const bool active = (l0 + c) < total && inst_active(mb);
   const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
   stage_twiddles(twl, a.fp);
-  // the line this thread loads and stores, and its elements e0 + j tpl
+  // the line this thread loads, transforms and stores, and its elements lane + j tpl
   const LineOwn ow = line_owner<CONTIG>(tpl, lpw);
+  const int lane = ow.e0;
   const int ol = l0 + ow.cc;
   const int ob = ol / nl;
   const bool oon = ol < total && inst_active(ob);
   const int oli = ol - ob * nl;
   const int oo = oli / a.ni, oi = oli - oo * a.ni;
-  double2* olds = lds + ow.cc * L;
+  double2* s = lds + ow.cc * lpitch<CONTIG>(L);
+  double2 v[BEC_EPT];
+  // kernel spectrum of the line's elements, loaded with the line itself
+  double kh[(MUL == MUL_KHAT2 || MUL == MUL_KHAT3) ? BEC_EPT : 1];
   {
     const long long ibase = ob * a.in_ib + oo * a.in_so + oi * a.in_si;
-    double2 v[BEC_EPT];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int e = ow.e0 + j * tpl;
+      const int e = lane + j * tpl;
       v[j] = make_double2(0.0, 0.0);
       if (!oon || e >= L) continue;
       if (LD == LD_HERM) {
@@ -497,49 +501,51 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         }
       }
     }
+    if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
+      // 2D spectrum [kx][ky] (element stride H); 3D spectrum [ky][kx][kz], the
+      // line's kz column contiguous
+      long long kb;
+      if (MUL == MUL_KHAT2) {
+        kb = oli;                                      // column index ky in [0, H)
+      } else {
+        const int ky = oli / a.H, kx = oli - (oli / a.H) * a.H;
+        kb = ((long long)min(ky, a.M - ky) * a.H + kx) * a.H;
+      }
+      const long long kst = (MUL == MUL_KHAT2) ? (long long)a.H : 1;
 #pragma unroll
-    for (int j = 0; j < BEC_EPT; ++j) {
-      const int e = ow.e0 + j * tpl;
-      if (e < L) olds[e] = v[j];
+      for (int j = 0; j < BEC_EPT; ++j) {
+        const int k = lane + j * tpl;
+        kh[j] = (oon && k < L) ? a.khat[kb + min(k, L - k) * kst] : 0.0;
+      }
     }
   }
-  __syncthreads();
-  double2* s = lds + c * L;
-  if (DIR == DIR_INV) fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
-  else fft_lds<-1>(s, a.fp, twl, lane, tpl, active);
+  const double sig = ((MUL == MUL_PRE2 || MUL == MUL_PRE3) && oon) ? a.sigma[ob] : 0.0;
+  if (DIR == DIR_INV) fft_line<+1>(v, s, a.fp, twl, lane, tpl, oon);
+  else fft_line<-1>(v, s, a.fp, twl, lane, tpl, oon);
   if (DIR == DIR_FMI) {
-    if (active) {
-      const int li = (l0 + c) - mb * nl;
+    if (oon) {
       if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
-        int other;
-        if (MUL == MUL_KHAT2) {
-          other = li;                                  // column index ky in [0, H)
-        } else {
-          const int ky = li / a.H, kz = li - (li / a.H) * a.H;
-          other = min(ky, a.M - ky) * a.H + kz;
-        }
-        const long long plane = (MUL == MUL_KHAT2) ? (long long)a.H : (long long)a.H * a.H;
-        for (int k = lane; k < L; k += tpl) {
-          const int kk = min(k, L - k);
-          s[k] = cscale(s[k], a.khat[kk * plane + other]);
-        }
+#pragma unroll
+        for (int j = 0; j < BEC_EPT; ++j) v[j] = cscale(v[j], kh[j]);
       } else {
         // denominator 1/2 (kx^2 + ky^2 [+ kz^2]) + sigma, summed in that order
-        const double sig = a.sigma[mb];
-        const double ky = kfreq(MUL == MUL_PRE2 ? li : li / a.Nk, a.Nk, a.invNdx);
-        const double kz = (MUL == MUL_PRE3) ? kfreq(li % a.Nk, a.Nk, a.invNdx) : 0.0;
-        for (int k = lane; k < L; k += tpl) {
+        const double ky = kfreq(MUL == MUL_PRE2 ? oli : oli / a.Nk, a.Nk, a.invNdx);
+        const double kz = (MUL == MUL_PRE3) ? kfreq(oli % a.Nk, a.Nk, a.invNdx) : 0.0;
+#pragma unroll
+        for (int j = 0; j < BEC_EPT; ++j) {
+          const int k = lane + j * tpl;
+          if (k >= L) continue;
           const double kx = kfreq(k, a.Nk, a.invNdx);
           double k2 = kx * kx + ky * ky;
           if (MUL == MUL_PRE3) k2 = k2 + kz * kz;
           const double den = 0.5 * k2 + sig;
-          s[k] = make_double2(s[k].x / den, s[k].y / den);
+          v[j] = make_double2(v[j].x / den, v[j].y / den);
         }
       }
     }
-    __syncthreads();
-    fft_lds<+1>(s, a.fp, twl, lane, tpl, active);
+    fft_line<+1>(v, s, a.fp, twl, lane, tpl, oon);
   }
+  // stores: line position q = lane + j tpl goes to output element q - off_store
   double dsum[1] = {0.0};
   double zv[ST == ST_CDOT ? 16 : 1];
 #pragma unroll
@@ -551,8 +557,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     double2 fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int e = ow.e0 + j * tpl;
-      if (e >= a.n_store) continue;
+      const int q = lane + j * tpl, e = q - a.off_store;
+      if (q >= L || e < 0 || e >= a.n_store) continue;
       const long long adr = obase + e * a.out_s;
       if (ST == ST_RE && a.dot) dv[j] = a.dot[adr];
       if (ST == ST_CDOT) {
@@ -562,18 +568,18 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     }
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
-      const int e = ow.e0 + j * tpl;
-      if (e >= a.n_store) continue;
+      const int q = lane + j * tpl, e = q - a.off_store;
+      if (q >= L || e < 0 || e >= a.n_store) continue;
       const long long adr = obase + e * a.out_s;
-      const double2 v = olds[a.off_store + e];
+      const double2 w = v[j];
       if (ST == ST_CDOT) {
-        const double2 zi = make_double2(v.x * a.out_scale, v.y * a.out_scale);
+        const double2 zi = make_double2(w.x * a.out_scale, w.y * a.out_scale);
         ((double2*)a.out)[adr] = zi;
         dot_accum(zv, zi, fp[j], fd[j], fg[j], fy[j], fs[j]);
       } else if (ST == ST_C) {
-        ((double2*)a.out)[adr] = make_double2(v.x * a.out_scale, v.y * a.out_scale);
+        ((double2*)a.out)[adr] = make_double2(w.x * a.out_scale, w.y * a.out_scale);
       } else {
-        const double r = v.x * a.out_scale;
+        const double r = w.x * a.out_scale;
         ((double*)a.out)[adr] = r;
         if (a.dot) dsum[0] += r * dv[j];
       }
@@ -1076,8 +1082,13 @@ __global__ __launch_bounds__(BEC_WG) void k_extract_khat(int ndim, int M, int H,
   const long long n = (ndim == 2) ? (long long)H * H : (long long)H * H * H;
   ELEM_LOOP(n) {
     long long src;
-    if (ndim == 2) src = (i / H) * M + (i % H);
-    else src = ((i / ((long long)H * H)) * M + (i / H) % H) * M + (i % H);
+    if (ndim == 2) {
+      src = (i / H) * M + (i % H);
+    } else {
+      // khat[(ky H + kx) H + kz] from the full [kz][ky][kx] spectrum
+      const long long kz = i % H, kx = (i / H) % H, ky = i / ((long long)H * H);
+      src = (kz * M + ky) * M + kx;
+    }
     khat[i] = kf[src].x;
   }
 }
@@ -1092,25 +1103,37 @@ int elem_blocks(long long n) {
   return (int)g;
 }
 
-LineCfg line_cfg(int L) {
-  int need = (L + 7) / 8;
+// Threads per line: L/8 for register-resident transforms, else the power of
+// two covering L with at most 8 elements per thread.
+LineCfg line_cfg(const FftPlan& fp) {
+  const int L = fp.L;
   int tpl = 1;
-  while (tpl < need) tpl <<= 1;
-  if (tpl > BEC_WG) tpl = BEC_WG;
+  if (fp.reg) {
+    tpl = L / 8;
+  } else {
+    const int need = (L + 7) / 8;
+    while (tpl < need) tpl <<= 1;
+    if (tpl > BEC_WG) tpl = BEC_WG;
+  }
   LineCfg c;
   c.tpl = tpl;
   c.lpw = BEC_WG / tpl;
   return c;
 }
 
-// Lines per workgroup for a launch: at least one full wavefront, at most
+// Lines per workgroup for a launch: whole wavefronts (at least one), at most
 // BEC_WG threads, a power of two dividing the lines of one instance when
 // possible (so that a reducing workgroup never spans two instances), aiming
 // at ~1024 workgroups so that small problems still spread over all 256 CUs.
 // A strided pass takes at least 4 adjacent columns per workgroup, so that
 // every row segment it touches is 64 contiguous bytes.
+int lpw_min(int tpl) {
+  int l = 1;
+  while ((l * tpl) % 64 != 0 && l * tpl < BEC_WG) l *= 2;
+  return l;
+}
 static int pick_lpw(int tpl, int nl, int total, bool strided) {
-  const int lmin = (tpl >= 64) ? 1 : 64 / tpl;
+  const int lmin = lpw_min(tpl);
   const int lmax = BEC_WG / tpl > lmin ? BEC_WG / tpl : lmin;
   const int want = total / 1024;
   int l = lmin;
@@ -1169,7 +1192,7 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   const int total = a.nb * a.nl;
   a.lpw = pick_lpw(a.tpl, a.nl, total, !CONTIG);
   const int blocks = xcd_grid(total, a.lpw);
-  const size_t shm = (size_t)(a.lpw + 1) * a.fp.L * sizeof(double2);
+  const size_t shm = ((size_t)a.lpw * lpitch<CONTIG>(a.fp.L) + a.fp.L) * sizeof(double2);
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL + (ST == ST_CDOT ? 5 : 0);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const double bpi = pass_bytes<LD, ST>(a);
@@ -1215,7 +1238,7 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   const int total = a.nb * a.nl;
   a.lpw = pick_lpw(a.tpl, a.nl, total, !Ax<AX>::contig);
   const int blocks = xcd_grid(total, a.lpw);
-  const size_t shm = (size_t)(a.lpw + 1) * a.N * sizeof(double2);
+  const size_t shm = ((size_t)a.lpw * lpitch<Ax<AX>::contig>(a.N) + a.N) * sizeof(double2);
   const int id = 100 + 10 * AX + KIND;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const double bpi = kin_bytes<KIND>(a);

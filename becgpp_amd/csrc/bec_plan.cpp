@@ -33,24 +33,48 @@ static void bad(const std::string& m) { throw BecError(1, m); }
 // ---------------------------------------------------------------------------
 // FFT plans
 // ---------------------------------------------------------------------------
-static bool factor(int L, std::vector<int>& rad) {
+// Stage radices of L = 2^a 3^b 5^c.  When 8 divides L the order starts and
+// ends with a power-of-two radix (8, 4 or 2): with tpl = L/8 threads per line
+// those two stages then run from and to registers (fft_reg).  reg reports it;
+// it also needs whole wavefronts: tpl times some power of two <= BEC_WG is a
+// multiple of 64 (L = 2^a or 3 2^a).
+static bool factor(int L, std::vector<int>& rad, bool& reg) {
   rad.clear();
-  int n = L;
-  while (n % 8 == 0) { rad.push_back(8); n /= 8; }
-  while (n % 3 == 0) { rad.push_back(3); n /= 3; }
-  while (n % 4 == 0) { rad.push_back(4); n /= 4; }
-  while (n % 2 == 0) { rad.push_back(2); n /= 2; }
-  while (n % 5 == 0) { rad.push_back(5); n /= 5; }
-  return n == 1 && (int)rad.size() <= BEC_MAX_STAGES;
+  int n = L, a = 0;
+  while (n % 2 == 0) { ++a; n /= 2; }
+  std::vector<int> p2, odd;
+  for (int i = 0; i < a / 3; ++i) p2.push_back(8);
+  if (a % 3 == 1) p2.push_back(2);
+  if (a % 3 == 2) p2.push_back(4);
+  while (n % 3 == 0) { odd.push_back(3); n /= 3; }
+  while (n % 5 == 0) { odd.push_back(5); n /= 5; }
+  if (n != 1) return false;
+  reg = false;
+  if (a >= 3)
+    for (int l = 1; l * (L / 8) <= BEC_WG; l *= 2)
+      if ((l * (L / 8)) % 64 == 0) reg = true;
+  if (L == 8) reg = true;              // one thread per line, one stage
+  if (a >= 3 && p2.size() == 1 && !odd.empty()) { p2 = {4, 2}; }
+  if (a >= 3) {
+    rad.push_back(p2[0]);
+    for (int r : odd) rad.push_back(r);
+    for (size_t i = 1; i < p2.size(); ++i) rad.push_back(p2[i]);
+  } else {
+    for (int r : p2) rad.push_back(r);
+    for (int r : odd) rad.push_back(r);
+  }
+  return (int)rad.size() <= BEC_MAX_STAGES;
 }
 
 static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
   std::vector<int> rad;
-  if (L < 2 || L > BEC_MAX_L || !factor(L, rad))
+  bool reg = false;
+  if (L < 2 || L > BEC_MAX_L || !factor(L, rad, reg))
     bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b 5^c <= " +
         std::to_string(BEC_MAX_L));
   fp.L = L;
   fp.nst = (int)rad.size();
+  fp.reg = reg ? 1 : 0;
   int ns = 1;
   for (int i = 0; i < BEC_MAX_STAGES; ++i) {
     fp.rad[i] = i < fp.nst ? rad[i] : 1;
@@ -528,8 +552,13 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     p.stream = 0;
     make_fft(N, p.fpN, &p.twN, p.stream);
     make_fft(p.M, p.fpM, &p.twM, p.stream);
-    p.lcN = line_cfg(N);
-    p.lcM = line_cfg(p.M);
+    // batched reducing passes need one instance's lines (n / N of them) to fill
+    // whole workgroups; keep the power-of-two thread count where L/8 would not
+    const int nlk = (int)(p.n / N);
+    if (p.fpN.reg && nlk % lpw_min(N / 8) != 0) p.fpN.reg = 0;
+    if (p.fpM.reg && nlk % lpw_min(p.M / 8) != 0) p.fpM.reg = 0;
+    p.lcN = line_cfg(p.fpN);
+    p.lcM = line_cfg(p.fpM);
     plan_reserve(p, 1);
     *out = h;
     return 0;

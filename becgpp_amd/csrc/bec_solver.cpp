@@ -109,8 +109,6 @@ struct Inst {
 
 static Batch mask_batch(int nb, unsigned long long m) { return Batch{nb, m}; }
 
-static int lmin_of(int tpl) { return tpl >= 64 ? 1 : 64 / tpl; }
-
 // Prints the progress records the controllers wrote since the last call.
 static void vlog_flush(const Plan& p, int nb, std::vector<int>& printed) {
   const double* cnt = p.hvlog + (size_t)BEC_BMAX * VLOG_CAP * VLOG_REC;
@@ -352,7 +350,7 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   if (nb > 1) {
     // reducing line passes need whole workgroups per instance
     const int nlk = (int)(p.n / p.N);
-    if (nlk % lmin_of(p.lcN.tpl) != 0 || nlk % lmin_of(p.lcM.tpl) != 0)
+    if (nlk % lpw_min(p.lcN.tpl) != 0 || nlk % lpw_min(p.lcM.tpl) != 0)
       throw BecError(1, "batched solves need a grid with N >= 32 (2D) or N >= 16 (3D)");
   }
   std::vector<Phys> ph(nb);
