@@ -18,7 +18,7 @@
 struct FftPlan {
   int L;                      // transform length (2^a 3^b 5^c)
   int nst;                    // number of Stockham stages
-  int rad[BEC_MAX_STAGES];    // radix per stage (8, 4, 3, 2 or 5)
+  int rad[BEC_MAX_STAGES];    // radix per stage (8, 6, 5, 4, 3 or 2)
   int ns[BEC_MAX_STAGES];     // Ns of the stage: product of the earlier radices
   int step[BEC_MAX_STAGES];   // twiddle stride L / (Ns R)
   unsigned mag[BEC_MAX_STAGES];   // floor(2^32 / Ns) + 1: j / Ns = umulhi(j, mag) for j < 2^12
@@ -100,11 +100,27 @@ template <int SIGN> __device__ __forceinline__ void bfly5(double2* v) {
   v[2] = cadd(r2, u2);
   v[3] = csub(r2, u2);
 }
+// radix 6 as two radix-3 DFTs of the even and odd inputs, A and B:
+// X_k = A_{k mod 3} + w^k B_{k mod 3}, w = exp(SIGN i pi/3), w^3 = -1
+template <int SIGN> __device__ __forceinline__ void bfly6(double2* v) {
+  const double h = SIGN * 0.86602540378443864676;    // sqrt(3)/2
+  double2 e[3] = {v[0], v[2], v[4]};
+  double2 o[3] = {v[1], v[3], v[5]};
+  bfly3<SIGN>(e);
+  bfly3<SIGN>(o);
+  // w o1 = (1/2 + i h) o1,  w^2 o2 = (-1/2 + i h) o2
+  const double2 o1 = make_double2(0.5 * o[1].x - h * o[1].y, 0.5 * o[1].y + h * o[1].x);
+  const double2 o2 = make_double2(-0.5 * o[2].x - h * o[2].y, -0.5 * o[2].y + h * o[2].x);
+  v[0] = cadd(e[0], o[0]); v[3] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);   v[4] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);   v[5] = csub(e[2], o2);
+}
 template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
   if constexpr (R == 2) bfly2<SIGN>(v);
   else if constexpr (R == 3) bfly3<SIGN>(v);
   else if constexpr (R == 4) bfly4<SIGN>(v);
   else if constexpr (R == 5) bfly5<SIGN>(v);
+  else if constexpr (R == 6) bfly6<SIGN>(v);
   else bfly8<SIGN>(v);
 }
 
@@ -166,6 +182,7 @@ __device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const doub
     else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 6) fft_stage<6, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
   }
 }
@@ -239,6 +256,7 @@ __device__ __forceinline__ void fft_reg(double2 (&v)[8], double2* s, const FftPl
     else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 6) fft_stage<6, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
   }
   if (rl == 8) reg_last<8, SIGN>(v, s, p.L, lane, tpl, twl, active);

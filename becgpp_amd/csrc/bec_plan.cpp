@@ -55,6 +55,11 @@ static bool factor(int L, std::vector<int>& rad, bool& reg) {
       if ((l * (L / 8)) % 64 == 0) reg = true;
   if (L == 8) reg = true;              // one thread per line, one stage
   if (a >= 3 && p2.size() == 1 && !odd.empty()) { p2 = {4, 2}; }
+  // a spare radix 2 between first and last pairs with a 3: one stage fewer
+  if (a >= 3 && p2.size() >= 3 && p2.back() == 2 && !odd.empty() && odd[0] == 3) {
+    p2.pop_back();
+    odd[0] = 6;
+  }
   if (a >= 3) {
     rad.push_back(p2[0]);
     for (int r : odd) rad.push_back(r);

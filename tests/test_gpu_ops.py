@@ -51,6 +51,21 @@ def test_apply_H(kw):
     assert relerr(got, ref) < TOL
 
 
+# line lengths whose transforms use every stage kind: N = 384 (radices 8, 6, 8),
+# conv M = 384 from N = 192, and N = 160 (8, 5, 4: LDS-resident transforms)
+@pytest.mark.parametrize("kw", [dict(dimension="2D", Ngrid=384, L=12, beta2=200, Omega=0.9),
+                                dict(dimension="2D", Ngrid=192, L=10, beta2=30, G_C=3, kernel="newton",
+                                     Omega=0.3),
+                                dict(dimension="2D", Ngrid=160, L=10, beta2=30, G_C=2, kernel="log")])
+def test_apply_H_long_lines(kw):
+    import becgpp_amd as B
+    c, Go, Gd = _pair(kw)
+    psi, _ = random_state(Go, 5)
+    ref = O.apply_H(psi, Go, c)
+    got = B.apply_H(_dev(psi), Gd, c).cpu().numpy()
+    assert relerr(got, ref) < TOL
+
+
 @pytest.mark.parametrize("kw", PHYSICS)
 def test_energy_components(kw):
     import becgpp_amd as B
