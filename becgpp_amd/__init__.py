@@ -12,7 +12,7 @@ from .interactions import long_range_phi, kernel_convolution, q2d_kernel, clear_
 from .operators import energy_components, energy_t, apply_H, compute_residual, observables, sensitivities
 from .thomasfermi import extract_tf, mass_quantiles
 from .seeds import make_seed
-from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, solve_native,
+from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, ground_state_uneven, solve_native,
                       solve_native_batch)
 
 from .modes import run, run_id, DISPATCH

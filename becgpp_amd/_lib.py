@@ -70,6 +70,7 @@ SIGNATURES = {
     "bec_ground_state_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Phys),
                                               ctypes.POINTER(SolverCfg), ctypes.c_void_p,
                                               ctypes.POINTER(SolveOut)]),
+    "bec_plan_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "bec_prof_watch": (ctypes.c_int, [ctypes.c_int]),
     "bec_prof_read": (ctypes.c_int, [c_double_p, ctypes.POINTER(ctypes.c_longlong), c_double_p]),
 }

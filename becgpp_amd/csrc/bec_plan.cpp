@@ -167,8 +167,8 @@ void plan_reserve(Plan& p, int nb) {
 }
 
 void plan_set_phys(Plan& p, const Phys* ph, int nb) {
+  HIPCHK(hipMemcpyAsync(p.iphys, ph, sizeof(InstPhys) * nb, hipMemcpyHostToDevice, p.stream));
   HIPCHK(hipStreamSynchronize(p.stream));
-  HIPCHK(hipMemcpy(p.iphys, ph, sizeof(InstPhys) * nb, hipMemcpyHostToDevice));
 }
 
 // Fold the pinned per-block rows of every queued reduction into hred, per

@@ -162,9 +162,11 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     di[b].n_tr = n_tr;
     p.hdone[b] = 0;
   }
+  // copies on the caller's stream, never the legacy one: other host threads
+  // may be capturing their own plans' loops meanwhile
+  HIPCHK(hipMemcpyAsync(p.dinst, di.data(), sizeof(DevInst) * nb, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p.flags, fl0.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
-  HIPCHK(hipMemcpy(p.dinst, di.data(), sizeof(DevInst) * nb, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(p.flags, fl0.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice));
 
   CtrlArgs c;
   memset(&c, 0, sizeof(c));
@@ -321,7 +323,8 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   if (cf->verbose) vlog_flush(p, nb, printed);
-  HIPCHK(hipMemcpy(di.data(), p.dinst, sizeof(DevInst) * nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(di.data(), p.dinst, sizeof(DevInst) * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
   for (int b = 0; b < nb; ++b) {
     Inst& q = I[b];
     q.E = di[b].E;
@@ -336,7 +339,8 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   }
   if (trace && di[0].n_tr > n_tr) {
     std::vector<double> rows(4 * (size_t)di[0].n_tr);
-    HIPCHK(hipMemcpy(rows.data(), p.dtrace, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(rows.data(), p.dtrace, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     for (int r = n_tr; r < di[0].n_tr; ++r)
       for (int k = 0; k < 4; ++k) trace[4 * r + k] = rows[4 * r + k];
     n_tr = di[0].n_tr;
@@ -704,6 +708,24 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
   try {
     if (!h || !php || !cf || !psi_io || !out) throw BecError(1, "null argument");
     return solve_batch(h->p, 1, php, cf, (double2*)psi_io, out, trace, trace_cap);
+  } catch (const BecError& e) {
+    bec_set_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    bec_set_error(e.what());
+    return 2;
+  }
+}
+
+extern "C" int bec_plan_reserve(bec_plan* h, int nb) {
+  try {
+    if (!h) throw BecError(1, "null plan");
+    Plan& p = h->p;
+    plan_reserve(p, nb);
+    solver_alloc(p, nb);
+    dev_alloc(p, nb, 0);
+    HIPCHK(hipStreamSynchronize(p.stream));
+    return 0;
   } catch (const BecError& e) {
     bec_set_error(e.what());
     return e.code;

@@ -117,6 +117,74 @@ def ground_state_batch(cfgs, G=None, psi0s=None, verbose=False):
     return out
 
 
+def _grid_key(G):
+    return (int(G["ndim"]), int(G["N"]), round(float(G["L"]), 14), int(G["pad"]), round(float(G["s"]), 14),
+            round(float(G.get("trap_coeff", 0.5)), 14), str(G["kernel"]), round(float(G.get("l_z", 0.0)), 14))
+
+
+def ground_state_uneven(cfgs, psi0s=None, grids=None, verbose=False):
+    """ground_state for configurations on different grids (uneven sizes).
+
+    Instances sharing a grid geometry form one batched device loop
+    (ground_state_batch); the loops of different grids run at the same time,
+    each from its own host thread on its own plan's stream, so small grids
+    fill the GPU together.  grids[i] (optional) is the grid of cfgs[i].
+    Returns, in input order, what ground_state(cfgs[i], grids[i], psi0s[i])
+    returns."""
+    import threading
+    n = len(cfgs)
+    if grids is None:
+        grids = [make_grid(c) for c in cfgs]
+    if psi0s is None:
+        psi0s = [make_seed(G, c) for G, c in zip(grids, cfgs)]
+    sets = {}
+    for i, G in enumerate(grids):
+        sets.setdefault(_grid_key(G), []).append(i)
+    groups = list(sets.values())
+    out = [None] * n
+    if len(groups) == 1:
+        ks = groups[0]
+        for k, r in zip(ks, ground_state_batch([cfgs[k] for k in ks], G=grids[ks[0]],
+                                               psi0s=[psi0s[k] for k in ks], verbose=verbose)):
+            out[k] = r
+        return out
+    # Plans, kernel spectra (K and, for observables, W) and batch workspaces
+    # are made here: the workers then allocate nothing in the library, and each
+    # runs on its own stream, as graph capture in one thread requires of the
+    # others.
+    for ks in groups:
+        G = grids[ks[0]]
+        pl = plan_of(G)
+        if G["kernel"] != "none":
+            pl = ensure_kernel(G, "K")
+            if G["kernel"] == "q2d":
+                ensure_kernel(G, "W")
+        _lib.check(pl.lib.bec_plan_reserve(pl.h, min(len(ks), BATCH_MAX)))
+    torch.cuda.synchronize()
+    errors = []
+
+    def work(ks):
+        try:
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                G = grids[ks[0]]
+                p0 = [psi0s[k].to(device=st.device) if torch.is_tensor(psi0s[k]) else psi0s[k] for k in ks]
+                for k, r in zip(ks, ground_state_batch([cfgs[k] for k in ks], G=G, psi0s=p0, verbose=verbose)):
+                    out[k] = r
+                st.synchronize()
+        except BaseException as e:          # re-raised in the caller's thread
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(ks,)) for ks in groups]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    if errors:
+        raise errors[0]
+    return out
+
+
 def _seed_cfgs(cfg, n):
     """Per-seed configurations of ground_state_multiseed (solvers.py:244-251)."""
     rotating = abs(float(cfg.get("Omega", 0.0))) >= 0.3

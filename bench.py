@@ -64,7 +64,8 @@ WORKLOADS = {
         points=[dict(Omega=om, G_C=gc) for om in (0.0, 0.5) for gc in (1.0, 2.0, 5.0, 10.0)],
         instance_grid=lambda i: dict(Ngrid=(128, 192, 256)[i % 3]),
         desc="quasi-2D q2d kernel l_z=0.5, beta2=50, L=16, Omega in {0,0.5} x G_C in {1,2,5,10}, nseeds=4 "
-             "(32 solves), N = 128/192/256 round-robin by instance id, one batch per grid, res_tol=1e-6, "
+             "(32 solves), N = 128/192/256 round-robin by instance id, one batch per grid (the three run "
+             "concurrently), res_tol=1e-6, "
              "linesearch_max=20"),
     # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
     "droplet_sweep_2d_n256": dict(
@@ -192,13 +193,9 @@ def main():
         return best
 
     def solve_all():
-        """Every instance through the batched solver, one batch set per grid."""
-        res = [None] * nsol
-        for gk, ks in groups.items():
-            out = B.ground_state_batch([cs[k] for k in ks], G=grids[gk], psi0s=[psi0s[k] for k in ks])
-            for k, r in zip(ks, out):
-                res[k] = r
-        return res
+        """Every instance through the batched solver: one batch set per grid,
+        the sets of different grids running concurrently."""
+        return B.ground_state_uneven(cs, psi0s=psi0s, grids=[grids[r[3]] for r in runs])
 
     def step():
         res = solve_all()

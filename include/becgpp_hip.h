@@ -156,6 +156,14 @@ int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg,
 int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
                            bec_solve_out* out);
 
+/* Sizes the plan's workspace, solver state and device-loop buffers for nb
+ * instances ahead of bec_ground_state_batch (which otherwise sizes them on
+ * first use).  No counterpart in the reference, which allocates per call:
+ * after it, a batched solve on this plan allocates nothing, so the solves of
+ * several plans (grid sizes) may run from separate host threads at once,
+ * one thread per plan, each on its own non-default stream. */
+int bec_plan_reserve(bec_plan* p, int nb);
+
 /* Measurement hooks (no counterpart in the reference, which has no profiler
  * hooks): time every launch of one kernel with HIP events on its stream;
  * bytes sums the algorithmic bytes of the timed launches (each array element

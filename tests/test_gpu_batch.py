@@ -141,3 +141,31 @@ def test_batch_argument_errors():
     assert not B.solvers.batchable(Gs)
     with pytest.raises(ValueError):
         B.solve_native_batch([small] * 2, Gs, [torch.ones((16, 16), dtype=torch.complex128, device="cuda")] * 2)
+
+
+def test_uneven_grids_run_concurrently_and_match():
+    """ground_state_uneven: instances on three grid sizes (one batch set per
+    grid, the sets running concurrently from separate host threads) give
+    exactly the per-grid batches' results, and each matches the oracle."""
+    import becgpp_amd as B
+    base = cfg(dimension="quasi2D", kernel="q2d", l_z=0.5, beta2=30.0, res_tol=1e-8, L=6.0)
+    sizes = (32, 48, 64)
+    cs, p0s = [], []
+    for i in range(6):
+        c = dict(base, Ngrid=sizes[i % 3], G_C=(1.0, 3.0)[i // 3], Omega=(0.0, 0.3)[i // 3])
+        cs.append(c)
+        p0s.append(O.noisy_gaussian(O.make_grid(c), 1e-3, seed=40 + i))
+    grids = [B.make_grid(c) for c in cs]
+    got = B.ground_state_uneven(cs, psi0s=[torch.as_tensor(p).cuda() for p in p0s], grids=grids)
+    for N in sizes:
+        ks = [k for k in range(6) if cs[k]["Ngrid"] == N]
+        ref = B.ground_state_batch([cs[k] for k in ks], G=grids[ks[0]],
+                                   psi0s=[torch.as_tensor(p0s[k]).cuda() for k in ks])
+        for k, (pr, _, orr) in zip(ks, ref):
+            pg, _, og = got[k]
+            assert og["iters"] == orr["iters"] and og["E"] == orr["E"], (N, k)
+            assert torch.equal(pg, pr)
+    for k in (0, 4):
+        po, _, oo = O.ground_state(cs[k], O.make_grid(cs[k]), p0s[k])
+        for key in ("E", "mu", "Lz"):
+            assert _close(got[k][2][key], oo[key], 1e-8), (k, key)
