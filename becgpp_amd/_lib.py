@@ -29,7 +29,7 @@ class SolverCfg(ctypes.Structure):
                 ("linesearch_max", ctypes.c_int), ("linesearch_stall", ctypes.c_int),
                 ("precond_shift_min", ctypes.c_double), ("cg_beta", ctypes.c_int),
                 ("cg_restart", ctypes.c_int), ("verbose", ctypes.c_int), ("check", ctypes.c_int),
-                ("host_loop", ctypes.c_int)]
+                ("host_loop", ctypes.c_int), ("warm_res_tol", ctypes.c_double)]
 
 
 class SolveOut(ctypes.Structure):
@@ -37,7 +37,8 @@ class SolveOut(ctypes.Structure):
                 ("stop_reason", ctypes.c_int), ("n_trace", ctypes.c_int),
                 ("E", ctypes.c_double), ("resid_rel", ctypes.c_double),
                 ("best_resid", ctypes.c_double), ("sigma", ctypes.c_double),
-                ("walltime", ctypes.c_double), ("n_energy", ctypes.c_longlong)]
+                ("walltime", ctypes.c_double), ("n_energy", ctypes.c_longlong),
+                ("warm_iters", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); every entry point returns an int status

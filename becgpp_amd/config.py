@@ -33,6 +33,15 @@ CFG_DEFAULTS = dict(
     tag="becgpp", show_inline=True, save_figs=True, save_ckpt=True, zip_output=False,
 )
 
+# Keys of this build with no counterpart in the reference: accepted by
+# default_cfg when given, absent otherwise (so a reference configuration, and
+# its run id, are unchanged).
+#   warm_fp32:    run the loop on complex64 fields to warm_res_tol first, then
+#                 polish in FP64 (instances with long range stay FP64)
+#   host_loop:    keep the loop's scalar logic on the host (one synchronisation
+#                 per iteration) instead of the device controllers
+CFG_EXTENSIONS = dict(warm_fp32=False, warm_res_tol=1e-4, host_loop=False)
+
 _ALIASES = {"res": "res_tol", "etol": "energy_tol", "N": "Ngrid", "dim": "dimension",
             "precond_shift": "precond_shift_min"}
 
@@ -54,7 +63,7 @@ def default_cfg(**overrides):
             if key in given:
                 raise ValueError(f"pass only one of {alias!r} or {key!r}")
             given[key] = given.pop(alias)
-    unknown = sorted(set(given) - set(CFG_DEFAULTS))
+    unknown = sorted(set(given) - set(CFG_DEFAULTS) - set(CFG_EXTENSIONS))
     if unknown:
         raise ValueError(f"unknown config key(s): {unknown}; "
                          f"see becgpp_amd.config.CFG_DEFAULTS for valid keys")

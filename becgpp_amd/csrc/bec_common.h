@@ -26,75 +26,101 @@ struct FftPlan {
   const double2* tw;          // tw[m] = exp(-2 pi i m / L), m in [0, L)
 };
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex arithmetic on interleaved pairs: double2 (complex128 fields) and
+// float2 (the complex64 fields of an FP32 phase).  CT<C>::F is the real type.
+template <class C> struct CT;
+template <> struct CT<double2> { using F = double; };
+template <> struct CT<float2> { using F = float; };
+template <class C> __device__ __forceinline__ C cmk(typename CT<C>::F x, typename CT<C>::F y) {
+  C c;
+  c.x = x;
+  c.y = y;
+  return c;
 }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+template <class C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
+template <class C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C cmul(C a, C b) {
+  return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class C> __device__ __forceinline__ C cscale(C a, double s) {
+  using F = typename CT<C>::F;
+  return cmk<C>(a.x * (F)s, a.y * (F)s);
+}
 // multiply by +i (S=+1) or -i (S=-1)
-template <int S> __device__ __forceinline__ double2 cmuli(double2 a) {
-  return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+template <int S, class C> __device__ __forceinline__ C cmuli(C a) {
+  return S > 0 ? cmk<C>(-a.y, a.x) : cmk<C>(a.y, -a.x);
 }
-__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+// |a|^2 in FP64 whatever the storage
+template <class C> __device__ __forceinline__ double cabs2(C a) {
+  return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+// widen / narrow one element
+__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <class C> __device__ __forceinline__ C from_d2(double2 a) {
+  return cmk<C>((typename CT<C>::F)a.x, (typename CT<C>::F)a.y);
+}
 
 // ----------------------------------------------------------------------------
 // radix-R DFT butterflies, X_p = sum_q v_q exp(SIGN 2 pi i p q / R)
 // ----------------------------------------------------------------------------
-template <int SIGN> __device__ __forceinline__ void bfly2(double2* v) {
-  double2 a = v[0], b = v[1];
+template <int SIGN, class C> __device__ __forceinline__ void bfly2(C* v) {
+  C a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
-template <int SIGN> __device__ __forceinline__ void bfly3(double2* v) {
+template <int SIGN, class C> __device__ __forceinline__ void bfly3(C* v) {
+  using F = typename CT<C>::F;
   // w = exp(SIGN 2 pi i / 3) = -1/2 + i SIGN sqrt(3)/2
-  const double s = SIGN * 0.86602540378443864676;   // sqrt(3)/2
-  double2 a = v[0], b = v[1], c = v[2];
-  double2 t1 = cadd(b, c);
-  double2 t2 = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
-  double2 d = csub(b, c);
-  double2 t3 = make_double2(-s * d.y, s * d.x);       // i s (b - c)
+  const F s = SIGN * 0.86602540378443864676;   // sqrt(3)/2
+  C a = v[0], b = v[1], c = v[2];
+  C t1 = cadd(b, c);
+  C t2 = cmk<C>(a.x - F(0.5) * t1.x, a.y - F(0.5) * t1.y);
+  C d = csub(b, c);
+  C t3 = cmk<C>(-s * d.y, s * d.x);       // i s (b - c)
   v[0] = cadd(a, t1);
   v[1] = cadd(t2, t3);
   v[2] = csub(t2, t3);
 }
-template <int SIGN> __device__ __forceinline__ void bfly4(double2* v) {
-  double2 a = v[0], b = v[1], c = v[2], d = v[3];
-  double2 t0 = cadd(a, c), t1 = csub(a, c);
-  double2 t2 = cadd(b, d), t3 = cmuli<SIGN>(csub(b, d));
+template <int SIGN, class C> __device__ __forceinline__ void bfly4(C* v) {
+  C a = v[0], b = v[1], c = v[2], d = v[3];
+  C t0 = cadd(a, c), t1 = csub(a, c);
+  C t2 = cadd(b, d), t3 = cmuli<SIGN>(csub(b, d));
   v[0] = cadd(t0, t2);
   v[2] = csub(t0, t2);
   v[1] = cadd(t1, t3);
   v[3] = csub(t1, t3);
 }
-template <int SIGN> __device__ __forceinline__ void bfly8(double2* v) {
-  const double h = 0.70710678118654752440;           // sqrt(2)/2
-  double2 e[4] = {v[0], v[2], v[4], v[6]};
-  double2 o[4] = {v[1], v[3], v[5], v[7]};
+template <int SIGN, class C> __device__ __forceinline__ void bfly8(C* v) {
+  using F = typename CT<C>::F;
+  const F h = 0.70710678118654752440;           // sqrt(2)/2
+  C e[4] = {v[0], v[2], v[4], v[6]};
+  C o[4] = {v[1], v[3], v[5], v[7]};
   bfly4<SIGN>(e);
   bfly4<SIGN>(o);
   // twiddles w8^p, w8 = exp(SIGN i pi/4)
-  double2 o1 = make_double2(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
-  double2 o2 = cmuli<SIGN>(o[2]);
-  double2 o3 = make_double2(h * (-o[3].x - SIGN * o[3].y), h * (-o[3].y + SIGN * o[3].x));
+  C o1 = cmk<C>(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
+  C o2 = cmuli<SIGN>(o[2]);
+  C o3 = cmk<C>(h * (-o[3].x - SIGN * o[3].y), h * (-o[3].y + SIGN * o[3].x));
   v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
   v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
   v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
 }
-template <int SIGN> __device__ __forceinline__ void bfly5(double2* v) {
+template <int SIGN, class C> __device__ __forceinline__ void bfly5(C* v) {
+  using F = typename CT<C>::F;
   // w = exp(SIGN 2 pi i / 5): cos and sin of 2 pi / 5 and 4 pi / 5
-  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-  const double s1 = SIGN * 0.95105651629515357212, s2 = SIGN * 0.58778525229247312917;
-  const double2 a = v[0];
-  const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  const double2 r1 = make_double2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
-  const double2 r2 = make_double2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+  const F c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const F s1 = SIGN * 0.95105651629515357212, s2 = SIGN * 0.58778525229247312917;
+  const C a = v[0];
+  const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const C t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const C r1 = cmk<C>(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+  const C r2 = cmk<C>(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
   // i (s1 t3 + s2 t4) and i (s2 t3 - s1 t4)
-  const double2 u1 = make_double2(-(s1 * t3.y + s2 * t4.y), s1 * t3.x + s2 * t4.x);
-  const double2 u2 = make_double2(-(s2 * t3.y - s1 * t4.y), s2 * t3.x - s1 * t4.x);
-  v[0] = make_double2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+  const C u1 = cmk<C>(-(s1 * t3.y + s2 * t4.y), s1 * t3.x + s2 * t4.x);
+  const C u2 = cmk<C>(-(s2 * t3.y - s1 * t4.y), s2 * t3.x - s1 * t4.x);
+  v[0] = cmk<C>(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
   v[1] = cadd(r1, u1);
   v[4] = csub(r1, u1);
   v[2] = cadd(r2, u2);
@@ -102,20 +128,21 @@ template <int SIGN> __device__ __forceinline__ void bfly5(double2* v) {
 }
 // radix 6 as two radix-3 DFTs of the even and odd inputs, A and B:
 // X_k = A_{k mod 3} + w^k B_{k mod 3}, w = exp(SIGN i pi/3), w^3 = -1
-template <int SIGN> __device__ __forceinline__ void bfly6(double2* v) {
-  const double h = SIGN * 0.86602540378443864676;    // sqrt(3)/2
-  double2 e[3] = {v[0], v[2], v[4]};
-  double2 o[3] = {v[1], v[3], v[5]};
+template <int SIGN, class C> __device__ __forceinline__ void bfly6(C* v) {
+  using F = typename CT<C>::F;
+  const F h = SIGN * 0.86602540378443864676;    // sqrt(3)/2
+  C e[3] = {v[0], v[2], v[4]};
+  C o[3] = {v[1], v[3], v[5]};
   bfly3<SIGN>(e);
   bfly3<SIGN>(o);
   // w o1 = (1/2 + i h) o1,  w^2 o2 = (-1/2 + i h) o2
-  const double2 o1 = make_double2(0.5 * o[1].x - h * o[1].y, 0.5 * o[1].y + h * o[1].x);
-  const double2 o2 = make_double2(-0.5 * o[2].x - h * o[2].y, -0.5 * o[2].y + h * o[2].x);
+  const C o1 = cmk<C>(F(0.5) * o[1].x - h * o[1].y, F(0.5) * o[1].y + h * o[1].x);
+  const C o2 = cmk<C>(-F(0.5) * o[2].x - h * o[2].y, -F(0.5) * o[2].y + h * o[2].x);
   v[0] = cadd(e[0], o[0]); v[3] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);   v[4] = csub(e[1], o1);
   v[2] = cadd(e[2], o2);   v[5] = csub(e[2], o2);
 }
-template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
+template <int R, int SIGN, class C> __device__ __forceinline__ void bfly(C* v) {
   if constexpr (R == 2) bfly2<SIGN>(v);
   else if constexpr (R == 3) bfly3<SIGN>(v);
   else if constexpr (R == 4) bfly4<SIGN>(v);
@@ -129,12 +156,12 @@ template <int R, int SIGN> __device__ __forceinline__ void bfly(double2* v) {
 // guaranteed by the host, so a thread owns at most ceil(8/R) butterflies.
 // tw is the twiddle table staged in LDS by the calling kernel.
 // Every thread of the workgroup must call it (it contains barriers).
-template <int R, int SIGN>
-__device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, unsigned mag, int step, int lane, int tpl,
-                                          const double2* __restrict__ tw, bool active) {
+template <int R, int SIGN, class C>
+__device__ __forceinline__ void fft_stage(C* s, int L, int Ns, unsigned mag, int step, int lane, int tpl,
+                                          const C* __restrict__ tw, bool active) {
   constexpr int MAXB = (8 + R - 1) / R;
   const int nb = L / R;
-  double2 v[MAXB][R];
+  C v[MAXB][R];
   if (active) {
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
@@ -155,7 +182,7 @@ __device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, unsigned ma
         if (Ns > 1) {
 #pragma unroll
           for (int r = 1; r < R; ++r) {
-            double2 w = tw[r * k * step];
+            C w = tw[r * k * step];
             if (SIGN > 0) w.y = -w.y;
             v[b][r] = cmul(v[b][r], w);
           }
@@ -172,8 +199,8 @@ __device__ __forceinline__ void fft_stage(double2* s, int L, int Ns, unsigned ma
 
 // In-place unnormalised DFT of the LDS line s (SIGN=-1 forward, +1 inverse);
 // twl = the plan's twiddle table copied to LDS (stage_twiddles).
-template <int SIGN>
-__device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const double2* twl, int lane, int tpl,
+template <int SIGN, class C>
+__device__ __forceinline__ void fft_lds(C* s, const FftPlan& p, const C* twl, int lane, int tpl,
                                         bool active) {
   for (int st = 0; st < p.nst; ++st) {
     const int R = p.rad[st], Ns = p.ns[st], step = p.step[st];
@@ -196,14 +223,14 @@ __device__ __forceinline__ void fft_lds(double2* s, const FftPlan& p, const doub
 // go through the LDS line, so a forward transform, a pointwise multiply and an
 // inverse transform keep the spectrum in registers across the multiply.
 // ----------------------------------------------------------------------------
-template <int R, int SIGN>
-__device__ __forceinline__ void reg_first(double2 (&v)[8], double2* s, int lane, int tpl, bool active) {
+template <int R, int SIGN, class C>
+__device__ __forceinline__ void reg_first(C (&v)[8], C* s, int lane, int tpl, bool active) {
   constexpr int Q = 8 / R;
   __syncthreads();                     // earlier readers of s are done
   if (active) {
 #pragma unroll
     for (int b = 0; b < Q; ++b) {
-      double2 u[R];
+      C u[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) u[r] = v[b + r * Q];
       bfly<R, SIGN>(u);
@@ -214,21 +241,21 @@ __device__ __forceinline__ void reg_first(double2 (&v)[8], double2* s, int lane,
   }
   __syncthreads();
 }
-template <int R, int SIGN>
-__device__ __forceinline__ void reg_last(double2 (&v)[8], const double2* s, int L, int lane, int tpl,
-                                         const double2* __restrict__ tw, bool active) {
+template <int R, int SIGN, class C>
+__device__ __forceinline__ void reg_last(C (&v)[8], const C* s, int L, int lane, int tpl,
+                                         const C* __restrict__ tw, bool active) {
   constexpr int Q = 8 / R;
   const int ns = L / R;                // twiddle stride L / (Ns R) = 1
   if (!active) return;
 #pragma unroll
   for (int b = 0; b < Q; ++b) {
     const int j = lane + b * tpl;
-    double2 u[R];
+    C u[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = s[j + r * ns];
 #pragma unroll
     for (int r = 1; r < R; ++r) {
-      double2 w = tw[r * j];
+      C w = tw[r * j];
       if (SIGN > 0) w.y = -w.y;
       u[r] = cmul(u[r], w);
     }
@@ -237,8 +264,8 @@ __device__ __forceinline__ void reg_last(double2 (&v)[8], const double2* s, int 
     for (int r = 0; r < R; ++r) v[b + r * Q] = u[r];
   }
 }
-template <int SIGN>
-__device__ __forceinline__ void fft_reg(double2 (&v)[8], double2* s, const FftPlan& p, const double2* twl,
+template <int SIGN, class C>
+__device__ __forceinline__ void fft_reg(C (&v)[8], C* s, const FftPlan& p, const C* twl,
                                         int lane, int tpl, bool active) {
   const int n = p.nst;
   if (n == 1) {                        // L = 8, one thread per line
@@ -266,8 +293,8 @@ __device__ __forceinline__ void fft_reg(double2 (&v)[8], double2* s, const FftPl
 // The DFT of a line whose elements lane + m tpl (m < 8, those below L) are in
 // v: register-resident when the plan allows it, else through the LDS line s
 // (fft_lds).  Every thread of the workgroup calls it; results return in v.
-template <int SIGN>
-__device__ __forceinline__ void fft_line(double2 (&v)[8], double2* s, const FftPlan& p, const double2* twl,
+template <int SIGN, class C>
+__device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, const C* twl,
                                          int lane, int tpl, bool active) {
   if (p.reg) {
     fft_reg<SIGN>(v, s, p, twl, lane, tpl, active);
@@ -287,7 +314,7 @@ __device__ __forceinline__ void fft_line(double2 (&v)[8], double2* s, const FftP
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int e = lane + m * tpl;
-      v[m] = (e < p.L) ? s[e] : make_double2(0.0, 0.0);
+      v[m] = (e < p.L) ? s[e] : cmk<C>(0.0, 0.0);
     }
   }
 }
@@ -295,8 +322,9 @@ __device__ __forceinline__ void fft_line(double2 (&v)[8], double2* s, const FftP
 // Copies the twiddle table (L entries) into LDS; the caller's next barrier
 // publishes it.  Issued together with the line loads, so its latency overlaps
 // theirs instead of sitting inside every FFT stage.
-__device__ __forceinline__ void stage_twiddles(double2* twl, const FftPlan& p) {
-  for (int i = threadIdx.x; i < p.L; i += blockDim.x) twl[i] = p.tw[i];
+template <class C>
+__device__ __forceinline__ void stage_twiddles(C* twl, const FftPlan& p) {
+  for (int i = threadIdx.x; i < p.L; i += blockDim.x) twl[i] = from_d2<C>(p.tw[i]);
 }
 
 // ----------------------------------------------------------------------------

@@ -83,6 +83,7 @@ struct PassArgs {
   unsigned long long* ucount;          // measurement: algorithmic bytes of the instances covered
   unsigned long long ubytes;           //   (ubytes per instance, added once per instance; nullable)
   const double2* dpsi; const double2* dd; const double2* dg; const double2* dy; const double2* ds;  // ST_CDOT
+  int f32;                             // complex fields (in/out, ST_CDOT) are float2: an FP32 phase
 };
 
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
@@ -113,6 +114,7 @@ struct KinArgs {
   const double2* z; const double2* g; double2* d_out;    // KE_FIRST: instances with comb_bit form
   unsigned comb_bit;                                     //   d = az z + ad d + ag g + gam psi first
   int acc_sc;                                            // KH_MID/KH_LAST: acc is divided by sc[b]
+  int f32;                                               // complex fields are float2: an FP32 phase
 };
 
 // per-instance combination d <- az z + ad d + ag g + gam psi
@@ -167,6 +169,8 @@ struct Plan {
   // solver state, sized for scap instances
   int scap;
   double2* st[8]; double* stphi[2];
+  float2* st32[8];                   // the same fields in complex64 (FP32 warm phase)
+  int f32;                           // 1 while an FP32 phase runs: device-loop passes use st32
   // device-driven loop state, sized for scap instances
   void* dinst; unsigned* flags; double* dtab; double* dtrace; int dtrace_cap;
   int* hdone; int* hdone_dev;        // pinned: instance finished (written by the controller)
@@ -255,10 +259,13 @@ int launch_moments(const double2* psi, int ndim, int N, double dx, double* parti
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s);
 // device-driven loop, after the Armijo controller: BEST <- psi for FL_BEST,
 // D <- az Z + gam psi (table coefficients) for FL_FAIL
+// f32: the complex fields hold float2 (FP32 phase; pointers reinterpreted)
 void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
-                 const unsigned* fl, long long n, int nb, hipStream_t s);
+                 const unsigned* fl, long long n, int nb, int f32, hipStream_t s);
+void launch_narrow(const double2* in, float2* out, long long n, hipStream_t s);
+void launch_widen(const float2* in, double2* out, long long n, hipStream_t s);
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
-                  const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s);
+                  const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s, int f32 = 0);
 void launch_kernel_real(int ndim, int M, double dx, int kind, double2* out, hipStream_t s);
 void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat, hipStream_t s);
 void launch_init_sigma(const Plan& p, int nb, hipStream_t s);

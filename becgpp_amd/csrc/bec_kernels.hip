@@ -218,12 +218,19 @@ __device__ __forceinline__ double line_V(const double* c2, int e, const KinArgs&
   return trap_value(r2, a.tc, a.s_mode, a.s_half);
 }
 
-template <int AX, int KIND>
+// C is the storage of the complex fields: double2, or float2 in an FP32 phase
+// (values are widened to FP64 in registers; the transforms run in C; every
+// sum accumulates in FP64).
+template <int AX, int KIND, class C>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
-  extern __shared__ double2 smem[];
-  double2* twl = smem;                 // twiddles [N]
+  extern __shared__ double2 smem_d[];
+  C* smem = (C*)smem_d;
+  C* twl = smem;                       // twiddles [N]
   using X = Ax<AX>;
-  double2* lds = smem + a.N;           // lines [lpw][lpitch<X::contig>(N)]
+  C* lds = smem + a.N;                 // lines [lpw][lpitch<X::contig>(N)]
+  const C* fpsi = (const C*)a.psi;
+  const C* fd = (const C*)a.d;
+  C* fout = (C*)a.out;
   constexpr bool KE = (KIND == KE_FIRST || KIND == KE_REST);
   const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -252,9 +259,9 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const KinScal& OS = (ob == b0) ? S0 : S1;
   double c2[3] = {0.0, 0.0, 0.0};
   if (KIND == KE_FIRST || KIND == KH_LAST) line_sq<AX>(oli, N, a.dx, c2);
-  double2* s = lds + ow.cc * lpitch<X::contig>(N);
+  C* s = lds + ow.cc * lpitch<X::contig>(N);
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
-  double2 x[BEC_EPT];
+  C x[BEC_EPT];
   {
     // phase 1: issue every load of this thread's elements
     double2 p0[BEC_EPT], dd[BEC_EPT];
@@ -263,8 +270,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       const int e = ow.e0 + j * tpl;
       if (oon && e < N) {
         const long long adr = obase + e * strd;
-        p0[j] = a.psi[adr];
-        if (a.d) dd[j] = a.d[adr];
+        p0[j] = to_d2(fpsi[adr]);
+        if (a.d) dd[j] = to_d2(fd[adr]);
       }
     }
     // new search directions (device-driven loop): d = az z + ad d + ag g + gam psi
@@ -277,8 +284,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         const int e = ow.e0 + j * tpl;
         if (e < N) {
           const long long adr = obase + e * strd;
-          if (az != 0.0) zz[j] = a.z[adr];
-          if (ag != 0.0) gg[j] = a.g[adr];
+          if (az != 0.0) zz[j] = to_d2(((const C*)a.z)[adr]);
+          if (ag != 0.0) gg[j] = to_d2(((const C*)a.g)[adr]);
         }
       }
 #pragma unroll
@@ -290,14 +297,14 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         if (ad != 0.0) { r.x += ad * dd[j].x; r.y += ad * dd[j].y; }
         if (ag != 0.0) { r.x += ag * gg[j].x; r.y += ag * gg[j].y; }
         dd[j] = r;
-        a.d_out[obase + e * strd] = r;
+        ((C*)a.d_out)[obase + e * strd] = from_d2<C>(r);
       }
     }
     // phase 2: the loaded values
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
       const int e = ow.e0 + j * tpl;
-      x[j] = make_double2(0.0, 0.0);
+      x[j] = cmk<C>(0, 0);
       if (e >= N) continue;
       double2 v = make_double2(0.0, 0.0);
       if (oon) {
@@ -309,8 +316,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         }
         if (OS.sc != 1.0) { v.x = v.x / OS.sc; v.y = v.y / OS.sc; }
         if (KIND == KH_FIRST || KIND == KH_LAST) {
-          if (a.psi_w) a.psi_w[adr] = v;
-          if (a.sdiff) a.sdiff[adr] = csub(v, p0[j]);
+          if (a.psi_w) ((C*)a.psi_w)[adr] = from_d2<C>(v);
+          if (a.sdiff) ((C*)a.sdiff)[adr] = from_d2<C>(csub(v, p0[j]));
         }
         if (KIND == KE_FIRST) {
           const double rho = cabs2(v);
@@ -322,7 +329,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
           if (a.rho_out) a.rho_out[adr] = rho;
         }
       }
-      x[j] = v;
+      x[j] = from_d2<C>(v);
     }
   }
   fft_line<-1>(x, s, a.fp, twl, lane, tpl, oon);
@@ -371,11 +378,11 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       const int e = ow.e0 + j * tpl;
       if (e >= N) continue;
       const long long adr = obase + e * strd;
-      if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? a.acc[adr] : make_double2(0.0, 0.0);
+      if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? to_d2(((const C*)a.acc)[adr]) : make_double2(0.0, 0.0);
       if (KIND == KH_LAST) {
-        ps[j] = a.psi[adr];
+        ps[j] = to_d2(fpsi[adr]);
         phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
-        if (a.ydiff) old[j] = a.out[adr];
+        if (a.ydiff) old[j] = to_d2(fout[adr]);
       }
     }
     // phase 2: results
@@ -384,15 +391,15 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       const int e = ow.e0 + j * tpl;
       if (e >= N) continue;
       const long long adr = obase + e * strd;
-      double2 u = x[j];
+      double2 u = to_d2(x[j]);
       u.x *= invN;
       u.y *= invN;
       if (KE) {
-        a.hy_out[adr] = u;
+        ((C*)a.hy_out)[adr] = from_d2<C>(u);
         continue;
       }
       if (KIND == KH_FIRST) {
-        a.out[adr] = u;
+        fout[adr] = from_d2<C>(u);
         continue;
       }
       double2 p = acc[j];
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         p.y /= OS.sc;
       }
       if (KIND == KH_MID) {
-        a.out[adr] = cadd(p, u);
+        fout[adr] = from_d2<C>(cadd(p, u));
         continue;
       }
       const double2 psv = ps[j];
@@ -419,8 +426,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       double2 h;
       h.x = p.x + u.x + W * psv.x - mu * psv.x;
       h.y = p.y + u.y + W * psv.y - mu * psv.y;
-      if (a.ydiff) a.ydiff[adr] = csub(h, old[j]);
-      a.out[adr] = h;
+      if (a.ydiff) ((C*)a.ydiff)[adr] = from_d2<C>(csub(h, old[j]));
+      fout[adr] = from_d2<C>(h);
       hsum[0] += cabs2(h);
       hsum[1] += psv.x * h.x + psv.y * h.y;
       hmax[0] = fmax(hmax[0], rho);
@@ -450,12 +457,16 @@ __device__ __forceinline__ void dot_accum(double* v, double2 zi, double2 pi, dou
   v[15] += cabs2(s);
 }
 
-template <bool CONTIG, int LD, int DIR, int MUL, int ST>
+// C: storage (and transform type) of the complex fields, as in k_kin; the
+// real fields (LD_R input, ST_RE output, khat, dot) are FP64.
+template <bool CONTIG, int LD, int DIR, int MUL, int ST, class C>
 __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
-  extern __shared__ double2 smem[];
+  using F = typename CT<C>::F;
+  extern __shared__ double2 smem_d[];
+  C* smem = (C*)smem_d;
   const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
-  double2* twl = smem;                 // twiddles [L]
-  double2* lds = smem + L;             // lines [lpw][lpitch<CONTIG>(L)]
+  C* twl = smem;                       // twiddles [L]
+  C* lds = smem + L;                   // lines [lpw][lpitch<CONTIG>(L)]
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -477,8 +488,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   const bool oon = ol < total && inst_active(ob);
   const int oli = ol - ob * nl;
   const int oo = oli / a.ni, oi = oli - oo * a.ni;
-  double2* s = lds + ow.cc * lpitch<CONTIG>(L);
-  double2 v[BEC_EPT];
+  C* s = lds + ow.cc * lpitch<CONTIG>(L);
+  C v[BEC_EPT];
   // kernel spectrum of the line's elements, loaded with the line itself
   double kh[(MUL == MUL_KHAT2 || MUL == MUL_KHAT3) ? BEC_EPT : 1];
   {
@@ -486,18 +497,18 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
       const int e = lane + j * tpl;
-      v[j] = make_double2(0.0, 0.0);
+      v[j] = cmk<C>(0, 0);
       if (!oon || e >= L) continue;
       if (LD == LD_HERM) {
         const int half = L / 2;
         const int src = (e <= half) ? e : L - e;
-        const double2 w = ((const double2*)a.in)[ibase + src * a.in_s];
-        v[j] = (e <= half) ? w : make_double2(w.x, -w.y);
+        const C w = ((const C*)a.in)[ibase + src * a.in_s];
+        v[j] = (e <= half) ? w : cmk<C>(w.x, -w.y);
       } else {
         const int r = e - a.off_load;
         if (r >= 0 && r < a.n_load) {
-          if (LD == LD_C) v[j] = ((const double2*)a.in)[ibase + r * a.in_s];
-          else v[j] = make_double2(((const double*)a.in)[ibase + r * a.in_s], 0.0);
+          if (LD == LD_C) v[j] = ((const C*)a.in)[ibase + r * a.in_s];
+          else v[j] = cmk<C>((F)((const double*)a.in)[ibase + r * a.in_s], 0);
         }
       }
     }
@@ -539,7 +550,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
           double k2 = kx * kx + ky * ky;
           if (MUL == MUL_PRE3) k2 = k2 + kz * kz;
           const double den = 0.5 * k2 + sig;
-          v[j] = make_double2(v[j].x / den, v[j].y / den);
+          v[j] = cmk<C>(v[j].x / (F)den, v[j].y / (F)den);
         }
       }
     }
@@ -553,8 +564,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   if (oon) {
     const long long obase = ob * a.out_ib + oo * a.out_so + oi * a.out_si;
     double dv[BEC_EPT];
-    double2 fp[ST == ST_CDOT ? BEC_EPT : 1], fd[ST == ST_CDOT ? BEC_EPT : 1], fg[ST == ST_CDOT ? BEC_EPT : 1];
-    double2 fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
+    C fp[ST == ST_CDOT ? BEC_EPT : 1], fd[ST == ST_CDOT ? BEC_EPT : 1], fg[ST == ST_CDOT ? BEC_EPT : 1];
+    C fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
       const int q = lane + j * tpl, e = q - a.off_store;
@@ -562,8 +573,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       const long long adr = obase + e * a.out_s;
       if (ST == ST_RE && a.dot) dv[j] = a.dot[adr];
       if (ST == ST_CDOT) {
-        fp[j] = a.dpsi[adr]; fd[j] = a.dd[adr]; fg[j] = a.dg[adr];
-        fy[j] = a.dy[adr]; fs[j] = a.ds[adr];
+        fp[j] = ((const C*)a.dpsi)[adr]; fd[j] = ((const C*)a.dd)[adr]; fg[j] = ((const C*)a.dg)[adr];
+        fy[j] = ((const C*)a.dy)[adr]; fs[j] = ((const C*)a.ds)[adr];
       }
     }
 #pragma unroll
@@ -571,15 +582,15 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       const int q = lane + j * tpl, e = q - a.off_store;
       if (q >= L || e < 0 || e >= a.n_store) continue;
       const long long adr = obase + e * a.out_s;
-      const double2 w = v[j];
+      const C w = v[j];
       if (ST == ST_CDOT) {
-        const double2 zi = make_double2(w.x * a.out_scale, w.y * a.out_scale);
-        ((double2*)a.out)[adr] = zi;
-        dot_accum(zv, zi, fp[j], fd[j], fg[j], fy[j], fs[j]);
+        const C zi = cscale(w, a.out_scale);
+        ((C*)a.out)[adr] = zi;
+        dot_accum(zv, to_d2(zi), to_d2(fp[j]), to_d2(fd[j]), to_d2(fg[j]), to_d2(fy[j]), to_d2(fs[j]));
       } else if (ST == ST_C) {
-        ((double2*)a.out)[adr] = make_double2(w.x * a.out_scale, w.y * a.out_scale);
+        ((C*)a.out)[adr] = cscale(w, a.out_scale);
       } else {
-        const double r = w.x * a.out_scale;
+        const double r = (double)w.x * a.out_scale;
         ((double*)a.out)[adr] = r;
         if (a.dot) dsum[0] += r * dv[j];
       }
@@ -683,8 +694,9 @@ __global__ __launch_bounds__(BEC_WG) void k_dots(const double2* __restrict__ z,
 // g = h - mu psi, and sum |g|^2 (one instance)
 // after the Armijo controller: keep the lowest-residual state, and after a
 // failed line search restart the direction from the preconditioned gradient
-__global__ __launch_bounds__(BEC_WG) void k_post(const double2* __restrict__ psi, double2* best, double2* d,
-                                                 const double2* __restrict__ z, const double* __restrict__ tab,
+template <class C>
+__global__ __launch_bounds__(BEC_WG) void k_post(const C* __restrict__ psi, C* best, C* d,
+                                                 const C* __restrict__ z, const double* __restrict__ tab,
                                                  const unsigned* __restrict__ fl, long long n) {
   const int b = blockIdx.y;
   const unsigned f = fl[b];
@@ -695,11 +707,11 @@ __global__ __launch_bounds__(BEC_WG) void k_post(const double2* __restrict__ psi
   const double az = t[DS_AZ], gre = t[DS_GRE], gim = t[DS_GIM];
   ELEM_LOOP(n) {
     const long long j = off + i;
-    const double2 p = psi[j];
-    if (cp) best[j] = p;
+    const C pc = psi[j];
+    if (cp) best[j] = pc;
     if (fail) {
-      const double2 zz = z[j];
-      d[j] = make_double2(az * zz.x + gre * p.x - gim * p.y, az * zz.y + gre * p.y + gim * p.x);
+      const double2 zz = to_d2(z[j]), p = to_d2(pc);
+      d[j] = from_d2<C>(make_double2(az * zz.x + gre * p.x - gim * p.y, az * zz.y + gre * p.y + gim * p.x));
     }
   }
 }
@@ -747,7 +759,8 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
 // in one workgroup), per instance (grid y).  No atomics across the grid on a
 // single address except the candidate counter.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BEC_WG) void k_sigma_local(const double2* __restrict__ psi,
+template <class C>
+__global__ __launch_bounds__(BEC_WG) void k_sigma_local(const C* __restrict__ psi,
                                                         const double* __restrict__ phi,
                                                         const double* mrows, int nrows, int mkp, int mq,
                                                         const InstPhys* iphys, int ndim, int N, double dx,
@@ -1148,18 +1161,19 @@ static unsigned long long lowmask(int nb) { return nb >= 64 ? ~0ull : ((1ull << 
 template <int LD, int ST>
 static double pass_bytes(const PassArgs& a) {
   const double lines = a.nl;
-  const double in_el = (LD == LD_R) ? 8.0 : 16.0;
+  const double cel = a.f32 ? 8.0 : 16.0;              // complex element
+  const double in_el = (LD == LD_R) ? 8.0 : cel;
   const double loaded = (LD == LD_HERM) ? (double)(a.fp.L / 2 + 1) : (double)a.n_load;
   double b = lines * loaded * in_el;
-  const double st_el = (ST == ST_RE) ? 8.0 : 16.0;
+  const double st_el = (ST == ST_RE) ? 8.0 : cel;
   b += lines * a.n_store * st_el;
   if (ST == ST_RE && a.dot) b += lines * a.n_store * 8.0;
-  if (ST == ST_CDOT) b += lines * a.n_store * 5.0 * 16.0;
+  if (ST == ST_CDOT) b += lines * a.n_store * 5.0 * cel;
   return b;
 }
 template <int KIND>
 static double kin_bytes(const KinArgs& a) {
-  const double n = (double)a.nl * a.N, c = 16.0 * n, r = 8.0 * n;
+  const double n = (double)a.nl * a.N, c = (a.f32 ? 8.0 : 16.0) * n, r = 8.0 * n;
   double b = c;                                          // psi
   if (a.d) b += c;
   if (KIND == KE_FIRST) {
@@ -1192,7 +1206,8 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   const int total = a.nb * a.nl;
   a.lpw = pick_lpw(a.tpl, a.nl, total, !CONTIG);
   const int blocks = xcd_grid(total, a.lpw);
-  const size_t shm = ((size_t)a.lpw * lpitch<CONTIG>(a.fp.L) + a.fp.L) * sizeof(double2);
+  const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
+  const size_t shm = ((size_t)a.lpw * lpitch<CONTIG>(a.fp.L) + a.fp.L) * csz;
   const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL + (ST == ST_CDOT ? 5 : 0);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const double bpi = pass_bytes<LD, ST>(a);
@@ -1200,11 +1215,21 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
     a.ucount = prof_counter(id);
     a.ubytes = (unsigned long long)bpi;
   }
-  if (prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb))))
-    hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1,
-                          0, a);
-  else
-    hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
+  const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb)));
+  const dim3 grid(blocks), block(a.tpl * a.lpw);
+  // FP32 phases run the plain transforms and the preconditioner passes only
+  constexpr bool kF32 = (LD == LD_C) && (MUL == MUL_NONE || MUL == MUL_PRE2 || MUL == MUL_PRE3);
+  if (a.f32) {
+    if constexpr (kF32) {
+      if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, float2>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, float2>), grid, block, shm, s, a);
+    } else {
+      abort();
+    }
+  } else {
+    if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, e0, e1, 0, a);
+    else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, a);
+  }
   return (a.nl + a.lpw - 1) / a.lpw;
 }
 
@@ -1238,7 +1263,8 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   const int total = a.nb * a.nl;
   a.lpw = pick_lpw(a.tpl, a.nl, total, !Ax<AX>::contig);
   const int blocks = xcd_grid(total, a.lpw);
-  const size_t shm = ((size_t)a.lpw * lpitch<Ax<AX>::contig>(a.N) + a.N) * sizeof(double2);
+  const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
+  const size_t shm = ((size_t)a.lpw * lpitch<Ax<AX>::contig>(a.N) + a.N) * csz;
   const int id = 100 + 10 * AX + KIND;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const double bpi = kin_bytes<KIND>(a);
@@ -1246,10 +1272,15 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
     a.ucount = prof_counter(id);
     a.ubytes = (unsigned long long)bpi;
   }
-  if (prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb))))
-    hipExtLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, e0, e1, 0, a);
-  else
-    hipLaunchKernelGGL((k_kin<AX, KIND>), dim3(blocks), dim3(a.tpl * a.lpw), shm, s, a);
+  const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb)));
+  const dim3 grid(blocks), block(a.tpl * a.lpw);
+  if (a.f32) {
+    if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, float2>), grid, block, shm, s, e0, e1, 0, a);
+    else hipLaunchKernelGGL((k_kin<AX, KIND, float2>), grid, block, shm, s, a);
+  } else {
+    if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, e0, e1, 0, a);
+    else hipLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, a);
+  }
   return (a.nl + a.lpw - 1) / a.lpw;
 }
 
@@ -1288,8 +1319,25 @@ void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_
   hipLaunchKernelGGL(k_rho, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, rho, n);
 }
 void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
-                 const unsigned* fl, long long n, int nb, hipStream_t s) {
-  hipLaunchKernelGGL(k_post, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, best, d, z, tab, fl, n);
+                 const unsigned* fl, long long n, int nb, int f32, hipStream_t s) {
+  if (f32)
+    hipLaunchKernelGGL(k_post<float2>, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, (const float2*)psi,
+                       (float2*)best, (float2*)d, (const float2*)z, tab, fl, n);
+  else
+    hipLaunchKernelGGL(k_post<double2>, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, best, d, z, tab, fl, n);
+}
+// FP64 <-> FP32 copies of nb complex fields of n points (phase changes)
+__global__ __launch_bounds__(BEC_WG) void k_narrow(const double2* __restrict__ in, float2* out, long long n) {
+  ELEM_LOOP(n) out[i] = from_d2<float2>(in[i]);
+}
+__global__ __launch_bounds__(BEC_WG) void k_widen(const float2* __restrict__ in, double2* out, long long n) {
+  ELEM_LOOP(n) out[i] = to_d2(in[i]);
+}
+void launch_narrow(const double2* in, float2* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_narrow, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, in, out, n);
+}
+void launch_widen(const float2* in, double2* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_widen, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, in, out, n);
 }
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);
@@ -1329,14 +1377,18 @@ int launch_moments(const double2* psi, int ndim, int N, double dx, double* parti
   return gr;
 }
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
-                  const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s) {
+                  const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s, int f32) {
   long long g1 = (p.n + 4 * BEC_WG - 1) / (4 * BEC_WG);
   if (g1 > SIGMA_MAX_BLOCKS) g1 = SIGMA_MAX_BLOCKS;
   const int g = elem_blocks(p.n);
   unsigned long long* part = (unsigned long long*)p.sig_part;
   prof_pre(SITE_SIGMA_LOCAL, s);
-  hipLaunchKernelGGL(k_sigma_local, dim3((int)g1, nb), dim3(BEC_WG), 0, s, psi, phi, mrows, nrows, mkp, mq,
-                     p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part);
+  if (f32)
+    hipLaunchKernelGGL(k_sigma_local<float2>, dim3((int)g1, nb), dim3(BEC_WG), 0, s, (const float2*)psi, phi, mrows,
+                       nrows, mkp, mq, p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part);
+  else
+    hipLaunchKernelGGL(k_sigma_local<double2>, dim3((int)g1, nb), dim3(BEC_WG), 0, s, psi, phi, mrows, nrows, mkp,
+                       mq, p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part);
   prof_post(SITE_SIGMA_LOCAL, s);
   prof_pre(SITE_SIGMA_HIST, s);
   hipLaunchKernelGGL(k_sigma_hist, dim3(g, nb), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,

@@ -32,6 +32,7 @@ struct Batch {
   unsigned bit = 0;
   const double* tab = nullptr;
   bool dev = false;
+  int f32 = 0;                // complex fields are float2 (FP32 phase)
 };
 inline Batch one_instance() { return Batch{1, 1ull}; }
 inline Batch dev_batch(const Plan& p, int nb, unsigned bit) {
@@ -40,6 +41,7 @@ inline Batch dev_batch(const Plan& p, int nb, unsigned bit) {
   B.bit = bit;
   B.tab = p.dtab;
   B.dev = true;
+  B.f32 = p.f32;
   return B;
 }
 inline Act act_of(const Batch& B) { return Act{B.act, B.fl, B.bit}; }

@@ -233,6 +233,7 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, c
   a.Nk = p.N;
   a.invNdx = p.invNdx;
   a.sigma = p.sigma;
+  a.f32 = B.f32;
   return a;
 }
 
@@ -302,6 +303,7 @@ static KinArgs kin_base(const Plan& p, const Batch& B) {
   a.tpl = p.lcN.tpl;
   a.lpw = p.lcN.lpw;
   a.N = p.N;
+  a.f32 = B.f32;
   a.nl = (int)(p.n / p.N);
   a.nb = B.nb;
   a.act = act_of(B);

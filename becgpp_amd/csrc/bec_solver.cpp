@@ -29,8 +29,11 @@ void solver_release(Plan& p) {
     if (p.st[i]) (void)hipFree(p.st[i]);
   for (int i = 0; i < 2; ++i)
     if (p.stphi[i]) (void)hipFree(p.stphi[i]);
+  for (int i = 0; i < 8; ++i)
+    if (p.st32[i]) (void)hipFree(p.st32[i]);
   memset(p.st, 0, sizeof(p.st));
   memset(p.stphi, 0, sizeof(p.stphi));
+  memset(p.st32, 0, sizeof(p.st32));
   p.scap = 0;
 }
 
@@ -41,6 +44,13 @@ static void solver_alloc(Plan& p, int nb) {
   for (int i = 0; i < S_COUNT; ++i) HIPCHK(hipMalloc(&p.st[i], sizeof(double2) * p.n * nb));
   for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&p.stphi[i], sizeof(double) * p.n * nb));
   p.scap = nb;
+}
+
+// complex64 copies of the solver state for an FP32 phase (sized like st)
+static void solver_alloc32(Plan& p) {
+  if (p.st32[0]) return;
+  HIPCHK(hipStreamSynchronize(p.stream));
+  for (int i = 0; i < S_COUNT; ++i) HIPCHK(hipMalloc(&p.st32[i], sizeof(float2) * p.n * p.scap));
 }
 
 static void dev_release(Plan& p) {
@@ -135,13 +145,16 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
                         double* trace, int trace_cap, int& n_tr) {
   const long long n = p.n;
   hipStream_t s = p.stream;
-  double2* PSI = p.st[S_PSI];
-  double2* SD = p.st[S_SD];
-  double2* G = p.st[S_G];
-  double2* Y = p.st[S_Y];
-  double2* D = p.st[S_D];
-  double2* Z = p.st[S_Z];
-  double2* BEST = p.st[S_BEST];
+  // an FP32 phase runs the same passes on the complex64 fields (pointers
+  // reinterpreted by the launches, which all see p.f32)
+  double2* const* F = p.f32 ? (double2* const*)p.st32 : p.st;
+  double2* PSI = F[S_PSI];
+  double2* SD = F[S_SD];
+  double2* G = F[S_G];
+  double2* Y = F[S_Y];
+  double2* D = F[S_D];
+  double2* Z = F[S_Z];
+  double2* BEST = F[S_BEST];
   double* phi = p.stphi[0];
   double* phit = p.stphi[1];
   dev_alloc(p, nb, trace ? trace_cap : 0);
@@ -220,7 +233,7 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     hipStream_t st = p.stream;           // the capture stream while the loop runs
     const Batch BD = dev_batch(p, nb, FL_DIR);
     launch_sigma(p, nb, act_of(BD), PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2, floor_,
-                 st);
+                 st, p.f32);
     // z = P^-1 g, with the CG inner products accumulated by its last pass
     const DotFields df{PSI, D, G, Y, SD};
     c.n_dots = run_precond(p, BD, G, Z, &df);
@@ -240,7 +253,7 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     c.n_e2 = er.r2;
     if (lr_any) c.n_gr = run_conv(p, 0, dev_batch(p, nb, FL_SLR), p.rho, 1.0, phit, p.rho);
     launch_ctrl_armijo(c, st);
-    launch_post(PSI, BEST, D, Z, p.dtab, p.flags, n, nb, st);
+    launch_post(PSI, BEST, D, Z, p.dtab, p.flags, n, nb, p.f32, st);
     HArgs a;
     memset(&a, 0, sizeof(a));
     a.psi = PSI;
@@ -347,8 +360,11 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   }
 }
 
-static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
-                       bec_solve_out* out, double* trace, int trace_cap) {
+// One run of the loop from psi_io (normalised first) to its result in psi_io.
+// f32: the loop runs on complex64 copies of the fields (set up from the FP64
+// initial energy and residual), and its states are widened back at the end.
+static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
+                       bec_solve_out* out, double* trace, int trace_cap, bool f32) {
   if (nb < 1 || nb > BEC_BMAX) throw BecError(1, "batch size must be in [1, 32]");
   if (cf->cg_beta < 0 || cf->cg_beta > 2) throw BecError(1, "cg_beta must be 'pr', 'pr_precond' or 'none'");
   if (nb > 1) {
@@ -459,7 +475,29 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   }
   launch_copy(PSI, BEST, n, nb, mask_act(all), s);
 
-  if (!cf->host_loop) {
+  if (f32) {
+    if (lr_any) throw BecError(1, "internal: FP32 phase with a long-range kernel");
+    solver_alloc32(p);
+    const long long nn = n * nb;
+    launch_narrow(PSI, p.st32[S_PSI], nn, s);
+    launch_narrow(G, p.st32[S_G], nn, s);
+    launch_narrow(BEST, p.st32[S_BEST], nn, s);
+    HIPCHK(hipMemsetAsync(p.st32[S_SD], 0, sizeof(float2) * nn, s));
+    HIPCHK(hipMemsetAsync(p.st32[S_Y], 0, sizeof(float2) * nn, s));
+    HIPCHK(hipMemsetAsync(p.st32[S_D], 0, sizeof(float2) * nn, s));
+    HIPCHK(hipGetLastError());
+    p.f32 = 1;
+    try {
+      device_loop(p, nb, cf, I, false, nullptr, 0, n_tr);
+    } catch (...) {
+      p.f32 = 0;
+      throw;
+    }
+    p.f32 = 0;
+    launch_widen(p.st32[S_PSI], PSI, nn, s);
+    launch_widen(p.st32[S_BEST], BEST, nn, s);
+    HIPCHK(hipGetLastError());
+  } else if (!cf->host_loop) {
     device_loop(p, nb, cf, I, lr_any, trace && nb == 1 ? trace : nullptr, trace_cap, n_tr);
   } else {
   int it = 0;
@@ -703,6 +741,33 @@ static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   return 0;
 }
 
+// The loop, preceded by an FP32 warm phase when warm_res_tol > res_tol and
+// no instance has long range (the convolution stays FP64 throughout): the
+// FP64 loop restarts from the warm phase's returned states.
+static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
+                       bec_solve_out* out, double* trace, int trace_cap) {
+  bool lr = false;
+  for (int b = 0; b < nb && b < BEC_BMAX; ++b)
+    lr = lr || plan_long_range(p, Phys{php[b].beta2, php[b].beta3, php[b].Omega, php[b].G_C});
+  if (!(cf->warm_res_tol > cf->res_tol) || lr) {
+    const int r = solve_phase(p, nb, php, cf, psi_io, out, trace, trace_cap, false);
+    for (int b = 0; b < nb; ++b) out[b].warm_iters = 0;
+    return r;
+  }
+  bec_solver_cfg w = *cf;
+  w.res_tol = cf->warm_res_tol;
+  std::vector<bec_solve_out> ow(nb);
+  solve_phase(p, nb, php, &w, psi_io, ow.data(), nullptr, 0, true);
+  const int r = solve_phase(p, nb, php, cf, psi_io, out, trace, trace_cap, false);
+  for (int b = 0; b < nb; ++b) {
+    out[b].warm_iters = ow[b].iters;
+    out[b].iters += ow[b].iters;
+    out[b].walltime += ow[b].walltime;
+    out[b].n_energy += ow[b].n_energy;
+  }
+  return r;
+}
+
 extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solver_cfg* cf, void* psi_io,
                                 bec_solve_out* out, double* trace, int trace_cap) {
   try {
@@ -723,6 +788,7 @@ extern "C" int bec_plan_reserve(bec_plan* h, int nb) {
     Plan& p = h->p;
     plan_reserve(p, nb);
     solver_alloc(p, nb);
+    solver_alloc32(p);
     dev_alloc(p, nb, 0);
     HIPCHK(hipStreamSynchronize(p.stream));
     return 0;

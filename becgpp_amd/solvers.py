@@ -34,7 +34,8 @@ def _solver_cfg(cfg, verbose):
         int(cfg.get("conv_window", 2000)), float(cfg.get("step", 1.0)), float(cfg.get("step_max", 3.0)),
         int(cfg.get("linesearch_max", 8)), int(cfg.get("linesearch_stall", 10)),
         float(cfg.get("precond_shift_min", 0.5)), _lib.CG_BETA[cg_beta], int(cfg.get("cg_restart", 30)),
-        1 if verbose else 0, int(cfg.get("check", 200)), 1 if cfg.get("host_loop", False) else 0)
+        1 if verbose else 0, int(cfg.get("check", 200)), 1 if cfg.get("host_loop", False) else 0,
+        float(cfg.get("warm_res_tol", 1e-4)) if cfg.get("warm_fp32", False) else 0.0)
 
 
 def solve_native(cfg, G, psi0, verbose=False, record=False):
@@ -97,6 +98,8 @@ def _finish(psi, res, G, cfg):
     obs.update(iters=res.iters, walltime=res.walltime, resid_abs=resid_abs, resid_rel=resid_rel,
                min_resid=res.best_resid, converged=converged,
                stop_reason=_lib.STOP_REASONS[res.stop_reason], preconditioner_shift=res.sigma)
+    if res.warm_iters:
+        obs["warm_iters"] = res.warm_iters
     return obs
 
 

@@ -71,11 +71,12 @@ WORKLOADS = {
     "droplet_sweep_2d_n256": dict(
         cfg=dict(mode="single", dimension="2D", s=2, trap_coeff=0.0, Omega=0.0, beta2=-120.0, beta3=1000.0,
                  G_C=0.0, kernel="none", L=20.0, Ngrid=256, nseeds=4, res_tol=1e-6, maxit=60000,
-                 linesearch_max=20),
+                 linesearch_max=20, warm_fp32=True, warm_res_tol=1e-4),
         points=[dict(beta2=float(b)) for b in
                 [-200.0 + 160.0 * i / 15.0 for i in range(16)]],
         desc="2D cubic-quintic droplet sweep, no trap, beta2 = 16 values in [-200, -40], beta3=1e3, "
-             "L=20, N=256, nseeds=4 (64 solves), res_tol=1e-6, linesearch_max=20"),
+             "L=20, N=256, nseeds=4 (64 solves), FP32 warm phase to res 1e-4 then FP64 to res_tol=1e-6, "
+             "linesearch_max=20"),
 }
 
 # watched kernel: launch-site id -> (name, algorithmic bytes per instance of its
@@ -118,6 +119,8 @@ def parse():
                     help="skip the single-solve comparison run")
     ap.add_argument("--no-polish", action="store_true",
                     help="skip the accuracy check of the selected state (bounded profiling runs only)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="run a workload's FP32 warm phase in FP64 too (comparison)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     return ap.parse_args()
@@ -147,6 +150,8 @@ def main():
 
     wl = WORKLOADS[a.workload]
     cfg = B.default_cfg(**wl["cfg"])
+    if a.fp64_only:
+        cfg["warm_fp32"] = False
     if a.maxit is not None:
         cfg["maxit"] = int(a.maxit)
     if a.calibrate_copy:
@@ -202,6 +207,7 @@ def main():
         best = select(res)
         Es = [obs["E"] for _, _, obs in res]
         stops = [f'{obs["stop_reason"]}@{obs["iters"]}' for _, _, obs in res]
+        warm_iters[0] += sum(obs.get("warm_iters", 0) for _, _, obs in res)
         return best, sum(obs["iters"] for _, _, obs in res), Es, stops
 
     def step_single():
@@ -209,9 +215,11 @@ def main():
         return select(res), sum(obs["iters"] for _, _, obs in res), [obs["E"] for _, _, obs in res]
 
     lib = _lib.load()
+    warm_iters = [0]
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
+    warm_iters[0] = 0
     t0 = time.perf_counter()
     tot_iters = 0
     results = []
@@ -284,10 +292,15 @@ def main():
         resid_rel=max(best[i][1]["resid_rel"] for i in range(npt)),
         roofline=roof,
     )
+    if cfg.get("warm_fp32", False):
+        out["warm_phase"] = dict(dtype="f32", res_tol=float(cfg["warm_res_tol"]),
+                                 iters_share=warm_iters[0] / max(1, tot_iters),
+                                 note="complex64 fields and transforms, FP64 sums and Armijo test; the FP64 loop "
+                                      "then runs from each warm state to res_tol")
     # |dE/E| of the first point's selected state: polish it to the FP64 residual floor
     if not a.no_polish:
         ib = min((k for k in range(nsol) if runs[k][0] == 0), key=lambda k: Es[k])
-        pc = dict(cs[ib], res_tol=1e-9, maxit=20000)
+        pc = dict(cs[ib], res_tol=1e-9, maxit=20000, warm_fp32=False)
         _, _, pol = B.ground_state(pc, G=grids[runs[ib][3]], psi0=b0psi, verbose=False)
         out["E_polished"] = pol["E"]
         out["dE_rel_vs_polished"] = abs(b0["E"] - pol["E"]) / abs(pol["E"])

@@ -69,6 +69,11 @@ typedef struct bec_solver_cfg {
   int host_loop;   /* 0: the loop's scalar logic runs in controller kernels on the
                       device (default); 1: on the host, one synchronisation per
                       iteration and Armijo trial (no reference counterpart) */
+  double warm_res_tol;  /* > res_tol: an FP32 warm phase first runs the loop on
+                           complex64 fields to this relative residual, then the
+                           FP64 loop polishes from its best state; instances with
+                           long range run FP64 throughout; 0 = FP64 only (no
+                           reference counterpart) */
 } bec_solver_cfg;
 
 typedef struct bec_solve_out {
@@ -82,6 +87,7 @@ typedef struct bec_solve_out {
   double sigma;         /* last Sobolev shift (solvers.py:232) */
   double walltime;      /* seconds, from after the initial energy (solvers.py:107) */
   long long n_energy;   /* energy evaluations (line-search trials + initial) */
+  int warm_iters;       /* of iters, those of the FP32 warm phase */
 } bec_solve_out;
 
 const char* bec_last_error(void);

@@ -116,3 +116,43 @@ def test_device_and_host_loops_agree(kw):
     assert abs(od["E"] - oh["E"]) <= 1e-9 * max(1.0, abs(oh["E"])), (od["E"], oh["E"])
     for k in ("mu", "Lz"):
         assert abs(od[k] - oh[k]) <= 1e-8 * max(1.0, abs(oh[k])), (k, od[k], oh[k])
+
+
+@pytest.mark.parametrize("kw", [dict(beta2=100.0), dict(beta2=30.0, beta3=5.0, Omega=0.4),
+                                dict(trap_coeff=0.0, beta2=-60.0, beta3=1000.0, L=8.0)])
+def test_fp32_warm_phase_then_fp64_polish(kw):
+    """warm_fp32: the loop first runs on complex64 fields to res 1e-4, then the
+    FP64 loop polishes; the converged state matches the oracle's FP64 solve
+    (E, mu, Lz to 1e-8, density L2 to 1e-6)."""
+    import becgpp_amd as B
+    c = cfg(res_tol=1e-8, maxit=20000, linesearch_max=20, **kw)
+    Go = O.make_grid(c)
+    p0 = O.noisy_gaussian(Go, 1e-3, seed=21)
+    po, _, oo = O.ground_state(c, Go, p0)
+    pd, _, od = B.ground_state(dict(c, warm_fp32=True), B.make_grid(c), torch.as_tensor(p0).cuda(), verbose=False)
+    assert od.get("warm_iters", 0) > 0
+    for k in ("E", "mu", "Lz"):
+        assert abs(od[k] - oo[k]) <= 1e-8 * max(1.0, abs(oo[k])), (k, od[k], oo[k])
+    ro, rd = np.abs(po) ** 2, np.abs(pd.cpu().numpy()) ** 2
+    assert np.sqrt(((rd - ro) ** 2).sum() / (ro ** 2).sum()) < 1e-6
+    assert od["resid_rel"] < 2e-8
+
+
+def test_fp32_warm_phase_batched_and_long_range_stays_fp64():
+    import becgpp_amd as B
+    # a grid with the Newton kernel; the three batched instances have G_C = 0
+    base = cfg(res_tol=1e-8, maxit=20000, linesearch_max=20, kernel="newton", G_C=3.0)
+    cs = [dict(base, beta2=b2, Omega=om, G_C=0.0, warm_fp32=True)
+          for b2, om in ((80.0, 0.0), (40.0, 0.3), (20.0, 0.5))]
+    Go = O.make_grid(base)
+    p0s = [O.noisy_gaussian(Go, 1e-3, seed=60 + i) for i in range(3)]
+    G = B.make_grid(base)
+    runs = B.ground_state_batch(cs, G=G, psi0s=[torch.as_tensor(p).cuda() for p in p0s])
+    for c, p0, (_, _, od) in zip(cs, p0s, runs):
+        assert od.get("warm_iters", 0) > 0
+        _, _, oo = O.ground_state(c, O.make_grid(c), p0)
+        assert abs(od["E"] - oo["E"]) <= 1e-8 * max(1.0, abs(oo["E"]))
+    # a long-range instance runs FP64 throughout
+    lr = dict(cs[0], G_C=3.0)
+    _, _, ol = B.ground_state(lr, G, torch.as_tensor(p0s[0]).cuda(), verbose=False)
+    assert "warm_iters" not in ol and ol["Egrav"] != 0.0
