@@ -72,6 +72,8 @@ SIGNATURES = {
                                               ctypes.POINTER(SolverCfg), ctypes.c_void_p,
                                               ctypes.POINTER(SolveOut)]),
     "bec_plan_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "bec_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p, c_double_p, ctypes.c_char_p,
+                                         ctypes.c_int]),
     "bec_prof_watch": (ctypes.c_int, [ctypes.c_int]),
     "bec_prof_read": (ctypes.c_int, [c_double_p, ctypes.POINTER(ctypes.c_longlong), c_double_p]),
 }

@@ -40,7 +40,9 @@ CFG_DEFAULTS = dict(
 #                 polish in FP64 (instances with long range stay FP64)
 #   host_loop:    keep the loop's scalar logic on the host (one synchronisation
 #                 per iteration) instead of the device controllers
-CFG_EXTENSIONS = dict(warm_fp32=False, warm_res_tol=1e-4, host_loop=False)
+#   autotune:     time the candidate FFT stage orders and lines per workgroup
+#                 on the first solve of a plan and keep the fastest
+CFG_EXTENSIONS = dict(warm_fp32=False, warm_res_tol=1e-4, host_loop=False, autotune=False)
 
 _ALIASES = {"res": "res_tol", "etol": "energy_tol", "N": "Ngrid", "dim": "dimension",
             "precond_shift": "precond_shift_min"}

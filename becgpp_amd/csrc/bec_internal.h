@@ -84,6 +84,7 @@ struct PassArgs {
   unsigned long long ubytes;           //   (ubytes per instance, added once per instance; nullable)
   const double2* dpsi; const double2* dd; const double2* dg; const double2* dy; const double2* ds;  // ST_CDOT
   int f32;                             // complex fields (in/out, ST_CDOT) are float2: an FP32 phase
+  int lsh[2];                          // tuned lines-per-workgroup shift: contiguous, strided passes
 };
 
 // Arguments of the kinetic/rotation/energy/H passes (line length N, no pad).
@@ -115,6 +116,7 @@ struct KinArgs {
   unsigned comb_bit;                                     //   d = az z + ad d + ag g + gam psi first
   int acc_sc;                                            // KH_MID/KH_LAST: acc is divided by sc[b]
   int f32;                                               // complex fields are float2: an FP32 phase
+  int lsh[2];                                            // tuned lines-per-workgroup shift (as PassArgs)
 };
 
 // per-instance combination d <- az z + ad d + ag g + gam psi
@@ -152,6 +154,7 @@ struct Plan {
   FftPlan fpN, fpM;
   double2* twN; double2* twM;
   LineCfg lcN, lcM;
+  int lsh[2];                        // autotuned log2 factor on the lines per workgroup (contiguous, strided)
   double* khat[2];                   // [0] = K, [1] = W (virial) real-even spectra
   int kkind[2];                      // 0 none, 1 newton, 2 log, 3 table
   // batch workspace, sized for cap instances

@@ -1155,6 +1155,15 @@ static int pick_lpw(int tpl, int nl, int total, bool strided) {
   while (l * 2 <= lmax && l * 2 <= want && nl % (l * 2) == 0) l *= 2;
   return l;
 }
+// The heuristic count scaled by 2^shift (the plan's autotuned schedule):
+// halving keeps l a divisor of nl; doubling only while it still divides nl and
+// the workgroup stays within BEC_WG threads.
+static int shifted_lpw(int l, int tpl, int nl, int shift) {
+  const int lmin = lpw_min(tpl);
+  while (shift > 0 && l * 2 * tpl <= BEC_WG && nl % (l * 2) == 0) { l *= 2; --shift; }
+  while (shift < 0 && l / 2 >= lmin) { l /= 2; ++shift; }
+  return l;
+}
 static unsigned long long lowmask(int nb) { return nb >= 64 ? ~0ull : ((1ull << nb) - 1ull); }
 // Algorithmic bytes per instance of a line pass: every array element the pass
 // must read or write, once (re-reads through the caches are not counted).
@@ -1204,7 +1213,7 @@ template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 static int go_pass(const PassArgs& a0, hipStream_t s) {
   PassArgs a = a0;
   const int total = a.nb * a.nl;
-  a.lpw = pick_lpw(a.tpl, a.nl, total, !CONTIG);
+  a.lpw = shifted_lpw(pick_lpw(a.tpl, a.nl, total, !CONTIG), a.tpl, a.nl, a.lsh[CONTIG ? 0 : 1]);
   const int blocks = xcd_grid(total, a.lpw);
   const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
   const size_t shm = ((size_t)a.lpw * lpitch<CONTIG>(a.fp.L) + a.fp.L) * csz;
@@ -1261,7 +1270,8 @@ template <int AX, int KIND>
 static int go_kin(const KinArgs& a0, hipStream_t s) {
   KinArgs a = a0;
   const int total = a.nb * a.nl;
-  a.lpw = pick_lpw(a.tpl, a.nl, total, !Ax<AX>::contig);
+  a.lpw = shifted_lpw(pick_lpw(a.tpl, a.nl, total, !Ax<AX>::contig), a.tpl, a.nl,
+                      a.lsh[Ax<AX>::contig ? 0 : 1]);
   const int blocks = xcd_grid(total, a.lpw);
   const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
   const size_t shm = ((size_t)a.lpw * lpitch<Ax<AX>::contig>(a.N) + a.N) * csz;

@@ -5,6 +5,7 @@
 #include "bec_internal.h"
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 struct BecError : std::runtime_error {
   int code;
@@ -113,6 +114,9 @@ struct DotFields {
 int run_precond(Plan& p, const Batch& B, const double2* g, double2* out, const DotFields* dots = nullptr);
 
 void solver_release(Plan& p);
+// FFT stage schedules (the autotuner's search space)
+void set_stage_order(FftPlan& fp, const std::vector<int>& rad);
+std::vector<std::vector<int>> stage_orders(const FftPlan& fp, int cap);
 void bec_set_error(const char* msg);
 
 struct bec_plan {

@@ -71,15 +71,11 @@ static bool factor(int L, std::vector<int>& rad, bool& reg) {
   return (int)rad.size() <= BEC_MAX_STAGES;
 }
 
-static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
-  std::vector<int> rad;
-  bool reg = false;
-  if (L < 2 || L > BEC_MAX_L || !factor(L, rad, reg))
-    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b 5^c <= " +
-        std::to_string(BEC_MAX_L));
-  fp.L = L;
+// Stage radices (in order) of fp, with each stage's Ns, twiddle stride and
+// j / Ns multiplier; any order of the factors is a valid Stockham transform.
+void set_stage_order(FftPlan& fp, const std::vector<int>& rad) {
+  const int L = fp.L;
   fp.nst = (int)rad.size();
-  fp.reg = reg ? 1 : 0;
   int ns = 1;
   for (int i = 0; i < BEC_MAX_STAGES; ++i) {
     fp.rad[i] = i < fp.nst ? rad[i] : 1;
@@ -90,6 +86,33 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
     fp.mag[i] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns) + 1u : 0u;
     if (i < fp.nst) ns *= rad[i];
   }
+}
+
+// Distinct stage orders of fp's radices that keep its kind of transform (a
+// register-resident one needs a first and a last radix dividing 8), at most
+// cap of them, the current order first.
+std::vector<std::vector<int>> stage_orders(const FftPlan& fp, int cap) {
+  std::vector<int> r(fp.rad, fp.rad + fp.nst);
+  std::vector<std::vector<int>> out{r};
+  std::vector<int> q = r;
+  std::sort(q.begin(), q.end());
+  do {
+    if ((int)out.size() >= cap) break;
+    if (fp.reg && fp.nst > 1 && (8 % q.front() != 0 || 8 % q.back() != 0)) continue;
+    if (q != r) out.push_back(q);
+  } while (std::next_permutation(q.begin(), q.end()));
+  return out;
+}
+
+static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
+  std::vector<int> rad;
+  bool reg = false;
+  if (L < 2 || L > BEC_MAX_L || !factor(L, rad, reg))
+    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b 5^c <= " +
+        std::to_string(BEC_MAX_L));
+  fp.L = L;
+  fp.reg = reg ? 1 : 0;
+  set_stage_order(fp, rad);
   // twiddles exp(-2 pi i m / L) evaluated in extended precision
   std::vector<double2> tw(L);
   const long double pi = 3.141592653589793238462643383279502884L;
@@ -234,6 +257,8 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, c
   a.invNdx = p.invNdx;
   a.sigma = p.sigma;
   a.f32 = B.f32;
+  a.lsh[0] = p.lsh[0];
+  a.lsh[1] = p.lsh[1];
   return a;
 }
 
@@ -304,6 +329,8 @@ static KinArgs kin_base(const Plan& p, const Batch& B) {
   a.lpw = p.lcN.lpw;
   a.N = p.N;
   a.f32 = B.f32;
+  a.lsh[0] = p.lsh[0];
+  a.lsh[1] = p.lsh[1];
   a.nl = (int)(p.n / p.N);
   a.nb = B.nb;
   a.act = act_of(B);

@@ -801,6 +801,121 @@ extern "C" int bec_plan_reserve(bec_plan* h, int nb) {
   }
 }
 
+// Autotuner: times the line passes of one loop step -- H psi on every axis,
+// the energies, the preconditioner and (with a kernel spectrum loaded) the
+// convolution -- over nb instances for each candidate schedule, and keeps the
+// fastest: the stage order of each transform length and a power-of-two factor
+// on the lines per workgroup of contiguous and of strided passes.
+extern "C" int bec_plan_autotune(bec_plan* h, int nb, double* ms_best, double* ms_default, char* report,
+                                 int report_cap) {
+  try {
+    if (!h) throw BecError(1, "null plan");
+    Plan& p = h->p;
+    plan_reserve(p, nb);
+    solver_alloc(p, nb);
+    hipStream_t s = p.stream;
+    const long long n = p.n;
+    double2* PSI = p.st[S_PSI];
+    double2* G = p.st[S_G];
+    double2* Z = p.st[S_Z];
+    // a smooth, finite state for every instance: psi = 1, rho = 1
+    std::vector<double2> one(n, make_double2(1.0, 0.0));
+    for (int b = 0; b < nb; ++b)
+      HIPCHK(hipMemcpyAsync(PSI + (size_t)b * n, one.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(p.rho, 0, sizeof(double) * n * nb, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<Phys> ph(nb, Phys{1.0, 0.0, 0.5, 0.0});
+    plan_set_phys(p, ph.data(), nb);
+    std::vector<double> mu(nb, 0.0);
+    const unsigned long long all = (nb >= 64) ? ~0ull : ((1ull << nb) - 1ull);
+    const Batch ALL{nb, all};
+    const bool conv = p.khat[0] != nullptr;
+    auto work = [&]() {
+      HArgs a;
+      memset(&a, 0, sizeof(a));
+      a.psi = PSI;
+      a.local_on = true;
+      a.mu = mu.data();
+      a.kf = 0.5;
+      a.rf_omega = 1;
+      a.out = G;
+      run_H(p, ALL, a);
+      run_precond(p, ALL, G, Z, nullptr);
+      run_energy_sums(p, ALL, PSI, nullptr, nullptr, conv ? p.rho : nullptr);
+      if (conv) run_conv(p, 0, ALL, p.rho, 1.0, p.loc, nullptr);
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto timed = [&]() {
+      work();                                  // warm-up
+      plan_sync(p);                            // fold (and drop) the queued host reductions
+      double tot = 0.0;
+      for (int r = 0; r < 3; ++r) {
+        HIPCHK(hipEventRecord(e0, s));
+        work();
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        tot += ms;
+        plan_sync(p);
+      }
+      return tot / 3.0;
+    };
+    const auto oN = stage_orders(p.fpN, 4);
+    const auto oM = conv ? stage_orders(p.fpM, 3) : std::vector<std::vector<int>>{{}};
+    double best = 1e300, base = -1.0;
+    std::vector<int> bN, bM;
+    int bl[2] = {0, 0};
+    for (const auto& rn : oN)
+      for (const auto& rm : oM)
+        for (int lc = -1; lc <= 1; ++lc)
+          for (int ls = -1; ls <= 1; ++ls) {
+            set_stage_order(p.fpN, rn);
+            if (conv) set_stage_order(p.fpM, rm);
+            p.lsh[0] = lc;
+            p.lsh[1] = ls;
+            const double t = timed();
+            if (rn == oN[0] && (!conv || rm == oM[0]) && lc == 0 && ls == 0) base = t;
+            if (t < best) {
+              best = t;
+              bN = rn;
+              bM = rm;
+              bl[0] = lc;
+              bl[1] = ls;
+            }
+          }
+    set_stage_order(p.fpN, bN);
+    if (conv) set_stage_order(p.fpM, bM);
+    p.lsh[0] = bl[0];
+    p.lsh[1] = bl[1];
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_best) *ms_best = best;
+    if (ms_default) *ms_default = base;
+    if (report && report_cap > 0) {
+      std::string r = "N=" + std::to_string(p.N) + " stages [";
+      for (size_t i = 0; i < bN.size(); ++i) r += (i ? "," : "") + std::to_string(bN[i]);
+      r += "]";
+      if (conv) {
+        r += " M=" + std::to_string(p.M) + " stages [";
+        for (size_t i = 0; i < bM.size(); ++i) r += (i ? "," : "") + std::to_string(bM[i]);
+        r += "]";
+      }
+      r += " lines/workgroup x2^" + std::to_string(bl[0]) + " contiguous, x2^" + std::to_string(bl[1]) + " strided";
+      snprintf(report, (size_t)report_cap, "%s", r.c_str());
+    }
+    return 0;
+  } catch (const BecError& e) {
+    bec_set_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    bec_set_error(e.what());
+    return 2;
+  }
+}
+
 extern "C" int bec_ground_state_batch(bec_plan* h, int nb, const bec_phys* php, const bec_solver_cfg* cf,
                                       void* psi_io, bec_solve_out* out) {
   try {

@@ -113,6 +113,17 @@ class NativePlan:
         self.ndim, self.N, self.L, self.pad, self.s, self.tc = ndim, N, L, pad, s, tc
         self.kernels = {}           # which -> key of the spectrum currently loaded
 
+    def autotune(self, nb):
+        """Choose this plan's FFT stage orders and lines per workgroup by timing
+        the line passes of a solver step over nb instances (once per plan)."""
+        if getattr(self, "schedule", None) is None:
+            best, dflt = ctypes.c_double(), ctypes.c_double()
+            rep = ctypes.create_string_buffer(256)
+            _lib.check(self.lib.bec_plan_autotune(self.h, int(nb), ctypes.byref(best), ctypes.byref(dflt), rep, 256))
+            self.schedule = dict(schedule=rep.value.decode(), step_passes_ms=best.value,
+                                 default_ms=dflt.value, tuned_for_instances=int(nb))
+        return self.schedule
+
     def bind_stream(self):
         st = torch.cuda.current_stream().cuda_stream
         _lib.check(self.lib.bec_plan_set_stream(self.h, ctypes.c_void_p(st)))

@@ -43,6 +43,8 @@ def solve_native(cfg, G, psi0, verbose=False, record=False):
     gc = float(cfg["G_C"])
     lr = G["kernel"] != "none" and abs(gc) > 1e-15
     pl = ensure_kernel(G, "K") if lr else plan_of(G)
+    if cfg.get("autotune", False):
+        pl.autotune(1)
     psi = as_field(psi0, G).clone()
     sc = _solver_cfg(cfg, verbose)
     ph = phys_for(G, cfg["beta2"], cfg["beta3"], cfg["Omega"], gc)
@@ -82,6 +84,8 @@ def solve_native_batch(cfgs, G, psi0s, verbose=False):
         raise ValueError(f"batch size must be in [1, {BATCH_MAX}]")
     lr_any = G["kernel"] != "none" and any(abs(float(c["G_C"])) > 1e-15 for c in cfgs)
     pl = ensure_kernel(G, "K") if lr_any else plan_of(G)
+    if cfgs[0].get("autotune", False):
+        pl.autotune(nb)
     psi = torch.stack([as_field(p0, G) for p0 in psi0s]).contiguous()
     sc = _solver_cfg(cfgs[0], verbose)
     ph = (_lib.Phys * nb)(*[phys_for(G, c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cfgs])
@@ -162,6 +166,8 @@ def ground_state_uneven(cfgs, psi0s=None, grids=None, verbose=False):
             pl = ensure_kernel(G, "K")
             if G["kernel"] == "q2d":
                 ensure_kernel(G, "W")
+        if cfgs[ks[0]].get("autotune", False):
+            pl.autotune(min(len(ks), BATCH_MAX))
         _lib.check(pl.lib.bec_plan_reserve(pl.h, min(len(ks), BATCH_MAX)))
     torch.cuda.synchronize()
     errors = []

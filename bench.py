@@ -119,6 +119,8 @@ def parse():
                     help="skip the single-solve comparison run")
     ap.add_argument("--no-polish", action="store_true",
                     help="skip the accuracy check of the selected state (bounded profiling runs only)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default FFT stage orders and lines per workgroup")
     ap.add_argument("--fp64-only", action="store_true",
                     help="run a workload's FP32 warm phase in FP64 too (comparison)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -215,6 +217,17 @@ def main():
         return select(res), sum(obs["iters"] for _, _, obs in res), [obs["E"] for _, _, obs in res]
 
     lib = _lib.load()
+    # schedule of each grid's plan: chosen by the autotuner before any timing
+    schedules = {}
+    if not a.no_autotune:
+        from becgpp_amd.grid import plan_of
+        from becgpp_amd.interactions import ensure_kernel
+        for gk, ks in groups.items():
+            Gg = grids[gk]
+            lr = Gg["kernel"] != "none" and any(abs(float(cs[k]["G_C"])) > 1e-15 for k in ks)
+            pl = ensure_kernel(Gg, "K") if lr else plan_of(Gg)
+            schedules["N=%d" % gk[0]] = pl.autotune(min(len(ks), 32))
+        torch.cuda.synchronize()
     warm_iters = [0]
     for _ in range(a.warmup):
         step()
@@ -292,6 +305,8 @@ def main():
         resid_rel=max(best[i][1]["resid_rel"] for i in range(npt)),
         roofline=roof,
     )
+    if schedules:
+        out["autotuned_schedule"] = schedules
     if cfg.get("warm_fp32", False):
         out["warm_phase"] = dict(dtype="f32", res_tol=float(cfg["warm_res_tol"]),
                                  iters_share=warm_iters[0] / max(1, tot_iters),

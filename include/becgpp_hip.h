@@ -170,6 +170,15 @@ int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_so
  * one thread per plan, each on its own non-default stream. */
 int bec_plan_reserve(bec_plan* p, int nb);
 
+/* Autotuner (no counterpart in the reference): times the line passes of one
+ * solver step over nb instances for every candidate schedule -- the stage
+ * (radix) order of each transform length and a power-of-two factor on the
+ * lines per workgroup of contiguous and of strided passes -- and keeps the
+ * fastest in the plan for its later calls.  ms_best / ms_default (nullable)
+ * receive the step-pass time of the chosen and of the default schedule;
+ * report (nullable) a one-line description. */
+int bec_plan_autotune(bec_plan* p, int nb, double* ms_best, double* ms_default, char* report, int report_cap);
+
 /* Measurement hooks (no counterpart in the reference, which has no profiler
  * hooks): time every launch of one kernel with HIP events on its stream;
  * bytes sums the algorithmic bytes of the timed launches (each array element

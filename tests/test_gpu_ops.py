@@ -204,3 +204,25 @@ def test_kernel_none_grid_ignores_G_C():
     assert abs(e_gc["E"].item() - eo["E"]) <= TOL * abs(eo["E"])
     h = B.apply_H(_dev(psi), G_none, dict(c_none, G_C=3.0)).cpu().numpy()
     assert relerr(h, O.apply_H(psi, Go, dict(c_none, G_C=3.0))) < TOL
+
+
+@pytest.mark.parametrize("kw", [dict(dimension="2D", Ngrid=96, L=8, beta2=30, G_C=3, kernel="newton", Omega=0.3),
+                                dict(dimension="3D", Ngrid=32, L=6, beta2=20, G_C=3, kernel="newton")])
+def test_autotuned_schedule_keeps_parity(kw):
+    """The autotuner may pick any stage order and lines per workgroup: the
+    operators stay within the parity bound under the schedule it keeps."""
+    import becgpp_amd as B
+    from becgpp_amd.interactions import ensure_kernel
+    c, Go, Gd = _pair(kw)
+    pl = ensure_kernel(Gd, "K")
+    pl.schedule = None
+    sched = pl.autotune(2)
+    assert sched["step_passes_ms"] > 0 and sched["step_passes_ms"] <= sched["default_ms"] * 1.0000001
+    assert "stages [" in sched["schedule"]
+    psi, _ = random_state(Go, 8)
+    ref = O.apply_H(psi, Go, c)
+    got = B.apply_H(_dev(psi), Gd, c).cpu().numpy()
+    assert relerr(got, ref) < TOL
+    e_ref = O.energy_components(psi, Go, c["beta2"], c["beta3"], c["Omega"], c["G_C"])
+    e_got = B.energy_components(_dev(psi), Gd, c["beta2"], c["beta3"], c["Omega"], c["G_C"])
+    assert abs(e_got["E"].item() - e_ref["E"]) <= TOL * max(1.0, abs(e_ref["E"]))
