@@ -249,7 +249,7 @@ def main():
     # replay of the same batched workload with direct launches.
     lib.bec_prof_watch(int(a.watch))
     k0 = groups[gk0][:32]
-    B.ground_state_batch([dict(cs[k], maxit=int(a.measure_iters), res_tol=0.0) for k in k0], G=G,
+    B.ground_state_batch([dict(cs[k], maxit=int(a.measure_iters), res_tol=0.0, warm_fp32=False) for k in k0], G=G,
                          psi0s=[psi0s[k] for k in k0])
     torch.cuda.synchronize()
     ms = ctypes.c_double()
