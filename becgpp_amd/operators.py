@@ -37,15 +37,22 @@ def phys_for(G, b2, b3, O, gc):
     return _lib.phys(b2, b3, O, gc if _lr_on(G, gc) else 0.0)
 
 
-def energy_components(p, G, beta2, beta3, Omega, G_C):
-    """operators.py:10-29: E and its parts as 0-d tensors, plus phi."""
+def _energy_host(t, G, beta2, beta3, Omega, G_C, want_phi):
+    """The energy components as host floats (one synchronisation), and phi
+    when asked for."""
     pl = _plan(G, G_C)
-    t = as_field(p, G)
-    phi = torch.empty(t.shape, dtype=torch.float64, device=DEV)
+    phi = torch.empty(t.shape, dtype=torch.float64, device=DEV) if want_phi else None
     comps = (ctypes.c_double * 8)()
     ph = phys_for(G, beta2, beta3, Omega, G_C)
-    _lib.check(pl.lib.bec_energy(pl.h, _ptr(t), ctypes.byref(ph), comps, _ptr(phi)))
-    out = {k: torch.tensor(comps[i], dtype=torch.float64, device=DEV) for i, k in enumerate(_COMP_KEYS)}
+    _lib.check(pl.lib.bec_energy(pl.h, _ptr(t), ctypes.byref(ph), comps, _ptr(phi) if want_phi else None))
+    return {k: comps[i] for i, k in enumerate(_COMP_KEYS)}, phi
+
+
+def energy_components(p, G, beta2, beta3, Omega, G_C):
+    """operators.py:10-29: E and its parts as 0-d tensors, plus phi."""
+    t = as_field(p, G)
+    comps, phi = _energy_host(t, G, beta2, beta3, Omega, G_C, True)
+    out = {k: torch.tensor(v, dtype=torch.float64, device=DEV) for k, v in comps.items()}
     out["phi"] = phi
     return out
 
@@ -87,8 +94,7 @@ def sensitivities(psi, G, cfg):
     t = as_field(psi, G)
     rho = (t.real ** 2 + t.imag ** 2)
     dv = dV(G)
-    c = {k: v.item() for k, v in energy_components(t, G, cfg["beta2"], cfg["beta3"], cfg["Omega"],
-                                                   cfg["G_C"]).items() if k != "phi"}
+    c, _ = _energy_host(t, G, cfg["beta2"], cfg["beta3"], cfg["Omega"], cfg["G_C"], False)
     out = dict(dE_dbeta2=0.5 * float((rho * rho).sum()) * dv,
                dE_dbeta3=float((rho * rho * rho).sum()) * dv / 3.0,
                dE_dOmega=-c["Lz"])
@@ -144,7 +150,7 @@ def observables(p, G, cfg):
     s = float(G["s"])
     dv = dV(G)
     t = as_field(p, G)
-    comp = {k: v.item() for k, v in energy_components(t, G, b2, b3, O, gc).items() if k != "phi"}
+    comp, _ = _energy_host(t, G, b2, b3, O, gc, False)
     E = comp["E"]
     msum, r2sum, rperp2, z2, peak = moments(t, G)
     mass = msum * dv

@@ -115,3 +115,22 @@ def test_run_id_is_the_reference_formula():
     assert B.run_id(dict(c, maxit=5, mode="sweep")) == a
     assert B.run_id(dict(c, beta2=1.0)) != a
     assert B.run_id(dict(c, tag="x")) == "x_" + a.split("_")[1]
+
+
+def test_build_extension_keys():
+    """Keys of this build (warm_fp32, warm_res_tol, autotune, host_loop) are
+    accepted when given, absent otherwise, and leave the run id unchanged; the
+    solver configuration carries the warm-phase tolerance only when asked."""
+    import becgpp_amd as B
+    from becgpp_amd.config import CFG_DEFAULTS, CFG_EXTENSIONS
+    from becgpp_amd.solvers import _solver_cfg
+    assert not set(CFG_EXTENSIONS) & set(CFG_DEFAULTS)
+    c = B.default_cfg()
+    assert "warm_fp32" not in c and "autotune" not in c
+    w = B.default_cfg(warm_fp32=True, warm_res_tol=3e-4, autotune=True, host_loop=False)
+    assert B.run_id(w) == B.run_id(c)
+    assert _solver_cfg(c, False).warm_res_tol == 0.0
+    assert _solver_cfg(w, False).warm_res_tol == 3e-4
+    assert _solver_cfg(B.default_cfg(warm_fp32=True), False).warm_res_tol == 1e-4
+    with pytest.raises(ValueError):
+        B.default_cfg(warm_fp64=True)
