@@ -61,29 +61,54 @@ class _Energy(torch.autograd.Function):
     """E[psi] with the adjoint as its backward.  For real E of complex psi,
     PyTorch's gradient is 2 dE/d(conj psi), and dE/d(conj psi) = H psi dV with
     the mean-field H of apply_H (the long-range term's derivative is phi psi,
-    since the kernel is symmetric), so the backward is 2 H psi dV."""
+    since the kernel is symmetric), so the backward is 2 H psi dV.
+
+    The physical parameters may be 0-d tensors that require grad; their
+    partial derivatives at fixed psi are dE/dbeta2 = 1/2 sum rho^2 dV,
+    dE/dbeta3 = 1/3 sum rho^3 dV, dE/dOmega = -L_z and dE/dG_C = Egrav / G_C
+    (the long-range energy per unit coupling; 0 without a kernel).  At a
+    converged ground state these are the Hellmann-Feynman derivatives of the
+    ground-state energy."""
 
     @staticmethod
     def forward(ctx, psi, G, beta2, beta3, Omega, G_C):
+        par = [float(x.detach()) if torch.is_tensor(x) else float(x) for x in (beta2, beta3, Omega, G_C)]
         t = as_field(psi.detach(), G)
-        comps = energy_components(t, G, beta2, beta3, Omega, G_C)
+        comps = energy_components(t, G, *par)
         ctx.save_for_backward(t, comps["phi"])
         ctx.G = G
-        ctx.cfg = dict(beta2=beta2, beta3=beta3, Omega=Omega, G_C=G_C)
+        ctx.cfg = dict(beta2=par[0], beta3=par[1], Omega=par[2], G_C=par[3])
+        ctx.comps = {k: float(comps[k]) for k in ("Lz", "Egrav")} if any(ctx.needs_input_grad[2:]) else None
         return comps["E"]
 
     @staticmethod
     def backward(ctx, gE):
         t, phi = ctx.saved_tensors
-        h = apply_H(t, ctx.G, ctx.cfg, phi=phi if _lr_on(ctx.G, ctx.cfg["G_C"]) else None)
-        return 2.0 * dV(ctx.G) * gE * h, None, None, None, None, None
+        gpsi = None
+        if ctx.needs_input_grad[0]:
+            h = apply_H(t, ctx.G, ctx.cfg, phi=phi if _lr_on(ctx.G, ctx.cfg["G_C"]) else None)
+            gpsi = 2.0 * dV(ctx.G) * gE * h
+        grads = [None, None, None, None]
+        if any(ctx.needs_input_grad[2:]):
+            dv = dV(ctx.G)
+            rho = t.real ** 2 + t.imag ** 2
+            gc = ctx.cfg["G_C"]
+            d = [0.5 * (rho * rho).sum() * dv, (rho * rho * rho).sum() * dv / 3.0,
+                 -ctx.comps["Lz"], (ctx.comps["Egrav"] / gc) if (_lr_on(ctx.G, gc) and gc != 0.0) else 0.0]
+            for i in range(4):
+                if ctx.needs_input_grad[2 + i]:
+                    grads[i] = gE * torch.as_tensor(d[i], dtype=torch.float64, device=gE.device)
+        return (gpsi, None, *grads)
 
 
 def energy_t(p, G, beta2, beta3, Omega, G_C):
     """operators.py:32-33: the energy as a 0-d tensor that autograd
-    differentiates with respect to psi (backward 2 H psi dV)."""
-    if not (torch.is_tensor(p) and p.requires_grad):
-        return energy_components(p, G, beta2, beta3, Omega, G_C)["E"]
+    differentiates with respect to psi (backward 2 H psi dV) and to any of
+    beta2, beta3, Omega, G_C given as tensors that require grad."""
+    tracked = (torch.is_tensor(p) and p.requires_grad) or any(
+        torch.is_tensor(x) and x.requires_grad for x in (beta2, beta3, Omega, G_C))
+    if not tracked:
+        return energy_components(p, G, *[float(x) for x in (beta2, beta3, Omega, G_C)])["E"]
     return _Energy.apply(p, G, beta2, beta3, Omega, G_C)
 
 

@@ -43,7 +43,8 @@ WORKLOADS = {
         cfg=dict(mode="single", dimension="2D", s=2, Omega=0.9, beta2=200.0, beta3=0.0, G_C=0.0,
                  kernel="none", L=12.0, Ngrid=384, nseeds=3, res_tol=1e-6, maxit=60000, linesearch_max=20),
         desc="2D rotating vortex lattice, Omega=0.9, beta2=200, L=12, N=384, nseeds=3, res_tol=1e-6, "
-             "linesearch_max=20"),
+             "linesearch_max=20",
+        check="grad_fd"),
     "harmonic_2d_n128": dict(
         cfg=dict(mode="single", dimension="2D", s=2, Omega=0.0, beta2=100.0, beta3=0.0, G_C=0.0,
                  kernel="none", L=16.0, Ngrid=128, nseeds=1, res_tol=1e-4, maxit=60000),
@@ -54,7 +55,8 @@ WORKLOADS = {
                  kernel="newton", L=12.0, Ngrid=128, nseeds=1, seed="gaussian", seed_noise=1e-3,
                  res_tol=1e-6, maxit=60000, linesearch_max=20),
         desc="3D Newton boson star, G_C=20, beta2=0, L=12, N=128 (conv 256^3), Gaussian seed with 1e-3 "
-             "noise, res_tol=1e-6, linesearch_max=20"),
+             "noise, res_tol=1e-6, linesearch_max=20",
+        check=("hf", "G_C")),
     # quasi-2D pancakes: K_eff(l_z=0.5); Omega x G_C = 8 points x 4 seeds = 32 solves on three grids,
     # instance i on N = (128, 192, 256)[i mod 3]
     "q2d_batch_uneven": dict(
@@ -76,7 +78,8 @@ WORKLOADS = {
                 [-200.0 + 160.0 * i / 15.0 for i in range(16)]],
         desc="2D cubic-quintic droplet sweep, no trap, beta2 = 16 values in [-200, -40], beta3=1e3, "
              "L=20, N=256, nseeds=4 (64 solves), FP32 warm phase to res 1e-4 then FP64 to res_tol=1e-6, "
-             "linesearch_max=20"),
+             "linesearch_max=20",
+        check=("hf", "beta2")),
 }
 
 # watched kernel: launch-site id -> (name, algorithmic bytes per instance of its
@@ -319,6 +322,47 @@ def main():
         _, _, pol = B.ground_state(pc, G=grids[runs[ib][3]], psi0=b0psi, verbose=False)
         out["E_polished"] = pol["E"]
         out["dE_rel_vs_polished"] = abs(b0["E"] - pol["E"]) / abs(pol["E"])
+    # the configuration's own derivative check, at its full size (after timing)
+    chk = wl.get("check")
+    if chk == "grad_fd":
+        # autograd backward (2 H psi dV) against a central difference of E on a
+        # random normalised psi, along a random direction
+        gen = torch.Generator(device="cpu").manual_seed(7)
+        shp = tuple(b0psi.shape)
+        rnd = lambda: torch.complex(torch.randn(shp, dtype=torch.float64, generator=gen),
+                                    torch.randn(shp, dtype=torch.float64, generator=gen)).cuda()
+        q = rnd()
+        q = q / (q.abs().pow(2).sum() * float(G["dx"]) ** int(G["ndim"])).sqrt()
+        v = rnd()
+        par = [float(cfg[k]) for k in ("beta2", "beta3", "Omega", "G_C")]
+        qq = q.clone().requires_grad_(True)
+        B.energy_t(qq, G, *par).backward()
+        an = float(torch.real(torch.vdot(qq.grad.reshape(-1), v.reshape(-1))))
+        eps = 1e-5
+        fd = (B.energy_components(q + eps * v, G, *par)["E"].item()
+              - B.energy_components(q - eps * v, G, *par)["E"].item()) / (2 * eps)
+        out["check"] = dict(kind="autograd backward vs central difference of E on a random normalised psi",
+                            autograd=an, finite_difference=fd, rel_diff=abs(an - fd) / max(abs(fd), 1e-300))
+    elif chk and chk[0] == "hf":
+        # Hellmann-Feynman: the autograd derivative of E in the parameter at the
+        # converged state against a central difference of converged energies
+        pname = chk[1]
+        ib = min((k for k in range(nsol) if runs[k][0] == 0), key=lambda k: Es[k])
+        c0 = dict(cs[ib], res_tol=1e-9, maxit=20000, warm_fp32=False)
+        Gb = grids[runs[ib][3]]
+        par = {k: float(c0[k]) for k in ("beta2", "beta3", "Omega", "G_C")}
+        t = torch.tensor(par[pname], dtype=torch.float64, requires_grad=True)
+        args = [t if k == pname else par[k] for k in ("beta2", "beta3", "Omega", "G_C")]
+        psi_c, _, _ = B.ground_state(c0, G=Gb, psi0=b0psi, verbose=False)
+        B.energy_t(psi_c, Gb, *args).backward()
+        h = 1e-3 * max(1.0, abs(par[pname]))
+        Ep = B.ground_state(dict(c0, **{pname: par[pname] + h}), G=Gb, psi0=psi_c, verbose=False)[2]["E"]
+        Em = B.ground_state(dict(c0, **{pname: par[pname] - h}), G=Gb, psi0=psi_c, verbose=False)[2]["E"]
+        fd = (Ep - Em) / (2 * h)
+        out["check"] = dict(kind=f"Hellmann-Feynman dE/d{pname}: autograd at the converged state vs central "
+                                 f"difference of converged energies (h = {h:g}, res_tol 1e-9)",
+                            autograd=t.grad.item(), finite_difference=fd,
+                            rel_diff=abs(t.grad.item() - fd) / max(abs(fd), 1e-300))
     if not a.no_single:
         t0 = time.perf_counter()
         sbest, si, sE = step_single()

@@ -65,3 +65,28 @@ def test_hellmann_feynman(param, kw, key):
     Em = _converged_E(B, dict(c, **{param: c[param] - h}), G, psi)["E"]
     fd = (Ep - Em) / (2 * h)
     assert abs(fd - hf) <= 1e-5 * max(abs(hf), 1e-3), (key, fd, hf)
+
+
+@pytest.mark.parametrize("kw", [dict(beta2=40.0, beta3=3.0, Omega=0.4, G_C=2.0, kernel="newton"),
+                                dict(dimension="3D", Ngrid=24, L=5, beta2=20.0, G_C=3.0, kernel="newton")])
+def test_energy_parameter_gradients(kw):
+    """Autograd derivatives of E with respect to beta2, beta3, Omega and G_C at
+    fixed psi equal central differences of E in each parameter."""
+    import becgpp_amd as B
+    c = cfg(**kw)
+    Go, Gd = O.make_grid(c), B.make_grid(c)
+    psi, _ = random_state(Go, 9)
+    psi = psi / O.norm_of(psi, Go)
+    p = torch.as_tensor(psi).cuda()
+    names = ("beta2", "beta3", "Omega", "G_C")
+    par = [torch.tensor(float(c[k]), dtype=torch.float64, requires_grad=True) for k in names]
+    E = B.energy_t(p, Gd, *par)
+    E.backward()
+    for i, k in enumerate(names):
+        h = 1e-4 * max(1.0, abs(float(c[k])))
+        vals = [float(c[n]) for n in names]
+        up, dn = list(vals), list(vals)
+        up[i] += h
+        dn[i] -= h
+        fd = (B.energy_components(p, Gd, *up)["E"].item() - B.energy_components(p, Gd, *dn)["E"].item()) / (2 * h)
+        assert abs(par[i].grad.item() - fd) <= 1e-7 * max(1.0, abs(fd)), (k, par[i].grad.item(), fd)
