@@ -65,6 +65,8 @@ SIGNATURES = {
                                         ctypes.POINTER(Phys), ctypes.c_void_p, ctypes.c_double,
                                         ctypes.c_void_p, c_double_p]),
     "bec_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double_p]),
+    "bec_tf_mu": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, c_double_p]),
     "bec_ground_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Phys), ctypes.POINTER(SolverCfg),
                                         ctypes.c_void_p, ctypes.POINTER(SolveOut), c_double_p,
                                         ctypes.c_int]),

@@ -155,6 +155,7 @@ struct Plan {
   double2* twN; double2* twM;
   LineCfg lcN, lcM;
   int lsh[2];                        // autotuned log2 factor on the lines per workgroup (contiguous, strided)
+  double* tfbuf;                     // Thomas-Fermi bisection: state {lo, hi, norm} and 1024 block rows
   double* khat[2];                   // [0] = K, [1] = W (virial) real-even spectra
   int kkind[2];                      // 0 none, 1 newton, 2 log, 3 table
   // batch workspace, sized for cap instances
@@ -266,6 +267,9 @@ void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask
 void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
                  const unsigned* fl, long long n, int nb, int f32, hipStream_t s);
 void launch_narrow(const double2* in, float2* out, long long n, hipStream_t s);
+// one Thomas-Fermi norm evaluation (+ bisection step unless at_hi) on st = {lo, hi, norm}
+void launch_tf_eval(const double* base, long long n, double* st, int at_hi, double b2, double b3, double dv,
+                    double* rows, hipStream_t s);
 void launch_widen(const float2* in, double2* out, long long n, hipStream_t s);
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
                   const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s, int f32 = 0);

@@ -1,14 +1,17 @@
 """Thomas-Fermi envelopes used by the tf and triangular seeds
 (becgpp/thomasfermi.py:21-156, fields.py:17-30).
 
-One-time set-up work on device tensors; the nonlocal (G_C != 0) fixed point
+One-time set-up work on device tensors; the chemical potential's bisection
+runs on the device (bec_tf_mu), and the nonlocal (G_C != 0) fixed point
 evaluates the long-range potential with the library's convolution.
 """
+import ctypes
 import math
 
 import torch
 
-from .grid import dV
+from . import _lib
+from .grid import _ptr, as_field, dV, plan_of
 from .interactions import long_range_phi
 
 _TF_CACHE = {}
@@ -58,22 +61,15 @@ def _invert_local(mu, Veff, b2, b3):
     return torch.zeros_like(rhs)
 
 
-def _bisect_mu_for_norm(f_rho, G, lo, hi, steps=90):
-    """Chemical potential giving unit norm by bisection (thomasfermi.py:48-62)."""
-    dv = dV(G)
-    guard = 0
-    while f_rho(hi).sum().item() * dv < 1.0:
-        hi = lo + 2.0 * (hi - lo) + 1e-9
-        guard += 1
-        if guard > 200:
-            return float("nan")
-    for _ in range(steps):
-        mid = 0.5 * (lo + hi)
-        if f_rho(mid).sum().item() * dv < 1.0:
-            lo = mid
-        else:
-            hi = mid
-    return 0.5 * (lo + hi)
+def _bisect_mu_for_norm(base, b2, b3, G, lo, hi, steps=90):
+    """Chemical potential giving unit norm by bisection (thomasfermi.py:48-62)
+    for rho = _invert_local(mu, base, b2, b3); the steps run on the device."""
+    pl = plan_of(G)
+    t = as_field(base, G, torch.float64)
+    mu = ctypes.c_double()
+    _lib.check(pl.lib.bec_tf_mu(pl.h, _ptr(t), float(b2), float(b3), float(lo), float(hi), int(steps),
+                                ctypes.byref(mu)))
+    return mu.value
 
 
 def extract_tf(cfg, G, iters=400, relax=0.4):
@@ -100,11 +96,11 @@ def extract_tf(cfg, G, iters=400, relax=0.4):
         for _ in range(int(iters)):
             phi = long_range_phi(n, G, gc)
             base = Veff + phi
-            f = lambda m, base=base: _invert_local(m, base, b2, b3)  # noqa: E731
-            mu = _bisect_mu_for_norm(f, G, base.min().item(), base.max().item() + abs(b2) + abs(b3) + 1.0)
+            mu = _bisect_mu_for_norm(base, b2, b3, G, base.min().item(),
+                                     base.max().item() + abs(b2) + abs(b3) + 1.0)
             if not math.isfinite(mu):
                 break
-            n_new = f(mu)
+            n_new = _invert_local(mu, base, b2, b3)
             ssum = n_new.sum().item() * dv
             if ssum <= 0:
                 break
@@ -132,9 +128,8 @@ def extract_tf(cfg, G, iters=400, relax=0.4):
             tf = dict(kind="local_flattop", n=n, R90=R90, R99=R99, mu=mu, rho0=rho0, R=R,
                       converged=True, final_dn=0.0)
         elif confined and (b2 > 0 or (b2 == 0 and b3 > 0)):
-            f = lambda m: _invert_local(m, Veff, b2, b3)  # noqa: E731
-            mu = _bisect_mu_for_norm(f, G, Veff.min().item(), Veff.max().item() + abs(b2) + abs(b3) + 1.0)
-            n = f(mu)
+            mu = _bisect_mu_for_norm(Veff, b2, b3, G, Veff.min().item(), Veff.max().item() + abs(b2) + abs(b3) + 1.0)
+            n = _invert_local(mu, Veff, b2, b3)
             n = n / (n.sum() * dv)
             R90, R99 = mass_quantiles(n, G, (0.90, 0.99))
             kind = "repulsive_parabola" if b3 <= 0 else "local_trapped_repulsive"

@@ -143,6 +143,15 @@ int bec_precondition(bec_plan* p, const void* g, const void* psi, const bec_phys
  * sum (x^2+y^2) rho, sum z^2 rho, max rho (host, before dV). */
 int bec_moments(bec_plan* p, const void* psi, double* out5);
 
+/* Thomas-Fermi chemical potential of unit norm (thomasfermi.py:48-62, the
+ * bisection of _bisect_mu_for_norm): rho(mu) is the upper root of
+ * b3 rho^2 + b2 rho = mu - base (clamped at 0) on the plan's grid; hi is
+ * widened while sum rho(hi) dV < 1 (at most 200 times, else *mu_out = NaN),
+ * then `steps` bisection steps run on the device with no host round trip.
+ * base: device float64 field (V_eff [+ phi]). */
+int bec_tf_mu(bec_plan* p, const double* base, double b2, double b3, double lo, double hi, int steps,
+              double* mu_out);
+
 /* ground_state loop (solvers.py:48-226): Riemannian Sobolev-preconditioned
  * CG with Polak-Ribiere beta, Barzilai-Borwein trial step and Armijo line
  * search.  psi_io holds psi0 on entry (normalised internally) and the
