@@ -6,13 +6,15 @@ solver: ``single`` (ground_state / ground_state_multiseed), ``sweep`` (every
 fresh seed of every point as one batched solve when continuation is off;
 point by point with the continuation candidate otherwise), ``validate`` (the
 analytic gate, every convolution and energy by libbecgpp_hip) and ``smoke``.
-Results are returned as the reference returns them; checkpoints, figures,
-CSV/JSON files, vortex and LLL diagnostics are not produced by this package,
-and the modes ``tf_only`` and ``convergence`` are not part of it (they raise).
+Results are returned as the reference returns them.  ``single`` resumes from
+and writes the reference's checkpoint, summary CSV and per-run JSON (records.py);
+figures, vortex and LLL diagnostics are not produced by this package, and the
+modes ``tf_only`` and ``convergence`` are not part of it (they raise).
 """
 import hashlib
 import json
 import math
+import os
 
 import numpy as np
 import torch
@@ -24,6 +26,7 @@ from .interactions import long_range_phi
 from .operators import energy_components
 from .solvers import batchable, ground_state, ground_state_batch, ground_state_multiseed
 from .thomasfermi import extract_tf
+from . import records
 
 # keys that cannot change the computed state (becgpp/io.py:9-19)
 _RID_EXCLUDE = frozenset({
@@ -61,11 +64,19 @@ def mode_single(cfg):
     batch and keeps the lowest energy."""
     rid = run_id(cfg)
     G = make_grid(cfg)
-    if int(cfg.get("nseeds", 1)) > 1:
+    psi0 = records.load_checkpoint_psi(rid)             # modes.py:216-224
+    if psi0 is not None:
+        print(f"[resume] {records.ckpt_path(rid)}")
+        psi0 = psi0.to(DEV)
+    if int(cfg.get("nseeds", 1)) > 1 and psi0 is None:
         psi, G, obs = ground_state_multiseed(cfg, G=G, verbose=True)
     else:
-        psi, G, obs = ground_state(cfg, G=G, verbose=True)
+        psi, G, obs = ground_state(cfg, G=G, psi0=psi0, verbose=True)
     d = _record(obs, G, cfg, rid)
+    if cfg.get("save_ckpt", True):                      # modes.py:234-247
+        records.save_checkpoint(rid, psi, cfg, d)
+    records.write_csv(os.path.join(records.out_dir(), "single_summary.csv"), [d])
+    records.write_run_json(rid, cfg, d, DEV)
     print(f"[single] {cfg['dimension']} s={cfg['s']} Omega={cfg['Omega']} beta2={cfg['beta2']} "
           f"beta3={cfg['beta3']} G_C={cfg['G_C']} kernel={G['kernel']}")
     print(f"         E={d['E']:.8f} mu={d['mu']:.6f} Lz={d['Lz']:.5f} virial_rel={d['virial_rel']:.2e}")

@@ -48,3 +48,19 @@ def test_sweep_with_continuation_keeps_lowest_candidate():
     assert rows[1]["branch"] in ("continuation", "seed:tf", "seed:gaussian")
     cand = [float(x.split("=")[1]) for x in rows[1]["candidate_E"].split(";")]   # printed to 10 digits
     assert abs(rows[1]["E"] - min(cand)) <= 1e-9 * abs(min(cand))
+
+
+def test_single_mode_writes_records_and_resumes(tmp_path, monkeypatch):
+    """modes.py:216-247: the checkpoint of a run is keyed by its run id; a second
+    run of the same configuration resumes from it and converges at once; the
+    summary CSV and the per-run JSON are written."""
+    import becgpp_amd as B
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
+    c = _cfg(B, mode="single", beta2=70.0, nseeds=1, seed="gaussian")
+    d1 = B.run(c)
+    rid = B.run_id(c)
+    assert (tmp_path / "ckpt" / f"{rid}.pt").is_file()
+    assert (tmp_path / "single_summary.csv").is_file() and (tmp_path / f"{rid}.json").is_file()
+    d2 = B.run(c)
+    assert d2["iters"] < d1["iters"] and d2["converged"]
+    assert abs(d2["E"] - d1["E"]) <= 1e-10 * abs(d1["E"])

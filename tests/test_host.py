@@ -134,3 +134,28 @@ def test_build_extension_keys():
     assert _solver_cfg(B.default_cfg(warm_fp32=True), False).warm_res_tol == 1e-4
     with pytest.raises(ValueError):
         B.default_cfg(warm_fp64=True)
+
+
+def test_run_records_round_trip(tmp_path, monkeypatch):
+    """records.py: checkpoint {psi, cfg, obs} under $GPP_OUTDIR/ckpt readable
+    with weights_only=True; CSV header = sorted union of keys; per-run JSON."""
+    import csv
+    import json
+    import torch
+    from becgpp_amd import records
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
+    psi = torch.randn(8, 8, dtype=torch.complex128)
+    cfg = dict(beta2=1.0, sweep_values=[1.0, 2.0], kernel="none")
+    obs = dict(E=1.5, converged=True, stop_reason="converged", Elist=torch.tensor([1.0, 2.0]))
+    records.save_checkpoint("becgpp_abc", psi, cfg, obs)
+    assert os.path.isfile(tmp_path / "ckpt" / "becgpp_abc.pt")
+    ck = torch.load(tmp_path / "ckpt" / "becgpp_abc.pt", weights_only=True)
+    assert torch.equal(ck["psi"], psi) and ck["cfg"]["sweep_values"] == [1.0, 2.0] and ck["obs"]["E"] == 1.5
+    assert torch.equal(records.load_checkpoint_psi("becgpp_abc"), psi)
+    assert records.load_checkpoint_psi("becgpp_missing") is None
+    records.write_csv(str(tmp_path / "s.csv"), [dict(b=1, a=2), dict(c=3)])
+    rows = list(csv.DictReader(open(tmp_path / "s.csv")))
+    assert list(rows[0].keys()) == ["a", "b", "c"] and rows[1]["c"] == "3"
+    records.write_run_json("becgpp_abc", cfg, obs, "cuda")
+    j = json.load(open(tmp_path / "becgpp_abc.json"))
+    assert j["cfg"]["beta2"] == 1.0 and j["diagnostics"]["Elist"] == [1.0, 2.0] and "code_version" in j
