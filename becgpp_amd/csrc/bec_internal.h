@@ -10,7 +10,7 @@
 #include <string>
 #include "bec_common.h"
 
-#define BEC_BMAX 32          // instances per launch (per-instance scalars by value)
+#define BEC_BMAX 64          // instances per launch (per-instance scalars by value; 64-bit masks)
 
 // line-pass axis modes (which axis a pass transforms, and how lines map to
 // memory).  2D: axis 1 = y (contiguous), axis 0 = x (stride N).

@@ -63,6 +63,7 @@ static void dev_release(Plan& p) {
   p.hvlog_dev = nullptr;
   if (p.cstream) (void)hipStreamDestroy(p.cstream);
   p.cstream = nullptr;
+
   p.dinst = nullptr; p.flags = nullptr; p.dtab = nullptr; p.dtrace = nullptr; p.dtrace_cap = 0;
   p.hdone = nullptr; p.hdone_dev = nullptr;
 }
@@ -81,6 +82,7 @@ static void dev_alloc(Plan& p, int nb, int trace_cap) {
     HIPCHK(hipHostGetDevicePointer((void**)&p.hvlog_dev, p.hvlog, 0));
   }
   if (!p.cstream) HIPCHK(hipStreamCreateWithFlags(&p.cstream, hipStreamNonBlocking));
+
   if (trace_cap > p.dtrace_cap) {
     if (p.dtrace) (void)hipFree(p.dtrace);
     p.dtrace = nullptr;
@@ -365,7 +367,7 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
 // initial energy and residual), and its states are widened back at the end.
 static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
                        bec_solve_out* out, double* trace, int trace_cap, bool f32) {
-  if (nb < 1 || nb > BEC_BMAX) throw BecError(1, "batch size must be in [1, 32]");
+  if (nb < 1 || nb > BEC_BMAX) throw BecError(1, "batch size must be in [1, " + std::to_string(BEC_BMAX) + "]");
   if (cf->cg_beta < 0 || cf->cg_beta > 2) throw BecError(1, "cg_beta must be 'pr', 'pr_precond' or 'none'");
   if (nb > 1) {
     // reducing line passes need whole workgroups per instance

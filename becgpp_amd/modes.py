@@ -124,7 +124,7 @@ def mode_sweep(cfg):
     """modes.py:295-387: vary one numeric key; each point keeps the lowest
     energy over its fresh seeds (and the previous point's state when
     sweep_continuation is on).  Without continuation all points' seeds are
-    independent and are solved as batches of up to 32 per grid."""
+    independent and are solved as batches of up to 64 per grid."""
     param, values = _sweep_args(cfg)
     use_cont = bool(cfg.get("sweep_continuation", True))
     nseeds = int(cfg.get("nseeds", 1))

@@ -16,7 +16,7 @@ from .interactions import ensure_kernel
 from .operators import compute_residual, observables, phys_for
 from .seeds import make_seed
 
-BATCH_MAX = 32
+BATCH_MAX = 64
 
 
 def batchable(G):
@@ -73,7 +73,7 @@ def ground_state(cfg, G=None, psi0=None, verbose=True):
 
 
 def solve_native_batch(cfgs, G, psi0s, verbose=False):
-    """Run up to 32 solves on one grid together (bec_ground_state_batch).
+    """Run up to 64 solves on one grid together (bec_ground_state_batch).
 
     cfgs share the grid, kernel and solver knobs; beta2, beta3, Omega and G_C
     may differ per instance.  Returns (list of psi, list of raw results)."""
@@ -110,7 +110,7 @@ def _finish(psi, res, G, cfg):
 def ground_state_batch(cfgs, G=None, psi0s=None, verbose=False):
     """ground_state for many configurations of one grid at once: each entry of
     the result is what ground_state(cfgs[i], G, psi0s[i]) returns, computed
-    by the batched device loop (32 instances per launch at most)."""
+    by the batched device loop (64 instances per launch at most)."""
     if G is None:
         G = make_grid(cfgs[0])
     if psi0s is None:

@@ -229,7 +229,7 @@ def main():
             Gg = grids[gk]
             lr = Gg["kernel"] != "none" and any(abs(float(cs[k]["G_C"])) > 1e-15 for k in ks)
             pl = ensure_kernel(Gg, "K") if lr else plan_of(Gg)
-            schedules["N=%d" % gk[0]] = pl.autotune(min(len(ks), 32))
+            schedules["N=%d" % gk[0]] = pl.autotune(min(len(ks), 64))
         torch.cuda.synchronize()
     warm_iters = [0]
     for _ in range(a.warmup):
@@ -251,7 +251,7 @@ def main():
     # events; the roofline kernel is timed with HIP events on its stream in a
     # replay of the same batched workload with direct launches.
     lib.bec_prof_watch(int(a.watch))
-    k0 = groups[gk0][:32]
+    k0 = groups[gk0][:64]
     B.ground_state_batch([dict(cs[k], maxit=int(a.measure_iters), res_tol=0.0, warm_fp32=False) for k in k0], G=G,
                          psi0s=[psi0s[k] for k in k0])
     torch.cuda.synchronize()

@@ -161,7 +161,7 @@ int bec_tf_mu(bec_plan* p, const double* base, double b2, double b3, double lo, 
 int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
                      bec_solve_out* out, double* trace, int trace_cap);
 
-/* Batched ground_state: nb (<= 32) independent solves on one grid, e.g. the
+/* Batched ground_state: nb (<= 64) independent solves on one grid, e.g. the
  * seeds of ground_state_multiseed (solvers.py:238-263) or the points of a
  * sweep without continuation (modes.py:295-387), advanced together so that
  * every kernel launch covers all instances still iterating.  ph[nb] gives
