@@ -122,8 +122,10 @@ def parse():
                     help="skip the single-solve comparison run")
     ap.add_argument("--no-polish", action="store_true",
                     help="skip the accuracy check of the selected state (bounded profiling runs only)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default FFT stage orders and lines per workgroup")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner choose each plan's FFT stage orders and lines per workgroup before "
+                         "timing (default: the fixed rules)")
+    ap.add_argument("--no-autotune", action="store_true", help="the default: the fixed schedule rules")
     ap.add_argument("--fp64-only", action="store_true",
                     help="run a workload's FP32 warm phase in FP64 too (comparison)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -222,7 +224,7 @@ def main():
     lib = _lib.load()
     # schedule of each grid's plan: chosen by the autotuner before any timing
     schedules = {}
-    if not a.no_autotune:
+    if a.autotune and not a.no_autotune:
         from becgpp_amd.grid import plan_of
         from becgpp_amd.interactions import ensure_kernel
         for gk, ks in groups.items():
