@@ -125,7 +125,8 @@ struct CombArgs {
 };
 
 struct SigmaStats {
-  unsigned long long kmin, kmax;
+  unsigned long long kmin, kmax;       // order keys of the bulk's min / max (this evaluation)
+  unsigned long long blo, bhi;         // value range of the histogram bins (the previous evaluation's)
   unsigned int cnt, bucket, rank_in, ncand;
 };
 
@@ -289,4 +290,4 @@ bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, double bytes);   /
 unsigned long long* prof_counter(int id);
 bool prof_active();                  // some launch site is being timed
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,
-                SITE_SIGMA_HIST = 21, SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };
+                SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };

@@ -755,22 +755,45 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Sobolev shift: lower median of the local potential over the bulk
-// {rho > 0.05 max rho} (value-range histogram, candidate gather, exact select
-// in one workgroup), per instance (grid y).  No atomics across the grid on a
-// single address except the candidate counter.
+// {rho > 0.05 max rho}, per instance (grid y), in three launches:
+//   1. the masked values, their min / max / count rows, and a histogram of
+//      SIGMA_BINS bins over the previous evaluation's value range (values
+//      outside it fall in the end bins: the bins stay ordered, which is all the
+//      selection needs);
+//   2. the rows folded, the bin holding rank (cnt-1)/2 located, its members
+//      gathered;
+//   3. exact selection among them in one workgroup; the range just found
+//      becomes the next evaluation's bin range.
+// No atomics across the grid on a single address except the candidate counter.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int sigma_bin(double v, double lo, double scale) {
+  int b = (int)((v - lo) * scale);
+  return b < 0 ? 0 : (b >= SIGMA_BINS ? SIGMA_BINS - 1 : b);
+}
+// the bins' value range: [lo, hi] with SIGMA_BINS / (hi - lo) bins per unit,
+// or a single bin when the range is empty
+__device__ __forceinline__ void sigma_range(const SigmaStats* st, double& lo, double& scale) {
+  lo = key_to_double(st->blo);
+  const double hi = key_to_double(st->bhi);
+  scale = (hi > lo) ? (double)SIGMA_BINS / (hi - lo) : 0.0;
+}
 template <class C>
 __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const C* __restrict__ psi,
                                                         const double* __restrict__ phi,
                                                         const double* mrows, int nrows, int mkp, int mq,
                                                         const InstPhys* iphys, int ndim, int N, double dx,
                                                         double tc, int s_mode, double s_half, long long n,
-                                                        Act act, double* loc, unsigned long long* part) {
+                                                        Act act, double* loc, unsigned long long* part,
+                                                        const SigmaStats* stats, unsigned* hists) {
   __shared__ unsigned long long smin[BEC_WG / 64], smax[BEC_WG / 64], scnt[BEC_WG / 64];
   __shared__ double smx[BEC_WG / 64];
+  __shared__ unsigned h[SIGMA_BINS];
   const int b = blockIdx.y;
   if (!act_on(act, b)) return;
   const long long off = (long long)b * n;
+  for (int k = threadIdx.x; k < SIGMA_BINS; k += blockDim.x) h[k] = 0u;
+  double blo, bscale;
+  sigma_range(stats + b, blo, bscale);
   // max rho of the state: fold the per-block maxima of the pass that computed it
   double rm = 0.0;
   const double* mr = mrows + (long long)b * nrows * mkp;
@@ -800,6 +823,7 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const C* __restrict__ ps
       kmn = k < kmn ? k : kmn;
       kmx = k > kmx ? k : kmx;
       cnt++;
+      atomicAdd(&h[sigma_bin(lv, blo, bscale)], 1u);
     }
     loc[off + i] = val;
   }
@@ -823,11 +847,10 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const C* __restrict__ ps
     pr[1] = kmx;
     pr[2] = cnt;
   }
-}
-
-__device__ __forceinline__ int sigma_bin(double v, double lo, double scale) {
-  int b = (int)((v - lo) * scale);
-  return b < 0 ? 0 : (b >= SIGMA_BINS ? SIGMA_BINS - 1 : b);
+  __syncthreads();
+  unsigned* hist = hists + (long long)b * SIGMA_BINS;
+  for (int k = threadIdx.x; k < SIGMA_BINS; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
 }
 
 // fold the local pass's per-block (min, max, count) rows of one instance;
@@ -862,33 +885,10 @@ __device__ __forceinline__ void sigma_fold(const unsigned long long* part, int n
   __syncthreads();
 }
 
-__global__ __launch_bounds__(BEC_WG) void k_sigma_hist(const double* __restrict__ loc, long long n,
-                                                       const unsigned long long* part, int nb1,
-                                                       SigmaStats* stats, unsigned* hists, Act act) {
-  __shared__ unsigned h[SIGMA_BINS];
-  const int b = blockIdx.y;
-  if (!act_on(act, b)) return;
-  const long long off = (long long)b * n;
-  unsigned* hist = hists + (long long)b * SIGMA_BINS;
-  unsigned long long kmn, kmx, cnt;
-  sigma_fold(part + (long long)b * nb1 * 4, nb1, stats + b, kmn, kmx, cnt);
-  const double lo = key_to_double(kmn), hi = key_to_double(kmx);
-  if (!(cnt > 0 && hi > lo)) return;
-  for (int k = threadIdx.x; k < SIGMA_BINS; k += blockDim.x) h[k] = 0u;
-  __syncthreads();
-  const double scale = (double)SIGMA_BINS / (hi - lo);
-  ELEM_LOOP(n) {
-    const double v = loc[off + i];
-    if (v == v) atomicAdd(&h[sigma_bin(v, lo, scale)], 1u);
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < SIGMA_BINS; k += blockDim.x)
-    if (h[k]) atomicAdd(&hist[k], h[k]);
-}
-
 // every block locates the bin holding the rank-k element, k = (cnt-1)/2
 // (parallel prefix over the histogram), then gathers that bin's members
 __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restrict__ loc, long long n,
+                                                         const unsigned long long* part, int nb1,
                                                          SigmaStats* stats, const unsigned* hists, double* cands,
                                                          Act act) {
   __shared__ unsigned pre[BEC_WG];
@@ -900,9 +900,12 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
   const unsigned* hist = hists + (long long)b * SIGMA_BINS;
   double* cand = cands + (long long)b * n;
   const long long off = (long long)b * n;
-  const double lo = key_to_double(st->kmin), hi = key_to_double(st->kmax);
-  const unsigned cnt = st->cnt;
-  if (!(cnt > 0 && hi > lo)) return;
+  unsigned long long kmn, kmx, cnt64;
+  sigma_fold(part + (long long)b * nb1 * 4, nb1, st, kmn, kmx, cnt64);
+  const unsigned cnt = (unsigned)cnt64;
+  if (!(cnt > 0 && key_to_double(kmx) > key_to_double(kmn))) return;
+  double lo, scale;
+  sigma_range(st, lo, scale);
   const unsigned k = (cnt - 1u) / 2u;
   constexpr int PER = SIGMA_BINS / BEC_WG;
   unsigned own = 0;
@@ -933,7 +936,6 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
     st->rank_in = srank;
   }
   const int bin = sbin;
-  const double scale = (double)SIGMA_BINS / (hi - lo);
   ELEM_LOOP(n) {
     const double v = loc[off + i];
     if (v == v && sigma_bin(v, lo, scale) == bin) cand[atomicAdd(&st->ncand, 1u)] = v;
@@ -1020,6 +1022,8 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* stats, unsi
     sg = fmax(sg, floor_);
     sigmas[b] = sg;
     if (hred) hred[(long long)b * RSTRIDE + R_SIGMA] = sg;
+    st->blo = st->kmin;                // the next evaluation bins over this range
+    st->bhi = st->kmax;
     st->kmin = ~0ull;
     st->kmax = 0ull;
     st->cnt = 0u;
@@ -1036,6 +1040,8 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_reset(SigmaStats* stats, unsig
     SigmaStats* st = stats + b;
     st->kmin = ~0ull;
     st->kmax = 0ull;
+    st->blo = 0ull;                    // no range yet: one bin
+    st->bhi = 0ull;
     st->cnt = 0u;
     st->bucket = 0u;
     st->rank_in = 0u;
@@ -1452,17 +1458,16 @@ void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const doub
   prof_pre(SITE_SIGMA_LOCAL, s);
   if (f32)
     hipLaunchKernelGGL(k_sigma_local<float2>, dim3((int)g1, nb), dim3(BEC_WG), 0, s, (const float2*)psi, phi, mrows,
-                       nrows, mkp, mq, p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part);
+                       nrows, mkp, mq, p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part,
+                       p.stats, p.hist);
   else
     hipLaunchKernelGGL(k_sigma_local<double2>, dim3((int)g1, nb), dim3(BEC_WG), 0, s, psi, phi, mrows, nrows, mkp,
-                       mq, p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part);
+                       mq, p.iphys, p.ndim, p.N, p.dx, p.tc, p.s_mode, p.s_half, p.n, act, p.loc, part, p.stats,
+                       p.hist);
   prof_post(SITE_SIGMA_LOCAL, s);
-  prof_pre(SITE_SIGMA_HIST, s);
-  hipLaunchKernelGGL(k_sigma_hist, dim3(g, nb), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,
-                     (int)g1, p.stats, p.hist, act);
-  prof_post(SITE_SIGMA_HIST, s);
   prof_pre(SITE_SIGMA_GATHER, s);
-  hipLaunchKernelGGL(k_sigma_gather, dim3(g, nb), dim3(BEC_WG), 0, s, p.loc, p.n, p.stats, p.hist, p.cand, act);
+  hipLaunchKernelGGL(k_sigma_gather, dim3(g, nb), dim3(BEC_WG), 0, s, p.loc, p.n, (const unsigned long long*)part,
+                     (int)g1, p.stats, p.hist, p.cand, act);
   prof_post(SITE_SIGMA_GATHER, s);
   prof_pre(SITE_SIGMA_SELECT, s);
   hipLaunchKernelGGL(k_sigma_select, dim3(nb), dim3(BEC_WG), 0, s, p.stats, p.hist, p.cand, p.n, floor_, p.sigma,
