@@ -65,6 +65,11 @@ SIGNATURES = {
                                         ctypes.POINTER(Phys), ctypes.c_void_p, ctypes.c_double,
                                         ctypes.c_void_p, c_double_p]),
     "bec_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double_p]),
+    "bec_lll_weight": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, c_double_p,
+                                      ctypes.POINTER(ctypes.c_int)]),
+    "bec_vortex_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_int)]),
     "bec_tf_mu": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, c_double_p]),
     "bec_ground_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Phys), ctypes.POINTER(SolverCfg),

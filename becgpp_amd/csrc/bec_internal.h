@@ -23,7 +23,7 @@ enum KinKind { KE_FIRST = 0, KE_REST = 1, KH_FIRST = 2, KH_MID = 3, KH_LAST = 4 
 // generic line-FFT pass options
 enum LoadKind { LD_C = 0, LD_R = 1, LD_HERM = 2 };
 enum DirKind { DIR_FWD = 0, DIR_INV = 1, DIR_FMI = 2 };
-enum MulKind { MUL_NONE = 0, MUL_KHAT2 = 1, MUL_KHAT3 = 2, MUL_PRE2 = 3, MUL_PRE3 = 4 };
+enum MulKind { MUL_NONE = 0, MUL_KHAT2 = 1, MUL_KHAT3 = 2, MUL_PRE2 = 3, MUL_PRE3 = 4, MUL_GAUSS = 5 };
 enum StoreKind { ST_C = 0, ST_RE = 1, ST_CDOT = 2 };   // ST_CDOT: complex store + the 16 CG inner products
 
 struct LineCfg {       // threads per line / lines per workgroup for a length
@@ -77,6 +77,7 @@ struct PassArgs {
   void* out;
   const double* khat; int H, M;        // real-even kernel spectrum, half table [H]^d
   const double* sigma; int Nk; double invNdx;   // per-instance Sobolev shifts and grid frequencies
+  double gl2;                          // MUL_GAUSS: the line's factor exp(-gl2 k^2)
   const double* dot;                   // optional: rows of sum(out_re * dot) (dot laid out like out)
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
@@ -157,6 +158,7 @@ struct Plan {
   LineCfg lcN, lcM;
   int lsh[2];                        // autotuned log2 factor on the lines per workgroup (contiguous, strided)
   double* tfbuf;                     // Thomas-Fermi bisection: state {lo, hi, norm} and 1024 block rows
+  double* dgbuf;                     // diagnostics: 1024 block rows of 16 values
   double* khat[2];                   // [0] = K, [1] = W (virial) real-even spectra
   int kkind[2];                      // 0 none, 1 newton, 2 log, 3 table
   // batch workspace, sized for cap instances
@@ -268,6 +270,13 @@ void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask
 void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
                  const unsigned* fl, long long n, int nb, int f32, hipStream_t s);
 void launch_narrow(const double2* in, float2* out, long long n, hipStream_t s);
+// diagnostics: LLL projections of a chunk of m (rows of 3 sums per m), phases
+// and plaquette windings of the vortex count
+int launch_lll(const double2* psi, int N, double dx, double sgn, int m0, int mcnt, double* rows, hipStream_t s);
+int launch_vortex(const double2* psi, const double* nbar, int N, double dx, double xcm, double ycm, double R,
+                  double thr, double* rows, hipStream_t s);
+int launch_nbar(const double2* f, long long n, double* nbar, double* rows, hipStream_t s);
+void launch_abs2c(const double2* psi, long long n, double2* out, hipStream_t s);
 // one Thomas-Fermi norm evaluation (+ bisection step unless at_hi) on st = {lo, hi, norm}
 void launch_tf_eval(const double* base, long long n, double* st, int at_hi, double b2, double b3, double dv,
                     double* rows, hipStream_t s);

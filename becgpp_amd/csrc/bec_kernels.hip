@@ -538,6 +538,14 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
 #pragma unroll
         for (int j = 0; j < BEC_EPT; ++j) v[j] = cscale(v[j], kh[j]);
+      } else if (MUL == MUL_GAUSS) {
+#pragma unroll
+        for (int j = 0; j < BEC_EPT; ++j) {
+          const int k = lane + j * tpl;
+          if (k >= L) continue;
+          const double kx = kfreq(k, a.Nk, a.invNdx);
+          v[j] = cscale(v[j], exp(-a.gl2 * kx * kx));
+        }
       } else {
         // denominator 1/2 (kx^2 + ky^2 [+ kz^2]) + sigma, summed in that order
         const double ky = kfreq(MUL == MUL_PRE2 ? oli : oli / a.Nk, a.Nk, a.invNdx);
@@ -1262,6 +1270,9 @@ int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a,
   // preconditioner middle pass
   BEC_P(0, LD_C, DIR_FMI, MUL_PRE2, ST_C)
   BEC_P(0, LD_C, DIR_FMI, MUL_PRE3, ST_C)
+  // Gaussian smoothing (vortex diagnostic)
+  BEC_P(1, LD_C, DIR_FMI, MUL_GAUSS, ST_C)
+  BEC_P(0, LD_C, DIR_FMI, MUL_GAUSS, ST_C)
   // convolution
   BEC_P(1, LD_R, DIR_FWD, MUL_NONE, ST_C)
   BEC_P(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C)
@@ -1342,6 +1353,113 @@ void launch_post(const double2* psi, double2* best, double2* d, const double2* z
   else
     hipLaunchKernelGGL(k_post<double2>, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, best, d, z, tab, fl, n);
 }
+// ---------------------------------------------------------------------------
+// Diagnostics of a 2D state (vortex.py:10-84)
+// ---------------------------------------------------------------------------
+// Lowest-Landau-level projections for m = m0 .. m0+mcnt-1 (mcnt <= 5): per
+// block the sums |q_m|^2 and conj(q_m) psi (re, im), q_m = r^m exp(-r^2/2)
+// exp(i m theta) / sqrt(pi m!), theta = sgn atan2(y, x); log |q_m| advances
+// by log r - log(m)/2 from one m to the next, the phase by e^{i theta}.
+__global__ __launch_bounds__(BEC_WG) void k_lll(const double2* __restrict__ psi, int N, double dx, double sgn,
+                                                int m0, int mcnt, double* rows) {
+  double v[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) v[q] = 0.0;
+  const long long n = (long long)N * N;
+  const double lpi = 1.1447298858494002;           // log(pi)
+  ELEM_LOOP(n) {
+    const int ix = (int)(i / N), iy = (int)(i % N);
+    const double x = xcoord(ix, N, dx), y = sgn * xcoord(iy, N, dx);
+    const double r2 = x * x + y * y, r = sqrt(r2);
+    const double lr = log(fmax(r, 1e-300));
+    double lm = (double)m0 * lr - 0.5 * r2 - 0.5 * (lpi + lgamma((double)m0 + 1.0));
+    const double th = atan2(y, x);
+    double2 e = make_double2(cos((double)m0 * th), sin((double)m0 * th));
+    const double2 u = make_double2(cos(th), sin(th));
+    const double2 p = psi[i];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (q < mcnt) {
+        const double mag = exp(lm);
+        const double2 qm = make_double2(mag * e.x, mag * e.y);
+        v[3 * q] += mag * mag;
+        v[3 * q + 1] += qm.x * p.x + qm.y * p.y;     // Re conj(q) psi
+        v[3 * q + 2] += qm.x * p.y - qm.y * p.x;     // Im conj(q) psi
+        const int m = m0 + q + 1;
+        lm += lr - 0.5 * log((double)m);
+        e = cmul(e, u);
+      }
+    }
+  }
+  block_partials<15, 0>(v, nullptr, rows, nullptr, blockIdx.x);
+}
+int launch_lll(const double2* psi, int N, double dx, double sgn, int m0, int mcnt, double* rows, hipStream_t s) {
+  const int g = elem_blocks((long long)N * N);
+  hipLaunchKernelGGL(k_lll, dim3(g), dim3(BEC_WG), 0, s, psi, N, dx, sgn, m0, mcnt, rows);
+  return g;
+}
+// wrap an angle difference into [-pi, pi): (d + pi) mod 2 pi - pi, the modulus
+// taking the sign of the divisor
+__device__ __forceinline__ double wrap_pi(double d) {
+  const double tp = 6.283185307179586;
+  double r = fmod(d + 3.141592653589793, tp);
+  if (r < 0.0) r += tp;
+  return r - 3.141592653589793;
+}
+// plaquettes (ix, iy), ix, iy < N-1: the winding of the phase around corners
+// (ix,iy) -> (ix+1,iy) -> (ix+1,iy+1) -> (ix,iy+1) -> back, counted when the
+// corner-averaged smoothed density exceeds thr and corner (ix,iy) lies within
+// R of (xcm, ycm); rows of (number of windings >= 1, number <= -1) per block
+__global__ __launch_bounds__(BEC_WG) void k_vortex(const double2* __restrict__ psi,
+                                                   const double* __restrict__ nbar, int N, double dx,
+                                                   double xcm, double ycm, double R, double thr, double* rows) {
+  double v[2] = {0.0, 0.0};
+  const long long np = (long long)(N - 1) * (N - 1);
+  ELEM_LOOP(np) {
+    const int ix = (int)(i / (N - 1)), iy = (int)(i % (N - 1));
+    const long long a0 = (long long)ix * N + iy, a1 = a0 + N, a2 = a1 + 1, a3 = a0 + 1;
+    const double cell = 0.25 * (nbar[a0] + nbar[a1] + nbar[a2] + nbar[a3]);
+    const double xx = xcoord(ix, N, dx) - xcm, yy = xcoord(iy, N, dx) - ycm;
+    if (!(cell > thr) || !(sqrt(xx * xx + yy * yy) < R)) continue;
+    const double t0 = atan2(psi[a0].y, psi[a0].x), t1 = atan2(psi[a1].y, psi[a1].x);
+    const double t2 = atan2(psi[a2].y, psi[a2].x), t3 = atan2(psi[a3].y, psi[a3].x);
+    const double tot = wrap_pi(t1 - t0) + wrap_pi(t2 - t1) + wrap_pi(t3 - t2) + wrap_pi(t0 - t3);
+    const double w = rint(tot / 6.283185307179586);   // round half to even
+    if (w >= 1.0) v[0] += 1.0;
+    if (w <= -1.0) v[1] += 1.0;
+  }
+  block_partials<2, 0>(v, nullptr, rows, nullptr, blockIdx.x);
+}
+int launch_vortex(const double2* psi, const double* nbar, int N, double dx, double xcm, double ycm, double R,
+                  double thr, double* rows, hipStream_t s) {
+  const int g = elem_blocks((long long)(N - 1) * (N - 1));
+  hipLaunchKernelGGL(k_vortex, dim3(g), dim3(BEC_WG), 0, s, psi, nbar, N, dx, xcm, ycm, R, thr, rows);
+  return g;
+}
+// nbar = max(Re f, 0) (the smoothed density) and its per-block maxima
+__global__ __launch_bounds__(BEC_WG) void k_nbar(const double2* __restrict__ f, long long n, double* nbar,
+                                                 double* rows) {
+  double m[1] = {0.0};
+  ELEM_LOOP(n) {
+    const double v = fmax(f[i].x, 0.0);
+    nbar[i] = v;
+    m[0] = fmax(m[0], v);
+  }
+  block_partials<0, 1>(nullptr, m, rows, nullptr, blockIdx.x);
+}
+int launch_nbar(const double2* f, long long n, double* nbar, double* rows, hipStream_t s) {
+  const int g = elem_blocks(n);
+  hipLaunchKernelGGL(k_nbar, dim3(g), dim3(BEC_WG), 0, s, f, n, nbar, rows);
+  return g;
+}
+// |psi|^2 as a complex field (its smoothing input)
+__global__ __launch_bounds__(BEC_WG) void k_abs2c(const double2* __restrict__ psi, long long n, double2* out) {
+  ELEM_LOOP(n) out[i] = make_double2(cabs2(psi[i]), 0.0);
+}
+void launch_abs2c(const double2* psi, long long n, double2* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_abs2c, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, psi, n, out);
+}
+
 // ---------------------------------------------------------------------------
 // Thomas-Fermi chemical potential by bisection on the device
 // (thomasfermi.py:33-62): rho(mu) = upper root of b3 rho^2 + b2 rho = mu - base,

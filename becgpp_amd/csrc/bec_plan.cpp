@@ -548,7 +548,7 @@ int run_precond(Plan& p, const Batch& B, const double2* g, double2* out, const D
 static void plan_free(Plan& p) {
   solver_release(p);
   free_batch(p);
-  void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1], p.tfbuf};
+  void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1], p.tfbuf, p.dgbuf};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
 }
@@ -938,6 +938,100 @@ extern "C" int bec_tf_mu(bec_plan* h, const double* base, double b2, double b3, 
     HIPCHK(hipMemcpyAsync(hs, st, sizeof(double) * 2, hipMemcpyDeviceToHost, p.stream));
     HIPCHK(hipStreamSynchronize(p.stream));
     *mu_out = 0.5 * (hs[0] + hs[1]);
+  });
+}
+
+// rows of a diagnostics launch, copied to the host (the caller folds them in order)
+static std::vector<double> diag_rows(Plan& p, int nblk, int kp) {
+  std::vector<double> r((size_t)nblk * kp);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(r.data(), p.dgbuf, sizeof(double) * r.size(), hipMemcpyDeviceToHost, p.stream));
+  HIPCHK(hipStreamSynchronize(p.stream));
+  return r;
+}
+static void diag_alloc(Plan& p) {
+  if (!p.dgbuf) HIPCHK(hipMalloc(&p.dgbuf, sizeof(double) * 1024 * 16));
+}
+
+// Weight of psi in the lowest Landau level of the rotating harmonic trap
+// (vortex.py:61-84): sum over m <= Mcap of |<q_m, psi>|^2 / (<q_m,q_m> <psi,psi>),
+// q_m = z^m exp(-r^2/2) (conj(z)^m for Omega < 0), Mcap from <r^2> and L.
+extern "C" int bec_lll_weight(bec_plan* h, const void* psi, double Omega, int Mmax, double* w_out, int* mcap_out) {
+  BEC_TRY({
+    if (!h || !psi || !w_out) bad("null argument");
+    Plan& p = h->p;
+    if (p.ndim != 2) bad("the LLL weight is two-dimensional only");
+    diag_alloc(p);
+    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream),
+            4, 1, R_MOM, one_instance());
+    HIPCHK(hipGetLastError());
+    plan_sync(p);
+    const double* mom = hres(p, 0) + R_MOM;
+    const double nrm2 = mom[0];
+    const double r2mean = nrm2 > 0.0 ? mom[1] / nrm2 : 0.0;
+    const int est = (int)ceil(r2mean + 10.0 * sqrt(std::max(r2mean, 1.0)) + 30.0);
+    const int cap2 = std::max(Mmax, (int)((0.80 * p.Lbox) * (0.80 * p.Lbox)));
+    const int Mcap = std::min(std::max(Mmax, est), cap2);
+    const double sgn = Omega >= 0.0 ? 1.0 : -1.0;
+    double w = 0.0;
+    for (int m0 = 0; m0 <= Mcap; m0 += 5) {
+      const int mcnt = std::min(5, Mcap + 1 - m0);
+      const int g = launch_lll((const double2*)psi, p.N, p.dx, sgn, m0, mcnt, p.dgbuf, p.stream);
+      const std::vector<double> r = diag_rows(p, g, 16);
+      for (int q = 0; q < mcnt; ++q) {
+        double nq = 0.0, re = 0.0, im = 0.0;
+        for (int b = 0; b < g; ++b) {
+          nq += r[(size_t)b * 16 + 3 * q];
+          re += r[(size_t)b * 16 + 3 * q + 1];
+          im += r[(size_t)b * 16 + 3 * q + 2];
+        }
+        if (nq > 0.0 && nrm2 > 0.0) w += (re * re + im * im) / (nq * nrm2);
+      }
+    }
+    *w_out = std::min(std::max(w, 0.0), 1.0);
+    if (mcap_out) *mcap_out = Mcap;
+  });
+}
+
+// Vortex count of a 2D state (vortex.py:10-48): the density smoothed with
+// exp(-ell^2 k^2 / 2) (two line passes), then the plaquettes' phase windings
+// where the corner-averaged smoothed density exceeds frac * its maximum and
+// the corner lies within R of (xcm, ycm).  out4 = {Nv, N+, N-, Nnet}.
+extern "C" int bec_vortex_count(bec_plan* h, const void* psi, double xcm, double ycm, double R, double ell,
+                                double frac, int* out4) {
+  BEC_TRY({
+    if (!h || !psi || !out4) bad("null argument");
+    Plan& p = h->p;
+    if (p.ndim != 2) bad("the vortex count takes a two-dimensional state (a 3D state's midplane)");
+    diag_alloc(p);
+    const int N = p.N;
+    const long long n = p.n;
+    const Batch B = one_instance();
+    launch_abs2c((const double2*)psi, n, p.w1, p.stream);
+    PassArgs a = pass_base(p, p.fpN, p.lcN, B);        // x: lines along ix (stride N)
+    a.nl = N; a.ni = N; a.in_si = 1; a.in_s = N; a.n_load = N; a.out_si = 1; a.out_s = N; a.n_store = N;
+    a.in = p.w1; a.out = p.w1; a.gl2 = 0.5 * ell * ell; a.out_scale = 1.0 / N;
+    launch_pass(0, LD_C, DIR_FMI, MUL_GAUSS, ST_C, a, p.stream);
+    PassArgs c = pass_base(p, p.fpN, p.lcN, B);        // y: contiguous lines
+    c.nl = N; c.in_so = N; c.in_s = 1; c.n_load = N; c.out_so = N; c.out_s = 1; c.n_store = N;
+    c.in = p.w1; c.out = p.w1; c.gl2 = 0.5 * ell * ell; c.out_scale = 1.0 / N;
+    launch_pass(1, LD_C, DIR_FMI, MUL_GAUSS, ST_C, c, p.stream);
+    const int g = launch_nbar(p.w1, n, p.rho, p.dgbuf, p.stream);
+    std::vector<double> r = diag_rows(p, g, 1);
+    double mx = 0.0;
+    for (int b = 0; b < g; ++b) mx = std::max(mx, r[b]);
+    const int gv = launch_vortex((const double2*)psi, p.rho, N, p.dx, xcm, ycm, R, frac * mx, p.dgbuf, p.stream);
+    r = diag_rows(p, gv, 2);
+    double np_ = 0.0, nm = 0.0;
+    for (int b = 0; b < gv; ++b) {
+      np_ += r[(size_t)b * 2];
+      nm += r[(size_t)b * 2 + 1];
+    }
+    const int ip = (int)llround(np_), im = (int)llround(nm);
+    out4[0] = ip + im;
+    out4[1] = ip;
+    out4[2] = im;
+    out4[3] = ip - im;
   });
 }
 

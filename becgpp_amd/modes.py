@@ -26,6 +26,7 @@ from .interactions import long_range_phi
 from .operators import energy_components
 from .solvers import batchable, ground_state, ground_state_batch, ground_state_multiseed
 from .thomasfermi import extract_tf
+from .diagnostics import diagnostics
 from . import records
 
 # keys that cannot change the computed state (becgpp/io.py:9-19)
@@ -46,8 +47,11 @@ def run_id(cfg):
     return f"{cfg.get('tag', 'becgpp')}_{hashlib.md5(raw.encode()).hexdigest()[:10]}"
 
 
-def _record(obs, G, cfg, rid):
+def _record(psi, obs, G, cfg, rid):
+    """The run's record: the solver's observables and the diagnostic bundle of
+    the state (diagnostics.py:18-47), then the run's identity (modes.py:227-236)."""
     d = {k: v for k, v in obs.items() if k != "trace"}
+    d.update(diagnostics(psi, G, cfg))
     d.update(rid=rid, seed_used=obs.get("seed_used", cfg.get("seed")),
              seed_energy_spread=obs.get("seed_energy_spread", 0.0),
              dimension=cfg["dimension"], kernel=G["kernel"], L=cfg["L"], N=cfg["Ngrid"], s=cfg["s"],
@@ -72,14 +76,16 @@ def mode_single(cfg):
         psi, G, obs = ground_state_multiseed(cfg, G=G, verbose=True)
     else:
         psi, G, obs = ground_state(cfg, G=G, psi0=psi0, verbose=True)
-    d = _record(obs, G, cfg, rid)
+    d = _record(psi, obs, G, cfg, rid)
     if cfg.get("save_ckpt", True):                      # modes.py:234-247
         records.save_checkpoint(rid, psi, cfg, d)
     records.write_csv(os.path.join(records.out_dir(), "single_summary.csv"), [d])
     records.write_run_json(rid, cfg, d, DEV)
     print(f"[single] {cfg['dimension']} s={cfg['s']} Omega={cfg['Omega']} beta2={cfg['beta2']} "
           f"beta3={cfg['beta3']} G_C={cfg['G_C']} kernel={G['kernel']}")
-    print(f"         E={d['E']:.8f} mu={d['mu']:.6f} Lz={d['Lz']:.5f} virial_rel={d['virial_rel']:.2e}")
+    print(f"         E={d['E']:.8f} mu={d['mu']:.6f} Lz={d['Lz']:.5f} Nv={d['Nv']}")
+    print(f"         R90={d['R90']:.4f} TF({d['tf_kind']}) R90={d['tf_R90']:.4f} "
+          f"virial_rel={d['virial_rel']:.2e} w_LLL={d['w_LLL']}")
     print(f"         resid={d['resid_rel']:.2e} converged={d['converged']} it={d['iters']}")
     d["psi"] = psi
     return d
@@ -177,7 +183,7 @@ def mode_sweep(cfg):
 def _pick(cands, c, param, val):
     cands = sorted(cands, key=lambda t: t[1][2]["E"])      # variational: lowest energy wins
     branch, (psi, G, obs) = cands[0]
-    d = _record(obs, G, c, run_id(c))
+    d = _record(psi, obs, G, c, run_id(c))
     d.update({param: val}, branch=branch, walltime=sum(t[1][2]["walltime"] for t in cands),
              candidate_E=";".join(f"{t[0]}={t[1][2]['E']:.10g}" for t in cands), psi=psi)
     print(f"    {param}={val} [{branch}] E={d['E']:.8f} mu={d['mu']:.5f} Lz={d['Lz']:.4f} "

@@ -143,6 +143,20 @@ int bec_precondition(bec_plan* p, const void* g, const void* psi, const bec_phys
  * sum (x^2+y^2) rho, sum z^2 rho, max rho (host, before dV). */
 int bec_moments(bec_plan* p, const void* psi, double* out5);
 
+/* Weight of a 2D state in the lowest Landau level (vortex.py:61-84, lll_weight):
+ * sum over m = 0..Mcap of |<q_m,psi>|^2 / (<q_m,q_m><psi,psi>) with
+ * q_m = r^m exp(-r^2/2) exp(i m theta) / sqrt(pi m!), theta = sign(Omega)
+ * atan2(y, x), Mcap = min(max(Mmax, ceil(<r^2> + 10 sqrt(max(<r^2>,1)) + 30)),
+ * max(Mmax, floor((0.8 L)^2))); clamped to [0, 1].  mcap_out (nullable). */
+int bec_lll_weight(bec_plan* p, const void* psi, double Omega, int Mmax, double* w_out, int* mcap_out);
+
+/* Vortex count of a 2D state (vortex.py:10-48, _count_vortices_2d): the phase
+ * winding of every grid plaquette, counted where the plaquette's mean density
+ * smoothed by exp(-ell^2 k^2 / 2) exceeds frac times its maximum and its first
+ * corner lies within R of (xcm, ycm).  out4 = {Nv, Nplus, Nminus, Nnet}. */
+int bec_vortex_count(bec_plan* p, const void* psi, double xcm, double ycm, double R, double ell, double frac,
+                     int* out4);
+
 /* Thomas-Fermi chemical potential of unit norm (thomasfermi.py:48-62, the
  * bisection of _bisect_mu_for_norm): rho(mu) is the upper root of
  * b3 rho^2 + b2 rho = mu - base (clamped at 0) on the plan's grid; hi is

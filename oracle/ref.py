@@ -600,6 +600,89 @@ def mass_quantiles(rho, G, fractions=(0.50, 0.90, 0.99), center=None):
     return out
 
 
+# ----------------------------------------------------------------------------
+# Diagnostics of a state  (becgpp/vortex.py:10-84, fields.py:40-67)
+# ----------------------------------------------------------------------------
+def _wrap_pi(d):
+    """(d + pi) mod 2 pi - pi, the modulus with the sign of the divisor."""
+    return np.mod(d + math.pi, 2.0 * math.pi) - math.pi
+
+
+def vortex_count_2d(psi, G, Omega=1.0, frac=0.15):
+    """vortex.py:10-48: phase windings of the grid plaquettes inside the
+    droplet (99% mass radius about the centre of mass) where the plaquette's
+    mean density, smoothed by exp(-ell^2 K2/2), exceeds frac of its maximum."""
+    rho = np.abs(psi) ** 2
+    dx = float(G["dx"])
+    X, Y = G["coords"][0], G["coords"][1]
+    if float(rho.sum()) * dx * dx <= 0:
+        return dict(Nv=0, Nplus=0, Nminus=0, Nnet=0)
+    xcm, ycm = center_of_mass(rho, G)
+    R = mass_quantiles(rho, G, (0.99,), center=(xcm, ycm))[0]
+    if not np.isfinite(R) or R <= 0:
+        return dict(Nv=0, Nplus=0, Nminus=0, Nnet=0)
+    th = np.angle(psi)
+    d1 = _wrap_pi(th[1:, :-1] - th[:-1, :-1])
+    d2 = _wrap_pi(th[1:, 1:] - th[1:, :-1])
+    d3 = _wrap_pi(th[:-1, 1:] - th[1:, 1:])
+    d4 = _wrap_pi(th[:-1, :-1] - th[:-1, 1:])
+    wind = np.round((d1 + d2 + d3 + d4) / (2.0 * math.pi))
+    ell = min(2.0 * math.sqrt(math.pi / max(abs(Omega), 1e-9)), max(R / 3.0, 2.0 * dx))
+    nbar = np.maximum(np.real(sfft.ifftn(sfft.fftn(rho) * np.exp(-0.5 * ell ** 2 * G["K2"]))), 0.0)
+    cell = 0.25 * (nbar[:-1, :-1] + nbar[1:, :-1] + nbar[1:, 1:] + nbar[:-1, 1:])
+    rc = np.sqrt((X[:-1, :-1] - xcm) ** 2 + (Y[:-1, :-1] - ycm) ** 2)
+    mask = (cell > frac * nbar.max()) & (rc < R)
+    npl, nmi = int(((wind >= 1) & mask).sum()), int(((wind <= -1) & mask).sum())
+    return dict(Nv=npl + nmi, Nplus=npl, Nminus=nmi, Nnet=npl - nmi)
+
+
+def droplet_radius(rho, G, frac=1e-2):
+    """fields.py:40-67: the outermost centre of the bins of [0, L] (bins =
+    int(min(400, max(20, L / (2 dx))))) whose mean rho about the centre of
+    mass exceeds frac of the largest bin mean."""
+    c = center_of_mass(rho, G)
+    r = np.sqrt(sum((x - c[i]) ** 2 for i, x in enumerate(G["coords"]))).ravel()
+    rmax = float(G["L"])
+    bins = int(min(400, max(20, rmax / (2.0 * float(G["dx"])))))
+    edges = np.linspace(0.0, rmax, bins + 1)
+    ctr = 0.5 * (edges[1:] + edges[:-1])
+    idx = np.clip(np.digitize(r, edges) - 1, 0, bins - 1)
+    f = np.asarray(rho).ravel()
+    prof = np.full(bins, np.nan)
+    for i in range(bins):
+        m = idx == i
+        if np.any(m):
+            prof[i] = f[m].mean()
+    pk = np.nanmax(prof)
+    if not np.isfinite(pk) or pk <= 0:
+        return float("nan")
+    valid = prof > frac * pk
+    return float(ctr[valid][-1]) if np.any(valid) else float("nan")
+
+
+def lll_weight(psi, G, Mmax=60, Omega=1.0):
+    """vortex.py:61-84: sum over m <= Mcap of |<q_m/|q_m|, psi/|psi|>|^2,
+    q_m = exp(m log r - r^2/2 - (log pi + lgamma(m+1))/2 + i m theta)."""
+    if G["ndim"] != 2:
+        return float("nan")
+    dv = dV(G)
+    pn = psi / norm_of(psi, G)
+    X, Y = G["coords"][0], G["coords"][1]
+    R2 = G["R2"]
+    r = np.sqrt(R2)
+    theta = np.arctan2(Y, X) * (1.0 if Omega >= 0 else -1.0)
+    r2mean = float((R2 * np.abs(pn) ** 2).sum()) * dv
+    est = int(math.ceil(r2mean + 10.0 * math.sqrt(max(r2mean, 1.0)) + 30.0))
+    Mcap = min(max(int(Mmax), est), max(int(Mmax), int((0.80 * G["L"]) ** 2)))
+    w = 0.0
+    for m in range(Mcap + 1):
+        lm = m * np.log(np.maximum(r, 1e-300)) - 0.5 * R2 - 0.5 * (math.log(math.pi) + math.lgamma(m + 1.0))
+        q = np.exp(lm) * np.exp(1j * m * theta)
+        q = q / norm_of(q, G)
+        w += abs(complex((np.conj(q) * pn).sum()) * dv) ** 2
+    return min(max(w, 0.0), 1.0)
+
+
 def tf_effective_potential(G, cfg):
     """thomasfermi.py:21-30."""
     O = float(cfg.get("Omega", 1.0))

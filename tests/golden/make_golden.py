@@ -7,6 +7,7 @@ Sources (under the reference root):
   benchmarks/results/itp_timing/timing.csv          converged energies at res 1e-8
   benchmarks/results/verification/pcg_variants.csv  converged energies at res 1e-6
   benchmarks/results/verification/kernels_log_q2d.csv  log/q2d/thin-disc energies
+  benchmarks/results/verification/sweep_omega_Omega.csv  rotation sweep: E, L_z, radii, vortex counts, w_LLL
 """
 import csv
 import json
@@ -36,6 +37,10 @@ def main():
                            for r in rows("benchmarks/results/verification/pcg_variants.csv")]
     out["kernels_log_q2d"] = [{k: v for k, v in r.items()}
                               for r in rows("benchmarks/results/verification/kernels_log_q2d.csv")]
+    keep = ("Omega", "E", "Lz", "Nv", "Nplus", "Nminus", "Nnet", "w_LLL", "lll_param", "R50", "R90", "R99",
+            "R_1pct", "peak", "resid_rel", "iters", "branch", "tf_R90", "tf_R99", "tf_mu", "points_per_R90",
+            "mem_estimate_gb")
+    out["sweep_omega"] = [{k: r[k] for k in keep} for r in rows("benchmarks/results/verification/sweep_omega_Omega.csv")]
     out["_source"] = "becGPP 1.2.1 benchmarks/results (reference repository)"
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)

@@ -47,3 +47,26 @@ def relerr(a, b):
 
 def oracle_grid(c):
     return O.make_grid(c)
+
+
+def zeros_state(G, plus=(), minus=(), sigma=2.5, shift=(0.0, 0.0)):
+    """prod (z - a) over plus, prod conj(z - b) over minus, times a Gaussian of
+    width sigma centred at shift (z = x + i y in the first two axes; a 3D grid
+    gets the factor exp(-z^2/2)): phase windings +1 at each a, -1 at each b."""
+    X, Y = G["coords"][0], G["coords"][1]
+    z = X + 1j * Y
+    psi = np.exp(-((X - shift[0]) ** 2 + (Y - shift[1]) ** 2) / (2.0 * sigma ** 2)).astype(np.complex128)
+    for a in plus:
+        psi = psi * (z - a)
+    for b in minus:
+        psi = psi * np.conj(z - b)
+    if G["ndim"] == 3:
+        psi = psi * np.exp(-0.5 * G["coords"][2] ** 2)
+    return psi
+
+
+def lll_state(G, m, conj=False):
+    """z^m exp(-r^2/2) (conj(z)^m with conj): the lowest-Landau-level orbitals."""
+    X, Y = G["coords"][0], G["coords"][1]
+    z = X - 1j * Y if conj else X + 1j * Y
+    return (z ** m * np.exp(-0.5 * G["R2"])).astype(np.complex128)

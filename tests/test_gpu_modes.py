@@ -64,3 +64,9 @@ def test_single_mode_writes_records_and_resumes(tmp_path, monkeypatch):
     d2 = B.run(c)
     assert d2["iters"] < d1["iters"] and d2["converged"]
     assert abs(d2["E"] - d1["E"]) <= 1e-10 * abs(d1["E"])
+    # the diagnostic bundle is part of the record (diagnostics.py:18-47)
+    header = (tmp_path / "single_summary.csv").read_text().splitlines()[0].split(",")
+    for k in ("Nv", "Nplus", "Nminus", "Nnet", "w_LLL", "lll_param", "R50", "R90", "R99", "R_1pct", "tf_kind",
+              "tf_R90", "tf_mu", "mem_estimate_gb", "points_per_R90"):
+        assert k in d1 and k in header, k
+    assert d1["Nv"] == 0 and d1["w_LLL"] != d1["w_LLL"] and d1["tf_kind"] == "repulsive_parabola"
