@@ -7,7 +7,7 @@ and the call signatures of its ``grid``, ``operators``, ``interactions`` and
 """
 from .constants import CODE_VERSION, DEV
 from .config import CFG_DEFAULTS, default_cfg, canonical_key
-from .grid import make_grid, geometry, resolve_kernel, auto_grid, dV, norm_of, Lz_op
+from .grid import make_grid, geometry, resolve_kernel, auto_grid, resample_state, dV, norm_of, Lz_op
 from .interactions import long_range_phi, kernel_convolution, q2d_kernel, clear_kernel_cache
 from .operators import energy_components, energy_t, apply_H, compute_residual, observables, sensitivities
 from .thomasfermi import extract_tf, mass_quantiles, center_of_mass

@@ -16,13 +16,14 @@
 #define BEC_MAX_L 1024      // longest line transform held in LDS
 
 struct FftPlan {
-  int L;                      // transform length (2^a 3^b 5^c)
+  int L;                      // transform length (2^a 3^b 5^c 7^d 11^e 13^f)
   int nst;                    // number of Stockham stages
-  int rad[BEC_MAX_STAGES];    // radix per stage (8, 6, 5, 4, 3 or 2)
+  int rad[BEC_MAX_STAGES];    // radix per stage (8, 6, 5, 4, 3, 2; 7, 11, 13 outside fft_reg)
   int ns[BEC_MAX_STAGES];     // Ns of the stage: product of the earlier radices
   int step[BEC_MAX_STAGES];   // twiddle stride L / (Ns R)
   unsigned mag[BEC_MAX_STAGES];   // floor(2^32 / Ns) + 1: j / Ns = umulhi(j, mag) for j < 2^12
   int reg;                    // 1: L = 8 tpl and the first and last radix divide 8 (fft_reg)
+  int odd;                    // 1: a radix 7, 11 or 13 (kernels built with ODD)
   const double2* tw;          // tw[m] = exp(-2 pi i m / L), m in [0, L)
 };
 
@@ -197,9 +198,99 @@ __device__ __forceinline__ void fft_stage(C* s, int L, int Ns, unsigned mag, int
   __syncthreads();
 }
 
+// cos and sin of 2 pi k / R, k = 0 .. (R-1)/2, for the odd prime radices
+// (rounded from extended-precision values)
+template <int R> struct OddRoot;
+template <> struct OddRoot<7> {
+  static constexpr double c[4] = {1.0, 0.6234898018587335, -0.2225209339563144, -0.9009688679024191};
+  static constexpr double s[4] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.4338837391175581};
+};
+template <> struct OddRoot<11> {
+  static constexpr double c[6] = {1.0, 0.8412535328311812, 0.41541501300188644, -0.14231483827328514,
+                                  -0.6548607339452851, -0.9594929736144974};
+  static constexpr double s[6] = {0.0, 0.5406408174555976, 0.9096319953545183, 0.9898214418809327,
+                                  0.7557495743542583, 0.28173255684142967};
+};
+template <> struct OddRoot<13> {
+  static constexpr double c[7] = {1.0, 0.8854560256532099, 0.5680647467311558, 0.12053668025532305,
+                                  -0.3546048870425356, -0.7485107481711011, -0.970941817426052};
+  static constexpr double s[7] = {0.0, 0.46472317204376856, 0.8229838658936564, 0.992708874098054,
+                                  0.9350162426854148, 0.6631226582407952, 0.23931566428755777};
+};
+
+// One Stockham stage of an odd prime radix R (7, 11 or 13).  With a_q =
+// u_q + u_{R-q} and b_q = u_q - u_{R-q} (q = 1 .. H, H = (R-1)/2),
+//   X_p, X_{R-p} = u_0 + sum_q cos(2 pi pq/R) a_q  +-  SIGN i sum_q sin(2 pi pq/R) b_q,
+// and each output pair is stored to LDS as soon as it is formed, so a thread
+// holds R values per butterfly.  Only lengths run through fft_lds take these
+// radices (a line of L = 8 tpl with a factor >= 7 never fills whole wavefronts).
+template <int R, int SIGN, class C>
+__device__ __forceinline__ void fft_stage_odd(C* s, int L, int Ns, unsigned mag, int step, int lane, int tpl,
+                                              const C* __restrict__ tw, bool active) {
+  using F = typename CT<C>::F;
+  constexpr int H = (R - 1) / 2;
+  constexpr int MAXB = (8 + R - 1) / R;
+  const int nb = L / R;
+  C u0[MAXB], a[MAXB][H], b[MAXB][H];
+  if (active) {
+#pragma unroll
+    for (int bb = 0; bb < MAXB; ++bb) {
+      const int j = lane + bb * tpl;
+      if (j < nb) {
+        const int k = (Ns == 1) ? 0 : j - (int)__umulhi((unsigned)j, mag) * Ns;   // j mod Ns
+        u0[bb] = s[j];
+#pragma unroll
+        for (int q = 1; q <= H; ++q) {
+          C x = s[j + q * nb], y = s[j + (R - q) * nb];
+          if (Ns > 1) {
+            C wx = tw[q * k * step], wy = tw[(R - q) * k * step];
+            if (SIGN > 0) { wx.y = -wx.y; wy.y = -wy.y; }
+            x = cmul(x, wx);
+            y = cmul(y, wy);
+          }
+          a[bb][q - 1] = cadd(x, y);
+          b[bb][q - 1] = csub(x, y);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int bb = 0; bb < MAXB; ++bb) {
+      const int j = lane + bb * tpl;
+      if (j < nb) {
+        const int k = (Ns == 1) ? 0 : j - (int)__umulhi((unsigned)j, mag) * Ns;
+        const int idxD = (j - k) * R + k;
+        C x0 = u0[bb];
+#pragma unroll
+        for (int q = 0; q < H; ++q) x0 = cadd(x0, a[bb][q]);
+        s[idxD] = x0;
+#pragma unroll
+        for (int p = 1; p <= H; ++p) {
+          C re = u0[bb], im = cmk<C>(0, 0);
+#pragma unroll
+          for (int q = 1; q <= H; ++q) {
+            const int m = (p * q) % R;
+            const int mm = m <= H ? m : R - m;
+            const F cs = (F)OddRoot<R>::c[mm];
+            const F sn = (F)(m <= H ? OddRoot<R>::s[mm] : -OddRoot<R>::s[mm]);
+            re = cmk<C>(re.x + cs * a[bb][q - 1].x, re.y + cs * a[bb][q - 1].y);
+            im = cmk<C>(im.x + sn * b[bb][q - 1].x, im.y + sn * b[bb][q - 1].y);
+          }
+          const C t = cmuli<SIGN>(im);
+          s[idxD + p * Ns] = cadd(re, t);
+          s[idxD + (R - p) * Ns] = csub(re, t);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // In-place unnormalised DFT of the LDS line s (SIGN=-1 forward, +1 inverse);
 // twl = the plan's twiddle table copied to LDS (stage_twiddles).
-template <int SIGN, class C>
+template <int SIGN, bool ODD = false, class C>
 __device__ __forceinline__ void fft_lds(C* s, const FftPlan& p, const C* twl, int lane, int tpl,
                                         bool active) {
   for (int st = 0; st < p.nst; ++st) {
@@ -210,7 +301,12 @@ __device__ __forceinline__ void fft_lds(C* s, const FftPlan& p, const C* twl, in
     else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if (R == 6) fft_stage<6, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (!ODD || R == 2) fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if constexpr (ODD) {
+      if (R == 7) fft_stage_odd<7, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+      else if (R == 11) fft_stage_odd<11, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+      else fft_stage_odd<13, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    }
   }
 }
 
@@ -293,7 +389,9 @@ __device__ __forceinline__ void fft_reg(C (&v)[8], C* s, const FftPlan& p, const
 // The DFT of a line whose elements lane + m tpl (m < 8, those below L) are in
 // v: register-resident when the plan allows it, else through the LDS line s
 // (fft_lds).  Every thread of the workgroup calls it; results return in v.
-template <int SIGN, class C>
+// ODD compiles in the radices 7, 11 and 13 (plans with FftPlan::odd): their
+// stages hold more registers, so the kernels of other plans are built without.
+template <int SIGN, bool ODD = false, class C>
 __device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, const C* twl,
                                          int lane, int tpl, bool active) {
   if (p.reg) {
@@ -309,7 +407,7 @@ __device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, cons
     }
   }
   __syncthreads();
-  fft_lds<SIGN>(s, p, twl, lane, tpl, active);
+  fft_lds<SIGN, ODD>(s, p, twl, lane, tpl, active);
   if (active) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {

@@ -277,6 +277,10 @@ int launch_vortex(const double2* psi, const double* nbar, int N, double dx, doub
                   double thr, double* rows, hipStream_t s);
 int launch_nbar(const double2* f, long long n, double* nbar, double* rows, hipStream_t s);
 void launch_abs2c(const double2* psi, long long n, double2* out, hipStream_t s);
+// multilinear resampling from an n0-point grid (first node x0, span (n0-1) dx0)
+// onto the n1-point grid of spacing dx1; per-block rows of sum |out|^2
+int launch_resample(const double2* in, int ndim, int n0, double x0, double span, int n1, double dx1, double2* out,
+                    double* rows, hipStream_t s);
 // one Thomas-Fermi norm evaluation (+ bisection step unless at_hi) on st = {lo, hi, norm}
 void launch_tf_eval(const double* base, long long n, double* st, int at_hi, double b2, double b3, double dv,
                     double* rows, hipStream_t s);

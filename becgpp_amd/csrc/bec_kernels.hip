@@ -221,7 +221,7 @@ __device__ __forceinline__ double line_V(const double* c2, int e, const KinArgs&
 // C is the storage of the complex fields: double2, or float2 in an FP32 phase
 // (values are widened to FP64 in registers; the transforms run in C; every
 // sum accumulates in FP64).
-template <int AX, int KIND, class C>
+template <int AX, int KIND, class C, bool ODD = false>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 smem_d[];
   C* smem = (C*)smem_d;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       x[j] = from_d2<C>(v);
     }
   }
-  fft_line<-1>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_line<-1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
   const double invN = 1.0 / (double)N;
   const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
   const bool mult = !KE || (KIND == KE_FIRST && a.hy_out);
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     }
     if (!mult) return;
   }
-  fft_line<+1>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_line<+1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   if (oon) {
@@ -459,7 +459,7 @@ __device__ __forceinline__ void dot_accum(double* v, double2 zi, double2 pi, dou
 
 // C: storage (and transform type) of the complex fields, as in k_kin; the
 // real fields (LD_R input, ST_RE output, khat, dot) are FP64.
-template <bool CONTIG, int LD, int DIR, int MUL, int ST, class C>
+template <bool CONTIG, int LD, int DIR, int MUL, int ST, class C, bool ODD = false>
 __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   using F = typename CT<C>::F;
   extern __shared__ double2 smem_d[];
@@ -531,8 +531,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     }
   }
   const double sig = ((MUL == MUL_PRE2 || MUL == MUL_PRE3) && oon) ? a.sigma[ob] : 0.0;
-  if (DIR == DIR_INV) fft_line<+1>(v, s, a.fp, twl, lane, tpl, oon);
-  else fft_line<-1>(v, s, a.fp, twl, lane, tpl, oon);
+  if (DIR == DIR_INV) fft_line<+1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
+  else fft_line<-1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
   if (DIR == DIR_FMI) {
     if (oon) {
       if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         }
       }
     }
-    fft_line<+1>(v, s, a.fp, twl, lane, tpl, oon);
+    fft_line<+1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
   }
   // stores: line position q = lane + j tpl goes to output element q - off_store
   double dsum[1] = {0.0};
@@ -1250,8 +1250,13 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
       abort();
     }
   } else {
-    if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, e0, e1, 0, a);
-    else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, a);
+    if (a.fp.odd) {
+      if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2, true>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2, true>), grid, block, shm, s, a);
+    } else {
+      if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, a);
+    }
   }
   return (a.nl + a.lpw - 1) / a.lpw;
 }
@@ -1305,8 +1310,13 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
     if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, float2>), grid, block, shm, s, e0, e1, 0, a);
     else hipLaunchKernelGGL((k_kin<AX, KIND, float2>), grid, block, shm, s, a);
   } else {
-    if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, e0, e1, 0, a);
-    else hipLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, a);
+    if (a.fp.odd) {
+      if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, true>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_kin<AX, KIND, double2, true>), grid, block, shm, s, a);
+    } else {
+      if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, a);
+    }
   }
   return (a.nl + a.lpw - 1) / a.lpw;
 }
@@ -1458,6 +1468,80 @@ __global__ __launch_bounds__(BEC_WG) void k_abs2c(const double2* __restrict__ ps
 }
 void launch_abs2c(const double2* psi, long long n, double2* out, hipStream_t s) {
   hipLaunchKernelGGL(k_abs2c, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, psi, n, out);
+}
+
+// ---------------------------------------------------------------------------
+// Resampling between grids (grid.py:140-161): multilinear interpolation of the
+// old grid's nodes x0 + j dx0 (j < n0) at the new grid's nodes, nodes outside
+// the old grid's span taken as zero (torch grid_sample, align_corners=True,
+// zero padding, whose arithmetic and summation order this follows: the
+// normalised coordinate g = 2 (c - x0) / span - 1, the index ((g + 1) / 2)
+// (n0 - 1), corner weights (hi - t) and (t - lo) multiplied last axis first).
+// Per block: the sum of |out|^2.
+// ---------------------------------------------------------------------------
+struct Lin {
+  int lo;
+  double w0, w1;    // weights of lo and lo + 1
+};
+__device__ __forceinline__ Lin lin_axis(int i, int n1, double dx1, double x0, double span, int n0) {
+  const double c = (double)(i - n1 / 2) * dx1;
+  const double g = 2.0 * (c - x0) / span - 1.0;
+  const double t = ((g + 1.0) / 2.0) * (double)(n0 - 1);
+  const double lo = floor(t);
+  Lin l;
+  l.lo = (int)lo;
+  l.w0 = (lo + 1.0) - t;
+  l.w1 = t - lo;
+  return l;
+}
+__global__ __launch_bounds__(BEC_WG) void k_resample(const double2* __restrict__ in, int ndim, int n0, double x0,
+                                                     double span, int n1, double dx1, double2* out, double* rows) {
+  double v[1] = {0.0};
+  const long long n = ndim == 2 ? (long long)n1 * n1 : (long long)n1 * n1 * n1;
+  ELEM_LOOP(n) {
+    double2 acc = make_double2(0.0, 0.0);
+    if (ndim == 2) {
+      const int ia = (int)(i / n1), ib = (int)(i % n1);
+      const Lin A = lin_axis(ia, n1, dx1, x0, span, n0);    // axis 0 (grid_sample H)
+      const Lin Bw = lin_axis(ib, n1, dx1, x0, span, n0);   // axis 1 (grid_sample W)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {                          // nw, ne, sw, se
+        const int da = q >> 1, db = q & 1;
+        const int ja = A.lo + da, jb = Bw.lo + db;
+        if (ja < 0 || ja >= n0 || jb < 0 || jb >= n0) continue;
+        const double w = (db ? Bw.w1 : Bw.w0) * (da ? A.w1 : A.w0);
+        const double2 s = in[(long long)ja * n0 + jb];
+        acc.x += s.x * w;
+        acc.y += s.y * w;
+      }
+    } else {
+      const long long nn = (long long)n1 * n1;
+      const int ia = (int)(i / nn), ib = (int)((i / n1) % n1), ic = (int)(i % n1);
+      const Lin A = lin_axis(ia, n1, dx1, x0, span, n0);    // axis 0 (D)
+      const Lin Bh = lin_axis(ib, n1, dx1, x0, span, n0);   // axis 1 (H)
+      const Lin Cw = lin_axis(ic, n1, dx1, x0, span, n0);   // axis 2 (W)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {                          // tnw .. bse
+        const int da = q >> 2, db = (q >> 1) & 1, dc = q & 1;
+        const int ja = A.lo + da, jb = Bh.lo + db, jc = Cw.lo + dc;
+        if (ja < 0 || ja >= n0 || jb < 0 || jb >= n0 || jc < 0 || jc >= n0) continue;
+        const double w = (dc ? Cw.w1 : Cw.w0) * (db ? Bh.w1 : Bh.w0) * (da ? A.w1 : A.w0);
+        const double2 s = in[((long long)ja * n0 + jb) * n0 + jc];
+        acc.x += s.x * w;
+        acc.y += s.y * w;
+      }
+    }
+    out[i] = acc;
+    v[0] += cabs2(acc);
+  }
+  block_partials<1, 0>(v, nullptr, rows, nullptr, blockIdx.x);
+}
+int launch_resample(const double2* in, int ndim, int n0, double x0, double span, int n1, double dx1, double2* out,
+                    double* rows, hipStream_t s) {
+  const long long n = ndim == 2 ? (long long)n1 * n1 : (long long)n1 * n1 * n1;
+  const int g = elem_blocks(n);
+  hipLaunchKernelGGL(k_resample, dim3(g), dim3(BEC_WG), 0, s, in, ndim, n0, x0, span, n1, dx1, out, rows);
+  return g;
 }
 
 // ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -33,7 +34,7 @@ static void bad(const std::string& m) { throw BecError(1, m); }
 // ---------------------------------------------------------------------------
 // FFT plans
 // ---------------------------------------------------------------------------
-// Stage radices of L = 2^a 3^b 5^c.  When 8 divides L the order starts and
+// Stage radices of L = 2^a 3^b 5^c 7^d 11^e 13^f.  When 8 divides L the order starts and
 // ends with a power-of-two radix (8, 4 or 2): with tpl = L/8 threads per line
 // those two stages then run from and to registers (fft_reg).  reg reports it;
 // it also needs whole wavefronts: tpl times some power of two <= BEC_WG is a
@@ -48,6 +49,8 @@ static bool factor(int L, std::vector<int>& rad, bool& reg) {
   if (a % 3 == 2) p2.push_back(4);
   while (n % 3 == 0) { odd.push_back(3); n /= 3; }
   while (n % 5 == 0) { odd.push_back(5); n /= 5; }
+  for (int q : {7, 11, 13})
+    while (n % q == 0) { odd.push_back(q); n /= q; }
   if (n != 1) return false;
   reg = false;
   if (a >= 3)
@@ -76,6 +79,9 @@ static bool factor(int L, std::vector<int>& rad, bool& reg) {
 void set_stage_order(FftPlan& fp, const std::vector<int>& rad) {
   const int L = fp.L;
   fp.nst = (int)rad.size();
+  fp.odd = 0;
+  for (int r : rad)
+    if (r == 7 || r == 11 || r == 13) fp.odd = 1;
   int ns = 1;
   for (int i = 0; i < BEC_MAX_STAGES; ++i) {
     fp.rad[i] = i < fp.nst ? rad[i] : 1;
@@ -108,7 +114,7 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
   std::vector<int> rad;
   bool reg = false;
   if (L < 2 || L > BEC_MAX_L || !factor(L, rad, reg))
-    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b 5^c <= " +
+    bad("transform length " + std::to_string(L) + " unsupported: need 2^a 3^b 5^c 7^d 11^e 13^f <= " +
         std::to_string(BEC_MAX_L));
   fp.L = L;
   fp.reg = reg ? 1 : 0;
@@ -997,6 +1003,46 @@ extern "C" int bec_lll_weight(bec_plan* h, const void* psi, double Omega, int Mm
 // exp(-ell^2 k^2 / 2) (two line passes), then the plaquettes' phase windings
 // where the corner-averaged smoothed density exceeds frac * its maximum and
 // the corner lies within R of (xcm, ycm).  out4 = {Nv, N+, N-, Nnet}.
+// Resampling between grids (grid.py:140-161): needs no plan (either grid may
+// be one the line transforms do not cover); its reduction rows live in a
+// temporary buffer of this call.
+extern "C" int bec_resample(int ndim, int n0, double dx0, const void* psi_old, int n1, double dx1, void* psi_new,
+                            double* norm_out, void* stream) {
+  BEC_TRY({
+    if (!psi_old || !psi_new || !norm_out) bad("null argument");
+    if (ndim != 2 && ndim != 3) bad("ndim must be 2 or 3");
+    if (n0 < 1 || n1 < 1 || !(dx0 > 0.0) || !(dx1 > 0.0)) bad("grid sizes and spacings must be positive");
+    *norm_out = 0.0;
+    const double x0 = -(double)(n0 / 2) * dx0;             // first and last node of the old grid
+    const double x1 = x0 + (double)(n0 - 1) * dx0;
+    if (!(x1 > x0)) return 0;                                // degenerate: no interpolation
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = ndim == 2 ? (long long)n1 * n1 : (long long)n1 * n1 * n1;
+    double* rows = nullptr;
+    HIPCHK(hipMalloc(&rows, 1024 * sizeof(double)));
+    std::vector<double> r(1024);
+    int g = 0;
+    try {
+      g = launch_resample((const double2*)psi_old, ndim, n0, x0, x1 - x0, n1, dx1, (double2*)psi_new, rows, s);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(r.data(), rows, (size_t)g * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(rows);
+      throw;
+    }
+    HIPCHK(hipFree(rows));
+    double sum = 0.0;
+    for (int b = 0; b < g; ++b) sum += r[b];
+    const double nn = sqrt(sum * (ndim == 2 ? dx1 * dx1 : dx1 * dx1 * dx1));
+    *norm_out = nn;
+    if (std::isfinite(nn) && nn > 1e-14) {
+      launch_scale((const double2*)psi_new, (double2*)psi_new, &nn, n, 1, s);
+      HIPCHK(hipGetLastError());
+    }
+  });
+}
+
 extern "C" int bec_vortex_count(bec_plan* h, const void* psi, double xcm, double ycm, double R, double ell,
                                 double frac, int* out4) {
   BEC_TRY({

@@ -743,15 +743,16 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   return 0;
 }
 
-// The loop, preceded by an FP32 warm phase when warm_res_tol > res_tol and
-// no instance has long range (the convolution stays FP64 throughout): the
-// FP64 loop restarts from the warm phase's returned states.
+// The loop, preceded by an FP32 warm phase when warm_res_tol > res_tol, no
+// instance has long range (the convolution stays FP64 throughout) and the
+// grid's transforms use no radix 7, 11 or 13 (built for FP64 only): the FP64
+// loop restarts from the warm phase's returned states.
 static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
                        bec_solve_out* out, double* trace, int trace_cap) {
   bool lr = false;
   for (int b = 0; b < nb && b < BEC_BMAX; ++b)
     lr = lr || plan_long_range(p, Phys{php[b].beta2, php[b].beta3, php[b].Omega, php[b].G_C});
-  if (!(cf->warm_res_tol > cf->res_tol) || lr) {
+  if (!(cf->warm_res_tol > cf->res_tol) || lr || p.fpN.odd) {
     const int r = solve_phase(p, nb, php, cf, psi_io, out, trace, trace_cap, false);
     for (int b = 0; b < nb; ++b) out[b].warm_iters = 0;
     return r;

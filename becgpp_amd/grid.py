@@ -215,3 +215,22 @@ def auto_grid(cfg):
     cap = 512 if ndim == 2 else 224
     N = int(min(cap, max(128, 32 * math.ceil(Nn / 32))))
     return float(L), N, R_est
+
+
+def resample_state(psi_old, G_old, G_new):
+    """Multilinear interpolation of a state between two physical grids,
+    renormalised on the new grid (grid.py:140-161); None when the
+    interpolation is degenerate.  Computed by bec_resample on the device."""
+    d = int(G_old["ndim"])
+    if int(G_new["ndim"]) != d:
+        raise ValueError("resample_state needs grids of the same dimension")
+    src = as_field(psi_old, G_old)
+    out = torch.empty((int(G_new["N"]),) * d, dtype=torch.complex128, device=src.device)
+    nn = ctypes.c_double()
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.bec_resample(d, int(G_old["N"]), float(G_old["dx"]), _ptr(src), int(G_new["N"]),
+                                float(G_new["dx"]), _ptr(out), ctypes.byref(nn), ctypes.c_void_p(st)))
+    if not math.isfinite(nn.value) or nn.value <= 1e-14:
+        return None
+    return out

@@ -21,7 +21,7 @@ import torch
 
 from .config import CFG_DEFAULTS, canonical_key
 from .constants import CODE_VERSION, DEV
-from .grid import dV, geometry, make_grid, norm_of, resolve_kernel
+from .grid import auto_grid, dV, geometry, make_grid, norm_of, resample_state, resolve_kernel
 from .interactions import long_range_phi
 from .operators import energy_components
 from .solvers import batchable, ground_state, ground_state_batch, ground_state_multiseed
@@ -138,10 +138,17 @@ def mode_sweep(cfg):
     if not values:
         print("[sweep] sweep_values is empty: nothing to do")
         return []
+    rot = [abs(float(v if param == "Omega" else cfg.get("Omega", 0.0))) for v in values]
+    if max(rot) >= 0.5 and nseeds < 2:
+        print("[hint] rotating sweep with nseeds=1: continuation alone can stay on a vortex-free")
+        print("       (metastable) branch. Use nseeds>=2 so each point also tries a vortex-lattice seed.")
     pts = []
     for val in values:
         c = dict(cfg)
         c[param] = val
+        if cfg.get("sweep_autobox", False):             # modes.py:343-345
+            L, N, _ = auto_grid(c)
+            c["L"], c["Ngrid"] = L, N
         c["tag"] = f"{cfg.get('tag', 'becgpp')}_{param}{val:g}"
         pts.append(c)
     rows = [None] * len(pts)
@@ -161,12 +168,14 @@ def mode_sweep(cfg):
                 cands = [(f"seed:{cj['seed']}", r) for (jp, cj), r in zip(jobs, res) if jp == ip]
                 rows[ip] = _pick(cands, pts[ip], param, values[ip])
         return rows
-    prev = None
+    prev, G_prev = None, None
     for ip, c in enumerate(pts):
         G = make_grid(c)
         cands = []
-        if prev is not None and tuple(prev.shape) == tuple(G["R2"].shape):
-            cands.append(("continuation", ground_state(c, G=G, psi0=prev, verbose=False)))
+        if prev is not None:                            # modes.py:350-355
+            psi0 = prev if tuple(prev.shape) == tuple(G["R2"].shape) else resample_state(prev, G_prev, G)
+            if psi0 is not None:
+                cands.append(("continuation", ground_state(c, G=G, psi0=psi0, verbose=False)))
         if nseeds > 1 or not cands:
             seeds = _seed_cfgs(c, nseeds)
             if len(seeds) > 1 and batchable(G):
@@ -176,7 +185,7 @@ def mode_sweep(cfg):
             best = min(range(len(res)), key=lambda j: res[j][2]["E"])
             cands.append((f"seed:{seeds[best]['seed']}", res[best]))
         rows[ip] = _pick(cands, c, param, values[ip])
-        prev = rows[ip]["psi"]
+        prev, G_prev = rows[ip]["psi"], G
     return rows
 
 

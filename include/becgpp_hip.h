@@ -95,8 +95,8 @@ const char* bec_last_error(void);
 /* Plan: grid geometry, FFT twiddles, workspaces, kernel spectra.
  * Replaces make_grid (grid.py:69-101) for the device side: x_i = (i-N/2) dx,
  * dx = 2L/N, k = 2 pi fftfreq(N, dx), V = trap_coeff r^s (0 if s = 0),
- * padded length M = pad N.  N and pad N must factor into 2^a 3^b 5^c with
- * pad N <= 1024; ndim is 2 or 3. */
+ * padded length M = pad N.  N and pad N must factor into 2^a 3^b 5^c 7^d
+ * 11^e 13^f with pad N <= 1024; ndim is 2 or 3. */
 int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff, bec_plan** out);
 int bec_plan_destroy(bec_plan* p);
 int bec_plan_set_stream(bec_plan* p, void* hip_stream);
@@ -149,6 +149,18 @@ int bec_moments(bec_plan* p, const void* psi, double* out5);
  * atan2(y, x), Mcap = min(max(Mmax, ceil(<r^2> + 10 sqrt(max(<r^2>,1)) + 30)),
  * max(Mmax, floor((0.8 L)^2))); clamped to [0, 1].  mcap_out (nullable). */
 int bec_lll_weight(bec_plan* p, const void* psi, double Omega, int Mmax, double* w_out, int* mcap_out);
+
+/* Resampling of a state between grids (grid.py:140-161, resample_state):
+ * multilinear interpolation of the n0^ndim grid with spacing dx0 (nodes
+ * (j - n0/2) dx0) at the nodes of the n1^ndim grid with spacing dx1, zero
+ * outside the old grid's span (torch grid_sample, align_corners=True, zero
+ * padding), then divided by its norm sqrt(sum |psi|^2 dx1^ndim).  norm_out
+ * receives that norm before the division; the state is left undivided (and
+ * the caller reports no state) when the norm is not finite or <= 1e-14, or
+ * when n0 = 1 (norm 0).  Needs no plan; runs on `stream` (a hipStream_t,
+ * null for the default stream) and synchronises it. */
+int bec_resample(int ndim, int n0, double dx0, const void* psi_old, int n1, double dx1, void* psi_new,
+                 double* norm_out, void* stream);
 
 /* Vortex count of a 2D state (vortex.py:10-48, _count_vortices_2d): the phase
  * winding of every grid plaquette, counted where the plaquette's mean density

@@ -601,6 +601,48 @@ def mass_quantiles(rho, G, fractions=(0.50, 0.90, 0.99), center=None):
 
 
 # ----------------------------------------------------------------------------
+# Resampling between grids  (becgpp/grid.py:140-161)
+# ----------------------------------------------------------------------------
+def resample_state(psi_old, G_old, G_new):
+    """grid.py:140-161: torch grid_sample (bilinear / trilinear, align_corners
+    = True, zero padding) restated: per axis t = ((g + 1) / 2) (n0 - 1) with
+    g = 2 (c - x0) / (x1 - x0) - 1, corners floor(t), floor(t) + 1 with weights
+    (floor(t) + 1 - t) and (t - floor(t)), multiplied last axis first and
+    summed corner by corner (last axis fastest); renormalised; None when
+    degenerate."""
+    d = G_old["ndim"]
+    n0, dx0 = int(G_old["N"]), float(G_old["dx"])
+    x0 = -(n0 // 2) * dx0
+    x1 = x0 + (n0 - 1) * dx0
+    if not (x1 > x0):
+        return None
+    axes = []
+    for c in G_new["coords"]:
+        g = 2.0 * (c - x0) / (x1 - x0) - 1.0
+        t = ((g + 1.0) / 2.0) * (n0 - 1)
+        lo = np.floor(t)
+        axes.append((lo.astype(np.int64), (lo + 1.0) - t, t - lo))
+    out = np.zeros(G_new["coords"][0].shape, complex)
+    for q in range(1 << d):
+        bits = [(q >> (d - 1 - a)) & 1 for a in range(d)]        # axis 0 slowest
+        idx, ok, w = [], np.ones(out.shape, bool), None
+        for a in reversed(range(d)):                              # weights: last axis first
+            lo, w0, w1 = axes[a]
+            j = lo + bits[a]
+            ok &= (j >= 0) & (j < n0)
+            wa = w1 if bits[a] else w0
+            w = wa if w is None else w * wa
+        for a in range(d):
+            idx.append(np.clip(axes[a][0] + bits[a], 0, n0 - 1))
+        val = psi_old[tuple(idx)]
+        out = out + np.where(ok, val.real * w, 0.0) + 1j * np.where(ok, val.imag * w, 0.0)
+    nn = math.sqrt(float((np.abs(out) ** 2).sum()) * dV(G_new))
+    if not np.isfinite(nn) or nn <= 1e-14:
+        return None
+    return out / nn
+
+
+# ----------------------------------------------------------------------------
 # Diagnostics of a state  (becgpp/vortex.py:10-84, fields.py:40-67)
 # ----------------------------------------------------------------------------
 def _wrap_pi(d):

@@ -144,3 +144,42 @@ def test_rotating_state_diagnostics_match_reference_sweep():
     assert O.vortex_count_2d(p, Go, Omega=0.5) == {k: d[k] for k in ("Nv", "Nplus", "Nminus", "Nnet")}
     assert abs(O.lll_weight(p, Go, Omega=0.5) - d["w_LLL"]) <= 1e-11
     assert O.droplet_radius(np.abs(p) ** 2, Go) == d["R_1pct"]
+
+
+@pytest.mark.parametrize("case", [("2D", 64, 6.0, 64, 6.0), ("2D", 65, 6.0, 65, 6.0), ("2D", 48, 6.0, 96, 8.0),
+                                  ("2D", 96, 8.0, 40, 4.0), ("3D", 24, 5.0, 32, 6.0), ("3D", 20, 4.0, 15, 3.0)])
+def test_resample_state_matches_oracle(case):
+    """grid.py:140-161 through bec_resample: identity (the reference's
+    tests/test_audit.py:139-143, 230-234, even and odd N), refinement into a
+    larger box (zeros outside), coarsening, 3D."""
+    import becgpp_amd as B
+    dim, n0, L0, n1, L1 = case
+    c0, c1 = cfg(dimension=dim, Ngrid=n0, L=L0), cfg(dimension=dim, Ngrid=n1, L=L1)
+    G0o, G1o = O.make_grid(c0), O.make_grid(c1)
+    G0d, G1d = B.make_grid(c0), B.make_grid(c1)
+    psi, _ = random_state(G0o, seed=9)
+    if (n0, L0) == (n1, L1):
+        psi = np.exp(-((G0o["coords"][0] - 1.0) ** 2 + G0o["coords"][1] ** 2)) + 0j
+        psi = psi / O.norm_of(psi, G0o)
+    ref = O.resample_state(psi, G0o, G1o)
+    got = B.resample_state(_dev(psi), G0d, G1d).cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    if (n0, L0) == (n1, L1):
+        assert np.abs(got - psi).max() < 1e-12
+
+
+def test_sweep_autobox_continues_across_grids(tmp_path, monkeypatch):
+    """modes.py:343-355: with sweep_autobox each point gets auto_grid's box; the
+    continuation candidate starts from the previous point's state resampled
+    onto the new grid (a self-bound droplet: N 192 -> 256 in the half-box 6)."""
+    import becgpp_amd as B
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
+    c = B.default_cfg(mode="sweep", dimension="2D", beta2=-40.0, beta3=200.0, trap_coeff=0.0, s=0,
+                      sweep_param="beta3", sweep_values=[200.0, 100.0], sweep_autobox=True,
+                      sweep_continuation=True, nseeds=1, seed="gaussian", res_tol=1e-6, maxit=20000)
+    rows = B.run(c)
+    grids = [B.auto_grid(dict(c, beta3=v))[:2] for v in (200.0, 100.0)]
+    assert grids[0] != grids[1]
+    assert [(r["L"], r["N"]) for r in rows] == [tuple(g) for g in grids]
+    assert "continuation" in rows[1]["candidate_E"]
+    assert rows[1]["converged"]

@@ -66,6 +66,37 @@ def test_apply_H_long_lines(kw):
     assert relerr(got, ref) < TOL
 
 
+# grids whose transforms need the radices 7, 11 and 13 (the sizes auto_grid
+# picks at 224, 352 and 416, scaled down), including odd N; the conv lines are
+# M = 2N.  H, the energy components and the preconditioner against the oracle.
+@pytest.mark.parametrize("kw", [dict(dimension="2D", Ngrid=224, L=10, beta2=100, Omega=0.5),
+                                dict(dimension="2D", Ngrid=88, L=8, beta2=30, G_C=3, kernel="newton", Omega=0.3),
+                                dict(dimension="2D", Ngrid=104, L=8, beta2=30, G_C=2, kernel="log"),
+                                dict(dimension="2D", Ngrid=63, L=6, beta2=30, beta3=5),
+                                dict(dimension="2D", Ngrid=65, L=6, beta2=30, G_C=2, kernel="newton"),
+                                dict(dimension="quasi2D", Ngrid=77, L=6, beta2=30, G_C=2, kernel="q2d", l_z=0.5),
+                                dict(dimension="3D", Ngrid=28, L=5, beta2=20, beta3=2, G_C=3, kernel="newton",
+                                     Omega=0.5)])
+def test_odd_radix_grids(kw):
+    import becgpp_amd as B
+    from becgpp_amd.operators import preconditioner
+    c, Go, Gd = _pair(kw)
+    psi, _ = random_state(Go, 4)
+    psi = psi / O.norm_of(psi, Go)
+    ref = O.apply_H(psi, Go, c)
+    got = B.apply_H(_dev(psi), Gd, c).cpu().numpy()
+    assert relerr(got, ref) < TOL
+    e_ref = O.energy_components(psi, Go, c["beta2"], c["beta3"], c["Omega"], c["G_C"])
+    e_got = B.energy_components(_dev(psi), Gd, c["beta2"], c["beta3"], c["Omega"], c["G_C"])
+    scale = max(abs(e_ref[k]) for k in ("Ekin", "Etrap", "Econtact", "Ethree", "Egrav", "Lz"))
+    for k in ("E", "Ekin", "Etrap", "Econtact", "Ethree", "Erot", "Egrav", "Lz"):
+        assert abs(e_got[k].item() - e_ref[k]) <= TOL * scale, (k, e_got[k].item(), e_ref[k])
+    pre_ref, sig_ref = O.preconditioner(ref, psi, Go, c)
+    pre_got, sig_got = preconditioner(_dev(ref), _dev(psi), Gd, c)
+    assert abs(sig_got - sig_ref) <= 1e-14 * abs(sig_ref)
+    assert relerr(pre_got.cpu().numpy(), pre_ref) < TOL
+
+
 @pytest.mark.parametrize("kw", PHYSICS)
 def test_energy_components(kw):
     import becgpp_amd as B

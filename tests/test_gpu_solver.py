@@ -118,6 +118,27 @@ def test_device_and_host_loops_agree(kw):
         assert abs(od[k] - oh[k]) <= 1e-8 * max(1.0, abs(oh[k])), (k, od[k], oh[k])
 
 
+@pytest.mark.parametrize("kw", [dict(beta2=30.0, Omega=0.4, Ngrid=56, res_tol=1e-8),
+                                dict(beta2=30.0, G_C=3.0, kernel="newton", Ngrid=63, res_tol=1e-8),
+                                dict(dimension="3D", beta2=20.0, G_C=3.0, kernel="newton", Ngrid=28, L=5,
+                                     res_tol=1e-8)])
+def test_converged_state_on_odd_radix_grids(kw):
+    """Grids whose transforms take the radices 7 and 11 / odd N (56 = 8 7,
+    63 = 7 9 with conv lines 126, 28 = 4 7 with 56): the converged state
+    matches the oracle; the FP32 warm phase, not built for these radices, is
+    declined and the solve runs in FP64 throughout."""
+    import becgpp_amd as B
+    c = cfg(**kw)
+    Go = O.make_grid(c)
+    psi0 = O.noisy_gaussian(Go, 1e-3, seed=7)
+    po, _, oo = O.ground_state(c, Go, psi0)
+    pd, _, od = B.ground_state(dict(c, warm_fp32=True), B.make_grid(c), torch.as_tensor(psi0).cuda(), verbose=False)
+    assert od.get("warm_iters", 0) == 0
+    for k in ("E", "mu", "Lz"):
+        assert abs(od[k] - oo[k]) <= 1e-8 * max(1.0, abs(oo[k])), (k, od[k], oo[k])
+    assert _dens_l2(pd.cpu().numpy(), po, Go) < 1e-6
+
+
 @pytest.mark.parametrize("kw", [dict(beta2=100.0), dict(beta2=30.0, beta3=5.0, Omega=0.4),
                                 dict(trap_coeff=0.0, beta2=-60.0, beta3=1000.0, L=8.0)])
 def test_fp32_warm_phase_then_fp64_polish(kw):
