@@ -34,11 +34,15 @@ static void bad(const std::string& m) { throw BecError(1, m); }
 // ---------------------------------------------------------------------------
 // FFT plans
 // ---------------------------------------------------------------------------
-// Stage radices of L = 2^a 3^b 5^c 7^d 11^e 13^f.  When 8 divides L the order starts and
-// ends with a power-of-two radix (8, 4 or 2): with tpl = L/8 threads per line
-// those two stages then run from and to registers (fft_reg).  reg reports it;
-// it also needs whole wavefronts: tpl times some power of two <= BEC_WG is a
-// multiple of 64 (L = 2^a or 3 2^a).
+// Stage radices of L = 2^a 3^b 5^c 7^d 11^e 13^f.  When 8 divides L the order
+// starts and ends with a power-of-two radix (8, 4 or 2): with tpl = L/8
+// threads per line those two stages can run from and to registers (fft_reg).
+// reg reports that they do, which is chosen for L = 2^a only: a line of
+// L = 3 2^a on tpl = 3 2^(a-3) threads needs four or more lines per
+// workgroup for whole wavefronts, and the same line through LDS on a power
+// of two threads (tpl = 64 at L = 384, one wavefront per line, four times the
+// workgroups) measured 9% more CG iterations per second on the 2D lattice at
+// N = 384 (3 instances) while L = 2^a lines lost 7-11% through LDS.
 static bool factor(int L, std::vector<int>& rad, bool& reg) {
   rad.clear();
   int n = L, a = 0;
@@ -53,7 +57,7 @@ static bool factor(int L, std::vector<int>& rad, bool& reg) {
     while (n % q == 0) { odd.push_back(q); n /= q; }
   if (n != 1) return false;
   reg = false;
-  if (a >= 3)
+  if (a >= 3 && (L & (L - 1)) == 0)    // L = 2^a: tpl = L/8 is a power of two
     for (int l = 1; l * (L / 8) <= BEC_WG; l *= 2)
       if ((l * (L / 8)) % 64 == 0) reg = true;
   if (L == 8) reg = true;              // one thread per line, one stage

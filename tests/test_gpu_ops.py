@@ -51,8 +51,9 @@ def test_apply_H(kw):
     assert relerr(got, ref) < TOL
 
 
-# line lengths whose transforms use every stage kind: N = 384 (radices 8, 6, 8),
-# conv M = 384 from N = 192, and N = 160 (8, 5, 4: LDS-resident transforms)
+# line lengths whose transforms use every stage kind: N = 384 (radices 8, 6, 8
+# through LDS on 64 threads per line), conv M = 384 from N = 192, N = 160
+# (8, 5, 4), and the register-resident 2^a lengths elsewhere in this file
 @pytest.mark.parametrize("kw", [dict(dimension="2D", Ngrid=384, L=12, beta2=200, Omega=0.9),
                                 dict(dimension="2D", Ngrid=192, L=10, beta2=30, G_C=3, kernel="newton",
                                      Omega=0.3),
