@@ -24,6 +24,7 @@ struct FftPlan {
   unsigned mag[BEC_MAX_STAGES];   // floor(2^32 / Ns) + 1: j / Ns = umulhi(j, mag) for j < 2^12
   int reg;                    // 1: L = 8 tpl and the first and last radix divide 8 (fft_reg)
   int odd;                    // 1: a radix 7, 11 or 13 (kernels built with ODD)
+  int wide;                   // 1: through LDS on about L/4 threads per line (not reg)
   const double2* tw;          // tw[m] = exp(-2 pi i m / L), m in [0, L)
 };
 

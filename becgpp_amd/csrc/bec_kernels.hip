@@ -1138,7 +1138,7 @@ LineCfg line_cfg(const FftPlan& fp) {
   if (fp.reg) {
     tpl = L / 8;
   } else {
-    const int need = (L + 7) / 8;
+    const int need = fp.wide ? (L + 3) / 4 : (L + 7) / 8;
     while (tpl < need) tpl <<= 1;
     if (tpl > BEC_WG) tpl = BEC_WG;
   }

@@ -114,6 +114,7 @@ struct DotFields {
 int run_precond(Plan& p, const Batch& B, const double2* g, double2* out, const DotFields* dots = nullptr);
 
 void solver_release(Plan& p);
+bool plan_batchable(const Plan& p);            // batched solves possible on this grid
 // FFT stage schedules (the autotuner's search space)
 void set_stage_order(FftPlan& fp, const std::vector<int>& rad);
 std::vector<std::vector<int>> stage_orders(const FftPlan& fp, int cap);

@@ -122,6 +122,7 @@ static void make_fft(int L, FftPlan& fp, double2** tw_dev, hipStream_t s) {
         std::to_string(BEC_MAX_L));
   fp.L = L;
   fp.reg = reg ? 1 : 0;
+  fp.wide = 0;
   set_stage_order(fp, rad);
   // twiddles exp(-2 pi i m / L) evaluated in extended precision
   std::vector<double2> tw(L);
@@ -601,6 +602,14 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     const int nlk = (int)(p.n / N);
     if (p.fpN.reg && nlk % lpw_min(N / 8) != 0) p.fpN.reg = 0;
     if (p.fpM.reg && nlk % lpw_min(p.M / 8) != 0) p.fpM.reg = 0;
+    // a 2D grid of N <= 128 has few lines per instance (N): its N-point lines
+    // run through LDS on about L/4 threads, one wavefront per line at N = 128
+    // (5% more CG iterations/s on 2D N = 128; 3D grids, with N^2 lines per
+    // instance, lost 17% that way and keep L/8)
+    if (ndim == 2 && N <= 128) {
+      p.fpN.reg = 0;
+      p.fpN.wide = 1;
+    }
     p.lcN = line_cfg(p.fpN);
     p.lcM = line_cfg(p.fpM);
     plan_reserve(p, 1);

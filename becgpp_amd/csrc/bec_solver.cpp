@@ -369,12 +369,9 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
                        bec_solve_out* out, double* trace, int trace_cap, bool f32) {
   if (nb < 1 || nb > BEC_BMAX) throw BecError(1, "batch size must be in [1, " + std::to_string(BEC_BMAX) + "]");
   if (cf->cg_beta < 0 || cf->cg_beta > 2) throw BecError(1, "cg_beta must be 'pr', 'pr_precond' or 'none'");
-  if (nb > 1) {
-    // reducing line passes need whole workgroups per instance
-    const int nlk = (int)(p.n / p.N);
-    if (nlk % lpw_min(p.lcN.tpl) != 0 || nlk % lpw_min(p.lcM.tpl) != 0)
-      throw BecError(1, "batched solves need a grid with N >= 32 (2D) or N >= 16 (3D)");
-  }
+  if (nb > 1 && !plan_batchable(p))
+    throw BecError(1, "batched solves need each instance's lines to fill whole workgroups, which this grid (N = " +
+                          std::to_string(p.N) + ") does not: solve its instances one at a time");
   std::vector<Phys> ph(nb);
   for (int b = 0; b < nb; ++b) ph[b] = Phys{php[b].beta2, php[b].beta3, php[b].Omega, php[b].G_C};
   plan_reserve(p, nb);
@@ -782,6 +779,24 @@ extern "C" int bec_ground_state(bec_plan* h, const bec_phys* php, const bec_solv
   } catch (const std::exception& e) {
     bec_set_error(e.what());
     return 2;
+  }
+}
+
+// Reducing line passes need whole workgroups per instance: one instance's
+// lines (n / N) must be a multiple of the smallest workgroup of both lengths.
+bool plan_batchable(const Plan& p) {
+  const int nlk = (int)(p.n / p.N);
+  return nlk % lpw_min(p.lcN.tpl) == 0 && nlk % lpw_min(p.lcM.tpl) == 0;
+}
+
+extern "C" int bec_plan_max_batch(bec_plan* h, int* nb_max) {
+  try {
+    if (!h || !nb_max) throw BecError(1, "null argument");
+    *nb_max = plan_batchable(h->p) ? BEC_BMAX : 1;
+    return 0;
+  } catch (const BecError& e) {
+    bec_set_error(e.what());
+    return e.code;
   }
 }
 

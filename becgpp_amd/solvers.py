@@ -20,9 +20,13 @@ BATCH_MAX = 64
 
 
 def batchable(G):
-    """Batched solves need whole workgroups of lines per instance: N >= 32 in
-    2D, N >= 16 in 3D."""
-    return G["N"] >= (32 if G["ndim"] == 2 else 16)
+    """Whether the grid supports batched solves (bec_plan_max_batch): each
+    instance's lines must fill whole workgroups of its line passes, which
+    excludes small or awkward N (2D N = 8, 20, 50, ...)."""
+    pl = plan_of(G)
+    nb = ctypes.c_int()
+    _lib.check(pl.lib.bec_plan_max_batch(pl.h, ctypes.byref(nb)))
+    return nb.value > 1
 
 
 def _solver_cfg(cfg, verbose):

@@ -197,6 +197,12 @@ int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg,
 int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
                            bec_solve_out* out);
 
+/* Largest batch the plan's grid supports in bec_ground_state_batch: 64, or 1
+ * when one instance's lines do not fill whole workgroups of its line passes
+ * (small or awkward N, e.g. 2D N = 20 or 50).  No reference counterpart: the
+ * reference solves seeds one after the other (solvers.py:238-263). */
+int bec_plan_max_batch(bec_plan* p, int* nb_max);
+
 /* Sizes the plan's workspace, solver state and device-loop buffers for nb
  * instances ahead of bec_ground_state_batch (which otherwise sizes them on
  * first use).  No counterpart in the reference, which allocates per call:

@@ -137,11 +137,15 @@ def test_batch_argument_errors():
     p0 = torch.zeros((48, 48), dtype=torch.complex128, device="cuda")
     with pytest.raises(ValueError):
         B.solve_native_batch([c] * 65, G, [p0] * 65)
-    small = cfg(Ngrid=16, L=4.0)
+    # 2D N = 20: 20 lines per instance do not fill the 8-line workgroups of
+    # its 8-thread lines; N = 16 (4 threads per line, 16 lines) does
+    small = cfg(Ngrid=20, L=4.0)
     Gs = B.make_grid(small)
     assert not B.solvers.batchable(Gs)
     with pytest.raises(ValueError):
-        B.solve_native_batch([small] * 2, Gs, [torch.ones((16, 16), dtype=torch.complex128, device="cuda")] * 2)
+        B.solve_native_batch([small] * 2, Gs, [torch.ones((20, 20), dtype=torch.complex128, device="cuda")] * 2)
+    assert B.solvers.batchable(B.make_grid(cfg(Ngrid=16, L=4.0)))
+    assert not B.solvers.batchable(B.make_grid(cfg(Ngrid=50, L=6.0)))
 
 
 def test_uneven_grids_run_concurrently_and_match():
