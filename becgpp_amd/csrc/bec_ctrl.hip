@@ -23,15 +23,25 @@
 // Sums (q < K) or maxima (q >= K) of the R rows of KP values at rows[],
 // folded as: lane j of value q takes rows j, j + J, ... in order, then a fixed
 // pairwise tree over the J lanes.  out[q] for q < KP; every thread calls it.
+// The loads of eight consecutive rows of a lane are issued together before
+// they are added in order, so a lane waits once per eight rows.
 template <int KP>
 __device__ void fold_rows(const double* __restrict__ rows, int R, int K, double* out, double* sh) {
   constexpr int J = CTRL_WG / KP;
+  constexpr int U = 8;
   const int q = threadIdx.x % KP, j = threadIdx.x / KP;
   const bool is_sum = q < K;
   double acc = is_sum ? 0.0 : -INFINITY;
-  for (int r = j; r < R; r += J) {
-    const double x = rows[(long long)r * KP + q];
-    acc = is_sum ? acc + x : fmax(acc, x);
+  for (int r0 = j; r0 < R; r0 += U * J) {
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u * J;
+      x[u] = r < R ? rows[(long long)r * KP + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r0 + u * J < R) acc = is_sum ? acc + x[u] : fmax(acc, x[u]);
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
