@@ -146,6 +146,26 @@ __device__ __forceinline__ KinScal kin_scal_all(const KinArgs& a, int b) {
   k.rf = a.rf_omega ? -ip.Omega : a.rf;
   return k;
 }
+// The new-direction coefficients of instance b for the first energy pass of
+// the device-driven loop (on when its flag word carries comb_bit), read at
+// the kernel's start so that z and g load together with psi and d.
+struct CombScal {
+  bool on;
+  double az, ad, ag, gre, gim;
+};
+__device__ __forceinline__ CombScal comb_scal(const KinArgs& a, int b, bool inst_on) {
+  CombScal c{false, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (a.d_out && inst_on && (a.act.fl[b] & a.comb_bit)) {
+    const double* t = a.tab + (long long)b * DS_N;
+    c.on = true;
+    c.az = t[DS_AZ];
+    c.ad = t[DS_AD];
+    c.ag = t[DS_AG];
+    c.gre = t[DS_GRE];
+    c.gim = t[DS_GIM];
+  }
+  return c;
+}
 
 
 // Kinetic + rotation line pass.  Along an axis with wavenumber k the operator
@@ -245,6 +265,11 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   // a workgroup's lines span instances b0 and b1 only
   const KinScal S0 = kin_scal_all(a, b0);
   const KinScal S1 = (b1 == b0) ? S0 : kin_scal_all(a, b1);
+  CombScal C0{false, 0.0, 0.0, 0.0, 0.0, 0.0}, C1 = C0;
+  if (KIND == KE_FIRST) {
+    C0 = comb_scal(a, b0, on0);
+    C1 = (b1 == b0) ? C0 : comb_scal(a, b1, on1);
+  }
   const bool brow = on0;               // this block stores a partial row
   if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
   stage_twiddles(twl, a.fp);
@@ -263,8 +288,11 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
   C x[BEC_EPT];
   {
-    // phase 1: issue every load of this thread's elements
-    double2 p0[BEC_EPT], dd[BEC_EPT];
+    // phase 1: issue every load of this thread's elements (with the new
+    // direction's z and g when this instance forms one)
+    const CombScal& OC = (ob == b0) ? C0 : C1;
+    const bool comb = KIND == KE_FIRST && oon && OC.on;
+    double2 p0[BEC_EPT], dd[BEC_EPT], zz[BEC_EPT], gg[BEC_EPT];
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
       const int e = ow.e0 + j * tpl;
@@ -272,22 +300,15 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         const long long adr = obase + e * strd;
         p0[j] = to_d2(fpsi[adr]);
         if (a.d) dd[j] = to_d2(fd[adr]);
+        if (KIND == KE_FIRST && comb) {
+          if (OC.az != 0.0) zz[j] = to_d2(((const C*)a.z)[adr]);
+          if (OC.ag != 0.0) gg[j] = to_d2(((const C*)a.g)[adr]);
+        }
       }
     }
     // new search directions (device-driven loop): d = az z + ad d + ag g + gam psi
-    if (KIND == KE_FIRST && a.d_out && oon && (a.act.fl[ob] & a.comb_bit)) {
-      const double* t = a.tab + (long long)ob * DS_N;
-      const double az = t[DS_AZ], ad = t[DS_AD], ag = t[DS_AG], gre = t[DS_GRE], gim = t[DS_GIM];
-      double2 zz[BEC_EPT], gg[BEC_EPT];
-#pragma unroll
-      for (int j = 0; j < BEC_EPT; ++j) {
-        const int e = ow.e0 + j * tpl;
-        if (e < N) {
-          const long long adr = obase + e * strd;
-          if (az != 0.0) zz[j] = to_d2(((const C*)a.z)[adr]);
-          if (ag != 0.0) gg[j] = to_d2(((const C*)a.g)[adr]);
-        }
-      }
+    if (KIND == KE_FIRST && comb) {
+      const double az = OC.az, ad = OC.ad, ag = OC.ag, gre = OC.gre, gim = OC.gim;
 #pragma unroll
       for (int j = 0; j < BEC_EPT; ++j) {
         const int e = ow.e0 + j * tpl;
