@@ -324,29 +324,93 @@ def mode_validate(cfg):
 
 
 # -----------------------------------------------------------------------------
+# tf_only: the Thomas-Fermi envelope alone
+# -----------------------------------------------------------------------------
+def mode_tf_only(cfg):
+    """modes.py:264-291 (without its figures): the TF envelope of the
+    configuration, appended to tf_only.csv; None when there is none."""
+    rid = run_id(cfg) + "_tf"
+    G = make_grid(cfg)
+    tf = extract_tf(cfg, G)
+    if tf is None:
+        print("[tf_only] no TF reference for these parameters")
+        return None
+    print(f"[tf_only] kind={tf.get('kind')} R90={tf.get('R90'):.4f} mu={tf.get('mu'):.5f} "
+          f"rho0={tf.get('rho0', float('nan'))} converged={tf.get('converged')}")
+    records.write_csv(os.path.join(records.out_dir(), "tf_only.csv"),
+                      [dict(rid=rid, kind=tf.get("kind"), R90=tf.get("R90"), R99=tf.get("R99"),
+                            mu=tf.get("mu"), rho0=tf.get("rho0", float("nan")))])
+    return tf
+
+
+# -----------------------------------------------------------------------------
+# convergence: refine N, box and padding at fixed physics
+# -----------------------------------------------------------------------------
+def mode_convergence(cfg):
+    """modes.py:392-436: solves at each N of conv_Ngrids, at the boxes
+    L conv_box_factors (N keeping dx, in multiples of 32, at least 96) and at
+    each padding of conv_pads; the finest grid's state seeds the later
+    variants of the same shape.  Rows carry the diagnostic bundle."""
+    rows = []
+    L0 = float(cfg["L"])
+    dx0 = 2.0 * L0 / int(cfg["Ngrid"])
+    variants = [("grid", L0, int(N), int(cfg.get("pad", 2))) for N in cfg.get("conv_Ngrids", [192, 256, 384])]
+    for f in cfg.get("conv_box_factors", [1.0, 1.2]):
+        L = L0 * f
+        N = max(96, 32 * math.ceil((2 * L / dx0) / 32))
+        variants.append(("box", L, N, int(cfg.get("pad", 2))))
+    for pp in cfg.get("conv_pads", [2, 3]):
+        variants.append(("pad", L0, int(cfg["Ngrid"]), int(pp)))
+    print(f"CONVERGENCE  {cfg['dimension']} s={cfg['s']} Omega={cfg['Omega']} beta2={cfg['beta2']} "
+          f"beta3={cfg['beta3']} G_C={cfg['G_C']}")
+    seen = set()
+    prev = None
+    for axis, L, N, pad in variants:
+        key = (round(L, 10), N, pad)
+        if key in seen:
+            continue
+        seen.add(key)
+        c = dict(cfg, L=float(L), Ngrid=int(N), pad=int(pad), tag=f"{cfg.get('tag', 'becgpp')}_{axis}_N{N}_p{pad}")
+        G = make_grid(c)
+        psi0 = prev if (prev is not None and tuple(prev.shape) == tuple(G["R2"].shape)) else None
+        psi, G, obs = ground_state(c, G=G, psi0=psi0, verbose=False)
+        d = diagnostics(psi, G, c)
+        d.update(axis=axis, L=L, N=N, pad=pad, dx=G["dx"], resid_rel=obs["resid_rel"], iters=obs["iters"],
+                 converged=obs["converged"], points_per_R90=d["R90"] / G["dx"])
+        rows.append(d)
+        if axis == "grid" and N == max(cfg.get("conv_Ngrids", [256])):
+            prev = psi.detach()
+        records.write_csv(os.path.join(records.out_dir(), "convergence.csv"), rows)
+        print(f"    {axis} L={L:.4g} N={N} p={pad}: E={d['E']:.9g} R90={d['R90']:.6g} "
+              f"vir={d['virial_rel']:.2e} ppR90={d['points_per_R90']:.1f} res={d['resid_rel']:.2e}")
+    grid = sorted((r for r in rows if r["axis"] == "grid"), key=lambda r: r["N"])
+    if len(grid) >= 2:
+        base = grid[-1]                                 # finest grid (largest N)
+        for r in grid:
+            print(f"    N={r['N']}: dE_vs_finest={r['E'] - base['E']:+.3e} "
+                  f"dR90_rel={(r['R90'] - base['R90']) / max(abs(base['R90']), 1e-30):+.3e}")
+    return rows
+
+
+# -----------------------------------------------------------------------------
 # smoke
 # -----------------------------------------------------------------------------
 def mode_smoke(cfg):
-    """modes.py:440-459: the gate at validate_N=256 and one small run."""
+    """modes.py:440-459: the gate at validate_N=256, one small run and the TF
+    envelope of that run."""
     gate = mode_validate(dict(cfg, validate_N=256))
     small = dict(cfg, L=10.0, Ngrid=(128 if geometry(cfg)[0] == 2 else 72), maxit=1500, res_tol=3e-3,
-                 nseeds=1, tag="smoke")
+                 nseeds=1, want_lll=True, tag="smoke")
     d = mode_single(small)
+    mode_tf_only(small)
     run_ok = bool(d) and bool(d.get("converged", False))
     ok = bool(gate["passed"]) and run_ok
     print(f"[smoke] validation_passed={gate['passed']} run_converged={run_ok} OVERALL={ok}")
     return dict(gate=gate, run=d, ok=ok)
 
 
-def _not_here(name):
-    def fn(cfg):
-        raise NotImplementedError(f"mode {name!r} is not part of this package (ground-state path only); "
-                                  "use the reference for it")
-    return fn
-
-
 DISPATCH = dict(validate=mode_validate, smoke=mode_smoke, single=mode_single,
-                tf_only=_not_here("tf_only"), sweep=mode_sweep, convergence=_not_here("convergence"))
+                tf_only=mode_tf_only, sweep=mode_sweep, convergence=mode_convergence)
 
 
 def run(cfg):

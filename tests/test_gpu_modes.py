@@ -70,3 +70,37 @@ def test_single_mode_writes_records_and_resumes(tmp_path, monkeypatch):
               "tf_R90", "tf_mu", "mem_estimate_gb", "points_per_R90"):
         assert k in d1 and k in header, k
     assert d1["Nv"] == 0 and d1["w_LLL"] != d1["w_LLL"] and d1["tf_kind"] == "repulsive_parabola"
+
+
+def test_tf_only_mode_writes_the_envelope(tmp_path, monkeypatch):
+    """modes.py:264-291: the TF envelope of the configuration (the oracle's
+    extract_tf), appended to tf_only.csv; None without one."""
+    import becgpp_amd as B
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
+    c = _cfg(B, mode="tf_only", beta2=100.0, tf_cache=False)
+    tf = B.run(c)
+    ref = O.extract_tf(c, O.make_grid(c))
+    assert tf["kind"] == ref["kind"] and abs(tf["mu"] - ref["mu"]) <= 1e-9 * abs(ref["mu"])
+    assert (tmp_path / "tf_only.csv").read_text().splitlines()[0].split(",") == ["R90", "R99", "kind", "mu",
+                                                                                 "rho0", "rid"]
+    # attractive contact with long range: no envelope (thomasfermi.py:86-87)
+    assert B.run(_cfg(B, mode="tf_only", beta2=-1.0, beta3=0.0, G_C=2.0, kernel="newton", tf_cache=False)) is None
+
+
+def test_convergence_mode_rows(tmp_path, monkeypatch):
+    """modes.py:392-436: one row per distinct (L, N, pad) variant with the
+    diagnostic bundle; each row equals a direct solve of that variant."""
+    import becgpp_amd as B
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
+    c = _cfg(B, mode="convergence", beta2=50.0, Ngrid=64, L=6.0, conv_Ngrids=[48, 64], conv_box_factors=[1.0, 1.25],
+             conv_pads=[2, 3], res_tol=1e-7)
+    rows = B.run(c)
+    keys = [(r["axis"], r["L"], r["N"], r["pad"]) for r in rows]
+    # boxes keep dx in multiples of 32 points, at least 96; pad 2 repeats the
+    # grid variant N = 64 and is skipped
+    assert keys == [("grid", 6.0, 48, 2), ("grid", 6.0, 64, 2), ("box", 6.0, 96, 2), ("box", 7.5, 96, 2),
+                    ("pad", 6.0, 64, 3)]
+    assert all("Nv" in r and "R_1pct" in r and r["converged"] for r in rows)
+    _, _, obs = B.ground_state(dict(c, L=7.5, Ngrid=96), verbose=False)
+    assert abs(rows[3]["E"] - obs["E"]) <= 1e-9 * abs(obs["E"])
+    assert (tmp_path / "convergence.csv").is_file()

@@ -91,14 +91,12 @@ def test_product_never_imports_the_oracle():
 
 
 def test_run_dispatch_errors():
-    """modes.py:473-478: an unknown mode is a ValueError; the modes outside the
-    ground-state path say so; sweep arguments are validated (modes.py:312-323)."""
+    """modes.py:473-478: an unknown mode is a ValueError, every reference mode
+    is dispatched; sweep arguments are validated (modes.py:312-323)."""
     import becgpp_amd as B
     with pytest.raises(ValueError):
         B.run(B.default_cfg(mode="single") | {"mode": "nope"})
-    for m in ("tf_only", "convergence"):
-        with pytest.raises(NotImplementedError):
-            B.run(B.default_cfg(mode=m))
+    assert sorted(B.DISPATCH) == ["convergence", "single", "smoke", "sweep", "tf_only", "validate"]
     with pytest.raises(ValueError):
         B.modes.mode_sweep(B.default_cfg(mode="sweep", sweep_param="seed"))
     with pytest.raises(ValueError):
