@@ -167,6 +167,8 @@ def mode_sweep(cfg):
             for ip in ips:
                 cands = [(f"seed:{cj['seed']}", r) for (jp, cj), r in zip(jobs, res) if jp == ip]
                 rows[ip] = _pick(cands, pts[ip], param, values[ip])
+        for ip in range(len(pts)):
+            _sweep_record(cfg, param, pts, rows, ip)
         return rows
     prev, G_prev = None, None
     for ip, c in enumerate(pts):
@@ -186,7 +188,20 @@ def mode_sweep(cfg):
             cands.append((f"seed:{seeds[best]['seed']}", res[best]))
         rows[ip] = _pick(cands, c, param, values[ip])
         prev, G_prev = rows[ip]["psi"], G
+        _sweep_record(cfg, param, pts, rows, ip)
     return rows
+
+
+def _sweep_record(cfg, param, pts, rows, ip):
+    """modes.py:370-380: point ip's checkpoint ckpt/<rid>.pt, and the sweep's
+    CSV rewritten with the points done so far (sweep_<param>.csv, or
+    sweep_<tag>_<param>.csv for a tagged sweep)."""
+    tag0 = cfg.get("tag", "becgpp")
+    stem = param if tag0 == "becgpp" else f"{tag0}_{param}"
+    plain = [{k: v for k, v in r.items() if k != "psi"} for r in rows[:ip + 1] if r is not None]
+    if cfg.get("save_ckpt", True):
+        records.save_checkpoint(rows[ip]["rid"], rows[ip]["psi"], pts[ip], plain[-1])
+    records.write_csv(os.path.join(records.out_dir(), f"sweep_{stem}.csv"), plain)
 
 
 def _pick(cands, c, param, val):

@@ -41,13 +41,22 @@ def test_sweep_without_continuation_is_batched_seeds():
         assert abs(r["E"] - oo["E"]) <= 1e-8 * abs(oo["E"]), (r["E"], oo["E"])
 
 
-def test_sweep_with_continuation_keeps_lowest_candidate():
+def test_sweep_with_continuation_keeps_lowest_candidate(tmp_path, monkeypatch):
+    """modes.py:295-387: each point keeps its lowest-energy candidate; the
+    point's checkpoint and the sweep CSV are written as the sweep goes."""
     import becgpp_amd as B
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
     c = _cfg(B, mode="sweep", sweep_param="Omega", sweep_values=[0.0, 0.2], nseeds=2, beta2=30.0)
     rows = B.run(c)
     assert rows[1]["branch"] in ("continuation", "seed:tf", "seed:gaussian")
     cand = [float(x.split("=")[1]) for x in rows[1]["candidate_E"].split(";")]   # printed to 10 digits
     assert abs(rows[1]["E"] - min(cand)) <= 1e-9 * abs(min(cand))
+    lines = (tmp_path / "sweep_Omega.csv").read_text().splitlines()
+    assert len(lines) == 3 and "Nv" in lines[0].split(",") and "psi" not in lines[0].split(",")
+    for r in rows:
+        assert (tmp_path / "ckpt" / f"{r['rid']}.pt").is_file()
+        psi = B.records.load_checkpoint_psi(r["rid"])
+        assert torch.equal(psi, r["psi"].detach().cpu())
 
 
 def test_single_mode_writes_records_and_resumes(tmp_path, monkeypatch):
