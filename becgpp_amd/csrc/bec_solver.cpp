@@ -283,33 +283,48 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   HIPCHK(hipEventRecord(ev[1], s));
   HIPCHK(hipStreamWaitEvent(cs, ev[1], 0));
-  const int K = 4;                    // steps per polling chunk
+  // steps per chunk: 4, and 16 once the loop has run K_LONG_AFTER steps (a
+  // long solve then launches a quarter of the graphs; a short one wastes at
+  // most two chunks of no-op steps after its last instance stops)
+  const int K = 4, K_LONG = 16;
+  const long long K_LONG_AFTER = 256;
   const bool use_graph = !prof_active();
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr, graph_l = nullptr;
+  hipGraphExec_t gexec = nullptr, gexec_l = nullptr;
   p.stream = cs;
   // every step resolves a trial of each running instance: maxit iterations of
   // at most linesearch_max trials bound the number of steps
   const long long max_steps = (long long)(cf->maxit + 2) * (long long)(std::max(1, cf->linesearch_max) + 1) + 64;
   try {
-    if (use_graph) {
+    auto capture = [&](int k_steps, hipGraph_t* g, hipGraphExec_t* ge) {
       HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        for (int k = 0; k < K; ++k) step();
+        for (int k = 0; k < k_steps; ++k) step();
       } catch (...) {
         hipGraph_t g2 = nullptr;
         (void)hipStreamEndCapture(cs, &g2);
         if (g2) (void)hipGraphDestroy(g2);
         throw;
       }
-      HIPCHK(hipStreamEndCapture(cs, &graph));
-      HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    }
+      HIPCHK(hipStreamEndCapture(cs, g));
+      HIPCHK(hipGraphInstantiate(ge, *g, nullptr, nullptr, 0));
+    };
+    if (use_graph) capture(K, &graph, &gexec);
+    long long steps = 0;
     for (long long chunk = 0;; ++chunk) {
-      if (use_graph) {
+      if (use_graph && steps >= K_LONG_AFTER) {
+        if (!gexec_l) {                  // captured once the loop runs long, on an idle stream
+          HIPCHK(hipStreamSynchronize(cs));
+          capture(K_LONG, &graph_l, &gexec_l);
+        }
+        HIPCHK(hipGraphLaunch(gexec_l, cs));
+        steps += K_LONG;
+      } else if (use_graph) {
         HIPCHK(hipGraphLaunch(gexec, cs));
+        steps += K;
       } else {
         for (int k = 0; k < K; ++k) step();
+        steps += K;
       }
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(ev[chunk & 1], cs));
@@ -320,7 +335,7 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
         for (int b = 0; b < nb; ++b) ndone += ((volatile int*)p.hdone)[b] != 0;
         if (ndone == nb) break;
       }
-      if ((chunk + 1) * K > max_steps) throw BecError(2, "internal: the device loop did not finish");
+      if (steps > max_steps) throw BecError(2, "internal: the device loop did not finish");
     }
     HIPCHK(hipStreamSynchronize(cs));
   } catch (...) {
@@ -328,6 +343,8 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     p.stream = s;
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (gexec_l) (void)hipGraphExecDestroy(gexec_l);
+    if (graph_l) (void)hipGraphDestroy(graph_l);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     throw;
@@ -335,6 +352,8 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   p.stream = s;
   if (gexec) (void)hipGraphExecDestroy(gexec);
   if (graph) (void)hipGraphDestroy(graph);
+  if (gexec_l) (void)hipGraphExecDestroy(gexec_l);
+  if (graph_l) (void)hipGraphDestroy(graph_l);
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   if (cf->verbose) vlog_flush(p, nb, printed);
