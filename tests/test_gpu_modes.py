@@ -10,7 +10,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _cfg(B, **kw):
-    return B.default_cfg(**dict(dict(L=6.0, Ngrid=48, res_tol=1e-8, maxit=20000), **kw))
+    # res_tol 1e-7: a solve to 1e-8 sits at the FP64 residual floor of these
+    # small grids and may stop as stalled just above it
+    return B.default_cfg(**dict(dict(L=6.0, Ngrid=48, res_tol=1e-7, maxit=20000), **kw))
 
 
 def test_validate_mode_passes():
