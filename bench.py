@@ -68,7 +68,8 @@ WORKLOADS = {
         desc="quasi-2D q2d kernel l_z=0.5, beta2=50, L=16, Omega in {0,0.5} x G_C in {1,2,5,10}, nseeds=4 "
              "(32 solves), N = 128/192/256 round-robin by instance id, one batch per grid (the three run "
              "concurrently), res_tol=1e-6, "
-             "linesearch_max=20"),
+             "linesearch_max=20",
+        replay_iters=250),
     # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
     "droplet_sweep_2d_n256": dict(
         cfg=dict(mode="single", dimension="2D", s=2, trap_coeff=0.0, Omega=0.0, beta2=-120.0, beta3=1000.0,
@@ -79,7 +80,8 @@ WORKLOADS = {
         desc="2D cubic-quintic droplet sweep, no trap, beta2 = 16 values in [-200, -40], beta3=1e3, "
              "L=20, N=256, nseeds=4 (64 solves), FP32 warm phase to res 1e-4 then FP64 to res_tol=1e-6, "
              "linesearch_max=20",
-        check=("hf", "beta2")),
+        check=("hf", "beta2"),
+        replay_iters=100),
 }
 
 # watched kernel: launch-site id -> (name, algorithmic bytes per instance of its
@@ -112,8 +114,9 @@ def parse():
     ap.add_argument("--watch", type=int, default=None,
                     help="launch-site id of the roofline kernel (default: the accepted-state H pass, "
                          "114 in 2D, 144 in 3D)")
-    ap.add_argument("--measure-iters", type=int, default=300,
-                    help="CG iterations of the event-timed replay of the roofline kernel")
+    ap.add_argument("--measure-iters", type=int, default=None,
+                    help="CG iterations of the event-timed replay of the roofline kernel (default: the "
+                         "workload's replay_iters, else 300)")
     ap.add_argument("--cpu-sample-iters", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--maxit", type=int, default=None,
@@ -252,6 +255,12 @@ def main():
     # The timed steps replay hipGraphs, whose launches cannot carry timing
     # events; the roofline kernel is timed with HIP events on its stream in a
     # replay of the same batched workload with direct launches.
+    # the replay runs the workload's solves from their seeds for fewer CG
+    # iterations than they take to converge (droplets ~130, q2d ~280 per
+    # solve), so the timed launches are those of solves in progress rather than
+    # of instances retrying line searches at the FP64 floor past res_tol
+    if a.measure_iters is None:
+        a.measure_iters = int(wl.get("replay_iters", 300))
     lib.bec_prof_watch(int(a.watch))
     k0 = groups[gk0][:64]
     B.ground_state_batch([dict(cs[k], maxit=int(a.measure_iters), res_tol=0.0, warm_fp32=False) for k in k0], G=G,
