@@ -287,6 +287,11 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   C* s = lds + ow.cc * lpitch<X::contig>(N);
   double loc[4] = {0.0, 0.0, 0.0, 0.0};
   C x[BEC_EPT];
+  // loads of the H passes' epilogue; on 2D axes they are issued before the
+  // transforms (in 3D the extra registers would halve the waves per SIMD)
+  constexpr bool EARLY = (KIND == KH_MID || KIND == KH_LAST) && X::ndim == 2;
+  double2 ps[BEC_EPT], acc[BEC_EPT], old[BEC_EPT];
+  double phv[BEC_EPT];
   {
     // phase 1: issue every load of this thread's elements (with the new
     // direction's z and g when this instance forms one)
@@ -351,6 +356,21 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         }
       }
       x[j] = from_d2<C>(v);
+      if (EARLY && KIND == KH_LAST) ps[j] = to_d2(x[j]);    // the state as written to psi_w
+    }
+  }
+  if (EARLY && oon) {
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int e = ow.e0 + j * tpl;
+      if (e >= N) continue;
+      const long long adr = obase + e * strd;
+      acc[j] = a.acc ? to_d2(((const C*)a.acc)[adr]) : make_double2(0.0, 0.0);
+      if (KIND == KH_LAST) {
+        if (!a.psi_w) ps[j] = to_d2(fpsi[adr]);
+        phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
+        if (a.ydiff) old[j] = to_d2(fout[adr]);
+      }
     }
   }
   fft_line<-1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
@@ -391,13 +411,11 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   if (oon) {
-    double2 ps[BEC_EPT], acc[BEC_EPT], old[BEC_EPT];
-    double phv[BEC_EPT];
-    // phase 1: loads of the epilogue
+    // phase 1: loads of the epilogue (issued earlier on 2D axes)
 #pragma unroll
     for (int j = 0; j < BEC_EPT; ++j) {
       const int e = ow.e0 + j * tpl;
-      if (e >= N) continue;
+      if (EARLY || e >= N) continue;
       const long long adr = obase + e * strd;
       if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? to_d2(((const C*)a.acc)[adr]) : make_double2(0.0, 0.0);
       if (KIND == KH_LAST) {
