@@ -8,6 +8,7 @@ Sources (under the reference root):
   benchmarks/results/verification/pcg_variants.csv  converged energies at res 1e-6
   benchmarks/results/verification/kernels_log_q2d.csv  log/q2d/thin-disc energies
   benchmarks/results/verification/sweep_omega_Omega.csv  rotation sweep: E, L_z, radii, vortex counts, w_LLL
+  benchmarks/results/verification/sweep_gc2d_G_C.csv    G_C sweep (newton, Omega = 1, autobox): the same columns
 """
 import csv
 import json
@@ -41,6 +42,8 @@ def main():
             "R_1pct", "peak", "resid_rel", "iters", "branch", "tf_R90", "tf_R99", "tf_mu", "points_per_R90",
             "mem_estimate_gb")
     out["sweep_omega"] = [{k: r[k] for k in keep} for r in rows("benchmarks/results/verification/sweep_omega_Omega.csv")]
+    out["sweep_gc2d"] = [{k: r[k] for k in keep[1:] + ("G_C", "L", "N")}
+                         for r in rows("benchmarks/results/verification/sweep_gc2d_G_C.csv")]
     out["_source"] = "becGPP 1.2.1 benchmarks/results (reference repository)"
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)

@@ -191,3 +191,31 @@ def test_sweep_autobox_continues_across_grids(tmp_path, monkeypatch):
     assert [(r["L"], r["N"]) for r in rows] == [tuple(g) for g in grids]
     assert "continuation" in rows[1]["candidate_E"]
     assert rows[1]["converged"]
+
+
+@pytest.mark.parametrize("gc", [32.0, 16.0, 10.0])
+def test_gc_sweep_points_match_reference(gc):
+    """Points of the reference's G_C sweep (benchmarks/verification.py S3: 2D,
+    newton kernel, Omega = 1, beta2 = 100, autobox grid, three seeds, lowest
+    energy kept, res_tol 1e-4 as there): auto_grid gives the published box
+    and grid, the state carries the same vortices, and E, w_LLL, the radii
+    and the Thomas-Fermi envelope agree with the published row."""
+    import becgpp_amd as B
+    row = next(r for r in GOLD["sweep_gc2d"] if float(r["G_C"]) == gc)
+    c = B.default_cfg(mode="single", dimension="2D", s=2, Omega=1.0, beta2=100.0, beta3=0.0, kernel="newton",
+                      G_C=gc, nseeds=3, maxit=80000)
+    L, N, _ = B.auto_grid(c)
+    assert (L, N) == (float(row["L"]), int(row["N"]))
+    c["L"], c["Ngrid"] = L, N
+    psi, G, obs = B.ground_state_multiseed(c)
+    d = B.diagnostics(psi, G, c)
+    print({k: d[k] for k in ("E", "Nv", "w_LLL", "R90")}, obs.get("seed_energies"), flush=True)
+    assert (d["Nv"], d["Nplus"], d["Nminus"]) == tuple(int(row[k]) for k in ("Nv", "Nplus", "Nminus"))
+    # both sides stop at resid ~1e-4 (the states differ by round-off
+    # trajectories and the seeds' noise): E to 1e-5, in practice 1e-11..2e-6
+    assert abs(d["E"] - float(row["E"])) <= 1e-5 * abs(float(row["E"]))
+    assert abs(d["w_LLL"] - float(row["w_LLL"])) <= 2e-3
+    for k in ("R50", "R90", "R99"):
+        assert abs(d[k] - float(row[k])) <= float(G["dx"]) + 1e-9, k
+    for k in ("tf_R90", "tf_R99", "tf_mu"):
+        assert abs(d[k] - float(row[k])) <= 1e-9 * abs(float(row[k])), k
