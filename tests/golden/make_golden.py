@@ -9,6 +9,7 @@ Sources (under the reference root):
   benchmarks/results/verification/kernels_log_q2d.csv  log/q2d/thin-disc energies
   benchmarks/results/verification/sweep_omega_Omega.csv  rotation sweep: E, L_z, radii, vortex counts, w_LLL
   benchmarks/results/verification/sweep_gc2d_G_C.csv    G_C sweep (newton, Omega = 1, autobox): the same columns
+  data/sample_output/scan/scan_singles.csv               single runs of the 1.0 scan: label, configuration, E, R90, Nv
 """
 import csv
 import json
@@ -44,6 +45,9 @@ def main():
     out["sweep_omega"] = [{k: r[k] for k in keep} for r in rows("benchmarks/results/verification/sweep_omega_Omega.csv")]
     out["sweep_gc2d"] = [{k: r[k] for k in keep[1:] + ("G_C", "L", "N")}
                          for r in rows("benchmarks/results/verification/sweep_gc2d_G_C.csv")]
+    out["scan_singles"] = [{k: r[k] for k in ("label", "dimension", "kernel", "N", "L", "beta2", "beta3", "G_C",
+                                               "Omega", "s", "seed", "nseeds", "E", "R90", "Nv", "resid_rel")}
+                           for r in rows("data/sample_output/scan/scan_singles.csv")]
     out["_source"] = "becGPP 1.2.1 benchmarks/results (reference repository)"
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
