@@ -596,7 +596,9 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     p.invNdx = 1.0 / (N * p.dx);
     p.stream = 0;
     make_fft(N, p.fpN, &p.twN, p.stream);
-    make_fft(p.M, p.fpM, &p.twM, p.stream);
+    // the padded transforms exist only for M <= BEC_MAX_L; a larger grid
+    // solves without long range and refuses a kernel spectrum (conv_check)
+    if (p.M <= BEC_MAX_L) make_fft(p.M, p.fpM, &p.twM, p.stream);
     // batched reducing passes need one instance's lines (n / N of them) to fill
     // whole workgroups; keep the power-of-two thread count where L/8 would not
     const int nlk = (int)(p.n / N);
@@ -611,7 +613,7 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
       p.fpN.wide = 1;
     }
     p.lcN = line_cfg(p.fpN);
-    p.lcM = line_cfg(p.fpM);
+    p.lcM = p.twM ? line_cfg(p.fpM) : p.lcN;
     plan_reserve(p, 1);
     *out = h;
     return 0;
@@ -684,6 +686,13 @@ static void finish_kernel(Plan& p, int which, double2* full) {
 // exact self-cell averages (interactions.py:37-43, 130-133, 146)
 static const double SELF_NEWTON_3D = 2.3800773640;
 
+// long-range kernels need the padded transforms (pad N <= BEC_MAX_L)
+static void conv_check(const Plan& p) {
+  if (!p.twM)
+    bad("the padded length pad*N = " + std::to_string(p.M) + " exceeds " + std::to_string(BEC_MAX_L) +
+        ": a long-range kernel needs pad*N <= " + std::to_string(BEC_MAX_L));
+}
+
 extern "C" int bec_plan_build_kernel(bec_plan* h, int which, int kind) {
   BEC_TRY({
     if (!h) bad("null plan");
@@ -691,6 +700,7 @@ extern "C" int bec_plan_build_kernel(bec_plan* h, int which, int kind) {
     if (which != BEC_WHICH_K && which != BEC_WHICH_W) bad("which must be K or W");
     if (kind != BEC_KERNEL_NEWTON && kind != BEC_KERNEL_LOG) bad("device-built kernels: newton or log");
     if (kind == BEC_KERNEL_LOG && p.ndim != 2) bad("kernel='log' is two-dimensional only");
+    conv_check(p);
     double2* full = nullptr;
     HIPCHK(hipMalloc(&full, sizeof(double2) * p.nM));
     try {
@@ -723,6 +733,7 @@ extern "C" int bec_plan_set_kernel_table(bec_plan* h, int which, const double* k
     if (!h || !ker_real) bad("null argument");
     Plan& p = h->p;
     if (which != BEC_WHICH_K && which != BEC_WHICH_W) bad("which must be K or W");
+    conv_check(p);
     double2* full = nullptr;
     HIPCHK(hipMalloc(&full, sizeof(double2) * p.nM));
     try {

@@ -95,8 +95,10 @@ const char* bec_last_error(void);
 /* Plan: grid geometry, FFT twiddles, workspaces, kernel spectra.
  * Replaces make_grid (grid.py:69-101) for the device side: x_i = (i-N/2) dx,
  * dx = 2L/N, k = 2 pi fftfreq(N, dx), V = trap_coeff r^s (0 if s = 0),
- * padded length M = pad N.  N and pad N must factor into 2^a 3^b 5^c 7^d
- * 11^e 13^f with pad N <= 1024; ndim is 2 or 3. */
+ * padded length M = pad N.  N <= 1024 must factor into 2^a 3^b 5^c 7^d
+ * 11^e 13^f; the long-range kernels also need pad N <= 1024 (a plan with a
+ * larger pad N solves without long range, and bec_plan_build_kernel /
+ * bec_plan_set_kernel_table refuse it); ndim is 2 or 3. */
 int bec_plan_create(int ndim, int N, double L, int pad, double s, double trap_coeff, bec_plan** out);
 int bec_plan_destroy(bec_plan* p);
 int bec_plan_set_stream(bec_plan* p, void* hip_stream);

@@ -10,6 +10,7 @@ Sources (under the reference root):
   benchmarks/results/verification/sweep_omega_Omega.csv  rotation sweep: E, L_z, radii, vortex counts, w_LLL
   benchmarks/results/verification/sweep_gc2d_G_C.csv    G_C sweep (newton, Omega = 1, autobox): the same columns
   data/sample_output/scan/scan_singles.csv               single runs of the 1.0 scan: label, configuration, E, R90, Nv
+  benchmarks/results/verification/cq_ladder.csv          self-bound cubic-quintic ladder at N = 1024: E, mu, peak
 """
 import csv
 import json
@@ -48,6 +49,8 @@ def main():
     out["scan_singles"] = [{k: r[k] for k in ("label", "dimension", "kernel", "N", "L", "beta2", "beta3", "G_C",
                                                "Omega", "s", "seed", "nseeds", "E", "R90", "Nv", "resid_rel")}
                            for r in rows("data/sample_output/scan/scan_singles.csv")]
+    out["cq_ladder"] = [{k: float(r[k]) for k in ("beta2", "beta3", "E", "mu", "peak", "R90", "virial_rel", "resid")}
+                        for r in rows("benchmarks/results/verification/cq_ladder.csv")]
     out["_source"] = "becGPP 1.2.1 benchmarks/results (reference repository)"
     with open(OUT, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)

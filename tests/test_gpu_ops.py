@@ -258,3 +258,17 @@ def test_autotuned_schedule_keeps_parity(kw):
     e_ref = O.energy_components(psi, Go, c["beta2"], c["beta3"], c["Omega"], c["G_C"])
     e_got = B.energy_components(_dev(psi), Gd, c["beta2"], c["beta3"], c["Omega"], c["G_C"])
     assert abs(e_got["E"].item() - e_ref["E"]) <= TOL * max(1.0, abs(e_ref["E"]))
+
+
+def test_long_range_refused_beyond_padded_limit():
+    """A grid whose padded length 2N exceeds 1024 solves without long range
+    (the N = 1024 ladder) but refuses a kernel spectrum with a ValueError."""
+    import becgpp_amd as B
+    c = cfg(dimension="2D", Ngrid=576, L=8.0, beta2=30.0, G_C=2.0, kernel="newton")
+    G = B.make_grid(c)
+    psi = torch.ones((576, 576), dtype=torch.complex128, device="cuda")
+    with pytest.raises(ValueError, match="padded length"):
+        B.apply_H(psi, G, c)
+    c0 = cfg(dimension="2D", Ngrid=576, L=8.0, beta2=30.0)
+    ref = O.apply_H(np.ones((576, 576), complex), O.make_grid(c0), c0)
+    assert relerr(B.apply_H(psi, B.make_grid(c0), c0).cpu().numpy(), ref) < TOL
