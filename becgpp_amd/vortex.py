@@ -4,7 +4,6 @@ counted on their z = 0 midplane."""
 import ctypes
 import math
 
-import torch
 
 from . import _lib
 from .config import default_cfg
