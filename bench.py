@@ -152,7 +152,6 @@ def main():
         print(json.dumps(dict(error="this build runs on one GPU: the ground-state path has no multi-GPU "
                                     "code; run with --gpus 1")))
         sys.exit(2)
-    import numpy as np
     import torch
     import becgpp_amd as B
     from becgpp_amd import _lib
