@@ -37,8 +37,8 @@ def radial_average(f, G, center=None, rmax=None, bins=None):
     e = torch.as_tensor(edges, dtype=r.dtype, device=r.device)
     idx = (torch.bucketize(r, e, right=True) - 1).clamp(0, bins - 1)
     ff = f.detach().real.reshape(-1).to(r.dtype)
-    s = torch.zeros(bins, dtype=r.dtype, device=r.device).index_add_(0, idx, ff)
-    c = torch.zeros(bins, dtype=r.dtype, device=r.device).index_add_(0, idx, torch.ones_like(ff))
+    s = torch.bincount(idx, weights=ff, minlength=bins).to(r.dtype)
+    c = torch.bincount(idx, minlength=bins).to(r.dtype)
     prof = torch.where(c > 0, s / c.clamp(min=1.0), torch.full_like(s, float("nan")))
     return ctr, prof.cpu().numpy()
 
