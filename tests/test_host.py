@@ -113,6 +113,20 @@ def test_run_id_is_the_reference_formula():
     assert B.run_id(dict(c, maxit=5, mode="sweep")) == a
     assert B.run_id(dict(c, beta2=1.0)) != a
     assert B.run_id(dict(c, tag="x")) == "x_" + a.split("_")[1]
+    # the reference's tests/test_audit.py:175-176
+    assert B.run_id(dict(c, seed_winding=0)) != B.run_id(dict(c, seed_winding=2))
+
+
+def test_smoke_ok_reflects_convergence(monkeypatch):
+    """The reference's tests/test_audit.py:207-211 (modes.py:440-459): smoke is
+    ok only when the gate passed and its run converged."""
+    import becgpp_amd as B
+    monkeypatch.setattr(B.modes, "mode_validate", lambda cfg: dict(passed=True))
+    monkeypatch.setattr(B.modes, "mode_single", lambda cfg: dict(converged=False, E=0.0))
+    monkeypatch.setattr(B.modes, "mode_tf_only", lambda cfg: None)
+    assert B.modes.mode_smoke(B.default_cfg())["ok"] is False
+    monkeypatch.setattr(B.modes, "mode_single", lambda cfg: dict(converged=True, E=0.0))
+    assert B.modes.mode_smoke(B.default_cfg())["ok"] is True
 
 
 def test_build_extension_keys():
