@@ -331,6 +331,7 @@ def mode_validate(cfg):
     passed = (mg < 5e-3 and mp < 5e-3 and mlg < 5e-3 and mlp < 5e-3 and mq < 5e-3
               and ml < 1e-4 and mtf < 1e-12 and m3 < 5e-3)
     order_ok = all(o > 1.7 for o in kernel_orders.values())
+    records.write_csv(os.path.join(records.out_dir(), "validation.csv"), rows)     # modes.py:191
     print(f"[validate] 2D-Coulomb Eg={mg:.2e} phi0={mp:.2e} | 2D-log Eg={mlg:.2e} phi0={mlp:.2e} | "
           f"q2D={mq:.2e} | LLL={ml:.2e} | flatTF={mtf:.2e} | 3D-phi0={m3:.2e} | "
           f"kernel_order={kernel_orders} | PASS={passed and order_ok}")

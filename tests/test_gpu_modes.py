@@ -15,11 +15,19 @@ def _cfg(B, **kw):
     return B.default_cfg(**dict(dict(L=6.0, Ngrid=48, res_tol=1e-7, maxit=20000), **kw))
 
 
-def test_validate_mode_passes():
+def test_validate_mode_passes(tmp_path, monkeypatch):
+    """The reference's tests/test_validation.py: the gate passes, its error
+    magnitudes are within bounds, and validation.csv is written."""
     import becgpp_amd as B
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
     g = B.run(B.default_cfg(mode="validate", validate_N=256))
     assert g["passed"], g
     assert g["ml"] < 1e-12 and g["mtf"] < 1e-12
+    for k in ("mg", "mp", "mlg", "mlp", "mq", "m3"):
+        assert g[k] < 5e-3, k
+    assert all(g["kernel_orders"][k] > 1.7 for k in ("2D", "2D_log", "3D"))
+    head = (tmp_path / "validation.csv").read_text().splitlines()[0].split(",")
+    assert {"test", "case", "quantity", "rel_error"} <= set(head)
 
 
 def test_single_mode_equals_multiseed():
