@@ -93,3 +93,28 @@ def test_quasi2d_auto_kernel():
     assert B.make_grid(B.default_cfg(dimension="quasi2D", G_C=1.0, l_z=0.3, Ngrid=32))["kernel"] == "q2d"
     assert B.make_grid(B.default_cfg(dimension="quasi2D", G_C=1.0, l_z=0.0, Ngrid=32))["kernel"] == "newton"
     assert B.make_grid(B.default_cfg(dimension="2D", G_C=1.0, Ngrid=32))["kernel"] == "log"
+
+
+def test_single_ground_state_converges():
+    """tests/test_smoke.py:24-36: grid -> operators -> solver -> diagnostics ->
+    Thomas-Fermi on a small grid."""
+    import math
+    import becgpp_amd as B
+    cfg = B.default_cfg(mode="single", dimension="2D", s=2, Omega=0.0, beta2=100, beta3=0, G_C=0, kernel="none",
+                        L=10, Ngrid=96, seed="tf", maxit=4000, res_tol=1e-4, save_figs=False, save_ckpt=False)
+    G = B.make_grid(cfg)
+    psi, G, obs = B.ground_state(cfg, G=G, verbose=False)
+    assert obs["converged"] and obs["resid_rel"] < 2e-4
+    d = B.diagnostics(psi, G, cfg)
+    assert d["tf_kind"] == "repulsive_parabola"
+    assert math.isfinite(d["E"]) and d["virial_rel"] < 1e-2
+
+
+def test_flattop_tf_exact():
+    """tests/test_smoke.py:39-45: the self-bound flat top rho0 = -3 b2 / (4 b3)."""
+    import becgpp_amd as B
+    cfg = B.default_cfg(dimension="2D", s=2, Omega=1.0, beta2=-1.0, beta3=1.0, G_C=0, kernel="none", L=20,
+                        Ngrid=192)
+    tf = B.extract_tf(cfg, B.make_grid(cfg))
+    assert tf is not None and tf["kind"] == "local_flattop"
+    assert abs(tf["rho0"] - 0.75) < 1e-9
