@@ -18,5 +18,6 @@ from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, 
                       solve_native_batch)
 
 from .modes import run, run_id, DISPATCH
+from .units import physical_scales, sim_params_from_physical, outputs_to_physical
 
 __version__ = "0.1.0"

@@ -10,3 +10,7 @@ torch.set_default_dtype(torch.float64)
 
 DEV = "cuda"
 CODE_VERSION = "becGPP_1.2.1"
+
+# CODATA 2018 (becgpp/constants.py:15-16)
+HBAR = 1.054571817e-34       # J s
+G_NEWTON = 6.67430e-11       # m^3 kg^-1 s^-2

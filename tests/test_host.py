@@ -171,3 +171,37 @@ def test_run_records_round_trip(tmp_path, monkeypatch):
     records.write_run_json("becgpp_abc", cfg, obs, "cuda")
     j = json.load(open(tmp_path / "becgpp_abc.json"))
     assert j["cfg"]["beta2"] == 1.0 and j["diagnostics"]["Elist"] == [1.0, 2.0] and "code_version" in j
+
+
+def test_units_maps():
+    """units.py:27-91 and the reference's tests/test_smoke.py:48-52: the trap
+    length of 87Rb at omega = 2 pi 100 Hz is 1.078 um (a textbook value), the
+    3D contact map is 4 pi N a_s / a_ho, the pancake map sqrt(8 pi) N a_s / l_z,
+    self-gravity G m^2 N / (a_ho hbar omega); outputs scale back by hbar omega
+    and a_ho; unknown regimes and pancakes without omega_z are ValueErrors."""
+    import math
+    import becgpp_amd as B
+    m, w = 86.909180527 * 1.66053906660e-27, 2.0 * math.pi * 100.0
+    sc = B.physical_scales(m, w)
+    assert abs(sc["a_ho_m"] - 1.0785e-6) < 5e-10
+    a_s, N = 5.3e-9, 1e5
+    out = B.sim_params_from_physical("bec3d", m, w, N, a_s=a_s, verbose=False)
+    assert out["dimension"] == "3D" and math.isclose(out["beta2"], 4 * math.pi * N * a_s / sc["a_ho_m"], rel_tol=1e-14)
+    wz = 2.0 * math.pi * 1000.0
+    pc = B.sim_params_from_physical("bec2d_pancake", m, w, N, a_s=a_s, omega_z=wz, verbose=False)
+    lz = math.sqrt(1.054571817e-34 / (m * wz))
+    assert pc["dimension"] == "quasi2D" and math.isclose(pc["beta2"], math.sqrt(8 * math.pi) * N * a_s / lz,
+                                                         rel_tol=1e-14)
+    g = B.sim_params_from_physical("selfgrav3d", m=1e-26, omega_perp=100.0, N=1e5, a_s=0.0, verbose=False)
+    assert g["dimension"] == "3D" and g["kernel"] == "newton" and g["G_C"] > 0.0 and g["beta2"] == 0.0
+    a = math.sqrt(1.054571817e-34 / (1e-26 * 100.0))
+    assert math.isclose(g["G_C"], 6.67430e-11 * 1e-52 * 1e5 / (a * 1.054571817e-34 * 100.0), rel_tol=1e-14)
+    gp = B.sim_params_from_physical("selfgrav_pancake", 1e-26, 100.0, 1e5, omega_z=1e4, verbose=False)
+    assert gp["kernel"] == "q2d" and math.isclose(gp["l_z"], math.sqrt(1.054571817e-34 / (1e-26 * 1e4)) / a,
+                                                  rel_tol=1e-14)
+    o = B.outputs_to_physical(sc, E=2.0, mu=3.0, R=4.0)
+    assert math.isclose(o["E_J"], 2.0 * 1.054571817e-34 * w) and math.isclose(o["R_m"], 4.0 * sc["a_ho_m"])
+    with pytest.raises(ValueError):
+        B.sim_params_from_physical("bec1d", m, w, N, verbose=False)
+    with pytest.raises(ValueError):
+        B.sim_params_from_physical("bec2d_pancake", m, w, N, a_s=a_s, verbose=False)
