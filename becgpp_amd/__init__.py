@@ -19,5 +19,6 @@ from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, 
 
 from .modes import run, run_id, DISPATCH
 from .units import physical_scales, sim_params_from_physical, outputs_to_physical
+from . import paths
 
 __version__ = "0.1.0"

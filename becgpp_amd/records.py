@@ -16,10 +16,11 @@ import os
 import torch
 
 from .constants import CODE_VERSION
+from . import paths
 
 
 def out_dir():
-    base = os.environ.get("GPP_OUTDIR") or "./becgpp_out"
+    base = paths.base_dir()
     os.makedirs(os.path.join(base, "ckpt"), exist_ok=True)
     return base
 

@@ -205,3 +205,15 @@ def test_units_maps():
         B.sim_params_from_physical("bec1d", m, w, N, verbose=False)
     with pytest.raises(ValueError):
         B.sim_params_from_physical("bec2d_pancake", m, w, N, a_s=a_s, verbose=False)
+
+
+def test_paths_configure(tmp_path, monkeypatch):
+    """paths.py: GPP_OUTDIR, then configure(base) at run time; the records go
+    where BASE points."""
+    import becgpp_amd as B
+    monkeypatch.setattr(B.paths, "_configured", None)
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path / "env"))
+    assert B.paths.BASE == str(tmp_path / "env") and B.records.out_dir() == str(tmp_path / "env")
+    B.paths.configure(tmp_path / "cfg")
+    assert B.paths.BASE == str(tmp_path / "cfg") and B.paths.CKPT_DIR == str(tmp_path / "cfg" / "ckpt")
+    assert B.records.out_dir() == str(tmp_path / "cfg") and (tmp_path / "cfg" / "ckpt").is_dir()
