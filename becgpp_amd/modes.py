@@ -425,6 +425,16 @@ def mode_smoke(cfg):
     return dict(gate=gate, run=d, ok=ok)
 
 
+def zip_results(name="becgpp_results"):
+    """modes.py:462-466: a zip archive of the output directory, beside it."""
+    import shutil
+    base = records.out_dir()
+    parent = os.path.dirname(os.path.abspath(base))
+    arch = shutil.make_archive(os.path.join(parent, name), "zip", root_dir=base)
+    print(f"[zip] {arch} ({os.path.getsize(arch) / 1e6:.1f} MB)")
+    return arch
+
+
 DISPATCH = dict(validate=mode_validate, smoke=mode_smoke, single=mode_single,
                 tf_only=mode_tf_only, sweep=mode_sweep, convergence=mode_convergence)
 

@@ -43,6 +43,11 @@ def mass_quantiles(rho, G, fractions=(0.50, 0.90, 0.99), center=None):
     return out
 
 
+def mass_radius(rho, G, fraction=0.99, center=None):
+    """fields.py:33-34: the radius enclosing the given mass fraction."""
+    return mass_quantiles(rho, G, (fraction,), center=center)[0]
+
+
 def tf_effective_potential(G, cfg):
     """V - Omega^2 (x^2 + y^2)/2, centrifugal term in-plane only (thomasfermi.py:21-30)."""
     O = float(cfg.get("Omega", 1.0))

@@ -107,6 +107,18 @@ def test_droplet_radius_matches_oracle():
     assert B.droplet_radius(_dev(rho), Gd) == O.droplet_radius(rho, Go)
 
 
+def test_mass_radius_is_the_quantile():
+    """fields.py:33-34: mass_radius is mass_quantiles at one fraction."""
+    import becgpp_amd as B
+    c, Go, Gd = _grids(dimension="2D", Ngrid=128, L=6.0)
+    psi, _ = random_state(Go, seed=7)
+    rho = _dev(np.abs(psi) ** 2)
+    q = B.mass_quantiles(rho, Gd, (0.5, 0.99))
+    assert B.mass_radius(rho, Gd) == q[1]
+    assert B.mass_radius(rho, Gd, fraction=0.5) == q[0]
+    assert 0.0 < q[0] < q[1]
+
+
 @pytest.mark.parametrize("omega", [0.5, 0.7, 0.85])
 def test_rotating_state_diagnostics_match_reference_sweep(omega):
     """Points of the reference's rotation sweep (2D, s = 2, beta2 = 200, L = 12,

@@ -217,3 +217,24 @@ def test_paths_configure(tmp_path, monkeypatch):
     B.paths.configure(tmp_path / "cfg")
     assert B.paths.BASE == str(tmp_path / "cfg") and B.paths.CKPT_DIR == str(tmp_path / "cfg" / "ckpt")
     assert B.records.out_dir() == str(tmp_path / "cfg") and (tmp_path / "cfg" / "ckpt").is_dir()
+
+
+def test_reference_exports(tmp_path, monkeypatch):
+    """Every name of the reference's becgpp.__all__ (becgpp/__init__.py) that
+    belongs to the path is exported; zip_results archives the output
+    directory (modes.py:462-466)."""
+    import zipfile
+    import becgpp_amd as B
+    for name in ("CODE_VERSION", "DEV", "CFG_DEFAULTS", "default_cfg", "paths", "make_grid", "geometry",
+                 "resolve_kernel", "auto_grid", "resample_state", "long_range_phi", "kernel_convolution",
+                 "q2d_kernel", "clear_kernel_cache", "energy_components", "apply_H", "compute_residual",
+                 "observables", "mass_quantiles", "mass_radius", "radial_average", "droplet_radius", "extract_tf",
+                 "make_seed", "ground_state", "ground_state_multiseed", "vortex_diagnostic", "lll_weight",
+                 "diagnostics", "physical_scales", "sim_params_from_physical", "outputs_to_physical", "run",
+                 "DISPATCH", "zip_results", "mode_validate", "mode_smoke", "mode_single", "mode_tf_only",
+                 "mode_sweep", "mode_convergence"):
+        assert hasattr(B, name), name
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path / "out"))
+    B.records.write_csv(os.path.join(B.records.out_dir(), "x.csv"), [dict(a=1)])
+    arch = B.zip_results("arch")
+    assert zipfile.ZipFile(arch).namelist().count("x.csv") == 1
