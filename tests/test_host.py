@@ -238,3 +238,19 @@ def test_reference_exports(tmp_path, monkeypatch):
     B.records.write_csv(os.path.join(B.records.out_dir(), "x.csv"), [dict(a=1)])
     arch = B.zip_results("arch")
     assert zipfile.ZipFile(arch).namelist().count("x.csv") == 1
+
+
+def test_reference_audit_host_cases():
+    """The reference's host-only cases for the path: an empty sweep returns no
+    rows and an unknown sweep key is refused (tests/test_audit.py:197-204); the
+    defaults are safe: no rotation, no long-range term, a small grid
+    (tests/test_kernels.py:75-78); G_C carries one factor of N, like
+    beta2 = g N (tests/test_kernels.py:81-85)."""
+    import becgpp_amd as B
+    assert B.modes.mode_sweep(B.default_cfg(mode="sweep", sweep_values=[])) == []
+    with pytest.raises(ValueError):
+        B.modes.mode_sweep(B.default_cfg(mode="sweep", sweep_param="omega", sweep_values=[0.0], Ngrid=32))
+    assert B.CFG_DEFAULTS["Omega"] == 0.0 and B.CFG_DEFAULTS["G_C"] == 0.0 and B.CFG_DEFAULTS["Ngrid"] <= 128
+    a = B.sim_params_from_physical("selfgrav3d", m=1e-26, omega_perp=100.0, N=1e4, verbose=False)
+    b = B.sim_params_from_physical("selfgrav3d", m=1e-26, omega_perp=100.0, N=2e4, verbose=False)
+    assert abs(b["G_C"] / a["G_C"] - 2.0) < 1e-12
