@@ -101,16 +101,21 @@ def extract_tf(cfg, G, iters=400, relax=0.4):
         for _ in range(int(iters)):
             phi = long_range_phi(n, G, gc)
             base = Veff + phi
-            mu = _bisect_mu_for_norm(base, b2, b3, G, base.min().item(),
-                                     base.max().item() + abs(b2) + abs(b3) + 1.0)
+            bmin, bmax = torch.stack(torch.aminmax(base)).tolist()      # one synchronisation for both
+            mu = _bisect_mu_for_norm(base, b2, b3, G, bmin, bmax + abs(b2) + abs(b3) + 1.0)
             if not math.isfinite(mu):
                 break
             n_new = _invert_local(mu, base, b2, b3)
-            ssum = n_new.sum().item() * dv
+            s_t = n_new.sum() * dv
+            # torch divides by a host scalar as a product with its reciprocal:
+            # the same product here keeps the iterates bit for bit
+            n_new = n_new * (1.0 / s_t)
+            # the norm and the change fetched together; a non-positive norm
+            # leaves n as it was, as when it is tested before the division
+            ssum, dn_t = torch.stack([s_t, (n_new - n).abs().sum() * dv]).tolist()
             if ssum <= 0:
                 break
-            n_new = n_new / ssum
-            dn = (n_new - n).abs().sum().item() * dv
+            dn = dn_t
             n = (1 - relax) * n + relax * n_new
             n = n / (n.sum() * dv)
             if dn < 1e-9:
