@@ -12,13 +12,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbecgpp_hip.so")
-SOURCES = ["bec_kernels.hip", "bec_ctrl.hip", "bec_plan.cpp", "bec_solver.cpp", "bec_prof.cpp"]
+SOURCES = ["bec_ctrl.hip", "bec_plan.cpp", "bec_solver.cpp", "bec_prof.cpp"]
+# bec_kernels.hip is compiled once per translation unit (its BEC_TU parts: the
+# elementwise kernels, one unit per kinetic-pass axis, two for the line passes)
+KERNEL_TUS = 8
 ARCH = os.environ.get("BECGPP_ARCH", "gfx950")
 
 FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17",
          # packed FP32 instructions are disabled for all device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-result", "-Wno-unused-value"]
+         "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-function"]
 
 
 def _hipcc():
@@ -28,9 +31,12 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src):
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc(), *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(job):
+    src, tu = job
+    stem = os.path.splitext(src)[0] + ("" if tu is None else f"_tu{tu}")
+    obj = os.path.join(OBJ, stem + ".o")
+    defs = [] if tu is None else [f"-DBEC_TU={tu}"]
+    cmd = [_hipcc(), *FLAGS, *defs, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -39,8 +45,10 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    jobs = [("bec_kernels.hip", k) for k in range(KERNEL_TUS)] + [(src, None) for src in SOURCES]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 8))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

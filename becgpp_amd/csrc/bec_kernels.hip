@@ -12,6 +12,15 @@
 
 #define TWO_PI 6.283185307179586
 
+// The file is compiled once per translation unit BEC_TU = 0..7 (the build
+// runs them in parallel): 0 holds the elementwise kernels and the launch
+// dispatch, 1..5 the kinetic passes of one axis mode each, 6 and 7 the
+// generic line passes.  Without BEC_TU every part is compiled.
+#ifndef BEC_TU
+#define BEC_TU -1
+#endif
+#define BEC_TU_IS(k) (BEC_TU < 0 || BEC_TU == (k))
+
 __device__ __forceinline__ double kfreq(int j, int N, double invNdx) {
   const int f = (j < (N + 1) / 2) ? j : j - N;
   return TWO_PI * ((double)f * invNdx);
@@ -647,6 +656,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   if (ST == ST_CDOT && brow) block_partials<16, 0>(zv, nullptr, a.partials, a.hpartials, lb);
 }
 
+#if BEC_TU_IS(0)
 // ---------------------------------------------------------------------------
 // elementwise kernels: grid (chunks, instances); reducing ones store row
 // blockIdx.y * gridDim.x + blockIdx.x
@@ -1198,6 +1208,8 @@ int lpw_min(int tpl) {
   while ((l * tpl) % 64 != 0 && l * tpl < BEC_WG) l *= 2;
   return l;
 }
+#endif  // BEC_TU_IS(0)
+
 static int pick_lpw(int tpl, int nl, int total, bool strided) {
   const int lmin = lpw_min(tpl);
   const int lmax = BEC_WG / tpl > lmin ? BEC_WG / tpl : lmin;
@@ -1300,11 +1312,17 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   return (a.nl + a.lpw - 1) / a.lpw;
 }
 
-int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
+// launch_pass: the cases are compiled in two translation units (BEC_TU 6:
+// plain transforms and the preconditioner; 7: smoothing and convolution); each
+// returns -1 for a case it does not hold.
 #define BEC_P(C, LDV, D, MU, S)                                                 \
   if (contig == (C) && ld == (LDV) && dir == (D) && mul == (MU) && st == (S)) { \
     return go_pass<C, LDV, D, MU, S>(a, s);                                     \
   }
+int launch_pass_plain(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
+int launch_pass_conv(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s);
+#if BEC_TU_IS(6)
+int launch_pass_plain(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
   // plain transforms (kernel spectra, preconditioner)
   BEC_P(1, LD_C, DIR_FWD, MUL_NONE, ST_C)
   BEC_P(0, LD_C, DIR_FWD, MUL_NONE, ST_C)
@@ -1314,6 +1332,11 @@ int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a,
   // preconditioner middle pass
   BEC_P(0, LD_C, DIR_FMI, MUL_PRE2, ST_C)
   BEC_P(0, LD_C, DIR_FMI, MUL_PRE3, ST_C)
+  return -1;
+}
+#endif
+#if BEC_TU_IS(7)
+int launch_pass_conv(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
   // Gaussian smoothing (vortex diagnostic)
   BEC_P(1, LD_C, DIR_FMI, MUL_GAUSS, ST_C)
   BEC_P(0, LD_C, DIR_FMI, MUL_GAUSS, ST_C)
@@ -1322,10 +1345,18 @@ int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a,
   BEC_P(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C)
   BEC_P(0, LD_C, DIR_FMI, MUL_KHAT3, ST_C)
   BEC_P(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE)
-#undef BEC_P
-  abort();
-  return 0;
+  return -1;
 }
+#endif
+#undef BEC_P
+#if BEC_TU_IS(0)
+int launch_pass(int contig, int ld, int dir, int mul, int st, const PassArgs& a, hipStream_t s) {
+  int r = launch_pass_plain(contig, ld, dir, mul, st, a, s);
+  if (r < 0) r = launch_pass_conv(contig, ld, dir, mul, st, a, s);
+  if (r < 0) abort();
+  return r;
+}
+#endif
 
 template <int AX, int KIND>
 static int go_kin(const KinArgs& a0, hipStream_t s) {
@@ -1360,23 +1391,45 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   return (a.nl + a.lpw - 1) / a.lpw;
 }
 
-int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
+// launch_kin: one translation unit per axis mode (BEC_TU 1..5)
 #define BEC_K(AXV)                                                  \
-  if (ax == AXV) {                                                  \
-    switch (kind) {                                                 \
-      case KE_FIRST: return go_kin<AXV, KE_FIRST>(a, s);            \
-      case KE_REST: return go_kin<AXV, KE_REST>(a, s);              \
-      case KH_FIRST: return go_kin<AXV, KH_FIRST>(a, s);            \
-      case KH_MID: return go_kin<AXV, KH_MID>(a, s);                \
-      default: return go_kin<AXV, KH_LAST>(a, s);                   \
-    }                                                               \
+  switch (kind) {                                                   \
+    case KE_FIRST: return go_kin<AXV, KE_FIRST>(a, s);              \
+    case KE_REST: return go_kin<AXV, KE_REST>(a, s);                \
+    case KH_FIRST: return go_kin<AXV, KH_FIRST>(a, s);              \
+    case KH_MID: return go_kin<AXV, KH_MID>(a, s);                  \
+    default: return go_kin<AXV, KH_LAST>(a, s);                     \
   }
-  BEC_K(A2Y)
-  BEC_K(A2X)
-  BEC_K(A3Z)
-  BEC_K(A3Y)
-  BEC_K(A3X)
+int launch_kin_a2y(int kind, const KinArgs& a, hipStream_t s);
+int launch_kin_a2x(int kind, const KinArgs& a, hipStream_t s);
+int launch_kin_a3z(int kind, const KinArgs& a, hipStream_t s);
+int launch_kin_a3y(int kind, const KinArgs& a, hipStream_t s);
+int launch_kin_a3x(int kind, const KinArgs& a, hipStream_t s);
+#if BEC_TU_IS(1)
+int launch_kin_a2y(int kind, const KinArgs& a, hipStream_t s) { BEC_K(A2Y) }
+#endif
+#if BEC_TU_IS(2)
+int launch_kin_a2x(int kind, const KinArgs& a, hipStream_t s) { BEC_K(A2X) }
+#endif
+#if BEC_TU_IS(3)
+int launch_kin_a3z(int kind, const KinArgs& a, hipStream_t s) { BEC_K(A3Z) }
+#endif
+#if BEC_TU_IS(4)
+int launch_kin_a3y(int kind, const KinArgs& a, hipStream_t s) { BEC_K(A3Y) }
+#endif
+#if BEC_TU_IS(5)
+int launch_kin_a3x(int kind, const KinArgs& a, hipStream_t s) { BEC_K(A3X) }
+#endif
 #undef BEC_K
+#if BEC_TU_IS(0)
+int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s) {
+  switch (ax) {
+    case A2Y: return launch_kin_a2y(kind, a, s);
+    case A2X: return launch_kin_a2x(kind, a, s);
+    case A3Z: return launch_kin_a3z(kind, a, s);
+    case A3Y: return launch_kin_a3y(kind, a, s);
+    case A3X: return launch_kin_a3x(kind, a, s);
+  }
   abort();
   return 0;
 }
@@ -1726,3 +1779,5 @@ void launch_extract_khat(int ndim, int M, int H, const double2* kf, double* khat
   const long long n = (ndim == 2) ? (long long)H * H : (long long)H * H * H;
   hipLaunchKernelGGL(k_extract_khat, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, ndim, M, H, kf, khat);
 }
+
+#endif  // BEC_TU_IS(0)
