@@ -71,6 +71,9 @@ SIGNATURES = {
     "bec_resample": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                     ctypes.c_void_p]),
+    "bec_resample_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "bec_vortex_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_int)]),

@@ -164,6 +164,17 @@ int bec_lll_weight(bec_plan* p, const void* psi, double Omega, int Mmax, double*
 int bec_resample(int ndim, int n0, double dx0, const void* psi_old, int n1, double dx1, void* psi_new,
                  double* norm_out, void* stream);
 
+/* bec_resample without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels, results and errors as bec_resample; its partial
+ * sums live in a fixed 1024-row device array of the library (one per GPU,
+ * held until the process ends) instead of a buffer allocated per call, and a
+ * grid needing more rows is refused (status 1).  Calls that overlap on one
+ * GPU share that array and race. */
+int bec_resample_noalloc(int ndim, int n0, double dx0, const void* psi_old, int n1, double dx1, void* psi_new,
+                         double* norm_out, void* stream);
+
 /* Vortex count of a 2D state (vortex.py:10-48, _count_vortices_2d): the phase
  * winding of every grid plaquette, counted where the plaquette's mean density
  * smoothed by exp(-ell^2 k^2 / 2) exceeds frac times its maximum and its first
