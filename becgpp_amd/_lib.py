@@ -29,7 +29,7 @@ class SolverCfg(ctypes.Structure):
                 ("linesearch_max", ctypes.c_int), ("linesearch_stall", ctypes.c_int),
                 ("precond_shift_min", ctypes.c_double), ("cg_beta", ctypes.c_int),
                 ("cg_restart", ctypes.c_int), ("verbose", ctypes.c_int), ("check", ctypes.c_int),
-                ("host_loop", ctypes.c_int), ("warm_res_tol", ctypes.c_double)]
+                ("host_loop", ctypes.c_int), ("warm_res_tol", ctypes.c_double), ("line_poly", ctypes.c_int)]
 
 
 class SolveOut(ctypes.Structure):

@@ -42,7 +42,10 @@ CFG_DEFAULTS = dict(
 #                 per iteration) instead of the device controllers
 #   autotune:     time the candidate FFT stage orders and lines per workgroup
 #                 on the first solve of a plan and keep the fastest
-CFG_EXTENSIONS = dict(warm_fp32=False, warm_res_tol=1e-4, host_loop=False, autotune=False)
+#   line_search:  "poly" prices every Armijo trial of an iteration from one
+#                 pass of line coefficients (one loop step per iteration);
+#                 "trial" evaluates one trial per loop step
+CFG_EXTENSIONS = dict(warm_fp32=False, warm_res_tol=1e-4, host_loop=False, autotune=False, line_search="poly")
 
 _ALIASES = {"res": "res_tol", "etol": "energy_tol", "N": "Ngrid", "dim": "dimension",
             "precond_shift": "precond_shift_min"}

@@ -263,6 +263,140 @@ __device__ __forceinline__ void ctrl_armijo_scalar(const CtrlArgs& c, int b, uns
   c.fl[b] = (f & FL_BEST) | FL_FAIL | FL_DIR;
 }
 
+// ---------------------------------------------------------------------------
+// k_ctrl_line: the whole Armijo backtracking of one iteration (solvers.py:
+// 160-196) from the line coefficients of this step.  With p = |psi|^2,
+// q = Re(conj(psi) d), r = |d|^2 the trial u = psi - t d has the density
+// p - 2 t q + t^2 r and the norm n(t) = 1 - 2 t <q> + t^2 <r> (psi has unit
+// norm), and its energy after normalisation is
+//   E(t) = QL(t)/n + (beta2/2) Q4(t)/n^2 + (beta3/3) Q6(t)/n^3 - (G_C/2) P(t)/n^2
+// with QL = <u,(T + V - Omega L_z) u>, Q4 = sum rho^2, Q6 = sum rho^3,
+// P = sum rho (K*rho) (all times dV).  Their values at t = 0 are the state's,
+// carried from the trial that produced it (L00, Spp, Sppp, Kpp); the cross
+// term of QL uses <psi, L d> = <H psi, d> - sum (beta2 p + beta3 p^2 + phi) q
+// with H psi = g + mu psi.  The reference's sequence of trials alpha,
+// alpha/2, ... and its acceptance test then run here, in the same order, on
+// these energies: every trial of the line costs no further pass over the
+// fields.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ctrl_line_scalar(const CtrlArgs& c, int b, unsigned f, DevInst* q, const double* e0,
+                                                 const double* e1, const double* e2, const double* kq,
+                                                 const double* kr) {
+  const double dv = c.dv;
+  const InstPhys ph = c.iphys[b];
+  double kin = e0[0] + e1[0], lz = e0[1] + e1[1];
+  if (c.ndim == 3) { kin += e2[0]; lz += e2[1]; }
+  const double a1 = e0[2] * dv, a2 = e0[3] * dv;
+  const double L11 = (kin - ph.Omega * lz + e0[4]) * dv;
+  const double L01 = q->gd + q->mu * a1 - (ph.beta2 * e0[5] + ph.beta3 * e0[10] + e0[19]) * dv;
+  // sum rho^2: Spp - 4t pq + t^2 (4 qq + 2 pr) - 4t^3 qr + t^4 rr
+  const double c41 = -4.0 * e0[5] * dv, c42 = (4.0 * e0[6] + 2.0 * e0[7]) * dv, c43 = -4.0 * e0[8] * dv,
+               c44 = e0[9] * dv;
+  // sum rho^3 with rho = p + t B + t^2 r, B = -2 q
+  const double c61 = -6.0 * e0[10] * dv, c62 = (3.0 * e0[11] + 12.0 * e0[12]) * dv,
+               c63 = (-8.0 * e0[13] - 12.0 * e0[14]) * dv, c64 = (12.0 * e0[15] + 3.0 * e0[16]) * dv,
+               c65 = -6.0 * e0[17] * dv, c66 = e0[18] * dv;
+  // sum rho (K*rho): Kpp - 4t Kpq + t^2 (4 Kqq + 2 Kpr) - 4t^3 Kqr + t^4 Krr; the
+  // stored potential is phi = -G_C K*p, so K(p,q) = sum phi q / (-G_C)
+  const bool lr = (f & FL_SLR) != 0u;
+  double cp1 = 0.0, cp2 = 0.0, cp3 = 0.0, cp4 = 0.0;
+  if (lr) {
+    const double ig = 1.0 / (-ph.G_C);
+    cp1 = -4.0 * e0[19] * dv * ig;
+    cp2 = 4.0 * kq[0] * dv + 2.0 * e0[20] * dv * ig;
+    cp3 = -4.0 * kq[1] * dv;
+    cp4 = kr[0] * dv;
+  }
+  double t = q->trial;
+  bool accepted = false;
+  double Et = 0.0, Ec = 0.0, E3 = 0.0, Eg = 0.0, nt = 1.0, QL = 0.0, Q4 = 0.0, Q6 = 0.0, P = 0.0;
+  for (int ls = 0; ls < c.ls_max; ++ls) {
+    nt = 1.0 + t * (-2.0 * a1 + t * a2);
+    const double i1 = 1.0 / nt, i2 = i1 * i1, i3 = i2 * i1;
+    QL = q->L00 + t * (-2.0 * L01 + t * L11);
+    Q4 = q->Spp + t * (c41 + t * (c42 + t * (c43 + t * c44)));
+    Q6 = q->Sppp + t * (c61 + t * (c62 + t * (c63 + t * (c64 + t * (c65 + t * c66)))));
+    P = lr ? q->Kpp + t * (cp1 + t * (cp2 + t * (cp3 + t * cp4))) : 0.0;
+    Ec = 0.5 * ph.beta2 * Q4 * i2;
+    E3 = (ph.beta3 / 3.0) * Q6 * i3;
+    Eg = lr ? 0.5 * (-ph.G_C) * P * i2 : 0.0;
+    Et = QL * i1 + Ec + E3 + Eg;
+    q->n_energy += 1;
+    if (isfinite(Et) && Et <= q->E - c.c1 * t * q->gd) {
+      accepted = true;
+      break;
+    }
+    t *= c.shrink;
+    if (t < c.alpha_min) break;
+  }
+  q->trial = t;
+  double* tb = c.tab + (long long)b * DS_N;
+  if (accepted) {                                                 // (solvers.py:186-196)
+    const double i1 = 1.0 / nt, i2 = i1 * i1;
+    q->n_fail = 0;
+    q->zg_prev = q->zg_cur;
+    q->gg_prev = q->gg_cur;
+    q->E = Et;
+    q->mu = Et + Ec + 2.0 * E3 + Eg;
+    q->alpha = fmin(c.alpha_max, t * c.growth);
+    q->L00 = QL * i1;
+    q->Spp = Q4 * i2;
+    q->Sppp = Q6 * i2 * i1;
+    q->Kpp = P * i2;
+    tb[DS_T] = t;
+    tb[DS_SC] = sqrt(nt);
+    tb[DS_MU] = q->mu;
+    tb[DS_PHISC] = 1.0;
+    // phi of the new state: -G_C K*(p - 2t q + t^2 r) / n = (phi + 2t G_C K*q - t^2 G_C K*r) / n
+    tb[DS_CQ] = 2.0 * t * ph.G_C;
+    tb[DS_CR] = -t * t * ph.G_C;
+    tb[DS_CS] = i1;
+    q->pending = 1;
+    q->it_acc = q->it;
+    q->trial_acc = t;
+    q->it += 1;
+    c.fl[b] = (f & FL_BEST) | FL_ACC | FL_DIR | (lr ? FL_PHIC : 0u);
+    return;
+  }
+  // line search failed: direction = z_p, psi unchanged (solvers.py:175-185)
+  q->n_fail += 1;
+  q->alpha = fmax(q->alpha * 0.5, c.alpha_min);
+  tb[DS_AZ] = 1.0; tb[DS_AD] = 0.0; tb[DS_AG] = 0.0; tb[DS_GRE] = -q->cz_re; tb[DS_GIM] = -q->cz_im;
+  q->it += 1;
+  if (q->alpha <= c.alpha_min || q->n_fail >= c.stall_max) {
+    q->stop = BEC_STOP_LINE_SEARCH_STALLED;
+    q->done = 1;
+    c.hdone[b] = 1;
+    if (c.verbose) vlog_put(c, b, q, VLOG_STALL, q->it - 1, q->E, q->resid_rel, q->best_resid, q->trial);
+    c.fl[b] = f & FL_BEST;
+    return;
+  }
+  c.fl[b] = (f & FL_BEST) | FL_FAIL | FL_DIR;
+}
+
+__global__ __launch_bounds__(CTRL_WG) void k_ctrl_line(CtrlArgs c) {
+  __shared__ double sh[CTRL_WG];
+  __shared__ double e0[32], e1[2], e2[2], kq[2], kr[1];
+  __shared__ DevInst qs;
+  const int b = blockIdx.x;
+  const unsigned f = c.fl[b];
+  if (!(f & FL_SEARCH)) {             // finished this step: only a pending best copy remains
+    if (threadIdx.x == 0) c.fl[b] = f & FL_BEST;
+    return;
+  }
+  inst_load(&qs, c.st + b);
+  fold_rows<32>(c.r_e0 + (long long)b * c.n_e0 * 32, c.n_e0, 21, e0, sh);
+  fold_rows<2>(c.r_e1 + (long long)b * c.n_e1 * 2, c.n_e1, 2, e1, sh);
+  if (c.ndim == 3) fold_rows<2>(c.r_e2 + (long long)b * c.n_e2 * 2, c.n_e2, 2, e2, sh);
+  if (f & FL_SLR) {
+    fold_rows<2>(c.r_kq + (long long)b * c.n_kq * 2, c.n_kq, 2, kq, sh);
+    fold_rows<1>(c.r_kr + (long long)b * c.n_kr, c.n_kr, 1, kr, sh);
+  }
+  if (threadIdx.x == 0) ctrl_line_scalar(c, b, f, &qs, e0, e1, e2, kq, kr);
+  __syncthreads();
+  inst_store(c.st + b, &qs);
+}
+
 __global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
   __shared__ double sh[CTRL_WG];
   __shared__ double r[16], hl[4];
@@ -308,4 +442,7 @@ void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s) {
 }
 void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s) {
   hipLaunchKernelGGL(k_ctrl_armijo, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
+}
+void launch_ctrl_line(const CtrlArgs& c, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl_line, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
 }

@@ -18,7 +18,8 @@
 enum AxisMode { A2Y = 0, A2X = 1, A3Z = 2, A3Y = 3, A3X = 4 };
 
 // kinetic / rotation / energy pass kinds
-enum KinKind { KE_FIRST = 0, KE_REST = 1, KH_FIRST = 2, KH_MID = 3, KH_LAST = 4 };
+enum KinKind { KE_FIRST = 0, KE_REST = 1, KH_FIRST = 2, KH_MID = 3, KH_LAST = 4,
+               KL_FIRST = 5 };   // KL_FIRST: line coefficients of the polynomial Armijo search (first axis)
 
 // generic line-FFT pass options
 enum LoadKind { LD_C = 0, LD_R = 1, LD_HERM = 2 };
@@ -55,11 +56,14 @@ enum InstFlag : unsigned {
   FL_FAIL = 16,     // the line search failed: direction <- preconditioned gradient
   FL_ACC = 32,      // the trial was accepted: psi and the gradient are updated
   FL_BEST = 64,     // the state is the lowest-residual one so far: keep a copy
+  FL_PHIC = 128,    // polynomial search accepted with long range: combine the new potential
 };
 
 // per-instance scalars of the device-driven loop, DS_N doubles per instance
+// (DS_CQ, DS_CR, DS_CS: the accepted state's potential (phi + CQ K*q + CR K*r) CS
+// in the polynomial search, q = Re(conj(psi) d), r = |d|^2)
 enum DevScal { DS_T = 0, DS_SC = 1, DS_MU = 2, DS_PHISC = 3, DS_AZ = 4, DS_AD = 5, DS_AG = 6, DS_GRE = 7,
-               DS_GIM = 8, DS_N = 16 };
+               DS_GIM = 8, DS_CQ = 9, DS_CR = 10, DS_CS = 11, DS_N = 16 };
 
 // Arguments of the generic line-FFT pass (pad-on-load / crop-on-store).
 struct PassArgs {
@@ -79,6 +83,7 @@ struct PassArgs {
   const double* sigma; int Nk; double invNdx;   // per-instance Sobolev shifts and grid frequencies
   double gl2;                          // MUL_GAUSS: the line's factor exp(-gl2 k^2)
   const double* dot;                   // optional: rows of sum(out_re * dot) (dot laid out like out)
+  const double* dot2;                  // optional with dot: a second sum(out_re * dot2) in the same rows
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
   unsigned long long* ucount;          // measurement: algorithmic bytes of the instances covered
@@ -98,7 +103,8 @@ struct KinArgs {
   const double2* psi; const double2* d;                  // loaded value (psi - t[b] d) / sc[b]
   double2* psi_w;                                        // KH_FIRST: loaded value written back
   double2* sdiff;                                        // KH_FIRST: loaded value - psi
-  double* rho_out;                                       // KE_FIRST: |loaded|^2
+  double* rho_out;                                       // KE_FIRST: |loaded|^2; KL_FIRST: Re(conj(psi) d)
+  double* r_out;                                         // KL_FIRST: |d|^2
   const double2* acc; double2* out;
   double2* ydiff;                                        // KH_LAST: new out - previous out
   double kf, rf; int rf_omega;                           // multiplier kf k^2 + rf c k (rf = -Omega[b])
@@ -175,7 +181,8 @@ struct Plan {
   double* hred; double* hred_dev;    // pinned results [instance][RSTRIDE]
   // solver state, sized for scap instances
   int scap;
-  double2* st[8]; double* stphi[2];
+  double2* st[8];
+  double* stphi[6];                  // phi (scaled), raw trial convolution; polynomial search: q, r, K*q, K*r
   float2* st32[8];                   // the same fields in complex64 (FP32 warm phase)
   int f32;                           // 1 while an FP32 phase runs: device-loop passes use st32
   // device-driven loop state, sized for scap instances
@@ -200,8 +207,10 @@ enum RedSlot {
 };
 
 // partial-row regions, one per reducing launch site
+// C_LINE (rows of up to 32 values: the polynomial search's first pass) is the
+// last site and spans two regions; the device buffer holds C_COUNT + 1.
 enum PartSite { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DOTS, C_MOM,
-                C_RES2, C_COUNT };
+                C_RES2, C_EGRAV2, C_LINE, C_COUNT };
 
 // ---------------------------------------------------------------------------
 // device-driven solver loop (bec_ctrl.hip): per-instance state of the loop of
@@ -211,6 +220,10 @@ enum PartSite { C_EFIRST = 0, C_EREST0, C_EREST1, C_EGRAV, C_HLAST, C_NORM, C_DO
 struct DevInst {
   double E, mu, resid_rel, best_resid, alpha, sigma, zg_prev, gg_prev, zg_cur, gg_cur, E_ref;
   double trial, gd, trial_acc, cz_re, cz_im;
+  // polynomial search: sums of the current (normalised) state, carried from
+  // the accepted trial: <psi,(T + V - Omega L_z) psi> dv, sum rho^2 dv,
+  // sum rho^3 dv, sum rho (K*rho) dv
+  double L00, Spp, Sppp, Kpp;
   double clk0;
   long long n_energy;
   int it, it_ref, n_fail, n_iter, stop, it_acc, ls, converged, done, pending, lr, n_tr, n_log, pad_;
@@ -229,6 +242,8 @@ struct CtrlArgs {
   const double* r_e1; int n_e1;          // second axis: 2 sums
   const double* r_e2; int n_e2;          // third axis (3D): 2 sums
   const double* r_gr; int n_gr;          // sum phi_raw rho (1 per row)
+  const double* r_kq; int n_kq;          // polynomial search: sum q K*q, sum r K*q (2 per row)
+  const double* r_kr; int n_kr;          //   sum r K*r (1 per row)
   const double* sigma;                   // Sobolev shift per instance
   const InstPhys* iphys;
   DevInst* st; unsigned* fl; double* tab;
@@ -242,6 +257,11 @@ struct CtrlArgs {
 enum { VLOG_PROGRESS = 1, VLOG_STALL = 2 };
 void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s);
 void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s);
+// polynomial Armijo search: every trial of the line from one pass of sums
+void launch_ctrl_line(const CtrlArgs& c, hipStream_t s);
+// accepted polynomial trials with long range: phi <- (phi + CQ kq + CR kr) CS
+void launch_phi_comb(double* phi, const double* kq, const double* kr, const double* tab, const unsigned* fl,
+                     long long n, int nb, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // launch wrappers (bec_kernels.hip); reducing launches return rows/instance

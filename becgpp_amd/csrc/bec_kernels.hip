@@ -68,9 +68,9 @@ template <int K, int KM>
 __device__ __forceinline__ void block_partials(const double* v, const double* m, double* partials,
                                                double* hpart, long long row) {
   constexpr int KT = K + KM;
-  constexpr int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : 16)));
-  static_assert(KT <= 16, "at most 16 reduced values");
-  __shared__ double red[16 * 16];
+  constexpr int KP = KT <= 1 ? 1 : (KT <= 2 ? 2 : (KT <= 4 ? 4 : (KT <= 8 ? 8 : (KT <= 16 ? 16 : 32))));
+  static_assert(KT <= 32, "at most 32 reduced values");
+  __shared__ double red[(KT <= 16 ? 16 : 32) * 16];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
@@ -485,6 +485,142 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     block_partials<2, 1>(hsum, hmax, a.partials, a.hpartials, lb);
 }
 
+// Line coefficients of the polynomial Armijo search (device-driven loop).
+// Along the search line u(t) = psi - t d every term of the energy is a
+// polynomial in t: with p = |psi|^2, q = Re(conj(psi) d), r = |d|^2 the
+// density is rho(t) = p - 2 t q + t^2 r, so sum rho^2 has degree 4, sum rho^3
+// degree 6, the norm and the quadratic forms of T + V - Omega L_z degree 2 and
+// the long-range pair sum degree 4 (with K*q and K*r).  One pass over psi and
+// d therefore prices every trial of the line search.  This is the pass of the
+// first axis: it forms the new direction d = az z + ad d + ag g + gam psi
+// (instances with the comb bit), accumulates the local sums, writes q and r
+// for the long-range convolutions (rho_out, r_out), and transforms d along
+// its axis for the kinetic and rotation forms of d (Parseval).  The other
+// axes run KE_REST on d alone.  Sums of a row (LC_K):
+//   0 kin(d) of this axis   1 L_z(d) of this axis   2 sum q   3 sum r
+//   4 sum V r   5 p q   6 q^2   7 p r   8 q r   9 r^2
+//   10 p^2 q   11 p^2 r   12 p q^2   13 q^3   14 p q r   15 q^2 r   16 p r^2
+//   17 q r^2   18 r^3   19 phi q   20 phi r     (phi: the state's potential)
+#define LC_K 21
+template <int AX, class C, bool ODD = false>
+__global__ __launch_bounds__(BEC_WG) void k_lcoef(KinArgs a) {
+  extern __shared__ double2 smem_d[];
+  C* smem = (C*)smem_d;
+  C* twl = smem;                       // twiddles [N]
+  using X = Ax<AX>;
+  C* lds = smem + a.N;                 // lines [lpw][lpitch<X::contig>(N)]
+  const C* fpsi = (const C*)a.psi;
+  const C* fd = (const C*)a.d;
+  const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int l0 = lb * lpw;
+  const int total = a.nb * nl;
+  const long long strd = X::stride(N);
+  if (l0 >= total) return;
+  const int b0 = l0 / nl, b1 = min(l0 + lpw - 1, total - 1) / nl;
+  const bool on0 = act_on(a.act, b0), on1 = (b1 == b0) ? on0 : act_on(a.act, b1);
+  if (!on0 && !on1) return;
+  const CombScal C0 = comb_scal(a, b0, on0);
+  const CombScal C1 = (b1 == b0) ? C0 : comb_scal(a, b1, on1);
+  const bool brow = on0;
+  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
+  stage_twiddles(twl, a.fp);
+  const LineOwn ow = line_owner<X::contig>(tpl, lpw);
+  const int lane = ow.e0;
+  const int ol = l0 + ow.cc;
+  const int ob = ol / nl;
+  const bool oon = ol < total && (ob == b0 ? on0 : on1);
+  const int oli = ol - ob * nl;
+  const long long obase = (long long)ob * a.istride + X::base(oli, N);
+  double c2[3];
+  line_sq<AX>(oli, N, a.dx, c2);
+  C* s = lds + ow.cc * lpitch<X::contig>(N);
+  double v[LC_K];
+#pragma unroll
+  for (int q = 0; q < LC_K; ++q) v[q] = 0.0;
+  C x[BEC_EPT];
+  {
+    const CombScal& OC = (ob == b0) ? C0 : C1;
+    const bool comb = oon && OC.on;
+    double2 p0[BEC_EPT], dd[BEC_EPT], zz[BEC_EPT], gg[BEC_EPT];
+    double fv[BEC_EPT];
+    // phase 1: every load of the thread's elements
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int e = ow.e0 + j * tpl;
+      fv[j] = 0.0;
+      if (oon && e < N) {
+        const long long adr = obase + e * strd;
+        p0[j] = to_d2(fpsi[adr]);
+        dd[j] = to_d2(fd[adr]);
+        if (comb) {
+          if (OC.az != 0.0) zz[j] = to_d2(((const C*)a.z)[adr]);
+          if (OC.ag != 0.0) gg[j] = to_d2(((const C*)a.g)[adr]);
+        }
+        if (a.phi) fv[j] = a.phi[adr];
+      }
+    }
+    // phase 2: the new direction, stored as the field's type holds it
+    if (comb) {
+      const double az = OC.az, ad = OC.ad, ag = OC.ag, gre = OC.gre, gim = OC.gim;
+#pragma unroll
+      for (int j = 0; j < BEC_EPT; ++j) {
+        const int e = ow.e0 + j * tpl;
+        if (e >= N) continue;
+        double2 r = make_double2(gre * p0[j].x - gim * p0[j].y, gre * p0[j].y + gim * p0[j].x);
+        if (az != 0.0) { r.x += az * zz[j].x; r.y += az * zz[j].y; }
+        if (ad != 0.0) { r.x += ad * dd[j].x; r.y += ad * dd[j].y; }
+        if (ag != 0.0) { r.x += ag * gg[j].x; r.y += ag * gg[j].y; }
+        const C rc = from_d2<C>(r);
+        ((C*)a.d_out)[obase + e * strd] = rc;
+        dd[j] = to_d2(rc);
+      }
+    }
+    // phase 3: the local sums and the fields of the convolutions
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int e = ow.e0 + j * tpl;
+      x[j] = cmk<C>(0, 0);
+      if (!oon || e >= N) continue;
+      const double P = cabs2(p0[j]);
+      const double Q = p0[j].x * dd[j].x + p0[j].y * dd[j].y;
+      const double R = cabs2(dd[j]);
+      const double V = line_V<AX>(c2, e, a);
+      const double PQ = P * Q, PR = P * R, QQ = Q * Q, QR = Q * R, RR = R * R;
+      v[2] += Q;        v[3] += R;        v[4] += V * R;
+      v[5] += PQ;       v[6] += QQ;       v[7] += PR;       v[8] += QR;       v[9] += RR;
+      v[10] += P * PQ;  v[11] += P * PR;  v[12] += P * QQ;  v[13] += Q * QQ;  v[14] += P * QR;
+      v[15] += Q * QR;  v[16] += P * RR;  v[17] += Q * RR;  v[18] += R * RR;
+      v[19] += fv[j] * Q;
+      v[20] += fv[j] * R;
+      if (a.rho_out) {
+        const long long adr = obase + e * strd;
+        a.rho_out[adr] = Q;
+        a.r_out[adr] = R;
+      }
+      x[j] = from_d2<C>(dd[j]);
+    }
+  }
+  fft_line<-1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
+  const double invN = 1.0 / (double)N;
+  const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
+  double kin = 0.0, lz = 0.0;
+  if (oon) {
+#pragma unroll
+    for (int j = 0; j < BEC_EPT; ++j) {
+      const int k = lane + j * tpl;
+      if (k >= N) continue;
+      const double kv = kfreq(k, N, a.invNdx);
+      const double a2 = cabs2(x[j]);
+      kin += 0.5 * kv * kv * a2;
+      lz += kv * a2;
+    }
+  }
+  v[0] = kin * invN;
+  v[1] = lz * lzc * invN;
+  if (brow) block_partials<LC_K, 0>(v, nullptr, a.partials, a.hpartials, lb);
+}
+
 // The 16 inner products of one CG step (sums, scaled by dv on the host):
 //  0,1 <psi,z>   2 <psi,psi>   3,4 <psi,d>   5,6 <psi,g>   7 Re<z,g>
 //  8 Re<z,y>   9,10 <psi,y>   11 Re<g,d>   12 <g,g>   13 Re<g,y>
@@ -613,13 +749,13 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     fft_line<+1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
   }
   // stores: line position q = lane + j tpl goes to output element q - off_store
-  double dsum[1] = {0.0};
+  double dsum[2] = {0.0, 0.0};
   double zv[ST == ST_CDOT ? 16 : 1];
 #pragma unroll
   for (int q = 0; q < (ST == ST_CDOT ? 16 : 1); ++q) zv[q] = 0.0;
   if (oon) {
     const long long obase = ob * a.out_ib + oo * a.out_so + oi * a.out_si;
-    double dv[BEC_EPT];
+    double dv[BEC_EPT], dv2[BEC_EPT];
     C fp[ST == ST_CDOT ? BEC_EPT : 1], fd[ST == ST_CDOT ? BEC_EPT : 1], fg[ST == ST_CDOT ? BEC_EPT : 1];
     C fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
 #pragma unroll
@@ -628,6 +764,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
       if (q >= L || e < 0 || e >= a.n_store) continue;
       const long long adr = obase + e * a.out_s;
       if (ST == ST_RE && a.dot) dv[j] = a.dot[adr];
+      if (ST == ST_RE && a.dot2) dv2[j] = a.dot2[adr];
       if (ST == ST_CDOT) {
         fp[j] = ((const C*)a.dpsi)[adr]; fd[j] = ((const C*)a.dd)[adr]; fg[j] = ((const C*)a.dg)[adr];
         fy[j] = ((const C*)a.dy)[adr]; fs[j] = ((const C*)a.ds)[adr];
@@ -649,10 +786,14 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         const double r = (double)w.x * a.out_scale;
         ((double*)a.out)[adr] = r;
         if (a.dot) dsum[0] += r * dv[j];
+        if (a.dot2) dsum[1] += r * dv2[j];
       }
     }
   }
-  if (ST == ST_RE && a.dot && brow) block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials, lb);
+  if (ST == ST_RE && a.dot && brow) {
+    if (a.dot2) block_partials<2, 0>(dsum, nullptr, a.partials, a.hpartials, lb);
+    else block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials, lb);
+  }
   if (ST == ST_CDOT && brow) block_partials<16, 0>(zv, nullptr, a.partials, a.hpartials, lb);
 }
 
@@ -1242,6 +1383,7 @@ static double pass_bytes(const PassArgs& a) {
   const double st_el = (ST == ST_RE) ? 8.0 : cel;
   b += lines * a.n_store * st_el;
   if (ST == ST_RE && a.dot) b += lines * a.n_store * 8.0;
+  if (ST == ST_RE && a.dot2) b += lines * a.n_store * 8.0;
   if (ST == ST_CDOT) b += lines * a.n_store * 5.0 * cel;
   return b;
 }
@@ -1391,6 +1533,45 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   return (a.nl + a.lpw - 1) / a.lpw;
 }
 
+// Line-coefficient pass of the polynomial search (first axis of a grid only).
+template <int AX>
+static int go_lcoef(const KinArgs& a0, hipStream_t s) {
+  if constexpr (AX != A2Y && AX != A3Z) {
+    abort();
+    return 0;
+  } else {
+    KinArgs a = a0;
+    const int total = a.nb * a.nl;
+    a.lpw = shifted_lpw(pick_lpw(a.tpl, a.nl, total, !Ax<AX>::contig), a.tpl, a.nl, a.lsh[0]);
+    const int blocks = xcd_grid(total, a.lpw);
+    const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
+    const size_t shm = ((size_t)a.lpw * lpitch<Ax<AX>::contig>(a.N) + a.N) * csz;
+    const int id = 100 + 10 * AX + KL_FIRST;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // psi and d read, z and g read and d written (direction steps), phi read,
+    // q and r written (long range)
+    const double n = (double)a.nl * a.N, c = (a.f32 ? 8.0 : 16.0) * n, r = 8.0 * n;
+    const double bpi = 2.0 * c + (a.d_out ? 3.0 * c : 0.0) + (a.phi ? r : 0.0) + (a.rho_out ? 2.0 * r : 0.0);
+    if (a.act.fl) {
+      a.ucount = prof_counter(id);
+      a.ubytes = (unsigned long long)bpi;
+    }
+    const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb)));
+    const dim3 grid(blocks), block(a.tpl * a.lpw);
+    if (a.f32) {
+      if (ev) hipExtLaunchKernelGGL((k_lcoef<AX, float2>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_lcoef<AX, float2>), grid, block, shm, s, a);
+    } else if (a.fp.odd) {
+      if (ev) hipExtLaunchKernelGGL((k_lcoef<AX, double2, true>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_lcoef<AX, double2, true>), grid, block, shm, s, a);
+    } else {
+      if (ev) hipExtLaunchKernelGGL((k_lcoef<AX, double2>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_lcoef<AX, double2>), grid, block, shm, s, a);
+    }
+    return (a.nl + a.lpw - 1) / a.lpw;
+  }
+}
+
 // launch_kin: one translation unit per axis mode (BEC_TU 1..5)
 #define BEC_K(AXV)                                                  \
   switch (kind) {                                                   \
@@ -1398,6 +1579,7 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
     case KE_REST: return go_kin<AXV, KE_REST>(a, s);                \
     case KH_FIRST: return go_kin<AXV, KH_FIRST>(a, s);              \
     case KH_MID: return go_kin<AXV, KH_MID>(a, s);                  \
+    case KL_FIRST: return go_lcoef<AXV>(a, s);                      \
     default: return go_kin<AXV, KH_LAST>(a, s);                     \
   }
 int launch_kin_a2y(int kind, const KinArgs& a, hipStream_t s);
@@ -1705,6 +1887,23 @@ void launch_narrow(const double2* in, float2* out, long long n, hipStream_t s) {
 }
 void launch_widen(const float2* in, double2* out, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_widen, dim3(elem_blocks(n)), dim3(BEC_WG), 0, s, in, out, n);
+}
+// phi <- (phi + CQ kq + CR kr) CS for the instances flagged FL_PHIC: the
+// potential of the accepted state of a polynomial line search, from the
+// current potential and the convolutions of q and r (coefficients in the table)
+__global__ __launch_bounds__(BEC_WG) void k_phi_comb(double* phi, const double* __restrict__ kq,
+                                                     const double* __restrict__ kr, const double* __restrict__ tab,
+                                                     const unsigned* __restrict__ fl, long long n) {
+  const int b = blockIdx.y;
+  if (!(fl[b] & FL_PHIC)) return;
+  const double* t = tab + (long long)b * DS_N;
+  const double cq = t[DS_CQ], cr = t[DS_CR], cs = t[DS_CS];
+  const long long off = (long long)b * n;
+  ELEM_LOOP(n) phi[off + i] = (phi[off + i] + cq * kq[off + i] + cr * kr[off + i]) * cs;
+}
+void launch_phi_comb(double* phi, const double* kq, const double* kr, const double* tab, const unsigned* fl,
+                     long long n, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_phi_comb, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, phi, kq, kr, tab, fl, n);
 }
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);

@@ -69,7 +69,10 @@ inline const double* hres(const Plan& p, int b) { return p.hred + (long long)b *
 
 // out = scale * dV * (K * rho)_crop per instance; with dot, the sums
 // sum(out_unscaled... ) -> hred[b][R_EGRAV] use out as computed (scale included)
-int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot);
+// dot2 (with dot): a second sum(out * dot2) in the same rows; site: the
+// partial-row region of the sums (device loop) or their host fold
+int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot,
+             const double* dot2 = nullptr, int site = C_EGRAV);
 // kinetic/L_z/local sums of psi - t[b] d into hred[b][R_EFIRST..], [R_EREST..];
 // rho_out (optional) receives |psi - t d|^2
 // Fusions of the device-driven loop into the first energy pass: instances
@@ -81,6 +84,13 @@ struct EnergyFuse {
 };
 EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const double2* d, const double* t,
                            double* rho_out, const EnergyFuse* fuse = nullptr);
+// Line coefficients of the polynomial Armijo search (device-driven loop, rows
+// only): the first axis' pass (KL_FIRST: new direction for instances with
+// FL_COMB, local sums, q = Re(conj(psi) d) and r = |d|^2 to q_out / r_out when
+// given, kinetic and rotation forms of d) into part_of(C_LINE); the other
+// axes' forms of d into C_EREST0 / C_EREST1.
+EnergyRows run_line_sums(Plan& p, const Batch& B, const double2* psi, double2* d, const double2* z,
+                         const double2* g, const double* phi, double* q_out, double* r_out);
 // assemble instance b's Energy from hred (after plan_sync); sgrav = sum(phi rho)
 // with phi the potential as stored (its scale is passed as gscale)
 Energy energy_from_sums(const Plan& p, int b, const Phys& ph, bool lr, bool normalise, double gscale);

@@ -187,7 +187,7 @@ void plan_reserve(Plan& p, int nb) {
   const long long rows = std::max<long long>(std::max<long long>(p.n / p.N, 1024), (long long)p.N + 8);
   p.max_rows = (int)rows;
   p.part_stride = 16LL * rows * B + 16 * BEC_XCDS;
-  HIPCHK(hipMalloc(&p.partials, sizeof(double) * p.part_stride * C_COUNT));
+  HIPCHK(hipMalloc(&p.partials, sizeof(double) * p.part_stride * (C_COUNT + 1)));
   HIPCHK(hipHostMalloc(&p.hpart, sizeof(double) * p.part_stride * C_COUNT,
                        hipHostMallocMapped | hipHostMallocCoherent));
   HIPCHK(hipHostGetDevicePointer((void**)&p.hpart_dev, p.hpart, 0));
@@ -274,7 +274,7 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, c
 }
 
 int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out,
-             const double* dot) {
+             const double* dot, const double* dot2, int site) {
   if (!p.khat[which]) throw BecError(1, "kernel spectrum not built for this plan");
   const int N = p.N, M = p.M, H = p.H, i0 = (p.M - p.N) / 2;
   const double fac = scale * p.dv / (double)p.nM;
@@ -294,8 +294,8 @@ int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale
     PassArgs c3 = pass_base(p, p.fpM, p.lcM, B);
     c3.nl = N; c3.in_so = H; c3.in_s = 1; c3.in_ib = S1;
     c3.out_so = N; c3.out_s = 1; c3.out_ib = n; c3.n_store = N; c3.off_store = i0; c3.out_scale = fac;
-    c3.in = p.s1; c3.out = out; c3.dot = dot;
-    if (dot) (B.dev ? c3.partials : c3.hpartials) = B.dev ? part_of(p, C_EGRAV) : hpart_of(p, C_EGRAV);
+    c3.in = p.s1; c3.out = out; c3.dot = dot; c3.dot2 = dot ? dot2 : nullptr;
+    if (dot) (B.dev ? c3.partials : c3.hpartials) = B.dev ? part_of(p, site) : hpart_of(p, site);
     nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c3, p.stream);
   } else {
     const long long NH = (long long)N * H, MH = (long long)M * H;
@@ -324,11 +324,11 @@ int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale
     PassArgs c5 = pass_base(p, p.fpM, p.lcM, B);
     c5.nl = N * N; c5.in_so = H; c5.in_s = 1; c5.in_ib = S1;
     c5.out_so = N; c5.out_s = 1; c5.out_ib = n; c5.n_store = N; c5.off_store = i0; c5.out_scale = fac;
-    c5.in = p.s1; c5.out = out; c5.dot = dot;
-    if (dot) (B.dev ? c5.partials : c5.hpartials) = B.dev ? part_of(p, C_EGRAV) : hpart_of(p, C_EGRAV);
+    c5.in = p.s1; c5.out = out; c5.dot = dot; c5.dot2 = dot ? dot2 : nullptr;
+    if (dot) (B.dev ? c5.partials : c5.hpartials) = B.dev ? part_of(p, site) : hpart_of(p, site);
     nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
   }
-  if (dot && !B.dev) add_job(p, C_EGRAV, nb_rows, 1, 0, R_EGRAV, B);
+  if (dot && !B.dev) add_job(p, site, nb_rows, dot2 ? 2 : 1, 0, R_EGRAV, B);
   return nb_rows;
 }
 
@@ -390,6 +390,34 @@ EnergyRows run_energy_sums(Plan& p, const Batch& B, const double2* psi, const do
     if (B.dev) continue;
     if (k == 0) add_job(p, C_EFIRST, rows, 6, 0, R_EFIRST, B);
     else add_job(p, site, rows, 2, 0, R_EREST + 2 * (k - 1), B);
+  }
+  return er;
+}
+
+EnergyRows run_line_sums(Plan& p, const Batch& B, const double2* psi, double2* d, const double2* z,
+                         const double2* g, const double* phi, double* q_out, double* r_out) {
+  const int* ax = p.ndim == 2 ? kAxes2 : kAxes3;
+  EnergyRows er{0, 0, 0};
+  for (int k = 0; k < p.ndim; ++k) {
+    KinArgs a = kin_base(p, B);
+    if (k == 0) {
+      a.psi = psi;
+      a.d = d;
+      a.z = z;
+      a.g = g;
+      a.d_out = d;
+      a.comb_bit = FL_COMB;
+      a.tab = B.tab;
+      a.phi = phi;
+      a.rho_out = q_out;
+      a.r_out = r_out;
+      a.partials = part_of(p, C_LINE);
+      er.r0 = launch_kin(ax[0], KL_FIRST, a, p.stream);
+    } else {
+      a.psi = d;                    // the direction alone: its kinetic and rotation forms
+      a.partials = part_of(p, k == 1 ? C_EREST0 : C_EREST1);
+      (k == 1 ? er.r1 : er.r2) = launch_kin(ax[k], KE_REST, a, p.stream);
+    }
   }
   return er;
 }

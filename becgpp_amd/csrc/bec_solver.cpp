@@ -27,7 +27,7 @@ void solver_release(Plan& p) {
   dev_release(p);
   for (int i = 0; i < 8; ++i)
     if (p.st[i]) (void)hipFree(p.st[i]);
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 6; ++i)
     if (p.stphi[i]) (void)hipFree(p.stphi[i]);
   for (int i = 0; i < 8; ++i)
     if (p.st32[i]) (void)hipFree(p.st32[i]);
@@ -42,7 +42,7 @@ static void solver_alloc(Plan& p, int nb) {
   HIPCHK(hipStreamSynchronize(p.stream));
   solver_release(p);
   for (int i = 0; i < S_COUNT; ++i) HIPCHK(hipMalloc(&p.st[i], sizeof(double2) * p.n * nb));
-  for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&p.stphi[i], sizeof(double) * p.n * nb));
+  for (int i = 0; i < 6; ++i) HIPCHK(hipMalloc(&p.stphi[i], sizeof(double) * p.n * nb));
   p.scap = nb;
 }
 
@@ -113,6 +113,7 @@ struct Inst {
   double E, mu, resid_rel, best_resid, alpha, sigma, zg_prev, gg_prev, zg_cur, gg_cur, E_ref, trial, gd,
       trial_acc;
   double cz_re, cz_im;
+  double L00, Spp, Sppp, Kpp;   // polynomial search: the state's quadratic form and pair sums (normalised)
   int it_ref, n_fail, n_iter, stop, it_acc, ls;
   bool converged, done, pending, lr;
   long long n_energy;
@@ -144,7 +145,7 @@ static void vlog_flush(const Plan& p, int nb, std::vector<int>& printed) {
 // steps behind, so the device always has work queued.  On return I[] holds
 // each instance's final state and n_tr the trace rows written.
 static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<Inst>& I, bool lr_any,
-                        double* trace, int trace_cap, int& n_tr) {
+                        double* trace, int trace_cap, int& n_tr, bool lp) {
   const long long n = p.n;
   hipStream_t s = p.stream;
   // an FP32 phase runs the same passes on the complex64 fields (pointers
@@ -174,6 +175,10 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     di[b].stop = BEC_STOP_MAXIT;
     di[b].lr = I[b].lr ? 1 : 0;
     di[b].n_energy = I[b].n_energy;
+    di[b].L00 = I[b].L00;
+    di[b].Spp = I[b].Spp;
+    di[b].Sppp = I[b].Sppp;
+    di[b].Kpp = I[b].Kpp;
     di[b].n_tr = n_tr;
     p.hdone[b] = 0;
   }
@@ -213,6 +218,13 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
   c.r_e1 = part_of(p, C_EREST0);
   c.r_e2 = part_of(p, C_EREST1);
   c.r_gr = part_of(p, C_EGRAV);
+  if (lp) c.r_e0 = part_of(p, C_LINE);
+  c.r_kq = part_of(p, C_EGRAV);
+  c.r_kr = part_of(p, C_EGRAV2);
+  double* LQ = p.stphi[2];      // q = Re(conj(psi) d), r = |d|^2 and their convolutions
+  double* LR = p.stphi[3];
+  double* KQ = p.stphi[4];
+  double* KR = p.stphi[5];
   c.sigma = p.sigma;
   c.iphys = p.iphys;
   c.st = (DevInst*)p.dinst;
@@ -273,6 +285,46 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     c.n_hl = run_H_accept_dev(p, dev_batch(p, nb, FL_ACC), a);
   };
 
+  // One step of the polynomial search: every running instance resolves a
+  // whole iteration (an accepted trial or a failed line search) per step.
+  auto step_lp = [&]() {
+    hipStream_t st = p.stream;
+    const Batch BD = dev_batch(p, nb, FL_DIR);
+    launch_sigma(p, nb, act_of(BD), PSI, lr_any ? phi : nullptr, part_of(p, C_HLAST), p.hlast_rows, 4, 2, floor_,
+                 st, p.f32);
+    const DotFields df{PSI, D, G, Y, SD};
+    c.n_dots = run_precond(p, BD, G, Z, &df);
+    launch_ctrl_dir(c, st);
+    const EnergyRows er = run_line_sums(p, dev_batch(p, nb, FL_SEARCH), PSI, D, Z, G, lr_any ? phi : nullptr,
+                                        lr_any ? LQ : nullptr, lr_any ? LR : nullptr);
+    c.n_e0 = er.r0;
+    c.n_e1 = er.r1;
+    c.n_e2 = er.r2;
+    if (lr_any) {
+      Batch BL = dev_batch(p, nb, FL_SLR);
+      BL.f32 = 0;                        // the convolutions are FP64 in every phase
+      c.n_kq = run_conv(p, 0, BL, LQ, 1.0, KQ, LQ, LR, C_EGRAV);
+      c.n_kr = run_conv(p, 0, BL, LR, 1.0, KR, LR, nullptr, C_EGRAV2);
+    }
+    launch_ctrl_line(c, st);
+    launch_post(PSI, BEST, D, Z, p.dtab, p.flags, n, nb, p.f32, st);
+    if (lr_any) launch_phi_comb(phi, KQ, KR, p.dtab, p.flags, n, nb, st);
+    HArgs a;
+    memset(&a, 0, sizeof(a));
+    a.psi = PSI;
+    a.d = D;
+    a.psi_w = PSI;
+    a.sdiff = SD;
+    a.local_on = true;
+    a.phi = lr_any ? phi : nullptr;
+    a.kf = 0.5;
+    a.rf_omega = 1;
+    a.out = G;
+    a.ydiff = Y;
+    a.reduce = true;
+    c.n_hl = run_H(p, dev_batch(p, nb, FL_ACC), a);
+  };
+
   // Steps run on the plan's capture stream, after the work queued so far on
   // the caller's stream.  A chunk of K steps is captured once into a hipGraph
   // and replayed (one launch per chunk); while a kernel is being timed the
@@ -299,7 +351,7 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
     auto capture = [&](int k_steps, hipGraph_t* g, hipGraphExec_t* ge) {
       HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        for (int k = 0; k < k_steps; ++k) step();
+        for (int k = 0; k < k_steps; ++k) lp ? step_lp() : step();
       } catch (...) {
         hipGraph_t g2 = nullptr;
         (void)hipStreamEndCapture(cs, &g2);
@@ -323,7 +375,7 @@ static void device_loop(Plan& p, int nb, const bec_solver_cfg* cf, std::vector<I
         HIPCHK(hipGraphLaunch(gexec, cs));
         steps += K;
       } else {
-        for (int k = 0; k < K; ++k) step();
+        for (int k = 0; k < K; ++k) lp ? step_lp() : step();
         steps += K;
       }
       HIPCHK(hipGetLastError());
@@ -457,6 +509,11 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   std::vector<double> mu_arr(nb), phisc(nb), tarr(nb), divarr(nb);
   for (int b = 0; b < nb; ++b) {
     const Energy e0 = energy_from_sums(p, b, ph[b], I[b].lr, false, -ph[b].G_C);
+    const double* rs = hres(p, b);
+    I[b].L00 = e0.Ekin + e0.Etrap + e0.Erot;
+    I[b].Spp = rs[R_EFIRST + 4] * dv;
+    I[b].Sppp = rs[R_EFIRST + 5] * dv;
+    I[b].Kpp = I[b].lr ? rs[R_EGRAV] * dv : 0.0;
     I[b].E = e0.E;
     I[b].mu = e0.E + e0.Econtact + 2.0 * e0.Ethree + e0.Egrav;
     I[b].E_ref = e0.E;
@@ -493,8 +550,11 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   }
   launch_copy(PSI, BEST, n, nb, mask_act(all), s);
 
+  // the polynomial line search (one step per iteration) unless the trial-per-step
+  // search is asked for or the loop runs on the host
+  const bool lp = cf->line_poly != 0 && !cf->host_loop;
   if (f32) {
-    if (lr_any) throw BecError(1, "internal: FP32 phase with a long-range kernel");
+    if (lr_any && !lp) throw BecError(1, "internal: FP32 phase with a long-range kernel needs the polynomial search");
     solver_alloc32(p);
     const long long nn = n * nb;
     launch_narrow(PSI, p.st32[S_PSI], nn, s);
@@ -506,7 +566,7 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
     HIPCHK(hipGetLastError());
     p.f32 = 1;
     try {
-      device_loop(p, nb, cf, I, false, nullptr, 0, n_tr);
+      device_loop(p, nb, cf, I, lp && lr_any, nullptr, 0, n_tr, lp);
     } catch (...) {
       p.f32 = 0;
       throw;
@@ -516,7 +576,7 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
     launch_widen(p.st32[S_BEST], BEST, nn, s);
     HIPCHK(hipGetLastError());
   } else if (!cf->host_loop) {
-    device_loop(p, nb, cf, I, lr_any, trace && nb == 1 ? trace : nullptr, trace_cap, n_tr);
+    device_loop(p, nb, cf, I, lr_any, trace && nb == 1 ? trace : nullptr, trace_cap, n_tr, lp);
   } else {
   int it = 0;
   while (true) {
@@ -759,16 +819,18 @@ static int solve_phase(Plan& p, int nb, const bec_phys* php, const bec_solver_cf
   return 0;
 }
 
-// The loop, preceded by an FP32 warm phase when warm_res_tol > res_tol, no
-// instance has long range (the convolution stays FP64 throughout) and the
-// grid's transforms use no radix 7, 11 or 13 (built for FP64 only): the FP64
-// loop restarts from the warm phase's returned states.
+// The loop, preceded by an FP32 warm phase when warm_res_tol > res_tol and the
+// grid's transforms use no radix 7, 11 or 13 (built for FP64 only); with long
+// range the warm phase needs the polynomial search (its convolutions, sums and
+// test stay FP64).  The FP64 loop restarts from the warm phase's returned
+// states.
 static int solve_batch(Plan& p, int nb, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
                        bec_solve_out* out, double* trace, int trace_cap) {
   bool lr = false;
   for (int b = 0; b < nb && b < BEC_BMAX; ++b)
     lr = lr || plan_long_range(p, Phys{php[b].beta2, php[b].beta3, php[b].Omega, php[b].G_C});
-  if (!(cf->warm_res_tol > cf->res_tol) || lr || p.fpN.odd) {
+  const bool lp = cf->line_poly != 0 && !cf->host_loop;
+  if (!(cf->warm_res_tol > cf->res_tol) || (lr && !lp) || p.fpN.odd) {
     const int r = solve_phase(p, nb, php, cf, psi_io, out, trace, trace_cap, false);
     for (int b = 0; b < nb; ++b) out[b].warm_iters = 0;
     return r;

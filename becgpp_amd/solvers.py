@@ -39,7 +39,15 @@ def _solver_cfg(cfg, verbose):
         int(cfg.get("linesearch_max", 8)), int(cfg.get("linesearch_stall", 10)),
         float(cfg.get("precond_shift_min", 0.5)), _lib.CG_BETA[cg_beta], int(cfg.get("cg_restart", 30)),
         1 if verbose else 0, int(cfg.get("check", 200)), 1 if cfg.get("host_loop", False) else 0,
-        float(cfg.get("warm_res_tol", 1e-4)) if cfg.get("warm_fp32", False) else 0.0)
+        float(cfg.get("warm_res_tol", 1e-4)) if cfg.get("warm_fp32", False) else 0.0,
+        _line_poly(cfg))
+
+
+def _line_poly(cfg):
+    mode = str(cfg.get("line_search", "poly")).lower()
+    if mode not in ("poly", "trial"):
+        raise ValueError(f"line_search must be 'poly' or 'trial', got {mode!r}")
+    return 1 if mode == "poly" else 0
 
 
 def solve_native(cfg, G, psi0, verbose=False, record=False):

@@ -71,9 +71,17 @@ typedef struct bec_solver_cfg {
                       iteration and Armijo trial (no reference counterpart) */
   double warm_res_tol;  /* > res_tol: an FP32 warm phase first runs the loop on
                            complex64 fields to this relative residual, then the
-                           FP64 loop polishes from its best state; instances with
-                           long range run FP64 throughout; 0 = FP64 only (no
+                           FP64 loop polishes from its best state; with long
+                           range this needs line_poly (the convolutions, sums
+                           and the Armijo test stay FP64); 0 = FP64 only (no
                            reference counterpart) */
+  int line_poly;        /* 1: the device loop prices every Armijo trial of an
+                           iteration from one pass of line coefficients (the
+                           energy along psi - t d is a polynomial in t over a
+                           polynomial norm), so each step completes an
+                           iteration of every instance; 0: one trial per step
+                           (no reference counterpart; the trials, their order
+                           and the acceptance test are the reference's) */
 } bec_solver_cfg;
 
 typedef struct bec_solve_out {
