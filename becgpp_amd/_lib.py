@@ -90,6 +90,8 @@ SIGNATURES = {
                                          ctypes.c_int]),
     "bec_prof_watch": (ctypes.c_int, [ctypes.c_int]),
     "bec_prof_read": (ctypes.c_int, [c_double_p, ctypes.POINTER(ctypes.c_longlong), c_double_p]),
+    "bec_prof_sites": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "bec_prof_read_site": (ctypes.c_int, [ctypes.c_int, c_double_p, ctypes.POINTER(ctypes.c_longlong), c_double_p]),
 }
 
 _LIB = None

@@ -438,11 +438,17 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
 }
 
 void launch_ctrl_dir(const CtrlArgs& c, hipStream_t s) {
+  prof_pre(SITE_CTRL_DIR, s);
   hipLaunchKernelGGL(k_ctrl_dir, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
+  prof_post(SITE_CTRL_DIR, s);
 }
 void launch_ctrl_armijo(const CtrlArgs& c, hipStream_t s) {
+  prof_pre(SITE_CTRL_ARMIJO, s);
   hipLaunchKernelGGL(k_ctrl_armijo, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
+  prof_post(SITE_CTRL_ARMIJO, s);
 }
 void launch_ctrl_line(const CtrlArgs& c, hipStream_t s) {
+  prof_pre(SITE_CTRL_LINE, s);
   hipLaunchKernelGGL(k_ctrl_line, dim3(c.nb), dim3(CTRL_WG), 0, s, c);
+  prof_post(SITE_CTRL_LINE, s);
 }

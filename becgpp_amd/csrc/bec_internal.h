@@ -323,4 +323,5 @@ bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, double bytes);   /
 unsigned long long* prof_counter(int id);
 bool prof_active();                  // some launch site is being timed
 enum ElemSite { SITE_DOTS = 10, SITE_COMBINE = 11, SITE_NORM = 12, SITE_SIGMA_LOCAL = 20,
-                SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23 };
+                SITE_SIGMA_GATHER = 22, SITE_SIGMA_SELECT = 23, SITE_CTRL_DIR = 30, SITE_CTRL_ARMIJO = 31,
+                SITE_CTRL_LINE = 32, SITE_POST = 33, SITE_PHI_COMB = 34 };

@@ -1631,11 +1631,13 @@ void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_
 }
 void launch_post(const double2* psi, double2* best, double2* d, const double2* z, const double* tab,
                  const unsigned* fl, long long n, int nb, int f32, hipStream_t s) {
+  prof_pre(SITE_POST, s);
   if (f32)
     hipLaunchKernelGGL(k_post<float2>, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, (const float2*)psi,
                        (float2*)best, (float2*)d, (const float2*)z, tab, fl, n);
   else
     hipLaunchKernelGGL(k_post<double2>, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, best, d, z, tab, fl, n);
+  prof_post(SITE_POST, s);
 }
 // ---------------------------------------------------------------------------
 // Diagnostics of a 2D state (vortex.py:10-84)
@@ -1903,7 +1905,9 @@ __global__ __launch_bounds__(BEC_WG) void k_phi_comb(double* phi, const double* 
 }
 void launch_phi_comb(double* phi, const double* kq, const double* kr, const double* tab, const unsigned* fl,
                      long long n, int nb, hipStream_t s) {
+  prof_pre(SITE_PHI_COMB, s);
   hipLaunchKernelGGL(k_phi_comb, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, phi, kq, kr, tab, fl, n);
+  prof_post(SITE_PHI_COMB, s);
 }
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, src, dst, n, mask);

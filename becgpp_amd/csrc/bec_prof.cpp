@@ -1,8 +1,11 @@
 // Live kernel timing: HIP events recorded on the launch stream around every
-// launch of one watched kernel (identified by its launch-site id), summed on
-// demand together with the algorithmic bytes each launch moved.  Used by
-// bench.py for the achieved-bandwidth figure of the dominant kernel; off (no
-// events) unless a kernel is watched.
+// launch of the watched kernels (identified by their launch-site ids), summed
+// per site on demand together with the algorithmic bytes each launch moved.
+// Watching one site times that kernel alone; watching every site (id -1)
+// gives the breakdown of a whole solver step, from which bench.py takes the
+// kernel with the largest share of the step time.  Off (no events, and the
+// solver loop replays hipGraphs) unless something is watched.
+#include <map>
 #include <vector>
 
 #include "../../include/becgpp_hip.h"
@@ -12,23 +15,33 @@ namespace {
 struct EvPair {
   hipEvent_t a, b;
   double bytes;
+  int site;
 };
-int g_watch = 0;
-unsigned long long* g_dcount = nullptr;    // device: instances covered by watched device-driven launches
+struct SiteSum {
+  double ms = 0.0;
+  long long cnt = 0;
+  double bytes = 0.0;
+  int slot = -1;                     // index of its device byte counter
+};
+constexpr int kMaxSlots = 256;
+int g_watch = 0;                       // 0 off, -1 every site, else one site id
+unsigned long long* g_dcount = nullptr;    // device: algorithmic bytes of device-driven launches, per slot
 std::vector<EvPair> g_pool;
 size_t g_used = 0;
-double g_ms = 0.0;
-long long g_cnt = 0;
-double g_bytes = 0.0;
+std::map<int, SiteSum> g_sites;
+int g_nslots = 0;
+
+bool watched(int id) { return g_watch == -1 || (g_watch != 0 && id == g_watch); }
 
 void collect() {
   for (size_t i = 0; i < g_used; ++i) {
     float ms = 0.f;
     if (hipEventSynchronize(g_pool[i].b) == hipSuccess &&
         hipEventElapsedTime(&ms, g_pool[i].a, g_pool[i].b) == hipSuccess) {
-      g_ms += ms;
-      ++g_cnt;
-      g_bytes += g_pool[i].bytes;
+      SiteSum& s = g_sites[g_pool[i].site];
+      s.ms += ms;
+      ++s.cnt;
+      s.bytes += g_pool[i].bytes;
     }
   }
   g_used = 0;
@@ -49,18 +62,20 @@ EvPair* next_pair() {
 }  // namespace
 
 void prof_pre(int id, hipStream_t s, double bytes) {
-  if (g_watch == 0 || id != g_watch) return;
+  if (!watched(id)) return;
   EvPair* e = next_pair();
   if (!e) return;
   e->bytes = bytes;
+  e->site = id;
   (void)hipEventRecord(e->a, s);
 }
 
 bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, double bytes) {
-  if (g_watch == 0 || id != g_watch) return false;
+  if (!watched(id)) return false;
   EvPair* e = next_pair();
   if (!e) return false;
   e->bytes = bytes;
+  e->site = id;
   *start = e->a;
   *stop = e->b;
   ++g_used;
@@ -68,7 +83,7 @@ bool prof_events(int id, hipEvent_t* start, hipEvent_t* stop, double bytes) {
 }
 
 void prof_post(int id, hipStream_t s) {
-  if (g_watch == 0 || id != g_watch || g_used == g_pool.size()) return;
+  if (!watched(id) || g_used == g_pool.size()) return;
   (void)hipEventRecord(g_pool[g_used].b, s);
   ++g_used;
 }
@@ -76,32 +91,62 @@ void prof_post(int id, hipStream_t s) {
 bool prof_active() { return g_watch != 0; }
 
 unsigned long long* prof_counter(int id) {
-  if (g_watch == 0 || id != g_watch) return nullptr;
-  return g_dcount;
+  if (!watched(id) || !g_dcount) return nullptr;
+  SiteSum& s = g_sites[id];
+  if (s.slot < 0) {
+    if (g_nslots == kMaxSlots) return nullptr;
+    s.slot = g_nslots++;
+  }
+  return g_dcount + s.slot;
 }
 
 extern "C" int bec_prof_watch(int id) {
   collect();
   g_watch = id;
-  g_ms = 0.0;
-  g_cnt = 0;
-  g_bytes = 0.0;
+  g_sites.clear();
+  g_nslots = 0;
   if (id != 0) {
-    if (!g_dcount && hipMalloc(&g_dcount, sizeof(unsigned long long)) != hipSuccess) {
+    if (!g_dcount && hipMalloc(&g_dcount, sizeof(unsigned long long) * kMaxSlots) != hipSuccess) {
       g_dcount = nullptr;
       return 2;
     }
-    if (hipMemset(g_dcount, 0, sizeof(unsigned long long)) != hipSuccess) return 2;
+    if (hipMemset(g_dcount, 0, sizeof(unsigned long long) * kMaxSlots) != hipSuccess) return 2;
   }
   return 0;
 }
 
-extern "C" int bec_prof_read(double* total_ms, long long* count, double* bytes) {
+extern "C" int bec_prof_read_site(int id, double* total_ms, long long* count, double* bytes) {
   collect();
-  unsigned long long dev = 0;
-  if (g_dcount && hipMemcpy(&dev, g_dcount, sizeof(dev), hipMemcpyDeviceToHost) != hipSuccess) return 2;
-  if (total_ms) *total_ms = g_ms;
-  if (count) *count = g_cnt;
-  if (bytes) *bytes = g_bytes + (double)dev;
+  double ms = 0.0, b = 0.0;
+  long long c = 0;
+  std::vector<unsigned long long> dev(kMaxSlots, 0ull);
+  if (g_dcount && g_nslots > 0 &&
+      hipMemcpy(dev.data(), g_dcount, sizeof(unsigned long long) * g_nslots, hipMemcpyDeviceToHost) != hipSuccess)
+    return 2;
+  for (const auto& kv : g_sites) {
+    if (id != -1 && kv.first != id) continue;
+    ms += kv.second.ms;
+    c += kv.second.cnt;
+    b += kv.second.bytes + (kv.second.slot >= 0 ? (double)dev[kv.second.slot] : 0.0);
+  }
+  if (total_ms) *total_ms = ms;
+  if (count) *count = c;
+  if (bytes) *bytes = b;
+  return 0;
+}
+
+extern "C" int bec_prof_read(double* total_ms, long long* count, double* bytes) {
+  return bec_prof_read_site(-1, total_ms, count, bytes);
+}
+
+extern "C" int bec_prof_sites(int* ids, int cap, int* n) {
+  collect();
+  int k = 0;
+  for (const auto& kv : g_sites)
+    if (kv.second.cnt > 0) {
+      if (ids && k < cap) ids[k] = kv.first;
+      ++k;
+    }
+  if (n) *n = k;
   return 0;
 }

@@ -250,6 +250,12 @@ int bec_plan_autotune(bec_plan* p, int nb, double* ms_best, double* ms_default, 
  * shift selection 20-23 (becgpp_amd/csrc/bec_internal.h). */
 int bec_prof_watch(int id);
 int bec_prof_read(double* total_ms, long long* count, double* bytes);
+/* id -1 in bec_prof_watch times every launch site at once; then
+ * bec_prof_sites lists the sites that launched (n_out = their number, at most
+ * cap ids written) and bec_prof_read_site gives one site's totals (id -1:
+ * all sites). */
+int bec_prof_sites(int* ids, int cap, int* n_out);
+int bec_prof_read_site(int id, double* total_ms, long long* count, double* bytes);
 
 #ifdef __cplusplus
 }
