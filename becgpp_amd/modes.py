@@ -1,15 +1,17 @@
 """run(cfg) and the modes of the ground-state path (becgpp/modes.py:30-500).
 
-The dispatch, mode names, run ids and error behaviour are the reference's.
-The modes that sit on the ground-state path are computed here by the GPU
-solver: ``single`` (ground_state / ground_state_multiseed), ``sweep`` (every
-fresh seed of every point as one batched solve when continuation is off;
-point by point with the continuation candidate otherwise), ``validate`` (the
-analytic gate, every convolution and energy by libbecgpp_hip) and ``smoke``.
-Results are returned as the reference returns them.  ``single`` resumes from
-and writes the reference's checkpoint, summary CSV and per-run JSON (records.py);
-figures, vortex and LLL diagnostics are not produced by this package, and the
-modes ``tf_only`` and ``convergence`` are not part of it (they raise).
+The dispatch, mode names, run ids and error behaviour are the reference's,
+and all six of its modes run here on the GPU solver: ``single``
+(ground_state / ground_state_multiseed), ``sweep`` (every fresh seed of every
+point as one batched solve when continuation is off; with continuation each
+point's fresh seeds and its continuation candidate form one batch),
+``validate`` (the analytic gate, every convolution and energy by
+libbecgpp_hip), ``smoke``, ``tf_only`` and ``convergence``.  Results are
+returned as the reference returns them.  ``single`` and ``sweep`` resume from
+and write the reference's checkpoints, CSV and per-run JSON (records.py) with
+the diagnostic bundle (energies, virial, radii, Thomas-Fermi reference,
+vortex count, LLL weight); figures are not produced.  ``zip_output`` archives
+the output directory after the mode, as the reference does.
 """
 import hashlib
 import json
@@ -27,6 +29,7 @@ from .operators import energy_components
 from .solvers import batchable, ground_state, ground_state_batch, ground_state_multiseed
 from .thomasfermi import extract_tf
 from .diagnostics import diagnostics
+from .seeds import make_seed
 from . import records
 
 # keys that cannot change the computed state (becgpp/io.py:9-19)
@@ -173,19 +176,29 @@ def mode_sweep(cfg):
     prev, G_prev = None, None
     for ip, c in enumerate(pts):
         G = make_grid(c)
+        # the point's candidates (modes.py:350-361): the continuation from the
+        # previous point's state (resampled onto a new grid) and, with several
+        # seeds or no continuation, the fresh seeds -- solved as one batch
+        cont = None
+        if prev is not None:
+            cont = prev if tuple(prev.shape) == tuple(G["R2"].shape) else resample_state(prev, G_prev, G)
+        seeds = _seed_cfgs(c, nseeds) if (nseeds > 1 or cont is None) else []
+        if nseeds <= 1:
+            seeds = [dict(c, seed=c.get("seed", "tf"))] if seeds else []
+        jobs = ([("continuation", c, cont)] if cont is not None else []) + \
+            [(f"seed:{cj['seed']}", cj, make_seed(G, cj)) for cj in seeds]
+        if len(jobs) > 1 and batchable(G):
+            res = ground_state_batch([cj for _, cj, _ in jobs], G=G, psi0s=[p0 for _, _, p0 in jobs])
+        else:
+            res = [ground_state(cj, G=G, psi0=p0, verbose=False) for _, cj, p0 in jobs]
         cands = []
-        if prev is not None:                            # modes.py:350-355
-            psi0 = prev if tuple(prev.shape) == tuple(G["R2"].shape) else resample_state(prev, G_prev, G)
-            if psi0 is not None:
-                cands.append(("continuation", ground_state(c, G=G, psi0=psi0, verbose=False)))
-        if nseeds > 1 or not cands:
-            seeds = _seed_cfgs(c, nseeds)
-            if len(seeds) > 1 and batchable(G):
-                res = ground_state_batch(seeds, G=G)
-            else:
-                res = [ground_state(cj, G=G, verbose=False) for cj in seeds]
+        if cont is not None:
+            cands.append(("continuation", res[0]))
+            res = res[1:]
+            jobs = jobs[1:]
+        if res:
             best = min(range(len(res)), key=lambda j: res[j][2]["E"])
-            cands.append((f"seed:{seeds[best]['seed']}", res[best]))
+            cands.append((jobs[best][0], res[best]))
         rows[ip] = _pick(cands, c, param, values[ip])
         prev, G_prev = rows[ip]["psi"], G
         _sweep_record(cfg, param, pts, rows, ip)
@@ -451,4 +464,11 @@ def run(cfg):
         print(f"[warn] kernel={requested!r} is set but G_C=0, so the long-range term is OFF "
               f"(reported kernel='none'). Set G_C>0 to activate it, e.g. G_C=20.")
     print(f"becGPP-amd {CODE_VERSION} | device={DEV} | mode={mode} | dim={cfg.get('dimension')} kernel={kkind}")
-    return fn(cfg)
+    result = fn(cfg)
+    if cfg.get("zip_output", False):            # modes.py:486-491
+        try:
+            zip_results()
+        except Exception as exc:
+            print(f"[zip-warn] {exc}")
+    print(f"\nDone. Output in: {records.out_dir()}")
+    return result

@@ -29,13 +29,33 @@ def ckpt_path(rid):
     return os.path.join(out_dir(), "ckpt", f"{rid}.pt")
 
 
+def _numpy_scalar_globals():
+    """The classes a pickled numpy scalar refers to (numpy 1.x and 2.x names)."""
+    import numpy as np
+    out = [np.dtype]
+    for mod in ("numpy._core.multiarray", "numpy.core.multiarray"):
+        try:
+            m = __import__(mod, fromlist=["scalar"])
+            out.append(m.scalar)
+        except (ImportError, AttributeError):
+            pass
+    try:
+        out.append(type(np.dtype(np.float64)))         # numpy.dtypes.Float64DType
+    except TypeError:
+        pass
+    return out
+
+
 def load_checkpoint_psi(rid):
     """psi of ckpt/<rid>.pt, or None (missing or not loadable safely)."""
     path = ckpt_path(rid)
     if not os.path.isfile(path):
         return None
     try:
-        return torch.load(path, map_location="cpu", weights_only=True)["psi"]
+        # still a weights-only load: the reference's saved observables hold
+        # numpy float64 scalars (its droplet_radius), which need these globals
+        with torch.serialization.safe_globals(_numpy_scalar_globals()):
+            return torch.load(path, map_location="cpu", weights_only=True)["psi"]
     except Exception as exc:                   # the reference warns and starts fresh
         print(f"[resume-warn] {exc}")
         return None

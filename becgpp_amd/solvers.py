@@ -119,26 +119,65 @@ def _finish(psi, res, G, cfg):
     return obs
 
 
+# the configuration keys a solve reads besides the physics (_solver_cfg): one
+# batched loop shares them, so configurations that differ in any of them are
+# solved in separate loops
+SOLVER_KEYS = ("maxit", "res_tol", "energy_tol", "conv_window", "step", "step_max", "linesearch_max",
+               "linesearch_stall", "precond_shift_min", "cg_beta", "cg_restart", "check", "host_loop",
+               "warm_fp32", "warm_res_tol", "line_search")
+
+
+def _solver_key(c):
+    sc = _solver_cfg(c, False)
+    return tuple(getattr(sc, f) for f, _ in sc._fields_)
+
+
 def ground_state_batch(cfgs, G=None, psi0s=None, verbose=False):
     """ground_state for many configurations of one grid at once: each entry of
     the result is what ground_state(cfgs[i], G, psi0s[i]) returns, computed
-    by the batched device loop (64 instances per launch at most)."""
+    by the batched device loop (64 instances per launch at most).  The
+    physics (beta2, beta3, Omega, G_C) may differ per instance; instances
+    whose solver settings differ (SOLVER_KEYS) run in loops of their own."""
     if G is None:
         G = make_grid(cfgs[0])
     if psi0s is None:
         psi0s = [make_seed(G, c) for c in cfgs]
-    out = []
-    for lo in range(0, len(cfgs), BATCH_MAX):
-        cs, ps = cfgs[lo:lo + BATCH_MAX], psi0s[lo:lo + BATCH_MAX]
-        psis, rs = solve_native_batch(cs, G, ps, verbose=verbose)
-        for c, p, r in zip(cs, psis, rs):
-            out.append((p.detach(), G, _finish(p, r, G, c)))
+    sets = {}
+    for i, c in enumerate(cfgs):
+        sets.setdefault(_solver_key(c), []).append(i)
+    out = [None] * len(cfgs)
+    for ks in sets.values():
+        for lo in range(0, len(ks), BATCH_MAX):
+            kk = ks[lo:lo + BATCH_MAX]
+            psis, rs = solve_native_batch([cfgs[k] for k in kk], G, [psi0s[k] for k in kk], verbose=verbose)
+            for k, p, r in zip(kk, psis, rs):
+                out[k] = (p.detach(), G, _finish(p, r, G, cfgs[k]))
     return out
+
+
+def _plan_key(G):
+    return (int(G["ndim"]), int(G["N"]), round(float(G["L"]), 14), int(G["pad"]), round(float(G["s"]), 14),
+            round(float(G.get("trap_coeff", 0.5)), 14))
 
 
 def _grid_key(G):
     return (int(G["ndim"]), int(G["N"]), round(float(G["L"]), 14), int(G["pad"]), round(float(G["s"]), 14),
             round(float(G.get("trap_coeff", 0.5)), 14), str(G["kernel"]), round(float(G.get("l_z", 0.0)), 14))
+
+
+def uneven_schedule(grids):
+    """Groups of instance indices sharing a grid (geometry, kernel, l_z), and
+    whether their loops may run concurrently.  Grids of one geometry share a
+    native plan (plan_of), whose kernel spectrum, stream and workspaces one
+    loop uses at a time, so groups that share a geometry (the same grid with
+    and without long range, or two l_z) run one after the other; concurrent
+    loops need a plan each."""
+    sets = {}
+    for i, G in enumerate(grids):
+        sets.setdefault(_grid_key(G), []).append(i)
+    groups = list(sets.values())
+    plan_keys = [_plan_key(grids[ks[0]]) for ks in groups]
+    return groups, len(groups) > 1 and len(set(plan_keys)) == len(plan_keys)
 
 
 def ground_state_uneven(cfgs, psi0s=None, grids=None, verbose=False):
@@ -156,16 +195,13 @@ def ground_state_uneven(cfgs, psi0s=None, grids=None, verbose=False):
         grids = [make_grid(c) for c in cfgs]
     if psi0s is None:
         psi0s = [make_seed(G, c) for G, c in zip(grids, cfgs)]
-    sets = {}
-    for i, G in enumerate(grids):
-        sets.setdefault(_grid_key(G), []).append(i)
-    groups = list(sets.values())
+    groups, concurrent = uneven_schedule(grids)
     out = [None] * n
-    if len(groups) == 1:
-        ks = groups[0]
-        for k, r in zip(ks, ground_state_batch([cfgs[k] for k in ks], G=grids[ks[0]],
-                                               psi0s=[psi0s[k] for k in ks], verbose=verbose)):
-            out[k] = r
+    if not concurrent:
+        for ks in groups:
+            for k, r in zip(ks, ground_state_batch([cfgs[k] for k in ks], G=grids[ks[0]],
+                                                   psi0s=[psi0s[k] for k in ks], verbose=verbose)):
+                out[k] = r
         return out
     # Plans, kernel spectra (K and, for observables, W) and batch workspaces
     # are made here: the workers then allocate nothing in the library, and each

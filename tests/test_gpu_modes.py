@@ -123,3 +123,40 @@ def test_convergence_mode_rows(tmp_path, monkeypatch):
     _, _, obs = B.ground_state(dict(c, L=7.5, Ngrid=96), verbose=False)
     assert abs(rows[3]["E"] - obs["E"]) <= 1e-9 * abs(obs["E"])
     assert (tmp_path / "convergence.csv").is_file()
+
+
+def test_sweep_over_a_solver_key_solves_each_point_with_its_own_value():
+    """Without continuation the sweep batches its points' seeds; a swept solver
+    setting (here maxit, and res_tol) must still apply per point, as the
+    reference's point-by-point solves apply it."""
+    import becgpp_amd as B
+    c = _cfg(B, mode="sweep", sweep_param="maxit", sweep_values=[5, 40], nseeds=1, seed="gaussian",
+             sweep_continuation=False, beta2=40.0, res_tol=1e-12)
+    rows = B.run(c)
+    assert [r["iters"] for r in rows] == [5, 40]
+    for r in rows:
+        _, _, o = B.ground_state(dict(c, mode="single", maxit=int(r["maxit"])), verbose=False)
+        assert abs(r["E"] - o["E"]) <= 1e-12 * abs(o["E"])
+    c2 = _cfg(B, mode="sweep", sweep_param="res_tol", sweep_values=[1e-2, 1e-7], nseeds=1, seed="gaussian",
+              sweep_continuation=False, beta2=40.0)
+    r2 = B.run(c2)
+    assert r2[0]["iters"] < r2[1]["iters"] and r2[1]["resid_rel"] < 1e-7
+
+
+def test_continuation_candidate_batched_with_the_seeds_equals_separate_solves():
+    """With continuation each point's continuation candidate and its fresh
+    seeds form one batch: every candidate's energy equals its own solve."""
+    import becgpp_amd as B
+    from becgpp_amd.seeds import make_seed
+    c = _cfg(B, mode="sweep", sweep_param="beta2", sweep_values=[30.0, 45.0], nseeds=2, res_tol=1e-9)
+    rows = B.run(c)
+    prev = rows[0]["psi"]
+    cp = dict(c, mode="single", beta2=45.0, tag=f"{c.get('tag', 'becgpp')}_beta245")
+    G = B.make_grid(cp)
+    _, _, oc = B.ground_state(cp, G=G, psi0=prev, verbose=False)
+    cand = dict(x.split("=") for x in rows[1]["candidate_E"].split(";"))
+    assert abs(float(cand["continuation"]) - oc["E"]) <= 1e-9 * abs(oc["E"])
+    seeds = [dict(cp, rng=int(cp["rng"]) + j, seed=("tf", "gaussian")[j]) for j in range(2)]
+    Es = [B.ground_state(s, G=G, psi0=make_seed(G, s), verbose=False)[2]["E"] for s in seeds]
+    key = next(k for k in cand if k.startswith("seed:"))
+    assert abs(float(cand[key]) - min(Es)) <= 1e-9 * abs(min(Es))

@@ -254,3 +254,48 @@ def test_reference_audit_host_cases():
     a = B.sim_params_from_physical("selfgrav3d", m=1e-26, omega_perp=100.0, N=1e4, verbose=False)
     b = B.sim_params_from_physical("selfgrav3d", m=1e-26, omega_perp=100.0, N=2e4, verbose=False)
     assert abs(b["G_C"] / a["G_C"] - 2.0) < 1e-12
+
+
+def test_uneven_schedule_serialises_groups_that_share_a_plan():
+    """Groups on one geometry (same N, L, pad, trap) share a native plan and
+    must not run concurrently; distinct geometries may."""
+    from becgpp_amd.solvers import uneven_schedule
+
+    def g(N, kernel="q2d", l_z=0.5):
+        return dict(ndim=2, N=N, L=16.0, pad=2, s=2.0, trap_coeff=0.5, kernel=kernel, l_z=l_z)
+
+    groups, conc = uneven_schedule([g(128), g(192), g(256), g(128), g(192)])
+    assert conc and sorted(map(sorted, groups)) == [[0, 3], [1, 4], [2]]
+    groups, conc = uneven_schedule([g(128), g(128, kernel="none", l_z=0.0), g(256)])
+    assert not conc and len(groups) == 3
+    groups, conc = uneven_schedule([g(128), g(128, l_z=0.25)])
+    assert not conc and len(groups) == 2
+    groups, conc = uneven_schedule([g(128), g(128)])
+    assert not conc and groups == [[0, 1]]
+
+
+def test_batches_split_by_solver_settings():
+    """A batched loop shares its solver settings: configurations that differ
+    in any solver key get different batch keys, the physics does not."""
+    import becgpp_amd as B
+    from becgpp_amd.solvers import SOLVER_KEYS, _solver_key
+    c = B.default_cfg(mode="single")
+    assert _solver_key(c) == _solver_key(dict(c, beta2=7.0, Omega=0.3, G_C=2.0, beta3=1.0))
+    for k, v in (("res_tol", 1e-9), ("maxit", 17), ("linesearch_max", 3), ("cg_beta", "pr"),
+                 ("precond_shift_min", 0.25), ("warm_fp32", True), ("line_search", "trial"), ("step", 0.5)):
+        assert k in SOLVER_KEYS
+        assert _solver_key(dict(c, **{k: v})) != _solver_key(c), k
+
+
+def test_checkpoint_with_numpy_scalars_resumes(tmp_path, monkeypatch):
+    """The reference saves obs holding numpy float64 scalars: its checkpoints
+    load with the weights-only loader."""
+    import numpy as np
+    import torch
+    monkeypatch.setenv("GPP_OUTDIR", str(tmp_path))
+    from becgpp_amd import records
+    psi = torch.arange(4, dtype=torch.float64).to(torch.complex128)
+    torch.save(dict(psi=psi, cfg={"beta2": 1.0}, obs={"R_1pct": np.float64(1.25), "E": 1.0}),
+               records.ckpt_path("rid123"))
+    got = records.load_checkpoint_psi("rid123")
+    assert got is not None and torch.equal(got, psi)
