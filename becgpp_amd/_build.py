@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbecgpp_hip.so")
-SOURCES = ["bec_ctrl.hip", "bec_plan.cpp", "bec_solver.cpp", "bec_prof.cpp", "bec_noalloc.hip"]
+SOURCES = ["bec_ctrl.hip", "bec_plan.cpp", "bec_solver.cpp", "bec_prof.cpp", "bec_noalloc.hip", "bec_tf.hip"]
 # bec_kernels.hip is compiled once per translation unit (its BEC_TU parts: the
 # elementwise kernels, one unit per kinetic-pass axis, two for the line passes)
 KERNEL_TUS = 8

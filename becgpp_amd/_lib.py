@@ -32,6 +32,12 @@ class SolverCfg(ctypes.Structure):
                 ("host_loop", ctypes.c_int), ("warm_res_tol", ctypes.c_double), ("line_poly", ctypes.c_int)]
 
 
+class SeedSpec(ctypes.Structure):
+    _fields_ = [("envelope", ctypes.c_int), ("amp_noise", ctypes.c_double), ("phase_noise", ctypes.c_double),
+                ("winding", ctypes.c_int), ("vortex_sign", ctypes.c_double), ("core", ctypes.c_double),
+                ("rng", ctypes.c_ulonglong)]
+
+
 class SolveOut(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int), ("converged", ctypes.c_int),
                 ("stop_reason", ctypes.c_int), ("n_trace", ctypes.c_int),
@@ -79,6 +85,13 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_int)]),
     "bec_tf_mu": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, c_double_p]),
+    "bec_tf_nonlocal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_void_p, c_double_p]),
+    "bec_seed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SeedSpec), ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_int, ctypes.c_void_p]),
+    "bec_noise": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_longlong,
+                                 c_double_p]),
     "bec_ground_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Phys), ctypes.POINTER(SolverCfg),
                                         ctypes.c_void_p, ctypes.POINTER(SolveOut), c_double_p,
                                         ctypes.c_int]),

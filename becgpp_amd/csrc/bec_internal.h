@@ -163,7 +163,8 @@ struct Plan {
   double2* twN; double2* twM;
   LineCfg lcN, lcM;
   int lsh[2];                        // autotuned log2 factor on the lines per workgroup (contiguous, strided)
-  double* tfbuf;                     // Thomas-Fermi bisection: state {lo, hi, norm} and 1024 block rows
+  double* tfbuf;                     // Thomas-Fermi search: its state and block rows
+  int* tfflag; int* tfflag_dev;      // pinned host-mapped: the TF fixed point finished
   double* dgbuf;                     // diagnostics: 1024 block rows of 16 values
   double* khat[2];                   // [0] = K, [1] = W (virial) real-even spectra
   int kkind[2];                      // 0 none, 1 newton, 2 log, 3 table
@@ -300,9 +301,6 @@ void launch_abs2c(const double2* psi, long long n, double2* out, hipStream_t s);
 // multilinear resampling from an n0-point grid (first node x0, span (n0-1) dx0)
 // onto the n1-point grid of spacing dx1; per-block rows of sum |out|^2
 int launch_resample(const double2* in, int ndim, int n0, double x0, double span, int n1, double dx1, double2* out,
-                    double* rows, hipStream_t s);
-// one Thomas-Fermi norm evaluation (+ bisection step unless at_hi) on st = {lo, hi, norm}
-void launch_tf_eval(const double* base, long long n, double* st, int at_hi, double b2, double b3, double dv,
                     double* rows, hipStream_t s);
 void launch_widen(const float2* in, double2* out, long long n, hipStream_t s);
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,

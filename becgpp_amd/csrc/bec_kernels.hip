@@ -1820,64 +1820,6 @@ int launch_resample(const double2* in, int ndim, int n0, double x0, double span,
   return g;
 }
 
-// ---------------------------------------------------------------------------
-// Thomas-Fermi chemical potential by bisection on the device
-// (thomasfermi.py:33-62): rho(mu) = upper root of b3 rho^2 + b2 rho = mu - base,
-// clamped at 0; the norm sum rho dV is increasing in mu.  st = {lo, hi, norm}.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double tf_rho(double rhs, double b2, double b3) {
-  if (b3 > 1e-30) {
-    const double disc = b2 * b2 + 4.0 * b3 * rhs;
-    if (!(disc > 0.0)) return 0.0;
-    const double root = (-b2 + sqrt(disc)) / (2.0 * b3);
-    return root > 0.0 ? root : 0.0;
-  }
-  if (fabs(b2) > 1e-30) {
-    const double r = rhs / b2;
-    return r > 0.0 ? r : 0.0;
-  }
-  return 0.0;
-}
-// per-block sums of rho(m) over base; m = hi (at_hi) or (lo + hi) / 2
-__global__ __launch_bounds__(BEC_WG) void k_tf_norm(const double* __restrict__ base, long long n,
-                                                    const double* st, int at_hi, double b2, double b3,
-                                                    double* rows) {
-  const double m = at_hi ? st[1] : 0.5 * (st[0] + st[1]);
-  double v[1] = {0.0};
-  ELEM_LOOP(n) v[0] += tf_rho(m - base[i], b2, b3);
-  block_partials<1, 0>(v, nullptr, rows, nullptr, blockIdx.x);
-}
-// one workgroup: the rows folded in a fixed order, times dV; a bisection step
-// keeps the half whose ends bracket norm 1
-__global__ __launch_bounds__(BEC_WG) void k_tf_step(double* st, const double* __restrict__ rows, int nrows,
-                                                    double dv, int at_hi) {
-  __shared__ double sh[BEC_WG];
-  double acc = 0.0;
-  for (int r = threadIdx.x; r < nrows; r += blockDim.x) acc += rows[r];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double norm = sh[0] * dv;
-    st[2] = norm;
-    if (!at_hi) {
-      const double mid = 0.5 * (st[0] + st[1]);
-      if (norm < 1.0) st[0] = mid;
-      else st[1] = mid;
-    }
-  }
-}
-void launch_tf_eval(const double* base, long long n, double* st, int at_hi, double b2, double b3, double dv,
-                    double* rows, hipStream_t s) {
-  const int g = elem_blocks(n);
-  hipLaunchKernelGGL(k_tf_norm, dim3(g), dim3(BEC_WG), 0, s, base, n, st, at_hi, b2, b3, rows);
-  hipLaunchKernelGGL(k_tf_step, dim3(1), dim3(BEC_WG), 0, s, st, rows, g, dv, at_hi);
-}
-
-// FP64 <-> FP32 copies of nb complex fields of n points (phase changes)
 __global__ __launch_bounds__(BEC_WG) void k_narrow(const double2* __restrict__ in, float2* out, long long n) {
   ELEM_LOOP(n) out[i] = from_d2<float2>(in[i]);
 }

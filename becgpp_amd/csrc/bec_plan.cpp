@@ -587,6 +587,9 @@ int run_precond(Plan& p, const Batch& B, const double2* g, double2* out, const D
 static void plan_free(Plan& p) {
   solver_release(p);
   free_batch(p);
+  if (p.tfflag) (void)hipHostFree(p.tfflag);
+  p.tfflag = nullptr;
+  p.tfflag_dev = nullptr;
   void* ptrs[] = {p.twN, p.twM, p.khat[0], p.khat[1], p.tfbuf, p.dgbuf};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -968,37 +971,6 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
 // Thomas-Fermi chemical potential with unit norm (thomasfermi.py:48-62): hi
 // is widened while the norm at hi is below 1 (at most 200 times: NaN), then
 // `steps` bisection steps run on the device without a host round trip.
-extern "C" int bec_tf_mu(bec_plan* h, const double* base, double b2, double b3, double lo, double hi, int steps,
-                         double* mu_out) {
-  BEC_TRY({
-    if (!h || !base || !mu_out) bad("null argument");
-    if (steps < 0) bad("steps must be >= 0");
-    Plan& p = h->p;
-    if (!p.tfbuf) HIPCHK(hipMalloc(&p.tfbuf, sizeof(double) * (8 + 1024)));
-    double* st = p.tfbuf;
-    double* rows = p.tfbuf + 8;
-    double hs[4] = {lo, hi, 0.0, 0.0};
-    int guard = 0;
-    while (true) {
-      HIPCHK(hipMemcpyAsync(st, hs, sizeof(double) * 3, hipMemcpyHostToDevice, p.stream));
-      launch_tf_eval(base, p.n, st, 1, b2, b3, p.dv, rows, p.stream);
-      HIPCHK(hipMemcpyAsync(hs + 2, st + 2, sizeof(double), hipMemcpyDeviceToHost, p.stream));
-      HIPCHK(hipStreamSynchronize(p.stream));
-      if (!(hs[2] < 1.0)) break;
-      hs[1] = hs[0] + 2.0 * (hs[1] - hs[0]) + 1e-9;
-      if (++guard > 200) {
-        *mu_out = NAN;
-        return 0;
-      }
-    }
-    for (int k = 0; k < steps; ++k) launch_tf_eval(base, p.n, st, 0, b2, b3, p.dv, rows, p.stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(hs, st, sizeof(double) * 2, hipMemcpyDeviceToHost, p.stream));
-    HIPCHK(hipStreamSynchronize(p.stream));
-    *mu_out = 0.5 * (hs[0] + hs[1]);
-  });
-}
-
 // rows of a diagnostics launch, copied to the host (the caller folds them in order)
 static std::vector<double> diag_rows(Plan& p, int nblk, int kp) {
   std::vector<double> r((size_t)nblk * kp);

@@ -1,40 +1,58 @@
-"""Initial states: Gaussian, Thomas-Fermi envelope, triangular vortex lattice
-(becgpp/seeds.py:11-103).
+"""Initial states on the device (the reference's seeds.py:11-103).
 
-The envelopes follow the reference.  The random perturbations come from a
-host ``numpy.random.default_rng(rng)`` stream (amplitude noise first, then
-phase noise; the lattice draws its angle and shift before the noise), so a
-seed is the same on every device; the reference's seed tensors depend on the
-torch device RNG and are not reproducible across devices.
+Every seed is one launch of ``bec_seed``: an envelope (the Gaussian
+exp(-r^2/2), or the square root of the Thomas-Fermi density), optional
+vortex cores and phases, amplitude and phase noise, a central winding, then
+normalisation.  The three kinds of the reference:
+
+* gaussian -- exp(-r^2/2) (1 + seed_noise xi), times e^{i m theta} for
+  seed_winding m;
+* tf -- sqrt(n_TF) (1 + seed_noise xi_a) e^{i seed_phase_noise xi_p} e^{i m theta},
+  falling back to the Gaussian where no TF density exists;
+* triangular (|Omega| >= 0.3, converged TF density) -- an Abrikosov lattice of
+  vortex lines along z: spacing a_v = sqrt(2 pi / (sqrt(3) |Omega|)), rows
+  sqrt(3) a_v / 2 apart, rotated by a random angle in [0, pi/3) and shifted by
+  a random fraction of a_v in [-1/4, 1/4)^2 (numpy default_rng(rng), drawn as
+  the reference draws them), kept inside R_v = max(a_v, 0.98 R_90,perp);
+  sqrt(n_TF) prod_k tanh(|r - r_k| / (a_v/4)) (1 + seed_noise xi)
+  e^{i sign(Omega) sum_k theta_k}.
+
+The noise xi is the library's counter-based generator (``bec_noise``: a
+function of rng, stream and point index), so a seed is the same on every run
+and reproducible on the CPU; the reference draws it from torch's generator.
 """
+import ctypes
 import math
 
 import numpy as np
 import torch
 
+from . import _lib
 from .constants import DEV
-from .grid import dV, norm_of
+from .grid import _ptr, as_field, dV, norm_of, plan_of
 from .thomasfermi import extract_tf
 
 
-def _normal(rng, shape):
-    return torch.as_tensor(rng.standard_normal(shape), dtype=torch.float64, device=DEV)
-
-
-def _winding(psi, G, cfg):
-    m = int(cfg.get("seed_winding", 0))
-    if m != 0:
-        psi = psi * torch.exp(1j * m * torch.atan2(G["Y"], G["X"]))
+def _device_seed(G, cfg, env=None, sites=None, vortex_sign=1.0, core=1.0, phase_noise=0.0, winding=None):
+    pl = plan_of(G)
+    spec = _lib.SeedSpec(1 if env is not None else 0, float(cfg.get("seed_noise", 1e-2)), float(phase_noise),
+                         int(cfg.get("seed_winding", 0)) if winding is None else int(winding), float(vortex_sign),
+                         float(core), int(cfg.get("rng", 0)) & 0xFFFFFFFFFFFFFFFF)
+    e = as_field(env, G, torch.float64) if env is not None else None
+    v = None
+    nv = 0
+    if sites is not None and len(sites):
+        v = torch.as_tensor(np.ascontiguousarray(sites, dtype=np.float64).reshape(-1), device=DEV)
+        nv = len(sites)
+    psi = torch.empty(tuple(G["R2"].shape), dtype=torch.complex128, device=DEV)
+    _lib.check(pl.lib.bec_seed(pl.h, ctypes.byref(spec), _ptr(e) if e is not None else None,
+                               _ptr(v) if v is not None else None, nv, _ptr(psi)))
     return psi
 
 
 def gaussian_seed(G, cfg):
     """seeds.py:11-19."""
-    rng = np.random.default_rng(int(cfg.get("rng", 0)))
-    env = torch.exp(-0.5 * G["R2"])
-    noise = 1.0 + cfg.get("seed_noise", 1e-2) * _normal(rng, tuple(env.shape))
-    psi = _winding((env * noise).to(torch.complex128), G, cfg)
-    return psi / norm_of(psi, G)
+    return _device_seed(G, cfg)
 
 
 def tf_seed(G, cfg):
@@ -42,77 +60,62 @@ def tf_seed(G, cfg):
     tf = extract_tf(cfg, G)
     if tf is None or tf.get("n") is None:
         return gaussian_seed(G, cfg)
-    amp = torch.sqrt(torch.clamp(tf["n"], min=0.0))
-    rng = np.random.default_rng(int(cfg.get("rng", 0)))
-    noise = 1.0 + cfg.get("seed_noise", 1e-2) * _normal(rng, tuple(amp.shape))
-    pn = cfg.get("seed_phase_noise", 5e-2) * _normal(rng, tuple(amp.shape))
-    psi = _winding((amp * noise).to(torch.complex128) * torch.exp(1j * pn), G, cfg)
-    return psi / norm_of(psi, G)
+    return _device_seed(G, cfg, env=tf["n"], phase_noise=float(cfg.get("seed_phase_noise", 5e-2)))
 
 
-def _inplane_radius(n, G, frac=0.90):
-    """In-plane radius enclosing a mass fraction (seeds.py:37-44)."""
-    w = (torch.clamp(n, min=0.0) * dV(G)).reshape(-1)
-    rperp = torch.sqrt(G["X"] ** 2 + G["Y"] ** 2).reshape(-1)
-    order = torch.argsort(rperp, stable=True)
-    cum = torch.cumsum(w[order], dim=0)
-    target = torch.tensor([float(frac)], dtype=cum.dtype, device=cum.device) * cum[-1]
-    jj = int(torch.searchsorted(cum, target).clamp(max=cum.numel() - 1).item())
-    return rperp[order[jj]].item()
+def inplane_radius(n, G, frac=0.90):
+    """The in-plane (x-y) radius about the axis enclosing the mass fraction
+    (seeds.py:37-44): the first point, in order of distance from the z axis,
+    at which the running mass reaches frac of the total."""
+    dist = torch.sqrt(G["X"] ** 2 + G["Y"] ** 2).reshape(-1)
+    order = torch.argsort(dist, stable=True)
+    running = torch.cumsum((torch.clamp(n, min=0.0) * dV(G)).reshape(-1)[order], dim=0)
+    target = torch.tensor([float(frac)], dtype=running.dtype, device=running.device) * running[-1]
+    at = int(torch.searchsorted(running, target).clamp(max=running.numel() - 1).item())
+    return dist[order[at]].item()
+
+
+def lattice_sites(Omega, radius, rng):
+    """Triangular lattice of vortex positions for rotation Omega inside
+    `radius`, rotated and shifted by the draws of `rng` (seeds.py:61-82), in
+    row order (rows y, then x along a row)."""
+    av = math.sqrt(2.0 * math.pi / (math.sqrt(3.0) * abs(Omega)))
+    row = math.sqrt(3.0) * av / 2.0
+    angle = float(rng.uniform(0.0, math.pi / 3.0))
+    shift = rng.uniform(-0.25, 0.25, size=2) * av
+    nrow = int(math.ceil(radius / row)) + 2
+    ncol = int(math.ceil(radius / av)) + 3
+    jj, ii = np.meshgrid(np.arange(-nrow, nrow + 1), np.arange(-ncol, ncol + 1), indexing="ij")
+    x0 = (ii + 0.5 * (jj & 1)) * av + shift[0]
+    y0 = jj * row + shift[1]
+    ca, sa = math.cos(angle), math.sin(angle)
+    xs = (ca * x0 - sa * y0).reshape(-1)
+    ys = (sa * x0 + ca * y0).reshape(-1)
+    keep = xs * xs + ys * ys < radius * radius
+    return np.stack([xs[keep], ys[keep]], axis=1), av
 
 
 def triangular_seed(G, cfg):
-    """Abrikosov lattice of straight vortex lines along z on the TF envelope,
-    winding sign(Omega) (seeds.py:47-94)."""
-    O = float(cfg.get("Omega", 0.0))
-    if abs(O) < 0.3:
+    """seeds.py:47-94."""
+    om = float(cfg.get("Omega", 0.0))
+    if abs(om) < 0.3:
         return tf_seed(G, cfg)
     tf = extract_tf(cfg, G)
     if tf is None or not tf.get("converged", False):
         return tf_seed(G, cfg)
-    amp = torch.sqrt(torch.clamp(tf["n"], min=0.0))
-    wsign = -1.0 if O < 0 else 1.0
-    O = abs(O)
-    av = math.sqrt(2.0 * math.pi / (math.sqrt(3.0) * O))
-    dy = math.sqrt(3.0) * av / 2.0
-    Rperp = _inplane_radius(tf["n"], G, 0.90)
-    Rv = max(av, 0.98 * Rperp)
-    rng = np.random.default_rng(int(cfg.get("rng", 0)))
-    angle = float(rng.uniform(0.0, math.pi / 3.0))
-    shift = rng.uniform(-0.25, 0.25, size=2) * av
-    ca, sa = math.cos(angle), math.sin(angle)
-    pos = []
-    jmax = int(math.ceil(Rv / dy)) + 2
-    imax = int(math.ceil(Rv / av)) + 3
-    for jj in range(-jmax, jmax + 1):
-        for ii in range(-imax, imax + 1):
-            x0 = (ii + 0.5 * (jj & 1)) * av + shift[0]
-            y0 = jj * dy + shift[1]
-            xx = ca * x0 - sa * y0
-            yy = sa * x0 + ca * y0
-            if xx * xx + yy * yy < Rv * Rv:
-                pos.append((xx, yy))
-    phase = torch.zeros_like(amp)
-    core = torch.ones_like(amp)
-    xi = 0.25 * av
-    for xx, yy in pos:
-        dxt = G["X"] - xx
-        dyt = G["Y"] - yy
-        phase = phase + wsign * torch.atan2(dyt, dxt)
-        core = core * torch.tanh(torch.sqrt(dxt * dxt + dyt * dyt) / max(xi, 1e-12))
-    noise = cfg.get("seed_noise", 1e-2) * _normal(rng, tuple(amp.shape))
-    psi = (amp * core * (1.0 + noise)).to(torch.complex128) * torch.exp(1j * phase)
-    return psi / norm_of(psi, G)
+    av = math.sqrt(2.0 * math.pi / (math.sqrt(3.0) * abs(om)))
+    radius = max(av, 0.98 * inplane_radius(tf["n"], G, 0.90))
+    sites, av = lattice_sites(om, radius, np.random.default_rng(int(cfg.get("rng", 0))))
+    return _device_seed(G, cfg, env=tf["n"], sites=sites, vortex_sign=-1.0 if om < 0 else 1.0,
+                        core=max(0.25 * av, 1e-12), winding=0)
+
+
+_KINDS = {"triangular": triangular_seed, "tf": tf_seed}
 
 
 def make_seed(G, cfg):
-    """seeds.py:97-103."""
-    kind = cfg.get("seed", "tf")
-    if kind == "triangular":
-        return triangular_seed(G, cfg)
-    if kind == "tf":
-        return tf_seed(G, cfg)
-    return gaussian_seed(G, cfg)
+    """seeds.py:97-103: the seed kind cfg["seed"] (anything else: gaussian)."""
+    return _KINDS.get(cfg.get("seed", "tf"), gaussian_seed)(G, cfg)
 
 
 def noisy_gaussian(G, noise=1e-3, seed=0, width=1.0):

@@ -190,14 +190,60 @@ int bec_resample_noalloc(int ndim, int n0, double dx0, const void* psi_old, int 
 int bec_vortex_count(bec_plan* p, const void* psi, double xcm, double ycm, double R, double ell, double frac,
                      int* out4);
 
-/* Thomas-Fermi chemical potential of unit norm (thomasfermi.py:48-62, the
- * bisection of _bisect_mu_for_norm): rho(mu) is the upper root of
- * b3 rho^2 + b2 rho = mu - base (clamped at 0) on the plan's grid; hi is
- * widened while sum rho(hi) dV < 1 (at most 200 times, else *mu_out = NaN),
- * then `steps` bisection steps run on the device with no host round trip.
- * base: device float64 field (V_eff [+ phi]). */
+/* Thomas-Fermi chemical potential of unit norm (thomasfermi.py:48-62,
+ * _bisect_mu_for_norm): rho(mu) is the upper root of b3 rho^2 + b2 rho =
+ * mu - base (0 where there is none) on the plan's grid; hi is widened as
+ * lo + 2 (hi - lo) + 1e-9 while sum rho(hi) dV < 1 (at most 200 times, else
+ * *mu_out = NaN), then the bracket is narrowed by 2^steps (32 points priced
+ * per pass over base) and *mu_out is its midpoint: the crossing of the norm
+ * through 1, as the reference's `steps` halvings give it.  The search runs on
+ * the device with one synchronisation at the end.  base: device float64
+ * field (V_eff [+ phi]). */
 int bec_tf_mu(bec_plan* p, const double* base, double b2, double b3, double lo, double hi, int steps,
               double* mu_out);
+
+/* The compressed Thomas-Fermi fixed point with long range
+ * (thomasfermi.py:87-119): from n_io (device, the normalised initial density)
+ * each iteration forms phi = -G_C K*n (the plan's kernel spectrum), base =
+ * veff + phi, mu of unit norm on base (as bec_tf_mu with the bracket
+ * [min base, max base + |b2| + |b3| + 1] and 90 halvings), n_new = rho(mu)
+ * normalised, its L1 change dn, n <- (1 - relax) n + relax n_new
+ * normalised; it stops when dn < tol or after iters iterations, or early when
+ * mu cannot be found (out4[0] = NaN) or n_new vanishes.  The iteration state
+ * stays on the device (no per-iteration host synchronisation).
+ * out4 (host) = mu of the last iteration, dn, iterations, converged (1/0);
+ * n_io holds the final density. */
+int bec_tf_nonlocal(bec_plan* p, const double* veff, double b2, double b3, double G_C, int iters, double relax,
+                    double tol, double* n_io, double* out4);
+
+/* Seed fields (seeds.py:11-103): psi = A core (1 + amp_noise xi_a)
+ * exp(i (phase_noise xi_p + winding theta + vortex_sign sum_k theta_k)),
+ * normalised to unit norm (norm_of), with A = exp(-r^2/2) (envelope 0) or
+ * sqrt(max(env, 0)) of the device field env (envelope 1), core = prod_k
+ * tanh(|r - r_k| / core) over the n_vortices in-plane positions
+ * vortices[2k], vortices[2k+1] (device; lines along z in 3D), theta =
+ * atan2(y, x), theta_k = atan2(y - y_k, x - x_k).  xi_a and xi_p are the
+ * normals of bec_noise(rng, 0, i) and bec_noise(rng, 1, i) at point i
+ * (C order).  The reference draws its noise from torch's generator; the
+ * envelopes, vortex positions and the order of the factors are its. */
+typedef struct bec_seed_spec {
+  int envelope;
+  double amp_noise;
+  double phase_noise;
+  int winding;
+  double vortex_sign;
+  double core;
+  unsigned long long rng;
+} bec_seed_spec;
+int bec_seed(bec_plan* p, const bec_seed_spec* spec, const double* env, const double* vortices, int n_vortices,
+             void* psi_out);
+
+/* The seeds' noise on the host (no device work): out[k] = the standard normal
+ * of point first + k in stream `stream` of seed rng -- Box-Muller on outputs
+ * 2i and 2i+1 of the splitmix64 sequence seeded at key = output 0 of the
+ * sequence seeded at (rng mod 2^32) + 2^32 stream.  No reference counterpart. */
+int bec_noise(unsigned long long rng, unsigned long long stream, unsigned long long first, long long count,
+              double* out);
 
 /* ground_state loop (solvers.py:48-226): Riemannian Sobolev-preconditioned
  * CG with Polak-Ribiere beta, Barzilai-Borwein trial step and Armijo line

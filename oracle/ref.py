@@ -28,10 +28,14 @@ How each part is pinned (tests in ``tests/test_oracle_*.py``):
   sort on small inputs and torch.median's documented semantics.  The
   reference's own tests never observe sigma: beyond that definition it is
   parity unpinned.
-* seeds -- the RNG streams are this build's own (numpy ``default_rng``); the
-  reference's seed tensors depend on the torch device RNG and cannot be
-  reproduced, so seed arrays are parity unpinned by construction and every
-  parity test passes the same explicit psi0 to both sides.
+* seeds -- the noise is this build's counter-based generator
+  (``counter_normals``: splitmix64 + Box-Muller; the splitmix64 stream is
+  pinned by its published outputs for seed 1234567, the normals by their
+  moments); the reference's seed tensors depend on the torch device RNG and
+  cannot be reproduced.  The envelopes, lattice positions (numpy default_rng
+  angle and shift, as the reference draws them) and the order of the factors
+  are the reference's; the noise-free seeds are pinned by closed forms
+  (Gaussian envelope, TF parabola, vortex windings).
 """
 import math
 import os
@@ -823,8 +827,41 @@ def extract_tf(cfg, G, iters=400, relax=0.4):
 
 
 # ----------------------------------------------------------------------------
-# seeds (becgpp/seeds.py:11-103), with this build's numpy RNG streams
+# seeds (becgpp/seeds.py:11-103), with this build's counter-based noise
 # ----------------------------------------------------------------------------
+_SM_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _sm_mix(z):
+    """splitmix64's output function on uint64 arrays (wrapping arithmetic)."""
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix64_outputs(state, k):
+    """Outputs k (0-based, uint64 array) of the splitmix64 sequence seeded at state."""
+    with np.errstate(over="ignore"):
+        return _sm_mix(np.uint64(state) + (np.asarray(k, dtype=np.uint64) + np.uint64(1)) * _SM_GAMMA)
+
+
+def counter_normals(rng, stream, count, first=0):
+    """Standard normals of points first .. first+count-1 of (rng, stream): Box-Muller
+    on outputs 2i, 2i+1 of the splitmix64 sequence seeded at output 0 of the
+    sequence seeded at (rng mod 2^32) + 2^32 stream -- the seeds' noise."""
+    key = int(splitmix64_outputs((int(rng) & 0xFFFFFFFF) | (int(stream) << 32), 0))
+    i = np.arange(first, first + count, dtype=np.uint64)
+    a = splitmix64_outputs(key, np.uint64(2) * i)
+    c = splitmix64_outputs(key, np.uint64(2) * i + np.uint64(1))
+    u1 = ((a >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * 2.0 ** -53
+    u2 = (c >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)
+
+
+def _noise(cfg, stream, shape):
+    return counter_normals(int(cfg.get("rng", 0)), stream, int(np.prod(shape))).reshape(shape)
+
+
 def _winding(psi, G, cfg):
     m = int(cfg.get("seed_winding", 0))
     if m != 0:
@@ -833,23 +870,21 @@ def _winding(psi, G, cfg):
 
 
 def gaussian_seed(G, cfg):
-    """seeds.py:11-19; noise xi ~ N(0,1) from default_rng(rng)."""
-    rng = np.random.default_rng(int(cfg.get("rng", 0)))
+    """seeds.py:11-19; noise xi ~ N(0,1): counter_normals(rng, stream 0)."""
     env = np.exp(-0.5 * G["R2"])
-    noise = 1.0 + cfg.get("seed_noise", 1e-2) * rng.standard_normal(env.shape)
+    noise = 1.0 + cfg.get("seed_noise", 1e-2) * _noise(cfg, 0, env.shape)
     psi = _winding((env * noise).astype(np.complex128), G, cfg)
     return psi / norm_of(psi, G)
 
 
 def tf_seed(G, cfg):
-    """seeds.py:22-34; amplitude then phase noise from default_rng(rng)."""
+    """seeds.py:22-34; amplitude noise from stream 0, phase noise from stream 1."""
     tf = extract_tf(cfg, G)
     if tf is None or tf.get("n") is None:
         return gaussian_seed(G, cfg)
     amp = np.sqrt(np.maximum(tf["n"], 0.0))
-    rng = np.random.default_rng(int(cfg.get("rng", 0)))
-    noise = 1.0 + cfg.get("seed_noise", 1e-2) * rng.standard_normal(amp.shape)
-    pn = cfg.get("seed_phase_noise", 5e-2) * rng.standard_normal(amp.shape)
+    noise = 1.0 + cfg.get("seed_noise", 1e-2) * _noise(cfg, 0, amp.shape)
+    pn = cfg.get("seed_phase_noise", 5e-2) * _noise(cfg, 1, amp.shape)
     psi = _winding((amp * noise).astype(np.complex128) * np.exp(1j * pn), G, cfg)
     return psi / norm_of(psi, G)
 
@@ -865,7 +900,8 @@ def inplane_radius(n, G, frac=0.90):
 
 
 def triangular_seed(G, cfg):
-    """seeds.py:47-94; angle and shift then amplitude noise from default_rng(rng)."""
+    """seeds.py:47-94; angle and shift from numpy default_rng(rng) as in the
+    reference, amplitude noise from stream 0."""
     O = float(cfg.get("Omega", 0.0))
     if abs(O) < 0.3:
         return tf_seed(G, cfg)
@@ -902,7 +938,7 @@ def triangular_seed(G, cfg):
         dyt = G["Y"] - yy
         phase = phase + wsign * np.arctan2(dyt, dxt)
         core = core * np.tanh(np.sqrt(dxt * dxt + dyt * dyt) / max(xi, 1e-12))
-    noise = cfg.get("seed_noise", 1e-2) * rng.standard_normal(amp.shape)
+    noise = cfg.get("seed_noise", 1e-2) * _noise(cfg, 0, amp.shape)
     psi = (amp * core * (1.0 + noise)).astype(np.complex128) * np.exp(1j * phase)
     return psi / norm_of(psi, G)
 

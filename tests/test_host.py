@@ -299,3 +299,19 @@ def test_checkpoint_with_numpy_scalars_resumes(tmp_path, monkeypatch):
                records.ckpt_path("rid123"))
     got = records.load_checkpoint_psi("rid123")
     assert got is not None and torch.equal(got, psi)
+
+
+def test_library_noise_equals_the_oracle_generator():
+    """bec_noise (host code of the library, no device work) draws the numbers
+    the oracle's counter_normals draws: the same integers, so the normals agree
+    to the last-bit rounding of log and cos in the two maths libraries."""
+    import ctypes
+    import numpy as np
+    import oracle as O
+    from becgpp_amd import _lib
+    lib = _lib.load()
+    for rng, stream, first in ((0, 0, 0), (3, 1, 12345), (2**33 + 5, 0, 7)):
+        out = (ctypes.c_double * 64)()
+        assert lib.bec_noise(rng, stream, first, 64, out) == 0
+        ref = O.counter_normals(rng, stream, 64, first=first)
+        assert np.abs(np.array(out) - ref).max() <= 4e-16 * max(1.0, np.abs(ref).max())

@@ -202,3 +202,35 @@ def test_planted_mistakes_are_caught():
     rp[i0:i0 + G["N"], i0:i0 + G["N"]] = rho
     shifted = O.dV(G) * O.ifftn(O.fftn(rp) * Kf).real[i0 - 1:i0 - 1 + G["N"], i0 - 1:i0 - 1 + G["N"]]
     assert np.abs(shifted - good).max() > 1e-6 * np.abs(good).max()
+
+
+def test_counter_noise_generator_known_answers():
+    """The seeds' generator: splitmix64 reproduces its published first outputs
+    for seed 1234567; the Box-Muller normals have mean 0 and variance 1 (1e6
+    draws, 5 sigma) and independent streams."""
+    import numpy as np
+    out = O.splitmix64_outputs(1234567, np.arange(5))
+    assert [int(x) for x in out] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                     4593380528125082431, 16408922859458223821]
+    x = O.counter_normals(7, 0, 1_000_000)
+    assert abs(x.mean()) < 5e-3 and abs(x.var() - 1.0) < 7e-3
+    y = O.counter_normals(7, 1, 1_000_000)
+    assert abs(float((x * y).mean())) < 5e-3
+    # a window of the stream equals the same points drawn from the start
+    assert np.array_equal(O.counter_normals(7, 0, 10, first=500), x[500:510])
+
+
+def test_noise_free_seeds_closed_forms():
+    """With the noise switched off the seeds are their envelopes: the normalised
+    Gaussian (times e^{i m theta} for a winding m), sqrt of the TF density."""
+    import numpy as np
+    c = cfg(dimension="2D", Ngrid=64, L=6.0, beta2=50.0, seed_noise=0.0, seed_phase_noise=0.0)
+    G = O.make_grid(c)
+    g = np.exp(-0.5 * G["R2"]).astype(np.complex128)
+    g = g / O.norm_of(g, G)
+    assert np.abs(O.gaussian_seed(G, c) - g).max() < 1e-14
+    w = O.gaussian_seed(G, dict(c, seed_winding=2))
+    assert np.abs(w - g * np.exp(2j * np.arctan2(G["Y"], G["X"]))).max() < 1e-14
+    tf = O.extract_tf(c, G)
+    t = np.sqrt(tf["n"]).astype(np.complex128)
+    assert np.abs(O.tf_seed(G, c) - t / O.norm_of(t, G)).max() < 1e-14
