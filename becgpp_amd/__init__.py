@@ -12,7 +12,7 @@ from .interactions import long_range_phi, kernel_convolution, q2d_kernel, clear_
 from .operators import energy_components, energy_t, apply_H, compute_residual, observables, sensitivities
 from .thomasfermi import extract_tf, mass_quantiles, mass_radius, center_of_mass
 from .vortex import vortex_diagnostic, lll_weight
-from .diagnostics import diagnostics, droplet_radius, radial_average
+from .diagnostics import diagnostics, diagnostics_batch, droplet_radius, radial_average
 from .seeds import make_seed
 from .solvers import (ground_state, ground_state_batch, ground_state_multiseed, ground_state_uneven, solve_native,
                       solve_native_batch)

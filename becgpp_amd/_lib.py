@@ -73,6 +73,12 @@ SIGNATURES = {
     "bec_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double_p]),
     "bec_lll_weight": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, c_double_p,
                                       ctypes.POINTER(ctypes.c_int)]),
+    "bec_observe_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Phys), ctypes.c_void_p,
+                                         c_double_p]),
+    "bec_lll_weight_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, c_double_p, ctypes.c_int,
+                                            c_double_p]),
+    "bec_vortex_count_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, c_double_p,
+                                              ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
     "bec_plan_max_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "bec_resample": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),

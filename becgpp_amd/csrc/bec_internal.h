@@ -82,6 +82,7 @@ struct PassArgs {
   const double* khat; int H, M;        // real-even kernel spectrum, half table [H]^d
   const double* sigma; int Nk; double invNdx;   // per-instance Sobolev shifts and grid frequencies
   double gl2;                          // MUL_GAUSS: the line's factor exp(-gl2 k^2)
+  const double* gl2s;                  // MUL_GAUSS: per-instance gl2 (device; overrides gl2)
   const double* dot;                   // optional: rows of sum(out_re * dot) (dot laid out like out)
   const double* dot2;                  // optional with dot: a second sum(out_re * dot2) in the same rows
   double* partials;                    // per-block partial rows, device (nullable)
@@ -199,6 +200,7 @@ enum RedSlot {
   R_EFIRST = 0,    // 6: kin, lz, srho, sV, s2, s3
   R_EREST = 8,     // 2 per remaining axis: 8..11
   R_EGRAV = 12,    // 1: sum(phi_raw * rho)
+  R_WVIR = 14,     // 1: sum(rho * W*rho) (q2d virial kernel)
   R_HLAST = 16,    // 3: |h|^2, Re(psi* h), max rho
   R_NORM = 20,     // 1
   R_DOTS = 24,     // 16 inner products
@@ -282,8 +284,9 @@ int launch_dots(const double2* z, const double2* psi, const double2* d, const do
 int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out, long long n, double* hpart,
                 hipStream_t s);
 void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s);
+// per instance of nb (fields [b][n]): rows b * blocks + block
 int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
-                   hipStream_t s);
+                   hipStream_t s, int nb = 1);
 void launch_copy(const double2* src, double2* dst, long long n, int nb, Act mask, hipStream_t s);
 // device-driven loop, after the Armijo controller: BEST <- psi for FL_BEST,
 // D <- az Z + gam psi (table coefficients) for FL_FAIL
@@ -293,10 +296,12 @@ void launch_post(const double2* psi, double2* best, double2* d, const double2* z
 void launch_narrow(const double2* in, float2* out, long long n, hipStream_t s);
 // diagnostics: LLL projections of a chunk of m (rows of 3 sums per m), phases
 // and plaquette windings of the vortex count
-int launch_lll(const double2* psi, int N, double dx, double sgn, int m0, int mcnt, double* rows, hipStream_t s);
-int launch_vortex(const double2* psi, const double* nbar, int N, double dx, double xcm, double ycm, double R,
-                  double thr, double* rows, hipStream_t s);
-int launch_nbar(const double2* f, long long n, double* nbar, double* rows, hipStream_t s);
+// (batched over nb instances, fields [b][n]; per-instance parameters in the
+// device array prm; rows of instance b at b * blocks + block)
+int launch_lll(const double2* psi, int N, double dx, const double* prm, int m0, int nb, double* rows, hipStream_t s);
+int launch_vortex(const double2* psi, const double* nbar, int N, double dx, const double* prm, int nb, double* rows,
+                  hipStream_t s);
+int launch_nbar(const double2* f, long long n, int nb, double* nbar, double* rows, hipStream_t s);
 void launch_abs2c(const double2* psi, long long n, double2* out, hipStream_t s);
 // multilinear resampling from an n0-point grid (first node x0, span (n0-1) dx0)
 // onto the n1-point grid of spacing dx1; per-block rows of sum |out|^2

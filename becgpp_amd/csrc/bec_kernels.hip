@@ -728,7 +728,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
           const int k = lane + j * tpl;
           if (k >= L) continue;
           const double kx = kfreq(k, a.Nk, a.invNdx);
-          v[j] = cscale(v[j], exp(-a.gl2 * kx * kx));
+          v[j] = cscale(v[j], exp(-(a.gl2s ? a.gl2s[ob] : a.gl2) * kx * kx));
         }
       } else {
         // denominator 1/2 (kx^2 + ky^2 [+ kz^2]) + sigma, summed in that order
@@ -928,11 +928,13 @@ __global__ __launch_bounds__(BEC_WG) void k_res2(const double2* __restrict__ h,
 }
 
 // moments (one instance): mass, sum R2 rho, sum (X^2+Y^2) rho, sum Z^2 rho; max rho
-__global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ psi, int ndim, int N,
+// per instance (blockIdx.y): sum rho, r^2 rho, (x^2 + y^2) rho, z^2 rho, max rho
+__global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ psi0, int ndim, int N,
                                                     double dx, double* partials, double* hpart) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   double m[1] = {0.0};
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
+  const double2* psi = psi0 + (long long)blockIdx.y * n;
   ELEM_LOOP(n) {
     const double rho = cabs2(psi[i]);
     int ix, iy, iz = 0;
@@ -948,7 +950,7 @@ __global__ __launch_bounds__(BEC_WG) void k_moments(const double2* __restrict__ 
     v[3] += zz * zz * rho;
     m[0] = fmax(m[0], rho);
   }
-  block_partials<4, 1>(v, m, partials, hpart, blockIdx.x);
+  block_partials<4, 1>(v, m, partials, hpart, (long long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1646,14 +1648,20 @@ void launch_post(const double2* psi, double2* best, double2* d, const double2* z
 // block the sums |q_m|^2 and conj(q_m) psi (re, im), q_m = r^m exp(-r^2/2)
 // exp(i m theta) / sqrt(pi m!), theta = sgn atan2(y, x); log |q_m| advances
 // by log r - log(m)/2 from one m to the next, the phase by e^{i theta}.
-__global__ __launch_bounds__(BEC_WG) void k_lll(const double2* __restrict__ psi, int N, double dx, double sgn,
-                                                int m0, int mcnt, double* rows) {
+// per instance b (blockIdx.y): sgn = prm[2b], orbitals m0 .. m0 + mcnt - 1
+// with mcnt = prm[2b + 1] (at most 5; 0 gives zero rows)
+__global__ __launch_bounds__(BEC_WG) void k_lll(const double2* __restrict__ psi0, int N, double dx,
+                                                const double* __restrict__ prm, int m0, double* rows) {
   double v[15];
 #pragma unroll
   for (int q = 0; q < 15; ++q) v[q] = 0.0;
   const long long n = (long long)N * N;
+  const int b = blockIdx.y;
+  const double sgn = prm[2 * b];
+  const int mcnt = (int)prm[2 * b + 1];
+  const double2* psi = psi0 + (long long)b * n;
   const double lpi = 1.1447298858494002;           // log(pi)
-  ELEM_LOOP(n) {
+  if (mcnt > 0) ELEM_LOOP(n) {
     const int ix = (int)(i / N), iy = (int)(i % N);
     const double x = xcoord(ix, N, dx), y = sgn * xcoord(iy, N, dx);
     const double r2 = x * x + y * y, r = sqrt(r2);
@@ -1677,11 +1685,12 @@ __global__ __launch_bounds__(BEC_WG) void k_lll(const double2* __restrict__ psi,
       }
     }
   }
-  block_partials<15, 0>(v, nullptr, rows, nullptr, blockIdx.x);
+  block_partials<15, 0>(v, nullptr, rows, nullptr, (long long)blockIdx.y * gridDim.x + blockIdx.x);
 }
-int launch_lll(const double2* psi, int N, double dx, double sgn, int m0, int mcnt, double* rows, hipStream_t s) {
+int launch_lll(const double2* psi, int N, double dx, const double* prm, int m0, int nb, double* rows,
+               hipStream_t s) {
   const int g = elem_blocks((long long)N * N);
-  hipLaunchKernelGGL(k_lll, dim3(g), dim3(BEC_WG), 0, s, psi, N, dx, sgn, m0, mcnt, rows);
+  hipLaunchKernelGGL(k_lll, dim3(g, nb), dim3(BEC_WG), 0, s, psi, N, dx, prm, m0, rows);
   return g;
 }
 // wrap an angle difference into [-pi, pi): (d + pi) mod 2 pi - pi, the modulus
@@ -1696,11 +1705,16 @@ __device__ __forceinline__ double wrap_pi(double d) {
 // (ix,iy) -> (ix+1,iy) -> (ix+1,iy+1) -> (ix,iy+1) -> back, counted when the
 // corner-averaged smoothed density exceeds thr and corner (ix,iy) lies within
 // R of (xcm, ycm); rows of (number of windings >= 1, number <= -1) per block
-__global__ __launch_bounds__(BEC_WG) void k_vortex(const double2* __restrict__ psi,
-                                                   const double* __restrict__ nbar, int N, double dx,
-                                                   double xcm, double ycm, double R, double thr, double* rows) {
+// per instance b (blockIdx.y): xcm, ycm, R, thr = prm[4b .. 4b + 3]
+__global__ __launch_bounds__(BEC_WG) void k_vortex(const double2* __restrict__ psi0,
+                                                   const double* __restrict__ nbar0, int N, double dx,
+                                                   const double* __restrict__ prm, double* rows) {
   double v[2] = {0.0, 0.0};
   const long long np = (long long)(N - 1) * (N - 1);
+  const int b = blockIdx.y;
+  const double2* psi = psi0 + (long long)b * N * N;
+  const double* nbar = nbar0 + (long long)b * N * N;
+  const double xcm = prm[4 * b], ycm = prm[4 * b + 1], R = prm[4 * b + 2], thr = prm[4 * b + 3];
   ELEM_LOOP(np) {
     const int ix = (int)(i / (N - 1)), iy = (int)(i % (N - 1));
     const long long a0 = (long long)ix * N + iy, a1 = a0 + N, a2 = a1 + 1, a3 = a0 + 1;
@@ -1714,28 +1728,29 @@ __global__ __launch_bounds__(BEC_WG) void k_vortex(const double2* __restrict__ p
     if (w >= 1.0) v[0] += 1.0;
     if (w <= -1.0) v[1] += 1.0;
   }
-  block_partials<2, 0>(v, nullptr, rows, nullptr, blockIdx.x);
+  block_partials<2, 0>(v, nullptr, rows, nullptr, (long long)blockIdx.y * gridDim.x + blockIdx.x);
 }
-int launch_vortex(const double2* psi, const double* nbar, int N, double dx, double xcm, double ycm, double R,
-                  double thr, double* rows, hipStream_t s) {
+int launch_vortex(const double2* psi, const double* nbar, int N, double dx, const double* prm, int nb, double* rows,
+                  hipStream_t s) {
   const int g = elem_blocks((long long)(N - 1) * (N - 1));
-  hipLaunchKernelGGL(k_vortex, dim3(g), dim3(BEC_WG), 0, s, psi, nbar, N, dx, xcm, ycm, R, thr, rows);
+  hipLaunchKernelGGL(k_vortex, dim3(g, nb), dim3(BEC_WG), 0, s, psi, nbar, N, dx, prm, rows);
   return g;
 }
 // nbar = max(Re f, 0) (the smoothed density) and its per-block maxima
 __global__ __launch_bounds__(BEC_WG) void k_nbar(const double2* __restrict__ f, long long n, double* nbar,
                                                  double* rows) {
   double m[1] = {0.0};
+  const long long off = (long long)blockIdx.y * n;
   ELEM_LOOP(n) {
-    const double v = fmax(f[i].x, 0.0);
-    nbar[i] = v;
+    const double v = fmax(f[off + i].x, 0.0);
+    nbar[off + i] = v;
     m[0] = fmax(m[0], v);
   }
-  block_partials<0, 1>(nullptr, m, rows, nullptr, blockIdx.x);
+  block_partials<0, 1>(nullptr, m, rows, nullptr, (long long)blockIdx.y * gridDim.x + blockIdx.x);
 }
-int launch_nbar(const double2* f, long long n, double* nbar, double* rows, hipStream_t s) {
+int launch_nbar(const double2* f, long long n, int nb, double* nbar, double* rows, hipStream_t s) {
   const int g = elem_blocks(n);
-  hipLaunchKernelGGL(k_nbar, dim3(g), dim3(BEC_WG), 0, s, f, n, nbar, rows);
+  hipLaunchKernelGGL(k_nbar, dim3(g, nb), dim3(BEC_WG), 0, s, f, n, nbar, rows);
   return g;
 }
 // |psi|^2 as a complex field (its smoothing input)
@@ -1882,10 +1897,10 @@ int launch_res2(const double2* h, const double2* psi, double mu, double2* g_out,
   return gr;
 }
 int launch_moments(const double2* psi, int ndim, int N, double dx, double* partials, double* hpart,
-                   hipStream_t s) {
+                   hipStream_t s, int nb) {
   const long long n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
   const int gr = elem_blocks(n);
-  hipLaunchKernelGGL(k_moments, dim3(gr), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials, hpart);
+  hipLaunchKernelGGL(k_moments, dim3(gr, nb), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials, hpart);
   return gr;
 }
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,

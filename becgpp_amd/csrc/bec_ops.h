@@ -72,7 +72,7 @@ inline const double* hres(const Plan& p, int b) { return p.hred + (long long)b *
 // dot2 (with dot): a second sum(out * dot2) in the same rows; site: the
 // partial-row region of the sums (device loop) or their host fold
 int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot,
-             const double* dot2 = nullptr, int site = C_EGRAV);
+             const double* dot2 = nullptr, int site = C_EGRAV, int slot = R_EGRAV);
 // kinetic/L_z/local sums of psi - t[b] d into hred[b][R_EFIRST..], [R_EREST..];
 // rho_out (optional) receives |psi - t d|^2
 // Fusions of the device-driven loop into the first energy pass: instances

@@ -274,7 +274,7 @@ static PassArgs pass_base(const Plan& p, const FftPlan& fp, const LineCfg& lc, c
 }
 
 int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out,
-             const double* dot, const double* dot2, int site) {
+             const double* dot, const double* dot2, int site, int slot) {
   if (!p.khat[which]) throw BecError(1, "kernel spectrum not built for this plan");
   const int N = p.N, M = p.M, H = p.H, i0 = (p.M - p.N) / 2;
   const double fac = scale * p.dv / (double)p.nM;
@@ -328,7 +328,7 @@ int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale
     if (dot) (B.dev ? c5.partials : c5.hpartials) = B.dev ? part_of(p, site) : hpart_of(p, site);
     nb_rows = launch_pass(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE, c5, p.stream);
   }
-  if (dot && !B.dev) add_job(p, site, nb_rows, dot2 ? 2 : 1, 0, R_EGRAV, B);
+  if (dot && !B.dev) add_job(p, site, nb_rows, dot2 ? 2 : 1, 0, slot, B);
   return nb_rows;
 }
 
@@ -971,65 +971,147 @@ extern "C" int bec_precondition(bec_plan* h, const void* g, const void* psi, con
 // Thomas-Fermi chemical potential with unit norm (thomasfermi.py:48-62): hi
 // is widened while the norm at hi is below 1 (at most 200 times: NaN), then
 // `steps` bisection steps run on the device without a host round trip.
-// rows of a diagnostics launch, copied to the host (the caller folds them in order)
-static std::vector<double> diag_rows(Plan& p, int nblk, int kp) {
-  std::vector<double> r((size_t)nblk * kp);
+// Diagnostics over a batch of nb states of one grid (fields [b][n]): every
+// kernel launch covers all the states, their rows are copied to the host once
+// per launch and folded per state in row order.
+#define DIAG_ROWS (1024 * 16)
+static double* diag_rows_dev(Plan& p) { return p.dgbuf; }
+static double* diag_prm_dev(Plan& p) { return p.dgbuf + (size_t)DIAG_ROWS * BEC_BMAX; }
+static void diag_alloc(Plan& p) {
+  if (!p.dgbuf) HIPCHK(hipMalloc(&p.dgbuf, sizeof(double) * ((size_t)DIAG_ROWS * BEC_BMAX + 4 * BEC_BMAX)));
+}
+// rows of a diagnostics launch over nb states (g rows of kp values each)
+static std::vector<double> diag_rows(Plan& p, int g, int kp, int nb) {
+  std::vector<double> r((size_t)g * kp * nb);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(r.data(), p.dgbuf, sizeof(double) * r.size(), hipMemcpyDeviceToHost, p.stream));
   HIPCHK(hipStreamSynchronize(p.stream));
   return r;
 }
-static void diag_alloc(Plan& p) {
-  if (!p.dgbuf) HIPCHK(hipMalloc(&p.dgbuf, sizeof(double) * 1024 * 16));
+static void diag_params(Plan& p, const std::vector<double>& prm) {
+  HIPCHK(hipMemcpyAsync(diag_prm_dev(p), prm.data(), sizeof(double) * prm.size(), hipMemcpyHostToDevice,
+                        p.stream));
+}
+static void check_batch(int nb) {
+  if (nb < 1 || nb > BEC_BMAX) bad("batch size must be in [1, " + std::to_string(BEC_BMAX) + "]");
 }
 
-// Weight of psi in the lowest Landau level of the rotating harmonic trap
-// (vortex.py:61-84): sum over m <= Mcap of |<q_m, psi>|^2 / (<q_m,q_m> <psi,psi>),
-// q_m = z^m exp(-r^2/2) (conj(z)^m for Omega < 0), Mcap from <r^2> and L.
-extern "C" int bec_lll_weight(bec_plan* h, const void* psi, double Omega, int Mmax, double* w_out, int* mcap_out) {
-  BEC_TRY({
-    if (!h || !psi || !w_out) bad("null argument");
-    Plan& p = h->p;
-    if (p.ndim != 2) bad("the LLL weight is two-dimensional only");
-    diag_alloc(p);
-    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream),
-            4, 1, R_MOM, one_instance());
-    HIPCHK(hipGetLastError());
-    plan_sync(p);
-    const double* mom = hres(p, 0) + R_MOM;
-    const double nrm2 = mom[0];
-    const double r2mean = nrm2 > 0.0 ? mom[1] / nrm2 : 0.0;
+// per state: sum rho, r^2 rho, (x^2 + y^2) rho, z^2 rho, max rho (before dV)
+static void moments_batch(Plan& p, int nb, const double2* psi, double* out5) {
+  plan_reserve(p, nb);
+  const Batch B{nb, nb >= 64 ? ~0ull : ((1ull << nb) - 1ull)};
+  add_job(p, C_MOM, launch_moments(psi, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream, nb), 4, 1, R_MOM,
+          B);
+  HIPCHK(hipGetLastError());
+  plan_sync(p);
+  for (int b = 0; b < nb; ++b) memcpy(out5 + 5 * b, hres(p, b) + R_MOM, sizeof(double) * 5);
+}
+
+// Weight in the lowest Landau level (vortex.py:61-84) of each state: the
+// orbitals m = 0 .. Mcap (Mcap from the state's <r^2> and L), five per launch
+// for every state at once.
+static void lll_batch(Plan& p, int nb, const double2* psi, const double* Omega, int Mmax, double* w_out,
+                      int* mcap_out) {
+  if (p.ndim != 2) bad("the LLL weight is two-dimensional only");
+  diag_alloc(p);
+  std::vector<double> mom(5 * (size_t)nb);
+  moments_batch(p, nb, psi, mom.data());
+  std::vector<int> mcap(nb);
+  int mmax = 0;
+  for (int b = 0; b < nb; ++b) {
+    const double nrm2 = mom[5 * b];
+    const double r2mean = nrm2 > 0.0 ? mom[5 * b + 1] / nrm2 : 0.0;
     const int est = (int)ceil(r2mean + 10.0 * sqrt(std::max(r2mean, 1.0)) + 30.0);
     const int cap2 = std::max(Mmax, (int)((0.80 * p.Lbox) * (0.80 * p.Lbox)));
-    const int Mcap = std::min(std::max(Mmax, est), cap2);
-    const double sgn = Omega >= 0.0 ? 1.0 : -1.0;
-    double w = 0.0;
-    for (int m0 = 0; m0 <= Mcap; m0 += 5) {
-      const int mcnt = std::min(5, Mcap + 1 - m0);
-      const int g = launch_lll((const double2*)psi, p.N, p.dx, sgn, m0, mcnt, p.dgbuf, p.stream);
-      const std::vector<double> r = diag_rows(p, g, 16);
+    mcap[b] = std::min(std::max(Mmax, est), cap2);
+    mmax = std::max(mmax, mcap[b]);
+  }
+  std::vector<double> w(nb, 0.0), prm(2 * (size_t)nb);
+  for (int m0 = 0; m0 <= mmax; m0 += 5) {
+    for (int b = 0; b < nb; ++b) {
+      prm[2 * b] = Omega[b] >= 0.0 ? 1.0 : -1.0;
+      prm[2 * b + 1] = std::max(0, std::min(5, mcap[b] + 1 - m0));
+    }
+    diag_params(p, prm);
+    const int g = launch_lll(psi, p.N, p.dx, diag_prm_dev(p), m0, nb, diag_rows_dev(p), p.stream);
+    const std::vector<double> r = diag_rows(p, g, 16, nb);
+    for (int b = 0; b < nb; ++b) {
+      const int mcnt = (int)prm[2 * b + 1];
+      const double nrm2 = mom[5 * b];
       for (int q = 0; q < mcnt; ++q) {
         double nq = 0.0, re = 0.0, im = 0.0;
-        for (int b = 0; b < g; ++b) {
-          nq += r[(size_t)b * 16 + 3 * q];
-          re += r[(size_t)b * 16 + 3 * q + 1];
-          im += r[(size_t)b * 16 + 3 * q + 2];
+        for (int k = 0; k < g; ++k) {
+          const double* row = r.data() + ((size_t)b * g + k) * 16;
+          nq += row[3 * q];
+          re += row[3 * q + 1];
+          im += row[3 * q + 2];
         }
-        if (nq > 0.0 && nrm2 > 0.0) w += (re * re + im * im) / (nq * nrm2);
+        if (nq > 0.0 && nrm2 > 0.0) w[b] += (re * re + im * im) / (nq * nrm2);
       }
     }
-    *w_out = std::min(std::max(w, 0.0), 1.0);
-    if (mcap_out) *mcap_out = Mcap;
-  });
+  }
+  for (int b = 0; b < nb; ++b) {
+    w_out[b] = std::min(std::max(w[b], 0.0), 1.0);
+    if (mcap_out) mcap_out[b] = mcap[b];
+  }
 }
 
-// Vortex count of a 2D state (vortex.py:10-48): the density smoothed with
-// exp(-ell^2 k^2 / 2) (two line passes), then the plaquettes' phase windings
-// where the corner-averaged smoothed density exceeds frac * its maximum and
-// the corner lies within R of (xcm, ycm).  out4 = {Nv, N+, N-, Nnet}.
-// Resampling between grids (grid.py:140-161): needs no plan (either grid may
-// be one the line transforms do not cover); its reduction rows live in a
-// temporary buffer of this call.
+// Vortex count of each 2D state (vortex.py:10-48): the density smoothed with
+// exp(-ell^2 k^2 / 2) (two line passes, ell per state), then the plaquettes'
+// phase windings where the corner-averaged smoothed density exceeds frac
+// times its maximum and the corner lies within R of (xcm, ycm).
+// prm4[4b..4b+3] = xcm, ycm, R, ell; out4[4b..] = {Nv, N+, N-, Nnet}.
+static void vortex_batch(Plan& p, int nb, const double2* psi, const double* prm4, double frac, int* out4) {
+  if (p.ndim != 2) bad("the vortex count takes a two-dimensional state (a 3D state's midplane)");
+  diag_alloc(p);
+  plan_reserve(p, nb);
+  const int N = p.N;
+  const long long n = p.n;
+  const Batch B{nb, nb >= 64 ? ~0ull : ((1ull << nb) - 1ull)};
+  std::vector<double> prm(4 * (size_t)nb);
+  for (int b = 0; b < nb; ++b) prm[b] = 0.5 * prm4[4 * b + 3] * prm4[4 * b + 3];
+  diag_params(p, prm);                                  // gl2 per state
+  launch_abs2c(psi, n * nb, p.w1, p.stream);
+  PassArgs a = pass_base(p, p.fpN, p.lcN, B);          // x: lines along ix (stride N)
+  a.nl = N; a.ni = N; a.in_si = 1; a.in_s = N; a.in_ib = n; a.n_load = N; a.out_si = 1; a.out_s = N;
+  a.out_ib = n; a.n_store = N;
+  a.in = p.w1; a.out = p.w1; a.gl2s = diag_prm_dev(p); a.out_scale = 1.0 / N;
+  launch_pass(0, LD_C, DIR_FMI, MUL_GAUSS, ST_C, a, p.stream);
+  PassArgs c = pass_base(p, p.fpN, p.lcN, B);          // y: contiguous lines
+  c.nl = N; c.in_so = N; c.in_s = 1; c.in_ib = n; c.n_load = N; c.out_so = N; c.out_s = 1; c.out_ib = n;
+  c.n_store = N;
+  c.in = p.w1; c.out = p.w1; c.gl2s = diag_prm_dev(p); c.out_scale = 1.0 / N;
+  launch_pass(1, LD_C, DIR_FMI, MUL_GAUSS, ST_C, c, p.stream);
+  const int g = launch_nbar(p.w1, n, nb, p.rho, diag_rows_dev(p), p.stream);
+  std::vector<double> r = diag_rows(p, g, 1, nb);
+  for (int b = 0; b < nb; ++b) {
+    double mx = 0.0;
+    for (int k = 0; k < g; ++k) mx = std::max(mx, r[(size_t)b * g + k]);
+    prm[4 * b] = prm4[4 * b];
+    prm[4 * b + 1] = prm4[4 * b + 1];
+    prm[4 * b + 2] = prm4[4 * b + 2];
+    prm[4 * b + 3] = frac * mx;
+  }
+  diag_params(p, prm);
+  const int gv = launch_vortex(psi, p.rho, N, p.dx, diag_prm_dev(p), nb, diag_rows_dev(p), p.stream);
+  r = diag_rows(p, gv, 2, nb);
+  for (int b = 0; b < nb; ++b) {
+    double np_ = 0.0, nm = 0.0;
+    for (int k = 0; k < gv; ++k) {
+      np_ += r[((size_t)b * gv + k) * 2];
+      nm += r[((size_t)b * gv + k) * 2 + 1];
+    }
+    const int ip = (int)llround(np_), im = (int)llround(nm);
+    out4[4 * b] = ip + im;
+    out4[4 * b + 1] = ip;
+    out4[4 * b + 2] = im;
+    out4[4 * b + 3] = ip - im;
+  }
+}
+
+
+
+// Resampling between grids (grid.py:140-161): see include/becgpp_hip.h.
 extern "C" int bec_resample(int ndim, int n0, double dx0, const void* psi_old, int n1, double dx1, void* psi_new,
                             double* norm_out, void* stream) {
   BEC_TRY({
@@ -1067,52 +1149,84 @@ extern "C" int bec_resample(int ndim, int n0, double dx0, const void* psi_old, i
   });
 }
 
+extern "C" int bec_lll_weight(bec_plan* h, const void* psi, double Omega, int Mmax, double* w_out, int* mcap_out) {
+  BEC_TRY({
+    if (!h || !psi || !w_out) bad("null argument");
+    lll_batch(h->p, 1, (const double2*)psi, &Omega, Mmax, w_out, mcap_out);
+  });
+}
+
+extern "C" int bec_lll_weight_batch(bec_plan* h, int nb, const void* psi, const double* Omega, int Mmax,
+                                    double* w_out) {
+  BEC_TRY({
+    if (!h || !psi || !Omega || !w_out) bad("null argument");
+    check_batch(nb);
+    lll_batch(h->p, nb, (const double2*)psi, Omega, Mmax, w_out, nullptr);
+  });
+}
+
 extern "C" int bec_vortex_count(bec_plan* h, const void* psi, double xcm, double ycm, double R, double ell,
                                 double frac, int* out4) {
   BEC_TRY({
     if (!h || !psi || !out4) bad("null argument");
-    Plan& p = h->p;
-    if (p.ndim != 2) bad("the vortex count takes a two-dimensional state (a 3D state's midplane)");
-    diag_alloc(p);
-    const int N = p.N;
-    const long long n = p.n;
-    const Batch B = one_instance();
-    launch_abs2c((const double2*)psi, n, p.w1, p.stream);
-    PassArgs a = pass_base(p, p.fpN, p.lcN, B);        // x: lines along ix (stride N)
-    a.nl = N; a.ni = N; a.in_si = 1; a.in_s = N; a.n_load = N; a.out_si = 1; a.out_s = N; a.n_store = N;
-    a.in = p.w1; a.out = p.w1; a.gl2 = 0.5 * ell * ell; a.out_scale = 1.0 / N;
-    launch_pass(0, LD_C, DIR_FMI, MUL_GAUSS, ST_C, a, p.stream);
-    PassArgs c = pass_base(p, p.fpN, p.lcN, B);        // y: contiguous lines
-    c.nl = N; c.in_so = N; c.in_s = 1; c.n_load = N; c.out_so = N; c.out_s = 1; c.n_store = N;
-    c.in = p.w1; c.out = p.w1; c.gl2 = 0.5 * ell * ell; c.out_scale = 1.0 / N;
-    launch_pass(1, LD_C, DIR_FMI, MUL_GAUSS, ST_C, c, p.stream);
-    const int g = launch_nbar(p.w1, n, p.rho, p.dgbuf, p.stream);
-    std::vector<double> r = diag_rows(p, g, 1);
-    double mx = 0.0;
-    for (int b = 0; b < g; ++b) mx = std::max(mx, r[b]);
-    const int gv = launch_vortex((const double2*)psi, p.rho, N, p.dx, xcm, ycm, R, frac * mx, p.dgbuf, p.stream);
-    r = diag_rows(p, gv, 2);
-    double np_ = 0.0, nm = 0.0;
-    for (int b = 0; b < gv; ++b) {
-      np_ += r[(size_t)b * 2];
-      nm += r[(size_t)b * 2 + 1];
-    }
-    const int ip = (int)llround(np_), im = (int)llround(nm);
-    out4[0] = ip + im;
-    out4[1] = ip;
-    out4[2] = im;
-    out4[3] = ip - im;
+    const double prm[4] = {xcm, ycm, R, ell};
+    vortex_batch(h->p, 1, (const double2*)psi, prm, frac, out4);
+  });
+}
+
+extern "C" int bec_vortex_count_batch(bec_plan* h, int nb, const void* psi, const double* prm4, double frac,
+                                      int* out4) {
+  BEC_TRY({
+    if (!h || !psi || !prm4 || !out4) bad("null argument");
+    check_batch(nb);
+    vortex_batch(h->p, nb, (const double2*)psi, prm4, frac, out4);
   });
 }
 
 extern "C" int bec_moments(bec_plan* h, const void* psi, double* out5) {
   BEC_TRY({
     if (!h || !psi || !out5) bad("null argument");
+    moments_batch(h->p, 1, (const double2*)psi, out5);
+  });
+}
+
+// The quantities observables() needs, for nb states of one grid in one set of
+// launches: out[16 b ..] = E, Ekin, Etrap, Econtact, Ethree, Erot, Egrav, Lz
+// (as bec_energy with ph[b]), sum rho, sum r^2 rho, sum (x^2+y^2) rho,
+// sum z^2 rho, max rho (before dV), and sum rho (W*rho) dV (the q2d virial
+// kernel's term, when the plan holds that spectrum and G_C != 0; else NaN).
+extern "C" int bec_observe_batch(bec_plan* h, int nb, const bec_phys* php, const void* psi, double* out) {
+  BEC_TRY({
+    if (!h || !php || !psi || !out) bad("null argument");
+    check_batch(nb);
     Plan& p = h->p;
-    add_job(p, C_MOM, launch_moments((const double2*)psi, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream),
-            4, 1, R_MOM, one_instance());
+    plan_reserve(p, nb);
+    std::vector<Phys> ph(nb);
+    unsigned long long all = nb >= 64 ? ~0ull : ((1ull << nb) - 1ull), lr = 0, wm = 0;
+    for (int b = 0; b < nb; ++b) {
+      ph[b] = phys_of(php + b);
+      if (plan_long_range(p, ph[b])) {
+        lr |= 1ull << b;
+        if (p.khat[1]) wm |= 1ull << b;
+      }
+    }
+    const double2* f = (const double2*)psi;
+    run_energy_sums(p, Batch{nb, all}, f, nullptr, nullptr, p.rho);
+    if (lr) run_conv(p, 0, Batch{nb, lr}, p.rho, 1.0, p.loc, p.rho);
+    if (wm) run_conv(p, 1, Batch{nb, wm}, p.rho, 1.0, p.cand, p.rho, nullptr, C_EGRAV2, R_WVIR);
+    add_job(p, C_MOM, launch_moments(f, p.ndim, p.N, p.dx, nullptr, hpart_of(p, C_MOM), p.stream, nb), 4, 1, R_MOM,
+            Batch{nb, all});
     HIPCHK(hipGetLastError());
     plan_sync(p);
-    memcpy(out5, hres(p, 0) + R_MOM, sizeof(double) * 5);
+    for (int b = 0; b < nb; ++b) {
+      const bool l = (lr >> b) & 1ull;
+      const Energy e = energy_from_sums(p, b, ph[b], l, false, -ph[b].G_C);
+      double* o = out + 16 * (size_t)b;
+      const double v[8] = {e.E, e.Ekin, e.Etrap, e.Econtact, e.Ethree, e.Erot, e.Egrav, e.Lz};
+      memcpy(o, v, sizeof(v));
+      memcpy(o + 8, hres(p, b) + R_MOM, sizeof(double) * 5);
+      o[13] = ((wm >> b) & 1ull) ? hres(p, b)[R_WVIR] * p.dv : NAN;
+      o[14] = o[15] = 0.0;
+    }
   });
 }

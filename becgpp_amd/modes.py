@@ -28,7 +28,7 @@ from .interactions import long_range_phi
 from .operators import energy_components
 from .solvers import batchable, ground_state, ground_state_batch, ground_state_multiseed
 from .thomasfermi import extract_tf
-from .diagnostics import diagnostics
+from .diagnostics import diagnostics, diagnostics_batch
 from .seeds import make_seed
 from . import records
 
@@ -50,11 +50,12 @@ def run_id(cfg):
     return f"{cfg.get('tag', 'becgpp')}_{hashlib.md5(raw.encode()).hexdigest()[:10]}"
 
 
-def _record(psi, obs, G, cfg, rid):
+def _record(psi, obs, G, cfg, rid, diag=None):
     """The run's record: the solver's observables and the diagnostic bundle of
-    the state (diagnostics.py:18-47), then the run's identity (modes.py:227-236)."""
+    the state (diagnostics.py:18-47; given when computed for a batch of
+    states), then the run's identity (modes.py:227-236)."""
     d = {k: v for k, v in obs.items() if k != "trace"}
-    d.update(diagnostics(psi, G, cfg))
+    d.update(diagnostics(psi, G, cfg) if diag is None else diag)
     d.update(rid=rid, seed_used=obs.get("seed_used", cfg.get("seed")),
              seed_energy_spread=obs.get("seed_energy_spread", 0.0),
              dimension=cfg["dimension"], kernel=G["kernel"], L=cfg["L"], N=cfg["Ngrid"], s=cfg["s"],
@@ -167,9 +168,16 @@ def mode_sweep(cfg):
                 res = ground_state_batch([cj for _, cj in jobs], G=G)
             else:
                 res = [ground_state(cj, G=G, verbose=False) for _, cj in jobs]
+            # the selected state of every point of the grid, and their
+            # diagnostic bundle in one batch
+            sel = {}
             for ip in ips:
                 cands = [(f"seed:{cj['seed']}", r) for (jp, cj), r in zip(jobs, res) if jp == ip]
-                rows[ip] = _pick(cands, pts[ip], param, values[ip])
+                sel[ip] = min(cands, key=lambda t: t[1][2]["E"])
+            diags = diagnostics_batch([sel[ip][1][0] for ip in ips], G, [pts[ip] for ip in ips])
+            for ip, dg in zip(ips, diags):
+                cands = [(f"seed:{cj['seed']}", r) for (jp, cj), r in zip(jobs, res) if jp == ip]
+                rows[ip] = _pick(cands, pts[ip], param, values[ip], dg)
         for ip in range(len(pts)):
             _sweep_record(cfg, param, pts, rows, ip)
         return rows
@@ -217,10 +225,10 @@ def _sweep_record(cfg, param, pts, rows, ip):
     records.write_csv(os.path.join(records.out_dir(), f"sweep_{stem}.csv"), plain)
 
 
-def _pick(cands, c, param, val):
+def _pick(cands, c, param, val, diag=None):
     cands = sorted(cands, key=lambda t: t[1][2]["E"])      # variational: lowest energy wins
     branch, (psi, G, obs) = cands[0]
-    d = _record(psi, obs, G, c, run_id(c))
+    d = _record(psi, obs, G, c, run_id(c), diag)
     d.update({param: val}, branch=branch, walltime=sum(t[1][2]["walltime"] for t in cands),
              candidate_E=";".join(f"{t[0]}={t[1][2]['E']:.10g}" for t in cands), psi=psi)
     print(f"    {param}={val} [{branch}] E={d['E']:.8f} mu={d['mu']:.5f} Lz={d['Lz']:.4f} "

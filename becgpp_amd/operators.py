@@ -167,17 +167,16 @@ def moments(psi, G):
     return tuple(out)
 
 
-def observables(p, G, cfg):
-    """operators.py:55-103: E, mu, Lz, radii, peak and the d-dimensional
-    virial identity with the long-range term of each kernel."""
+def _observables_from(comp, mom, wsum, G, cfg):
+    """observables (operators.py:55-103) from the energy components, the
+    moments (sum rho, r^2 rho, (x^2+y^2) rho, z^2 rho, max rho before dV) and,
+    for the q2d kernel, sum rho (W*rho) dV."""
     b2, b3, O, gc = cfg["beta2"], cfg["beta3"], cfg["Omega"], cfg["G_C"]
     d = float(G["ndim"])
     s = float(G["s"])
     dv = dV(G)
-    t = as_field(p, G)
-    comp, _ = _energy_host(t, G, b2, b3, O, gc, False)
     E = comp["E"]
-    msum, r2sum, rperp2, z2, peak = moments(t, G)
+    msum, r2sum, rperp2, z2, peak = mom
     mass = msum * dv
     mu = E + comp["Econtact"] + 2.0 * comp["Ethree"] + comp["Egrav"]
     r2mean = r2sum * dv
@@ -189,8 +188,7 @@ def observables(p, G, cfg):
     elif kind == "log":
         T_lr = -0.5 * gc * mass * mass
     else:
-        rho = (t.real ** 2 + t.imag ** 2).contiguous()
-        T_lr = 0.5 * gc * (rho * kernel_convolution(rho, G, "W")).sum().item() * dv
+        T_lr = 0.5 * gc * wsum
     virial = (2.0 * comp["Ekin"] - s * comp["Etrap"] + d * comp["Econtact"] + 2.0 * d * comp["Ethree"]) + T_lr
     vscale = sum(abs(comp[k]) for k in ("Ekin", "Etrap", "Econtact", "Ethree", "Egrav")) + abs(T_lr)
     binding_ref = 1.0 if (abs(abs(O) - 1.0) < 1e-12 and s == 2 and G["ndim"] == 2) else float("nan")
@@ -205,6 +203,42 @@ def observables(p, G, cfg):
         out["oblateness"] = (out["R_perp_rms"] / math.sqrt(2.0)) / max(out["R_z_rms"], 1e-30)
     else:
         out["R_perp_rms"] = out["R_z_rms"] = out["oblateness"] = float("nan")
+    return out
+
+
+def observables(p, G, cfg):
+    """operators.py:55-103: E, mu, Lz, radii, peak and the d-dimensional
+    virial identity with the long-range term of each kernel."""
+    b2, b3, O, gc = cfg["beta2"], cfg["beta3"], cfg["Omega"], cfg["G_C"]
+    t = as_field(p, G)
+    comp, _ = _energy_host(t, G, b2, b3, O, gc, False)
+    wsum = float("nan")
+    if G["kernel"] == "q2d" and abs(gc) > 1e-15:
+        rho = (t.real ** 2 + t.imag ** 2).contiguous()
+        wsum = (rho * kernel_convolution(rho, G, "W")).sum().item() * dV(G)
+    return _observables_from(comp, moments(t, G), wsum, G, cfg)
+
+
+def observables_batch(psis, G, cfgs):
+    """observables of several states of one grid (up to 64 per call), every
+    kernel launch covering all of them (bec_observe_batch)."""
+    out = []
+    for lo in range(0, len(cfgs), 64):
+        cs = cfgs[lo:lo + 64]
+        nb = len(cs)
+        q2d = G["kernel"] == "q2d" and any(abs(float(c["G_C"])) > 1e-15 for c in cs)
+        lr = G["kernel"] != "none" and any(abs(float(c["G_C"])) > 1e-15 for c in cs)
+        pl = ensure_kernel(G, "K") if lr else plan_of(G)
+        if q2d:
+            ensure_kernel(G, "W")
+        t = torch.stack([as_field(x, G) for x in psis[lo:lo + 64]]).contiguous()
+        ph = (_lib.Phys * nb)(*[phys_for(G, c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cs])
+        raw = (ctypes.c_double * (16 * nb))()
+        _lib.check(pl.lib.bec_observe_batch(pl.h, nb, ph, _ptr(t), raw))
+        for b, c in enumerate(cs):
+            r = raw[16 * b:16 * b + 16]
+            comp = {k: r[i] for i, k in enumerate(_COMP_KEYS)}
+            out.append(_observables_from(comp, tuple(r[8:13]), r[13], G, c))
     return out
 
 

@@ -175,3 +175,24 @@ def extract_tf(cfg, G, iters=400, relax=0.4):
     if tf is not None and use_cache:
         _CACHE[key] = tf
     return tf
+
+
+def center_of_mass_batch(rho, G):
+    """Centres of mass of nb densities rho [nb, n] (flattened in the grid's
+    order): [nb, ndim], zeros for a state without mass (fields.py:17-22)."""
+    mass = rho.sum(dim=1) * dV(G)
+    cs = [(rho * c.reshape(1, -1)).sum(dim=1) * dV(G) for c in G["coords"]]
+    safe = torch.where(mass > 0, mass, torch.ones_like(mass))
+    return torch.stack([torch.where(mass > 0, c / safe, torch.zeros_like(c)) for c in cs], dim=1)
+
+
+def mass_quantiles_batch(rho, G, fractions, centers):
+    """mass_quantiles of nb densities at once (rows of rho [nb, n], centres
+    [nb, ndim]): [nb, len(fractions)] radii, one sort of all rows."""
+    dist = torch.sqrt(sum((c.reshape(1, -1) - centers[:, i:i + 1]) ** 2 for i, c in enumerate(G["coords"])))
+    order = torch.argsort(dist, dim=1, stable=True)
+    running = torch.cumsum(torch.gather(rho.clamp(min=0.0) * dV(G), 1, order), dim=1)
+    targets = running[:, -1:] * torch.tensor([float(f) for f in fractions], dtype=running.dtype,
+                                             device=running.device).reshape(1, -1)
+    at = torch.searchsorted(running.contiguous(), targets.contiguous()).clamp(max=running.shape[1] - 1)
+    return torch.gather(dist, 1, torch.gather(order, 1, at))

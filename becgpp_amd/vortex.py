@@ -52,3 +52,58 @@ def lll_weight(psi, G, Mmax=60, Omega=1.0):
     w = ctypes.c_double()
     _lib.check(pl.lib.bec_lll_weight(pl.h, _ptr(t), float(Omega), int(Mmax), ctypes.byref(w), None))
     return w.value
+
+
+def vortex_diagnostic_batch(psis, G, Omegas, dens_thresh_frac=0.15):
+    """vortex_diagnostic of several states of one grid (up to 64 per call),
+    every launch covering all of them (bec_vortex_count_batch)."""
+    import torch
+    from .thomasfermi import center_of_mass_batch, mass_quantiles_batch
+    out = []
+    for lo in range(0, len(psis), 64):
+        chunk = [_midplane(p, G) for p in psis[lo:lo + 64]]
+        G2 = chunk[0][1]
+        t = torch.stack([c[0] for c in chunk]).contiguous()
+        nb = t.shape[0]
+        rho = (t.real ** 2 + t.imag ** 2).reshape(nb, -1)
+        dx = float(G2["dx"])
+        mass = (rho.sum(dim=1) * dx * dx).tolist()
+        com = center_of_mass_batch(rho, G2)
+        R = mass_quantiles_batch(rho, G2, (0.99,), com)[:, 0].tolist()
+        com = com.tolist()
+        prm, live = [], []
+        for b in range(nb):
+            if mass[b] <= 0 or not math.isfinite(R[b]) or R[b] <= 0:
+                continue
+            om = float(Omegas[lo + b])
+            ell = min(2.0 * math.sqrt(math.pi / max(abs(om), 1e-9)), max(R[b] / 3.0, 2.0 * dx))
+            prm += [com[b][0], com[b][1], R[b], ell]
+            live.append(b)
+        res = [dict(Nv=0, Nplus=0, Nminus=0, Nnet=0) for _ in range(nb)]
+        if live:
+            tl = t[live].contiguous()
+            arr = (ctypes.c_double * len(prm))(*prm)
+            o = (ctypes.c_int * (4 * len(live)))()
+            pl = plan_of(G2)
+            _lib.check(pl.lib.bec_vortex_count_batch(pl.h, len(live), _ptr(tl), arr, float(dens_thresh_frac), o))
+            for k, b in enumerate(live):
+                res[b] = dict(Nv=o[4 * k], Nplus=o[4 * k + 1], Nminus=o[4 * k + 2], Nnet=o[4 * k + 3])
+        out += res
+    return out
+
+
+def lll_weight_batch(psis, G, Mmax, Omegas):
+    """lll_weight of several 2D states of one grid (bec_lll_weight_batch)."""
+    import torch
+    if int(G["ndim"]) != 2:
+        return [float("nan")] * len(psis)
+    out = []
+    pl = plan_of(G)
+    for lo in range(0, len(psis), 64):
+        t = torch.stack([as_field(p, G) for p in psis[lo:lo + 64]]).contiguous()
+        nb = t.shape[0]
+        om = (ctypes.c_double * nb)(*[float(x) for x in Omegas[lo:lo + nb]])
+        w = (ctypes.c_double * nb)()
+        _lib.check(pl.lib.bec_lll_weight_batch(pl.h, nb, _ptr(t), om, int(Mmax), w))
+        out += list(w)
+    return out

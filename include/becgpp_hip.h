@@ -172,6 +172,21 @@ int bec_lll_weight(bec_plan* p, const void* psi, double Omega, int Mmax, double*
 int bec_resample(int ndim, int n0, double dx0, const void* psi_old, int n1, double dx1, void* psi_new,
                  double* norm_out, void* stream);
 
+/* Batched diagnostics (the diagnostic bundle of a sweep's states in one set
+ * of launches; no reference counterpart beyond the per-state functions they
+ * batch).  psi holds nb <= 64 states of the plan's grid one after the other.
+ *  - bec_observe_batch (operators.py:55-103 and observables' inputs):
+ *    out[16 b + 0..7] = the energy components as bec_energy with ph[b],
+ *    [8..12] = the moments of bec_moments, [13] = sum rho (W*rho) dV with the
+ *    q2d virial kernel W (when the plan holds it and G_C != 0, else NaN);
+ *  - bec_lll_weight_batch (vortex.py:61-84): w_out[b] as bec_lll_weight with
+ *    Omega[b];
+ *  - bec_vortex_count_batch (vortex.py:10-48): out4[4 b..] as
+ *    bec_vortex_count with xcm, ycm, R, ell = prm4[4 b .. 4 b + 3]. */
+int bec_observe_batch(bec_plan* p, int nb, const bec_phys* ph, const void* psi, double* out);
+int bec_lll_weight_batch(bec_plan* p, int nb, const void* psi, const double* Omega, int Mmax, double* w_out);
+int bec_vortex_count_batch(bec_plan* p, int nb, const void* psi, const double* prm4, double frac, int* out4);
+
 /* bec_resample without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

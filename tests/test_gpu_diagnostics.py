@@ -276,3 +276,42 @@ def test_cubic_quintic_ladder_matches_reference(b2):
     assert abs(d["mu"] - row["mu"]) <= 1e-6 * abs(row["mu"])
     assert abs(d["peak"] - row["peak"]) <= 1e-6 * row["peak"]
     assert d["virial_rel"] <= max(10.0 * row["virial_rel"], 1e-8)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(dimension="2D", Ngrid=96, L=8.0, beta2=120.0, s=2, trap_coeff=0.5),
+    dict(dimension="2D", Ngrid=64, L=6.0, beta2=30.0, G_C=3.0, kernel="newton"),
+    dict(dimension="quasi2D", Ngrid=64, L=6.0, beta2=30.0, G_C=2.0, kernel="q2d", l_z=0.5),
+    dict(dimension="3D", Ngrid=32, L=5.0, beta2=40.0, G_C=2.0, kernel="newton"),
+])
+def test_batched_diagnostics_equal_per_state(kw):
+    """diagnostics_batch over states with different physics and vortex content
+    (planted windings, rotation on and off) equals diagnostics of each state:
+    counts exactly, every float to 1e-12 (radii to within a grid distance's
+    rounding)."""
+    import becgpp_amd as B
+    base = cfg(**kw)
+    G = B.make_grid(base)
+    Go = O.make_grid(base)
+    cfgs, psis = [], []
+    for k, (om, plus, minus) in enumerate([(0.0, (), ()), (0.6, (1.0 + 0.5j,), ()), (-0.4, (), (0.8j, -1.0)),
+                                          (0.9, (1.2, -1.2, 1.2j), (-0.5j,))]):
+        c = dict(base, Omega=om, beta2=base["beta2"] * (1.0 + 0.1 * k))
+        if abs(float(base["G_C"])) > 0:
+            c["G_C"] = float(base["G_C"]) * (1.0 + 0.2 * k)
+        psi = zeros_state(Go, plus=plus, minus=minus, sigma=1.6)
+        psi = psi / O.norm_of(psi, Go)
+        cfgs.append(c)
+        psis.append(torch.as_tensor(psi).cuda())
+    got = B.diagnostics_batch(psis, G, cfgs)
+    for c, p, d in zip(cfgs, psis, got):
+        ref = B.diagnostics(p, G, c)
+        assert set(ref) == set(d)
+        for key, v in ref.items():
+            g = d[key]
+            if isinstance(v, str) or isinstance(v, bool) or isinstance(v, int):
+                assert g == v, (key, g, v)
+            elif isinstance(v, float) and math.isnan(v):
+                assert math.isnan(g), key
+            else:
+                assert abs(g - v) <= 1e-12 * max(1.0, abs(v)), (key, g, v)
