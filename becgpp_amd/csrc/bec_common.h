@@ -153,12 +153,19 @@ template <int R, int SIGN, class C> __device__ __forceinline__ void bfly(C* v) {
   else bfly8<SIGN>(v);
 }
 
+// LDS position of element i of a line.  With PD one slot is skipped after
+// every eight elements: on a contiguous axis the tpl threads of a line are
+// adjacent lanes, and the first stage's stores (element j R + r by lane j)
+// would otherwise put the 16-byte values of neighbouring lanes R * 16 bytes
+// apart, all in the same few banks; with the skip they fall in distinct banks.
+template <bool PD> __host__ __device__ __forceinline__ int lx(int i) { return PD ? i + (i >> 3) : i; }
+
 // One Stockham autosort stage of radix R on a line of length L held in LDS.
 // tpl threads cooperate on the line (lane in [0, tpl)); L/tpl <= 8 is
 // guaranteed by the host, so a thread owns at most ceil(8/R) butterflies.
 // tw is the twiddle table staged in LDS by the calling kernel.
 // Every thread of the workgroup must call it (it contains barriers).
-template <int R, int SIGN, class C>
+template <int R, int SIGN, bool PD = false, class C>
 __device__ __forceinline__ void fft_stage(C* s, int L, int Ns, unsigned mag, int step, int lane, int tpl,
                                           const C* __restrict__ tw, bool active) {
   constexpr int MAXB = (8 + R - 1) / R;
@@ -170,7 +177,7 @@ __device__ __forceinline__ void fft_stage(C* s, int L, int Ns, unsigned mag, int
       const int j = lane + b * tpl;
       if (j < nb) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = s[j + r * nb];
+        for (int r = 0; r < R; ++r) v[b][r] = s[lx<PD>(j + r * nb)];
       }
     }
   }
@@ -192,7 +199,7 @@ __device__ __forceinline__ void fft_stage(C* s, int L, int Ns, unsigned mag, int
         bfly<R, SIGN>(v[b]);
         const int idxD = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[idxD + r * Ns] = v[b][r];
+        for (int r = 0; r < R; ++r) s[lx<PD>(idxD + r * Ns)] = v[b][r];
       }
     }
   }
@@ -225,7 +232,7 @@ template <> struct OddRoot<13> {
 // and each output pair is stored to LDS as soon as it is formed, so a thread
 // holds R values per butterfly.  Only lengths run through fft_lds take these
 // radices (a line of L = 8 tpl with a factor >= 7 never fills whole wavefronts).
-template <int R, int SIGN, class C>
+template <int R, int SIGN, bool PD = false, class C>
 __device__ __forceinline__ void fft_stage_odd(C* s, int L, int Ns, unsigned mag, int step, int lane, int tpl,
                                               const C* __restrict__ tw, bool active) {
   using F = typename CT<C>::F;
@@ -239,10 +246,10 @@ __device__ __forceinline__ void fft_stage_odd(C* s, int L, int Ns, unsigned mag,
       const int j = lane + bb * tpl;
       if (j < nb) {
         const int k = (Ns == 1) ? 0 : j - (int)__umulhi((unsigned)j, mag) * Ns;   // j mod Ns
-        u0[bb] = s[j];
+        u0[bb] = s[lx<PD>(j)];
 #pragma unroll
         for (int q = 1; q <= H; ++q) {
-          C x = s[j + q * nb], y = s[j + (R - q) * nb];
+          C x = s[lx<PD>(j + q * nb)], y = s[lx<PD>(j + (R - q) * nb)];
           if (Ns > 1) {
             C wx = tw[q * k * step], wy = tw[(R - q) * k * step];
             if (SIGN > 0) { wx.y = -wx.y; wy.y = -wy.y; }
@@ -266,7 +273,7 @@ __device__ __forceinline__ void fft_stage_odd(C* s, int L, int Ns, unsigned mag,
         C x0 = u0[bb];
 #pragma unroll
         for (int q = 0; q < H; ++q) x0 = cadd(x0, a[bb][q]);
-        s[idxD] = x0;
+        s[lx<PD>(idxD)] = x0;
 #pragma unroll
         for (int p = 1; p <= H; ++p) {
           C re = u0[bb], im = cmk<C>(0, 0);
@@ -280,8 +287,8 @@ __device__ __forceinline__ void fft_stage_odd(C* s, int L, int Ns, unsigned mag,
             im = cmk<C>(im.x + sn * b[bb][q - 1].x, im.y + sn * b[bb][q - 1].y);
           }
           const C t = cmuli<SIGN>(im);
-          s[idxD + p * Ns] = cadd(re, t);
-          s[idxD + (R - p) * Ns] = csub(re, t);
+          s[lx<PD>(idxD + p * Ns)] = cadd(re, t);
+          s[lx<PD>(idxD + (R - p) * Ns)] = csub(re, t);
         }
       }
     }
@@ -291,22 +298,22 @@ __device__ __forceinline__ void fft_stage_odd(C* s, int L, int Ns, unsigned mag,
 
 // In-place unnormalised DFT of the LDS line s (SIGN=-1 forward, +1 inverse);
 // twl = the plan's twiddle table copied to LDS (stage_twiddles).
-template <int SIGN, bool ODD = false, class C>
+template <int SIGN, bool ODD = false, bool PD = false, class C>
 __device__ __forceinline__ void fft_lds(C* s, const FftPlan& p, const C* twl, int lane, int tpl,
                                         bool active) {
   for (int st = 0; st < p.nst; ++st) {
     const int R = p.rad[st], Ns = p.ns[st], step = p.step[st];
     const unsigned mag = p.mag[st];
-    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 6) fft_stage<6, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (!ODD || R == 2) fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    if (R == 8) fft_stage<8, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 4) fft_stage<4, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 3) fft_stage<3, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 5) fft_stage<5, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 6) fft_stage<6, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (!ODD || R == 2) fft_stage<2, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     else if constexpr (ODD) {
-      if (R == 7) fft_stage_odd<7, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-      else if (R == 11) fft_stage_odd<11, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-      else fft_stage_odd<13, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+      if (R == 7) fft_stage_odd<7, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+      else if (R == 11) fft_stage_odd<11, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+      else fft_stage_odd<13, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
     }
   }
 }
@@ -320,7 +327,7 @@ __device__ __forceinline__ void fft_lds(C* s, const FftPlan& p, const C* twl, in
 // go through the LDS line, so a forward transform, a pointwise multiply and an
 // inverse transform keep the spectrum in registers across the multiply.
 // ----------------------------------------------------------------------------
-template <int R, int SIGN, class C>
+template <int R, int SIGN, bool PD = false, class C>
 __device__ __forceinline__ void reg_first(C (&v)[8], C* s, int lane, int tpl, bool active) {
   constexpr int Q = 8 / R;
   __syncthreads();                     // earlier readers of s are done
@@ -333,12 +340,12 @@ __device__ __forceinline__ void reg_first(C (&v)[8], C* s, int lane, int tpl, bo
       bfly<R, SIGN>(u);
       const int j = lane + b * tpl;
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[j * R + r] = u[r];
+      for (int r = 0; r < R; ++r) s[lx<PD>(j * R + r)] = u[r];
     }
   }
   __syncthreads();
 }
-template <int R, int SIGN, class C>
+template <int R, int SIGN, bool PD = false, class C>
 __device__ __forceinline__ void reg_last(C (&v)[8], const C* s, int L, int lane, int tpl,
                                          const C* __restrict__ tw, bool active) {
   constexpr int Q = 8 / R;
@@ -349,7 +356,7 @@ __device__ __forceinline__ void reg_last(C (&v)[8], const C* s, int L, int lane,
     const int j = lane + b * tpl;
     C u[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) u[r] = s[j + r * ns];
+    for (int r = 0; r < R; ++r) u[r] = s[lx<PD>(j + r * ns)];
 #pragma unroll
     for (int r = 1; r < R; ++r) {
       C w = tw[r * j];
@@ -361,7 +368,7 @@ __device__ __forceinline__ void reg_last(C (&v)[8], const C* s, int L, int lane,
     for (int r = 0; r < R; ++r) v[b + r * Q] = u[r];
   }
 }
-template <int SIGN, class C>
+template <int SIGN, bool PD = false, class C>
 __device__ __forceinline__ void fft_reg(C (&v)[8], C* s, const FftPlan& p, const C* twl,
                                         int lane, int tpl, bool active) {
   const int n = p.nst;
@@ -370,33 +377,33 @@ __device__ __forceinline__ void fft_reg(C (&v)[8], C* s, const FftPlan& p, const
     return;
   }
   const int r0 = p.rad[0], rl = p.rad[n - 1];
-  if (r0 == 8) reg_first<8, SIGN>(v, s, lane, tpl, active);
-  else if (r0 == 4) reg_first<4, SIGN>(v, s, lane, tpl, active);
-  else reg_first<2, SIGN>(v, s, lane, tpl, active);
+  if (r0 == 8) reg_first<8, SIGN, PD>(v, s, lane, tpl, active);
+  else if (r0 == 4) reg_first<4, SIGN, PD>(v, s, lane, tpl, active);
+  else reg_first<2, SIGN, PD>(v, s, lane, tpl, active);
   for (int st = 1; st < n - 1; ++st) {
     const int R = p.rad[st], Ns = p.ns[st], step = p.step[st];
     const unsigned mag = p.mag[st];
-    if (R == 8) fft_stage<8, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 4) fft_stage<4, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 3) fft_stage<3, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 5) fft_stage<5, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else if (R == 6) fft_stage<6, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
-    else fft_stage<2, SIGN>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    if (R == 8) fft_stage<8, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 4) fft_stage<4, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 3) fft_stage<3, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 5) fft_stage<5, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else if (R == 6) fft_stage<6, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
+    else fft_stage<2, SIGN, PD>(s, p.L, Ns, mag, step, lane, tpl, twl, active);
   }
-  if (rl == 8) reg_last<8, SIGN>(v, s, p.L, lane, tpl, twl, active);
-  else if (rl == 4) reg_last<4, SIGN>(v, s, p.L, lane, tpl, twl, active);
-  else reg_last<2, SIGN>(v, s, p.L, lane, tpl, twl, active);
+  if (rl == 8) reg_last<8, SIGN, PD>(v, s, p.L, lane, tpl, twl, active);
+  else if (rl == 4) reg_last<4, SIGN, PD>(v, s, p.L, lane, tpl, twl, active);
+  else reg_last<2, SIGN, PD>(v, s, p.L, lane, tpl, twl, active);
 }
 // The DFT of a line whose elements lane + m tpl (m < 8, those below L) are in
 // v: register-resident when the plan allows it, else through the LDS line s
 // (fft_lds).  Every thread of the workgroup calls it; results return in v.
 // ODD compiles in the radices 7, 11 and 13 (plans with FftPlan::odd): their
 // stages hold more registers, so the kernels of other plans are built without.
-template <int SIGN, bool ODD = false, class C>
+template <int SIGN, bool ODD = false, bool PD = false, class C>
 __device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, const C* twl,
                                          int lane, int tpl, bool active) {
   if (p.reg) {
-    fft_reg<SIGN>(v, s, p, twl, lane, tpl, active);
+    fft_reg<SIGN, PD>(v, s, p, twl, lane, tpl, active);
     return;
   }
   __syncthreads();
@@ -404,16 +411,16 @@ __device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, cons
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int e = lane + m * tpl;
-      if (e < p.L) s[e] = v[m];
+      if (e < p.L) s[lx<PD>(e)] = v[m];
     }
   }
   __syncthreads();
-  fft_lds<SIGN, ODD>(s, p, twl, lane, tpl, active);
+  fft_lds<SIGN, ODD, PD>(s, p, twl, lane, tpl, active);
   if (active) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int e = lane + m * tpl;
-      v[m] = (e < p.L) ? s[e] : cmk<C>(0.0, 0.0);
+      v[m] = (e < p.L) ? s[lx<PD>(e)] : cmk<C>(0.0, 0.0);
     }
   }
 }

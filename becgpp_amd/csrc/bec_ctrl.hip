@@ -18,17 +18,18 @@
 #include "../../include/becgpp_hip.h"
 #include "bec_internal.h"
 
-#define CTRL_WG 256
+#define CTRL_WG 1024
 
 // Sums (q < K) or maxima (q >= K) of the R rows of KP values at rows[],
 // folded as: lane j of value q takes rows j, j + J, ... in order, then a fixed
 // pairwise tree over the J lanes.  out[q] for q < KP; every thread calls it.
-// The loads of eight consecutive rows of a lane are issued together before
-// they are added in order, so a lane waits once per eight rows.
+// The loads of sixteen consecutive rows of a lane are issued together before
+// they are added in order, so a lane waits once per sixteen rows (a solve on
+// 128^3 points folds about 2000 rows of 32 values in four such waits).
 template <int KP>
 __device__ void fold_rows(const double* __restrict__ rows, int R, int K, double* out, double* sh) {
   constexpr int J = CTRL_WG / KP;
-  constexpr int U = 8;
+  constexpr int U = 16;
   const int q = threadIdx.x % KP, j = threadIdx.x / KP;
   const bool is_sum = q < K;
   double acc = is_sum ? 0.0 : -INFINITY;

@@ -201,9 +201,10 @@ struct LineOwn {
 };
 // LDS pitch of a block's lines.  On a strided axis adjacent threads hold the
 // same element of adjacent lines; one extra element per line puts those 16-byte
-// accesses in different banks.
+// accesses in different banks.  A contiguous line is stored with one skipped
+// slot per eight elements (lx<true>), L + L/8 slots.
 template <bool CONTIG>
-__host__ __device__ __forceinline__ int lpitch(int L) { return CONTIG ? L : L + 1; }
+__host__ __device__ __forceinline__ int lpitch(int L) { return CONTIG ? L + (L >> 3) : L + 1; }
 template <bool CONTIG>
 __device__ __forceinline__ LineOwn line_owner(int tpl, int lpw) {
   LineOwn o;
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       }
     }
   }
-  fft_line<-1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_line<-1, ODD, X::contig>(x, s, a.fp, twl, lane, tpl, oon);
   const double invN = 1.0 / (double)N;
   const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
   const bool mult = !KE || (KIND == KE_FIRST && a.hy_out);
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     }
     if (!mult) return;
   }
-  fft_line<+1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_line<+1, ODD, X::contig>(x, s, a.fp, twl, lane, tpl, oon);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   if (oon) {
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(BEC_WG) void k_lcoef(KinArgs a) {
       x[j] = from_d2<C>(dd[j]);
     }
   }
-  fft_line<-1, ODD>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_line<-1, ODD, X::contig>(x, s, a.fp, twl, lane, tpl, oon);
   const double invN = 1.0 / (double)N;
   const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
   double kin = 0.0, lz = 0.0;
@@ -715,8 +716,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     }
   }
   const double sig = ((MUL == MUL_PRE2 || MUL == MUL_PRE3) && oon) ? a.sigma[ob] : 0.0;
-  if (DIR == DIR_INV) fft_line<+1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
-  else fft_line<-1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
+  if (DIR == DIR_INV) fft_line<+1, ODD, CONTIG>(v, s, a.fp, twl, lane, tpl, oon);
+  else fft_line<-1, ODD, CONTIG>(v, s, a.fp, twl, lane, tpl, oon);
   if (DIR == DIR_FMI) {
     if (oon) {
       if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
@@ -746,7 +747,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         }
       }
     }
-    fft_line<+1, ODD>(v, s, a.fp, twl, lane, tpl, oon);
+    fft_line<+1, ODD, CONTIG>(v, s, a.fp, twl, lane, tpl, oon);
   }
   // stores: line position q = lane + j tpl goes to output element q - off_store
   double dsum[2] = {0.0, 0.0};

@@ -414,6 +414,7 @@ def run_workload(name, steps, warmup, a, cpu_threads, cpu_desc):
         roof["traffic"] = tr
         roof["traffic_source"] = tsrc
         roof["traffic_over_algorithmic"] = tr / bpl if bpl else None
+    roof["replay_kernel_us_per_iter"] = 1e3 * tot_ms / replay_iters
     roof["breakdown"] = [dict(site=p[0], kernel=site_name(p[0]), share=p[1] / tot_ms if tot_ms else None,
                               avg_us=1e3 * p[1] / max(1, p[2]),
                               GBs=(p[3] / max(1, p[2])) / (p[1] / 1e3 / max(1, p[2])) / 1e9 if p[1] and p[3] else None)
