@@ -394,6 +394,78 @@ __device__ __forceinline__ void fft_reg(C (&v)[8], C* s, const FftPlan& p, const
   else if (rl == 4) reg_last<4, SIGN, PD>(v, s, p.L, lane, tpl, twl, active);
   else reg_last<2, SIGN, PD>(v, s, p.L, lane, tpl, twl, active);
 }
+// The register-resident transform of a power-of-two line of compile-time
+// length LC in the stage order the plans use (radix 8 first, radix 8 in the
+// middle stages, the remaining 8, 4 or 2 last; tpl = LC / 8).  It performs
+// the same operations in the same order as fft_reg on such a plan, with the
+// strides, Ns and j mod Ns folded at compile time.
+template <int LC> struct RegPlan {
+  static constexpr int lg() { int a = 0; for (int n = LC; n > 1; n >>= 1) ++a; return a; }
+  static constexpr int A = lg();
+  static constexpr int TPL = LC / 8;
+  static constexpr int RL = (A % 3 == 1) ? 2 : (A % 3 == 2 ? 4 : 8);
+  static constexpr int NMID = (A % 3 == 0 ? A / 3 : A / 3 + 1) - 2;   // stages between the first and the last
+  static_assert(LC >= 64 && (LC & (LC - 1)) == 0 && NMID <= 2, "register plans of 64..1024 points");
+};
+template <int R, int SIGN, bool PD, int L, int NS, int TPL, class C>
+__device__ __forceinline__ void fft_stage_c(C* s, int lane, const C* __restrict__ tw, bool active) {
+  constexpr int MAXB = (8 + R - 1) / R;
+  constexpr int nb = L / R;
+  constexpr int step = L / (NS * R);
+  C v[MAXB][R];
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = lane + b * TPL;
+      if (j < nb) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = s[lx<PD>(j + r * nb)];
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = lane + b * TPL;
+      if (j < nb) {
+        const int k = j & (NS - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C w = tw[r * k * step];
+          if (SIGN > 0) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+        bfly<R, SIGN>(v[b]);
+        const int idxD = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[lx<PD>(idxD + r * NS)] = v[b][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+template <int SIGN, bool PD, int LC, class C>
+__device__ __forceinline__ void fft_reg_c(C (&v)[8], C* s, const C* twl, int lane, bool active) {
+  using P = RegPlan<LC>;
+  reg_first<8, SIGN, PD>(v, s, lane, P::TPL, active);
+  if constexpr (P::NMID >= 1) fft_stage_c<8, SIGN, PD, LC, 8, P::TPL>(s, lane, twl, active);
+  if constexpr (P::NMID >= 2) fft_stage_c<8, SIGN, PD, LC, 64, P::TPL>(s, lane, twl, active);
+  reg_last<P::RL, SIGN, PD>(v, s, LC, lane, P::TPL, twl, active);
+}
+// host side: the compile-time length whose transform equals plan p's, or 0
+__host__ __device__ inline int reg_plan_len(const FftPlan& p) {
+  if (!p.reg || p.odd || p.L < 64 || p.L > 1024 || (p.L & (p.L - 1)) != 0) return 0;
+  int a = 0;
+  for (int n = p.L; n > 1; n >>= 1) ++a;
+  const int rl = (a % 3 == 1) ? 2 : (a % 3 == 2 ? 4 : 8);
+  const int nst = (a % 3 == 0) ? a / 3 : a / 3 + 1;
+  if (p.nst != nst || nst > 4) return 0;
+  for (int i = 0; i < nst - 1; ++i)
+    if (p.rad[i] != 8) return 0;
+  return p.rad[nst - 1] == rl ? p.L : 0;
+}
+
 // The DFT of a line whose elements lane + m tpl (m < 8, those below L) are in
 // v: register-resident when the plan allows it, else through the LDS line s
 // (fft_lds).  Every thread of the workgroup calls it; results return in v.
@@ -423,6 +495,14 @@ __device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, cons
       v[m] = (e < p.L) ? s[lx<PD>(e)] : cmk<C>(0.0, 0.0);
     }
   }
+}
+
+// fft_line, or fft_reg_c when the kernel was built for the plan's length LC
+template <int SIGN, bool ODD, bool PD, int LC, class C>
+__device__ __forceinline__ void fft_any(C (&v)[8], C* s, const FftPlan& p, const C* twl, int lane, int tpl,
+                                        bool active) {
+  if constexpr (LC != 0) fft_reg_c<SIGN, PD, LC>(v, s, twl, lane, active);
+  else fft_line<SIGN, ODD, PD>(v, s, p, twl, lane, tpl, active);
 }
 
 // Copies the twiddle table (L entries) into LDS; the caller's next barrier

@@ -9,6 +9,7 @@
 // The work is HBM/LDS-bound FP64 vector arithmetic: there is no MFMA here.
 #include "bec_internal.h"
 #include <hip/hip_ext.h>
+#include <type_traits>
 
 #define TWO_PI 6.283185307179586
 
@@ -251,7 +252,7 @@ __device__ __forceinline__ double line_V(const double* c2, int e, const KinArgs&
 // C is the storage of the complex fields: double2, or float2 in an FP32 phase
 // (values are widened to FP64 in registers; the transforms run in C; every
 // sum accumulates in FP64).
-template <int AX, int KIND, class C, bool ODD = false>
+template <int AX, int KIND, class C, bool ODD = false, int LC = 0>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 smem_d[];
   C* smem = (C*)smem_d;
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const C* fd = (const C*)a.d;
   C* fout = (C*)a.out;
   constexpr bool KE = (KIND == KE_FIRST || KIND == KE_REST);
-  const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
+  const int N = LC ? LC : a.N, tpl = LC ? LC / 8 : a.tpl, lpw = a.lpw, nl = a.nl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       }
     }
   }
-  fft_line<-1, ODD, X::contig>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_any<-1, ODD, X::contig, LC>(x, s, a.fp, twl, lane, tpl, oon);
   const double invN = 1.0 / (double)N;
   const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
   const bool mult = !KE || (KIND == KE_FIRST && a.hy_out);
@@ -417,7 +418,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
     }
     if (!mult) return;
   }
-  fft_line<+1, ODD, X::contig>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_any<+1, ODD, X::contig, LC>(x, s, a.fp, twl, lane, tpl, oon);
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   if (oon) {
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
 //   10 p^2 q   11 p^2 r   12 p q^2   13 q^3   14 p q r   15 q^2 r   16 p r^2
 //   17 q r^2   18 r^3   19 phi q   20 phi r     (phi: the state's potential)
 #define LC_K 21
-template <int AX, class C, bool ODD = false>
+template <int AX, class C, bool ODD = false, int LC = 0>
 __global__ __launch_bounds__(BEC_WG) void k_lcoef(KinArgs a) {
   extern __shared__ double2 smem_d[];
   C* smem = (C*)smem_d;
@@ -512,7 +513,7 @@ __global__ __launch_bounds__(BEC_WG) void k_lcoef(KinArgs a) {
   C* lds = smem + a.N;                 // lines [lpw][lpitch<X::contig>(N)]
   const C* fpsi = (const C*)a.psi;
   const C* fd = (const C*)a.d;
-  const int N = a.N, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
+  const int N = LC ? LC : a.N, tpl = LC ? LC / 8 : a.tpl, lpw = a.lpw, nl = a.nl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(BEC_WG) void k_lcoef(KinArgs a) {
       x[j] = from_d2<C>(dd[j]);
     }
   }
-  fft_line<-1, ODD, X::contig>(x, s, a.fp, twl, lane, tpl, oon);
+  fft_any<-1, ODD, X::contig, LC>(x, s, a.fp, twl, lane, tpl, oon);
   const double invN = 1.0 / (double)N;
   const double lzc = (X::lsign == 0 || !oon) ? 0.0 : (double)X::lsign * xcoord(X::lidx(oli, N), N, a.dx);
   double kin = 0.0, lz = 0.0;
@@ -643,13 +644,14 @@ __device__ __forceinline__ void dot_accum(double* v, double2 zi, double2 pi, dou
 }
 
 // C: storage (and transform type) of the complex fields, as in k_kin; the
-// real fields (LD_R input, ST_RE output, khat, dot) are FP64.
-template <bool CONTIG, int LD, int DIR, int MUL, int ST, class C, bool ODD = false>
+// real fields (LD_R input, ST_RE output, khat, dot) are FP64.  LC != 0: built
+// for lines of LC points on the register plan of that length (fft_reg_c).
+template <bool CONTIG, int LD, int DIR, int MUL, int ST, class C, bool ODD = false, int LC = 0>
 __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   using F = typename CT<C>::F;
   extern __shared__ double2 smem_d[];
   C* smem = (C*)smem_d;
-  const int L = a.fp.L, tpl = a.tpl, lpw = a.lpw, nl = a.nl;
+  const int L = LC ? LC : a.fp.L, tpl = LC ? LC / 8 : a.tpl, lpw = a.lpw, nl = a.nl;
   C* twl = smem;                       // twiddles [L]
   C* lds = smem + L;                   // lines [lpw][lpitch<CONTIG>(L)]
   const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -716,8 +718,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     }
   }
   const double sig = ((MUL == MUL_PRE2 || MUL == MUL_PRE3) && oon) ? a.sigma[ob] : 0.0;
-  if (DIR == DIR_INV) fft_line<+1, ODD, CONTIG>(v, s, a.fp, twl, lane, tpl, oon);
-  else fft_line<-1, ODD, CONTIG>(v, s, a.fp, twl, lane, tpl, oon);
+  if (DIR == DIR_INV) fft_any<+1, ODD, CONTIG, LC>(v, s, a.fp, twl, lane, tpl, oon);
+  else fft_any<-1, ODD, CONTIG, LC>(v, s, a.fp, twl, lane, tpl, oon);
   if (DIR == DIR_FMI) {
     if (oon) {
       if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
@@ -747,7 +749,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         }
       }
     }
-    fft_line<+1, ODD, CONTIG>(v, s, a.fp, twl, lane, tpl, oon);
+    fft_any<+1, ODD, CONTIG, LC>(v, s, a.fp, twl, lane, tpl, oon);
   }
   // stores: line position q = lane + j tpl goes to output element q - off_store
   double dsum[2] = {0.0, 0.0};
@@ -1419,6 +1421,25 @@ static int xcd_grid(int nlines, int lpw) {
   return ((b + BEC_XCDS - 1) / BEC_XCDS) * BEC_XCDS;
 }
 
+// Calls f(std::integral_constant<int, LC>) with LC the compile-time length of
+// the plan (reg_plan_len) when the kernels were built for it, else LC = 0
+// (the generic transform).  Built: 128, 256 and, with P512, 512.
+template <bool P512, class F>
+static void with_lc(const FftPlan& fp, F&& f) {
+  switch (reg_plan_len(fp)) {
+    case 128: f(std::integral_constant<int, 128>{}); return;
+    case 256: f(std::integral_constant<int, 256>{}); return;
+    case 512:
+      if constexpr (P512) {
+        f(std::integral_constant<int, 512>{});
+        return;
+      }
+      break;
+    default: break;
+  }
+  f(std::integral_constant<int, 0>{});
+}
+
 template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 static int go_pass(const PassArgs& a0, hipStream_t s) {
   PassArgs a = a0;
@@ -1450,8 +1471,11 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
       if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2, true>), grid, block, shm, s, e0, e1, 0, a);
       else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2, true>), grid, block, shm, s, a);
     } else {
-      if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, e0, e1, 0, a);
-      else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2>), grid, block, shm, s, a);
+      with_lc<true>(a.fp, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2, false, LC>), grid, block, shm, s, e0, e1, 0, a);
+        else hipLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, double2, false, LC>), grid, block, shm, s, a);
+      });
     }
   }
   return (a.nl + a.lpw - 1) / a.lpw;
@@ -1529,8 +1553,11 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
       if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, true>), grid, block, shm, s, e0, e1, 0, a);
       else hipLaunchKernelGGL((k_kin<AX, KIND, double2, true>), grid, block, shm, s, a);
     } else {
-      if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, e0, e1, 0, a);
-      else hipLaunchKernelGGL((k_kin<AX, KIND, double2>), grid, block, shm, s, a);
+      with_lc<false>(a.fp, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, false, LC>), grid, block, shm, s, e0, e1, 0, a);
+        else hipLaunchKernelGGL((k_kin<AX, KIND, double2, false, LC>), grid, block, shm, s, a);
+      });
     }
   }
   return (a.nl + a.lpw - 1) / a.lpw;
@@ -1568,8 +1595,11 @@ static int go_lcoef(const KinArgs& a0, hipStream_t s) {
       if (ev) hipExtLaunchKernelGGL((k_lcoef<AX, double2, true>), grid, block, shm, s, e0, e1, 0, a);
       else hipLaunchKernelGGL((k_lcoef<AX, double2, true>), grid, block, shm, s, a);
     } else {
-      if (ev) hipExtLaunchKernelGGL((k_lcoef<AX, double2>), grid, block, shm, s, e0, e1, 0, a);
-      else hipLaunchKernelGGL((k_lcoef<AX, double2>), grid, block, shm, s, a);
+      with_lc<false>(a.fp, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        if (ev) hipExtLaunchKernelGGL((k_lcoef<AX, double2, false, LC>), grid, block, shm, s, e0, e1, 0, a);
+        else hipLaunchKernelGGL((k_lcoef<AX, double2, false, LC>), grid, block, shm, s, a);
+      });
     }
     return (a.nl + a.lpw - 1) / a.lpw;
   }

@@ -418,7 +418,7 @@ def run_workload(name, steps, warmup, a, cpu_threads, cpu_desc):
     roof["breakdown"] = [dict(site=p[0], kernel=site_name(p[0]), share=p[1] / tot_ms if tot_ms else None,
                               avg_us=1e3 * p[1] / max(1, p[2]),
                               GBs=(p[3] / max(1, p[2])) / (p[1] / 1e3 / max(1, p[2])) / 1e9 if p[1] and p[3] else None)
-                         for p in per[:6]]
+                         for p in per[:16] if tot_ms and p[1] / tot_ms >= 0.01]
     out["roofline"] = roof
 
     if not a.no_cpu_baseline and not a.replay_only:
