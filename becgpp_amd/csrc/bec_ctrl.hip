@@ -21,41 +21,64 @@
 #define CTRL_WG 1024
 
 // Sums (q < K) or maxima (q >= K) of the R rows of KP values at rows[],
-// folded as: lane j of value q takes rows j, j + J, ... in order, then a fixed
-// pairwise tree over the J lanes.  out[q] for q < KP; every thread calls it.
-// The loads of sixteen consecutive rows of a lane are issued together before
-// they are added in order, so a lane waits once per sixteen rows (a solve on
-// 128^3 points folds about 2000 rows of 32 values in four such waits).
-template <int KP>
-__device__ void fold_rows(const double* __restrict__ rows, int R, int K, double* out, double* sh) {
-  constexpr int J = CTRL_WG / KP;
-  constexpr int U = 16;
-  const int q = threadIdx.x % KP, j = threadIdx.x / KP;
-  const bool is_sum = q < K;
-  double acc = is_sum ? 0.0 : -INFINITY;
-  for (int r0 = j; r0 < R; r0 += U * J) {
-    double x[U];
+// folded as: lane j of value q takes rows j, j + J, ... in order (J =
+// CTRL_WG / KP), a fixed pairwise tree combines the lanes of each wavefront,
+// and the wavefronts' results are added in order; out[q] for q < KP.
+// A fold runs in two phases so that a controller issues the loads of all its
+// folds (and of its instance record) before it waits for any: issue() loads
+// the first U rows of the lane, finish() adds them in order, continues over
+// any further rows U at a time, and reduces over the lanes (every thread
+// calls both).  The rows were written by other workgroups on other XCDs, so
+// each wait is a trip to memory: one such trip per controller in the usual
+// case (at most U J rows per fold).
+template <int KP, int U>
+struct RowFold {
+  static constexpr int J = CTRL_WG / KP;
+  double x[U];
+  __device__ __forceinline__ void issue(const double* __restrict__ rows, int R) {
+    const int q = threadIdx.x % KP, j = threadIdx.x / KP;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int r = r0 + u * J;
+      const int r = j + u * J;
       x[u] = r < R ? rows[(long long)r * KP + q] : 0.0;
     }
+  }
+  __device__ __forceinline__ void finish(const double* __restrict__ rows, int R, int K, double* out, double* sh) {
+    const int q = threadIdx.x % KP, j = threadIdx.x / KP;
+    const bool is_sum = q < K;
+    double acc = is_sum ? 0.0 : -INFINITY;
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (r0 + u * J < R) acc = is_sum ? acc + x[u] : fmax(acc, x[u]);
-  }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = J / 2; h > 0; h >>= 1) {
-    if (j < h) {
-      const double y = sh[threadIdx.x + h * KP];
-      sh[threadIdx.x] = is_sum ? sh[threadIdx.x] + y : fmax(sh[threadIdx.x], y);
+      if (j + u * J < R) acc = is_sum ? acc + x[u] : fmax(acc, x[u]);
+    for (int r0 = j + U * J; r0 < R; r0 += U * J) {
+      double y[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int r = r0 + u * J;
+        y[u] = r < R ? rows[(long long)r * KP + q] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (r0 + u * J < R) acc = is_sum ? acc + y[u] : fmax(acc, y[u]);
+    }
+    // the lanes' values: a pairwise tree over the W lanes of a wavefront
+    // (shuffles), then the wavefronts' results in wavefront order
+    constexpr int W = 64 / KP, NW = CTRL_WG / 64;
+#pragma unroll
+    for (int h = W / 2; h >= 1; h >>= 1) {
+      const double v = __shfl_down(acc, h * KP, 64);
+      acc = is_sum ? acc + v : fmax(acc, v);
+    }
+    if ((threadIdx.x & 63) < KP) sh[(threadIdx.x >> 6) * KP + q] = acc;
+    __syncthreads();
+    if (threadIdx.x < KP) {
+      double t = sh[q];
+      for (int w = 1; w < NW; ++w) t = is_sum ? t + sh[w * KP + q] : fmax(t, sh[w * KP + q]);
+      out[q] = t;
     }
     __syncthreads();
   }
-  if (threadIdx.x < KP) out[threadIdx.x] = sh[threadIdx.x];
-  __syncthreads();
-}
+};
 
 __device__ __forceinline__ double clk_now() { return (double)wall_clock64(); }
 
@@ -385,13 +408,29 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_line(CtrlArgs c) {
     if (threadIdx.x == 0) c.fl[b] = f & FL_BEST;
     return;
   }
+  const bool lr = (f & FL_SLR) != 0u;
+  const double* r0 = c.r_e0 + (long long)b * c.n_e0 * 32;
+  const double* r1 = c.r_e1 + (long long)b * c.n_e1 * 2;
+  const double* r2 = c.ndim == 3 ? c.r_e2 + (long long)b * c.n_e2 * 2 : nullptr;
+  const double* rq = lr ? c.r_kq + (long long)b * c.n_kq * 2 : nullptr;
+  const double* rr = lr ? c.r_kr + (long long)b * c.n_kr : nullptr;
+  RowFold<32, 32> f0;
+  RowFold<2, 2> f1, f2, fq;
+  RowFold<1, 2> fr;
+  f0.issue(r0, c.n_e0);
+  f1.issue(r1, c.n_e1);
+  if (r2) f2.issue(r2, c.n_e2);
+  if (lr) {
+    fq.issue(rq, c.n_kq);
+    fr.issue(rr, c.n_kr);
+  }
   inst_load(&qs, c.st + b);
-  fold_rows<32>(c.r_e0 + (long long)b * c.n_e0 * 32, c.n_e0, 21, e0, sh);
-  fold_rows<2>(c.r_e1 + (long long)b * c.n_e1 * 2, c.n_e1, 2, e1, sh);
-  if (c.ndim == 3) fold_rows<2>(c.r_e2 + (long long)b * c.n_e2 * 2, c.n_e2, 2, e2, sh);
-  if (f & FL_SLR) {
-    fold_rows<2>(c.r_kq + (long long)b * c.n_kq * 2, c.n_kq, 2, kq, sh);
-    fold_rows<1>(c.r_kr + (long long)b * c.n_kr, c.n_kr, 1, kr, sh);
+  f0.finish(r0, c.n_e0, 21, e0, sh);
+  f1.finish(r1, c.n_e1, 2, e1, sh);
+  if (r2) f2.finish(r2, c.n_e2, 2, e2, sh);
+  if (lr) {
+    fq.finish(rq, c.n_kq, 2, kq, sh);
+    fr.finish(rr, c.n_kr, 1, kr, sh);
   }
   if (threadIdx.x == 0) ctrl_line_scalar(c, b, f, &qs, e0, e1, e2, kq, kr);
   __syncthreads();
@@ -406,11 +445,21 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
   DevInst* q = &qs;
   const unsigned f = c.fl[b];
   const bool dir = (f & FL_DIR) != 0u;
+  // the rows of the accepted state's H pass are loaded with the inner
+  // products and used when the record says that a step is pending
+  const double* rd = c.r_dots + (long long)b * c.n_dots * 16;
+  const double* rh = c.r_hl + (long long)b * c.n_hl * 4;
+  RowFold<16, 16> fd;
+  RowFold<4, 4> fh;
+  if (dir) {
+    fd.issue(rd, c.n_dots);
+    fh.issue(rh, c.n_hl);
+  }
   inst_load(q, c.st + b);
   __syncthreads();
   if (dir) {
-    fold_rows<16>(c.r_dots + (long long)b * c.n_dots * 16, c.n_dots, 16, r, sh);
-    if (q->pending) fold_rows<4>(c.r_hl + (long long)b * c.n_hl * 4, c.n_hl, 2, hl, sh);
+    fd.finish(rd, c.n_dots, 16, r, sh);
+    if (q->pending) fh.finish(rh, c.n_hl, 2, hl, sh);
   }
   if (threadIdx.x == 0) ctrl_dir_scalar(c, b, f, dir, q, r, hl);
   __syncthreads();
@@ -428,11 +477,23 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_armijo(CtrlArgs c) {
     if (threadIdx.x == 0) c.fl[b] = f & FL_BEST;
     return;
   }
+  const bool lr = (f & FL_SLR) != 0u;
+  const double* r0 = c.r_e0 + (long long)b * c.n_e0 * 8;
+  const double* r1 = c.r_e1 + (long long)b * c.n_e1 * 2;
+  const double* r2 = c.ndim == 3 ? c.r_e2 + (long long)b * c.n_e2 * 2 : nullptr;
+  const double* rg = lr ? c.r_gr + (long long)b * c.n_gr : nullptr;
+  RowFold<8, 8> f0;
+  RowFold<2, 2> f1, f2;
+  RowFold<1, 2> fg;
+  f0.issue(r0, c.n_e0);
+  f1.issue(r1, c.n_e1);
+  if (r2) f2.issue(r2, c.n_e2);
+  if (lr) fg.issue(rg, c.n_gr);
   inst_load(&qs, c.st + b);
-  fold_rows<8>(c.r_e0 + (long long)b * c.n_e0 * 8, c.n_e0, 6, e0, sh);
-  fold_rows<2>(c.r_e1 + (long long)b * c.n_e1 * 2, c.n_e1, 2, e1, sh);
-  if (c.ndim == 3) fold_rows<2>(c.r_e2 + (long long)b * c.n_e2 * 2, c.n_e2, 2, e2, sh);
-  if (f & FL_SLR) fold_rows<1>(c.r_gr + (long long)b * c.n_gr, c.n_gr, 1, gr, sh);
+  f0.finish(r0, c.n_e0, 6, e0, sh);
+  f1.finish(r1, c.n_e1, 2, e1, sh);
+  if (r2) f2.finish(r2, c.n_e2, 2, e2, sh);
+  if (lr) fg.finish(rg, c.n_gr, 1, gr, sh);
   if (threadIdx.x == 0) ctrl_armijo_scalar(c, b, f, &qs, e0, e1, e2, gr);
   __syncthreads();
   inst_store(c.st + b, &qs);

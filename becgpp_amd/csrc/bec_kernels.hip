@@ -197,6 +197,15 @@ __device__ __forceinline__ CombScal comb_scal(const KinArgs& a, int b, bool inst
 // (N <= BEC_EPT tpl).  The line's addressing is computed once per thread, and
 // all of a phase's loads are issued before any is used.
 #define BEC_EPT 8
+#ifndef BEC_KH_EARLY2D
+#define BEC_KH_EARLY2D 1       // 2D H passes load their epilogue before the transforms
+#endif
+#ifndef BEC_KH_CHUNK
+#define BEC_KH_CHUNK 4         // elements per chunk of the H passes' epilogue (3D axes)
+#endif
+#ifndef BEC_CDOT_CHUNK
+#define BEC_CDOT_CHUNK 2       // elements per chunk of the inner-product epilogue
+#endif
 struct LineOwn {
   int cc, e0;                 // block line and first element of the thread
 };
@@ -300,7 +309,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   C x[BEC_EPT];
   // loads of the H passes' epilogue; on 2D axes they are issued before the
   // transforms (in 3D the extra registers would halve the waves per SIMD)
-  constexpr bool EARLY = (KIND == KH_MID || KIND == KH_LAST) && X::ndim == 2;
+  constexpr bool EARLY = BEC_KH_EARLY2D && (KIND == KH_MID || KIND == KH_LAST) && X::ndim == 2;
   double2 ps[BEC_EPT], acc[BEC_EPT], old[BEC_EPT];
   double phv[BEC_EPT];
   {
@@ -422,65 +431,70 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   double hsum[2] = {0.0, 0.0};
   double hmax[1] = {0.0};
   if (oon) {
-    // phase 1: loads of the epilogue (issued earlier on 2D axes)
+    // the epilogue in chunks of EC elements: loads of a chunk (issued before
+    // the transforms on 2D axes, all in one chunk there), then its results,
+    // in element order
+    constexpr int EC = EARLY ? BEC_EPT : BEC_KH_CHUNK;
 #pragma unroll
-    for (int j = 0; j < BEC_EPT; ++j) {
-      const int e = ow.e0 + j * tpl;
-      if (EARLY || e >= N) continue;
-      const long long adr = obase + e * strd;
-      if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? to_d2(((const C*)a.acc)[adr]) : make_double2(0.0, 0.0);
-      if (KIND == KH_LAST) {
-        ps[j] = to_d2(fpsi[adr]);
-        phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
-        if (a.ydiff) old[j] = to_d2(fout[adr]);
-      }
-    }
-    // phase 2: results
+    for (int j0 = 0; j0 < BEC_EPT; j0 += EC) {
 #pragma unroll
-    for (int j = 0; j < BEC_EPT; ++j) {
-      const int e = ow.e0 + j * tpl;
-      if (e >= N) continue;
-      const long long adr = obase + e * strd;
-      double2 u = to_d2(x[j]);
-      u.x *= invN;
-      u.y *= invN;
-      if (KE) {
-        ((C*)a.hy_out)[adr] = from_d2<C>(u);
-        continue;
-      }
-      if (KIND == KH_FIRST) {
-        fout[adr] = from_d2<C>(u);
-        continue;
-      }
-      double2 p = acc[j];
-      if (a.acc_sc) {
-        p.x /= OS.sc;
-        p.y /= OS.sc;
-      }
-      if (KIND == KH_MID) {
-        fout[adr] = from_d2<C>(cadd(p, u));
-        continue;
-      }
-      const double2 psv = ps[j];
-      const double rho = cabs2(psv);
-      double W = 0.0;
-      if (a.local_on) {
-        double ph = 0.0;
-        if (a.phi) {
-          ph = phv[j] * OS.phisc;
-          if (a.phi_out) a.phi_out[adr] = ph;
+      for (int j = j0; j < j0 + EC; ++j) {
+        const int e = ow.e0 + j * tpl;
+        if (EARLY || e >= N) continue;
+        const long long adr = obase + e * strd;
+        if (KIND == KH_MID || KIND == KH_LAST) acc[j] = a.acc ? to_d2(((const C*)a.acc)[adr]) : make_double2(0.0, 0.0);
+        if (KIND == KH_LAST) {
+          ps[j] = to_d2(fpsi[adr]);
+          phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
+          if (a.ydiff) old[j] = to_d2(fout[adr]);
         }
-        W = line_V<AX>(c2, e, a) + OS.beta2 * rho + OS.beta3 * rho * rho + ph;
       }
-      const double mu = OS.mu;
-      double2 h;
-      h.x = p.x + u.x + W * psv.x - mu * psv.x;
-      h.y = p.y + u.y + W * psv.y - mu * psv.y;
-      if (a.ydiff) ((C*)a.ydiff)[adr] = from_d2<C>(csub(h, old[j]));
-      fout[adr] = from_d2<C>(h);
-      hsum[0] += cabs2(h);
-      hsum[1] += psv.x * h.x + psv.y * h.y;
-      hmax[0] = fmax(hmax[0], rho);
+#pragma unroll
+      for (int j = j0; j < j0 + EC; ++j) {
+        const int e = ow.e0 + j * tpl;
+        if (e >= N) continue;
+        const long long adr = obase + e * strd;
+        double2 u = to_d2(x[j]);
+        u.x *= invN;
+        u.y *= invN;
+        if (KE) {
+          ((C*)a.hy_out)[adr] = from_d2<C>(u);
+          continue;
+        }
+        if (KIND == KH_FIRST) {
+          fout[adr] = from_d2<C>(u);
+          continue;
+        }
+        double2 p = acc[j];
+        if (a.acc_sc) {
+          p.x /= OS.sc;
+          p.y /= OS.sc;
+        }
+        if (KIND == KH_MID) {
+          fout[adr] = from_d2<C>(cadd(p, u));
+          continue;
+        }
+        const double2 psv = ps[j];
+        const double rho = cabs2(psv);
+        double W = 0.0;
+        if (a.local_on) {
+          double ph = 0.0;
+          if (a.phi) {
+            ph = phv[j] * OS.phisc;
+            if (a.phi_out) a.phi_out[adr] = ph;
+          }
+          W = line_V<AX>(c2, e, a) + OS.beta2 * rho + OS.beta3 * rho * rho + ph;
+        }
+        const double mu = OS.mu;
+        double2 h;
+        h.x = p.x + u.x + W * psv.x - mu * psv.x;
+        h.y = p.y + u.y + W * psv.y - mu * psv.y;
+        if (a.ydiff) ((C*)a.ydiff)[adr] = from_d2<C>(csub(h, old[j]));
+        fout[adr] = from_d2<C>(h);
+        hsum[0] += cabs2(h);
+        hsum[1] += psv.x * h.x + psv.y * h.y;
+        hmax[0] = fmax(hmax[0], rho);
+      }
     }
   }
   if (KIND == KH_LAST && brow && (a.partials || a.hpartials))
@@ -758,38 +772,47 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   for (int q = 0; q < (ST == ST_CDOT ? 16 : 1); ++q) zv[q] = 0.0;
   if (oon) {
     const long long obase = ob * a.out_ib + oo * a.out_so + oi * a.out_si;
-    double dv[BEC_EPT], dv2[BEC_EPT];
-    C fp[ST == ST_CDOT ? BEC_EPT : 1], fd[ST == ST_CDOT ? BEC_EPT : 1], fg[ST == ST_CDOT ? BEC_EPT : 1];
-    C fy[ST == ST_CDOT ? BEC_EPT : 1], fs[ST == ST_CDOT ? BEC_EPT : 1];
+    // elements in chunks of EC: the five fields of the inner products are
+    // loaded for EC elements at a time (fewer registers), summed in the same
+    // element order as one chunk would
+    constexpr int EC = (ST == ST_CDOT) ? BEC_CDOT_CHUNK : BEC_EPT;
 #pragma unroll
-    for (int j = 0; j < BEC_EPT; ++j) {
-      const int q = lane + j * tpl, e = q - a.off_store;
-      if (q >= L || e < 0 || e >= a.n_store) continue;
-      const long long adr = obase + e * a.out_s;
-      if (ST == ST_RE && a.dot) dv[j] = a.dot[adr];
-      if (ST == ST_RE && a.dot2) dv2[j] = a.dot2[adr];
-      if (ST == ST_CDOT) {
-        fp[j] = ((const C*)a.dpsi)[adr]; fd[j] = ((const C*)a.dd)[adr]; fg[j] = ((const C*)a.dg)[adr];
-        fy[j] = ((const C*)a.dy)[adr]; fs[j] = ((const C*)a.ds)[adr];
+    for (int j0 = 0; j0 < BEC_EPT; j0 += EC) {
+      double dv[EC], dv2[EC];
+      C fp[ST == ST_CDOT ? EC : 1], fd[ST == ST_CDOT ? EC : 1], fg[ST == ST_CDOT ? EC : 1];
+      C fy[ST == ST_CDOT ? EC : 1], fs[ST == ST_CDOT ? EC : 1];
+#pragma unroll
+      for (int i = 0; i < EC; ++i) {
+        const int j = j0 + i;
+        const int q = lane + j * tpl, e = q - a.off_store;
+        if (q >= L || e < 0 || e >= a.n_store) continue;
+        const long long adr = obase + e * a.out_s;
+        if (ST == ST_RE && a.dot) dv[i] = a.dot[adr];
+        if (ST == ST_RE && a.dot2) dv2[i] = a.dot2[adr];
+        if (ST == ST_CDOT) {
+          fp[i] = ((const C*)a.dpsi)[adr]; fd[i] = ((const C*)a.dd)[adr]; fg[i] = ((const C*)a.dg)[adr];
+          fy[i] = ((const C*)a.dy)[adr]; fs[i] = ((const C*)a.ds)[adr];
+        }
       }
-    }
 #pragma unroll
-    for (int j = 0; j < BEC_EPT; ++j) {
-      const int q = lane + j * tpl, e = q - a.off_store;
-      if (q >= L || e < 0 || e >= a.n_store) continue;
-      const long long adr = obase + e * a.out_s;
-      const C w = v[j];
-      if (ST == ST_CDOT) {
-        const C zi = cscale(w, a.out_scale);
-        ((C*)a.out)[adr] = zi;
-        dot_accum(zv, to_d2(zi), to_d2(fp[j]), to_d2(fd[j]), to_d2(fg[j]), to_d2(fy[j]), to_d2(fs[j]));
-      } else if (ST == ST_C) {
-        ((C*)a.out)[adr] = cscale(w, a.out_scale);
-      } else {
-        const double r = (double)w.x * a.out_scale;
-        ((double*)a.out)[adr] = r;
-        if (a.dot) dsum[0] += r * dv[j];
-        if (a.dot2) dsum[1] += r * dv2[j];
+      for (int i = 0; i < EC; ++i) {
+        const int j = j0 + i;
+        const int q = lane + j * tpl, e = q - a.off_store;
+        if (q >= L || e < 0 || e >= a.n_store) continue;
+        const long long adr = obase + e * a.out_s;
+        const C w = v[j];
+        if (ST == ST_CDOT) {
+          const C zi = cscale(w, a.out_scale);
+          ((C*)a.out)[adr] = zi;
+          dot_accum(zv, to_d2(zi), to_d2(fp[i]), to_d2(fd[i]), to_d2(fg[i]), to_d2(fy[i]), to_d2(fs[i]));
+        } else if (ST == ST_C) {
+          ((C*)a.out)[adr] = cscale(w, a.out_scale);
+        } else {
+          const double r = (double)w.x * a.out_scale;
+          ((double*)a.out)[adr] = r;
+          if (a.dot) dsum[0] += r * dv[i];
+          if (a.dot2) dsum[1] += r * dv2[i];
+        }
       }
     }
   }
