@@ -429,12 +429,14 @@ __device__ __forceinline__ void fft_stage_c(C* s, int lane, const C* __restrict_
     for (int b = 0; b < MAXB; ++b) {
       const int j = lane + b * TPL;
       if (j < nb) {
-        const int k = j & (NS - 1);
+        const int k = (NS == 1) ? 0 : j % NS;
+        if constexpr (NS > 1) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          C w = tw[r * k * step];
-          if (SIGN > 0) w.y = -w.y;
-          v[b][r] = cmul(v[b][r], w);
+          for (int r = 1; r < R; ++r) {
+            C w = tw[r * k * step];
+            if (SIGN > 0) w.y = -w.y;
+            v[b][r] = cmul(v[b][r], w);
+          }
         }
         bfly<R, SIGN>(v[b]);
         const int idxD = (j - k) * R + k;
@@ -453,7 +455,52 @@ __device__ __forceinline__ void fft_reg_c(C (&v)[8], C* s, const C* twl, int lan
   if constexpr (P::NMID >= 2) fft_stage_c<8, SIGN, PD, LC, 64, P::TPL>(s, lane, twl, active);
   reg_last<P::RL, SIGN, PD>(v, s, LC, lane, P::TPL, twl, active);
 }
+// The transform through LDS of a line of compile-time length LC = 3 2^a
+// (192, 384) in the plans' stage order (8, 3 or 6, 8) on the power-of-two
+// thread count covering LC/8, as fft_line does for such a plan.
+template <int LC> struct LdsPlan;
+template <> struct LdsPlan<192> {
+  static constexpr int TPL = 32, R0 = 8, R1 = 3, R2 = 8;
+};
+template <> struct LdsPlan<384> {
+  static constexpr int TPL = 64, R0 = 8, R1 = 6, R2 = 8;
+};
+template <int SIGN, bool PD, int LC, class C>
+__device__ __forceinline__ void fft_lds_c(C (&v)[8], C* s, const C* twl, int lane, bool active) {
+  using P = LdsPlan<LC>;
+  constexpr int T = P::TPL;
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int e = lane + m * T;
+      if (m * T < LC && e < LC) s[lx<PD>(e)] = v[m];
+    }
+  }
+  __syncthreads();
+  fft_stage_c<P::R0, SIGN, PD, LC, 1, T>(s, lane, twl, active);
+  fft_stage_c<P::R1, SIGN, PD, LC, P::R0, T>(s, lane, twl, active);
+  fft_stage_c<P::R2, SIGN, PD, LC, P::R0 * P::R1, T>(s, lane, twl, active);
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int e = lane + m * T;
+      v[m] = (m * T < LC && e < LC) ? s[lx<PD>(e)] : cmk<C>(0.0, 0.0);
+    }
+  }
+}
+// threads per line of the kernels built for length LC
+__host__ __device__ constexpr int lc_tpl(int LC) {
+  return (LC & (LC - 1)) == 0 ? LC / 8 : (LC == 192 ? 32 : 64);
+}
+
 // host side: the compile-time length whose transform equals plan p's, or 0
+__host__ __device__ inline int lds_plan_len(const FftPlan& p) {
+  if (p.reg || p.wide || p.odd || p.nst != 3 || p.rad[0] != 8 || p.rad[2] != 8) return 0;
+  if (p.L == 192 && p.rad[1] == 3) return 192;
+  if (p.L == 384 && p.rad[1] == 6) return 384;
+  return 0;
+}
 __host__ __device__ inline int reg_plan_len(const FftPlan& p) {
   if (!p.reg || p.odd || p.L < 64 || p.L > 1024 || (p.L & (p.L - 1)) != 0) return 0;
   int a = 0;
@@ -497,11 +544,12 @@ __device__ __forceinline__ void fft_line(C (&v)[8], C* s, const FftPlan& p, cons
   }
 }
 
-// fft_line, or fft_reg_c when the kernel was built for the plan's length LC
+// fft_line, or fft_reg_c / fft_lds_c when the kernel was built for the plan's length LC
 template <int SIGN, bool ODD, bool PD, int LC, class C>
 __device__ __forceinline__ void fft_any(C (&v)[8], C* s, const FftPlan& p, const C* twl, int lane, int tpl,
                                         bool active) {
-  if constexpr (LC != 0) fft_reg_c<SIGN, PD, LC>(v, s, twl, lane, active);
+  if constexpr (LC != 0 && (LC & (LC - 1)) == 0) fft_reg_c<SIGN, PD, LC>(v, s, twl, lane, active);
+  else if constexpr (LC != 0) fft_lds_c<SIGN, PD, LC>(v, s, twl, lane, active);
   else fft_line<SIGN, ODD, PD>(v, s, p, twl, lane, tpl, active);
 }
 

@@ -272,7 +272,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const C* fd = (const C*)a.d;
   C* fout = (C*)a.out;
   constexpr bool KE = (KIND == KE_FIRST || KIND == KE_REST);
-  const int N = LC ? LC : a.N, tpl = LC ? LC / 8 : a.tpl, lpw = a.lpw, nl = a.nl;
+  const int N = LC ? LC : a.N, tpl = LC ? lc_tpl(LC) : a.tpl, lpw = a.lpw, nl = a.nl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(BEC_WG) void k_lcoef(KinArgs a) {
   C* lds = smem + a.N;                 // lines [lpw][lpitch<X::contig>(N)]
   const C* fpsi = (const C*)a.psi;
   const C* fd = (const C*)a.d;
-  const int N = LC ? LC : a.N, tpl = LC ? LC / 8 : a.tpl, lpw = a.lpw, nl = a.nl;
+  const int N = LC ? LC : a.N, tpl = LC ? lc_tpl(LC) : a.tpl, lpw = a.lpw, nl = a.nl;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int l0 = lb * lpw;
   const int total = a.nb * nl;
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   using F = typename CT<C>::F;
   extern __shared__ double2 smem_d[];
   C* smem = (C*)smem_d;
-  const int L = LC ? LC : a.fp.L, tpl = LC ? LC / 8 : a.tpl, lpw = a.lpw, nl = a.nl;
+  const int L = LC ? LC : a.fp.L, tpl = LC ? lc_tpl(LC) : a.tpl, lpw = a.lpw, nl = a.nl;
   C* twl = smem;                       // twiddles [L]
   C* lds = smem + L;                   // lines [lpw][lpitch<CONTIG>(L)]
   const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -1446,12 +1446,15 @@ static int xcd_grid(int nlines, int lpw) {
 
 // Calls f(std::integral_constant<int, LC>) with LC the compile-time length of
 // the plan (reg_plan_len) when the kernels were built for it, else LC = 0
-// (the generic transform).  Built: 128, 256 and, with P512, 512.
+// (the generic transform).  Built: 128, 192, 256, 384 and, with P512, 512.
 template <bool P512, class F>
 static void with_lc(const FftPlan& fp, F&& f) {
-  switch (reg_plan_len(fp)) {
+  const int r = reg_plan_len(fp);
+  switch (r ? r : lds_plan_len(fp)) {
     case 128: f(std::integral_constant<int, 128>{}); return;
+    case 192: f(std::integral_constant<int, 192>{}); return;
     case 256: f(std::integral_constant<int, 256>{}); return;
+    case 384: f(std::integral_constant<int, 384>{}); return;
     case 512:
       if constexpr (P512) {
         f(std::integral_constant<int, 512>{});
