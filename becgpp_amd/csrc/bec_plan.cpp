@@ -635,13 +635,14 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     const int nlk = (int)(p.n / N);
     if (p.fpN.reg && nlk % lpw_min(N / 8) != 0) p.fpN.reg = 0;
     if (p.fpM.reg && nlk % lpw_min(p.M / 8) != 0) p.fpM.reg = 0;
-    // a 2D grid of N <= 128 has few lines per instance (N): its N-point lines
-    // run through LDS on about L/4 threads, one wavefront per line at N = 128
-    // (5% more CG iterations/s on 2D N = 128; 3D grids, with N^2 lines per
-    // instance, lost 17% that way and keep L/8)
-    if (ndim == 2 && N <= 128) {
-      p.fpN.reg = 0;
+    // a line of L/8 threads that leaves one instance's lines short of a
+    // whole-wavefront workgroup (2D N = 16: 16 lines of 2 threads) runs
+    // through LDS on about L/4 threads, which fills one (the register and
+    // compile-time plans measured faster wherever they batch: 2D N = 128
+    // ran 18% more CG iterations/s on them than in this form)
+    if (ndim == 2 && !p.fpN.reg && nlk % lpw_min(line_cfg(p.fpN).tpl) != 0) {
       p.fpN.wide = 1;
+      if (nlk % lpw_min(line_cfg(p.fpN).tpl) != 0) p.fpN.wide = 0;
     }
     p.lcN = line_cfg(p.fpN);
     p.lcM = p.twM ? line_cfg(p.fpM) : p.lcN;
