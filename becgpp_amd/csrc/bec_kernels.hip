@@ -1604,10 +1604,11 @@ static int go_lcoef(const KinArgs& a0, hipStream_t s) {
     const size_t shm = ((size_t)a.lpw * lpitch<Ax<AX>::contig>(a.N) + a.N) * csz;
     const int id = 100 + 10 * AX + KL_FIRST;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    // psi and d read, z and g read and d written (direction steps), phi read,
+    // psi and d read, z read and d written (direction steps; g is read too
+    // only on a step that falls back to the gradient direction), phi read,
     // q and r written (long range)
     const double n = (double)a.nl * a.N, c = (a.f32 ? 8.0 : 16.0) * n, r = 8.0 * n;
-    const double bpi = 2.0 * c + (a.d_out ? 3.0 * c : 0.0) + (a.phi ? r : 0.0) + (a.rho_out ? 2.0 * r : 0.0);
+    const double bpi = 2.0 * c + (a.d_out ? 2.0 * c : 0.0) + (a.phi ? r : 0.0) + (a.rho_out ? 2.0 * r : 0.0);
     if (a.act.fl) {
       a.ucount = prof_counter(id);
       a.ubytes = (unsigned long long)bpi;

@@ -24,18 +24,21 @@ How each part is pinned (tests in ``tests/test_oracle_*.py``):
 * solver -- converged energies of benchmarks/results/itp_timing/timing.csv:2-3
   and pcg_variants.csv; stall / maxit semantics (tests/test_audit.py:165-172,
   252-255); virial identities (tests/test_kernels.py:30-50).
-* preconditioner shift sigma -- the lower median; pinned against a brute-force
-  sort on small inputs and torch.median's documented semantics.  The
-  reference's own tests never observe sigma: beyond that definition it is
-  parity unpinned.
+* preconditioner (shift sigma and z) -- vectors the reference computed itself
+  for nine cases (tests/golden/make_precond_golden.py: the six physics sets
+  of tests/test_audit.py:55-62, the floor, a cubic-quintic and a 3D state;
+  tests/test_oracle_precond.py), which fix the local stiffness, the bulk mask
+  and the floor; the closed form P((k^2/2 + sigma) v) = proj_tangent(v); the
+  lower median against a brute-force sort and a hand-computed density.
 * seeds -- the noise is this build's counter-based generator
   (``counter_normals``: splitmix64 + Box-Muller; the splitmix64 stream is
   pinned by its published outputs for seed 1234567, the normals by their
   moments); the reference's seed tensors depend on the torch device RNG and
-  cannot be reproduced.  The envelopes, lattice positions (numpy default_rng
-  angle and shift, as the reference draws them) and the order of the factors
-  are the reference's; the noise-free seeds are pinned by closed forms
-  (Gaussian envelope, TF parabola, vortex windings).
+  cannot be reproduced, so the noise itself is parity unpinned against the
+  reference by construction.  The envelopes, lattice positions (numpy
+  default_rng angle and shift, as the reference draws them) and the order of
+  the factors are the reference's; the noise-free seeds are pinned by closed
+  forms (Gaussian envelope, TF parabola, vortex windings).
 """
 import math
 import os
