@@ -11,6 +11,7 @@
 #include "bec_common.h"
 
 #define BEC_BMAX 64          // instances per launch (per-instance scalars by value; 64-bit masks)
+#define BEC_GMAX 8           // grids advanced together by bec_ground_state_uneven
 
 // line-pass axis modes (which axis a pass transforms, and how lines map to
 // memory).  2D: axis 1 = y (contiguous), axis 0 = x (stride N).

@@ -17,6 +17,7 @@ from .operators import compute_residual, observables, phys_for
 from .seeds import make_seed
 
 BATCH_MAX = 64
+GRIDS_MAX = 8           # grids advanced together by one native call
 
 
 def batchable(G):
@@ -167,11 +168,11 @@ def _grid_key(G):
 
 def uneven_schedule(grids):
     """Groups of instance indices sharing a grid (geometry, kernel, l_z), and
-    whether their loops may run concurrently.  Grids of one geometry share a
-    native plan (plan_of), whose kernel spectrum, stream and workspaces one
-    loop uses at a time, so groups that share a geometry (the same grid with
-    and without long range, or two l_z) run one after the other; concurrent
-    loops need a plan each."""
+    whether their loops may run together in one native call.  Grids of one
+    geometry share a native plan (plan_of), whose kernel spectrum, streams
+    and workspaces one loop uses at a time, so groups that share a geometry
+    (the same grid with and without long range, or two l_z) run one after
+    the other; loops run together need a plan each."""
     sets = {}
     for i, G in enumerate(grids):
         sets.setdefault(_grid_key(G), []).append(i)
@@ -180,65 +181,90 @@ def uneven_schedule(grids):
     return groups, len(groups) > 1 and len(set(plan_keys)) == len(plan_keys)
 
 
+def solve_native_uneven(cfg_sets, grids, psi_sets, verbose=False):
+    """Run the batches of several grids together (bec_ground_state_uneven):
+    cfg_sets[g] and psi_sets[g] are the configurations and initial states of
+    grid grids[g] (distinct native plans, at most 64 instances each, one set
+    of solver settings per grid).  The steps of every grid are captured into
+    one hipGraph, so each graph launch advances all instances of all grids.
+    Returns, per grid, (list of psi, list of raw results)."""
+    ng = len(grids)
+    if not (ng == len(cfg_sets) == len(psi_sets)):
+        raise ValueError("one configuration set and one state set per grid")
+    if not 1 <= ng <= GRIDS_MAX:
+        raise ValueError(f"number of grids must be in [1, {GRIDS_MAX}]")
+    plans, psis, phs, scs, nbs = [], [], [], [], []
+    for cs, G, p0s in zip(cfg_sets, grids, psi_sets):
+        if len(cs) != len(p0s):
+            raise ValueError("one initial state per configuration")
+        if not 1 <= len(cs) <= BATCH_MAX:
+            raise ValueError(f"batch size must be in [1, {BATCH_MAX}]")
+        lr_any = G["kernel"] != "none" and any(abs(float(c["G_C"])) > 1e-15 for c in cs)
+        pl = ensure_kernel(G, "K") if lr_any else plan_of(G)
+        if cs[0].get("autotune", False):
+            pl.autotune(len(cs))
+        plans.append(pl)
+        psis.append(torch.stack([as_field(p0, G) for p0 in p0s]).contiguous())
+        phs += [phys_for(G, c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cs]
+        scs.append(_solver_cfg(cs[0], verbose))
+        nbs.append(len(cs))
+    lib = plans[0].lib
+    tot = sum(nbs)
+    res = (_lib.SolveOut * tot)()
+    _lib.check(lib.bec_ground_state_uneven(
+        ng, (ctypes.c_void_p * ng)(*[pl.h.value for pl in plans]), (ctypes.c_int * ng)(*nbs),
+        (_lib.Phys * tot)(*phs), (ctypes.POINTER(_lib.SolverCfg) * ng)(*[ctypes.pointer(sc) for sc in scs]),
+        (ctypes.c_void_p * ng)(*[t.data_ptr() for t in psis]), res))
+    out, o = [], 0
+    for psi, nb in zip(psis, nbs):
+        out.append(([psi[b] for b in range(nb)], [res[o + b] for b in range(nb)]))
+        o += nb
+    return out
+
+
 def ground_state_uneven(cfgs, psi0s=None, grids=None, verbose=False):
     """ground_state for configurations on different grids (uneven sizes).
 
-    Instances sharing a grid geometry form one batched device loop
-    (ground_state_batch); the loops of different grids run at the same time,
-    each from its own host thread on its own plan's stream, so small grids
-    fill the GPU together.  grids[i] (optional) is the grid of cfgs[i].
-    Returns, in input order, what ground_state(cfgs[i], grids[i], psi0s[i])
-    returns."""
-    import threading
+    Instances sharing a grid form batches as in ground_state_batch (at most
+    64, split by solver settings); the batches of different grids are solved
+    by one call of the native library, whose every graph launch advances all
+    instances of all grids (solve_native_uneven).  Grids that share a native
+    plan (the same geometry with another kernel or l_z), and host-driven
+    loops, are solved one grid after another.  grids[i] (optional) is the
+    grid of cfgs[i].  Returns, in input order, what ground_state(cfgs[i],
+    grids[i], psi0s[i]) returns."""
     n = len(cfgs)
     if grids is None:
         grids = [make_grid(c) for c in cfgs]
     if psi0s is None:
         psi0s = [make_seed(G, c) for G, c in zip(grids, cfgs)]
-    groups, concurrent = uneven_schedule(grids)
+    groups, together = uneven_schedule(grids)
+    together = together and not any(c.get("host_loop", False) for c in cfgs)
     out = [None] * n
-    if not concurrent:
+    if not together:
         for ks in groups:
             for k, r in zip(ks, ground_state_batch([cfgs[k] for k in ks], G=grids[ks[0]],
                                                    psi0s=[psi0s[k] for k in ks], verbose=verbose)):
                 out[k] = r
         return out
-    # Plans, kernel spectra (K and, for observables, W) and batch workspaces
-    # are made here: the workers then allocate nothing in the library, and each
-    # runs on its own stream, as graph capture in one thread requires of the
-    # others.
+    # each grid's batches, in order; call r solves the r-th batch of every grid
+    queues = []
     for ks in groups:
-        G = grids[ks[0]]
-        pl = plan_of(G)
-        if G["kernel"] != "none":
-            pl = ensure_kernel(G, "K")
-            if G["kernel"] == "q2d":
-                ensure_kernel(G, "W")
-        if cfgs[ks[0]].get("autotune", False):
-            pl.autotune(min(len(ks), BATCH_MAX))
-        _lib.check(pl.lib.bec_plan_reserve(pl.h, min(len(ks), BATCH_MAX)))
-    torch.cuda.synchronize()
-    errors = []
-
-    def work(ks):
-        try:
-            st = torch.cuda.Stream()
-            with torch.cuda.stream(st):
-                G = grids[ks[0]]
-                p0 = [psi0s[k].to(device=st.device) if torch.is_tensor(psi0s[k]) else psi0s[k] for k in ks]
-                for k, r in zip(ks, ground_state_batch([cfgs[k] for k in ks], G=G, psi0s=p0, verbose=verbose)):
-                    out[k] = r
-                st.synchronize()
-        except BaseException as e:          # re-raised in the caller's thread
-            errors.append(e)
-
-    th = [threading.Thread(target=work, args=(ks,)) for ks in groups]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    if errors:
-        raise errors[0]
+        cap = BATCH_MAX if batchable(grids[ks[0]]) else 1
+        sets = {}
+        for k in ks:
+            sets.setdefault(_solver_key(cfgs[k]), []).append(k)
+        queues.append([kk[lo:lo + cap] for kk in sets.values() for lo in range(0, len(kk), cap)])
+    for r in range(max(len(q) for q in queues)):
+        row = [q[r] for q in queues if r < len(q)]
+        for lo in range(0, len(row), GRIDS_MAX):
+            part = row[lo:lo + GRIDS_MAX]
+            got = solve_native_uneven([[cfgs[k] for k in kk] for kk in part], [grids[kk[0]] for kk in part],
+                                      [[psi0s[k] for k in kk] for kk in part], verbose=verbose)
+            for kk, (psis, rs) in zip(part, got):
+                G = grids[kk[0]]
+                for k, p, r_ in zip(kk, psis, rs):
+                    out[k] = (p.detach(), G, _finish(p, r_, G, cfgs[k]))
     return out
 
 

@@ -84,8 +84,8 @@ WORKLOADS = {
         instance_grid=lambda i: dict(Ngrid=(128, 192, 256)[i % 3]),
         desc="quasi-2D q2d kernel l_z=0.5, beta2=50, L=16, Omega in {0,0.5} x G_C in {1,2,5,10}, nseeds=4 "
              "(32 solves), N = 128/192/256 round-robin by instance id; the instances of one grid form one "
-             "batched loop and the three grids' loops run concurrently on their own streams; res_tol=1e-6, "
-             "linesearch_max=20",
+             "batched loop and one hipGraph advances the three grids' loops together (a branch per grid); "
+             "res_tol=1e-6, linesearch_max=20",
         replay_iters=200),
     # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
     "droplet_sweep_2d_n256": dict(
@@ -286,7 +286,7 @@ def run_workload(name, steps, warmup, a, cpu_threads, cpu_desc):
                         line_search=str(cfg.get("line_search", "poly")),
                         warm_fp32=bool(cfg.get("warm_fp32", False)),
                         path="batched" if len(grids) == 1 else
-                        "batched per grid, the grids' loops concurrent (one host thread and stream each)"),
+                        "batched per grid, every grid's steps in one hipGraph (bec_ground_state_uneven)"),
             solves_per_s=steps * nsol / wall, cg_iters_per_s=steps * sum(its) / wall,
             iters_per_solve=sum(its) / nsol, iters_min=min(its), iters_max=max(its),
             E_selected=[best[i][1]["E"] for i in range(npt)] if npt > 1 else best[0][1]["E"],

@@ -279,6 +279,19 @@ int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg,
 int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
                            bec_solve_out* out);
 
+/* Batched ground_state over ng (<= 8) grids at once, e.g. the seeds and
+ * points of a sweep whose autobox grids differ in size (modes.py:295-387,
+ * which solves them one after another).  Grid g has its own plan plans[g]
+ * (distinct plans), nb[g] (<= 64) instances with physics ph[off_g ..] and
+ * solver settings cfg[g], and its states at psi_io[g] ([nb[g]][n_g]); ph and
+ * out[] run group after group (off_g = nb[0] + .. + nb[g-1]).  Every grid's
+ * instances follow the ground_state loop exactly, as in
+ * bec_ground_state_batch; the steps of all grids are captured into one
+ * hipGraph (one branch per grid), so one graph launch advances every
+ * instance of every grid.  The device-driven loop only (cfg host_loop = 0). */
+int bec_ground_state_uneven(int ng, bec_plan* const* plans, const int* nb, const bec_phys* ph,
+                            const bec_solver_cfg* const* cfg, void* const* psi_io, bec_solve_out* out);
+
 /* Largest batch the plan's grid supports in bec_ground_state_batch: 64, or 1
  * when one instance's lines do not fill whole workgroups of its line passes
  * (small or awkward N, e.g. 2D N = 20 or 50).  No reference counterpart: the
@@ -288,9 +301,7 @@ int bec_plan_max_batch(bec_plan* p, int* nb_max);
 /* Sizes the plan's workspace, solver state and device-loop buffers for nb
  * instances ahead of bec_ground_state_batch (which otherwise sizes them on
  * first use).  No counterpart in the reference, which allocates per call:
- * after it, a batched solve on this plan allocates nothing, so the solves of
- * several plans (grid sizes) may run from separate host threads at once,
- * one thread per plan, each on its own non-default stream. */
+ * after it, a batched solve on this plan allocates nothing. */
 int bec_plan_reserve(bec_plan* p, int nb);
 
 /* Autotuner (no counterpart in the reference): times the line passes of one

@@ -148,9 +148,9 @@ def test_batch_argument_errors():
     assert not B.solvers.batchable(B.make_grid(cfg(Ngrid=50, L=6.0)))
 
 
-def test_uneven_grids_run_concurrently_and_match():
-    """ground_state_uneven: instances on three grid sizes (one batch set per
-    grid, the sets running concurrently from separate host threads) give
+def test_uneven_grids_run_together_and_match():
+    """ground_state_uneven: instances on three grid sizes (one batch per grid,
+    all advanced by the graph of one bec_ground_state_uneven call) give
     exactly the per-grid batches' results, and each matches the oracle."""
     import becgpp_amd as B
     base = cfg(dimension="quasi2D", kernel="q2d", l_z=0.5, beta2=30.0, res_tol=1e-8, L=6.0)
@@ -174,3 +174,35 @@ def test_uneven_grids_run_concurrently_and_match():
         po, _, oo = O.ground_state(cs[k], O.make_grid(cs[k]), p0s[k])
         for key in ("E", "mu", "Lz"):
             assert _close(got[k][2][key], oo[key], 1e-8), (k, key)
+
+
+def test_uneven_call_with_mixed_settings_and_kernels():
+    """Grids of one uneven call differ in kernel (none / newton), dimension
+    (2D / 3D) and solver settings (res_tol, FP32 warm phase): the grids whose
+    settings split into two batches take two native calls, each advancing
+    every grid's next batch together.  Every instance equals the per-grid
+    ground_state_batch result bit for bit."""
+    import becgpp_amd as B
+    from becgpp_amd.solvers import solve_native_uneven
+    a = cfg(Ngrid=32, L=6.0, beta2=20.0, res_tol=1e-8)
+    b = cfg(Ngrid=48, L=7.0, beta2=10.0, kernel="newton", G_C=2.0, res_tol=1e-8, warm_fp32=True)
+    c = cfg(dimension="3D", Ngrid=16, L=6.0, beta2=5.0, kernel="newton", G_C=1.0, res_tol=1e-7)
+    cs = [a, dict(a, res_tol=1e-6, Omega=0.3), dict(a, beta2=40.0), b, dict(b, G_C=4.0), c]
+    p0s = [O.noisy_gaussian(O.make_grid(x), 1e-3, seed=70 + i) for i, x in enumerate(cs)]
+    grids = [B.make_grid(x) for x in cs]
+    print("[batch] uneven call: 2D N=32 (two settings), 2D N=48 newton FP32 warm, 3D N=16", flush=True)
+    got = B.ground_state_uneven(cs, psi0s=[torch.as_tensor(p).cuda() for p in p0s], grids=grids)
+    for ks in ([0, 1, 2], [3, 4], [5]):
+        ref = B.ground_state_batch([cs[k] for k in ks], G=grids[ks[0]],
+                                   psi0s=[torch.as_tensor(p0s[k]).cuda() for k in ks])
+        for k, (pr, _, orr) in zip(ks, ref):
+            pg, _, og = got[k]
+            assert og["iters"] == orr["iters"] and og["E"] == orr["E"], k
+            assert og.get("warm_iters", 0) == orr.get("warm_iters", 0), k
+            assert torch.equal(pg, pr), k
+    assert got[3][2].get("warm_iters", 0) > 0
+    for k in range(len(cs)):
+        assert got[k][2]["converged"], (k, got[k][2]["stop_reason"])
+    with pytest.raises(ValueError):         # one plan per grid of a call
+        solve_native_uneven([[a], [dict(a, beta2=30.0)]], [grids[0], grids[2]],
+                            [[torch.as_tensor(p0s[0]).cuda()], [torch.as_tensor(p0s[2]).cuda()]])
