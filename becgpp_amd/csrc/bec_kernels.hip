@@ -679,7 +679,10 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   if (!any) return;
   auto inst_active = [&](int b) { return b == b0 ? on0 : (b == b1 ? on1 : act_on(a.act, b)); };
   const bool brow = on0;               // this block stores a partial row
-  if (a.ucount && brow && threadIdx.x == 0 && l0 % nl == 0) atomicAdd(a.ucount, a.ubytes);
+  // measurement: the bytes of each active instance whose first line is here
+  if (a.ucount && threadIdx.x == 0)
+    for (int b = b0; b <= b1; ++b)
+      if ((long long)b * nl >= l0 && inst_active(b)) atomicAdd(a.ucount, a.ubytes);
   stage_twiddles(twl, a.fp);
   // the line this thread loads, transforms and stores, and its elements lane + j tpl
   const LineOwn ow = line_owner<CONTIG>(tpl, lpw);
@@ -1379,22 +1382,26 @@ int lpw_min(int tpl) {
 }
 #endif  // BEC_TU_IS(0)
 
-static int pick_lpw(int tpl, int nl, int total, bool strided) {
+// whole = false (passes that store no partial rows): a workgroup may span two
+// instances, so l need not divide nl -- the 2D convolution's strided pass runs
+// over the H = M/2 + 1 columns of the half spectrum, an odd count.
+static int pick_lpw(int tpl, int nl, int total, bool strided, bool whole = true) {
   const int lmin = lpw_min(tpl);
   const int lmax = BEC_WG / tpl > lmin ? BEC_WG / tpl : lmin;
   const int want = total / 1024;
+  auto fits = [&](int l) { return !whole || nl % l == 0; };
   int l = lmin;
   if (strided)
-    while (l < 4 && l * 2 <= lmax && nl % (l * 2) == 0) l *= 2;
-  while (l * 2 <= lmax && l * 2 <= want && nl % (l * 2) == 0) l *= 2;
+    while (l < 4 && l * 2 <= lmax && fits(l * 2)) l *= 2;
+  while (l * 2 <= lmax && l * 2 <= want && fits(l * 2)) l *= 2;
   return l;
 }
 // The heuristic count scaled by 2^shift (the plan's autotuned schedule):
-// halving keeps l a divisor of nl; doubling only while it still divides nl and
-// the workgroup stays within BEC_WG threads.
-static int shifted_lpw(int l, int tpl, int nl, int shift) {
+// halving keeps l a divisor of nl; doubling only while it still divides nl
+// (whole) and the workgroup stays within BEC_WG threads.
+static int shifted_lpw(int l, int tpl, int nl, int shift, bool whole = true) {
   const int lmin = lpw_min(tpl);
-  while (shift > 0 && l * 2 * tpl <= BEC_WG && nl % (l * 2) == 0) { l *= 2; --shift; }
+  while (shift > 0 && l * 2 * tpl <= BEC_WG && (!whole || nl % (l * 2) == 0)) { l *= 2; --shift; }
   while (shift < 0 && l / 2 >= lmin) { l /= 2; ++shift; }
   return l;
 }
@@ -1470,7 +1477,9 @@ template <bool CONTIG, int LD, int DIR, int MUL, int ST>
 static int go_pass(const PassArgs& a0, hipStream_t s) {
   PassArgs a = a0;
   const int total = a.nb * a.nl;
-  a.lpw = shifted_lpw(pick_lpw(a.tpl, a.nl, total, !CONTIG), a.tpl, a.nl, a.lsh[CONTIG ? 0 : 1]);
+  // only passes that store partial rows need whole workgroups per instance
+  const bool whole = ST == ST_CDOT || (ST == ST_RE && a.dot);
+  a.lpw = shifted_lpw(pick_lpw(a.tpl, a.nl, total, !CONTIG, whole), a.tpl, a.nl, a.lsh[CONTIG ? 0 : 1], whole);
   const int blocks = xcd_grid(total, a.lpw);
   const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
   const size_t shm = ((size_t)a.lpw * lpitch<CONTIG>(a.fp.L) + a.fp.L) * csz;
