@@ -23,10 +23,13 @@ enum KinKind { KE_FIRST = 0, KE_REST = 1, KH_FIRST = 2, KH_MID = 3, KH_LAST = 4,
                KL_FIRST = 5 };   // KL_FIRST: line coefficients of the polynomial Armijo search (first axis)
 
 // generic line-FFT pass options
-enum LoadKind { LD_C = 0, LD_R = 1, LD_HERM = 2 };
+enum LoadKind { LD_C = 0, LD_R = 1, LD_HERM = 2, LD_R2 = 3 };   // LD_R2: two real fields as re + i im
 enum DirKind { DIR_FWD = 0, DIR_INV = 1, DIR_FMI = 2 };
 enum MulKind { MUL_NONE = 0, MUL_KHAT2 = 1, MUL_KHAT3 = 2, MUL_PRE2 = 3, MUL_PRE3 = 4, MUL_GAUSS = 5 };
-enum StoreKind { ST_C = 0, ST_RE = 1, ST_CDOT = 2 };   // ST_CDOT: complex store + the 16 CG inner products
+// ST_CDOT: complex store + the 16 CG inner products; ST_RE2: the real part to
+// out and the imaginary part to out2 (a pair of real convolutions), with
+// sum(re dot), sum(re dot2) in partials and sum(im dot2) in partials2
+enum StoreKind { ST_C = 0, ST_RE = 1, ST_CDOT = 2, ST_RE2 = 3 };
 
 struct LineCfg {       // threads per line / lines per workgroup for a length
   int tpl, lpw;
@@ -80,7 +83,11 @@ struct PassArgs {
   double out_scale;
   const void* in;
   void* out;
+  const void* in2;                     // LD_R2: the field loaded as the imaginary part
+  void* out2;                          // ST_RE2: the imaginary part's output
   const double* khat; int H, M;        // real-even kernel spectrum, half table [H]^d
+  int kfull;                           // the lines cover all M frequencies of the transformed real axis
+                                       //   (a pair convolution), not its H = M/2 + 1
   const double* sigma; int Nk; double invNdx;   // per-instance Sobolev shifts and grid frequencies
   double gl2;                          // MUL_GAUSS: the line's factor exp(-gl2 k^2)
   const double* gl2s;                  // MUL_GAUSS: per-instance gl2 (device; overrides gl2)
@@ -88,6 +95,7 @@ struct PassArgs {
   const double* dot2;                  // optional with dot: a second sum(out_re * dot2) in the same rows
   double* partials;                    // per-block partial rows, device (nullable)
   double* hpartials;                   // the same rows in pinned host memory (nullable)
+  double* partials2;                   // ST_RE2: rows of sum(im dot2), device
   unsigned long long* ucount;          // measurement: algorithmic bytes of the instances covered
   unsigned long long ubytes;           //   (ubytes per instance, added once per instance; nullable)
   const double2* dpsi; const double2* dd; const double2* dg; const double2* dy; const double2* ds;  // ST_CDOT

@@ -712,19 +712,24 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         const int r = e - a.off_load;
         if (r >= 0 && r < a.n_load) {
           if (LD == LD_C) v[j] = ((const C*)a.in)[ibase + r * a.in_s];
+          else if (LD == LD_R2)
+            v[j] = cmk<C>((F)((const double*)a.in)[ibase + r * a.in_s], (F)((const double*)a.in2)[ibase + r * a.in_s]);
           else v[j] = cmk<C>((F)((const double*)a.in)[ibase + r * a.in_s], 0);
         }
       }
     }
     if (MUL == MUL_KHAT2 || MUL == MUL_KHAT3) {
       // 2D spectrum [kx][ky] (element stride H); 3D spectrum [ky][kx][kz], the
-      // line's kz column contiguous
+      // line's kz column contiguous.  The line's other frequencies run over
+      // [0, H) (half spectrum) or [0, M) (kfull: a pair convolution), both
+      // folded to [0, H) by the spectrum's evenness.
       long long kb;
+      const int nz = a.kfull ? a.M : a.H;
       if (MUL == MUL_KHAT2) {
-        kb = oli;                                      // column index ky in [0, H)
+        kb = min(oli, a.M - oli);                      // column index ky
       } else {
-        const int ky = oli / a.H, kx = oli - (oli / a.H) * a.H;
-        kb = ((long long)min(ky, a.M - ky) * a.H + kx) * a.H;
+        const int ky = oli / nz, kx = oli - ky * nz;
+        kb = ((long long)min(ky, a.M - ky) * a.H + min(kx, a.M - kx)) * a.H;
       }
       const long long kst = (MUL == MUL_KHAT2) ? (long long)a.H : 1;
 #pragma unroll
@@ -769,7 +774,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     fft_any<+1, ODD, CONTIG, LC>(v, s, a.fp, twl, lane, tpl, oon);
   }
   // stores: line position q = lane + j tpl goes to output element q - off_store
-  double dsum[2] = {0.0, 0.0};
+  constexpr bool RE = (ST == ST_RE || ST == ST_RE2);
+  double dsum[3] = {0.0, 0.0, 0.0};
   double zv[ST == ST_CDOT ? 16 : 1];
 #pragma unroll
   for (int q = 0; q < (ST == ST_CDOT ? 16 : 1); ++q) zv[q] = 0.0;
@@ -790,8 +796,8 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
         const int q = lane + j * tpl, e = q - a.off_store;
         if (q >= L || e < 0 || e >= a.n_store) continue;
         const long long adr = obase + e * a.out_s;
-        if (ST == ST_RE && a.dot) dv[i] = a.dot[adr];
-        if (ST == ST_RE && a.dot2) dv2[i] = a.dot2[adr];
+        if (RE && a.dot) dv[i] = a.dot[adr];
+        if (RE && a.dot2) dv2[i] = a.dot2[adr];
         if (ST == ST_CDOT) {
           fp[i] = ((const C*)a.dpsi)[adr]; fd[i] = ((const C*)a.dd)[adr]; fg[i] = ((const C*)a.dg)[adr];
           fy[i] = ((const C*)a.dy)[adr]; fs[i] = ((const C*)a.ds)[adr];
@@ -810,6 +816,15 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
           dot_accum(zv, to_d2(zi), to_d2(fp[i]), to_d2(fd[i]), to_d2(fg[i]), to_d2(fy[i]), to_d2(fs[i]));
         } else if (ST == ST_C) {
           ((C*)a.out)[adr] = cscale(w, a.out_scale);
+        } else if (ST == ST_RE2) {
+          const double r = (double)w.x * a.out_scale, im = (double)w.y * a.out_scale;
+          ((double*)a.out)[adr] = r;
+          ((double*)a.out2)[adr] = im;
+          if (a.dot) {
+            dsum[0] += r * dv[i];
+            dsum[1] += r * dv2[i];
+            dsum[2] += im * dv2[i];
+          }
         } else {
           const double r = (double)w.x * a.out_scale;
           ((double*)a.out)[adr] = r;
@@ -824,6 +839,10 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
     else block_partials<1, 0>(dsum, nullptr, a.partials, a.hpartials, lb);
   }
   if (ST == ST_CDOT && brow) block_partials<16, 0>(zv, nullptr, a.partials, a.hpartials, lb);
+  if (ST == ST_RE2 && a.dot && brow) {
+    block_partials<2, 0>(dsum, nullptr, a.partials, nullptr, lb);
+    block_partials<1, 0>(dsum + 2, nullptr, a.partials2, nullptr, lb);
+  }
 }
 
 #if BEC_TU_IS(0)
@@ -1412,13 +1431,13 @@ template <int LD, int ST>
 static double pass_bytes(const PassArgs& a) {
   const double lines = a.nl;
   const double cel = a.f32 ? 8.0 : 16.0;              // complex element
-  const double in_el = (LD == LD_R) ? 8.0 : cel;
+  const double in_el = (LD == LD_R) ? 8.0 : (LD == LD_R2 ? 16.0 : cel);
   const double loaded = (LD == LD_HERM) ? (double)(a.fp.L / 2 + 1) : (double)a.n_load;
   double b = lines * loaded * in_el;
-  const double st_el = (ST == ST_RE) ? 8.0 : cel;
+  const double st_el = (ST == ST_RE) ? 8.0 : (ST == ST_RE2 ? 16.0 : cel);
   b += lines * a.n_store * st_el;
-  if (ST == ST_RE && a.dot) b += lines * a.n_store * 8.0;
-  if (ST == ST_RE && a.dot2) b += lines * a.n_store * 8.0;
+  if ((ST == ST_RE || ST == ST_RE2) && a.dot) b += lines * a.n_store * 8.0;
+  if ((ST == ST_RE || ST == ST_RE2) && a.dot2) b += lines * a.n_store * 8.0;
   if (ST == ST_CDOT) b += lines * a.n_store * 5.0 * cel;
   return b;
 }
@@ -1478,12 +1497,16 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   PassArgs a = a0;
   const int total = a.nb * a.nl;
   // only passes that store partial rows need whole workgroups per instance
-  const bool whole = ST == ST_CDOT || (ST == ST_RE && a.dot);
+  const bool whole = ST == ST_CDOT || ((ST == ST_RE || ST == ST_RE2) && a.dot);
   a.lpw = shifted_lpw(pick_lpw(a.tpl, a.nl, total, !CONTIG, whole), a.tpl, a.nl, a.lsh[CONTIG ? 0 : 1], whole);
   const int blocks = xcd_grid(total, a.lpw);
   const size_t csz = a.f32 ? sizeof(float2) : sizeof(double2);
   const size_t shm = ((size_t)a.lpw * lpitch<CONTIG>(a.fp.L) + a.fp.L) * csz;
-  const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * LD + 10 * DIR + MUL + (ST == ST_CDOT ? 5 : 0);
+  // site id: 2000 + 1000 contig + 100 load + 10 direction + multiplier (+5 for
+  // the 16 inner products, +7 for a pair's two real outputs); the complex
+  // passes of a pair convolution count load 4
+  const int id = 2000 + 1000 * (CONTIG ? 1 : 0) + 100 * (LD == LD_C && a.kfull ? 4 : LD) + 10 * DIR + MUL +
+                 (ST == ST_CDOT ? 5 : (ST == ST_RE2 ? 7 : 0));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const double bpi = pass_bytes<LD, ST>(a);
   if (a.act.fl) {
@@ -1493,7 +1516,7 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
   const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb)));
   const dim3 grid(blocks), block(a.tpl * a.lpw);
   // FP32 phases run the plain transforms and the preconditioner passes only
-  constexpr bool kF32 = (LD == LD_C) && (MUL == MUL_NONE || MUL == MUL_PRE2 || MUL == MUL_PRE3);
+  constexpr bool kF32 = (LD == LD_C) && ST != ST_RE2 && (MUL == MUL_NONE || MUL == MUL_PRE2 || MUL == MUL_PRE3);
   if (a.f32) {
     if constexpr (kF32) {
       if (ev) hipExtLaunchKernelGGL((k_pass<CONTIG, LD, DIR, MUL, ST, float2>), grid, block, shm, s, e0, e1, 0, a);
@@ -1549,6 +1572,9 @@ int launch_pass_conv(int contig, int ld, int dir, int mul, int st, const PassArg
   BEC_P(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C)
   BEC_P(0, LD_C, DIR_FMI, MUL_KHAT3, ST_C)
   BEC_P(1, LD_HERM, DIR_INV, MUL_NONE, ST_RE)
+  // pair convolution (two real fields at once)
+  BEC_P(1, LD_R2, DIR_FWD, MUL_NONE, ST_C)
+  BEC_P(1, LD_C, DIR_INV, MUL_NONE, ST_RE2)
   return -1;
 }
 #endif

@@ -73,6 +73,12 @@ inline const double* hres(const Plan& p, int b) { return p.hred + (long long)b *
 // partial-row region of the sums (device loop) or their host fold
 int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale, double* out, const double* dot,
              const double* dot2 = nullptr, int site = C_EGRAV, int slot = R_EGRAV);
+// The convolutions of two real fields at once (device-driven loop only): kq =
+// dV (K * q)_crop and kr = dV (K * r)_crop from one complex convolution of
+// q + i r (K is real and even); rows of sum(kq q), sum(kq r) in part_of(site_qr)
+// and of sum(kr r) in part_of(site_rr), the same count per instance (returned).
+int run_conv_pair(Plan& p, const Batch& B, const double* q, const double* r, double* kq, double* kr, int site_qr,
+                  int site_rr);
 // kinetic/L_z/local sums of psi - t[b] d into hred[b][R_EFIRST..], [R_EREST..];
 // rho_out (optional) receives |psi - t d|^2
 // Fusions of the device-driven loop into the first energy pass: instances

@@ -332,6 +332,77 @@ int run_conv(Plan& p, int which, const Batch& B, const double* rho, double scale
   return nb_rows;
 }
 
+// K*q and K*r as one complex convolution of q + i r: K is real and even, so
+// the result's real and imaginary parts are the two convolutions.  The first
+// pass loads the pair and keeps all M frequencies of its axis (no half
+// spectrum), the middle passes run over those M columns (kfull), and the last
+// one stores the real part to kq and the imaginary part to kr with the sums
+// the polynomial line search needs.  Half the launches of two real
+// convolutions, one read of the kernel spectrum, and no Hermitian handling.
+int run_conv_pair(Plan& p, const Batch& B, const double* q, const double* r, double* kq, double* kr, int site_qr,
+                  int site_rr) {
+  if (!p.khat[0]) throw BecError(1, "kernel spectrum not built for this plan");
+  if (!B.dev) throw BecError(2, "internal: pair convolutions run in the device-driven loop only");
+  const int N = p.N, M = p.M, i0 = (p.M - p.N) / 2;
+  const double fac = p.dv / (double)p.nM;
+  const long long n = p.n, S1 = p.nS1, S2 = p.nS2;
+  int nb_rows = 0;
+  PassArgs last;
+  if (p.ndim == 2) {
+    // rows x: y lines (contiguous), the pair padded -> s1 [N][M]
+    PassArgs c1 = pass_base(p, p.fpM, p.lcM, B);
+    c1.nl = N; c1.in_so = N; c1.in_s = 1; c1.in_ib = n; c1.n_load = N; c1.off_load = i0;
+    c1.out_so = M; c1.out_s = 1; c1.out_ib = S1; c1.n_store = M; c1.off_store = 0;
+    c1.in = q; c1.in2 = r; c1.out = p.s1;
+    launch_pass(1, LD_R2, DIR_FWD, MUL_NONE, ST_C, c1, p.stream);
+    // the M columns ky: x lines (stride M), padded, x K^, cropped
+    PassArgs c2 = pass_base(p, p.fpM, p.lcM, B);
+    c2.nl = M; c2.ni = M; c2.in_si = 1; c2.in_s = M; c2.in_ib = S1; c2.n_load = N; c2.off_load = i0;
+    c2.out_si = 1; c2.out_s = M; c2.out_ib = S1; c2.n_store = N; c2.off_store = i0;
+    c2.in = p.s1; c2.out = p.s1; c2.khat = p.khat[0]; c2.kfull = 1;
+    launch_pass(0, LD_C, DIR_FMI, MUL_KHAT2, ST_C, c2, p.stream);
+    last = pass_base(p, p.fpM, p.lcM, B);
+    last.nl = N; last.in_so = M; last.in_s = 1; last.in_ib = S1; last.n_load = M; last.off_load = 0;
+  } else {
+    const long long NM = (long long)N * M, MM = (long long)M * M;
+    // z lines (contiguous), the pair padded -> s1 [x][y][kz], kz in [0, M)
+    PassArgs c1 = pass_base(p, p.fpM, p.lcM, B);
+    c1.nl = N * N; c1.in_so = N; c1.in_s = 1; c1.in_ib = n; c1.n_load = N; c1.off_load = i0;
+    c1.out_so = M; c1.out_s = 1; c1.out_ib = S1; c1.n_store = M; c1.off_store = 0;
+    c1.in = q; c1.in2 = r; c1.out = p.s1;
+    launch_pass(1, LD_R2, DIR_FWD, MUL_NONE, ST_C, c1, p.stream);
+    // y lines (x, kz), padded -> s2 [x][ky][kz]
+    PassArgs c2 = pass_base(p, p.fpM, p.lcM, B);
+    c2.nl = (int)NM; c2.ni = M; c2.in_so = NM; c2.in_si = 1; c2.in_s = M; c2.in_ib = S1; c2.n_load = N;
+    c2.off_load = i0;
+    c2.out_so = MM; c2.out_si = 1; c2.out_s = M; c2.out_ib = S2; c2.n_store = M; c2.off_store = 0;
+    c2.in = p.s1; c2.out = p.s2; c2.kfull = 1;
+    launch_pass(0, LD_C, DIR_FWD, MUL_NONE, ST_C, c2, p.stream);
+    // x lines (ky, kz): padded, x K^, cropped
+    PassArgs c3 = pass_base(p, p.fpM, p.lcM, B);
+    c3.nl = (int)MM; c3.ni = (int)MM; c3.in_si = 1; c3.in_s = MM; c3.in_ib = S2; c3.n_load = N; c3.off_load = i0;
+    c3.out_si = 1; c3.out_s = MM; c3.out_ib = S2; c3.n_store = N; c3.off_store = i0;
+    c3.in = p.s2; c3.out = p.s2; c3.khat = p.khat[0]; c3.kfull = 1;
+    launch_pass(0, LD_C, DIR_FMI, MUL_KHAT3, ST_C, c3, p.stream);
+    // y lines back (x, kz), cropped -> s1
+    PassArgs c4 = pass_base(p, p.fpM, p.lcM, B);
+    c4.nl = (int)NM; c4.ni = M; c4.in_so = MM; c4.in_si = 1; c4.in_s = M; c4.in_ib = S2; c4.n_load = M;
+    c4.off_load = 0;
+    c4.out_so = NM; c4.out_si = 1; c4.out_s = M; c4.out_ib = S1; c4.n_store = N; c4.off_store = i0;
+    c4.in = p.s2; c4.out = p.s1; c4.kfull = 1;
+    launch_pass(0, LD_C, DIR_INV, MUL_NONE, ST_C, c4, p.stream);
+    last = pass_base(p, p.fpM, p.lcM, B);
+    last.nl = N * N; last.in_so = M; last.in_s = 1; last.in_ib = S1; last.n_load = M; last.off_load = 0;
+  }
+  // the contiguous axis back, cropped: K*q = re, K*r = im, with their sums
+  last.out_so = N; last.out_s = 1; last.out_ib = n; last.n_store = N; last.off_store = i0; last.out_scale = fac;
+  last.in = p.s1; last.out = kq; last.out2 = kr; last.dot = q; last.dot2 = r;
+  last.partials = part_of(p, site_qr);
+  last.partials2 = part_of(p, site_rr);
+  nb_rows = launch_pass(1, LD_C, DIR_INV, MUL_NONE, ST_RE2, last, p.stream);
+  return nb_rows;
+}
+
 static KinArgs kin_base(const Plan& p, const Batch& B) {
   KinArgs a;
   memset(&a, 0, sizeof(a));
@@ -615,8 +686,11 @@ extern "C" int bec_plan_create(int ndim, int N, double L, int pad, double s, dou
     if (p.M % 2) bad("padded length must be even");
     p.n = (ndim == 2) ? (long long)N * N : (long long)N * N * N;
     p.nM = (ndim == 2) ? (long long)p.M * p.M : (long long)p.M * p.M * p.M;
-    p.nS1 = (ndim == 2) ? (long long)N * p.H : (long long)N * N * p.H;
-    p.nS2 = (ndim == 2) ? 0 : (long long)N * p.M * p.H;
+    // convolution spectra per instance, sized for a pair convolution (all M
+    // frequencies of the first transformed axis; a single real field's
+    // convolution uses H = M/2 + 1 of them)
+    p.nS1 = (ndim == 2) ? (long long)N * p.M : (long long)N * N * p.M;
+    p.nS2 = (ndim == 2) ? 0 : (long long)N * p.M * p.M;
     p.Lbox = L;
     p.dx = 2.0 * L / N;
     p.dv = (ndim == 2) ? p.dx * p.dx : p.dx * p.dx * p.dx;

@@ -307,8 +307,8 @@ struct DevLoop {
       if (lr_any) {
         Batch BL = dev_batch(p, nb, FL_SLR);
         BL.f32 = 0;                      // the convolutions are FP64 in every phase
-        c.n_kq = run_conv(p, 0, BL, LQ, 1.0, KQ, LQ, LR, C_EGRAV);
-        c.n_kr = run_conv(p, 0, BL, LR, 1.0, KR, LR, nullptr, C_EGRAV2);
+        // K*q and K*r as one complex convolution of q + i r
+        c.n_kq = c.n_kr = run_conv_pair(p, BL, LQ, LR, KQ, KR, C_EGRAV, C_EGRAV2);
       }
       launch_ctrl_line(c, st);
       launch_post(PSI, BEST, D, Z, p.dtab, p.flags, p.n, nb, p.f32, st);

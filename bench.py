@@ -127,10 +127,12 @@ def site_name(sid):
         ld, r = r // 100, r % 100
         dr, mul = r // 10, r % 10
         cdot = mul >= 5 and mul - 5 == 0 and ld == 0 and dr == 1
-        lds = {0: "complex", 1: "real, padded", 2: "half spectrum"}.get(ld, ld)
+        lds = {0: "complex", 1: "real, padded", 2: "half spectrum", 3: "real pair, padded",
+               4: "complex, pair spectrum"}.get(ld, ld)
         drs = {0: "fwd", 1: "inv", 2: "fwd*mul*inv"}.get(dr, dr)
         muls = {0: "", 1: " x K^ (2D)", 2: " x K^ (3D)", 3: " / (k^2/2+sigma) (2D)", 4: " / (k^2/2+sigma) (3D)",
-                5: " + 16 CG inner products" if cdot else " x Gaussian"}.get(mul, f" mul {mul}")
+                5: " + 16 CG inner products" if cdot else " x Gaussian",
+                7: " -> K*q, K*r + 3 sums"}.get(mul, f" mul {mul}")
         return f"k_pass<{'contig' if contig else 'strided'}, {lds}, {drs}{muls}>"
     return f"site {sid}"
 
