@@ -162,6 +162,7 @@ struct RedJobs {
 #define SIGMA_BINS 2048
 #define SIGMA_LDS_CAND 4096
 #define SIGMA_MAX_BLOCKS 256
+#define SIGMA_U 4                    // elements per thread and round of the shift's element passes
 
 struct Plan {
   int ndim, N, pad, M, H;

@@ -9,6 +9,7 @@
 // The work is HBM/LDS-bound FP64 vector arithmetic: there is no MFMA here.
 #include "bec_internal.h"
 #include <hip/hip_ext.h>
+#include <algorithm>
 #include <type_traits>
 
 #define TWO_PI 6.283185307179586
@@ -1054,26 +1055,43 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_local(const C* __restrict__ ps
   const double thr = 0.05 * rm;
   const double b2 = iphys[b].beta2, b3 = iphys[b].beta3;
   unsigned long long kmn = ~0ull, kmx = 0ull, cnt = 0ull;
-  ELEM_LOOP(n) {
-    const double rho = cabs2(psi[off + i]);
-    double val = __longlong_as_double(0x7ff8000000000000ll);    // NaN = outside the bulk
-    if (rho > thr) {
-      int ix, iy, iz = 0;
-      if (ndim == 2) { ix = (int)(i / N); iy = (int)(i % N); }
-      else { ix = (int)(i / ((long long)N * N)); iy = (int)((i / N) % N); iz = (int)(i % N); }
-      const double xx = xcoord(ix, N, dx), yy = xcoord(iy, N, dx);
-      double r2 = xx * xx + yy * yy;
-      if (ndim == 3) { const double zz = xcoord(iz, N, dx); r2 += zz * zz; }
-      double lv = trap_value(r2, tc, s_mode, s_half) + 2.0 * b2 * rho + 3.0 * b3 * (rho * rho);
-      if (phi) lv = lv + phi[off + i];
-      val = lv;
-      const unsigned long long k = order_key(lv);
-      kmn = k < kmn ? k : kmn;
-      kmx = k > kmx ? k : kmx;
-      cnt++;
-      atomicAdd(&h[sigma_bin(lv, blo, bscale)], 1u);
+  // SIGMA_U elements per thread and round, their loads issued first
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; i0 < n; i0 += SIGMA_U * stride) {
+    C pv[SIGMA_U];
+    double fv[SIGMA_U];
+#pragma unroll
+    for (int u = 0; u < SIGMA_U; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        pv[u] = psi[off + i];
+        fv[u] = phi ? phi[off + i] : 0.0;
+      }
     }
-    loc[off + i] = val;
+#pragma unroll
+    for (int u = 0; u < SIGMA_U; ++u) {
+      const long long i = i0 + u * stride;
+      if (i >= n) continue;
+      const double rho = cabs2(pv[u]);
+      double val = __longlong_as_double(0x7ff8000000000000ll);    // NaN = outside the bulk
+      if (rho > thr) {
+        int ix, iy, iz = 0;
+        if (ndim == 2) { ix = (int)(i / N); iy = (int)(i % N); }
+        else { ix = (int)(i / ((long long)N * N)); iy = (int)((i / N) % N); iz = (int)(i % N); }
+        const double xx = xcoord(ix, N, dx), yy = xcoord(iy, N, dx);
+        double r2 = xx * xx + yy * yy;
+        if (ndim == 3) { const double zz = xcoord(iz, N, dx); r2 += zz * zz; }
+        double lv = trap_value(r2, tc, s_mode, s_half) + 2.0 * b2 * rho + 3.0 * b3 * (rho * rho);
+        if (phi) lv = lv + fv[u];
+        val = lv;
+        const unsigned long long k = order_key(lv);
+        kmn = k < kmn ? k : kmn;
+        kmx = k > kmx ? k : kmx;
+        cnt++;
+        atomicAdd(&h[sigma_bin(lv, blo, bscale)], 1u);
+      }
+      loc[off + i] = val;
+    }
   }
   for (int o2 = 32; o2 > 0; o2 >>= 1) {
     const unsigned long long x = __shfl_down(kmn, o2, 64), y = __shfl_down(kmx, o2, 64);
@@ -1184,9 +1202,17 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
     st->rank_in = srank;
   }
   const int bin = sbin;
-  ELEM_LOOP(n) {
-    const double v = loc[off + i];
-    if (v == v && sigma_bin(v, lo, scale) == bin) cand[atomicAdd(&st->ncand, 1u)] = v;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; i0 < n; i0 += SIGMA_U * stride) {
+    double v[SIGMA_U];
+#pragma unroll
+    for (int u = 0; u < SIGMA_U; ++u) {
+      const long long i = i0 + u * stride;
+      v[u] = (i < n) ? loc[off + i] : __longlong_as_double(0x7ff8000000000000ll);
+    }
+#pragma unroll
+    for (int u = 0; u < SIGMA_U; ++u)
+      if (v[u] == v[u] && sigma_bin(v[u], lo, scale) == bin) cand[atomicAdd(&st->ncand, 1u)] = v[u];
   }
 }
 
@@ -1996,11 +2022,21 @@ int launch_moments(const double2* psi, int ndim, int N, double dx, double* parti
   hipLaunchKernelGGL(k_moments, dim3(gr, nb), dim3(BEC_WG), 0, s, psi, ndim, N, dx, partials, hpart);
   return gr;
 }
+// Workgroups per instance of the shift's element passes: one per 4 BEC_WG
+// (local pass) or BEC_WG (gather) elements, capped so that a batch launches
+// about 2048 (local) or 4096 (gather) in all -- each workgroup pays a fixed
+// share (the histogram merge, the rank search over the bins) -- but never
+// fewer than 8 / 16 per instance.
+static int sigma_blocks(long long n, int per, int nb, int cap, int total, int floor_) {
+  long long g = (n + per - 1) / per;
+  const long long lim = std::max<long long>(floor_, total / std::max(nb, 1));
+  g = std::min<long long>(g, std::min<long long>(cap, lim));
+  return (int)std::max<long long>(g, 1);
+}
 void launch_sigma(const Plan& p, int nb, Act act, const double2* psi, const double* phi,
                   const double* mrows, int nrows, int mkp, int mq, double floor_, hipStream_t s, int f32) {
-  long long g1 = (p.n + 4 * BEC_WG - 1) / (4 * BEC_WG);
-  if (g1 > SIGMA_MAX_BLOCKS) g1 = SIGMA_MAX_BLOCKS;
-  const int g = elem_blocks(p.n);
+  const int g1 = sigma_blocks(p.n, 4 * BEC_WG, nb, SIGMA_MAX_BLOCKS, 2048, 8);
+  const int g = sigma_blocks(p.n, BEC_WG, nb, 1024, 4096, 16);
   unsigned long long* part = (unsigned long long*)p.sig_part;
   prof_pre(SITE_SIGMA_LOCAL, s);
   if (f32)
