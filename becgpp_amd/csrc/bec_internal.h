@@ -122,6 +122,9 @@ struct KinArgs {
   double tc, s_half; int s_mode;
   const double* phi;                                     // long-range potential rows x phisc[b]
   double* phi_out;
+  // KH_LAST (polynomial search): instances flagged FL_PHIC first take
+  // phi <- (phi + CQ kq + CR kr) CS (table scalars), written to phi_out
+  const double* kq; const double* kr;
   double t[BEC_BMAX], sc[BEC_BMAX], mu[BEC_BMAX], phisc[BEC_BMAX];
   const double* tab; unsigned tabm;                      // device scalars: bit k of tabm reads DS slot k
   double* partials; double* hpartials;

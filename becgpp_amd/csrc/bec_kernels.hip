@@ -143,7 +143,10 @@ __device__ __forceinline__ double kin_scal(const KinArgs& a, int slot, int b, co
 // the per-instance scalars a kinetic pass uses, read once per workgroup
 struct KinScal {
   double t, sc, mu, phisc, beta2, beta3, rf;
+  bool pc;                             // phi <- (phi + cq kq + cr kr) cs first (KH_LAST)
+  double cq, cr, cs;
 };
+template <bool PC = false>
 __device__ __forceinline__ KinScal kin_scal_all(const KinArgs& a, int b) {
   KinScal k;
   k.t = kin_scal(a, DS_T, b, a.t);
@@ -155,6 +158,10 @@ __device__ __forceinline__ KinScal kin_scal_all(const KinArgs& a, int b) {
   k.beta2 = ip.beta2;
   k.beta3 = ip.beta3;
   k.rf = a.rf_omega ? -ip.Omega : a.rf;
+  k.pc = PC && a.kq && a.act.fl && (a.act.fl[b] & FL_PHIC);
+  k.cq = k.pc ? a.tab[(long long)b * DS_N + DS_CQ] : 0.0;
+  k.cr = k.pc ? a.tab[(long long)b * DS_N + DS_CR] : 0.0;
+  k.cs = k.pc ? a.tab[(long long)b * DS_N + DS_CS] : 1.0;
   return k;
 }
 // The new-direction coefficients of instance b for the first energy pass of
@@ -262,7 +269,9 @@ __device__ __forceinline__ double line_V(const double* c2, int e, const KinArgs&
 // C is the storage of the complex fields: double2, or float2 in an FP32 phase
 // (values are widened to FP64 in registers; the transforms run in C; every
 // sum accumulates in FP64).
-template <int AX, int KIND, class C, bool ODD = false, int LC = 0>
+// PC: the KH_LAST variant that first combines the accepted state's potential
+// (KinArgs::kq; long-range grids of the polynomial search)
+template <int AX, int KIND, class C, bool ODD = false, int LC = 0, bool PC = false>
 __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   extern __shared__ double2 smem_d[];
   C* smem = (C*)smem_d;
@@ -284,8 +293,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
   const bool on0 = act_on(a.act, b0), on1 = (b1 == b0) ? on0 : act_on(a.act, b1);
   if (!on0 && !on1) return;            // lpw <= nl: no instance between b0 and b1
   // a workgroup's lines span instances b0 and b1 only
-  const KinScal S0 = kin_scal_all(a, b0);
-  const KinScal S1 = (b1 == b0) ? S0 : kin_scal_all(a, b1);
+  const KinScal S0 = kin_scal_all<PC>(a, b0);
+  const KinScal S1 = (b1 == b0) ? S0 : kin_scal_all<PC>(a, b1);
   CombScal C0{false, 0.0, 0.0, 0.0, 0.0, 0.0}, C1 = C0;
   if (KIND == KE_FIRST) {
     C0 = comb_scal(a, b0, on0);
@@ -390,6 +399,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
       if (KIND == KH_LAST) {
         if (!a.psi_w) ps[j] = to_d2(fpsi[adr]);
         phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
+        if (OS.pc) phv[j] = (phv[j] + OS.cq * a.kq[adr] + OS.cr * a.kr[adr]) * OS.cs;
         if (a.ydiff) old[j] = to_d2(fout[adr]);
       }
     }
@@ -447,6 +457,7 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
         if (KIND == KH_LAST) {
           ps[j] = to_d2(fpsi[adr]);
           phv[j] = (a.local_on && a.phi) ? a.phi[adr] : 0.0;
+          if (OS.pc) phv[j] = (phv[j] + OS.cq * a.kq[adr] + OS.cr * a.kr[adr]) * OS.cs;
           if (a.ydiff) old[j] = to_d2(fout[adr]);
         }
       }
@@ -482,7 +493,8 @@ __global__ __launch_bounds__(BEC_WG) void k_kin(KinArgs a) {
           double ph = 0.0;
           if (a.phi) {
             ph = phv[j] * OS.phisc;
-            if (a.phi_out) a.phi_out[adr] = ph;
+            if (OS.pc) a.phi_out[adr] = phv[j];          // the accepted state's potential
+            else if (a.phi_out) a.phi_out[adr] = ph;
           }
           W = line_V<AX>(c2, e, a) + OS.beta2 * rho + OS.beta3 * rho * rho + ph;
         }
@@ -1486,6 +1498,7 @@ static double kin_bytes(const KinArgs& a) {
   if (KIND == KH_LAST) {
     if (a.local_on && a.phi) b += r;
     if (a.phi_out) b += r;
+    if (a.kq) b += 2.0 * r;                              // K*q, K*r of the accepted state's potential
     if (a.ydiff) b += 2.0 * c;                           // previous out read, difference written
   }
   return b;
@@ -1632,20 +1645,27 @@ static int go_kin(const KinArgs& a0, hipStream_t s) {
   }
   const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb)));
   const dim3 grid(blocks), block(a.tpl * a.lpw);
-  if (a.f32) {
-    if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, float2>), grid, block, shm, s, e0, e1, 0, a);
-    else hipLaunchKernelGGL((k_kin<AX, KIND, float2>), grid, block, shm, s, a);
-  } else {
-    if (a.fp.odd) {
-      if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, true>), grid, block, shm, s, e0, e1, 0, a);
-      else hipLaunchKernelGGL((k_kin<AX, KIND, double2, true>), grid, block, shm, s, a);
+  auto go = [&](auto pc) {
+    constexpr bool PC = decltype(pc)::value;
+    if (a.f32) {
+      if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, float2, false, 0, PC>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_kin<AX, KIND, float2, false, 0, PC>), grid, block, shm, s, a);
+    } else if (a.fp.odd) {
+      if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, true, 0, PC>), grid, block, shm, s, e0, e1, 0, a);
+      else hipLaunchKernelGGL((k_kin<AX, KIND, double2, true, 0, PC>), grid, block, shm, s, a);
     } else {
       with_lc<false>(a.fp, [&](auto lc) {
         constexpr int LC = decltype(lc)::value;
-        if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, false, LC>), grid, block, shm, s, e0, e1, 0, a);
-        else hipLaunchKernelGGL((k_kin<AX, KIND, double2, false, LC>), grid, block, shm, s, a);
+        if (ev) hipExtLaunchKernelGGL((k_kin<AX, KIND, double2, false, LC, PC>), grid, block, shm, s, e0, e1, 0, a);
+        else hipLaunchKernelGGL((k_kin<AX, KIND, double2, false, LC, PC>), grid, block, shm, s, a);
       });
     }
+  };
+  if constexpr (KIND == KH_LAST) {
+    if (a.kq) go(std::true_type{});
+    else go(std::false_type{});
+  } else {
+    go(std::false_type{});
   }
   return (a.nl + a.lpw - 1) / a.lpw;
 }

@@ -109,6 +109,7 @@ struct HArgs {
   const double2* psi; const double2* d; const double* t; const double* div;
   double2* psi_w; double2* sdiff;
   bool local_on; const double* phi; const double* phisc; double* phi_out;
+  const double* kq; const double* kr;   // last pass: the accepted state's phi combined first (KinArgs::kq)
   const double* mu;
   double kf, rf; int rf_omega;
   double2* out; double2* ydiff;

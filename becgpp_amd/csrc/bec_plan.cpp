@@ -552,6 +552,8 @@ int run_H(Plan& p, const Batch& B, const HArgs& h) {
       if (!h.local_on) a.s_mode = 0;
       a.phi = h.local_on ? h.phi : nullptr;
       a.phi_out = h.phi_out;
+      a.kq = h.kq;
+      a.kr = h.kr;
       for (int b = 0; b < B.nb; ++b) {
         if (h.phisc) a.phisc[b] = h.phisc[b];
         if (h.mu) a.mu[b] = h.mu[b];

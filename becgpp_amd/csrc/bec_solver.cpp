@@ -312,8 +312,12 @@ struct DevLoop {
       }
       launch_ctrl_line(c, st);
       launch_post(PSI, BEST, D, Z, p.dtab, p.flags, p.n, nb, p.f32, st);
-      if (lr_any) launch_phi_comb(phi, KQ, KR, p.dtab, p.flags, p.n, nb, st);
+      // the last H pass first forms the accepted state's potential
+      // phi <- (phi + 2t G_C K*q - t^2 G_C K*r) / n (instances flagged FL_PHIC)
       a.phi = lr_any ? phi : nullptr;
+      a.phi_out = lr_any ? phi : nullptr;
+      a.kq = lr_any ? KQ : nullptr;
+      a.kr = lr_any ? KR : nullptr;
       c.n_hl = run_H(p, dev_batch(p, nb, FL_ACC), a);
       return;
     }
@@ -497,7 +501,10 @@ static void run_loops(const std::vector<DevLoop*>& L) {
         }
         if (still.empty()) break;
         if (still.size() != live.size()) {
+          // the chunk just launched may still run: its executable graph is
+          // destroyed only once every stream has drained
           live.swap(still);
+          drain();
           drop_graphs();
         }
       }
