@@ -98,6 +98,8 @@ struct PassArgs {
   double* partials2;                   // ST_RE2: rows of sum(im dot2), device
   unsigned long long* ucount;          // measurement: algorithmic bytes of the instances covered
   unsigned long long ubytes;           //   (ubytes per instance, added once per instance; nullable)
+  unsigned long long lbytes;           //   and lbytes once per launch that covers an instance (the
+                                       //   kernel spectrum, shared by the instances)
   const double2* dpsi; const double2* dd; const double2* dg; const double2* dy; const double2* ds;  // ST_CDOT
   int f32;                             // complex fields (in/out, ST_CDOT) are float2: an FP32 phase
   int lsh[2];                          // tuned lines-per-workgroup shift: contiguous, strided passes

@@ -692,10 +692,15 @@ __global__ __launch_bounds__(BEC_WG) void k_pass(PassArgs a) {
   if (!any) return;
   auto inst_active = [&](int b) { return b == b0 ? on0 : (b == b1 ? on1 : act_on(a.act, b)); };
   const bool brow = on0;               // this block stores a partial row
-  // measurement: the bytes of each active instance whose first line is here
+  // measurement: the bytes of each active instance whose first line is here,
+  // and the launch's shared bytes with its first active instance
   if (a.ucount && threadIdx.x == 0)
     for (int b = b0; b <= b1; ++b)
-      if ((long long)b * nl >= l0 && inst_active(b)) atomicAdd(a.ucount, a.ubytes);
+      if ((long long)b * nl >= l0 && inst_active(b)) {
+        bool first = true;
+        for (int c = 0; c < b && first && a.lbytes; ++c) first = !act_on(a.act, c);
+        atomicAdd(a.ucount, a.ubytes + (first ? a.lbytes : 0ull));
+      }
   stage_twiddles(twl, a.fp);
   // the line this thread loads, transforms and stores, and its elements lane + j tpl
   const LineOwn ow = line_owner<CONTIG>(tpl, lpw);
@@ -1548,11 +1553,17 @@ static int go_pass(const PassArgs& a0, hipStream_t s) {
                  (ST == ST_CDOT ? 5 : (ST == ST_RE2 ? 7 : 0));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const double bpi = pass_bytes<LD, ST>(a);
+  // the kernel spectrum is read once per launch, whatever the instances
+  const double bpl = (MUL == MUL_KHAT2 || MUL == MUL_KHAT3)
+                         ? 8.0 * (MUL == MUL_KHAT2 ? (double)a.H * a.H : (double)a.H * a.H * a.H)
+                         : 0.0;
   if (a.act.fl) {
     a.ucount = prof_counter(id);
     a.ubytes = (unsigned long long)bpi;
+    a.lbytes = (unsigned long long)bpl;
   }
-  const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * __builtin_popcountll(a.act.m & lowmask(a.nb)));
+  const int nact = __builtin_popcountll(a.act.m & lowmask(a.nb));
+  const bool ev = prof_events(id, &e0, &e1, a.act.fl ? 0.0 : bpi * nact + (nact ? bpl : 0.0));
   const dim3 grid(blocks), block(a.tpl * a.lpw);
   // FP32 phases run the plain transforms and the preconditioner passes only
   constexpr bool kF32 = (LD == LD_C) && ST != ST_RE2 && (MUL == MUL_NONE || MUL == MUL_PRE2 || MUL == MUL_PRE3);
