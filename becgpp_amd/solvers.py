@@ -247,25 +247,36 @@ def ground_state_uneven(cfgs, psi0s=None, grids=None, verbose=False):
                                                    psi0s=[psi0s[k] for k in ks], verbose=verbose)):
                 out[k] = r
         return out
-    # each grid's batches, in order; call r solves the r-th batch of every grid
+    caps = [BATCH_MAX if batchable(grids[ks[0]]) else 1 for ks in groups]
+    for part in uneven_calls(groups, [_solver_key(c) for c in cfgs], caps):
+        got = solve_native_uneven([[cfgs[k] for k in kk] for kk in part], [grids[kk[0]] for kk in part],
+                                  [[psi0s[k] for k in kk] for kk in part], verbose=verbose)
+        for kk, (psis, rs) in zip(part, got):
+            G = grids[kk[0]]
+            for k, p, r_ in zip(kk, psis, rs):
+                out[k] = (p.detach(), G, _finish(p, r_, G, cfgs[k]))
+    return out
+
+
+def uneven_calls(groups, keys, caps):
+    """The native calls of ground_state_uneven: each grid's instances
+    (groups[g]) split into batches of one solver setting (keys[i] of
+    instance i) and at most caps[g] instances, in order; call r takes the r-th
+    batch of every grid that has one (at most GRIDS_MAX grids per call, the
+    rest in further calls).  Returns a list of calls, each a list of batches
+    (lists of instance indices) on distinct grids."""
     queues = []
-    for ks in groups:
-        cap = BATCH_MAX if batchable(grids[ks[0]]) else 1
+    for ks, cap in zip(groups, caps):
         sets = {}
         for k in ks:
-            sets.setdefault(_solver_key(cfgs[k]), []).append(k)
+            sets.setdefault(keys[k], []).append(k)
         queues.append([kk[lo:lo + cap] for kk in sets.values() for lo in range(0, len(kk), cap)])
-    for r in range(max(len(q) for q in queues)):
+    calls = []
+    for r in range(max((len(q) for q in queues), default=0)):
         row = [q[r] for q in queues if r < len(q)]
         for lo in range(0, len(row), GRIDS_MAX):
-            part = row[lo:lo + GRIDS_MAX]
-            got = solve_native_uneven([[cfgs[k] for k in kk] for kk in part], [grids[kk[0]] for kk in part],
-                                      [[psi0s[k] for k in kk] for kk in part], verbose=verbose)
-            for kk, (psis, rs) in zip(part, got):
-                G = grids[kk[0]]
-                for k, p, r_ in zip(kk, psis, rs):
-                    out[k] = (p.detach(), G, _finish(p, r_, G, cfgs[k]))
-    return out
+            calls.append(row[lo:lo + GRIDS_MAX])
+    return calls
 
 
 def _seed_cfgs(cfg, n):

@@ -274,6 +274,30 @@ def test_uneven_schedule_serialises_groups_that_share_a_plan():
     assert not conc and groups == [[0, 1]]
 
 
+def test_uneven_calls_take_one_batch_of_every_grid():
+    """ground_state_uneven's native calls: each grid's instances split by
+    solver setting and batch cap, call r advancing the r-th batch of every
+    grid; every instance solved exactly once; at most eight grids a call."""
+    from becgpp_amd.solvers import GRIDS_MAX, uneven_calls
+    # grid 0: 5 instances, two settings; grid 1: 3 instances (cap 2); grid 2: 1
+    groups = [[0, 1, 2, 3, 4], [5, 6, 7], [8]]
+    keys = ["a", "b", "a", "a", "b", "a", "a", "a", "c"]
+    calls = uneven_calls(groups, keys, [64, 2, 64])
+    assert calls == [[[0, 2, 3], [5, 6], [8]], [[1, 4], [7]]]
+    # a batch never mixes settings or grids, and the cap holds
+    gid = {k: g for g, ks in enumerate(groups) for k in ks}
+    for call in calls:
+        assert len({gid[kk[0]] for kk in call}) == len(call)
+        for kk in call:
+            assert len({keys[k] for k in kk}) == 1 and len({gid[k] for k in kk}) == 1
+    assert sorted(k for call in calls for kk in call for k in kk) == list(range(9))
+    # more grids than one call takes
+    many = [[i] for i in range(GRIDS_MAX + 3)]
+    calls = uneven_calls(many, ["a"] * len(many), [64] * len(many))
+    assert [len(c) for c in calls] == [GRIDS_MAX, 3]
+    assert uneven_calls([], [], []) == []
+
+
 def test_batches_split_by_solver_settings():
     """A batched loop shares its solver settings: configurations that differ
     in any solver key get different batch keys, the physics does not."""
