@@ -115,7 +115,7 @@ __device__ __forceinline__ void inst_store(DevInst* dst, const DevInst* src) {
 // ---------------------------------------------------------------------------
 // thread 0 of k_ctrl_dir: the scalar logic on the LDS copy of the record
 __device__ __forceinline__ void ctrl_dir_scalar(const CtrlArgs& c, int b, unsigned f, bool dir, DevInst* q,
-                                             const double* r, const double* hl) {
+                                             const double* r, const double* hl, double sigma) {
   if (!dir) {                         // retrying a trial, or finished
     c.fl[b] = (f & FL_SEARCH) ? (FL_SEARCH | (q->lr ? FL_SLR : 0u)) : 0u;
     return;
@@ -166,7 +166,7 @@ __device__ __forceinline__ void ctrl_dir_scalar(const CtrlArgs& c, int b, unsign
   }
   const int it = q->it;
   q->n_iter = it + 1;
-  q->sigma = c.sigma[b];
+  q->sigma = sigma;
   // <psi,z>, <psi,psi>, <psi,d>, <psi,g>, Re<z,g>, Re<z,y>, <psi,y>, Re<g,d>, <g,g>, Re<g,y>, Re<s,y>, <s,s>
   const double czr = r[0] * dv, czi = r[1] * dv;
   const double pp = r[2] * dv;
@@ -305,9 +305,8 @@ __device__ __forceinline__ void ctrl_armijo_scalar(const CtrlArgs& c, int b, uns
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void ctrl_line_scalar(const CtrlArgs& c, int b, unsigned f, DevInst* q, const double* e0,
                                                  const double* e1, const double* e2, const double* kq,
-                                                 const double* kr) {
+                                                 const double* kr, const InstPhys& ph) {
   const double dv = c.dv;
-  const InstPhys ph = c.iphys[b];
   double kin = e0[0] + e1[0], lz = e0[1] + e1[1];
   if (c.ndim == 3) { kin += e2[0]; lz += e2[1]; }
   const double a1 = e0[2] * dv, a2 = e0[3] * dv;
@@ -403,27 +402,29 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_line(CtrlArgs c) {
   __shared__ double e0[32], e1[2], e2[2], kq[2], kr[1];
   __shared__ DevInst qs;
   const int b = blockIdx.x;
+  // every load is issued before any is waited for: the flag word, the rows
+  // (the long-range ones too, used only when the flag asks for them), the
+  // instance record and its physics
   const unsigned f = c.fl[b];
-  if (!(f & FL_SEARCH)) {             // finished this step: only a pending best copy remains
-    if (threadIdx.x == 0) c.fl[b] = f & FL_BEST;
-    return;
-  }
-  const bool lr = (f & FL_SLR) != 0u;
   const double* r0 = c.r_e0 + (long long)b * c.n_e0 * 32;
   const double* r1 = c.r_e1 + (long long)b * c.n_e1 * 2;
   const double* r2 = c.ndim == 3 ? c.r_e2 + (long long)b * c.n_e2 * 2 : nullptr;
-  const double* rq = lr ? c.r_kq + (long long)b * c.n_kq * 2 : nullptr;
-  const double* rr = lr ? c.r_kr + (long long)b * c.n_kr : nullptr;
+  const double* rq = c.r_kq + (long long)b * c.n_kq * 2;
+  const double* rr = c.r_kr + (long long)b * c.n_kr;
   RowFold<32, 32> f0;
   RowFold<2, 2> f1, f2, fq;
   RowFold<1, 2> fr;
   f0.issue(r0, c.n_e0);
   f1.issue(r1, c.n_e1);
   if (r2) f2.issue(r2, c.n_e2);
-  if (lr) {
-    fq.issue(rq, c.n_kq);
-    fr.issue(rr, c.n_kr);
+  fq.issue(rq, c.n_kq);
+  fr.issue(rr, c.n_kr);
+  const InstPhys ph = c.iphys[b];
+  if (!(f & FL_SEARCH)) {             // finished this step: only a pending best copy remains
+    if (threadIdx.x == 0) c.fl[b] = f & FL_BEST;
+    return;
   }
+  const bool lr = (f & FL_SLR) != 0u;
   inst_load(&qs, c.st + b);
   f0.finish(r0, c.n_e0, 21, e0, sh);
   f1.finish(r1, c.n_e1, 2, e1, sh);
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_line(CtrlArgs c) {
     fq.finish(rq, c.n_kq, 2, kq, sh);
     fr.finish(rr, c.n_kr, 1, kr, sh);
   }
-  if (threadIdx.x == 0) ctrl_line_scalar(c, b, f, &qs, e0, e1, e2, kq, kr);
+  if (threadIdx.x == 0) ctrl_line_scalar(c, b, f, &qs, e0, e1, e2, kq, kr, ph);
   __syncthreads();
   inst_store(c.st + b, &qs);
 }
@@ -444,24 +445,24 @@ __global__ __launch_bounds__(CTRL_WG) void k_ctrl_dir(CtrlArgs c) {
   const int b = blockIdx.x;
   DevInst* q = &qs;
   const unsigned f = c.fl[b];
-  const bool dir = (f & FL_DIR) != 0u;
   // the rows of the accepted state's H pass are loaded with the inner
-  // products and used when the record says that a step is pending
+  // products and used when the record says that a step is pending; all
+  // loads (flags, rows, record, shift) are issued before any is waited for
   const double* rd = c.r_dots + (long long)b * c.n_dots * 16;
   const double* rh = c.r_hl + (long long)b * c.n_hl * 4;
   RowFold<16, 16> fd;
   RowFold<4, 4> fh;
-  if (dir) {
-    fd.issue(rd, c.n_dots);
-    fh.issue(rh, c.n_hl);
-  }
+  fd.issue(rd, c.n_dots);
+  fh.issue(rh, c.n_hl);
+  const double sig = c.sigma[b];
   inst_load(q, c.st + b);
+  const bool dir = (f & FL_DIR) != 0u;
   __syncthreads();
   if (dir) {
     fd.finish(rd, c.n_dots, 16, r, sh);
     if (q->pending) fh.finish(rh, c.n_hl, 2, hl, sh);
   }
-  if (threadIdx.x == 0) ctrl_dir_scalar(c, b, f, dir, q, r, hl);
+  if (threadIdx.x == 0) ctrl_dir_scalar(c, b, f, dir, q, r, hl, sig);
   __syncthreads();
   inst_store(c.st + b, q);
 }
