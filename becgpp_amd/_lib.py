@@ -104,11 +104,13 @@ SIGNATURES = {
     "bec_ground_state_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Phys),
                                               ctypes.POINTER(SolverCfg), ctypes.c_void_p,
                                               ctypes.POINTER(SolveOut)]),
+    "bec_ground_state_queue": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Phys),
+                                              ctypes.POINTER(SolverCfg), ctypes.c_void_p, ctypes.POINTER(SolveOut)]),
     "bec_ground_state_uneven": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Phys),
                                                ctypes.POINTER(ctypes.POINTER(SolverCfg)),
                                                ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(SolveOut)]),
-    "bec_plan_reserve":(ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "bec_plan_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "bec_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p, c_double_p, ctypes.c_char_p,
                                          ctypes.c_int]),
     "bec_prof_watch": (ctypes.c_int, [ctypes.c_int]),

@@ -291,7 +291,8 @@ int launch_kin(int ax, int kind, const KinArgs& a, hipStream_t s);
 int elem_blocks(long long n);
 int launch_norm2(const double2* psi, long long n, int nb, unsigned long long act, double* hpart, hipStream_t s);
 int reduce_blocks(long long n);      // workgroups per instance of the reducing elementwise kernels
-void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s);
+void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s,
+                  unsigned long long act = ~0ull);   // the instances of act only
 void launch_combine(double2* d, const double2* z, const double2* g, const double2* psi, const CombArgs& c,
                     const double* tab, long long n, int nb, Act act, hipStream_t s);
 int launch_dots(const double2* z, const double2* psi, const double2* d, const double2* g, const double2* y,

@@ -885,9 +885,11 @@ __global__ __launch_bounds__(BEC_WG) void k_norm2(const double2* __restrict__ ps
   block_partials<1, 0>(v, nullptr, nullptr, hpart, (long long)b * gridDim.x + blockIdx.x);
 }
 
-__global__ __launch_bounds__(BEC_WG) void k_scale(const double2* __restrict__ in, double2* out, DblArr div,
-                                                  long long n) {
+// out = in / div[b] for the instances of act (in place when in == out)
+__global__ __launch_bounds__(BEC_WG) void k_scale(const double2* in, double2* out, DblArr div, long long n,
+                                                  unsigned long long act) {
   const int b = blockIdx.y;
+  if (!((act >> b) & 1ull)) return;
   const double sc = div.v[b];
   ELEM_LOOP(n) {
     const double2 v = in[(long long)b * n + i];
@@ -1773,10 +1775,11 @@ int launch_norm2(const double2* psi, long long n, int nb, unsigned long long act
   hipLaunchKernelGGL(k_norm2, dim3(g, nb), dim3(BEC_WG), 0, s, psi, n, act, hpart);
   return g;
 }
-void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s) {
+void launch_scale(const double2* in, double2* out, const double* divs, long long n, int nb, hipStream_t s,
+                  unsigned long long act) {
   DblArr d;
   for (int b = 0; b < BEC_BMAX; ++b) d.v[b] = b < nb ? divs[b] : 1.0;
-  hipLaunchKernelGGL(k_scale, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, in, out, d, n);
+  hipLaunchKernelGGL(k_scale, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, in, out, d, n, act);
 }
 void launch_rho(const double2* psi, double* rho, long long n, int nb, hipStream_t s) {
   hipLaunchKernelGGL(k_rho, dim3(elem_blocks(n), nb), dim3(BEC_WG), 0, s, psi, rho, n);

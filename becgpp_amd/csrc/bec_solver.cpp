@@ -14,6 +14,7 @@
 #include <string.h>
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <vector>
 
 #include "../../include/becgpp_hip.h"
@@ -121,6 +122,7 @@ struct Inst {
 };
 
 static Batch mask_batch(int nb, unsigned long long m) { return Batch{nb, m}; }
+static unsigned long long low_bits(int nb) { return nb >= 64 ? ~0ull : ((1ull << nb) - 1ull); }
 
 // Prints the progress records the controllers wrote since the last call.
 static void vlog_flush(const Plan& p, int nb, std::vector<int>& printed) {
@@ -159,6 +161,12 @@ struct DevLoop {
   std::vector<int> printed;
   double floor_ = 0.0;
   long long max_steps = 0;
+  // a queue of more solves than slots: whether stopped slots should be
+  // refilled now, the refill itself (called with every stream drained), and
+  // whether solves are still waiting for a slot
+  std::function<bool()> want_refill;
+  std::function<void()> refill;
+  std::function<bool()> pending;
   double2 *PSI = nullptr, *SD = nullptr, *G = nullptr, *Y = nullptr, *D = nullptr, *Z = nullptr, *BEST = nullptr;
   double *phi = nullptr, *phit = nullptr, *LQ = nullptr, *LR = nullptr, *KQ = nullptr, *KR = nullptr;
 
@@ -193,31 +201,8 @@ struct DevLoop {
     KR = p.stphi[5];
     dev_alloc(p, nb, trace ? trace_cap : 0);
     di.assign(nb, DevInst{});
-    std::vector<unsigned> fl0(nb, FL_DIR);
-    for (int b = 0; b < nb; ++b) {
-      const Inst& q = I_[b];
-      memset(&di[b], 0, sizeof(DevInst));
-      di[b].E = q.E;
-      di[b].mu = q.mu;
-      di[b].E_ref = q.E_ref;
-      di[b].resid_rel = q.resid_rel;
-      di[b].best_resid = q.best_resid;
-      di[b].alpha = q.alpha;
-      di[b].sigma = q.sigma;
-      di[b].stop = BEC_STOP_MAXIT;
-      di[b].lr = q.lr ? 1 : 0;
-      di[b].n_energy = q.n_energy;
-      di[b].L00 = q.L00;
-      di[b].Spp = q.Spp;
-      di[b].Sppp = q.Sppp;
-      di[b].Kpp = q.Kpp;
-      di[b].n_tr = n_tr_;
-      p.hdone[b] = 0;
-    }
-    // copies on the caller's stream, never the legacy one
-    HIPCHK(hipMemcpyAsync(p.dinst, di.data(), sizeof(DevInst) * nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(p.flags, fl0.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    printed.assign(nb, 0);
+    upload(low_bits(nb), s);
 
     memset(&c, 0, sizeof(c));
     c.nb = nb;
@@ -264,9 +249,6 @@ struct DevLoop {
     c.clk_hz = 1e3 * (double)std::max(khz, 1);
     c.hdone = p.hdone_dev;
     c.vlog = cf->verbose ? p.hvlog_dev : nullptr;
-    printed.assign(nb, 0);
-    if (cf->verbose)
-      for (int b = 0; b < nb; ++b) p.hvlog[(size_t)BEC_BMAX * VLOG_CAP * VLOG_REC + b] = 0.0;
     floor_ = cf->precond_shift_min;
     // every step resolves a trial of each running instance: maxit iterations of
     // at most linesearch_max trials bound the number of steps
@@ -342,11 +324,74 @@ struct DevLoop {
     c.n_hl = run_H_accept_dev(p, dev_batch(p, nb, FL_ACC), a);
   }
 
-  // every instance has stopped (the controllers' host-mapped flags)
-  bool finished() const {
+  // The slots of mask start their instances: each record from I[] (the
+  // scalar state slots_begin set up), flags FL_DIR, done flag cleared.  On
+  // stream st, synchronised.
+  void upload(unsigned long long mask, hipStream_t st) {
+    Plan& p = *pp;
+    const unsigned fdir = FL_DIR;
+    for (int b = 0; b < nb; ++b) {
+      if (!((mask >> b) & 1ull)) continue;
+      const Inst& q = (*I)[b];
+      memset(&di[b], 0, sizeof(DevInst));
+      di[b].E = q.E;
+      di[b].mu = q.mu;
+      di[b].E_ref = q.E_ref;
+      di[b].resid_rel = q.resid_rel;
+      di[b].best_resid = q.best_resid;
+      di[b].alpha = q.alpha;
+      di[b].sigma = q.sigma;
+      di[b].stop = BEC_STOP_MAXIT;
+      di[b].lr = q.lr ? 1 : 0;
+      di[b].n_energy = q.n_energy;
+      di[b].L00 = q.L00;
+      di[b].Spp = q.Spp;
+      di[b].Sppp = q.Sppp;
+      di[b].Kpp = q.Kpp;
+      di[b].n_tr = *n_tr;
+      p.hdone[b] = 0;
+      printed[b] = 0;
+      if (cf->verbose) p.hvlog[(size_t)BEC_BMAX * VLOG_CAP * VLOG_REC + b] = 0.0;
+      // copies on the given stream, never the legacy one
+      HIPCHK(hipMemcpyAsync((DevInst*)p.dinst + b, &di[b], sizeof(DevInst), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(p.flags + b, &fdir, sizeof(unsigned), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // the slots whose instances have stopped (the controllers' host-mapped flags)
+  unsigned long long done_mask() const {
+    unsigned long long m = 0;
     for (int b = 0; b < nb; ++b)
-      if (((volatile int*)pp->hdone)[b] == 0) return false;
-    return true;
+      if (((volatile int*)pp->hdone)[b] != 0) m |= 1ull << b;
+    return m;
+  }
+
+  // every instance has stopped and no solve waits for a slot
+  bool finished() const {
+    return done_mask() == low_bits(nb) && !(pending && pending());
+  }
+
+  // The final state of the slots of mask read back into I[] (on stream st).
+  void harvest(unsigned long long mask, hipStream_t st) {
+    Plan& p = *pp;
+    for (int b = 0; b < nb; ++b)
+      if ((mask >> b) & 1ull)
+        HIPCHK(hipMemcpyAsync(&di[b], (DevInst*)p.dinst + b, sizeof(DevInst), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int b = 0; b < nb; ++b) {
+      if (!((mask >> b) & 1ull)) continue;
+      Inst& q = (*I)[b];
+      q.E = di[b].E;
+      q.resid_rel = di[b].resid_rel;
+      q.best_resid = di[b].best_resid;
+      q.n_iter = di[b].n_iter;
+      q.converged = di[b].converged != 0;
+      q.stop = di[b].stop;
+      q.sigma = di[b].sigma;
+      q.n_energy = di[b].n_energy;
+      q.done = true;
+    }
   }
 
   void log() {
@@ -497,6 +542,12 @@ static void run_loops(const std::vector<DevLoop*>& L) {
         std::vector<int> still;
         for (int i : live) {
           L[i]->log();
+          if (L[i]->want_refill && L[i]->want_refill()) {
+            // stopped slots take waiting solves: every stream drained first
+            // (the graph stays valid: the same launches over the same slots)
+            drain();
+            L[i]->refill();
+          }
           if (!L[i]->finished()) still.push_back(i);
         }
         if (still.empty()) break;
@@ -518,30 +569,36 @@ static void run_loops(const std::vector<DevLoop*>& L) {
   cleanup();
 }
 
-// One plan's share of a solve phase: its instances and their scalar state,
-// set up by phase_begin (normalised psi, initial energy and residual) and
-// written out by phase_end.
+// One plan's share of a solve phase: its slots (instances in the loop) and
+// their scalar state.  Slot b solves from the state at src[b] with physics
+// ph[b] and returns its state to dst[b] and its results to out[b];
+// slots_begin sets slots up (normalised psi, initial energy and residual),
+// slots_end writes them out.  A plain batch fills every slot once; a queue
+// (more solves than slots) refills the slots whose solves have stopped.
 struct Phase {
   Plan* pp = nullptr;
   int nb = 0;
-  const bec_phys* php = nullptr;
+  const bec_phys* php = nullptr;       // plain batch: physics of slot b at php[b]
   const bec_solver_cfg* cf = nullptr;
-  double2* psi_io = nullptr;
-  bec_solve_out* out = nullptr;
+  double2* psi_io = nullptr;           // plain batch: states [nb][n], in and out
+  bec_solve_out* out = nullptr;        // plain batch: results [nb]
   double* trace = nullptr;
   int trace_cap = 0;
   bool f32 = false;
   std::vector<Phys> ph;
   std::vector<Inst> I;
+  std::vector<const double2*> src;
+  std::vector<double2*> dst;
+  std::vector<bec_solve_out*> outp;
+  std::vector<double> t0;
   unsigned long long all = 0, lrmask = 0;
   bool lr_any = false, lp = false;
   int n_tr = 0;
-  double t0 = 0.0;
 };
 
-// psi = psi0 / norm_of(psi0), the initial energy and residual (solvers.py:56-112);
-// f32: the state is narrowed to complex64 for an FP32 phase.
-static void phase_begin(Phase& f) {
+// Checks and sizes a phase of nb slots; lr_any: some solve of the phase has a
+// long-range kernel (the loop's steps then carry the convolutions).
+static void phase_init(Phase& f, bool lr_any) {
   Plan& p = *f.pp;
   const int nb = f.nb;
   const bec_solver_cfg* cf = f.cf;
@@ -550,11 +607,36 @@ static void phase_begin(Phase& f) {
   if (nb > 1 && !plan_batchable(p))
     throw BecError(1, "batched solves need each instance's lines to fill whole workgroups, which this grid (N = " +
                           std::to_string(p.N) + ") does not: solve its instances one at a time");
-  f.ph.resize(nb);
-  for (int b = 0; b < nb; ++b) f.ph[b] = Phys{f.php[b].beta2, f.php[b].beta3, f.php[b].Omega, f.php[b].G_C};
+  if (lr_any && !p.khat[0]) throw BecError(1, "kernel spectrum not built for this plan");
   plan_reserve(p, nb);
   solver_alloc(p, nb);
-  plan_set_phys(p, f.ph.data(), nb);
+  f.all = low_bits(nb);
+  f.ph.assign(nb, Phys{0.0, 0.0, 0.0, 0.0});
+  f.I.assign(nb, Inst{});
+  f.src.assign(nb, nullptr);
+  f.dst.assign(nb, nullptr);
+  f.outp.assign(nb, nullptr);
+  f.t0.assign(nb, 0.0);
+  f.lrmask = 0;
+  f.lr_any = lr_any;
+  f.n_tr = 0;
+  // the polynomial line search (one step per iteration) unless the trial-per-step
+  // search is asked for or the loop runs on the host
+  f.lp = cf->line_poly != 0 && !cf->host_loop;
+  if (f.f32) {
+    if (lr_any && !f.lp) throw BecError(1, "internal: FP32 phase with a long-range kernel needs the polynomial search");
+    solver_alloc32(p);
+  }
+}
+
+// Sets up the slots of mask (ph, src, dst, outp filled by the caller):
+// psi = psi0 / norm_of(psi0), the initial energy and residual
+// (solvers.py:56-112); f32: their states are narrowed to complex64 for an
+// FP32 phase.  The other slots' fields are untouched.
+static void slots_begin(Phase& f, unsigned long long mask) {
+  Plan& p = *f.pp;
+  const int nb = f.nb;
+  const bec_solver_cfg* cf = f.cf;
   const double dv = p.dv;
   const long long n = p.n;
   double2* PSI = p.st[S_PSI];
@@ -567,45 +649,63 @@ static void phase_begin(Phase& f) {
   double* phit = p.stphi[1];    // unscaled convolution of the latest trial
   hipStream_t s = p.stream;
   const double floor_ = cf->precond_shift_min;
-  f.all = (nb == 64) ? ~0ull : ((1ull << nb) - 1ull);
-  const unsigned long long all = f.all;
-
-  f.I.assign(nb, Inst{});
-  f.lrmask = 0;
-  for (int b = 0; b < nb; ++b) {
+  const bool whole = mask == f.all;
+  auto slots = [&](auto fn) {
+    for (int b = 0; b < nb; ++b)
+      if ((mask >> b) & 1ull) fn(b);
+  };
+  plan_set_phys(p, f.ph.data(), nb);
+  slots([&](int b) {
     Inst& q = f.I[b];
     memset(&q, 0, sizeof(Inst));
     q.lr = plan_long_range(p, f.ph[b]);
     if (q.lr) f.lrmask |= 1ull << b;
+    else f.lrmask &= ~(1ull << b);
     q.alpha = cf->step;
     q.sigma = floor_;
     q.stop = BEC_STOP_MAXIT;
-  }
-  if (f.lrmask && !p.khat[0]) throw BecError(1, "kernel spectrum not built for this plan");
-  f.lr_any = f.lrmask != 0;
+  });
+  if ((f.lrmask & mask) && !f.lr_any) throw BecError(2, "internal: a long-range slot in a phase without long range");
   const bool lr_any = f.lr_any;
-  f.n_tr = 0;
 
   // psi = psi0 / norm_of(psi0)                                        (solvers.py:56-58)
-  add_job(p, C_NORM, launch_norm2(f.psi_io, n, nb, all, hpart_of(p, C_NORM), s), 1, 0, R_NORM, Batch{nb, all});
+  // the initial states in place [nb][n] when they are, else staged in SD
+  bool in_place = true;
+  slots([&](int b) { in_place = in_place && f.src[b] == f.psi_io + (long long)b * n; });
+  const double2* in = f.psi_io;
+  if (!in_place) {
+    slots([&](int b) {
+      HIPCHK(hipMemcpyAsync(SD + (long long)b * n, f.src[b], sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    });
+    in = SD;
+  }
+  add_job(p, C_NORM, launch_norm2(in, n, nb, mask, hpart_of(p, C_NORM), s), 1, 0, R_NORM, Batch{nb, mask});
   plan_sync(p);
-  std::vector<double> nrm0(nb);
-  for (int b = 0; b < nb; ++b) nrm0[b] = sqrt(hres(p, b)[R_NORM] * dv);
-  launch_scale(f.psi_io, PSI, nrm0.data(), n, nb, s);
-  HIPCHK(hipMemsetAsync(SD, 0, sizeof(double2) * n * nb, s));
-  HIPCHK(hipMemsetAsync(Y, 0, sizeof(double2) * n * nb, s));
-  HIPCHK(hipMemsetAsync(D, 0, sizeof(double2) * n * nb, s));
-  HIPCHK(hipMemsetAsync(phi, 0, sizeof(double) * n * nb, s));
-  HIPCHK(hipMemsetAsync(phit, 0, sizeof(double) * n * nb, s));
+  std::vector<double> nrm0(nb, 1.0);
+  slots([&](int b) { nrm0[b] = sqrt(hres(p, b)[R_NORM] * dv); });
+  launch_scale(in, PSI, nrm0.data(), n, nb, s, mask);
+  auto zero = [&](void* base, size_t el) {
+    if (whole) {
+      HIPCHK(hipMemsetAsync(base, 0, el * n * nb, s));
+    } else {
+      slots([&](int b) { HIPCHK(hipMemsetAsync((char*)base + el * n * b, 0, el * n, s)); });
+    }
+  };
+  zero(SD, sizeof(double2));
+  zero(Y, sizeof(double2));
+  zero(D, sizeof(double2));
+  zero(phi, sizeof(double));
+  zero(phit, sizeof(double));
 
   // E, phi = energy_phi(psi)                                            (solvers.py:106)
-  const Batch ALL{nb, all};
-  run_energy_sums(p, ALL, PSI, nullptr, nullptr, lr_any ? p.rho : nullptr);
-  if (lr_any) run_conv(p, 0, mask_batch(nb, f.lrmask), p.rho, 1.0, phi, p.rho);
+  const Batch SL{nb, mask};
+  run_energy_sums(p, SL, PSI, nullptr, nullptr, lr_any ? p.rho : nullptr);
+  if (f.lrmask & mask) run_conv(p, 0, mask_batch(nb, f.lrmask & mask), p.rho, 1.0, phi, p.rho);
   HIPCHK(hipGetLastError());
   plan_sync(p);
-  std::vector<double> mu_arr(nb), phisc(nb);
-  for (int b = 0; b < nb; ++b) {
+  std::vector<double> mu_arr(nb, 0.0), phisc(nb, 0.0);
+  const double now = now_s();
+  slots([&](int b) {
     Inst& q = f.I[b];
     const Energy e0 = energy_from_sums(p, b, f.ph[b], q.lr, false, -f.ph[b].G_C);
     const double* rs = hres(p, b);
@@ -619,8 +719,8 @@ static void phase_begin(Phase& f) {
     q.n_energy = 1;
     mu_arr[b] = q.mu;
     phisc[b] = q.lr ? -f.ph[b].G_C : 0.0;
-  }
-  f.t0 = now_s();
+    f.t0[b] = now;
+  });
   if (f.trace && f.trace_cap > 0 && nb == 1) {
     f.trace[0] = 0.0; f.trace[1] = 0.0; f.trace[2] = f.I[0].E; f.trace[3] = NAN;
     f.n_tr = 1;
@@ -639,31 +739,55 @@ static void phase_begin(Phase& f) {
     a.rf_omega = 1;
     a.out = G;
     a.reduce = true;
-    run_H(p, ALL, a);
+    run_H(p, SL, a);
   }
   HIPCHK(hipGetLastError());
   plan_sync(p);
-  for (int b = 0; b < nb; ++b) {
+  slots([&](int b) {
     f.I[b].resid_rel = sqrt(hres(p, b)[R_HLAST] * dv) / std::max(1.0, fabs(f.I[b].mu));
     f.I[b].best_resid = f.I[b].resid_rel;
-  }
-  launch_copy(PSI, BEST, n, nb, mask_act(all), s);
+  });
+  launch_copy(PSI, BEST, n, nb, mask_act(mask), s);
 
-  // the polynomial line search (one step per iteration) unless the trial-per-step
-  // search is asked for or the loop runs on the host
-  f.lp = cf->line_poly != 0 && !cf->host_loop;
   if (f.f32) {
-    if (lr_any && !f.lp) throw BecError(1, "internal: FP32 phase with a long-range kernel needs the polynomial search");
-    solver_alloc32(p);
-    const long long nn = n * nb;
-    launch_narrow(PSI, p.st32[S_PSI], nn, s);
-    launch_narrow(G, p.st32[S_G], nn, s);
-    launch_narrow(BEST, p.st32[S_BEST], nn, s);
-    HIPCHK(hipMemsetAsync(p.st32[S_SD], 0, sizeof(float2) * nn, s));
-    HIPCHK(hipMemsetAsync(p.st32[S_Y], 0, sizeof(float2) * nn, s));
-    HIPCHK(hipMemsetAsync(p.st32[S_D], 0, sizeof(float2) * nn, s));
+    auto narrow = [&](int i) {
+      if (whole) {
+        launch_narrow(p.st[i], p.st32[i], n * nb, s);
+      } else {
+        slots([&](int b) { launch_narrow(p.st[i] + (long long)b * n, p.st32[i] + (long long)b * n, n, s); });
+      }
+    };
+    narrow(S_PSI);
+    narrow(S_G);
+    narrow(S_BEST);
+    auto zero32 = [&](int i) {
+      if (whole) {
+        HIPCHK(hipMemsetAsync(p.st32[i], 0, sizeof(float2) * n * nb, s));
+      } else {
+        slots([&](int b) { HIPCHK(hipMemsetAsync(p.st32[i] + (long long)b * n, 0, sizeof(float2) * n, s)); });
+      }
+    };
+    zero32(S_SD);
+    zero32(S_Y);
+    zero32(S_D);
     HIPCHK(hipGetLastError());
   }
+}
+
+// A plain batch: every slot once, states in place at psi_io ([nb][n]).
+static void phase_begin(Phase& f) {
+  const long long n = f.pp->n;
+  bool lr = false;
+  for (int b = 0; b < f.nb; ++b)
+    lr = lr || plan_long_range(*f.pp, Phys{f.php[b].beta2, f.php[b].beta3, f.php[b].Omega, f.php[b].G_C});
+  phase_init(f, lr);
+  for (int b = 0; b < f.nb; ++b) {
+    f.ph[b] = Phys{f.php[b].beta2, f.php[b].beta3, f.php[b].Omega, f.php[b].G_C};
+    f.src[b] = f.psi_io + (long long)b * n;
+    f.dst[b] = f.psi_io + (long long)b * n;
+    f.outp[b] = f.out + b;
+  }
+  slots_begin(f, f.all);
 }
 
 // The same loop with its scalar logic on the host (cfg host_loop): one
@@ -703,7 +827,7 @@ static void host_loop(Phase& f) {
   double* trace = f.trace;
   const int trace_cap = f.trace_cap;
   int& n_tr = f.n_tr;
-  const double t0 = f.t0;
+  const double t0 = f.t0[0];
   std::vector<double> mu_arr(nb), phisc(nb), tarr(nb), divarr(nb);
 
 int it = 0;
@@ -924,31 +1048,48 @@ while (true) {
 }
 }
 
-// The returned iterate and the outputs (solvers.py:225-226); an FP32 phase's
-// states are widened back first.
-static void phase_end(Phase& f) {
+// The slots of mask written out: the returned iterate (solvers.py:225-226)
+// to dst[b] and the results to outp[b]; an FP32 phase's states are widened
+// back first.
+static void slots_end(Phase& f, unsigned long long mask) {
   Plan& p = *f.pp;
   const int nb = f.nb;
   const long long n = p.n;
   hipStream_t s = p.stream;
   double2* PSI = p.st[S_PSI];
   double2* BEST = p.st[S_BEST];
+  const bool whole = mask == f.all;
+  auto slots = [&](auto fn) {
+    for (int b = 0; b < nb; ++b)
+      if ((mask >> b) & 1ull) fn(b);
+  };
   if (f.f32) {
-    const long long nn = n * nb;
-    launch_widen(p.st32[S_PSI], PSI, nn, s);
-    launch_widen(p.st32[S_BEST], BEST, nn, s);
+    for (int i : {S_PSI, S_BEST}) {
+      if (whole) launch_widen(p.st32[i], p.st[i], n * nb, s);
+      else slots([&](int b) { launch_widen(p.st32[i] + (long long)b * n, p.st[i] + (long long)b * n, n, s); });
+    }
     HIPCHK(hipGetLastError());
   }
   // return the lowest-residual iterate seen                             (solvers.py:225-226)
   unsigned long long from_best = 0;
-  for (int b = 0; b < nb; ++b)
+  slots([&](int b) {
     if (f.I[b].best_resid < f.I[b].resid_rel) from_best |= 1ull << b;
-  launch_copy(BEST, f.psi_io, n, nb, mask_act(from_best), s);
-  launch_copy(PSI, f.psi_io, n, nb, mask_act(f.all & ~from_best), s);
+  });
+  bool in_place = true;
+  slots([&](int b) { in_place = in_place && f.dst[b] == f.psi_io + (long long)b * n; });
+  if (in_place) {
+    launch_copy(BEST, f.psi_io, n, nb, mask_act(from_best), s);
+    launch_copy(PSI, f.psi_io, n, nb, mask_act(mask & ~from_best), s);
+  } else {
+    slots([&](int b) {
+      const double2* from = (((from_best >> b) & 1ull) ? BEST : PSI) + (long long)b * n;
+      HIPCHK(hipMemcpyAsync(f.dst[b], from, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    });
+  }
   plan_sync(p);
-  const double wall = now_s() - f.t0;
-  for (int b = 0; b < nb; ++b) {
-    bec_solve_out& o = f.out[b];
+  const double now = now_s();
+  slots([&](int b) {
+    bec_solve_out& o = *f.outp[b];
     o.iters = f.I[b].n_iter;
     o.converged = f.I[b].converged ? 1 : 0;
     o.stop_reason = f.I[b].stop;
@@ -957,10 +1098,12 @@ static void phase_end(Phase& f) {
     o.resid_rel = f.I[b].resid_rel;
     o.best_resid = f.I[b].best_resid;
     o.sigma = f.I[b].sigma;
-    o.walltime = wall;
+    o.walltime = now - f.t0[b];
     o.n_energy = f.I[b].n_energy;
-  }
+  });
 }
+
+static void phase_end(Phase& f) { slots_end(f, f.all); }
 
 // One phase of the solve over the plans of F (distinct): each plan's set-up,
 // then their loops together (run_loops; the host-driven loop for a single
@@ -1050,6 +1193,102 @@ static int solve_groups(int ng, Plan* const* P, const int* nbs, const bec_phys* 
       out[b].walltime += ow[b].walltime;
       out[b].n_energy += ow[b].n_energy;
     }
+  }
+  return 0;
+}
+
+// One phase of a queue of nq solves (states psi_io [nq][n], physics php[nq],
+// results out[nq]) on nb slots of plan p: the first nb solves fill the slots;
+// whenever at least an eighth of the slots (or all occupied ones) have
+// stopped and solves are waiting, the loop drains, writes the stopped slots
+// out and sets the waiting solves up in them, and goes on with the same
+// graph.  Every solve follows the ground_state loop exactly, as in a batch.
+static void queue_phase(Plan& p, int nb, int nq, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
+                        bec_solve_out* out, bool f32) {
+  bool lr = false;
+  for (int k = 0; k < nq; ++k)
+    lr = lr || plan_long_range(p, Phys{php[k].beta2, php[k].beta3, php[k].Omega, php[k].G_C});
+  Phase f;
+  f.pp = &p;
+  f.nb = nb;
+  f.cf = cf;
+  f.f32 = f32;
+  phase_init(f, lr);
+  const long long n = p.n;
+  int next = 0;
+  auto assign = [&](int b) {
+    const int k = next++;
+    f.ph[b] = Phys{php[k].beta2, php[k].beta3, php[k].Omega, php[k].G_C};
+    f.src[b] = psi_io + (long long)k * n;
+    f.dst[b] = psi_io + (long long)k * n;
+    f.outp[b] = out + k;
+  };
+  unsigned long long occ = 0;          // slots holding a solve not yet written out
+  for (int b = 0; b < nb && next < nq; ++b) {
+    assign(b);
+    occ |= 1ull << b;
+  }
+  slots_begin(f, occ);
+  DevLoop L;
+  try {
+    if (f32) p.f32 = 1;
+    L.begin(p, nb, cf, f.I, f.lr_any, nullptr, 0, f.n_tr, f.lp);
+    L.pending = [&]() { return next < nq; };
+    L.want_refill = [&]() {
+      if (next >= nq) return false;
+      const unsigned long long dn = L.done_mask() & occ;
+      const int ready = __builtin_popcountll(dn), busy = __builtin_popcountll(occ);
+      return ready > 0 && (ready >= std::max(1, nb / 8) || ready == busy);
+    };
+    L.refill = [&]() {
+      const unsigned long long dn = L.done_mask() & occ;
+      L.harvest(dn, p.stream);
+      slots_end(f, dn);                // (FP64 set-up and write-out: the launches' own
+      occ &= ~dn;                      //  batches carry the precision, not p.f32)
+      unsigned long long fresh = 0;
+      for (int b = 0; b < nb && next < nq; ++b)
+        if ((dn >> b) & 1ull) {
+          assign(b);
+          fresh |= 1ull << b;
+        }
+      if (fresh) {
+        slots_begin(f, fresh);
+        L.upload(fresh, p.stream);
+        occ |= fresh;
+      }
+    };
+    run_loops({&L});
+    L.harvest(occ, p.stream);
+  } catch (...) {
+    p.f32 = 0;
+    throw;
+  }
+  p.f32 = 0;
+  slots_end(f, occ);
+}
+
+// The queue of nq solves on nb slots: the FP32 warm phase over the whole
+// queue when it applies (as for a batch), then the FP64 phase over it.
+static int solve_queue(Plan& p, int nb, int nq, const bec_phys* php, const bec_solver_cfg* cf, double2* psi_io,
+                       bec_solve_out* out) {
+  bool lr = false;
+  for (int k = 0; k < nq; ++k)
+    lr = lr || plan_long_range(p, Phys{php[k].beta2, php[k].beta3, php[k].Omega, php[k].G_C});
+  const bool lp = cf->line_poly != 0 && !cf->host_loop;
+  const bool warm = cf->warm_res_tol > cf->res_tol && !(lr && !lp) && !p.fpN.odd;
+  std::vector<bec_solve_out> ow(nq);
+  if (warm) {
+    bec_solver_cfg w = *cf;
+    w.res_tol = cf->warm_res_tol;
+    queue_phase(p, nb, nq, php, &w, psi_io, ow.data(), true);
+  }
+  queue_phase(p, nb, nq, php, cf, psi_io, out, false);
+  for (int k = 0; k < nq; ++k) {
+    out[k].warm_iters = warm ? ow[k].iters : 0;
+    if (!warm) continue;
+    out[k].iters += ow[k].iters;
+    out[k].walltime += ow[k].walltime;
+    out[k].n_energy += ow[k].n_energy;
   }
   return 0;
 }
@@ -1259,6 +1498,23 @@ extern "C" int bec_ground_state_uneven(int ng, bec_plan* const* plans, const int
         if (P[h] == P[g]) throw BecError(1, "each grid of one call needs a plan of its own");
     }
     return solve_groups(ng, P.data(), nb, php, cfg, psi.data(), out, nullptr, 0);
+  } catch (const BecError& e) {
+    bec_set_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    bec_set_error(e.what());
+    return 2;
+  }
+}
+
+extern "C" int bec_ground_state_queue(bec_plan* h, int nb, int nq, const bec_phys* php, const bec_solver_cfg* cf,
+                                      void* psi_io, bec_solve_out* out) {
+  try {
+    if (!h || !php || !cf || !psi_io || !out) throw BecError(1, "null argument");
+    if (nb < 1 || nb > BEC_BMAX) throw BecError(1, "slots must be in [1, " + std::to_string(BEC_BMAX) + "]");
+    if (nq < 1) throw BecError(1, "the queue needs at least one solve");
+    if (cf->host_loop) throw BecError(1, "a queue of solves runs the device-driven loop only");
+    return solve_queue(h->p, std::min(nb, nq), nq, php, cf, (double2*)psi_io, out);
   } catch (const BecError& e) {
     bec_set_error(e.what());
     return e.code;

@@ -107,6 +107,29 @@ def solve_native_batch(cfgs, G, psi0s, verbose=False):
     return [psi[b] for b in range(nb)], [res[b] for b in range(nb)]
 
 
+def solve_native_queue(cfgs, G, psi0s, slots=BATCH_MAX, verbose=False):
+    """Run more solves than one batch holds on one grid as a queue
+    (bec_ground_state_queue): `slots` instances advance together, and each
+    slot whose solve has stopped takes the next waiting one, so the batch
+    stays full until the queue drains.  cfgs share the solver knobs, as in
+    solve_native_batch.  Returns (list of psi, list of raw results)."""
+    nq = len(cfgs)
+    if nq != len(psi0s):
+        raise ValueError("one initial state per configuration")
+    if nq < 1 or not 1 <= slots <= BATCH_MAX:
+        raise ValueError(f"a queue needs solves and 1 to {BATCH_MAX} slots")
+    lr_any = G["kernel"] != "none" and any(abs(float(c["G_C"])) > 1e-15 for c in cfgs)
+    pl = ensure_kernel(G, "K") if lr_any else plan_of(G)
+    if cfgs[0].get("autotune", False):
+        pl.autotune(min(nq, slots))
+    psi = torch.stack([as_field(p0, G) for p0 in psi0s]).contiguous()
+    sc = _solver_cfg(cfgs[0], verbose)
+    ph = (_lib.Phys * nq)(*[phys_for(G, c["beta2"], c["beta3"], c["Omega"], c["G_C"]) for c in cfgs])
+    res = (_lib.SolveOut * nq)()
+    _lib.check(pl.lib.bec_ground_state_queue(pl.h, int(slots), nq, ph, ctypes.byref(sc), _ptr(psi), res))
+    return [psi[b] for b in range(nq)], [res[b] for b in range(nq)]
+
+
 def _finish(psi, res, G, cfg):
     """Final residual and observables of a returned state (solvers.py:227-235)."""
     resid_abs, _mu, resid_rel = compute_residual(psi, G, cfg)
@@ -148,6 +171,12 @@ def ground_state_batch(cfgs, G=None, psi0s=None, verbose=False):
         sets.setdefault(_solver_key(c), []).append(i)
     out = [None] * len(cfgs)
     for ks in sets.values():
+        if len(ks) > BATCH_MAX and not cfgs[ks[0]].get("host_loop", False):
+            # more solves than one batch: a queue that refills stopped slots
+            psis, rs = solve_native_queue([cfgs[k] for k in ks], G, [psi0s[k] for k in ks], verbose=verbose)
+            for k, p, r in zip(ks, psis, rs):
+                out[k] = (p.detach(), G, _finish(p, r, G, cfgs[k]))
+            continue
         for lo in range(0, len(ks), BATCH_MAX):
             kk = ks[lo:lo + BATCH_MAX]
             psis, rs = solve_native_batch([cfgs[k] for k in kk], G, [psi0s[k] for k in kk], verbose=verbose)

@@ -279,6 +279,17 @@ int bec_ground_state(bec_plan* p, const bec_phys* ph, const bec_solver_cfg* cfg,
 int bec_ground_state_batch(bec_plan* p, int nb, const bec_phys* ph, const bec_solver_cfg* cfg, void* psi_io,
                            bec_solve_out* out);
 
+/* A queue of nq solves on one grid, more than one batch holds: nb (<= 64)
+ * slots advance together as in bec_ground_state_batch, and whenever some of
+ * them have stopped while solves are waiting, those slots take the next
+ * solves (their set-up between two graph launches of the same loop), so the
+ * batch stays full until the queue drains.  ph[nq], psi_io ([nq][n]) and
+ * out[nq] as in bec_ground_state_batch; every solve follows the ground_state
+ * loop exactly.  The reference solves a sweep's points and seeds one after
+ * another (modes.py:295-387, solvers.py:238-263).  Device-driven loop only. */
+int bec_ground_state_queue(bec_plan* p, int nb, int nq, const bec_phys* ph, const bec_solver_cfg* cfg,
+                           void* psi_io, bec_solve_out* out);
+
 /* Batched ground_state over ng (<= 8) grids at once, e.g. the seeds and
  * points of a sweep whose autobox grids differ in size (modes.py:295-387,
  * which solves them one after another).  Grid g has its own plan plans[g]

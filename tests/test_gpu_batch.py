@@ -206,3 +206,54 @@ def test_uneven_call_with_mixed_settings_and_kernels():
     with pytest.raises(ValueError):         # one plan per grid of a call
         solve_native_uneven([[a], [dict(a, beta2=30.0)]], [grids[0], grids[2]],
                             [[torch.as_tensor(p0s[0]).cuda()], [torch.as_tensor(p0s[2]).cuda()]])
+
+
+@pytest.mark.parametrize("variant", ["contact", "newton_warm"])
+def test_queue_refills_slots_and_matches_the_batch(variant):
+    """A queue of ten solves on three slots (bec_ground_state_queue): stopped
+    slots take the waiting solves, and every solve returns exactly what the
+    same instance returns in one batch of all ten (E, iterations, state bit
+    for bit), with long range and the FP32 warm phase too."""
+    import becgpp_amd as B
+    from becgpp_amd.solvers import solve_native_batch, solve_native_queue
+    if variant == "contact":
+        base = cfg(Ngrid=32, L=6.0, res_tol=1e-8)
+        cs = [dict(base, beta2=10.0 + 12.0 * i, Omega=(0.0, 0.3)[i % 2]) for i in range(10)]
+    else:
+        base = cfg(Ngrid=48, L=7.0, kernel="newton", res_tol=1e-8, warm_fp32=True)
+        cs = [dict(base, beta2=5.0 + 6.0 * i, G_C=(0.0, 1.5, 3.0)[i % 3]) for i in range(10)]
+    p0s = [torch.as_tensor(O.noisy_gaussian(O.make_grid(c), 1e-3, seed=90 + i)).cuda() for i, c in enumerate(cs)]
+    G = B.make_grid(cs[0])
+    print(f"[batch] queue of 10 on 3 slots ({variant})", flush=True)
+    qp, qr = solve_native_queue(cs, G, p0s, slots=3)
+    bp, br = solve_native_batch(cs, G, p0s)
+    its = []
+    for k in range(len(cs)):
+        assert qr[k].iters == br[k].iters and qr[k].E == br[k].E, (k, qr[k].iters, br[k].iters)
+        assert qr[k].stop_reason == br[k].stop_reason and qr[k].warm_iters == br[k].warm_iters, k
+        assert torch.equal(qp[k], bp[k]), k
+        its.append(qr[k].iters)
+    assert len(set(its)) > 1                 # the solves stop at different times: slots were refilled
+    if variant == "newton_warm":
+        assert all(r.warm_iters > 0 for r in qr)
+
+
+def test_batch_of_more_than_64_runs_as_a_queue():
+    """ground_state_batch with 70 solves on one grid runs one queue on 64
+    slots; each result equals the solve run alone (ground_state), and the
+    oracle's for two of them."""
+    import becgpp_amd as B
+    base = cfg(Ngrid=16, L=4.0, res_tol=1e-8)
+    cs = [dict(base, beta2=2.0 + 0.5 * i) for i in range(70)]
+    G = B.make_grid(cs[0])
+    p0s = [torch.as_tensor(O.noisy_gaussian(O.make_grid(c), 1e-3, seed=200 + i)).cuda() for i, c in enumerate(cs)]
+    print("[batch] 70 solves through ground_state_batch", flush=True)
+    got = B.ground_state_batch(cs, G=G, psi0s=p0s)
+    for k in (0, 33, 64, 69):
+        ps, _, os_ = B.ground_state(cs[k], G=G, psi0=p0s[k], verbose=False)
+        assert got[k][2]["iters"] == os_["iters"] and got[k][2]["E"] == os_["E"], k
+        assert torch.equal(got[k][0], ps), k
+    for k in (5, 66):
+        po, _, oo = O.ground_state(cs[k], O.make_grid(cs[k]), p0s[k].cpu().numpy())
+        assert _close(got[k][2]["E"], oo["E"], 1e-8), k
+        assert _dens_l2(got[k][0].cpu().numpy(), po) < 1e-6, k
