@@ -1196,6 +1196,10 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
   unsigned own = 0;
   for (int j = 0; j < PER; ++j) own += hist[threadIdx.x * PER + j];
   pre[threadIdx.x] = own;
+  if (threadIdx.x == 0) {              // defined even if no bin holds rank k
+    sbin = 0;
+    srank = 0u;
+  }
   __syncthreads();
   for (int o2 = 1; o2 < BEC_WG; o2 <<= 1) {          // inclusive Hillis-Steele scan
     const unsigned add = (int)threadIdx.x >= o2 ? pre[threadIdx.x - o2] : 0u;
@@ -1231,7 +1235,10 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_gather(const double* __restric
     }
 #pragma unroll
     for (int u = 0; u < SIGMA_U; ++u)
-      if (v[u] == v[u] && sigma_bin(v[u], lo, scale) == bin) cand[atomicAdd(&st->ncand, 1u)] = v[u];
+      if (v[u] == v[u] && sigma_bin(v[u], lo, scale) == bin) {
+        const unsigned at = atomicAdd(&st->ncand, 1u);
+        if (at < (unsigned long long)n) cand[at] = v[u];     // an instance holds n candidates at most
+      }
   }
 }
 
@@ -1258,7 +1265,9 @@ __global__ __launch_bounds__(BEC_WG) void k_sigma_select(SigmaStats* stats, unsi
     med = lo;
     have = true;
   } else if (cnt > 0) {
-    const unsigned m = st->ncand, r = st->rank_in;
+    // the candidates read stay within the instance's n slots, the rank within them
+    const unsigned m = (unsigned)min((unsigned long long)st->ncand, (unsigned long long)n);
+    const unsigned r = m ? min(st->rank_in, m - 1u) : 0u;
     if (m <= SIGMA_LDS_CAND) {
       unsigned P = 1;
       while (P < m) P <<= 1;
