@@ -339,3 +339,41 @@ def test_library_noise_equals_the_oracle_generator():
         assert lib.bec_noise(rng, stream, first, 64, out) == 0
         ref = O.counter_normals(rng, stream, 64, first=first)
         assert np.abs(np.array(out) - ref).max() <= 4e-16 * max(1.0, np.abs(ref).max())
+
+
+def test_solver_entry_points_reject_bad_arguments_before_device_work():
+    """bec_ground_state_queue and bec_ground_state_uneven check their
+    arguments on the host and return status 1 with a message (a ValueError
+    through _lib.check) before touching a device: null arguments, a grid
+    count outside [1, 8], batch sizes outside [1, 64], the host-driven loop
+    and two grids of one call sharing a plan.  The plan handles here are
+    never dereferenced."""
+    import ctypes
+    from becgpp_amd import _lib
+    lib = _lib.load()
+
+    def err(status):
+        assert status == 1, status
+        return lib.bec_last_error().decode()
+
+    out = (_lib.SolveOut * 4)()
+    ph = (_lib.Phys * 4)()
+    sc = _lib.SolverCfg()
+    assert "null" in err(lib.bec_ground_state_queue(None, 3, 10, ph, ctypes.byref(sc), None, out))
+    fake = [ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)]
+
+    def uneven(ng, plans, nbs, host_loop=0):
+        cfg = _lib.SolverCfg()
+        cfg.host_loop = host_loop
+        cps = (ctypes.POINTER(_lib.SolverCfg) * max(ng, 1))(*([ctypes.pointer(cfg)] * max(ng, 1)))
+        psi = (ctypes.c_void_p * max(ng, 1))(*([0x3000] * max(ng, 1)))
+        hs = (ctypes.c_void_p * max(ng, 1))(*[p.value for p in plans])
+        return lib.bec_ground_state_uneven(ng, hs, (ctypes.c_int * max(ng, 1))(*nbs), ph, cps, psi, out)
+
+    assert "number of grids" in err(uneven(0, fake[:1], [1]))
+    assert "number of grids" in err(uneven(9, fake[:1] * 9, [1] * 9))
+    assert "batch size" in err(uneven(2, fake, [1, 65]))
+    assert "device-driven" in err(uneven(1, fake[:1], [1], host_loop=1))
+    assert "plan of its own" in err(uneven(2, [fake[0], fake[0]], [1, 1]))
+    with pytest.raises(ValueError):
+        _lib.check(uneven(2, [fake[1], fake[1]], [2, 2]))
