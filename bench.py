@@ -70,33 +70,31 @@ WORKLOADS = {
     "newton_3d_n128": dict(
         cfg=dict(mode="single", dimension="3D", s=2, trap_coeff=0.5, Omega=0.0, beta2=0.0, beta3=0.0, G_C=20.0,
                  kernel="newton", L=12.0, Ngrid=128, nseeds=1, seed="gaussian", seed_noise=1e-3,
-                 res_tol=1e-6, maxit=60000, linesearch_max=20),
+                 res_tol=1e-6, maxit=60000),
         desc="3D Newton boson star, G_C=20, beta2=0, L=12, N=128 (conv 256^3), Gaussian seed with 1e-3 "
-             "noise, res_tol=1e-6, linesearch_max=20",
+             "noise, res_tol=1e-6",
         check=("hf", "G_C"),
         replay_iters=200),
     # quasi-2D pancakes: K_eff(l_z=0.5); Omega x G_C = 8 points x 4 seeds = 32 solves on three grids
     "q2d_batch_uneven": dict(
         cfg=dict(mode="single", dimension="quasi2D", s=2, trap_coeff=0.5, Omega=0.0, beta2=50.0, beta3=0.0,
-                 G_C=1.0, kernel="q2d", l_z=0.5, L=16.0, Ngrid=128, nseeds=4, res_tol=1e-6, maxit=60000,
-                 linesearch_max=20),
+                 G_C=1.0, kernel="q2d", l_z=0.5, L=16.0, Ngrid=128, nseeds=4, res_tol=1e-6, maxit=60000),
         points=[dict(Omega=om, G_C=gc) for om in (0.0, 0.5) for gc in (1.0, 2.0, 5.0, 10.0)],
         instance_grid=lambda i: dict(Ngrid=(128, 192, 256)[i % 3]),
         desc="quasi-2D q2d kernel l_z=0.5, beta2=50, L=16, Omega in {0,0.5} x G_C in {1,2,5,10}, nseeds=4 "
              "(32 solves), N = 128/192/256 round-robin by instance id; the instances of one grid form one "
              "batched loop and one hipGraph advances the three grids' loops together (a branch per grid); "
-             "res_tol=1e-6, linesearch_max=20",
+             "res_tol=1e-6",
         replay_iters=200),
     # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
     "droplet_sweep_2d_n256": dict(
         cfg=dict(mode="single", dimension="2D", s=2, trap_coeff=0.0, Omega=0.0, beta2=-120.0, beta3=1000.0,
                  G_C=0.0, kernel="none", L=20.0, Ngrid=256, nseeds=4, res_tol=1e-8, maxit=60000,
-                 linesearch_max=20, warm_fp32=True, warm_res_tol=1e-4),
+                 warm_fp32=True, warm_res_tol=1e-4),
         points=[dict(beta2=float(b)) for b in
                 [-200.0 + 160.0 * i / 15.0 for i in range(16)]],
         desc="2D cubic-quintic droplet sweep, no trap, beta2 = 16 values in [-200, -40], beta3=1e3, "
-             "L=20, N=256, nseeds=4 (64 solves), FP32 warm phase to res 1e-4 then FP64 to res_tol=1e-8, "
-             "linesearch_max=20",
+             "L=20, N=256, nseeds=4 (64 solves), FP32 warm phase to res 1e-4 then FP64 to res_tol=1e-8",
         check=("hf", "beta2"),
         replay_iters=100),
 }
@@ -155,6 +153,8 @@ def parse():
                     help="no timed region: only the event-timed replay of the default workload (profiling)")
     ap.add_argument("--line-search", default=None, choices=("poly", "trial"),
                     help="override the workloads' line search (default: poly)")
+    ap.add_argument("--linesearch-max", type=int, default=None,
+                    help="override the workloads' Armijo halvings per iteration (comparison runs)")
     ap.add_argument("--fp64-only", action="store_true", help="run FP32 warm phases in FP64 too (comparison)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner choose each plan's FFT stage orders and lines per workgroup first")
@@ -218,6 +218,8 @@ def run_workload(name, steps, warmup, a, cpu_threads, cpu_desc):
         cfg["warm_fp32"] = False
     if a.line_search:
         cfg["line_search"] = a.line_search
+    if a.linesearch_max is not None:
+        cfg["linesearch_max"] = int(a.linesearch_max)
     nseeds = int(cfg["nseeds"])
     points = wl.get("points", [{}])
     igrid = wl.get("instance_grid", lambda i: {})
