@@ -74,6 +74,7 @@ WORKLOADS = {
         desc="3D Newton boson star, G_C=20, beta2=0, L=12, N=128 (conv 256^3), Gaussian seed with 1e-3 "
              "noise, res_tol=1e-6",
         check=("hf", "G_C"),
+        mixed_variant=True,
         replay_iters=200),
     # quasi-2D pancakes: K_eff(l_z=0.5); Omega x G_C = 8 points x 4 seeds = 32 solves on three grids
     "q2d_batch_uneven": dict(
@@ -85,6 +86,7 @@ WORKLOADS = {
              "(32 solves), N = 128/192/256 round-robin by instance id; the instances of one grid form one "
              "batched loop and one hipGraph advances the three grids' loops together (a branch per grid); "
              "res_tol=1e-6",
+        mixed_variant=True,
         replay_iters=200),
     # self-bound cubic-quintic droplets: no trap, 16 attractive beta2 x 4 seeds = 64 solves
     "droplet_sweep_2d_n256": dict(
@@ -378,6 +380,27 @@ def run_workload(name, steps, warmup, a, cpu_threads, cpu_desc):
                                             cg_iters_per_s=si / ts, iters_per_solve=si / nsol,
                                             E_selected=sb[0][1]["E"])
             out["batched_speedup"] = out["value"] / out["single_solve_path"]["ground_states_per_s"]
+        if wl.get("mixed_variant") and not a.no_checks:
+            # the same solves with the optional FP32 warm phase (to res 1e-4) before
+            # the FP64 loop, one untimed and one timed pass: its measured effect
+            cm = [dict(c, warm_fp32=True, warm_res_tol=1e-4) for c in cs]
+            mixed = lambda: B.ground_state_uneven(cm, psi0s=psi0s, grids=[grids[r[3]] for r in runs])
+            mixed()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mres = mixed()
+            torch.cuda.synchronize()
+            tm = time.perf_counter() - t0
+            mb = select(mres)
+            mi = sum(obs["iters"] for _, _, obs in mres)
+            mw = sum(obs.get("warm_iters", 0) for _, _, obs in mres)
+            out["mixed_precision"] = dict(
+                warm_fp32=True, warm_res_tol=1e-4, ground_states_per_s=npt / tm, ms_per_step=1e3 * tm,
+                iters_per_solve=mi / nsol, warm_iters_share=mw / max(1, mi),
+                speedup_vs_fp64=(npt / tm) / out["value"],
+                dE_rel_vs_fp64=max(abs(mb[i][1]["E"] - best[i][1]["E"]) / abs(best[i][1]["E"]) for i in range(npt)),
+                note="complex64 fields and transforms in the warm phase; FP64 sums, convolutions and Armijo test; "
+                     "then the FP64 loop to res_tol from each warm state")
 
     # roofline: every kernel of the largest grid group timed in a direct-launch replay
     replay_iters = int(wl.get("replay_iters", 300))
