@@ -21,8 +21,11 @@ How each part is pinned (tests in ``tests/test_oracle_*.py``):
   (tests/test_kernels.py:53-63).
 * energy / H / mu -- central finite differences of E vs 2 Re<H psi, v> dv over
   the six physics sets of tests/test_audit.py:55-85; mu = <H> (ibid. 99-106).
-* solver -- converged energies of benchmarks/results/itp_timing/timing.csv:2-3
-  and pcg_variants.csv; stall / maxit semantics (tests/test_audit.py:165-172,
+* solver -- trajectories the reference ran itself for eleven small cases
+  (tests/golden/make_trajectory_golden.py, tests/test_oracle_trajectory.py):
+  every accepted step's E and residual, the iteration count and stop reason;
+  converged energies of benchmarks/results/itp_timing/timing.csv:2-3 and
+  pcg_variants.csv; stall / maxit semantics (tests/test_audit.py:165-172,
   252-255); virial identities (tests/test_kernels.py:30-50).
 * preconditioner (shift sigma and z) -- vectors the reference computed itself
   for nine cases (tests/golden/make_precond_golden.py: the six physics sets

@@ -1,0 +1,54 @@
+"""The HIP solver loop against trajectories the reference ran itself
+(tests/golden/trajectory_reference.json, written by
+tests/golden/make_trajectory_golden.py on the CPU; the oracle follows the same
+runs in tests/test_oracle_trajectory.py).
+
+From the same initial state, bec_ground_state's trace (every accepted step's
+iteration number, energy and relative residual, recorded by the controller
+kernels) must equal the reference's over the whole run where that run is
+insensitive to round-off (steepest descent over 3135 iterations, the stalled
+line search, the maxit stop), and over the first 40 accepted steps elsewhere;
+Polak-Ribiere CG then amplifies the last-bit differences of the sums and
+transforms into a drift of the later steps, while both converge to the same
+state: the final E within 1e-9 (within the window's tolerance for the
+energy-window stop), mu within 1e-6, and the iteration count and stop reason
+where the reference's run is exact or drifts only at its end.
+"""
+import json
+import os
+
+import pytest
+
+from tests.golden.make_trajectory_golden import CASES, case_overrides, inputs
+from tests.helpers import cfg
+from tests.test_oracle_trajectory import EXACT, FIRST, SAME_STOP, compare_trace
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trajectory_reference.json")
+
+
+@pytest.mark.parametrize("i", range(len(CASES)), ids=[c["name"] for c in CASES])
+def test_hip_loop_follows_reference_trajectory(i):
+    import becgpp_amd as B
+    kw = CASES[i]
+    with open(GOLD) as f:
+        g = json.load(f)["cases"][i]
+    assert g["name"] == kw["name"]
+    c = cfg(**case_overrides(kw), record_trace=True)
+    G = B.make_grid(c)
+    print(f"[trajectory] {kw['name']}: {case_overrides(kw)}", flush=True)
+    _, _, obs = B.ground_state(c, G=G, psi0=torch.as_tensor(inputs(kw, seed=g["seed"])).cuda(), verbose=False)
+    tr = [(int(t[1]), float(t[2]), float(t[3])) for t in obs["trace"] if t[1] > 0]
+    ref = [tuple(t) for t in g["trace"] if t[0] > 0]
+    if kw["name"] in SAME_STOP:
+        assert obs["iters"] == g["iters"] and obs["stop_reason"] == g["stop_reason"]
+        assert len(tr) == len(ref)
+    if kw["name"] in EXACT:
+        compare_trace(tr, ref, len(ref))
+    else:
+        compare_trace(tr, ref, min(FIRST, len(ref)))
+    e_tol = float(kw.get("energy_tol", 1e-9)) if g["stop_reason"] == "energy_converged" else 1e-9
+    assert abs(obs["E"] - g["E"]) <= e_tol * abs(g["E"])
+    assert abs(obs["mu"] - g["mu"]) <= 1e-6 * abs(g["mu"])

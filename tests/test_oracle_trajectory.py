@@ -1,0 +1,77 @@
+"""Pins of the oracle's solver loop (reference solvers.py:48-235) by
+trajectories the reference ran itself (tests/golden/make_trajectory_golden.py,
+run on the CPU where the reference is mounted; only its output is committed).
+
+Eleven small cases cover the contact, rotating, newton (PR variant 'pr' with
+restarts every 7 iterations), log, q2d, 3D newton, self-bound cubic-quintic
+and quartic-trap (cg_beta 'none') physics, a stalled line search, the maxit
+stop and the energy-window stop.  From the same initial state the oracle must
+reproduce every accepted step's iteration number, energy and residual:
+exactly (E to 1e-12, residual to 1e-6) over the whole run where the reference's
+own trajectory is insensitive to round-off (steepest descent over 3135
+iterations, the stall, maxit), and over the first 40 accepted steps elsewhere --
+Polak-Ribiere CG amplifies a 1e-16 difference in the FFTs by ~10x per step
+once the residual is small, so later steps of two correct implementations
+drift apart (first visible after 41-160 steps in these cases) while they
+converge to the same state: final E to 1e-9 (to the window's 1e-7 for the
+energy-window stop), and the iteration count and stop reason where the run
+is exact or, as in the cubic-quintic case, drifts only in its last steps.  A
+dropped term, a wrong sign or a mis-indexed restart shows up in the first
+steps.
+"""
+import json
+import os
+
+import pytest
+
+import oracle as O
+from tests.golden.make_trajectory_golden import CASES, case_overrides, inputs
+from tests.helpers import cfg
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trajectory_reference.json")
+EXACT = {"quartic_no_cg", "stall", "maxit"}          # the whole trace
+SAME_STOP = EXACT | {"cubic_quintic"}                  # iteration count and stop reason
+FIRST = 40
+
+
+def gold():
+    with open(GOLD) as f:
+        return json.load(f)["cases"]
+
+
+def compare_trace(mine, ref, n, e_tol=1e-12, r_tol=1e-6, r_abs=1e-12):
+    """mine, ref: [(iteration, E, resid_rel)] of the accepted steps; a small
+    residual is a norm of H psi - mu psi after cancellation, so it also gets
+    an absolute tolerance"""
+    assert len(mine) >= n and len(ref) >= n
+    for k in range(n):
+        it, E, r = mine[k]
+        rit, rE, rr = ref[k]
+        assert it == rit, (k, it, rit)
+        assert abs(E - rE) <= e_tol * max(1.0, abs(rE)), (k, E, rE)
+        assert abs(r - rr) <= r_tol * rr + r_abs, (k, r, rr)
+
+
+@pytest.mark.parametrize("i", range(len(CASES)), ids=[c["name"] for c in CASES])
+def test_oracle_follows_reference_trajectory(i):
+    kw = CASES[i]
+    g = gold()[i]
+    assert g["name"] == kw["name"]
+    c = cfg(**case_overrides(kw))
+    G = O.make_grid(c)
+    tr = []
+    _, _, obs = O.ground_state(c, G, inputs(kw, seed=g["seed"]),
+                               callback=lambda it, E, r, t: tr.append((it + 1, E, r)))
+    ref = [tuple(t) for t in g["trace"] if t[0] > 0]      # the reference's first row is the initial state
+    if kw["name"] in SAME_STOP:
+        assert obs["iters"] == g["iters"] and obs["stop_reason"] == g["stop_reason"]
+        assert len(tr) == len(ref)
+    if kw["name"] in EXACT:
+        compare_trace(tr, ref, len(ref))
+        assert abs(obs["preconditioner_shift"] - g["preconditioner_shift"]) <= 1e-12 * max(1.0, g["preconditioner_shift"])
+    else:
+        compare_trace(tr, ref, min(FIRST, len(ref)))
+    # a run stopped by the energy window is converged to that window's tolerance
+    e_tol = float(kw.get("energy_tol", 1e-9)) if g["stop_reason"] == "energy_converged" else 1e-9
+    assert abs(obs["E"] - g["E"]) <= e_tol * abs(g["E"])
+    assert abs(obs["mu"] - g["mu"]) <= 1e-6 * abs(g["mu"])
