@@ -33,6 +33,9 @@ How each part is pinned (tests in ``tests/test_oracle_*.py``):
   tests/test_oracle_precond.py), which fix the local stiffness, the bulk mask
   and the floor; the closed form P((k^2/2 + sigma) v) = proj_tangent(v); the
   lower median against a brute-force sort and a hand-computed density.
+* Thomas-Fermi envelopes -- envelopes the reference computed itself for
+  eight cases (tests/golden/make_tf_golden.py, tests/test_oracle_tf.py): the
+  regime, mu, radii and density, the nonlocal fixed point included.
 * seeds -- the noise is this build's counter-based generator
   (``counter_normals``: splitmix64 + Box-Muller; the splitmix64 stream is
   pinned by its published outputs for seed 1234567, the normals by their

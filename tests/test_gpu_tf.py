@@ -51,3 +51,18 @@ def test_tf_bisection_matches_host_bisection():
         ref = O.bisect_mu_for_norm(lambda m: O.invert_local(m, Veff, 50.0, 2.0), Go, lo, hi)
         got = _bisect_mu_for_norm(torch.as_tensor(Veff).cuda(), 50.0, 2.0, Gd, lo, hi)
         assert math.isclose(got, ref, rel_tol=1e-13), (got, ref)
+
+
+@pytest.mark.parametrize("i", range(8))
+def test_extract_tf_matches_reference_envelopes(i):
+    """The device envelopes against those the reference computed itself
+    (tests/golden/tf_reference.npz): regime, mu, radii and density."""
+    import becgpp_amd as B
+    from tests.golden.make_tf_golden import CASES, case_cfg
+    from tests.test_oracle_tf import check_against_reference
+    assert len(CASES) == 8
+    c = cfg(tf_cache=False, **case_cfg(CASES[i]))
+    Gd = B.make_grid(c)
+    print(f"[tf] reference envelope case {i}: {CASES[i]}", flush=True)
+    check_against_reference(B.extract_tf(c, Gd), i, float(Gd["dx"]) ** int(Gd["ndim"]), float(Gd["dx"]),
+                            lambda tf: tf["n"].cpu().numpy())
