@@ -33,6 +33,10 @@ How each part is pinned (tests in ``tests/test_oracle_*.py``):
   tests/test_oracle_precond.py), which fix the local stiffness, the bulk mask
   and the floor; the closed form P((k^2/2 + sigma) v) = proj_tangent(v); the
   lower median against a brute-force sort and a hand-computed density.
+* diagnostics (observables, radii, vortex counts, LLL weight) and
+  resampling -- the reference's own diagnostics bundle on eight states and
+  three resampled states (tests/golden/make_diag_golden.py,
+  tests/test_oracle_diag_bundle.py).
 * Thomas-Fermi envelopes -- envelopes the reference computed itself for
   eight cases (tests/golden/make_tf_golden.py, tests/test_oracle_tf.py): the
   regime, mu, radii and density, the nonlocal fixed point included.
