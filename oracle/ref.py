@@ -40,7 +40,10 @@ How each part is pinned (tests in ``tests/test_oracle_*.py``):
 * Thomas-Fermi envelopes -- envelopes the reference computed itself for
   eight cases (tests/golden/make_tf_golden.py, tests/test_oracle_tf.py): the
   regime, mu, radii and density, the nonlocal fixed point included.
-* seeds -- the noise is this build's counter-based generator
+* seeds -- the recipes (envelopes, windings, the triangular lattice) by the
+  reference's own seeds with the noise switched off (tests/golden/
+  make_seed_golden.py, tests/test_oracle_seed_recipe.py); the noise is this
+  build's counter-based generator
   (``counter_normals``: splitmix64 + Box-Muller; the splitmix64 stream is
   pinned by its published outputs for seed 1234567, the normals by their
   moments); the reference's seed tensors depend on the torch device RNG and

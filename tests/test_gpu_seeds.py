@@ -88,3 +88,17 @@ def test_seed_vortices_wind_with_the_rotation():
         d = B.vortex_diagnostic(psi, G, om)
         assert d["Nv"] > 5
         assert (d["Nplus"], d["Nminus"]) == ((d["Nv"], 0) if om > 0 else (0, d["Nv"])), d
+
+
+@pytest.mark.parametrize("i", range(11))
+def test_device_seed_recipe_matches_reference(i):
+    """bec_seed with the noise switched off against the reference's own
+    initial states (tests/golden/seed_reference.npz): envelopes, windings and
+    the triangular lattice to 1e-12."""
+    import becgpp_amd as B
+    from tests.golden.make_seed_golden import CASES, case_cfg
+    from tests.test_oracle_seed_recipe import check_seed
+    assert len(CASES) == 11
+    c = cfg(tf_cache=False, **case_cfg(CASES[i]))
+    print(f"[seeds] reference recipe case {i}: {CASES[i]}", flush=True)
+    check_seed(B.make_seed(B.make_grid(c), c).cpu().numpy(), i)
