@@ -9,7 +9,8 @@ reference root, its output directory redirected to a temporary folder) with
 ``record_trace`` on, from initial states the tests regenerate from the recipe
 below, and writes tests/golden/trajectory_reference.json: per case the
 iteration count, stop reason, final E, mu, L_z, residuals, shift and the
-trace (iteration, E, relative residual) of every accepted step.
+trace (iteration, E, relative residual) of every accepted step; and
+tests/golden/trajectory_density.npz: the returned state's density.
 Run:  python tests/golden/make_trajectory_golden.py [/root/reference]
 """
 import json
@@ -22,6 +23,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "trajectory_reference.json")
+OUT_RHO = os.path.join(HERE, "trajectory_density.npz")     # the returned states' densities
 
 # small grids; each case exercises one part of the loop
 CASES = [
@@ -76,11 +78,13 @@ def main(ref_root):
     from becgpp.solvers import ground_state
 
     out = []
+    rho = {}
     for i, kw in enumerate(CASES):
         cfg = becgpp.default_cfg(mode="single", record_trace=True, **case_overrides(kw))
         G = make_grid(cfg)
         psi0 = torch.as_tensor(inputs(kw, seed=300 + i))
-        _, _, obs = ground_state(cfg, G=G, psi0=psi0, verbose=False)
+        psi, _, obs = ground_state(cfg, G=G, psi0=psi0, verbose=False)
+        rho[f"rho{i}"] = (psi.abs() ** 2).numpy().astype(np.float64)
 
         def num(v):
             v = float(v)
@@ -95,7 +99,8 @@ def main(ref_root):
     with open(OUT, "w") as f:
         json.dump(dict(_source="becGPP 1.2.1 becgpp/solvers.py ground_state (record_trace), CPU", cases=out), f,
                   indent=1)
-    print(f"wrote {OUT}")
+    np.savez_compressed(OUT_RHO, **rho)
+    print(f"wrote {OUT} and {OUT_RHO}")
 
 
 if __name__ == "__main__":

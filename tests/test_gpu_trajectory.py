@@ -11,7 +11,8 @@ line search, the maxit stop), and over the first 40 accepted steps elsewhere;
 Polak-Ribiere CG then amplifies the last-bit differences of the sums and
 transforms into a drift of the later steps, while both converge to the same
 state: the final E within 1e-9 (within the window's tolerance for the
-energy-window stop), mu within 1e-6, and the iteration count and stop reason
+energy-window stop), mu within 1e-6, the returned density within 1e-6 in L2
+and L_z within 1e-8, and the iteration count and stop reason
 where the reference's run is exact or drifts only at its end.
 """
 import json
@@ -21,7 +22,7 @@ import pytest
 
 from tests.golden.make_trajectory_golden import CASES, case_overrides, inputs
 from tests.helpers import cfg
-from tests.test_oracle_trajectory import EXACT, FIRST, SAME_STOP, compare_trace
+from tests.test_oracle_trajectory import EXACT, FIRST, SAME_STOP, compare_final_state, compare_trace
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -39,7 +40,7 @@ def test_hip_loop_follows_reference_trajectory(i):
     c = cfg(**case_overrides(kw), record_trace=True)
     G = B.make_grid(c)
     print(f"[trajectory] {kw['name']}: {case_overrides(kw)}", flush=True)
-    _, _, obs = B.ground_state(c, G=G, psi0=torch.as_tensor(inputs(kw, seed=g["seed"])).cuda(), verbose=False)
+    psi, _, obs = B.ground_state(c, G=G, psi0=torch.as_tensor(inputs(kw, seed=g["seed"])).cuda(), verbose=False)
     tr = [(int(t[1]), float(t[2]), float(t[3])) for t in obs["trace"] if t[1] > 0]
     ref = [tuple(t) for t in g["trace"] if t[0] > 0]
     if kw["name"] in SAME_STOP:
@@ -52,3 +53,4 @@ def test_hip_loop_follows_reference_trajectory(i):
     e_tol = float(kw.get("energy_tol", 1e-9)) if g["stop_reason"] == "energy_converged" else 1e-9
     assert abs(obs["E"] - g["E"]) <= e_tol * abs(g["E"])
     assert abs(obs["mu"] - g["mu"]) <= 1e-6 * abs(g["mu"])
+    compare_final_state((psi.abs() ** 2).cpu().numpy(), obs["Lz"], g, i)

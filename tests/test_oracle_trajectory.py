@@ -14,13 +14,16 @@ Polak-Ribiere CG amplifies a 1e-16 difference in the FFTs by ~10x per step
 once the residual is small, so later steps of two correct implementations
 drift apart (first visible after 41-160 steps in these cases) while they
 converge to the same state: final E to 1e-9 (to the window's 1e-7 for the
-energy-window stop), and the iteration count and stop reason where the run
+energy-window stop), the returned density to 1e-6 in L2 and L_z to 1e-8, and
+the iteration count and stop reason where the run
 is exact or, as in the cubic-quintic case, drifts only in its last steps.  A
 dropped term, a wrong sign or a mis-indexed restart shows up in the first
 steps.
 """
 import json
 import os
+
+import numpy as np
 
 import pytest
 
@@ -29,6 +32,7 @@ from tests.golden.make_trajectory_golden import CASES, case_overrides, inputs
 from tests.helpers import cfg
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trajectory_reference.json")
+GOLD_RHO = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trajectory_density.npz")
 EXACT = {"quartic_no_cg", "stall", "maxit"}          # the whole trace
 SAME_STOP = EXACT | {"cubic_quintic"}                  # iteration count and stop reason
 FIRST = 40
@@ -37,6 +41,15 @@ FIRST = 40
 def gold():
     with open(GOLD) as f:
         return json.load(f)["cases"]
+
+
+def compare_final_state(rho, Lz, g, i):
+    """the returned state's density to 1e-6 in L2 (1e-5 for a run stopped by
+    the energy window, converged only to its energy tolerance) and L_z to 1e-8"""
+    ref = np.load(GOLD_RHO)[f"rho{i}"]
+    d_tol = 1e-5 if g["stop_reason"] == "energy_converged" else 1e-6
+    assert np.linalg.norm(rho - ref) <= d_tol * np.linalg.norm(ref), np.linalg.norm(rho - ref) / np.linalg.norm(ref)
+    assert abs(Lz - g["Lz"]) <= 1e-8 * max(1.0, abs(g["Lz"])), (Lz, g["Lz"])
 
 
 def compare_trace(mine, ref, n, e_tol=1e-12, r_tol=1e-6, r_abs=1e-12):
@@ -60,8 +73,8 @@ def test_oracle_follows_reference_trajectory(i):
     c = cfg(**case_overrides(kw))
     G = O.make_grid(c)
     tr = []
-    _, _, obs = O.ground_state(c, G, inputs(kw, seed=g["seed"]),
-                               callback=lambda it, E, r, t: tr.append((it + 1, E, r)))
+    psi, _, obs = O.ground_state(c, G, inputs(kw, seed=g["seed"]),
+                                 callback=lambda it, E, r, t: tr.append((it + 1, E, r)))
     ref = [tuple(t) for t in g["trace"] if t[0] > 0]      # the reference's first row is the initial state
     if kw["name"] in SAME_STOP:
         assert obs["iters"] == g["iters"] and obs["stop_reason"] == g["stop_reason"]
@@ -75,3 +88,4 @@ def test_oracle_follows_reference_trajectory(i):
     e_tol = float(kw.get("energy_tol", 1e-9)) if g["stop_reason"] == "energy_converged" else 1e-9
     assert abs(obs["E"] - g["E"]) <= e_tol * abs(g["E"])
     assert abs(obs["mu"] - g["mu"]) <= 1e-6 * abs(g["mu"])
+    compare_final_state(np.abs(psi) ** 2, obs["Lz"], g, i)
